@@ -1,0 +1,164 @@
+"""Benchmark: FP32 pull PageRank on R-MAT scale 24 on one MI355X.
+
+A step is one pull PageRank iteration (gather + update + FP64 reductions,
+one kernel) over the whole device-resident graph.  Prints one JSON line:
+GTEPS (E edges per iteration / wall time per iteration), the HBM roofline of
+the iteration kernel (algorithmic bytes 8E + 16N + 4 per launch over its mean
+HIP-event duration) and the GraphBlox reference's OpenMP pull PageRank timed
+on this host's cores on the same graph.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--scale 24]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SEED = 24
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(workload_key: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc, if recorded."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload_key}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(g, iterations: int, threads: int):
+    """Reference pageRankPullGraphCSR (oracle/_ref) on this graph's CSR arrays,
+    or the oracle's restatement of it when the reference build is absent."""
+    import ctypes as C
+    from oracle import oracle as O
+
+    fwd = g.export(inverse=False)
+    inv = g.export(inverse=True)
+    N, E = g.N, g.E
+    ranks = np.zeros(N, np.float32)
+    it, err, t = C.c_uint32(), C.c_double(), C.c_double()
+    kind = "reference"
+    if O.ref_available(1, 0):
+        L = O.ref(1, 0)
+        t0 = time.perf_counter()
+        L.refd_pagerank_on_csr(N, E, fwd["out_degree"], inv["out_degree"], inv["edges_idx"], inv["dst"],
+                               iterations, 0.0, threads, ranks, C.byref(it), C.byref(err), C.byref(t))
+        wall = time.perf_counter() - t0
+        secs = t.value if t.value > 0 else wall
+    else:
+        kind = "port"
+        fo = O.OracleCSR(N, E, fwd["out_degree"], fwd["in_degree"], fwd["edges_idx"], fwd["src"], fwd["dst"], fwd["w"])
+        io = O.OracleCSR(N, E, inv["out_degree"], inv["in_degree"], inv["edges_idx"], inv["src"], inv["dst"], inv["w"])
+        t0 = time.perf_counter()
+        O.pr_pull_f32(fo, io, iterations, 0.0, nthreads=threads)
+        secs = time.perf_counter() - t0
+    gteps = E * iterations / secs / 1e9
+    return dict(value=round(gteps, 4), unit="GTEPS", cores=threads, kind=kind,
+                sample=f"{iterations} iterations of pull PageRank (FP32, reference semantics) on the same "
+                       f"R-MAT graph (N={N}, E={E}), {threads} OpenMP threads, whole loop timed")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--cpu-iters", type=int, default=5)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
+                                   "(see DESIGN.md), so --gpus must be 1"}))
+        sys.exit(2)
+
+    import graphblox_amd as gbx
+
+    ctx = gbx.Context(0)
+    t0 = time.perf_counter()
+    g = ctx.rmat_graph(args.scale, args.edge_factor, SEED)
+    log(f"R-MAT s{args.scale}: N={g.N} E={g.E} built on GPU in {time.perf_counter() - t0:.2f}s")
+    pr = g.session(iterations=args.warmup + args.steps, epsilon=0.0, mode=gbx.PR_REF, stop=gbx.STOP_NONE)
+    pr.reset()
+    pr.launch(args.warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    kernel_ms = pr.launch_timed(args.steps)  # HIP events around every iteration kernel; syncs at the end
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ranks, rep = pr.read()
+    assert rep["executed"] == args.warmup + args.steps
+    assert np.isfinite(ranks).all()
+    ms_per_step = wall * 1e3 / args.steps
+    N, E = g.N, g.E
+    value = E / (ms_per_step * 1e-3) / 1e9
+    bytes_per_launch = 8 * E + 16 * N + 4
+    achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+    wk = f"pr_pull_f32_rmat_s{args.scale}"
+    traffic = traffic_from_profiles(wk)
+    log(f"{ms_per_step:.4f} ms/iter wall, kernel {kernel_ms:.4f} ms, {value:.1f} GTEPS, "
+        f"{achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of peak)")
+    for c in pr.plan_info():
+        log(f"  bin lanes={c['lanes']:>3} rows={c['rows']:>9} edges={c['edges']:>10}")
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline ...")
+        cpu = cpu_baseline(g, args.cpu_iters, args.cpu_threads)
+        log(f"CPU baseline {cpu['value']} GTEPS ({cpu['kind']}, {cpu['cores']} threads)")
+    out = {
+        "metric": "GTEPS & % HBM roofline, pull PageRank R-MAT s24, 1 MI355X, vs plain FP32",
+        "value": round(value, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded R-MAT generated on the GPU)",
+        "config": {
+            "workload": f"pull PageRank, reference semantics (pageRank.c), FP32, R-MAT scale {args.scale} "
+                        f"edge factor {args.edge_factor} a=.57 b=.19 c=.19 d=.05, ids permuted, "
+                        f"self loops and duplicates removed",
+            "scale": args.scale, "edge_factor": args.edge_factor, "seed": SEED,
+            "num_vertices": N, "num_edges": E, "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "pr_iter_kernel<K_PR_REF>",
+            "kernel_ms": round(kernel_ms, 5),
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    pr.free()
+    g.free()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

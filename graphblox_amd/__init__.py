@@ -1,0 +1,303 @@
+"""graphblox_amd -- MI355X (gfx950) backend for GraphBlox's pull PageRank / SpMV path.
+
+The product is the native C ABI in ``libgbx_hip.so`` (declared in
+``include/gbx.h``) and the reference-API adapter ``libgraphblox_hip.so``
+(``include/graphblox_hip.h``).  This module is a thin ctypes mirror used by
+tests and the benchmark: every call goes straight to the HIP library, and a
+missing or unloadable library raises instead of falling back to anything.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_HIP = os.path.join(PKG, "libgbx_hip.so")
+LIB_HOST = os.path.join(PKG, "libgraphblox_hip.so")
+
+GBX_OK = 0
+SORT_RADIX_SRC, SORT_RADIX_SRC_DST, SORT_COUNT_SRC, SORT_COUNT_SRC_DST = 0, 1, 2, 3
+GRAPH_DEDUPE, GRAPH_WEIGHTED, GRAPH_SYMMETRIC = 1, 2, 4
+PR_REF, PR_STOCHASTIC = 0, 1
+STOP_ACTIVE, STOP_L1, STOP_NONE = 0, 1, 2
+F32, BF16 = 0, 1
+PULL_IN, PULL_FWD = 0, 1
+
+
+class GbxError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"gbx status {status}: {msg}")
+        self.status = status
+
+
+class PrParams(C.Structure):
+    _fields_ = [("damp", C.c_float), ("max_iters", C.c_uint32), ("epsilon", C.c_double),
+                ("mode", C.c_int), ("stop", C.c_int), ("precision", C.c_int),
+                ("direction", C.c_int), ("weighted", C.c_int), ("reserved", C.c_int)]
+
+
+class PrReport(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("executed", C.c_uint32), ("active", C.c_uint32),
+                ("converged", C.c_uint32), ("error_total", C.c_double), ("rank_sum", C.c_double),
+                ("time_total_s", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+_vp = C.c_void_p
+
+
+def lib():
+    """Load libgbx_hip.so (built by ``python -m graphblox_amd.build``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_HIP):
+        raise RuntimeError(f"{LIB_HIP} is not built; run __graft_entry__.build() first")
+    L = C.CDLL(LIB_HIP)
+    u32, u64, i32, f64 = C.c_uint32, C.c_uint64, C.c_int, C.c_double
+    sig = {
+        "gbx_init": (i32, [i32, C.POINTER(_vp)]),
+        "gbx_free": (i32, [_vp]),
+        "gbx_last_error": (C.c_char_p, [_vp]),
+        "gbx_status_string": (C.c_char_p, [i32]),
+        "gbx_sync": (i32, [_vp]),
+        "gbx_stream": (_vp, [_vp]),
+        "gbx_version": (C.c_char_p, []),
+        "gbx_csr_build": (i32, [_vp, _vp, _vp, _vp, u32, u32, i32, i32, _vp, _vp, _vp, _vp, _vp]),
+        "gbx_graph_create": (i32, [_vp, _vp, _vp, _vp, u32, u32, i32, C.c_uint, C.POINTER(_vp)]),
+        "gbx_graph_create_rmat": (i32, [_vp, u32, u32, f64, f64, f64, u64, i32, u64, i32, C.c_uint,
+                                        C.POINTER(_vp)]),
+        "gbx_rmat_edges": (i32, [_vp, u32, u32, f64, f64, f64, u64, i32, i32, u64, _vp, _vp, _vp,
+                                 C.POINTER(u64)]),
+        "gbx_graph_info": (i32, [_vp, C.POINTER(u32), C.POINTER(u32)]),
+        "gbx_graph_export": (i32, [_vp, i32, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "gbx_graph_free": (i32, [_vp]),
+        "gbx_pr_params_default": (None, [C.POINTER(PrParams)]),
+        "gbx_pr_pull": (i32, [_vp, _vp, C.POINTER(PrParams), _vp, _vp, C.POINTER(PrReport)]),
+        "gbx_pr_create": (i32, [_vp, _vp, C.POINTER(PrParams), C.POINTER(_vp)]),
+        "gbx_pr_reset": (i32, [_vp, _vp]),
+        "gbx_pr_launch": (i32, [_vp, u32]),
+        "gbx_pr_launch_timed": (i32, [_vp, u32, C.POINTER(f64)]),
+        "gbx_pr_read": (i32, [_vp, _vp, C.POINTER(PrReport)]),
+        "gbx_pr_free": (i32, [_vp]),
+        "gbx_pr_plan_info": (i32, [_vp, C.POINTER(u32), _vp, _vp, _vp]),
+        "gbx_spmv_pull": (i32, [_vp, _vp, u32, i32, i32, _vp, _vp, _vp, C.POINTER(f64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names declared in include/gbx.h that this binding expects."""
+    return [
+        "gbx_init", "gbx_free", "gbx_last_error", "gbx_status_string", "gbx_sync", "gbx_stream",
+        "gbx_version", "gbx_csr_build", "gbx_graph_create", "gbx_graph_create_rmat",
+        "gbx_rmat_edges", "gbx_graph_info", "gbx_graph_export", "gbx_graph_free",
+        "gbx_pr_params_default", "gbx_pr_pull", "gbx_pr_create", "gbx_pr_reset", "gbx_pr_launch",
+        "gbx_pr_launch_timed", "gbx_pr_read", "gbx_pr_free", "gbx_pr_plan_info", "gbx_spmv_pull",
+    ]
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Context:
+    """A gbx_ctx on one HIP device."""
+
+    def __init__(self, device: int = 0):
+        self.L = lib()
+        h = _vp()
+        rc = self.L.gbx_init(device, C.byref(h))
+        if rc != GBX_OK:
+            raise GbxError(rc, self.L.gbx_last_error(None).decode())
+        self.h = h
+
+    def check(self, rc: int):
+        if rc != GBX_OK:
+            raise GbxError(rc, self.L.gbx_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gbx_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        self.check(self.L.gbx_sync(self.h))
+
+    # -- CSR build
+    def csr_build(self, src, dst, w, num_vertices, sort_mode, move_weights=False):
+        src, dst, w = _u32(src), _u32(dst), _f32(w)
+        E = src.size
+        out = dict(src=np.empty(E, np.uint32), dst=np.empty(E, np.uint32), w=np.empty(E, np.float32),
+                   out_degree=np.empty(num_vertices, np.uint32), edges_idx=np.empty(num_vertices, np.uint32))
+        self.check(self.L.gbx_csr_build(self.h, _ptr(src), _ptr(dst), _ptr(w), E, num_vertices, sort_mode,
+                                        int(move_weights), _ptr(out["src"]), _ptr(out["dst"]), _ptr(out["w"]),
+                                        _ptr(out["out_degree"]), _ptr(out["edges_idx"])))
+        return out
+
+    def graph(self, src, dst, w=None, num_vertices=0, sort_mode=SORT_RADIX_SRC_DST, flags=0) -> "Graph":
+        src, dst, w = _u32(src), _u32(dst), _f32(w)
+        h = _vp()
+        self.check(self.L.gbx_graph_create(self.h, _ptr(src), _ptr(dst), _ptr(w), src.size, num_vertices,
+                                           sort_mode, flags, C.byref(h)))
+        return Graph(self, h)
+
+    def rmat_graph(self, scale, edge_factor, seed, a=0.57, b=0.19, c=0.19, weight_kind=0, weight_seed=0,
+                   sort_mode=SORT_RADIX_SRC_DST, flags=0) -> "Graph":
+        h = _vp()
+        self.check(self.L.gbx_graph_create_rmat(self.h, scale, edge_factor, a, b, c, seed, weight_kind,
+                                                weight_seed, sort_mode, flags, C.byref(h)))
+        return Graph(self, h)
+
+    def rmat_edges(self, scale, edge_factor, seed, a=0.57, b=0.19, c=0.19, dedupe=True, weight_kind=0,
+                   weight_seed=0):
+        M = edge_factor << scale
+        s = np.empty(M, np.uint32)
+        d = np.empty(M, np.uint32)
+        w = np.empty(M, np.float32)
+        n = C.c_uint64()
+        self.check(self.L.gbx_rmat_edges(self.h, scale, edge_factor, a, b, c, seed, int(dedupe), weight_kind,
+                                         weight_seed, _ptr(s), _ptr(d), _ptr(w), C.byref(n)))
+        k = n.value
+        return s[:k].copy(), d[:k].copy(), w[:k].copy()
+
+
+class Graph:
+    """A device-resident gbx_graph (forward + transpose CSR)."""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx = ctx
+        self.h = h
+        n, e = C.c_uint32(), C.c_uint32()
+        ctx.check(ctx.L.gbx_graph_info(h, C.byref(n), C.byref(e)))
+        self.N, self.E = n.value, e.value
+
+    def export(self, inverse=False):
+        N, E = self.N, self.E
+        o = dict(out_degree=np.empty(N, np.uint32), in_degree=np.empty(N, np.uint32),
+                 edges_idx=np.empty(N, np.uint32), src=np.empty(E, np.uint32), dst=np.empty(E, np.uint32),
+                 w=np.empty(E, np.float32))
+        self.ctx.check(self.ctx.L.gbx_graph_export(self.h, int(inverse), *(_ptr(o[k]) for k in
+                                                   ("out_degree", "in_degree", "edges_idx", "src", "dst", "w"))))
+        return o
+
+    def free(self):
+        if self.h:
+            self.ctx.L.gbx_graph_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def pagerank(self, iterations=20, epsilon=1e-4, mode=PR_REF, stop=STOP_ACTIVE, damp=0.85,
+                 direction=PULL_IN, weighted=False, precision=F32, teleport=None):
+        p = params(iterations, epsilon, mode, stop, damp, direction, weighted, precision)
+        ranks = np.empty(self.N, np.float32)
+        rep = PrReport()
+        tp = _f32(teleport)
+        self.ctx.check(self.ctx.L.gbx_pr_pull(self.ctx.h, self.h, C.byref(p), _ptr(tp), _ptr(ranks), C.byref(rep)))
+        return ranks, rep.as_dict()
+
+    def spmv(self, iterations=1, weighted=False, direction=PULL_IN, x=None):
+        xo = np.empty(self.N, np.float32)
+        yo = np.empty(self.N, np.float32)
+        sec = C.c_double()
+        xi = _f32(x)
+        self.ctx.check(self.ctx.L.gbx_spmv_pull(self.ctx.h, self.h, iterations, int(weighted), direction, _ptr(xi),
+                                                _ptr(xo), _ptr(yo), C.byref(sec)))
+        return xo, yo, sec.value
+
+    def session(self, **kw) -> "PageRank":
+        return PageRank(self, params(**kw))
+
+
+def params(iterations=20, epsilon=1e-4, mode=PR_REF, stop=STOP_ACTIVE, damp=0.85, direction=PULL_IN,
+           weighted=False, precision=F32) -> PrParams:
+    p = PrParams()
+    lib().gbx_pr_params_default(C.byref(p))
+    p.damp = damp
+    p.max_iters = iterations
+    p.epsilon = epsilon
+    p.mode = mode
+    p.stop = stop
+    p.precision = precision
+    p.direction = direction
+    p.weighted = int(weighted)
+    return p
+
+
+class PageRank:
+    """Step-wise gbx_pr object (create / reset / launch / read)."""
+
+    def __init__(self, graph: Graph, p: PrParams):
+        self.g = graph
+        self.ctx = graph.ctx
+        h = _vp()
+        self.ctx.check(self.ctx.L.gbx_pr_create(self.ctx.h, graph.h, C.byref(p), C.byref(h)))
+        self.h = h
+
+    def reset(self, teleport=None):
+        tp = _f32(teleport)
+        self.ctx.check(self.ctx.L.gbx_pr_reset(self.h, _ptr(tp)))
+
+    def launch(self, n):
+        self.ctx.check(self.ctx.L.gbx_pr_launch(self.h, n))
+
+    def launch_timed(self, n) -> float:
+        ms = C.c_double()
+        self.ctx.check(self.ctx.L.gbx_pr_launch_timed(self.h, n, C.byref(ms)))
+        return ms.value
+
+    def read(self):
+        ranks = np.empty(self.g.N, np.float32)
+        rep = PrReport()
+        self.ctx.check(self.ctx.L.gbx_pr_read(self.h, _ptr(ranks), C.byref(rep)))
+        return ranks, rep.as_dict()
+
+    def plan_info(self):
+        n = C.c_uint32()
+        lanes = np.zeros(16, np.uint32)
+        rows = np.zeros(16, np.uint32)
+        edges = np.zeros(16, np.uint64)
+        self.ctx.check(self.ctx.L.gbx_pr_plan_info(self.h, C.byref(n), _ptr(lanes), _ptr(rows), _ptr(edges)))
+        k = n.value
+        return [dict(lanes=int(lanes[i]), rows=int(rows[i]), edges=int(edges[i])) for i in range(k)]
+
+    def free(self):
+        if self.h:
+            self.ctx.L.gbx_pr_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,205 @@
+/*
+ * gbx.h -- C ABI of libgbx_hip.so, the MI355X (gfx950) backend for the
+ * GraphBlox vertex-centric pull PageRank / SpMV path.
+ *
+ * Plain C: plain pointers and sizes, no HIP or torch types.  Every entry
+ * point returns an int status (GBX_OK = 0, negative on error) and leaves a
+ * message retrievable with gbx_last_error().  Host pointers are borrowed for
+ * the duration of the call.  Device memory is owned by the objects below.
+ * One context per host thread; objects are not thread safe.
+ *
+ * Reference interfaces each group replaces (paths relative to the GraphBlox
+ * repository root):
+ *   gbx_csr_build, gbx_graph_create*  sortRunAlgorithms (01_host/src/preprocess/sortRun.c:26-69),
+ *                                     countSortEdgesBySource/-AndDestination (countsort.c:28-270),
+ *                                     mapVerticesWithInOutDegree_base (01_host/src/structures/vertex.c:204-328),
+ *                                     removeDulpicatesSelfLoopEdges (edgeList.c:255-321),
+ *                                     graphCSRPreProcessingStep's forward+transpose pipeline
+ *                                     (01_host/src/structures/graphCSR.c:243-327)
+ *   gbx_pr_*                          pageRankPullGraphCSR (01_host/src/algorithms/openmp/pageRank.c:245-368)
+ *                                     and its stats (pageRank.h:19-30)
+ *   gbx_spmv_pull                     SPMVPullGraphCSR (01_host/src/algorithms/openmp/SPMV.c:117-215)
+ *   gbx_ppr_batched, gbx_graph_batch_*, gbx_pr_adjoint, gbx_rmat_*
+ *                                     no reference counterpart (batched, differentiable
+ *                                     and generator forms of the same path)
+ */
+#ifndef GBX_H
+#define GBX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define GBX_OK 0
+#define GBX_ERR_ARG (-1)     /* invalid argument (NULL, size, mode) */
+#define GBX_ERR_NOMEM (-2)   /* device or host allocation failed */
+#define GBX_ERR_HIP (-3)     /* HIP runtime error */
+#define GBX_ERR_RANGE (-4)   /* vertex id >= 2^30 or beyond num_vertices */
+#define GBX_ERR_STATE (-5)   /* object used in a state it does not support */
+
+typedef struct gbx_ctx gbx_ctx;
+typedef struct gbx_graph gbx_graph;
+typedef struct gbx_pr gbx_pr;
+
+/* Create a context on HIP device `device` (its own non-blocking stream). */
+int gbx_init(int device, gbx_ctx **out);
+int gbx_free(gbx_ctx *ctx);
+/* Last error message for this context (never NULL). */
+const char *gbx_last_error(const gbx_ctx *ctx);
+const char *gbx_status_string(int status);
+/* Wait for all work on the context's stream. */
+int gbx_sync(gbx_ctx *ctx);
+/* The context's hipStream_t, as an opaque pointer. */
+void *gbx_stream(gbx_ctx *ctx);
+/* Library build id string (architecture and version). */
+const char *gbx_version(void);
+
+/* ---------------------------------------------------------------- CSR build */
+/* Sort modes of sortRunAlgorithms (sortRun.c:26-69): 0 radix by source,
+ * 1 radix by source+destination, 2 count by source, 3 count by
+ * source+destination.  0==2 and 1==3 produce identical arrays. */
+#define GBX_SORT_RADIX_SRC 0
+#define GBX_SORT_RADIX_SRC_DST 1
+#define GBX_SORT_COUNT_SRC 2
+#define GBX_SORT_COUNT_SRC_DST 3
+
+/* One CSR from a host edge list, on the GPU, bit-exact with the reference:
+ * stable sort (mode as above), then out_degree[v] / edges_idx[v]
+ * (edges_idx[v] = 0 when out_degree[v] = 0, vertex.c:39-45).
+ * w may be NULL; with move_weights the weights follow the sort
+ * (`#if WEIGHTED`, countsort.c:118-120), else sorted_w receives w unchanged.
+ * Outputs: sorted_src/sorted_dst/sorted_w [num_edges] (nullable),
+ * out_degree/edges_idx [num_vertices] (nullable).
+ * Ids must be < num_vertices <= 2^30. */
+int gbx_csr_build(gbx_ctx *ctx, const uint32_t *src, const uint32_t *dst,
+                  const float *w, uint32_t num_edges, uint32_t num_vertices,
+                  int sort_mode, int move_weights, uint32_t *sorted_src,
+                  uint32_t *sorted_dst, float *sorted_w, uint32_t *out_degree,
+                  uint32_t *edges_idx);
+
+/* ---------------------------------------------------------------- graphs */
+#define GBX_GRAPH_DEDUPE 1u     /* -k: (src,dst) sort + removeDulpicatesSelfLoopEdges */
+#define GBX_GRAPH_WEIGHTED 2u   /* WEIGHTED=1: weights move through the sorts */
+#define GBX_GRAPH_SYMMETRIC 4u  /* -s: append the reversed edges (edgeList.c:560-578) */
+
+/* Device-resident graph: forward CSR and transpose (in-)CSR built on the GPU
+ * from a host edge list, in graphCSRPreProcessingStep's order
+ * (graphCSR.c:243-327, DIRECTED=1).  num_vertices = 0 selects the
+ * reference's rule N = max id + 1 (edgeList.c:671-679). */
+int gbx_graph_create(gbx_ctx *ctx, const uint32_t *src, const uint32_t *dst,
+                     const float *w, uint32_t num_edges, uint32_t num_vertices,
+                     int sort_mode, unsigned flags, gbx_graph **out);
+
+/* Seeded R-MAT graph generated on the GPU: (edge_factor << scale) edges,
+ * quadrant probabilities a, b, c, d = 1-a-b-c, ids relabelled by a seeded
+ * bijection of [0, 2^scale), self loops and duplicates removed (first
+ * occurrence kept, generation order preserved), N = max id + 1.
+ * weight_kind: 0 none (1.0f), 1 uniform(0,1), 2 lognormal(0,1); weights are
+ * keyed by (weight_seed, kept-edge index). */
+int gbx_graph_create_rmat(gbx_ctx *ctx, uint32_t scale, uint32_t edge_factor,
+                          double a, double b, double c, uint64_t seed,
+                          int weight_kind, uint64_t weight_seed, int sort_mode,
+                          unsigned flags, gbx_graph **out);
+
+/* The generated edge list itself (host copies), for checking the generator:
+ * src/dst/w sized for edge_factor << scale entries; *num_edges receives the
+ * kept count (after self-loop/duplicate removal when dedupe != 0). */
+int gbx_rmat_edges(gbx_ctx *ctx, uint32_t scale, uint32_t edge_factor, double a,
+                   double b, double c, uint64_t seed, int dedupe, int weight_kind,
+                   uint64_t weight_seed, uint32_t *src, uint32_t *dst, float *w,
+                   uint64_t *num_edges);
+
+int gbx_graph_info(const gbx_graph *g, uint32_t *num_vertices, uint32_t *num_edges);
+
+/* Copy the forward (inverse = 0) or transpose (inverse = 1) CSR arrays to
+ * host, in the reference's layout (struct Vertex / struct EdgeList of
+ * vertex.h:17-23, edgeList.h:12-25).  Any output may be NULL. */
+int gbx_graph_export(const gbx_graph *g, int inverse, uint32_t *out_degree,
+                     uint32_t *in_degree, uint32_t *edges_idx, uint32_t *src,
+                     uint32_t *dst, float *w);
+
+int gbx_graph_free(gbx_graph *g);
+
+/* ---------------------------------------------------------------- PageRank */
+#define GBX_PR_REF 0         /* pageRank.c semantics: r0 = 1-d, r' = (1-d) + d*sum, no dangling
+                                redistribution, output r/N */
+#define GBX_PR_STOCHASTIC 1  /* r0 = t, r' = (1-d) t + d (sum + D/N), D = dangling mass,
+                                normalised by the previous sum in FP64 */
+
+#define GBX_STOP_ACTIVE 0    /* stop when no vertex has |dr| >= epsilon (pageRank.c:327-339) */
+#define GBX_STOP_L1 1        /* stop when sum |dr| < epsilon (normalised scale) */
+#define GBX_STOP_NONE 2      /* run exactly max_iters */
+
+#define GBX_F32 0            /* FP32 contributions and row sums */
+#define GBX_BF16 1           /* BF16 contribution gathers, FP32 Kahan row sums */
+
+#define GBX_PULL_IN 0        /* rows = transpose CSR (DIRECTED=1, pageRank.c:266-268) */
+#define GBX_PULL_FWD 1       /* rows = forward CSR (DIRECTED=0, pageRank.c:269-272) */
+
+typedef struct gbx_pr_params {
+    float damp;          /* damping factor, 0.85f in the reference (pageRank.h:13) */
+    uint32_t max_iters;  /* arguments->iterations */
+    double epsilon;      /* threshold for the stop rule */
+    int mode;            /* GBX_PR_REF / GBX_PR_STOCHASTIC */
+    int stop;            /* GBX_STOP_* */
+    int precision;       /* GBX_F32 / GBX_BF16 */
+    int direction;       /* GBX_PULL_IN / GBX_PULL_FWD */
+    int weighted;        /* 1: transition = edge weight / sum of the source's out-weights */
+    int reserved;
+} gbx_pr_params;
+
+typedef struct gbx_pr_report {
+    uint32_t iterations;   /* reference convention: index of the stopping iteration, or max_iters */
+    uint32_t executed;     /* iterations actually executed */
+    uint32_t active;       /* active vertices in the last executed iteration */
+    uint32_t converged;    /* 1 if the stop rule fired */
+    double error_total;    /* REF: sum|dr|/N; STOCHASTIC: sum|dr| of the last iteration */
+    double rank_sum;       /* FP64 sum of the returned ranks */
+    double time_total_s;   /* wall time of the iteration loop (device, HIP events) */
+} gbx_pr_report;
+
+/* Fill params with the reference defaults (damp 0.85, 20 iterations,
+ * epsilon 1e-4, REF, ACTIVE stop, F32, PULL_IN, unweighted). */
+void gbx_pr_params_default(gbx_pr_params *p);
+
+/* One call: upload-free run on a device-resident graph to the stop rule,
+ * ranks (normalised, original vertex order) copied to ranks_out[N].
+ * teleport: NULL (uniform) or host array [N] summing to 1 (STOCHASTIC). */
+int gbx_pr_pull(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p,
+                const float *teleport, float *ranks_out, gbx_pr_report *rep);
+
+/* Step-wise interface: create (builds or reuses the degree-binned plan),
+ * reset (initial ranks), launch n iterations asynchronously (iterations after
+ * the stop rule fired are no-ops on the device), read back. */
+int gbx_pr_create(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, gbx_pr **out);
+int gbx_pr_reset(gbx_pr *pr, const float *teleport);
+int gbx_pr_launch(gbx_pr *pr, uint32_t n_iters);
+/* As gbx_pr_launch, with a HIP event pair around every iteration kernel on
+ * the context stream; *kernel_ms_avg receives the mean kernel duration. */
+int gbx_pr_launch_timed(gbx_pr *pr, uint32_t n_iters, double *kernel_ms_avg);
+int gbx_pr_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep);
+int gbx_pr_free(gbx_pr *pr);
+
+/* Plan statistics: counts of rows per degree class and edges per class
+ * (for reporting and the autotuner); arrays of length *n_classes (<= 16). */
+int gbx_pr_plan_info(const gbx_pr *pr, uint32_t *n_classes, uint32_t *class_lanes,
+                     uint32_t *class_rows, uint64_t *class_edges);
+
+/* ---------------------------------------------------------------- SpMV */
+/* SPMVPullGraphCSR semantics: x[v] = 1/outdeg[v] (0.001f when 0) unless
+ * x_in is given; y starts at 0 and accumulates y += w[j] x[col[j]] over
+ * `iterations` passes without reset; w = 0.0001f unless weighted.
+ * Outputs in original vertex order; x_out/y_out [N] nullable;
+ * *seconds (nullable) receives the device time of the iterations. */
+int gbx_spmv_pull(gbx_ctx *ctx, gbx_graph *g, uint32_t iterations, int weighted,
+                  int direction, const float *x_in, float *x_out, float *y_out,
+                  double *seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GBX_H */

@@ -1,0 +1,436 @@
+/*
+ * gbx_oracle.c -- CPU oracle for the pull PageRank / SpMV path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see gbx_oracle.h): a clean-room restatement of
+ * the GraphBlox reference algorithms used to check the HIP path and to time
+ * the CPU baseline.  Never linked into the product library.
+ */
+#include "gbx_oracle.h"
+
+#include <math.h>
+#include <omp.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#define VERTEX_ID_MASK 0x3FFFFFFFu /* reorder.h:23,30 (EXTRACT_VALUE) */
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* One stable counting-sort pass keyed on key[] (countsort.c:28-140 with P=1;
+ * the reference's P-thread version produces the same order because its scan
+ * is vertex-major, thread-minor over contiguous chunks, countsort.c:93-104). */
+static int stable_pass(uint32_t *key, uint32_t *other, float *w,
+                       uint32_t num_edges, uint32_t num_vertices,
+                       int move_weights, int key_is_src)
+{
+    uint32_t *count = calloc((size_t)num_vertices + 1, sizeof(uint32_t));
+    uint32_t *k2 = malloc((size_t)num_edges * sizeof(uint32_t) + 4);
+    uint32_t *o2 = malloc((size_t)num_edges * sizeof(uint32_t) + 4);
+    float *w2 = move_weights ? malloc((size_t)num_edges * sizeof(float) + 4) : NULL;
+    if (!count || !k2 || !o2 || (move_weights && !w2)) {
+        free(count); free(k2); free(o2); free(w2);
+        return -1;
+    }
+    (void)key_is_src;
+    for (uint32_t i = 0; i < num_edges; i++)
+        count[key[i]]++;
+    uint32_t base = 0;
+    for (uint32_t v = 0; v < num_vertices; v++) {
+        uint32_t c = count[v];
+        count[v] = base;
+        base += c;
+    }
+    for (uint32_t i = 0; i < num_edges; i++) {
+        uint32_t pos = count[key[i]]++;
+        k2[pos] = key[i];
+        o2[pos] = other[i];
+        if (move_weights) w2[pos] = w[i];
+    }
+    memcpy(key, k2, (size_t)num_edges * 4);
+    memcpy(other, o2, (size_t)num_edges * 4);
+    if (move_weights) memcpy(w, w2, (size_t)num_edges * 4);
+    free(count); free(k2); free(o2); free(w2);
+    return 0;
+}
+
+int orc_count_sort(uint32_t *src, uint32_t *dst, float *w, uint32_t num_edges,
+                   uint32_t num_vertices, int mode, int move_weights)
+{
+    if (num_edges == 0) return 0;
+    if (move_weights && !w) return -1;
+    /* countSortEdgesBySourceAndDestination = by destination, then by source
+     * (countsort.c:262-270). */
+    if (mode == 1 || mode == 3) {
+        if (stable_pass(dst, src, w, num_edges, num_vertices, move_weights, 0))
+            return -1;
+    }
+    return stable_pass(src, dst, w, num_edges, num_vertices, move_weights, 1);
+}
+
+int orc_map_vertices(const uint32_t *sorted_src, uint32_t num_edges,
+                     uint32_t num_vertices, uint32_t *out_degree,
+                     uint32_t *edges_idx)
+{
+    memset(out_degree, 0, (size_t)num_vertices * 4);
+    memset(edges_idx, 0, (size_t)num_vertices * 4);
+    for (uint32_t i = 0; i < num_edges; i++) {
+        uint32_t v = sorted_src[i] & VERTEX_ID_MASK;
+        out_degree[v]++;
+        if (i == 0 || sorted_src[i] != sorted_src[i - 1])
+            edges_idx[v] = i;
+    }
+    return 0;
+}
+
+uint32_t orc_remove_duplicates(uint32_t *src, uint32_t *dst, float *w,
+                               uint32_t num_edges)
+{
+    uint32_t i = 0, j = 0;
+    /* edgeList.c:266-274 skips a leading run of self loops. */
+    while (i < num_edges && src[i] == dst[i]) i++;
+    if (i == num_edges) return 0;
+    for (; i < num_edges; i++) {
+        uint32_t s = src[i], d = dst[i];
+        uint32_t wi = (uint32_t)w[i]; /* edgeList.c:261,270: uint32_t temp */
+        if (s == d) continue;
+        if (j > 0 && src[j - 1] == s && dst[j - 1] == d) continue;
+        src[j] = s;
+        dst[j] = d;
+        w[j] = (float)wi;
+        j++;
+    }
+    return j;
+}
+
+int orc_pr_pull_f32(uint32_t N, const uint32_t *fwd_out_degree,
+                    const uint32_t *row_degree, const uint32_t *row_idx,
+                    const uint32_t *row_col, uint32_t max_iters, double epsilon,
+                    int nthreads, float *ranks_out, uint32_t *iters_out,
+                    double *err_out, double *iter_seconds)
+{
+    /* pageRank.c:54-55,68: damp = 0.85f, base_pr = 1.0f - damp, r0 = base_pr */
+    const float damp = 0.85f;
+    const float base_pr = 1.0f - damp;
+    float *r = ranks_out;
+    float *next = malloc((size_t)N * sizeof(float) + 4);
+    float *contrib = malloc((size_t)N * sizeof(float) + 4);
+    if (!next || !contrib) { free(next); free(contrib); return -1; }
+    if (nthreads < 1) nthreads = 1;
+    double error_total = 0.0;
+    uint32_t it;
+
+#pragma omp parallel for num_threads(nthreads)
+    for (uint32_t v = 0; v < N; v++) {
+        r[v] = base_pr;
+        next[v] = 0.0f;
+    }
+
+    for (it = 0; it < max_iters; it++) {
+        double t0 = now_s();
+        uint32_t active = 0;
+        error_total = 0.0;
+        /* phase 1: contribution r/outdeg (pageRank.c:295-302) */
+#pragma omp parallel for num_threads(nthreads)
+        for (uint32_t v = 0; v < N; v++)
+            contrib[v] = fwd_out_degree[v] ? r[v] / (float)fwd_out_degree[v] : 0.0f;
+        /* phase 2: sequential float gather per row (pageRank.c:304-317) */
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(nthreads)
+        for (uint32_t v = 0; v < N; v++) {
+            float acc = 0.0f;
+            uint32_t beg = row_idx[v], end = beg + row_degree[v];
+            for (uint32_t j = beg; j < end; j++)
+                acc += contrib[row_col[j] & VERTEX_ID_MASK];
+            next[v] = acc;
+        }
+        /* phase 3: update, error and active count (pageRank.c:320-334) */
+#pragma omp parallel for num_threads(nthreads) reduction(+ : error_total, active)
+        for (uint32_t v = 0; v < N; v++) {
+            float prev = r[v];
+            float nr = base_pr + damp * next[v];
+            r[v] = nr;
+            next[v] = 0.0f;
+            double e = fabs(nr - prev);
+            error_total += e / N;
+            if (e >= epsilon) active++;
+        }
+        if (iter_seconds) iter_seconds[it] = now_s() - t0;
+        if (active == 0) break; /* pageRank.c:338-339 */
+    }
+    /* pageRank.c:344-350 */
+#pragma omp parallel for num_threads(nthreads)
+    for (uint32_t v = 0; v < N; v++)
+        r[v] = r[v] / N;
+    if (iters_out) *iters_out = it;
+    if (err_out) *err_out = error_total;
+    free(next);
+    free(contrib);
+    return 0;
+}
+
+int orc_spmv_pull_f32(uint32_t N, const uint32_t *fwd_out_degree,
+                      const uint32_t *row_degree, const uint32_t *row_idx,
+                      const uint32_t *row_col, const float *w,
+                      uint32_t iterations, int nthreads, float *x_out,
+                      float *y_out, double *iter_seconds)
+{
+    if (nthreads < 1) nthreads = 1;
+    /* SPMV.c:156-163 */
+#pragma omp parallel for num_threads(nthreads)
+    for (uint32_t v = 0; v < N; v++) {
+        x_out[v] = fwd_out_degree[v] ? 1.0f / fwd_out_degree[v] : 0.001f;
+        y_out[v] = 0.0f;
+    }
+    for (uint32_t it = 0; it < iterations; it++) {
+        double t0 = now_s();
+        /* SPMV.c:167-194: y accumulates across iterations, never reset */
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(nthreads)
+        for (uint32_t v = 0; v < N; v++) {
+            uint32_t beg = row_idx[v], end = beg + row_degree[v];
+            for (uint32_t j = beg; j < end; j++) {
+                float weight = w ? w[j] : 0.0001f;
+                y_out[v] += weight * x_out[row_col[j] & VERTEX_ID_MASK];
+            }
+        }
+        if (iter_seconds) iter_seconds[it] = now_s() - t0;
+    }
+    return 0;
+}
+
+int orc_pr_f64(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
+               const uint32_t *col, const double *w, const double *teleport,
+               double damp, int mode, uint32_t max_iters, double epsilon,
+               double *ranks_out, uint32_t *iters_out, double *err_out)
+{
+    double *r = ranks_out;
+    double *c = malloc((size_t)N * sizeof(double) + 8);
+    double *nx = malloc((size_t)N * sizeof(double) + 8);
+    if (!c || !nx) { free(c); free(nx); return -1; }
+    const double base = 1.0 - damp;
+    uint32_t it = 0;
+    double err = 0.0;
+    for (uint32_t v = 0; v < N; v++)
+        r[v] = mode == 0 ? base : (teleport ? teleport[v] : 1.0 / N);
+    for (it = 0; it < max_iters; it++) {
+        double dang = 0.0;
+        for (uint32_t v = 0; v < N; v++) {
+            if (outdeg[v]) {
+                c[v] = w ? r[v] : r[v] / outdeg[v];
+            } else {
+                c[v] = 0.0;
+                dang += r[v];
+            }
+        }
+        uint32_t active = 0;
+        err = 0.0;
+        for (uint32_t v = 0; v < N; v++) {
+            double acc = 0.0;
+            for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++)
+                acc += w ? w[j] * c[col[j]] : c[col[j]];
+            double nr;
+            if (mode == 0)
+                nr = base + damp * acc;
+            else
+                nr = base * (teleport ? teleport[v] : 1.0 / N) + damp * (acc + dang / N);
+            double e = fabs(nr - r[v]);
+            nx[v] = nr;
+            if (mode == 0) {
+                err += e / N;
+                if (e >= epsilon) active++;
+            } else {
+                err += e;
+            }
+        }
+        memcpy(r, nx, (size_t)N * sizeof(double));
+        if (mode == 0 ? active == 0 : err < epsilon) {
+            it++;
+            break;
+        }
+    }
+    if (mode == 0)
+        for (uint32_t v = 0; v < N; v++) r[v] /= N;
+    if (iters_out) *iters_out = it;
+    if (err_out) *err_out = err;
+    free(c);
+    free(nx);
+    return 0;
+}
+
+int orc_pr_adjoint_f64(uint32_t N, const uint64_t *row_ptr, const uint32_t *col,
+                       const double *wraw, const double *teleport,
+                       const double *g, double damp, uint32_t max_iters,
+                       double tol, double *p_out, double *lambda_out,
+                       double *grad_w_out, double *grad_t_out)
+{
+    double *S = calloc((size_t)N + 1, sizeof(double));
+    double *P = malloc((size_t)row_ptr[N] * sizeof(double) + 8);
+    double *Plam = malloc((size_t)N * sizeof(double) + 8);
+    double *tmp = malloc((size_t)N * sizeof(double) + 8);
+    if (!S || !P || !Plam || !tmp) { free(S); free(P); free(Plam); free(tmp); return -1; }
+    uint64_t E = row_ptr[N];
+    for (uint32_t v = 0; v < N; v++)
+        for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++)
+            S[col[j]] += wraw[j];
+    for (uint32_t v = 0; v < N; v++)
+        for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++)
+            P[j] = wraw[j] / S[col[j]];
+    /* forward fixed point */
+    for (uint32_t v = 0; v < N; v++) p_out[v] = teleport ? teleport[v] : 1.0 / N;
+    for (uint32_t it = 0; it < max_iters; it++) {
+        double dang = 0.0, err = 0.0;
+        for (uint32_t u = 0; u < N; u++)
+            if (S[u] == 0.0) dang += p_out[u];
+        for (uint32_t v = 0; v < N; v++) {
+            double acc = 0.0;
+            for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++)
+                acc += P[j] * p_out[col[j]];
+            tmp[v] = (1.0 - damp) * (teleport ? teleport[v] : 1.0 / N) + damp * (acc + dang / N);
+            err += fabs(tmp[v] - p_out[v]);
+        }
+        memcpy(p_out, tmp, (size_t)N * sizeof(double));
+        if (err < tol) break;
+    }
+    /* adjoint fixed point lambda = g + d (P lambda + delta (1.lambda)/N) */
+    for (uint32_t v = 0; v < N; v++) lambda_out[v] = g[v];
+    for (uint32_t it = 0; it < max_iters; it++) {
+        double sum = 0.0, err = 0.0;
+        for (uint32_t v = 0; v < N; v++) { Plam[v] = 0.0; sum += lambda_out[v]; }
+        for (uint32_t v = 0; v < N; v++)
+            for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++)
+                Plam[col[j]] += P[j] * lambda_out[v];
+        for (uint32_t u = 0; u < N; u++) {
+            double nl = g[u] + damp * (Plam[u] + (S[u] == 0.0 ? sum / N : 0.0));
+            err += fabs(nl - lambda_out[u]);
+            tmp[u] = nl;
+        }
+        memcpy(lambda_out, tmp, (size_t)N * sizeof(double));
+        if (err < tol) break;
+    }
+    /* (P lambda)[u] for the weight gradient */
+    for (uint32_t v = 0; v < N; v++) Plam[v] = 0.0;
+    for (uint32_t v = 0; v < N; v++)
+        for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++)
+            Plam[col[j]] += P[j] * lambda_out[v];
+    if (grad_w_out)
+        for (uint32_t v = 0; v < N; v++)
+            for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) {
+                uint32_t u = col[j];
+                grad_w_out[j] = damp * p_out[u] * (lambda_out[v] - Plam[u]) / S[u];
+            }
+    if (grad_t_out)
+        for (uint32_t v = 0; v < N; v++) grad_t_out[v] = (1.0 - damp) * lambda_out[v];
+    (void)E;
+    free(S); free(P); free(Plam); free(tmp);
+    return 0;
+}
+
+/* ---- R-MAT generator (counter-based; identical integer arithmetic on GPU) */
+
+static uint64_t mix64(uint64_t z)
+{
+    /* SplitMix64 finaliser (Steele, Lea, Flood 2014). */
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+uint32_t orc_rmat_permute(uint32_t x, uint32_t scale, uint64_t seed)
+{
+    if (scale == 0) return 0;
+    const uint32_t mask = scale >= 32 ? 0xFFFFFFFFu : ((1u << scale) - 1u);
+    const uint32_t sh = scale / 2 + 1;
+    for (uint32_t round = 0; round < 4; round++) {
+        uint64_t k = mix64(seed ^ (0xA5A5A5A5ull + round * 0x9E3779B97F4A7C15ull));
+        uint32_t mul = ((uint32_t)k | 1u);
+        uint32_t add = (uint32_t)(k >> 32);
+        x = (x * mul + add) & mask;  /* odd multiply + add: bijection mod 2^s */
+        x ^= x >> sh;                /* xorshift-right: bijection on s bits */
+    }
+    return x & mask;
+}
+
+int orc_rmat_generate(uint32_t scale, uint32_t edge_factor, double a, double b,
+                      double c, uint64_t seed, uint32_t *src, uint32_t *dst)
+{
+    const uint64_t M = (uint64_t)edge_factor << scale;
+    const uint32_t ta = (uint32_t)(a * 4294967296.0);
+    const uint32_t tab = (uint32_t)((a + b) * 4294967296.0);
+    const uint32_t tabc = (uint32_t)((a + b + c) * 4294967296.0);
+    const uint64_t key = mix64(seed);
+#pragma omp parallel for schedule(static)
+    for (uint64_t e = 0; e < M; e++) {
+        uint32_t u = 0, v = 0;
+        for (uint32_t l = 0; l < scale; l += 2) {
+            uint64_t h = mix64(key + (e << 5) + (l >> 1) + 1);
+            for (uint32_t q = 0; q < 2 && l + q < scale; q++) {
+                uint32_t r = q ? (uint32_t)(h >> 32) : (uint32_t)h;
+                uint32_t bu = r >= tab;                 /* quadrants c,d */
+                uint32_t bv = (r >= ta && r < tab) || r >= tabc; /* b,d */
+                u = (u << 1) | bu;
+                v = (v << 1) | bv;
+            }
+        }
+        src[e] = orc_rmat_permute(u, scale, seed);
+        dst[e] = orc_rmat_permute(v, scale, seed);
+    }
+    return 0;
+}
+
+/* LSD radix sort of (64-bit key, 32-bit index) pairs, 16-bit digits. */
+static int sort_key_index(uint64_t *key, uint32_t *idx, uint64_t n)
+{
+    uint64_t *k2 = malloc(n * sizeof(uint64_t) + 8);
+    uint32_t *i2 = malloc(n * sizeof(uint32_t) + 8);
+    uint64_t *cnt = malloc(65537 * sizeof(uint64_t));
+    if (!k2 || !i2 || !cnt) { free(k2); free(i2); free(cnt); return -1; }
+    for (int pass = 0; pass < 4; pass++) {
+        int sh = 16 * pass;
+        memset(cnt, 0, 65537 * sizeof(uint64_t));
+        for (uint64_t i = 0; i < n; i++) cnt[((key[i] >> sh) & 0xFFFF) + 1]++;
+        for (int d = 0; d < 65536; d++) cnt[d + 1] += cnt[d];
+        for (uint64_t i = 0; i < n; i++) {
+            uint64_t p = cnt[(key[i] >> sh) & 0xFFFF]++;
+            k2[p] = key[i];
+            i2[p] = idx[i];
+        }
+        memcpy(key, k2, n * sizeof(uint64_t));
+        memcpy(idx, i2, n * sizeof(uint32_t));
+    }
+    free(k2); free(i2); free(cnt);
+    return 0;
+}
+
+uint64_t orc_dedupe_keep_first(uint32_t *src, uint32_t *dst, float *w,
+                               uint64_t num_edges)
+{
+    uint64_t n = num_edges;
+    uint64_t *key = malloc(n * sizeof(uint64_t) + 8);
+    uint32_t *idx = malloc(n * sizeof(uint32_t) + 8);
+    uint8_t *keep = calloc(n + 1, 1);
+    if (!key || !idx || !keep) { free(key); free(idx); free(keep); return (uint64_t)-1; }
+    for (uint64_t i = 0; i < n; i++) {
+        key[i] = ((uint64_t)src[i] << 32) | dst[i];
+        idx[i] = (uint32_t)i;
+    }
+    if (sort_key_index(key, idx, n)) { free(key); free(idx); free(keep); return (uint64_t)-1; }
+    for (uint64_t i = 0; i < n; i++) {
+        int first = (i == 0) || key[i] != key[i - 1];
+        int loop = (uint32_t)(key[i] >> 32) == (uint32_t)key[i];
+        if (first && !loop) keep[idx[i]] = 1; /* stable sort: first = lowest index */
+    }
+    uint64_t j = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        if (!keep[i]) continue;
+        src[j] = src[i];
+        dst[j] = dst[i];
+        if (w) w[j] = w[i];
+        j++;
+    }
+    free(key); free(idx); free(keep);
+    return j;
+}

@@ -1,0 +1,302 @@
+"""CPU oracle for the pull PageRank / SpMV path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg import this module, and only as the checker or as the
+timed CPU baseline.  The product package (``graphblox_amd``) never imports it.
+
+It wraps two native libraries:
+
+* ``oracle/liboracle.so`` -- the clean-room C/OpenMP restatement in
+  ``gbx_oracle.c`` (each function there cites the reference lines it follows);
+* ``oracle/_ref/libgbxref_d<D>w<W>.so`` -- the GraphBlox reference's own host
+  sources compiled by ``oracle/Makefile`` (present only where the reference
+  sources were available at build time; optional everywhere).
+
+and adds numpy helpers: the ``.bin``/``.wbin`` reader with the reference's
+``N = max id + 1`` rule (01_host/src/structures/edgeList.c:450-706), the CSR
+text-fixture parser (01_host/src/structures/graphCSR.c:529-635 writes them),
+and dense FP64 closed forms used to pin the iterative oracles on tiny graphs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+
+_lib = None
+_ref = {}
+
+
+def lib():
+    """The clean-room oracle library (built by ``make -C oracle``)."""
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built; run `make -C oracle`")
+        L = C.CDLL(path)
+        L.orc_count_sort.argtypes = [_u32p, _u32p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_int]
+        L.orc_map_vertices.argtypes = [_u32p, C.c_uint32, C.c_uint32, _u32p, _u32p]
+        L.orc_remove_duplicates.argtypes = [_u32p, _u32p, _f32p, C.c_uint32]
+        L.orc_remove_duplicates.restype = C.c_uint32
+        L.orc_pr_pull_f32.argtypes = [C.c_uint32, _u32p, _u32p, _u32p, _u32p, C.c_uint32, C.c_double,
+                                      C.c_int, _f32p, C.POINTER(C.c_uint32), C.POINTER(C.c_double),
+                                      C.c_void_p]
+        L.orc_spmv_pull_f32.argtypes = [C.c_uint32, _u32p, _u32p, _u32p, _u32p, C.c_void_p, C.c_uint32,
+                                        C.c_int, _f32p, _f32p, C.c_void_p]
+        L.orc_pr_f64.argtypes = [C.c_uint32, _u32p, _u64p, _u32p, C.c_void_p, C.c_void_p, C.c_double,
+                                 C.c_int, C.c_uint32, C.c_double, _f64p, C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_double)]
+        L.orc_pr_adjoint_f64.argtypes = [C.c_uint32, _u64p, _u32p, _f64p, C.c_void_p, _f64p, C.c_double,
+                                         C.c_uint32, C.c_double, _f64p, _f64p, _f64p, _f64p]
+        L.orc_rmat_generate.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                        C.c_uint64, _u32p, _u32p]
+        L.orc_rmat_permute.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64]
+        L.orc_rmat_permute.restype = C.c_uint32
+        L.orc_dedupe_keep_first.argtypes = [_u32p, _u32p, C.c_void_p, C.c_uint64]
+        L.orc_dedupe_keep_first.restype = C.c_uint64
+        _lib = L
+    return _lib
+
+
+def ref_available(directed: int = 1, weighted: int = 0) -> bool:
+    return os.path.exists(os.path.join(HERE, "_ref", f"libgbxref_d{directed}w{weighted}.so"))
+
+
+def ref(directed: int = 1, weighted: int = 0):
+    """The reference's own code compiled from its sources (optional)."""
+    key = (directed, weighted)
+    if key not in _ref:
+        path = os.path.join(HERE, "_ref", f"libgbxref_d{directed}w{weighted}.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} not built (needs the reference sources; `make -C oracle ref`)")
+        L = C.CDLL(path)
+        L.refd_build.argtypes = [C.c_char_p, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.refd_build.restype = C.c_void_p
+        L.refd_num_vertices.argtypes = [C.c_void_p]
+        L.refd_num_vertices.restype = C.c_uint32
+        L.refd_num_edges.argtypes = [C.c_void_p]
+        L.refd_num_edges.restype = C.c_uint32
+        L.refd_export.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
+        L.refd_free.argtypes = [C.c_void_p]
+        L.refd_pagerank.argtypes = [C.c_void_p, C.c_uint32, C.c_double, C.c_int, _f32p,
+                                    C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.refd_spmv.argtypes = [C.c_void_p, C.c_uint32, C.c_int, _f32p, _f32p, C.POINTER(C.c_double)]
+        if directed:
+            L.refd_pagerank_on_csr.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p, _u32p, _u32p,
+                                               C.c_uint32, C.c_double, C.c_int, _f32p,
+                                               C.POINTER(C.c_uint32), C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double)]
+        _ref[key] = L
+    return _ref[key]
+
+
+# --------------------------------------------------------------------------- I/O
+
+def read_bin(path: str, weighted: bool = False):
+    """Edge list reader: records of (u32 src, u32 dst[, f32 w]); N = max id + 1
+    (01_host/src/structures/edgeList.c:495-519, 671-679)."""
+    raw = np.fromfile(path, dtype=np.uint32)
+    rec = 3 if weighted else 2
+    E = raw.size // rec
+    raw = raw[: E * rec].reshape(E, rec)
+    src = np.ascontiguousarray(raw[:, 0])
+    dst = np.ascontiguousarray(raw[:, 1])
+    w = np.ascontiguousarray(raw[:, 2]).view(np.float32) if weighted else np.ones(E, np.float32)
+    N = int(max(src.max(initial=0), dst.max(initial=0))) + 1 if E else 1
+    return src, dst, w, N
+
+
+def write_bin(path: str, src, dst, w=None):
+    if w is None:
+        rec = np.stack([src.astype(np.uint32), dst.astype(np.uint32)], axis=1)
+    else:
+        rec = np.stack([src.astype(np.uint32), dst.astype(np.uint32),
+                        np.asarray(w, np.float32).view(np.uint32)], axis=1)
+    np.ascontiguousarray(rec).tofile(path)
+
+
+def read_csr_text_fixture(prefix: str):
+    """Parse the reference's CSR text dump (graphCSR.c:587-604): .out_degree
+    and .edges_array_src start with a count line; the rest do not."""
+    def col(ext, header):
+        vals = np.loadtxt(prefix + ext, dtype=np.int64, ndmin=1)
+        return (int(vals[0]), vals[1:].astype(np.uint32)) if header else (None, vals.astype(np.uint32))
+    N, out_degree = col(".out_degree", True)
+    E, src = col(".edges_array_src", True)
+    _, dst = col(".edges_array_dest", False)
+    _, idx = col(".edges_idx", False)
+    _, ind = col(".in_degree", False)
+    return dict(N=N, E=E, out_degree=out_degree, edges_idx=idx, in_degree=ind, src=src, dst=dst)
+
+
+# --------------------------------------------------------------------------- CSR build
+
+def count_sort(src, dst, w, N, mode, move_weights=False):
+    s = np.ascontiguousarray(src, np.uint32).copy()
+    d = np.ascontiguousarray(dst, np.uint32).copy()
+    ww = np.ascontiguousarray(w, np.float32).copy() if w is not None else np.ones(s.size, np.float32)
+    rc = lib().orc_count_sort(s, d, ww.ctypes.data, s.size, N, mode, int(move_weights))
+    assert rc == 0
+    return s, d, ww
+
+
+def map_vertices(sorted_src, N):
+    od = np.zeros(N, np.uint32)
+    idx = np.zeros(N, np.uint32)
+    lib().orc_map_vertices(np.ascontiguousarray(sorted_src, np.uint32), sorted_src.size, N, od, idx)
+    return od, idx
+
+
+@dataclass
+class OracleCSR:
+    N: int
+    E: int
+    out_degree: np.ndarray
+    in_degree: np.ndarray
+    edges_idx: np.ndarray
+    src: np.ndarray
+    dst: np.ndarray
+    w: np.ndarray
+
+
+def build_graph(src, dst, w, N, mode=1, dedupe=False, move_weights=False):
+    """Forward CSR then transpose, in graphCSRPreProcessingStep's order
+    (01_host/src/structures/graphCSR.c:243-327 with DIRECTED=1)."""
+    if dedupe:
+        mode = 1  # GraphBlox.c:358-359
+    s, d, ww = count_sort(src, dst, w, N, mode, move_weights)
+    if dedupe:
+        E = lib().orc_remove_duplicates(s, d, ww, s.size)
+        s, d, ww = s[:E].copy(), d[:E].copy(), ww[:E].copy()
+    od, idx = map_vertices(s, N)
+    # transpose: swap (edgeList.c:799-801) and run the same sort
+    ts, td, tw = count_sort(d, s, ww, N, mode, move_weights)
+    iod, iidx = map_vertices(ts, N)
+    fwd = OracleCSR(N, s.size, od, iod.copy(), idx, s, d, ww)
+    inv = OracleCSR(N, s.size, iod, od.copy(), iidx, ts, td, tw)
+    return fwd, inv
+
+
+def row_ptr_from(csr: OracleCSR) -> np.ndarray:
+    rp = np.zeros(csr.N + 1, np.uint64)
+    np.cumsum(csr.out_degree, out=rp[1:])
+    return rp
+
+
+# --------------------------------------------------------------------------- algorithms
+
+def pr_pull_f32(fwd: OracleCSR, inv: OracleCSR, iterations=20, epsilon=0.0, nthreads=1,
+                timings=False):
+    """Reference-semantics FP32 pull PageRank (pageRank.c:245-368)."""
+    N = fwd.N
+    ranks = np.zeros(N, np.float32)
+    it = C.c_uint32()
+    err = C.c_double()
+    ts = np.zeros(max(iterations, 1), np.float64)
+    lib().orc_pr_pull_f32(N, fwd.out_degree, inv.out_degree, inv.edges_idx, inv.dst, iterations, epsilon,
+                          nthreads, ranks, C.byref(it), C.byref(err), ts.ctypes.data)
+    out = dict(ranks=ranks, iterations=it.value, error_total=err.value)
+    if timings:
+        out["iter_seconds"] = ts[: max(it.value, 1)]
+    return out
+
+
+def spmv_pull_f32(fwd: OracleCSR, inv: OracleCSR, iterations=1, weights=None, nthreads=1):
+    N = fwd.N
+    x = np.zeros(N, np.float32)
+    y = np.zeros(N, np.float32)
+    wp = np.ascontiguousarray(weights, np.float32) if weights is not None else None
+    lib().orc_spmv_pull_f32(N, fwd.out_degree, inv.out_degree, inv.edges_idx, inv.dst,
+                            wp.ctypes.data if wp is not None else None, iterations, nthreads, x, y, None)
+    return x, y
+
+
+def pr_f64(N, outdeg, row_ptr, col, mode=0, damp=0.85, iterations=20, epsilon=0.0,
+           weights=None, teleport=None):
+    """FP64 PageRank oracle (mode 0: reference semantics, 1: stochastic)."""
+    r = np.zeros(N, np.float64)
+    it = C.c_uint32()
+    err = C.c_double()
+    wp = np.ascontiguousarray(weights, np.float64) if weights is not None else None
+    tp = np.ascontiguousarray(teleport, np.float64) if teleport is not None else None
+    lib().orc_pr_f64(N, np.ascontiguousarray(outdeg, np.uint32), np.ascontiguousarray(row_ptr, np.uint64),
+                     np.ascontiguousarray(col, np.uint32), wp.ctypes.data if wp is not None else None,
+                     tp.ctypes.data if tp is not None else None, damp, mode, iterations, epsilon, r,
+                     C.byref(it), C.byref(err))
+    return r, it.value, err.value
+
+
+def adjoint_f64(N, row_ptr, col, wraw, g, teleport=None, damp=0.85, iterations=1000, tol=1e-14):
+    p = np.zeros(N)
+    lam = np.zeros(N)
+    gw = np.zeros(int(row_ptr[-1]))
+    gt = np.zeros(N)
+    tp = np.ascontiguousarray(teleport, np.float64) if teleport is not None else None
+    lib().orc_pr_adjoint_f64(N, np.ascontiguousarray(row_ptr, np.uint64), np.ascontiguousarray(col, np.uint32),
+                             np.ascontiguousarray(wraw, np.float64), tp.ctypes.data if tp is not None else None,
+                             np.ascontiguousarray(g, np.float64), damp, iterations, tol, p, lam, gw, gt)
+    return p, lam, gw, gt
+
+
+def rmat(scale, edge_factor, seed, a=0.57, b=0.19, c=0.19):
+    M = edge_factor << scale
+    src = np.zeros(M, np.uint32)
+    dst = np.zeros(M, np.uint32)
+    lib().orc_rmat_generate(scale, edge_factor, a, b, c, seed, src, dst)
+    return src, dst
+
+
+def rmat_permute(x: int, scale: int, seed: int) -> int:
+    return int(lib().orc_rmat_permute(x, scale, seed))
+
+
+def dedupe_keep_first(src, dst, w=None):
+    s = np.ascontiguousarray(src, np.uint32).copy()
+    d = np.ascontiguousarray(dst, np.uint32).copy()
+    ww = np.ascontiguousarray(w, np.float32).copy() if w is not None else None
+    n = lib().orc_dedupe_keep_first(s, d, ww.ctypes.data if ww is not None else None, s.size)
+    return s[:n].copy(), d[:n].copy(), (ww[:n].copy() if ww is not None else None)
+
+
+def rmat_graph(scale, edge_factor, seed, weights=None, wseed=None):
+    """Generated, permuted, deduplicated R-MAT edge list and its N (max id+1)."""
+    s, d = rmat(scale, edge_factor, seed)
+    s, d, _ = dedupe_keep_first(s, d)
+    N = int(max(s.max(), d.max())) + 1
+    return s, d, N
+
+
+# --------------------------------------------------------------------------- closed forms
+
+def dense_ref_fixed_point(N, outdeg, row_ptr, col, damp=0.85):
+    """REF-mode fixed point: r = (1-d) (I - d A^T D^+)^-1 1, returned /N."""
+    M = np.zeros((N, N))
+    for v in range(N):
+        for j in range(int(row_ptr[v]), int(row_ptr[v + 1])):
+            u = int(col[j])
+            M[v, u] += 1.0 / outdeg[u]
+    r = np.linalg.solve(np.eye(N) - damp * M, (1 - damp) * np.ones(N))
+    return r / N
+
+
+def dense_stochastic(N, outdeg, row_ptr, col, damp=0.85, teleport=None, weights=None):
+    """Stochastic PageRank with uniform dangling redistribution, dense solve."""
+    t = np.full(N, 1.0 / N) if teleport is None else np.asarray(teleport, np.float64)
+    M = np.zeros((N, N))
+    for v in range(N):
+        for j in range(int(row_ptr[v]), int(row_ptr[v + 1])):
+            u = int(col[j])
+            M[v, u] += (weights[j] if weights is not None else 1.0 / outdeg[u])
+    dang = np.array([1.0 if outdeg[u] == 0 else 0.0 for u in range(N)])
+    A = damp * (M + np.outer(np.ones(N), dang) / N)
+    return np.linalg.solve(np.eye(N) - A, (1 - damp) * t)

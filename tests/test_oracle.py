@@ -1,0 +1,280 @@
+"""CPU tests: pin the oracle against the reference's golden vectors and against
+closed forms, and check the native libraries load and export their ABI."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import FIXTURES, RMAT_SMALL, ROOT, l1
+from oracle import oracle as O
+
+GRAPHS = [f"fixture_{f}" for f in FIXTURES] + [f"rmat_s{s}" for s in RMAT_SMALL]
+
+
+# ---------------------------------------------------------------- CSR build
+
+@pytest.mark.parametrize("name", GRAPHS)
+@pytest.mark.parametrize("tag,mode,dedupe", [("m1", 1, False), ("m2", 2, False), ("k", 1, True)])
+def test_oracle_csr_matches_reference(golden, name, tag, mode, dedupe):
+    z = golden(name)
+    src, dst = z["edges_src"], z["edges_dst"]
+    N = int(z[f"{tag}_N"])
+    fwd, inv = O.build_graph(src, dst, None, N, mode=mode, dedupe=dedupe)
+    assert fwd.E == int(z[f"{tag}_E"])
+    for side, c in (("fwd", fwd), ("inv", inv)):
+        for k in ("out_degree", "in_degree", "edges_idx", "src", "dst"):
+            np.testing.assert_array_equal(getattr(c, k), z[f"{tag}_{side}_{k}"], err_msg=f"{side}.{k}")
+
+
+@pytest.mark.parametrize("fx", FIXTURES)
+def test_oracle_csr_matches_shipped_text_dumps(golden, fx):
+    """The reference's own 03_data/TEST CSR dumps (sort src+dst, no dedupe)."""
+    t = golden(f"textdump_{fx}")
+    z = golden(f"fixture_{fx}")
+    fwd, _ = O.build_graph(z["edges_src"], z["edges_dst"], None, int(t["N"]), mode=3)
+    np.testing.assert_array_equal(fwd.out_degree, t["out_degree"])
+    np.testing.assert_array_equal(fwd.edges_idx, t["edges_idx"])
+    np.testing.assert_array_equal(fwd.src, t["src"])
+    np.testing.assert_array_equal(fwd.dst, t["dst"])
+    assert int(t["E"]) == fwd.E
+
+
+def test_edges_idx_zero_for_empty_rows(golden):
+    z = golden("fixture_v500_e500")
+    od, idx = z["m1_fwd_out_degree"], z["m1_fwd_edges_idx"]
+    assert (od == 0).sum() > 100
+    assert np.all(idx[od == 0] == 0)
+
+
+def test_count_sort_is_stable_brute_force():
+    rng = np.random.default_rng(1)
+    src = rng.integers(0, 17, 500).astype(np.uint32)
+    dst = rng.integers(0, 17, 500).astype(np.uint32)
+    w = np.arange(500, dtype=np.float32)
+    s, d, ww = O.count_sort(src, dst, w, 17, mode=2, move_weights=True)
+    order = sorted(range(500), key=lambda i: int(src[i]))  # Python's sort is stable
+    np.testing.assert_array_equal(ww, w[order])
+    s3, d3, w3 = O.count_sort(src, dst, w, 17, mode=3, move_weights=True)
+    order3 = sorted(range(500), key=lambda i: (int(src[i]), int(dst[i])))
+    np.testing.assert_array_equal(w3, w[order3])
+
+
+def test_dedupe_set_semantics():
+    rng = np.random.default_rng(2)
+    src = rng.integers(0, 20, 800).astype(np.uint32)
+    dst = rng.integers(0, 20, 800).astype(np.uint32)
+    fwd, _ = O.build_graph(src, dst, None, 20, dedupe=True)
+    pairs = sorted({(int(a), int(b)) for a, b in zip(src, dst) if a != b})
+    assert fwd.E == len(pairs)
+    np.testing.assert_array_equal(fwd.src, [p[0] for p in pairs])
+    np.testing.assert_array_equal(fwd.dst, [p[1] for p in pairs])
+
+
+# ---------------------------------------------------------------- PageRank / SpMV
+
+@pytest.mark.parametrize("name", GRAPHS)
+def test_oracle_pr_f32_bit_exact_with_reference(golden, name):
+    z = golden(name)
+    fwd, inv = O.build_graph(z["edges_src"], z["edges_dst"], None, int(z["m1_N"]), mode=1)
+    r1 = O.pr_pull_f32(fwd, inv, 1, 0.0, nthreads=2)
+    r20 = O.pr_pull_f32(fwd, inv, 20, 0.0, nthreads=2)
+    np.testing.assert_array_equal(r1["ranks"], z["pr_r1"])
+    np.testing.assert_array_equal(r20["ranks"], z["pr_r20"])
+    assert r20["error_total"] == pytest.approx(float(z["pr_err20"]), rel=1e-9)
+    re = O.pr_pull_f32(fwd, inv, 100, 1e-4, nthreads=2)
+    assert re["iterations"] == int(z["pr_eps_iters"])
+    np.testing.assert_array_equal(re["ranks"], z["pr_eps_ranks"])
+
+
+@pytest.mark.parametrize("name", GRAPHS)
+def test_oracle_spmv_bit_exact_with_reference(golden, name):
+    z = golden(name)
+    fwd, inv = O.build_graph(z["edges_src"], z["edges_dst"], None, int(z["m1_N"]), mode=1)
+    x, y = O.spmv_pull_f32(fwd, inv, 3, nthreads=2)
+    np.testing.assert_array_equal(x, z["spmv_x"])
+    np.testing.assert_array_equal(y, z["spmv_y3"])
+
+
+@pytest.mark.parametrize("s", RMAT_SMALL)
+def test_oracle_weighted_spmv_bit_exact_with_reference(golden, s):
+    z = golden(f"rmat_s{s}")
+    ws, wd, ww = z["wedges_src"], z["wedges_dst"], z["wedges_w"]
+    fwd, inv = O.build_graph(ws, wd, ww, int(z["m1_N"]), mode=1, move_weights=True)
+    np.testing.assert_array_equal(inv.w, z["wm1_inv_w"])
+    x, y = O.spmv_pull_f32(fwd, inv, 1, weights=inv.w, nthreads=2)
+    np.testing.assert_array_equal(y, z["wspmv_y1"])
+
+
+@pytest.mark.parametrize("name", GRAPHS)
+def test_fp64_ref_oracle_close_to_reference_f32(golden, name):
+    z = golden(name)
+    fwd, inv = O.build_graph(z["edges_src"], z["edges_dst"], None, int(z["m1_N"]), mode=1)
+    r64, it, _ = O.pr_f64(fwd.N, fwd.out_degree, O.row_ptr_from(inv), inv.dst, mode=0, iterations=20)
+    assert it == 20
+    assert l1(r64, z["pr_r20"]) < 1e-6
+
+
+def test_fp64_ref_matches_dense_fixed_point(golden):
+    z = golden("fixture_v500_e500")
+    fwd, inv = O.build_graph(z["edges_src"], z["edges_dst"], None, int(z["m1_N"]), mode=1)
+    rp = O.row_ptr_from(inv)
+    r64, _, _ = O.pr_f64(fwd.N, fwd.out_degree, rp, inv.dst, mode=0, iterations=400)
+    dense = O.dense_ref_fixed_point(fwd.N, fwd.out_degree, rp, inv.dst)
+    assert np.abs(r64 - dense).max() < 1e-14
+
+
+def test_fp64_stochastic_matches_dense_solve_and_sums_to_one(golden):
+    z = golden("rmat_s8")
+    fwd, inv = O.build_graph(z["edges_src"], z["edges_dst"], None, int(z["m1_N"]), mode=1)
+    rp = O.row_ptr_from(inv)
+    r, it, err = O.pr_f64(fwd.N, fwd.out_degree, rp, inv.dst, mode=1, iterations=500, epsilon=1e-15)
+    dense = O.dense_stochastic(fwd.N, fwd.out_degree, rp, inv.dst)
+    assert abs(r.sum() - 1.0) < 1e-12
+    assert np.abs(r - dense).max() < 1e-13
+    # personalised: linear in the teleport vector
+    rng = np.random.default_rng(3)
+    t1 = rng.random(fwd.N); t1 /= t1.sum()
+    t2 = rng.random(fwd.N); t2 /= t2.sum()
+    p1, _, _ = O.pr_f64(fwd.N, fwd.out_degree, rp, inv.dst, mode=1, iterations=500, epsilon=1e-15, teleport=t1)
+    p2, _, _ = O.pr_f64(fwd.N, fwd.out_degree, rp, inv.dst, mode=1, iterations=500, epsilon=1e-15, teleport=t2)
+    p12, _, _ = O.pr_f64(fwd.N, fwd.out_degree, rp, inv.dst, mode=1, iterations=500, epsilon=1e-15,
+                         teleport=0.3 * t1 + 0.7 * t2)
+    assert np.abs(p12 - (0.3 * p1 + 0.7 * p2)).max() < 1e-13
+
+
+def _cycle(n):
+    src = np.arange(n, dtype=np.uint32)
+    dst = ((np.arange(n) + 1) % n).astype(np.uint32)
+    return O.build_graph(src, dst, None, n, mode=1)
+
+
+@pytest.mark.parametrize("k", [1, 5, 20])
+def test_cycle_closed_forms(k):
+    n = 37
+    fwd, inv = _cycle(n)
+    rp = O.row_ptr_from(inv)
+    r, _, _ = O.pr_f64(n, fwd.out_degree, rp, inv.dst, mode=0, iterations=k)
+    # REF: r_k = 1 - 0.85^(k+1) at every vertex (r_0 = 0.15), output / N
+    np.testing.assert_allclose(r, (1 - 0.85 ** (k + 1)) / n, rtol=1e-13)
+    f32 = O.pr_pull_f32(fwd, inv, k, 0.0)
+    np.testing.assert_allclose(f32["ranks"], (1 - 0.85 ** (k + 1)) / n, rtol=1e-5)
+    s, _, _ = O.pr_f64(n, fwd.out_degree, rp, inv.dst, mode=1, iterations=k)
+    np.testing.assert_allclose(s, 1.0 / n, rtol=1e-13)
+
+
+def test_planted_mistakes_are_caught(golden):
+    """Mistakes planted in the oracle's inputs fail the closed forms above."""
+    n = 37
+    fwd, inv = _cycle(n)
+    rp = O.row_ptr_from(inv)
+    good = (1 - 0.85 ** 6) / n
+    # wrong damping
+    r, _, _ = O.pr_f64(n, fwd.out_degree, rp, inv.dst, mode=0, iterations=5, damp=0.8)
+    assert not np.allclose(r, good, rtol=1e-6)
+    # transposed operand on a non-symmetric graph: forward rows instead of in-rows
+    z = golden("fixture_v1024_e4096")
+    f, i = O.build_graph(z["edges_src"], z["edges_dst"], None, int(z["m1_N"]), mode=1)
+    wrong, _, _ = O.pr_f64(f.N, f.out_degree, O.row_ptr_from(f), f.dst, mode=0, iterations=20)
+    assert l1(wrong, z["pr_r20"]) > 1e-3
+    # dropped dangling term breaks the sum-to-one invariant
+    z8 = golden("rmat_s8")
+    f8, i8 = O.build_graph(z8["edges_src"], z8["edges_dst"], None, int(z8["m1_N"]), mode=1)
+    assert (f8.out_degree == 0).any()
+    r_ref, _, _ = O.pr_f64(f8.N, f8.out_degree, O.row_ptr_from(i8), i8.dst, mode=0, iterations=50)
+    assert abs(r_ref.sum() - 1.0) > 1e-3
+
+
+def test_adjoint_matches_finite_differences_and_dense():
+    rng = np.random.default_rng(7)
+    n = 60
+    src = rng.integers(0, n, 400).astype(np.uint32)
+    dst = rng.integers(0, n, 400).astype(np.uint32)
+    fwd, inv = O.build_graph(src, dst, None, n, dedupe=True)
+    rp = O.row_ptr_from(inv)
+    wraw = np.exp(rng.standard_normal(inv.E))
+    g = rng.standard_normal(n)
+    t = rng.random(n); t /= t.sum()
+    p, lam, gw, gt = O.adjoint_f64(n, rp, inv.dst, wraw, g, teleport=t)
+
+    def loss(wr, tt):
+        S = np.zeros(n)
+        np.add.at(S, inv.dst, wr)
+        P = wr / S[inv.dst]
+        pp = O.dense_stochastic(n, fwd.out_degree, rp, inv.dst, teleport=tt, weights=P)
+        return float(g @ pp)
+
+    assert loss(wraw, t) == pytest.approx(float(g @ p), rel=1e-10)
+    h = 1e-6
+    for e in rng.choice(inv.E, 12, replace=False):
+        wp, wm = wraw.copy(), wraw.copy()
+        wp[e] += h
+        wm[e] -= h
+        fd = (loss(wp, t) - loss(wm, t)) / (2 * h)
+        assert gw[e] == pytest.approx(fd, rel=1e-5, abs=1e-10)
+    for v in rng.choice(n, 6, replace=False):
+        tp, tm = t.copy(), t.copy()
+        tp[v] += h
+        tm[v] -= h
+        fd = (loss(wraw, tp) - loss(wraw, tm)) / (2 * h)
+        assert gt[v] == pytest.approx(fd, rel=1e-6, abs=1e-10)
+
+
+# ---------------------------------------------------------------- generator
+
+def test_rmat_permutation_is_bijection():
+    for scale in (1, 2, 5, 11, 14):
+        img = {O.rmat_permute(x, scale, 99) for x in range(1 << scale)}
+        assert img == set(range(1 << scale))
+
+
+def test_rmat_quadrant_frequencies():
+    s, d = O.rmat(1, 1 << 15, 5)  # scale 1: one level, ids permuted on {0,1}
+    inv = {O.rmat_permute(x, 1, 5): x for x in (0, 1)}
+    u = np.vectorize(inv.get)(s)
+    v = np.vectorize(inv.get)(d)
+    M = s.size
+    freq = np.array([((u == a) & (v == b)).sum() for a in (0, 1) for b in (0, 1)]) / M
+    np.testing.assert_allclose(freq, [0.57, 0.19, 0.19, 0.05], atol=0.01)
+
+
+def test_rmat_dedupe_keeps_first_in_order():
+    s, d = O.rmat(6, 16, 11)
+    ks, kd, _ = O.dedupe_keep_first(s, d)
+    seen, exp = set(), []
+    for a, b in zip(s.tolist(), d.tolist()):
+        if a != b and (a, b) not in seen:
+            seen.add((a, b))
+            exp.append((a, b))
+    assert list(zip(ks.tolist(), kd.tolist())) == exp
+
+
+def test_rmat_s16_regenerates_golden(golden):
+    import hashlib
+    z = golden("rmat_s16")
+    s, d, N = O.rmat_graph(16, 16, 20261203 + 16)
+    assert N == int(z["m1_N"]) and s.size == int(z["m1_E"])
+    assert hashlib.sha256(np.ascontiguousarray(np.stack([s, d])).tobytes()).hexdigest() == str(z["edges_sha"])
+
+
+# ---------------------------------------------------------------- native libraries
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gbx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_hip_library_exports_every_declared_symbol():
+    import ctypes
+    import graphblox_amd as gbx
+
+    if not os.path.exists(gbx.LIB_HIP):
+        pytest.fail("libgbx_hip.so not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(gbx.LIB_HIP)
+    names = _declared("gbx.h")
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(gbx.exported_symbols())
+    assert b"gfx950" in gbx.lib().gbx_version()
