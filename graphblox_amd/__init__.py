@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -130,6 +131,12 @@ class Context:
         if rc != GBX_OK:
             raise GbxError(rc, self.L.gbx_last_error(None).decode())
         self.h = h
+        # graphs and PageRank objects must be released before the context
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
+        return obj
 
     def check(self, rc: int):
         if rc != GBX_OK:
@@ -137,6 +144,11 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
+            for child in list(self._children):
+                if isinstance(child, PageRank):
+                    child.free()
+            for child in list(self._children):
+                child.free()
             self.L.gbx_free(self.h)
             self.h = None
 
@@ -193,6 +205,7 @@ class Graph:
     def __init__(self, ctx: Context, h):
         self.ctx = ctx
         self.h = h
+        ctx._adopt(self)
         n, e = C.c_uint32(), C.c_uint32()
         ctx.check(ctx.L.gbx_graph_info(h, C.byref(n), C.byref(e)))
         self.N, self.E = n.value, e.value
@@ -208,6 +221,9 @@ class Graph:
 
     def free(self):
         if self.h:
+            for child in list(self.ctx._children):
+                if isinstance(child, PageRank) and child.g is self:
+                    child.free()
             self.ctx.L.gbx_graph_free(self.h)
             self.h = None
 
@@ -263,6 +279,7 @@ class PageRank:
         h = _vp()
         self.ctx.check(self.ctx.L.gbx_pr_create(self.ctx.h, graph.h, C.byref(p), C.byref(h)))
         self.h = h
+        self.ctx._adopt(self)
 
     def reset(self, teleport=None):
         tp = _f32(teleport)

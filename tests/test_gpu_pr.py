@@ -66,7 +66,9 @@ def test_spmv_matches_reference(gpu_ctx, golden, name):
     g = _graph(gpu_ctx, z)
     x, y, _ = g.spmv(iterations=3)
     np.testing.assert_array_equal(x, z["spmv_x"])
-    np.testing.assert_allclose(y, z["spmv_y3"], rtol=2e-6, atol=1e-9)
+    # rows summed by one lane are bit-exact; wider rows are reassociated
+    np.testing.assert_allclose(y, z["spmv_y3"], rtol=2e-5, atol=1e-9)
+    assert l1(y, z["spmv_y3"]) <= 1e-6 * float(np.abs(z["spmv_y3"]).sum())
 
 
 @pytest.mark.parametrize("s", RMAT_SMALL)
@@ -74,7 +76,8 @@ def test_weighted_spmv_matches_reference(gpu_ctx, golden, s):
     z = golden(f"rmat_s{s}")
     g = gpu_ctx.graph(z["wedges_src"], z["wedges_dst"], z["wedges_w"], sort_mode=1, flags=gbx.GRAPH_WEIGHTED)
     x, y, _ = g.spmv(iterations=1, weighted=True)
-    np.testing.assert_allclose(y, z["wspmv_y1"], rtol=2e-6, atol=1e-9)
+    np.testing.assert_allclose(y, z["wspmv_y1"], rtol=2e-5, atol=1e-9)
+    assert l1(y, z["wspmv_y1"]) <= 1e-6 * float(np.abs(z["wspmv_y1"]).sum())
 
 
 @pytest.mark.parametrize("name", ["rmat_s10", "rmat_s12", "fixture_v8192_e32768"])
