@@ -72,6 +72,20 @@ def cpu_baseline(g, iterations: int, threads: int):
                        f"R-MAT graph (N={N}, E={E}), {threads} OpenMP threads, whole loop timed")
 
 
+def calibrate_copy(gib: float):
+    """A plain streaming copy of known size: y = x * 1 over gib GiB of float32
+    (reads gib GiB, writes gib GiB), three launches."""
+    import torch
+
+    n = int(gib * (1 << 30)) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes_read": n * 4, "calibrate_copy_bytes_written": n * 4}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,7 +96,12 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=5)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0,
+                    help="run only a plain device copy of this many GiB (reference for PMC byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy > 0:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
                                    "(see DESIGN.md), so --gpus must be 1"}))

@@ -76,6 +76,7 @@ def lib():
                                         C.POINTER(_vp)]),
         "gbx_rmat_edges": (i32, [_vp, u32, u32, f64, f64, f64, u64, i32, i32, u64, _vp, _vp, _vp,
                                  C.POINTER(u64)]),
+        "gbx_graph_from_csr": (i32, [_vp, u32, u32] + [_vp] * 10 + [C.POINTER(_vp)]),
         "gbx_graph_info": (i32, [_vp, C.POINTER(u32), C.POINTER(u32)]),
         "gbx_graph_export": (i32, [_vp, i32, _vp, _vp, _vp, _vp, _vp, _vp]),
         "gbx_graph_free": (i32, [_vp]),
@@ -103,7 +104,7 @@ def exported_symbols():
     return [
         "gbx_init", "gbx_free", "gbx_last_error", "gbx_status_string", "gbx_sync", "gbx_stream",
         "gbx_version", "gbx_csr_build", "gbx_graph_create", "gbx_graph_create_rmat",
-        "gbx_rmat_edges", "gbx_graph_info", "gbx_graph_export", "gbx_graph_free",
+        "gbx_rmat_edges", "gbx_graph_from_csr", "gbx_graph_info", "gbx_graph_export", "gbx_graph_free",
         "gbx_pr_params_default", "gbx_pr_pull", "gbx_pr_create", "gbx_pr_reset", "gbx_pr_launch",
         "gbx_pr_launch_timed", "gbx_pr_read", "gbx_pr_free", "gbx_pr_plan_info", "gbx_spmv_pull",
     ]
@@ -177,6 +178,17 @@ class Context:
         h = _vp()
         self.check(self.L.gbx_graph_create(self.h, _ptr(src), _ptr(dst), _ptr(w), src.size, num_vertices,
                                            sort_mode, flags, C.byref(h)))
+        return Graph(self, h)
+
+    def graph_from_csr(self, fwd: dict, inv: dict) -> "Graph":
+        """Upload CSR arrays in the reference layout (keys out_degree,
+        edges_idx, src, dst[, w]) without re-sorting."""
+        N, E = fwd["out_degree"].size, fwd["src"].size
+        arrs = [_u32(fwd["out_degree"]), _u32(fwd["edges_idx"]), _u32(fwd["src"]), _u32(fwd["dst"]),
+                _f32(fwd.get("w")), _u32(inv["out_degree"]), _u32(inv["edges_idx"]), _u32(inv["src"]),
+                _u32(inv["dst"]), _f32(inv.get("w"))]
+        h = _vp()
+        self.check(self.L.gbx_graph_from_csr(self.h, N, E, *(_ptr(a) for a in arrs), C.byref(h)))
         return Graph(self, h)
 
     def rmat_graph(self, scale, edge_factor, seed, a=0.57, b=0.19, c=0.19, weight_kind=0, weight_seed=0,

@@ -25,7 +25,8 @@ HIP_SOURCES = ["gbx_core.hip", "gbx_sort.hip", "gbx_rmat.hip", "gbx_pr.hip"]
 HOST_SOURCES = ["graphblox_hip.c"]
 
 LIB_HIP = os.path.join(PKG, "libgbx_hip.so")
-LIB_HOST = os.path.join(PKG, "libgraphblox_hip.so")
+LIB_HOST = os.path.join(PKG, "libgraphblox_hip.so")        # WEIGHTED=0 layout
+LIB_HOST_W = os.path.join(PKG, "libgraphblox_hip_w.so")    # WEIGHTED=1 layout
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -73,14 +74,15 @@ def build_hip(verbose: bool = False) -> str:
 
 def build_host(verbose: bool = False) -> str:
     cc = shutil.which("gcc") or "cc"
-    tmp = LIB_HOST + ".tmp"
-    cmd = [cc, "-O2", "-fPIC", "-shared", "-std=gnu11", "-Wall", "-I", INC,
-           *[os.path.join(HOST, s) for s in HOST_SOURCES],
-           "-o", tmp, "-L", PKG, "-lgbx_hip", "-Wl,-rpath,$ORIGIN", "-lm"]
-    out = _run(cmd)
-    if verbose and out.strip():
-        print(out)
-    os.replace(tmp, LIB_HOST)
+    for weighted, lib in ((0, LIB_HOST), (1, LIB_HOST_W)):
+        tmp = lib + ".tmp"
+        cmd = [cc, "-O2", "-fPIC", "-shared", "-std=gnu11", "-Wall", "-Wextra", f"-DGBX_WEIGHTED={weighted}",
+               "-I", INC, *[os.path.join(HOST, s) for s in HOST_SOURCES],
+               "-o", tmp, "-L", PKG, "-lgbx_hip", "-Wl,-rpath,$ORIGIN", "-lm"]
+        out = _run(cmd)
+        if verbose and out.strip():
+            print(out)
+        os.replace(tmp, lib)
     return LIB_HOST
 
 

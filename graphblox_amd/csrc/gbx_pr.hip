@@ -30,7 +30,13 @@
 
 namespace gbx {
 
-constexpr int kBlock = 256;
+#ifndef GBX_BLOCK
+#define GBX_BLOCK 256
+#endif
+constexpr int kBlock = GBX_BLOCK;
+#ifndef GBX_LDS_DEFAULT
+#define GBX_LDS_DEFAULT 0
+#endif
 constexpr int kMaxClasses = 9;
 
 enum Kind : int {
@@ -69,12 +75,12 @@ struct Partial {
 struct gbx_plan {
     int direction = 0;
     uint32_t N = 0, E = 0;
-    gbx::DevBuf perm, inv, rp, col, wt, wn, outdeg, chunks, hub_cnt, hub_part;
+    gbx::DevBuf perm, inv, rp, col, wt, wn, wn16, outdeg, chunks, hub_cnt, hub_part;
     uint32_t nclass = 0;
     gbx::ClassDesc cls[gbx::kMaxClasses];
     uint64_t cls_edges[gbx::kMaxClasses];
     uint32_t n_chunks = 0, n_hubs = 0, n_tiles = 0;
-    bool has_wt = false, has_wn = false;
+    bool has_wt = false, has_wn = false, has_wn16 = false;
 };
 
 struct gbx_pr {
@@ -83,6 +89,8 @@ struct gbx_pr {
     gbx_plan *plan = nullptr;
     gbx_pr_params p;
     int kind = 0;
+    size_t elem = 4;      // bytes per contribution (4: f32, 2: bf16)
+    uint32_t hot = 0;     // contributions held in LDS per block
     gbx::DevBuf r, c[2], tele, slots, partials, counter, out;
     bool has_tele = false;
     uint32_t t = 0, grid = 0;
@@ -108,53 +116,187 @@ struct PlanView {
 
 struct IterView {
     float *__restrict__ r;
-    const float *__restrict__ c_in;
-    float *__restrict__ c_out;
+    const void *__restrict__ c_in;
+    void *__restrict__ c_out;
     const float *__restrict__ tele;
     Slot *slots;
     Partial *partials;
     uint32_t *counter;
     uint32_t t;
+    uint32_t hot;  // vertices of the contribution vector held in LDS
     float damp;
     double eps;
     int stop;
 };
 
+// Streaming (read-once) data is loaded and stored non-temporally so that the
+// L2 keeps the gathered contribution vector.
+#ifndef GBX_NT
+#define GBX_NT 0
+#endif
+#ifndef GBX_UNROLL
+#define GBX_UNROLL 8
+#endif
+
+template <class T>
+__device__ __forceinline__ T ld_stream(const T *p)
+{
+#if GBX_NT
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+
+template <class T>
+__device__ __forceinline__ void st_stream(T *p, T v)
+{
+#if GBX_NT
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
+// ------------------------------------------------------------------ accumulators
+
+// Plain FP32 running sum: `acc += x` in visit order, as pageRank.c:313-314.
+struct AccF32 {
+    float s;
+    __device__ __forceinline__ void init(float v) { s = v; }
+    __device__ __forceinline__ void add(float x) { s += x; }
+    __device__ __forceinline__ void merge(const AccF32 &o) { s += o.s; }
+    __device__ __forceinline__ void shfl_merge(int off) { s += __shfl_xor(s, off, 64); }
+    __device__ __forceinline__ float value() const { return s; }
+};
+
+// Compensated FP32 sum (Kahan 1965); partial sums are merged with Knuth's
+// TwoSum so the rounding error of every merge is carried as well.
+// Represented value: s - c.
+struct AccKahan {
+    float s, c;
+    __device__ __forceinline__ void init(float v) { s = v; c = 0.0f; }
+    __device__ __forceinline__ void add(float x)
+    {
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ void merge(const AccKahan &o)
+    {
+        const float t = s + o.s;
+        const float bp = t - s;
+        const float e = (s - (t - bp)) + (o.s - bp);  // s + o.s == t + e exactly
+        c = c + o.c - e;
+        s = t;
+    }
+    __device__ __forceinline__ void shfl_merge(int off)
+    {
+        AccKahan o;
+        o.s = __shfl_xor(s, off, 64);
+        o.c = __shfl_xor(c, off, 64);
+        merge(o);
+    }
+    __device__ __forceinline__ float value() const { return s - c; }
+};
+
+// ------------------------------------------------------------------ element types
+
+__device__ __forceinline__ float load_val(const float *p, uint32_t i) { return p[i]; }
+__device__ __forceinline__ float load_val(const uint16_t *p, uint32_t i)
+{
+    return __uint_as_float((uint32_t)p[i] << 16);  // bf16 -> f32 is exact
+}
+__device__ __forceinline__ uint16_t f32_to_bf16(float v)
+{
+    // round to nearest even; values here are finite
+    uint32_t u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ void store_val(float *p, uint32_t i, float v) { st_stream(p + i, v); }
+__device__ __forceinline__ void store_val(uint16_t *p, uint32_t i, float v) { p[i] = f32_to_bf16(v); }
+
+// Hot prefix of the contribution vector held in LDS for the whole kernel.
+extern __shared__ __align__(16) unsigned char gbx_dyn_lds[];
+
 // ------------------------------------------------------------------ gathers
 
-struct GatherPlain {
-    const float *__restrict__ v;
-    __device__ __forceinline__ float operator()(uint32_t, uint32_t c) const { return v[c]; }
+// Contribution gather; with LDS the first `hot` (most gathered) vertices
+// come from the block's LDS copy.
+template <class T, bool LDS>
+struct GatherC {
+    const T *__restrict__ v;
+    uint32_t hot;
+    __device__ __forceinline__ float operator()(uint32_t, uint32_t c) const
+    {
+        if constexpr (LDS) {
+            if (c < hot) return load_val(reinterpret_cast<const T *>(gbx_dyn_lds), c);
+        }
+        return load_val(v, c);
+    }
 };
-struct GatherW {
-    const float *__restrict__ v;
-    const float *__restrict__ w;
-    __device__ __forceinline__ float operator()(uint32_t j, uint32_t c) const { return w[j] * v[c]; }
+
+// Weighted contribution gather: w[j] * v[c], weights of type T.
+template <class T, bool LDS>
+struct GatherCW {
+    const T *__restrict__ v;
+    const T *__restrict__ w;
+    uint32_t hot;
+    __device__ __forceinline__ float operator()(uint32_t j, uint32_t c) const
+    {
+        float x;
+        if constexpr (LDS) {
+            if (c < hot) x = load_val(reinterpret_cast<const T *>(gbx_dyn_lds), c);
+            else x = load_val(v, c);
+        } else {
+            x = load_val(v, c);
+        }
+        float wj;
+        if constexpr (sizeof(T) == 4) wj = ld_stream(reinterpret_cast<const float *>(w) + j);
+        else wj = load_val(w, j);
+        return wj * x;
+    }
 };
+
 struct GatherConstW {
     const float *__restrict__ v;
     float w;
     __device__ __forceinline__ float operator()(uint32_t, uint32_t c) const { return w * v[c]; }
 };
 
-// Lane-strided, in-order partial sum of one row with 4 loads in flight.
-template <int G, class Gather>
-__device__ __forceinline__ float lane_sum(uint32_t j, uint32_t end, const uint32_t *__restrict__ col,
-                                          const Gather &g, float acc)
+// Lane-strided, in-order partial sum of one row: GBX_UNROLL column loads,
+// then GBX_UNROLL gathers in flight; the tail is one predicated batch.
+template <int G, class Acc, class Gather>
+__device__ __forceinline__ void lane_sum(uint32_t j, uint32_t end, const uint32_t *__restrict__ col,
+                                         const Gather &g, Acc &acc)
 {
-    for (; j + 3u * G < end; j += 4u * G) {
-        uint32_t c0 = col[j], c1 = col[j + G], c2 = col[j + 2 * G], c3 = col[j + 3 * G];
-        float t0 = g(j, c0), t1 = g(j + G, c1), t2 = g(j + 2 * G, c2), t3 = g(j + 3 * G, c3);
-        acc += t0;
-        acc += t1;
-        acc += t2;
-        acc += t3;
+    constexpr uint32_t U = GBX_UNROLL;
+    for (; j + (U - 1) * G < end; j += U * G) {
+        uint32_t c[U];
+        float t[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) c[k] = ld_stream(col + j + k * G);
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) t[k] = g(j + k * G, c[k]);
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) acc.add(t[k]);
     }
-    for (; j < end; j += G) acc += g(j, col[j]);
-    return acc;
+    if (j < end) {
+        uint32_t c[U];
+        float t[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) c[k] = (j + k * G < end) ? ld_stream(col + j + k * G) : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) t[k] = (j + k * G < end) ? g(j + k * G, c[k]) : 0.0f;
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++)
+            if (j + k * G < end) acc.add(t[k]);
+    }
 }
 
-template <int G, class Gather, class Epi>
+template <int G, class Acc, class Gather, class Epi>
 __device__ __forceinline__ void do_rows(const ClassDesc &cd, uint32_t local, const PlanView &P,
                                         const Gather &g, Epi &epi)
 {
@@ -162,53 +304,57 @@ __device__ __forceinline__ void do_rows(const ClassDesc &cd, uint32_t local, con
     const uint32_t grp = threadIdx.x / G, lig = threadIdx.x % G;
     const uint32_t row = cd.row_begin + local * RPT + grp;
     const bool valid = row < cd.row_end;
-    float acc = 0.0f;
+    Acc acc;
+    acc.init((G == 1 && valid) ? epi.init(row) : 0.0f);
     if (valid) {
-        const uint32_t beg = P.rp[row], end = P.rp[row + 1];
-        acc = lane_sum<G>(beg + lig, end, P.col, g, G == 1 ? epi.init(row) : 0.0f);
+        const uint32_t beg = ld_stream(P.rp + row), end = ld_stream(P.rp + row + 1);
+        lane_sum<G>(beg + lig, end, P.col, g, acc);
     }
     if constexpr (G > 1) {
 #pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        for (int off = G / 2; off > 0; off >>= 1) acc.shfl_merge(off);
     }
-    if (valid && lig == 0) epi(row, G == 1 ? acc : epi.init(row) + acc);
+    if (valid && lig == 0) epi(row, G == 1 ? acc.value() : epi.init(row) + acc.value());
 }
 
-template <class Gather, class Epi>
+template <class Acc, class Gather, class Epi>
 __device__ __forceinline__ void do_hub(uint32_t tile, const PlanView &P, const Gather &g, Epi &epi)
 {
-    __shared__ float red[kBlock / 64];
+    __shared__ Acc red[kBlock / 64];
     const HubChunk hc = P.chunks[tile];
-    float acc = lane_sum<kBlock>(hc.ebeg + threadIdx.x, hc.eend, P.col, g, 0.0f);
+    Acc acc;
+    acc.init(0.0f);
+    lane_sum<kBlock>(hc.ebeg + threadIdx.x, hc.eend, P.col, g, acc);
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    for (int off = 32; off > 0; off >>= 1) acc.shfl_merge(off);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        float s = red[0];
+        Acc s = red[0];
 #pragma unroll
-        for (int w = 1; w < kBlock / 64; w++) s += red[w];
-        P.hub_part[tile] = s;
+        for (int w = 1; w < kBlock / 64; w++) s.merge(red[w]);
+        P.hub_part[tile] = s.value();
         __threadfence();
         uint32_t prev = atomicAdd(&P.hub_cnt[hc.hub], 1u);
         if (prev == hc.nch - 1) {
             __threadfence();
-            float tot = 0.0f;
+            Acc tot;
+            tot.init(0.0f);
             const volatile float *hp = P.hub_part;
-            for (uint32_t k = 0; k < hc.nch; k++) tot += hp[hc.first + k];
+            for (uint32_t k = 0; k < hc.nch; k++) tot.add(hp[hc.first + k]);
             P.hub_cnt[hc.hub] = 0;
-            epi(hc.row, epi.init(hc.row) + tot);
+            epi(hc.row, epi.init(hc.row) + tot.value());
         }
     }
     __syncthreads();
 }
 
-template <class Gather, class Epi>
+template <class Acc, class Gather, class Epi>
 __device__ __forceinline__ void tile_loop(const PlanView &P, const Gather &g, Epi &epi)
 {
     for (uint32_t tile = blockIdx.x; tile < P.n_tiles; tile += gridDim.x) {
         if (tile < P.n_chunks) {
-            do_hub(tile, P, g, epi);
+            do_hub<Acc>(tile, P, g, epi);
             continue;
         }
         uint32_t c = 1;
@@ -216,15 +362,27 @@ __device__ __forceinline__ void tile_loop(const PlanView &P, const Gather &g, Ep
         const ClassDesc &cd = P.cls[c];
         const uint32_t local = tile - cd.tile_begin;
         switch (cd.lanes) {
-        case 64: do_rows<64>(cd, local, P, g, epi); break;
-        case 32: do_rows<32>(cd, local, P, g, epi); break;
-        case 16: do_rows<16>(cd, local, P, g, epi); break;
-        case 8: do_rows<8>(cd, local, P, g, epi); break;
-        case 4: do_rows<4>(cd, local, P, g, epi); break;
-        case 2: do_rows<2>(cd, local, P, g, epi); break;
-        default: do_rows<1>(cd, local, P, g, epi); break;
+        case 64: do_rows<64, Acc>(cd, local, P, g, epi); break;
+        case 32: do_rows<32, Acc>(cd, local, P, g, epi); break;
+        case 16: do_rows<16, Acc>(cd, local, P, g, epi); break;
+        case 8: do_rows<8, Acc>(cd, local, P, g, epi); break;
+        case 4: do_rows<4, Acc>(cd, local, P, g, epi); break;
+        case 2: do_rows<2, Acc>(cd, local, P, g, epi); break;
+        default: do_rows<1, Acc>(cd, local, P, g, epi); break;
         }
     }
+}
+
+// Copy the hot prefix of the contribution vector into LDS.
+template <class T>
+__device__ __forceinline__ void load_hot(const T *__restrict__ c_in, uint32_t hot)
+{
+    T *h = reinterpret_cast<T *>(gbx_dyn_lds);
+    const uint32_t words = (uint32_t)((hot * sizeof(T) + 15) / 16);
+    const uint4 *src = reinterpret_cast<const uint4 *>(c_in);
+    uint4 *dst = reinterpret_cast<uint4 *>(h);
+    for (uint32_t i = threadIdx.x; i < words; i += kBlock) dst[i] = src[i];
+    __syncthreads();
 }
 
 // ------------------------------------------------------------------ reductions
@@ -294,10 +452,10 @@ __device__ __forceinline__ void grid_reduce(Partial p, Partial *partials, uint32
 
 // ------------------------------------------------------------------ PageRank
 
-template <int KIND>
+template <int KIND, class T>
 struct PrEpi {
     float *__restrict__ r;
-    float *__restrict__ c_out;
+    T *__restrict__ c_out;
     const uint32_t *__restrict__ outdeg;
     const float *__restrict__ tele;
     float base, damp, fa, fs, fd, uni;
@@ -306,20 +464,20 @@ struct PrEpi {
     __device__ __forceinline__ float init(uint32_t) const { return 0.0f; }
     __device__ __forceinline__ void operator()(uint32_t i, float acc)
     {
-        const float r_old = r[i];
+        const float r_old = ld_stream(r + i);
         float r_new;
         if constexpr (KIND == K_PR_REF) {
             r_new = base + damp * acc;  // pageRank.c:324 (no contraction: -ffp-contract=off)
         } else {
-            const float ti = tele ? tele[i] : uni;
+            const float ti = tele ? ld_stream(tele + i) : uni;
             r_new = fa * ti + fs * (acc + fd);
         }
-        r[i] = r_new;
-        const uint32_t od = outdeg[i];
+        st_stream(r + i, r_new);
+        const uint32_t od = ld_stream(outdeg + i);
         float cn;
         if constexpr (KIND == K_PR_STOCH_W) cn = od ? r_new : 0.0f;
         else cn = od ? r_new / (float)od : 0.0f;  // pageRank.c:297-301
-        c_out[i] = cn;
+        store_val(c_out, i, cn);
         const float diff = fabsf(r_new - r_old);
         if constexpr (KIND == K_PR_REF) part.err += (double)diff * invN;
         else part.err += (double)diff;
@@ -329,7 +487,9 @@ struct PrEpi {
     }
 };
 
-template <int KIND>
+// One iteration.  T: contribution element type (float or bf16 bits);
+// Acc: row accumulator; LDS: hot prefix of the contributions in LDS.
+template <int KIND, class T, class Acc, bool LDS>
 __global__ void __launch_bounds__(kBlock) pr_iter_kernel(PlanView P, IterView A)
 {
     const Slot *cur = A.slots + (A.t & 1);
@@ -338,9 +498,11 @@ __global__ void __launch_bounds__(kBlock) pr_iter_kernel(PlanView P, IterView A)
         if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = *cur;
         return;
     }
-    PrEpi<KIND> epi;
+    const T *c_in = reinterpret_cast<const T *>(A.c_in);
+    if constexpr (LDS) load_hot(c_in, A.hot);
+    PrEpi<KIND, T> epi;
     epi.r = A.r;
-    epi.c_out = A.c_out;
+    epi.c_out = reinterpret_cast<T *>(A.c_out);
     epi.outdeg = P.outdeg;
     epi.tele = A.tele;
     epi.base = 1.0f - A.damp;  // pageRank.c:55
@@ -356,11 +518,11 @@ __global__ void __launch_bounds__(kBlock) pr_iter_kernel(PlanView P, IterView A)
     }
     epi.part = {0.0, 0.0, 0.0, 0u, 0u};
     if constexpr (KIND == K_PR_STOCH_W) {
-        GatherW g{A.c_in, P.wt};
-        tile_loop(P, g, epi);
+        GatherCW<T, LDS> g{c_in, reinterpret_cast<const T *>(P.wt), A.hot};
+        tile_loop<Acc>(P, g, epi);
     } else {
-        GatherPlain g{A.c_in};
-        tile_loop(P, g, epi);
+        GatherC<T, LDS> g{c_in, A.hot};
+        tile_loop<Acc>(P, g, epi);
     }
     const uint32_t iters = cur->iters;
     const double eps = A.eps;
@@ -382,12 +544,13 @@ __global__ void __launch_bounds__(kBlock) pr_iter_kernel(PlanView P, IterView A)
 }
 
 // r0, c0 and slot 0 (sum and dangling mass of r0).
-template <int KIND>
+template <int KIND, class T>
 __global__ void __launch_bounds__(kBlock) pr_reset_kernel(PlanView P, IterView A)
 {
     Partial part = {0.0, 0.0, 0.0, 0u, 0u};
     const float base = 1.0f - A.damp;
     const float uni = (float)(1.0 / (double)P.N);
+    T *c_out = reinterpret_cast<T *>(A.c_out);
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < P.N; i += gridDim.x * kBlock) {
         float r0 = KIND == K_PR_REF ? base : (A.tele ? A.tele[i] : uni);
         A.r[i] = r0;
@@ -395,7 +558,7 @@ __global__ void __launch_bounds__(kBlock) pr_reset_kernel(PlanView P, IterView A
         float c0;
         if (KIND == K_PR_STOCH_W) c0 = od ? r0 : 0.0f;
         else c0 = od ? r0 / (float)od : 0.0f;
-        A.c_out[i] = c0;
+        store_val(c_out, i, c0);
         if (!od) part.dang += (double)r0;
         part.sum += (double)r0;
     }
@@ -440,11 +603,11 @@ spmv_kernel(PlanView P, const float *__restrict__ x, float *__restrict__ y)
 {
     SpmvEpi epi{y};
     if constexpr (KIND == K_SPMV_W) {
-        GatherW g{x, P.wt};
-        tile_loop(P, g, epi);
+        GatherCW<float, false> g{x, P.wt, 0};
+        tile_loop<AccF32>(P, g, epi);
     } else {
         GatherConstW g{x, 0.0001f};  // SPMV.c:177
-        tile_loop(P, g, epi);
+        tile_loop<AccF32>(P, g, epi);
     }
 }
 
@@ -525,6 +688,12 @@ __global__ void normalise_weights(const uint32_t *__restrict__ col, const float 
         const float s = S[perm[col[j]]];
         wn[j] = s > 0.0f ? (float)((double)wt[j] / (double)s) : 0.0f;
     }
+}
+
+__global__ void to_bf16(const float *__restrict__ in, uint32_t n, uint16_t *__restrict__ out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = f32_to_bf16(in[i]);
 }
 
 __global__ void gather_perm(const float *__restrict__ in, const uint32_t *__restrict__ perm,
@@ -743,6 +912,21 @@ static int ensure_normalised(gbx_ctx *ctx, gbx_graph *g, gbx_plan *p)
     return GBX_OK;
 }
 
+static int ensure_normalised_bf16(gbx_ctx *ctx, gbx_graph *g, gbx_plan *p)
+{
+    if (p->has_wn16) return GBX_OK;
+    GBX_TRY(ensure_normalised(ctx, g, p));
+    GBX_TRY(dev_alloc(ctx, p->wn16, (size_t)p->E * 2 + 16));
+    if (p->E) {
+        to_bf16<<<grid_for(p->E, 256, 16384), 256, 0, ctx->stream>>>(p->wn.as<float>(), p->E,
+                                                                      p->wn16.as<uint16_t>());
+        GBX_HIP(ctx, hipGetLastError());
+    }
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    p->has_wn16 = true;
+    return GBX_OK;
+}
+
 void plan_free(gbx_plan *p) { delete p; }
 
 static PlanView plan_view(const gbx_plan *p, const float *w)
@@ -765,7 +949,7 @@ static PlanView plan_view(const gbx_plan *p, const float *w)
 
 static unsigned iter_grid(gbx_ctx *ctx, const gbx_plan *p)
 {
-    unsigned per_cu = 8;
+    unsigned per_cu = 4;
     if (const char *s = getenv("GBX_BLOCKS_PER_CU")) {
         int v = atoi(s);
         if (v >= 1 && v <= 64) per_cu = (unsigned)v;
@@ -780,33 +964,74 @@ static IterView iter_view(gbx_pr *pr)
 {
     IterView a;
     a.r = pr->r.as<float>();
-    a.c_in = pr->c[pr->t & 1].as<float>();
-    a.c_out = pr->c[(pr->t + 1) & 1].as<float>();
+    a.c_in = pr->c[pr->t & 1].p;
+    a.c_out = pr->c[(pr->t + 1) & 1].p;
     a.tele = pr->has_tele ? pr->tele.as<float>() : nullptr;
     a.slots = pr->slots.as<Slot>();
     a.partials = pr->partials.as<Partial>();
     a.counter = pr->counter.as<uint32_t>();
     a.t = pr->t;
+    a.hot = pr->hot;
     a.damp = pr->p.damp;
     a.eps = pr->p.epsilon;
     a.stop = pr->p.stop;
     return a;
 }
 
+template <int KIND, class T, class Acc>
+static void launch_pr_kernel(bool lds, unsigned grid, size_t shm, hipStream_t st, const PlanView &P,
+                             const IterView &A)
+{
+    if (lds) pr_iter_kernel<KIND, T, Acc, true><<<grid, kBlock, shm, st>>>(P, A);
+    else pr_iter_kernel<KIND, T, Acc, false><<<grid, kBlock, 0, st>>>(P, A);
+}
+
+static const float *plan_weights(const gbx_pr *pr)
+{
+    if (pr->kind != K_PR_STOCH_W) return nullptr;
+    if (pr->elem == 2) return reinterpret_cast<const float *>(pr->plan->wn16.p);
+    return pr->plan->wn.as<float>();
+}
+
 static int launch_iter(gbx_pr *pr)
 {
     gbx_ctx *ctx = pr->ctx;
-    const float *w = pr->kind == K_PR_STOCH_W ? pr->plan->wn.as<float>() : nullptr;
-    PlanView P = plan_view(pr->plan, w);
+    PlanView P = plan_view(pr->plan, plan_weights(pr));
     IterView A = iter_view(pr);
-    switch (pr->kind) {
-    case K_PR_REF: pr_iter_kernel<K_PR_REF><<<pr->grid, kBlock, 0, ctx->stream>>>(P, A); break;
-    case K_PR_STOCH: pr_iter_kernel<K_PR_STOCH><<<pr->grid, kBlock, 0, ctx->stream>>>(P, A); break;
-    default: pr_iter_kernel<K_PR_STOCH_W><<<pr->grid, kBlock, 0, ctx->stream>>>(P, A); break;
+    const bool lds = pr->hot > 0;
+    const size_t shm = (size_t)pr->hot * pr->elem;
+    hipStream_t st = ctx->stream;
+    const unsigned g = pr->grid;
+    if (pr->elem == 4) {
+        switch (pr->kind) {
+        case K_PR_REF: launch_pr_kernel<K_PR_REF, float, AccF32>(lds, g, shm, st, P, A); break;
+        case K_PR_STOCH: launch_pr_kernel<K_PR_STOCH, float, AccF32>(lds, g, shm, st, P, A); break;
+        default: launch_pr_kernel<K_PR_STOCH_W, float, AccF32>(lds, g, shm, st, P, A); break;
+        }
+    } else {
+        switch (pr->kind) {
+        case K_PR_REF: launch_pr_kernel<K_PR_REF, uint16_t, AccKahan>(lds, g, shm, st, P, A); break;
+        case K_PR_STOCH: launch_pr_kernel<K_PR_STOCH, uint16_t, AccKahan>(lds, g, shm, st, P, A); break;
+        default: launch_pr_kernel<K_PR_STOCH_W, uint16_t, AccKahan>(lds, g, shm, st, P, A); break;
+        }
     }
     GBX_HIP(ctx, hipGetLastError());
     pr->t++;
     return GBX_OK;
+}
+
+// Contributions kept in LDS per block: GBX_LDS_BYTES (default below), a
+// multiple of 16 bytes, never more than the vector itself.
+static uint32_t hot_count(uint32_t N, size_t elem)
+{
+    size_t bytes = GBX_LDS_DEFAULT;
+    if (const char *s = getenv("GBX_LDS_BYTES")) bytes = (size_t)strtoull(s, nullptr, 10);
+    if (bytes > 160 * 1024) bytes = 160 * 1024;
+    size_t n = bytes / elem;
+    const size_t align = 16 / elem;
+    if (n > N) n = N;
+    n -= n % align;
+    return (uint32_t)n;
 }
 
 }  // namespace gbx
@@ -836,7 +1061,7 @@ int gbx_pr_create(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, gbx_pr **o
     if (p->mode != GBX_PR_REF && p->mode != GBX_PR_STOCHASTIC)
         return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: mode %d", p->mode);
     if (p->stop < 0 || p->stop > 2) return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: stop %d", p->stop);
-    if (p->precision != GBX_F32)
+    if (p->precision != GBX_F32 && p->precision != GBX_BF16)
         return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: precision %d not supported", p->precision);
     if (p->weighted && p->mode == GBX_PR_REF)
         return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: weighted PageRank needs STOCHASTIC mode");
@@ -845,7 +1070,10 @@ int gbx_pr_create(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, gbx_pr **o
     GBX_HIP(ctx, hipSetDevice(ctx->device));
     gbx_plan *plan = nullptr;
     GBX_TRY(ensure_plan(ctx, g, p->direction, &plan));
-    if (p->weighted) GBX_TRY(ensure_normalised(ctx, g, plan));
+    if (p->weighted) {
+        if (p->precision == GBX_BF16) GBX_TRY(ensure_normalised_bf16(ctx, g, plan));
+        else GBX_TRY(ensure_normalised(ctx, g, plan));
+    }
     gbx_pr *pr = new (std::nothrow) gbx_pr();
     if (!pr) return set_error(ctx, GBX_ERR_NOMEM, "gbx_pr_create: out of host memory");
     pr->ctx = ctx;
@@ -854,11 +1082,14 @@ int gbx_pr_create(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, gbx_pr **o
     pr->p = *p;
     pr->kind = p->mode == GBX_PR_REF ? K_PR_REF : (p->weighted ? K_PR_STOCH_W : K_PR_STOCH);
     pr->grid = iter_grid(ctx, plan);
+    pr->elem = p->precision == GBX_BF16 ? 2 : 4;
+    pr->hot = hot_count(g->N, pr->elem);
     const size_t nb = (size_t)g->N * 4;
+    const size_t cb = (size_t)g->N * pr->elem + 16;
     int rc = GBX_OK;
     if (!rc) rc = dev_alloc(ctx, pr->r, nb);
-    if (!rc) rc = dev_alloc(ctx, pr->c[0], nb);
-    if (!rc) rc = dev_alloc(ctx, pr->c[1], nb);
+    if (!rc) rc = dev_alloc(ctx, pr->c[0], cb);
+    if (!rc) rc = dev_alloc(ctx, pr->c[1], cb);
     if (!rc) rc = dev_alloc(ctx, pr->out, nb);
     if (!rc) rc = dev_alloc(ctx, pr->slots, 2 * sizeof(Slot));
     if (!rc) rc = dev_alloc(ctx, pr->partials, (size_t)std::max(pr->grid, 16384u) * sizeof(Partial));
@@ -899,15 +1130,23 @@ int gbx_pr_reset(gbx_pr *pr, const float *teleport)
     pr->t = 0;
     pr->timing_started = false;
     pr->elapsed_ms = 0.0;
-    const float *w = pr->kind == K_PR_STOCH_W ? pr->plan->wn.as<float>() : nullptr;
-    PlanView P = plan_view(pr->plan, w);
+    PlanView P = plan_view(pr->plan, plan_weights(pr));
     IterView A = iter_view(pr);
-    A.c_out = pr->c[0].as<float>();
+    A.c_out = pr->c[0].p;
     const unsigned grid = grid_for(N, kBlock, (unsigned)std::max(1, ctx->num_cus) * 8);
-    switch (pr->kind) {
-    case K_PR_REF: pr_reset_kernel<K_PR_REF><<<grid, kBlock, 0, ctx->stream>>>(P, A); break;
-    case K_PR_STOCH: pr_reset_kernel<K_PR_STOCH><<<grid, kBlock, 0, ctx->stream>>>(P, A); break;
-    default: pr_reset_kernel<K_PR_STOCH_W><<<grid, kBlock, 0, ctx->stream>>>(P, A); break;
+    hipStream_t st = ctx->stream;
+    if (pr->elem == 4) {
+        switch (pr->kind) {
+        case K_PR_REF: pr_reset_kernel<K_PR_REF, float><<<grid, kBlock, 0, st>>>(P, A); break;
+        case K_PR_STOCH: pr_reset_kernel<K_PR_STOCH, float><<<grid, kBlock, 0, st>>>(P, A); break;
+        default: pr_reset_kernel<K_PR_STOCH_W, float><<<grid, kBlock, 0, st>>>(P, A); break;
+        }
+    } else {
+        switch (pr->kind) {
+        case K_PR_REF: pr_reset_kernel<K_PR_REF, uint16_t><<<grid, kBlock, 0, st>>>(P, A); break;
+        case K_PR_STOCH: pr_reset_kernel<K_PR_STOCH, uint16_t><<<grid, kBlock, 0, st>>>(P, A); break;
+        default: pr_reset_kernel<K_PR_STOCH_W, uint16_t><<<grid, kBlock, 0, st>>>(P, A); break;
+        }
     }
     GBX_HIP(ctx, hipGetLastError());
     return GBX_OK;

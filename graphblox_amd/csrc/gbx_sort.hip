@@ -229,6 +229,18 @@ __global__ void dedupe_compact(const uint32_t *__restrict__ src, const uint32_t 
     }
 }
 
+// flag[0] |= 1 if src is not sorted, |= 2 if an id is >= N
+__global__ void check_sorted(const uint32_t *__restrict__ src, const uint32_t *__restrict__ dst,
+                             uint32_t E, uint32_t N, uint32_t *__restrict__ flag)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < E; i += gridDim.x * blockDim.x) {
+        uint32_t f = 0;
+        if (i > 0 && src[i - 1] > src[i]) f |= 1u;
+        if ((src[i] & kIdMask) >= N || (dst[i] & kIdMask) >= N) f |= 2u;
+        if (f) atomicOr(flag, f);
+    }
+}
+
 __global__ void fill_f32_kernel(float *p, float v, size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -435,6 +447,90 @@ int gbx_graph_create(gbx_ctx *ctx, const uint32_t *src, const uint32_t *dst, con
     uint32_t N = num_vertices;
     GBX_TRY(check_ids(ctx, ds, dd, E, &N));
     return graph_from_device_edges(ctx, ds, dd, dw, E, N, sort_mode, flags, out);
+}
+
+int gbx_graph_from_csr(gbx_ctx *ctx, uint32_t num_vertices, uint32_t num_edges,
+                       const uint32_t *fwd_out_degree, const uint32_t *fwd_edges_idx,
+                       const uint32_t *fwd_src, const uint32_t *fwd_dst, const float *fwd_w,
+                       const uint32_t *inv_out_degree, const uint32_t *inv_edges_idx,
+                       const uint32_t *inv_src, const uint32_t *inv_dst, const float *inv_w,
+                       gbx_graph **out)
+{
+    if (!ctx || !out) return set_error(ctx, GBX_ERR_ARG, "gbx_graph_from_csr: NULL argument");
+    *out = nullptr;
+    if (num_vertices == 0 || num_vertices > kMaxVertices)
+        return set_error(ctx, GBX_ERR_ARG, "gbx_graph_from_csr: num_vertices %u", num_vertices);
+    if (!fwd_out_degree || !fwd_edges_idx || !inv_out_degree || !inv_edges_idx ||
+        (num_edges && (!fwd_src || !fwd_dst || !inv_src || !inv_dst)))
+        return set_error(ctx, GBX_ERR_ARG, "gbx_graph_from_csr: NULL array");
+    GBX_HIP(ctx, hipSetDevice(ctx->device));
+    gbx_graph *g = new (std::nothrow) gbx_graph();
+    if (!g) return set_error(ctx, GBX_ERR_NOMEM, "gbx_graph_from_csr: out of host memory");
+    g->ctx = ctx;
+    g->N = num_vertices;
+    g->E = num_edges;
+    g->weighted = fwd_w && inv_w;
+    const size_t nb = (size_t)num_vertices * 4, eb = (size_t)num_edges * 4;
+    struct Side {
+        gbx_csr_dev *d;
+        const uint32_t *od, *idx, *src, *dst;
+        const float *w;
+    } sides[2] = {{&g->fwd, fwd_out_degree, fwd_edges_idx, fwd_src, fwd_dst, fwd_w},
+                  {&g->inv, inv_out_degree, inv_edges_idx, inv_src, inv_dst, inv_w}};
+    int rc = GBX_OK;
+    for (auto &sd : sides) {
+        if (!rc) rc = dev_alloc(ctx, sd.d->out_degree, nb);
+        if (!rc) rc = dev_alloc(ctx, sd.d->in_degree, nb);
+        if (!rc) rc = dev_alloc(ctx, sd.d->edges_idx, nb);
+        if (!rc) rc = dev_alloc(ctx, sd.d->src, eb);
+        if (!rc) rc = dev_alloc(ctx, sd.d->dst, eb);
+        if (!rc) rc = dev_alloc(ctx, sd.d->w, eb);
+        if (rc) break;
+        if (hipMemcpyAsync(sd.d->out_degree.p, sd.od, nb, hipMemcpyHostToDevice, ctx->stream) ||
+            hipMemcpyAsync(sd.d->edges_idx.p, sd.idx, nb, hipMemcpyHostToDevice, ctx->stream) ||
+            (eb && hipMemcpyAsync(sd.d->src.p, sd.src, eb, hipMemcpyHostToDevice, ctx->stream)) ||
+            (eb && hipMemcpyAsync(sd.d->dst.p, sd.dst, eb, hipMemcpyHostToDevice, ctx->stream))) {
+            rc = set_error(ctx, GBX_ERR_HIP, "gbx_graph_from_csr: upload failed");
+            break;
+        }
+        if (eb && sd.w) {
+            if (hipMemcpyAsync(sd.d->w.p, sd.w, eb, hipMemcpyHostToDevice, ctx->stream))
+                rc = set_error(ctx, GBX_ERR_HIP, "gbx_graph_from_csr: upload failed");
+        } else if (eb) {
+            fill_f32_kernel<<<grid_for(num_edges, 256, 8192), 256, 0, ctx->stream>>>(sd.d->w.as<float>(), 1.0f,
+                                                                                   num_edges);
+        }
+    }
+    if (!rc) {
+        // in_degree of each side = out_degree of the other (vertex.c:291-311)
+        if (hipMemcpyAsync(g->fwd.in_degree.p, g->inv.out_degree.p, nb, hipMemcpyDeviceToDevice, ctx->stream) ||
+            hipMemcpyAsync(g->inv.in_degree.p, g->fwd.out_degree.p, nb, hipMemcpyDeviceToDevice, ctx->stream))
+            rc = set_error(ctx, GBX_ERR_HIP, "gbx_graph_from_csr: copy failed");
+    }
+    if (!rc && num_edges) {
+        DevBuf flag;
+        uint32_t hflag = 0;
+        rc = dev_alloc(ctx, flag, 4);
+        if (!rc) rc = fill_u32(ctx, flag.as<uint32_t>(), 0, 1);
+        for (int k = 0; k < 2 && !rc; k++) {
+            check_sorted<<<grid_for(num_edges, 256, 16384), 256, 0, ctx->stream>>>(
+                sides[k].d->src.as<uint32_t>(), sides[k].d->dst.as<uint32_t>(), num_edges, num_vertices,
+                flag.as<uint32_t>());
+        }
+        if (!rc && (hipMemcpyAsync(&hflag, flag.p, 4, hipMemcpyDeviceToHost, ctx->stream) ||
+                    hipStreamSynchronize(ctx->stream)))
+            rc = set_error(ctx, GBX_ERR_HIP, "gbx_graph_from_csr: check failed");
+        if (!rc && (hflag & 2)) rc = set_error(ctx, GBX_ERR_RANGE, "gbx_graph_from_csr: vertex id >= num_vertices");
+        if (!rc && (hflag & 1)) rc = set_error(ctx, GBX_ERR_ARG, "gbx_graph_from_csr: edges not sorted by source");
+    }
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess)
+        rc = set_error(ctx, GBX_ERR_HIP, "gbx_graph_from_csr: sync failed");
+    if (rc) {
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return GBX_OK;
 }
 
 int gbx_graph_info(const gbx_graph *g, uint32_t *num_vertices, uint32_t *num_edges)

@@ -1,0 +1,495 @@
+/*
+ * graphblox_hip.c -- the GraphBlox host API for pull PageRank / SpMV on the
+ * MI355X backend.  Each function keeps the name, arguments, ownership and
+ * error behaviour of the reference function cited above it and runs the
+ * compute through the C ABI of libgbx_hip.so (include/gbx.h).
+ */
+#define _GNU_SOURCE
+#include "graphblox_hip.h"
+
+#include <fcntl.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include "gbx.h"
+
+#define GBX_HOST_ALIGN 4096 /* myMalloc.h:9-11 (ALIGNED, CACHELINE_BYTES) */
+
+/* 4096-byte aligned allocation that ends the process on failure
+ * (my_malloc, myMalloc.c:24-67). */
+static void *host_malloc(size_t size)
+{
+    void *p = NULL;
+    if (size == 0) size = 4;
+    if (posix_memalign(&p, GBX_HOST_ALIGN, size) != 0 || !p) {
+        fprintf(stderr, "graphblox_hip: out of host memory (%zu bytes)\n", size);
+        exit(EXIT_FAILURE);
+    }
+    return p;
+}
+
+static void die_io(const char *what, const char *fname)
+{
+    fprintf(stderr, "graphblox_hip: %s: %s\n", what, fname);
+    exit(1);
+}
+
+/* ------------------------------------------------------------ backend state */
+
+static gbx_ctx *g_ctx = NULL;
+
+struct graph_entry {
+    const struct GraphCSR *host;
+    gbx_graph *dev;
+    struct graph_entry *next;
+};
+static struct graph_entry *g_graphs = NULL;
+
+static gbx_ctx *backend(void)
+{
+    if (!g_ctx) {
+        int dev = 0;
+        const char *s = getenv("GBX_DEVICE");
+        if (s) dev = atoi(s);
+        int rc = gbx_init(dev, &g_ctx);
+        if (rc != GBX_OK) {
+            fprintf(stderr, "graphblox_hip: HIP backend unavailable: %s\n", gbx_last_error(NULL));
+            g_ctx = NULL;
+        }
+    }
+    return g_ctx;
+}
+
+static void register_graph(const struct GraphCSR *host, gbx_graph *dev)
+{
+    struct graph_entry *e = host_malloc(sizeof(*e));
+    e->host = host;
+    e->dev = dev;
+    e->next = g_graphs;
+    g_graphs = e;
+}
+
+static gbx_graph *lookup_graph(const struct GraphCSR *host)
+{
+    for (struct graph_entry *e = g_graphs; e; e = e->next)
+        if (e->host == host) return e->dev;
+    return NULL;
+}
+
+static void unregister_graph(const struct GraphCSR *host)
+{
+    struct graph_entry **pp = &g_graphs;
+    while (*pp) {
+        if ((*pp)->host == host) {
+            struct graph_entry *e = *pp;
+            *pp = e->next;
+            gbx_graph_free(e->dev);
+            free(e);
+            return;
+        }
+        pp = &(*pp)->next;
+    }
+}
+
+void graphbloxHipShutdown(void)
+{
+    while (g_graphs) {
+        struct graph_entry *e = g_graphs;
+        g_graphs = e->next;
+        gbx_graph_free(e->dev);
+        free(e);
+    }
+    if (g_ctx) gbx_free(g_ctx);
+    g_ctx = NULL;
+}
+
+/* Device graph for a host GraphCSR: the one built by graphCSRPreProcessingStep,
+ * or an upload of arrays built elsewhere (gbx_graph_from_csr). */
+static gbx_graph *device_graph(struct GraphCSR *graph)
+{
+    gbx_graph *g = lookup_graph(graph);
+    if (g) return g;
+    gbx_ctx *ctx = backend();
+    if (!ctx) return NULL;
+    struct Vertex *fv = graph->vertices, *iv = graph->inverse_vertices;
+    struct EdgeList *fe = graph->sorted_edges_array, *ie = graph->inverse_sorted_edges_array;
+    if (!fv || !iv || !fe || !ie) {
+        fprintf(stderr, "graphblox_hip: graph has no transpose CSR (DIRECTED=1 layout required)\n");
+        return NULL;
+    }
+    const float *fw = GBX_WEIGHTED ? fe->edges_array_weight : NULL;
+    const float *iw = GBX_WEIGHTED ? ie->edges_array_weight : NULL;
+    int rc = gbx_graph_from_csr(ctx, graph->num_vertices, graph->num_edges, fv->out_degree, fv->edges_idx,
+                                fe->edges_array_src, fe->edges_array_dest, fw, iv->out_degree, iv->edges_idx,
+                                ie->edges_array_src, ie->edges_array_dest, iw, &g);
+    if (rc != GBX_OK) {
+        fprintf(stderr, "graphblox_hip: upload failed: %s\n", gbx_last_error(ctx));
+        return NULL;
+    }
+    register_graph(graph, g);
+    return g;
+}
+
+/* ------------------------------------------------------------ arguments */
+
+/* argumentsNew, graphConfig.c:47-101 (defaults) */
+struct Arguments *argumentsNew(void)
+{
+    struct Arguments *a = host_malloc(sizeof(*a));
+    memset(a, 0, sizeof(*a));
+    a->binSize = 512;
+    a->iterations = 20;
+    a->trials = 1;
+    a->epsilon = 0.0001;
+    a->cache_size = 32768;
+    a->delta = 1;
+    a->pre_numThreads = 1;
+    a->algo_numThreads = 1;
+    a->ker_numThreads = 1;
+    a->fnameb_format = 1;
+    a->convert_format = 1;
+    a->flush_cache = 1;
+    return a;
+}
+
+void argumentsFree(struct Arguments *arguments) { free(arguments); }
+
+/* ------------------------------------------------------------ edge lists */
+
+static struct EdgeList *new_edge_list(uint32_t num_edges)
+{
+    struct EdgeList *e = host_malloc(sizeof(*e));
+    memset(e, 0, sizeof(*e));
+    e->edges_array_src = host_malloc((size_t)num_edges * 4);
+    e->edges_array_dest = host_malloc((size_t)num_edges * 4);
+    e->edges_array_weight = host_malloc((size_t)num_edges * 4);
+    for (uint32_t i = 0; i < num_edges; i++) e->edges_array_weight[i] = 1.0f; /* edgeList.c:108-114 */
+    e->num_edges = num_edges;
+    e->max_weight = 1.0f;
+    return e;
+}
+
+static void label_arrays(struct EdgeList *e, uint32_t num_vertices)
+{
+    e->mask_array = host_malloc(((size_t)num_vertices + 1) * 4);
+    e->label_array = host_malloc(((size_t)num_vertices + 1) * 4);
+    e->inverse_label_array = host_malloc(((size_t)num_vertices + 1) * 4);
+    for (uint32_t i = 0; i < num_vertices; i++) {
+        e->mask_array[i] = 0;
+        e->label_array[i] = i;
+        e->inverse_label_array[i] = i;
+    }
+}
+
+/* readEdgeListsbin, edgeList.c:450-706 */
+struct EdgeList *readEdgeListsbin(const char *fname, uint8_t inverse, uint32_t symmetric, uint32_t weighted)
+{
+    int fd = open(fname, O_RDONLY);
+    if (fd == -1) die_io("open", fname);
+    struct stat fs;
+    if (fstat(fd, &fs) == -1) die_io("stat", fname);
+    const size_t rec = weighted ? 12 : 8;
+    const uint64_t n64 = (uint64_t)fs.st_size / rec;
+    if (n64 > 0xFFFFFFFFull || (symmetric && n64 > 0x7FFFFFFFull)) die_io("too many edges", fname);
+    const uint32_t num_edges = (uint32_t)n64;
+    const uint32_t *buf = NULL;
+    void *map = NULL;
+    if (fs.st_size > 0) {
+        map = mmap(NULL, (size_t)fs.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (map == MAP_FAILED) die_io("mmap", fname);
+        buf = (const uint32_t *)map;
+    }
+    const uint32_t total = symmetric ? 2 * num_edges : num_edges;
+    struct EdgeList *e = new_edge_list(total);
+    const size_t stride = rec / 4;
+    uint32_t maxid = 0;
+    float max_weight = 0.0f;
+    for (uint32_t i = 0; i < num_edges; i++) {
+        uint32_t s = buf[stride * i], d = buf[stride * i + 1];
+        if (inverse) {
+            uint32_t t = s;
+            s = d;
+            d = t;
+        }
+        e->edges_array_src[i] = s;
+        e->edges_array_dest[i] = d;
+        if (symmetric) {
+            e->edges_array_src[i + num_edges] = d;
+            e->edges_array_dest[i + num_edges] = s;
+        }
+#if GBX_WEIGHTED
+        float w = 1.0f;
+        if (weighted) memcpy(&w, &buf[stride * i + 2], 4);
+        e->edges_array_weight[i] = w;
+        if (symmetric) e->edges_array_weight[i + num_edges] = w;
+        if (w > max_weight) max_weight = w;
+#endif
+        if (s > maxid) maxid = s;
+        if (d > maxid) maxid = d;
+    }
+    e->num_vertices = maxid + 1; /* edgeList.c:679 */
+    e->avg_degree = e->num_edges / e->num_vertices;
+    e->max_weight = GBX_WEIGHTED ? max_weight : 1.0f;
+    label_arrays(e, e->num_vertices);
+    if (map) munmap(map, (size_t)fs.st_size);
+    close(fd);
+    return e;
+}
+
+void freeEdgeList(struct EdgeList *e)
+{
+    if (!e) return;
+    free(e->edges_array_src);
+    free(e->edges_array_dest);
+    free(e->edges_array_weight);
+    free(e->mask_array);
+    free(e->label_array);
+    free(e->inverse_label_array);
+    free(e);
+}
+
+/* ------------------------------------------------------------ CSR */
+
+static struct Vertex *new_vertex_array(uint32_t num_vertices)
+{
+    struct Vertex *v = host_malloc(sizeof(*v));
+    v->num_vertices = num_vertices;
+    v->out_degree = host_malloc(((size_t)num_vertices + 1) * 4);
+    v->in_degree = host_malloc(((size_t)num_vertices + 1) * 4);
+    v->edges_idx = host_malloc(((size_t)num_vertices + 1) * 4);
+    v->out_degree[num_vertices] = v->in_degree[num_vertices] = v->edges_idx[num_vertices] = 0;
+    return v;
+}
+
+static void free_vertex_array(struct Vertex *v)
+{
+    if (!v) return;
+    free(v->out_degree);
+    free(v->in_degree);
+    free(v->edges_idx);
+    free(v);
+}
+
+/* graphCSRPreProcessingStep, graphCSR.c:243-327 (DIRECTED=1): the reader
+ * runs on the host; both count sorts, the -k dedupe and the vertex mapping
+ * run on the GPU. */
+struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
+{
+    gbx_ctx *ctx = backend();
+    if (!ctx) exit(EXIT_FAILURE);
+    if (arguments->lmode || arguments->lmode_l2 || arguments->lmode_l3 || arguments->mmode)
+        fprintf(stderr, "graphblox_hip: relabelling/mask modes are not provided by this backend; ignored\n");
+    struct EdgeList *in = readEdgeListsbin(arguments->fnameb, 0, arguments->symmetric, arguments->weighted);
+    unsigned flags = 0;
+    uint32_t sort = arguments->sort;
+    if (arguments->dflag) {
+        flags |= GBX_GRAPH_DEDUPE;
+        sort = 1; /* GraphBlox.c:358-359 */
+    }
+    if (GBX_WEIGHTED) flags |= GBX_GRAPH_WEIGHTED;
+    if (sort > 3) sort = 0; /* sortRun.c:59-64: unknown modes fall back to radix by source */
+    gbx_graph *g = NULL;
+    int rc = gbx_graph_create(ctx, in->edges_array_src, in->edges_array_dest, in->edges_array_weight,
+                              in->num_edges, in->num_vertices, (int)sort, flags, &g);
+    if (rc != GBX_OK) {
+        fprintf(stderr, "graphblox_hip: CSR build failed: %s\n", gbx_last_error(ctx));
+        exit(EXIT_FAILURE);
+    }
+    uint32_t N = 0, E = 0;
+    gbx_graph_info(g, &N, &E);
+    struct GraphCSR *csr = host_malloc(sizeof(*csr));
+    memset(csr, 0, sizeof(*csr));
+    csr->num_vertices = N;
+    csr->num_edges = E;
+    csr->avg_degree = E / N;
+#if GBX_WEIGHTED
+    csr->max_weight = in->max_weight;
+#endif
+    for (int inv = 0; inv < 2; inv++) {
+        struct Vertex *v = new_vertex_array(N);
+        struct EdgeList *el = new_edge_list(E);
+        el->num_vertices = N;
+        el->avg_degree = E / N;
+        el->max_weight = in->max_weight;
+        label_arrays(el, N);
+        rc = gbx_graph_export(g, inv, v->out_degree, v->in_degree, v->edges_idx, el->edges_array_src,
+                              el->edges_array_dest, el->edges_array_weight);
+        if (rc != GBX_OK) {
+            fprintf(stderr, "graphblox_hip: CSR export failed: %s\n", gbx_last_error(ctx));
+            exit(EXIT_FAILURE);
+        }
+        if (inv) {
+            csr->inverse_vertices = v;
+            csr->inverse_sorted_edges_array = el;
+        } else {
+            csr->vertices = v;
+            csr->sorted_edges_array = el;
+        }
+    }
+    freeEdgeList(in);
+    register_graph(csr, g);
+    printf(" -----------------------------------------------------\n");
+    printf("| %-51s | \n", "GraphCSR (HIP backend)");
+    printf(" -----------------------------------------------------\n");
+    printf("| %-30s | %-18u | \n", "Number of Vertices (V)", N);
+    printf("| %-30s | %-18u | \n", "Number of Edges (E)", E);
+    printf(" -----------------------------------------------------\n");
+    return csr;
+}
+
+void graphCSRFree(struct GraphCSR *graph)
+{
+    if (!graph) return;
+    unregister_graph(graph);
+    free_vertex_array(graph->vertices);
+    free_vertex_array(graph->inverse_vertices);
+    freeEdgeList(graph->sorted_edges_array);
+    freeEdgeList(graph->inverse_sorted_edges_array);
+    free(graph);
+}
+
+/* ------------------------------------------------------------ PageRank */
+
+/* newPageRankStatsGraphCSR / freePageRankStats, pageRank.c:47-88 */
+struct PageRankStats *newPageRankStatsGraphCSR(struct GraphCSR *graph)
+{
+    struct PageRankStats *s = host_malloc(sizeof(*s));
+    s->damp = 0.85f; /* Damp, pageRank.h:13 */
+    s->base_pr = 1.0f - s->damp;
+    s->iterations = 0;
+    s->num_vertices = graph->num_vertices;
+    s->activeVertices = 0;
+    s->time_total = 0.0;
+    s->error_total = 0.0;
+    s->realRanks = host_malloc((size_t)graph->num_vertices * 4);
+    s->pageRanks = host_malloc((size_t)graph->num_vertices * 4);
+    for (uint32_t v = 0; v < graph->num_vertices; v++) {
+        s->pageRanks[v] = s->base_pr;
+        s->realRanks[v] = v;
+    }
+    return s;
+}
+
+void freePageRankStats(struct PageRankStats *stats)
+{
+    if (!stats) return;
+    free(stats->realRanks);
+    free(stats->pageRanks);
+    free(stats);
+}
+
+/* pageRankPullGraphCSR, pageRank.c:245-368 (DIRECTED=1 rows, FP32) */
+struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    gbx_graph *g = device_graph(graph);
+    if (!g) return NULL;
+    struct PageRankStats *stats = newPageRankStatsGraphCSR(graph);
+    gbx_pr_params p;
+    gbx_pr_params_default(&p);
+    p.damp = stats->damp;
+    p.max_iters = arguments->iterations;
+    p.epsilon = arguments->epsilon;
+    p.mode = GBX_PR_REF;
+    p.stop = GBX_STOP_ACTIVE;
+    p.precision = GBX_F32;
+    p.direction = GBX_PULL_IN;
+    gbx_pr_report rep;
+    int rc = gbx_pr_pull(g_ctx, g, &p, NULL, stats->pageRanks, &rep);
+    if (rc != GBX_OK) {
+        fprintf(stderr, "graphblox_hip: pageRankPullGraphCSR failed: %s\n", gbx_last_error(g_ctx));
+        freePageRankStats(stats);
+        return NULL;
+    }
+    stats->iterations = rep.iterations;
+    stats->activeVertices = rep.active;
+    stats->error_total = rep.error_total;
+    stats->time_total = rep.time_total_s;
+    printf(" -----------------------------------------------------\n");
+    printf("| %-10s | %-8s | %-15s | %-9s | \n", "Iterations", "PR Sum", "Error", "Time (S)");
+    printf(" -----------------------------------------------------\n");
+    printf("| %-10u | %-8lf | %-15.13lf | %-9f | \n", stats->iterations, rep.rank_sum, stats->error_total,
+           stats->time_total);
+    printf(" -----------------------------------------------------\n");
+    return stats;
+}
+
+/* pageRankGraphCSR, pageRank.c:177-242: variant 0 (and unknown codes) is
+ * the pull kernel; the other reference variants (push, fixed point,
+ * quantised, data driven) are not provided by this backend -> NULL. */
+struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    if (arguments->pushpull >= 1 && arguments->pushpull <= 13) {
+        fprintf(stderr, "graphblox_hip: PageRank variant %u is not provided by the HIP backend\n",
+                arguments->pushpull);
+        return NULL;
+    }
+    return pageRankPullGraphCSR(arguments, graph);
+}
+
+/* ------------------------------------------------------------ SpMV */
+
+/* newSPMVStatsGraphCSR / freeSPMVStats, SPMV.c:27-68 */
+struct SPMVStats *newSPMVStatsGraphCSR(struct GraphCSR *graph)
+{
+    struct SPMVStats *s = host_malloc(sizeof(*s));
+    s->iterations = 0;
+    s->arguments = NULL;
+    s->num_vertices = graph->num_vertices;
+    s->time_total = 0.0;
+    s->vector_output = host_malloc((size_t)graph->num_vertices * 4);
+    s->vector_input = host_malloc((size_t)graph->num_vertices * 4);
+    for (uint32_t v = 0; v < graph->num_vertices; v++) s->vector_output[v] = s->vector_input[v] = 0.0f;
+    return s;
+}
+
+void freeSPMVStats(struct SPMVStats *stats)
+{
+    if (!stats) return;
+    free(stats->vector_output);
+    free(stats->vector_input);
+    free(stats);
+}
+
+/* SPMVPullGraphCSR, SPMV.c:117-215 (DIRECTED=1 rows; weighted when GBX_WEIGHTED) */
+struct SPMVStats *SPMVPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    gbx_graph *g = device_graph(graph);
+    if (!g) return NULL;
+    struct SPMVStats *stats = newSPMVStatsGraphCSR(graph);
+    stats->arguments = arguments;
+    double secs = 0.0;
+    int rc = gbx_spmv_pull(g_ctx, g, arguments->iterations, GBX_WEIGHTED, GBX_PULL_IN, NULL,
+                           stats->vector_input, stats->vector_output, &secs);
+    if (rc != GBX_OK) {
+        fprintf(stderr, "graphblox_hip: SPMVPullGraphCSR failed: %s\n", gbx_last_error(g_ctx));
+        freeSPMVStats(stats);
+        return NULL;
+    }
+    stats->iterations = arguments->iterations;
+    stats->time_total = secs;
+    double sum = 0.0;
+    for (uint32_t v = 0; v < graph->num_vertices; v++)
+        sum += ((int)(stats->vector_output[v] * 100 + .5) / 100.0); /* SPMV.c:196-200 */
+    printf(" -----------------------------------------------------\n");
+    printf("| %-15s | %-15s | %-15s | \n", "Iterations", "Sum", "Time (S)");
+    printf(" -----------------------------------------------------\n");
+    printf("| %-15u | %-15lf | %-15f | \n", stats->iterations, sum, stats->time_total);
+    printf(" -----------------------------------------------------\n");
+    return stats;
+}
+
+/* SPMVGraphCSR, SPMV.c:89-116: variant 0 (and unknown codes) is the pull
+ * kernel; push and fixed-point variants are not provided -> NULL. */
+struct SPMVStats *SPMVGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    if (arguments->pushpull >= 1 && arguments->pushpull <= 3) {
+        fprintf(stderr, "graphblox_hip: SpMV variant %u is not provided by the HIP backend\n",
+                arguments->pushpull);
+        return NULL;
+    }
+    return SPMVPullGraphCSR(arguments, graph);
+}

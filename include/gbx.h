@@ -113,6 +113,18 @@ int gbx_rmat_edges(gbx_ctx *ctx, uint32_t scale, uint32_t edge_factor, double a,
                    uint64_t weight_seed, uint32_t *src, uint32_t *dst, float *w,
                    uint64_t *num_edges);
 
+/* Device-resident graph from CSR arrays already built on the host in the
+ * reference's layout (e.g. by graphCSRPreProcessingStep): forward and
+ * transpose out_degree/edges_idx [num_vertices], src/dst [num_edges],
+ * w [num_edges] (nullable).  Arrays are copied; nothing is re-sorted.
+ * Sorted order is checked (src non-decreasing on both sides). */
+int gbx_graph_from_csr(gbx_ctx *ctx, uint32_t num_vertices, uint32_t num_edges,
+                       const uint32_t *fwd_out_degree, const uint32_t *fwd_edges_idx,
+                       const uint32_t *fwd_src, const uint32_t *fwd_dst, const float *fwd_w,
+                       const uint32_t *inv_out_degree, const uint32_t *inv_edges_idx,
+                       const uint32_t *inv_src, const uint32_t *inv_dst, const float *inv_w,
+                       gbx_graph **out);
+
 int gbx_graph_info(const gbx_graph *g, uint32_t *num_vertices, uint32_t *num_edges);
 
 /* Copy the forward (inverse = 0) or transpose (inverse = 1) CSR arrays to

@@ -1,0 +1,174 @@
+/*
+ * graphblox_hip.h -- the GraphBlox host API for the pull PageRank / SpMV
+ * path, served by the MI355X backend (libgraphblox_hip.so over libgbx_hip.so).
+ *
+ * The structs below have the same field order and types as the reference's
+ * own headers so that reference host code (the L2 dispatch in
+ * 01_host/src/utils_graph/graphRun.c) can call these functions unchanged:
+ *   struct Arguments      01_host/include/config/graphConfig.h:18-63
+ *   struct EdgeList       01_host/include/structures/edgeList.h:12-25
+ *   struct Vertex         01_host/include/structures/vertex.h:17-23
+ *   struct GraphCSR       01_host/include/structures/graphCSR.h:14-31 (DIRECTED=1)
+ *   struct PageRankStats  01_host/include/algorithms/openmp/pageRank.h:19-30
+ *   struct SPMVStats      01_host/include/algorithms/openmp/SPMV.h:17-26
+ * GBX_WEIGHTED selects the WEIGHTED=1 layout (GraphCSR.max_weight) and the
+ * weighted SpMV, as the reference's compile-time WEIGHTED switch does.
+ *
+ * Ownership follows the reference: the callee allocates stats and arrays
+ * (4096-byte aligned, myMalloc.c:24-39); the caller frees them with
+ * freePageRankStats / freeSPMVStats / graphCSRFree.  I/O failures end the
+ * process as err(1, ...) does in the reference (edgeList.c:466-486); a GPU
+ * failure inside an algorithm prints the backend's message and returns NULL,
+ * which the reference's dispatch treats as "not available".
+ */
+#ifndef GRAPHBLOX_HIP_H
+#define GRAPHBLOX_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef GBX_WEIGHTED
+#define GBX_WEIGHTED 0
+#endif
+
+/* layout of the reference's mt19937 state (mt19937.h): 624 words + index */
+typedef struct gbx_mt19937_state {
+    unsigned long mt[624];
+    int mti;
+} gbx_mt19937_state;
+
+struct Arguments {
+    int wflag;
+    int xflag;
+    int sflag;
+    int Sflag;
+    int dflag;
+    uint32_t binSize;
+    uint32_t verbosity;
+    uint32_t iterations;
+    uint32_t trials;
+    double epsilon;
+    uint32_t source;
+    uint32_t algorithm;
+    uint32_t datastructure;
+    uint32_t pushpull;
+    uint32_t sort;
+    uint32_t lmode;
+    uint32_t lmode_l2;
+    uint32_t lmode_l3;
+    uint32_t mmode;
+    uint32_t symmetric;
+    uint32_t weighted;
+    uint32_t delta;
+    int pre_numThreads;
+    int algo_numThreads;
+    int ker_numThreads;
+    char *fnameb;
+    char *fnamel;
+    uint32_t fnameb_format;
+    uint32_t convert_format;
+    gbx_mt19937_state mt19937var;
+    uint32_t cache_size;
+    char *kernel_name;
+    int cu_id;
+    int device_index;
+    char *xclbin_path;
+    char *overlay_path;
+    int ctrl_mode;
+    int endian_read;
+    int endian_write;
+    int flush_cache;
+    int bankGroupIndex;
+};
+
+struct EdgeList {
+    uint32_t num_edges;
+    uint32_t num_vertices;
+    uint32_t avg_degree;
+    float max_weight;
+    float *edges_array_weight;
+    uint32_t *edges_array_src;
+    uint32_t *edges_array_dest;
+    uint32_t *mask_array;
+    uint32_t *label_array;
+    uint32_t *inverse_label_array;
+};
+
+struct Vertex {
+    uint32_t num_vertices;
+    uint32_t *out_degree;
+    uint32_t *in_degree;
+    uint32_t *edges_idx;
+};
+
+struct GraphCSR {
+    uint32_t num_edges;
+    uint32_t num_vertices;
+    uint32_t avg_degree;
+#if GBX_WEIGHTED
+    float max_weight;
+#endif
+    struct Vertex *vertices;
+    struct EdgeList *sorted_edges_array;
+    struct Vertex *inverse_vertices;
+    struct EdgeList *inverse_sorted_edges_array;
+};
+
+struct PageRankStats {
+    float damp;
+    float base_pr;
+    uint32_t iterations;
+    uint32_t num_vertices;
+    uint32_t activeVertices;
+    uint32_t *realRanks;
+    float *pageRanks;
+    double time_total;
+    double error_total;
+};
+
+struct SPMVStats {
+    uint32_t iterations;
+    struct Arguments *arguments;
+    uint32_t num_vertices;
+    float *vector_output;
+    float *vector_input;
+    double time_total;
+};
+
+/* graphConfig.c: defaults of argumentsNew (iterations 20, epsilon 1e-4, ...) */
+struct Arguments *argumentsNew(void);
+void argumentsFree(struct Arguments *arguments);
+
+/* edgeList.c:450-706: mmap'd .bin/.wbin reader, N = max id + 1 */
+struct EdgeList *readEdgeListsbin(const char *fname, uint8_t inverse, uint32_t symmetric,
+                                  uint32_t weighted);
+void freeEdgeList(struct EdgeList *edgeList);
+
+/* graphCSR.c:243-327: read, sort (arguments->sort; -k dedupe forces 1),
+ * forward CSR, then the transpose -- the sorts run on the GPU. */
+struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments);
+void graphCSRFree(struct GraphCSR *graphCSR);
+
+/* pageRank.c:47-88, 177-368 */
+struct PageRankStats *newPageRankStatsGraphCSR(struct GraphCSR *graph);
+void freePageRankStats(struct PageRankStats *stats);
+struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+
+/* SPMV.c:27-215 */
+struct SPMVStats *newSPMVStatsGraphCSR(struct GraphCSR *graph);
+void freeSPMVStats(struct SPMVStats *stats);
+struct SPMVStats *SPMVGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct SPMVStats *SPMVPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+
+/* Backend housekeeping (no reference counterpart): release the process-wide
+ * HIP context and every cached device graph. */
+void graphbloxHipShutdown(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRAPHBLOX_HIP_H */

@@ -162,3 +162,49 @@ def test_pr_scale_20_against_oracle_f32(gpu_ctx):
     e = l1(ranks, ref["ranks"])
     print(f"[gpu] s20 L1 vs oracle f32 {e:.3e}")
     assert e <= TOL_F32
+
+
+@pytest.mark.parametrize("lds_bytes", ["0", "4096", "65536"])
+def test_lds_hot_prefix_is_bit_identical(gpu_ctx, golden, monkeypatch, lds_bytes):
+    """The LDS copy of the hot contributions holds the same values, so the
+    ranks must not change in any bit."""
+    z = golden("rmat_s12")
+    g = _graph(gpu_ctx, z)
+    monkeypatch.setenv("GBX_LDS_BYTES", "0")
+    r0, _ = g.pagerank(iterations=20, epsilon=0.0)
+    monkeypatch.setenv("GBX_LDS_BYTES", lds_bytes)
+    r1, _ = g.pagerank(iterations=20, epsilon=0.0)
+    np.testing.assert_array_equal(r0, r1)
+    r2, _ = g.pagerank(iterations=20, epsilon=0.0, precision=gbx.BF16)
+    monkeypatch.setenv("GBX_LDS_BYTES", "0")
+    r3, _ = g.pagerank(iterations=20, epsilon=0.0, precision=gbx.BF16)
+    np.testing.assert_array_equal(r2, r3)
+
+
+@pytest.mark.parametrize("name", ["rmat_s12", "fixture_v8192_e32768", "fixture_v1000_e10000"])
+def test_bf16_gather_error_bound(gpu_ctx, golden, name):
+    """BF16 contribution gathers with Kahan FP32 row sums: L1 <= 2e-3 vs FP64."""
+    z = golden(name)
+    g = _graph(gpu_ctx, z)
+    fwd, inv = _oracle_csr(z)
+    rp = O.row_ptr_from(inv)
+    rs, rep = g.pagerank(iterations=200, epsilon=1e-7, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_L1,
+                         precision=gbx.BF16)
+    r64, _, _ = O.pr_f64(fwd.N, fwd.out_degree, rp, inv.dst, mode=1, iterations=200, epsilon=1e-9)
+    e = l1(rs, r64)
+    rel = float(np.max(np.abs(rs - r64) / r64))
+    print(f"[gpu] bf16 stochastic {name}: L1 vs FP64 {e:.3e}, max rel {rel:.3e}, iters {rep['executed']}")
+    assert e <= 2e-3
+    assert abs(rep["rank_sum"] - 1.0) < 1e-5
+    rr, _ = g.pagerank(iterations=20, epsilon=0.0, precision=gbx.BF16)
+    assert l1(rr, z["pr_r20"]) <= 2e-3
+
+
+def test_bf16_scale_20_error(gpu_ctx):
+    g = gpu_ctx.rmat_graph(20, 16, 2020)
+    r32, _ = g.pagerank(iterations=100, epsilon=1e-7, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_L1)
+    r16, _ = g.pagerank(iterations=100, epsilon=1e-7, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_L1,
+                        precision=gbx.BF16)
+    e = l1(r16, r32)
+    print(f"[gpu] s20 bf16 vs f32 L1 {e:.3e}")
+    assert e <= 2e-3
