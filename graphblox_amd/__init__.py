@@ -86,10 +86,12 @@ def lib():
         "gbx_pr_reset": (i32, [_vp, _vp]),
         "gbx_pr_launch": (i32, [_vp, u32]),
         "gbx_pr_launch_timed": (i32, [_vp, u32, C.POINTER(f64)]),
-        "gbx_pr_read": (i32, [_vp, _vp, C.POINTER(PrReport)]),
+        "gbx_pr_read": (i32, [_vp, _vp, _vp]),
         "gbx_pr_free": (i32, [_vp]),
         "gbx_pr_plan_info": (i32, [_vp, C.POINTER(u32), _vp, _vp, _vp]),
         "gbx_spmv_pull": (i32, [_vp, _vp, u32, i32, i32, _vp, _vp, _vp, C.POINTER(f64)]),
+        "gbx_ppr_create": (i32, [_vp, _vp, C.POINTER(PrParams), u32, _vp, _vp, C.POINTER(_vp)]),
+        "gbx_ppr_info": (i32, [_vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -107,6 +109,7 @@ def exported_symbols():
         "gbx_rmat_edges", "gbx_graph_from_csr", "gbx_graph_info", "gbx_graph_export", "gbx_graph_free",
         "gbx_pr_params_default", "gbx_pr_pull", "gbx_pr_create", "gbx_pr_reset", "gbx_pr_launch",
         "gbx_pr_launch_timed", "gbx_pr_read", "gbx_pr_free", "gbx_pr_plan_info", "gbx_spmv_pull",
+        "gbx_ppr_create", "gbx_ppr_info",
     ]
 
 
@@ -266,6 +269,13 @@ class Graph:
     def session(self, **kw) -> "PageRank":
         return PageRank(self, params(**kw))
 
+    def ppr_session(self, num_vectors, seed_sets=None, iterations=100, epsilon=1e-7, stop=STOP_L1, damp=0.85,
+                    direction=PULL_IN) -> "PageRank":
+        """Batched personalised PageRank: seed_sets is a list of vertex-id
+        arrays (one per vector) or None for uniform teleports."""
+        p = params(iterations, epsilon, PR_STOCHASTIC, stop, damp, direction)
+        return PageRank(self, p, num_vectors=num_vectors, seed_sets=seed_sets)
+
 
 def params(iterations=20, epsilon=1e-4, mode=PR_REF, stop=STOP_ACTIVE, damp=0.85, direction=PULL_IN,
            weighted=False, precision=F32) -> PrParams:
@@ -285,11 +295,24 @@ def params(iterations=20, epsilon=1e-4, mode=PR_REF, stop=STOP_ACTIVE, damp=0.85
 class PageRank:
     """Step-wise gbx_pr object (create / reset / launch / read)."""
 
-    def __init__(self, graph: Graph, p: PrParams):
+    def __init__(self, graph: Graph, p: PrParams, num_vectors: int = 0, seed_sets=None):
         self.g = graph
         self.ctx = graph.ctx
+        self.k = num_vectors
         h = _vp()
-        self.ctx.check(self.ctx.L.gbx_pr_create(self.ctx.h, graph.h, C.byref(p), C.byref(h)))
+        if num_vectors:
+            if seed_sets is not None:
+                sizes = [len(s) for s in seed_sets]
+                seed_ptr = np.zeros(num_vectors + 1, np.uint32)
+                seed_ptr[1:] = np.cumsum(sizes)
+                seeds = _u32(np.concatenate([np.asarray(s, np.uint32) for s in seed_sets]))
+                self.ctx.check(self.ctx.L.gbx_ppr_create(self.ctx.h, graph.h, C.byref(p), num_vectors,
+                                                         _ptr(seed_ptr), _ptr(seeds), C.byref(h)))
+            else:
+                self.ctx.check(self.ctx.L.gbx_ppr_create(self.ctx.h, graph.h, C.byref(p), num_vectors, None,
+                                                         None, C.byref(h)))
+        else:
+            self.ctx.check(self.ctx.L.gbx_pr_create(self.ctx.h, graph.h, C.byref(p), C.byref(h)))
         self.h = h
         self.ctx._adopt(self)
 
@@ -306,10 +329,20 @@ class PageRank:
         return ms.value
 
     def read(self):
+        if self.k:
+            ranks = np.empty((self.g.N, self.k), np.float32)
+            reps = (PrReport * self.k)()
+            self.ctx.check(self.ctx.L.gbx_pr_read(self.h, _ptr(ranks), C.cast(reps, _vp)))
+            return ranks, [r.as_dict() for r in reps]
         ranks = np.empty(self.g.N, np.float32)
         rep = PrReport()
-        self.ctx.check(self.ctx.L.gbx_pr_read(self.h, _ptr(ranks), C.byref(rep)))
+        self.ctx.check(self.ctx.L.gbx_pr_read(self.h, _ptr(ranks), C.cast(C.pointer(rep), _vp)))
         return ranks, rep.as_dict()
+
+    def ppr_info(self):
+        k, v, h, c = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self.ctx.check(self.ctx.L.gbx_ppr_info(self.h, C.byref(k), C.byref(v), C.byref(h), C.byref(c)))
+        return dict(num_vectors=k.value, vector_width=v.value, hub_rows=h.value, hub_chunks=c.value)
 
     def plan_info(self):
         n = C.c_uint32()

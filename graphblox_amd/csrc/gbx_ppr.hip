@@ -1,0 +1,567 @@
+// gbx_ppr.hip -- batched personalised PageRank: k teleport vectors advanced
+// together by one multi-vector pull launch per iteration.
+//
+// No reference counterpart; the single-vector semantics are those of the
+// stochastic form in gbx_pr.hip (uniform dangling redistribution, FP64 mass
+// and residual reductions), applied to each vector independently:
+//   r_q' = (1-d) t_q + d (P^T r_q + D_q/N) / S_q
+// with t_q uniform over seed set q, D_q the dangling mass and S_q the rank
+// sum of vector q.  Ranks and contributions are stored vertex-major,
+// [vertex][V] with V = k rounded up to a power of two, so one gather of a
+// source vertex reads all k contributions from one or two cache lines.
+// A team of V lanes (lane = vector) walks one row; hub rows are split into
+// edge chunks whose per-team partial rows are reduced in LDS and combined in
+// chunk order by the last chunk to finish.  Per-vector FP64 block partials
+// are reduced by a second small kernel that also evaluates the stop rule.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+
+#include "gbx_plan.hpp"
+
+namespace gbx {
+
+constexpr uint32_t kMaxVec = 64;
+
+struct BScal {
+    double err, dang, sum;
+};
+
+struct BState {
+    uint32_t done;    // every vector met the stop rule
+    uint32_t iters;   // iterations executed
+    uint32_t last;    // scalar slot written by the last executed iteration
+    uint32_t pad;
+    uint32_t conv_iter[kMaxVec];  // iteration at which vector q first met the rule (0: not yet)
+};
+
+struct BPlan {
+    const uint32_t *__restrict__ rp;
+    const uint32_t *__restrict__ col;
+    const uint32_t *__restrict__ outdeg;
+    const HubChunk *__restrict__ chunks;
+    uint32_t *__restrict__ hub_cnt;
+    float *__restrict__ hub_part;  // [n_chunks][V]
+    uint32_t N, n_chunks, row_begin, n_tiles;
+};
+
+struct BatchView {
+    float *__restrict__ R;           // [N][V] ranks, plan order, updated in place
+    const float *__restrict__ C_in;  // [N][V] contributions of the previous iteration
+    float *__restrict__ C_out;
+    const uint64_t *__restrict__ mask;  // [N] bit q: vertex in seed set q (nullable: uniform)
+    const float *__restrict__ inv_size; // [V]
+    const BScal *__restrict__ scal_in;  // [V]
+    double *__restrict__ partials;      // [grid][3][V]
+    BState *state;
+    float damp;
+};
+
+}  // namespace gbx
+
+struct gbx_ppr_state {
+    uint32_t k = 0, V = 0;
+    uint32_t hub_threshold = 0, hub_chunk = 0, n_chunks = 0, n_hub_rows = 0, n_tiles = 0;
+    uint32_t grid = 0;
+    gbx::DevBuf R, C[2], mask, inv_size, scal, partials, state, counter, chunks, hub_cnt, hub_part, out;
+    bool has_mask = false;
+};
+
+namespace gbx {
+
+template <int V>
+__device__ __forceinline__ float team_row_sum(uint32_t j, uint32_t end, uint32_t step,
+                                              const uint32_t *__restrict__ col,
+                                              const float *__restrict__ C, uint32_t q)
+{
+    constexpr uint32_t U = GBX_UNROLL;
+    float acc = 0.0f;
+    for (; j + (U - 1) * step < end; j += U * step) {
+        uint32_t c[U];
+        float x[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) c[k] = col[j + k * step];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) x[k] = C[(size_t)c[k] * V + q];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) acc += x[k];
+    }
+    for (; j < end; j += step) acc += C[(size_t)col[j] * V + q];
+    return acc;
+}
+
+template <int V>
+__global__ void __launch_bounds__(kBlock) ppr_iter_kernel(BPlan P, BatchView A)
+{
+    if (A.state->done) return;
+    constexpr uint32_t TPB = kBlock / V;  // teams per block
+    const uint32_t q = threadIdx.x % V, team = threadIdx.x / V;
+    const BScal sc = A.scal_in[q];
+    const double S = sc.sum > 0.0 ? sc.sum : 1.0;
+    const float fa = 1.0f - A.damp;
+    const float fs = sc.sum > 0.0 ? (float)((double)A.damp / S) : 0.0f;
+    const float fd = (float)(sc.dang / (double)P.N);
+    const float uni = (float)(1.0 / (double)P.N);
+    const float isz = A.mask ? A.inv_size[q] : uni;
+    double perr = 0.0, pdang = 0.0, psum = 0.0;
+    auto epi = [&](uint32_t i, float acc) {
+        const size_t o = (size_t)i * V + q;
+        const float r_old = A.R[o];
+        const float t = A.mask ? (((A.mask[i] >> q) & 1ull) ? isz : 0.0f) : uni;
+        const float r_new = fa * t + fs * (acc + fd);
+        A.R[o] = r_new;
+        const uint32_t od = P.outdeg[i];
+        A.C_out[o] = od ? r_new / (float)od : 0.0f;
+        perr += (double)fabsf(r_new - r_old);
+        if (!od) pdang += (double)r_new;
+        psum += (double)r_new;
+    };
+    __shared__ float red[kBlock];
+    __shared__ bool last_chunk;
+    for (uint32_t tile = blockIdx.x; tile < P.n_tiles; tile += gridDim.x) {
+        if (tile < P.n_chunks) {
+            const HubChunk hc = P.chunks[tile];
+            red[threadIdx.x] = team_row_sum<V>(hc.ebeg + team, hc.eend, TPB, P.col, A.C_in, q);
+            __syncthreads();
+            if (team == 0) {
+                float s = red[q];
+                for (uint32_t t2 = 1; t2 < TPB; t2++) s += red[t2 * V + q];
+                P.hub_part[(size_t)tile * V + q] = s;
+                __threadfence();
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const uint32_t prev = atomicAdd(&P.hub_cnt[hc.hub], 1u);
+                last_chunk = prev == hc.nch - 1;
+            }
+            __syncthreads();
+            if (last_chunk && team == 0) {
+                __threadfence();
+                const volatile float *hp = P.hub_part;
+                float tot = 0.0f;
+                for (uint32_t k = 0; k < hc.nch; k++) tot += hp[(size_t)(hc.first + k) * V + q];
+                epi(hc.row, tot);
+                if (q == 0) P.hub_cnt[hc.hub] = 0;
+            }
+            __syncthreads();
+        } else {
+            const uint32_t row = P.row_begin + (tile - P.n_chunks) * TPB + team;
+            if (row < P.N) {
+                const uint32_t beg = P.rp[row], end = P.rp[row + 1];
+                epi(row, team_row_sum<V>(beg, end, 1, P.col, A.C_in, q));
+            }
+        }
+    }
+    // per-vector block partials: reduce over teams through LDS, in team order
+    __shared__ double pr[3][kBlock];
+    pr[0][threadIdx.x] = perr;
+    pr[1][threadIdx.x] = pdang;
+    pr[2][threadIdx.x] = psum;
+    __syncthreads();
+    if (team == 0) {
+        double e = pr[0][q], d = pr[1][q], s = pr[2][q];
+        for (uint32_t t2 = 1; t2 < TPB; t2++) {
+            e += pr[0][t2 * V + q];
+            d += pr[1][t2 * V + q];
+            s += pr[2][t2 * V + q];
+        }
+        double *out = A.partials + (size_t)blockIdx.x * 3 * V;
+        out[q] = e;
+        out[V + q] = d;
+        out[2 * V + q] = s;
+    }
+}
+
+// Initial ranks (teleport vectors) and contributions; block partials of the
+// rank sum and dangling mass.
+template <int V>
+__global__ void __launch_bounds__(kBlock) ppr_reset_kernel(BPlan P, BatchView A)
+{
+    constexpr uint32_t TPB = kBlock / V;
+    const uint32_t q = threadIdx.x % V, team = threadIdx.x / V;
+    const float uni = (float)(1.0 / (double)P.N);
+    const float isz = A.mask ? A.inv_size[q] : uni;
+    double pdang = 0.0, psum = 0.0;
+    for (uint32_t i = blockIdx.x * TPB + team; i < P.N; i += gridDim.x * TPB) {
+        const size_t o = (size_t)i * V + q;
+        const float t = A.mask ? (((A.mask[i] >> q) & 1ull) ? isz : 0.0f) : uni;
+        A.R[o] = t;
+        const uint32_t od = P.outdeg[i];
+        A.C_out[o] = od ? t / (float)od : 0.0f;
+        if (!od) pdang += (double)t;
+        psum += (double)t;
+    }
+    __shared__ double pr[2][kBlock];
+    pr[0][threadIdx.x] = pdang;
+    pr[1][threadIdx.x] = psum;
+    __syncthreads();
+    if (team == 0) {
+        double d = pr[0][q], s = pr[1][q];
+        for (uint32_t t2 = 1; t2 < TPB; t2++) {
+            d += pr[0][t2 * V + q];
+            s += pr[1][t2 * V + q];
+        }
+        double *out = A.partials + (size_t)blockIdx.x * 3 * V;
+        out[q] = 0.0;
+        out[V + q] = d;
+        out[2 * V + q] = s;
+    }
+}
+
+// One block per vector: sum the block partials in block order into
+// scal_out[q]; the last block evaluates the stop rule over the k vectors.
+__global__ void __launch_bounds__(256)
+ppr_reduce_kernel(const double *__restrict__ partials, uint32_t nblocks, uint32_t V, uint32_t k,
+                  BScal *__restrict__ scal_out, uint32_t slot, BState *state, uint32_t *counter,
+                  int stop, double eps, int is_reset)
+{
+    if (!is_reset && state->done) return;
+    const uint32_t q = blockIdx.x;
+    double e = 0.0, d = 0.0, s = 0.0;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += 256) {
+        const double *p = partials + (size_t)b * 3 * V;
+        e += p[q];
+        d += p[V + q];
+        s += p[2 * V + q];
+    }
+    __shared__ double red[3][256];
+    red[0][threadIdx.x] = e;
+    red[1][threadIdx.x] = d;
+    red[2][threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+            red[2][threadIdx.x] += red[2][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        scal_out[q] = BScal{red[0][0], red[1][0], red[2][0]};
+        __threadfence();
+        last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    __threadfence();
+    *counter = 0;
+    if (is_reset) {
+        state->done = 0;
+        state->iters = 0;
+        state->last = slot;
+        for (uint32_t v = 0; v < kMaxVec; v++) state->conv_iter[v] = 0;
+        return;
+    }
+    const uint32_t it = state->iters + 1;
+    bool all = true;
+    const volatile BScal *vs = scal_out;
+    for (uint32_t v = 0; v < k; v++) {
+        const bool ok = stop == GBX_STOP_L1 && vs[v].err < eps;
+        if (ok && state->conv_iter[v] == 0) state->conv_iter[v] = it;
+        all = all && ok;
+    }
+    state->iters = it;
+    state->last = slot;
+    if (stop != GBX_STOP_NONE && all) state->done = 1;
+}
+
+__global__ void ppr_finalize_kernel(const float *__restrict__ R, const uint32_t *__restrict__ perm,
+                                    uint32_t N, uint32_t V, uint32_t k, const BScal *__restrict__ scal,
+                                    float *__restrict__ out)
+{
+    const size_t total = (size_t)N * k;
+    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < total;
+         x += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t i = (uint32_t)(x / k), q = (uint32_t)(x % k);
+        const double S = scal[q].sum > 0.0 ? scal[q].sum : 1.0;
+        out[(size_t)perm[i] * k + q] = R[(size_t)i * V + q] / (float)S;
+    }
+}
+
+// seed-set membership bits in plan order
+__global__ void ppr_mask_kernel(const uint32_t *__restrict__ seeds, const uint32_t *__restrict__ owner,
+                                uint32_t nseeds, const uint32_t *__restrict__ inv, uint32_t N,
+                                unsigned long long *__restrict__ mask)
+{
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < nseeds; x += gridDim.x * blockDim.x) {
+        const uint32_t v = seeds[x];
+        if (v < N) atomicOr(&mask[inv[v]], 1ull << owner[x]);
+    }
+}
+
+static BPlan bplan(const gbx_pr *pr)
+{
+    const gbx_plan *p = pr->plan;
+    const gbx_ppr_state *b = pr->batch;
+    BPlan v;
+    v.rp = p->rp.as<uint32_t>();
+    v.col = p->col.as<uint32_t>();
+    v.outdeg = p->outdeg.as<uint32_t>();
+    v.chunks = b->chunks.as<HubChunk>();
+    v.hub_cnt = b->hub_cnt.as<uint32_t>();
+    v.hub_part = b->hub_part.as<float>();
+    v.N = p->N;
+    v.n_chunks = b->n_chunks;
+    v.row_begin = b->n_hub_rows;
+    v.n_tiles = b->n_tiles;
+    return v;
+}
+
+static BatchView bview(const gbx_pr *pr, uint32_t t)
+{
+    const gbx_ppr_state *b = pr->batch;
+    BatchView a;
+    a.R = b->R.as<float>();
+    a.C_in = b->C[t & 1].as<float>();
+    a.C_out = b->C[(t + 1) & 1].as<float>();
+    a.mask = b->has_mask ? b->mask.as<uint64_t>() : nullptr;
+    a.inv_size = b->inv_size.as<float>();
+    a.scal_in = b->scal.as<BScal>() + (size_t)(t & 1) * kMaxVec;
+    a.partials = b->partials.as<double>();
+    a.state = b->state.as<BState>();
+    a.damp = pr->p.damp;
+    return a;
+}
+
+template <int V>
+static void launch_v(bool reset, unsigned grid, hipStream_t st, const BPlan &P, const BatchView &A)
+{
+    if (reset) ppr_reset_kernel<V><<<grid, kBlock, 0, st>>>(P, A);
+    else ppr_iter_kernel<V><<<grid, kBlock, 0, st>>>(P, A);
+}
+
+static int launch_any(gbx_pr *pr, bool reset)
+{
+    gbx_ctx *ctx = pr->ctx;
+    gbx_ppr_state *b = pr->batch;
+    const uint32_t t = reset ? 1u : pr->t;  // reset writes C[0] (= C_out of t=1... see below)
+    BPlan P = bplan(pr);
+    BatchView A = bview(pr, t);
+    if (reset) A.C_out = b->C[0].as<float>();
+    const unsigned grid = b->grid;
+    hipStream_t st = ctx->stream;
+    switch (b->V) {
+    case 1: launch_v<1>(reset, grid, st, P, A); break;
+    case 2: launch_v<2>(reset, grid, st, P, A); break;
+    case 4: launch_v<4>(reset, grid, st, P, A); break;
+    case 8: launch_v<8>(reset, grid, st, P, A); break;
+    case 16: launch_v<16>(reset, grid, st, P, A); break;
+    case 32: launch_v<32>(reset, grid, st, P, A); break;
+    default: launch_v<64>(reset, grid, st, P, A); break;
+    }
+    GBX_HIP(ctx, hipGetLastError());
+    const uint32_t slot = reset ? 0u : ((pr->t + 1) & 1);
+    ppr_reduce_kernel<<<b->V, 256, 0, st>>>(b->partials.as<double>(), grid, b->V, b->k,
+                                            b->scal.as<BScal>() + (size_t)slot * kMaxVec, slot,
+                                            b->state.as<BState>(), b->counter.as<uint32_t>(), pr->p.stop,
+                                            pr->p.epsilon, reset ? 1 : 0);
+    GBX_HIP(ctx, hipGetLastError());
+    return GBX_OK;
+}
+
+int ppr_reset(gbx_pr *pr)
+{
+    pr->t = 0;
+    return launch_any(pr, true);
+}
+
+int ppr_launch_iter(gbx_pr *pr)
+{
+    GBX_TRY(launch_any(pr, false));
+    pr->t++;
+    return GBX_OK;
+}
+
+int ppr_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep)
+{
+    gbx_ctx *ctx = pr->ctx;
+    gbx_ppr_state *b = pr->batch;
+    BState hs;
+    GBX_HIP(ctx, hipMemcpyAsync(&hs, b->state.p, sizeof(BState), hipMemcpyDeviceToHost, ctx->stream));
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const BScal *sc = b->scal.as<BScal>() + (size_t)hs.last * kMaxVec;
+    const uint32_t N = pr->plan->N;
+    ppr_finalize_kernel<<<grid_for((size_t)N * b->k, 256, 16384), 256, 0, ctx->stream>>>(
+        b->R.as<float>(), pr->plan->perm.as<uint32_t>(), N, b->V, b->k, sc, b->out.as<float>());
+    GBX_HIP(ctx, hipGetLastError());
+    std::vector<BScal> hsc(b->k);
+    GBX_HIP(ctx, hipMemcpyAsync(hsc.data(), sc, b->k * sizeof(BScal), hipMemcpyDeviceToHost, ctx->stream));
+    if (ranks_out)
+        GBX_HIP(ctx, hipMemcpyAsync(ranks_out, b->out.p, (size_t)N * b->k * 4, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (rep) {
+        // one report per vector
+        for (uint32_t q = 0; q < b->k; q++) {
+            gbx_pr_report &r = rep[q];
+            r.executed = hs.iters;
+            r.converged = hs.conv_iter[q] != 0;
+            r.iterations = r.converged ? hs.conv_iter[q] - 1 : hs.iters;
+            r.active = 0;
+            r.error_total = hsc[q].err;
+            double s = 0.0;
+            if (ranks_out)
+                for (uint32_t i = 0; i < N; i++) s += ranks_out[(size_t)i * b->k + q];
+            r.rank_sum = s;
+            r.time_total_s = pr->elapsed_ms * 1e-3;
+        }
+    }
+    return GBX_OK;
+}
+
+void ppr_state_free(gbx_ppr_state *s) { delete s; }
+
+// Vector width, hub threshold and chunk size: GBX_PPR_HUB (edges, default
+// 1024) and GBX_PPR_CHUNK (edges per hub chunk, default 2048).
+static int build_batch(gbx_pr *pr, uint32_t k, const uint32_t *seed_ptr, const uint32_t *seeds)
+{
+    gbx_ctx *ctx = pr->ctx;
+    gbx_plan *p = pr->plan;
+    gbx_ppr_state *b = pr->batch;
+    b->k = k;
+    uint32_t V = 1;
+    while (V < k) V <<= 1;
+    b->V = V;
+    b->hub_threshold = 1024;
+    b->hub_chunk = 2048;
+    if (const char *s = getenv("GBX_PPR_HUB")) b->hub_threshold = std::max(64u, (uint32_t)atoi(s));
+    if (const char *s = getenv("GBX_PPR_CHUNK")) b->hub_chunk = std::max(256u, (uint32_t)atoi(s));
+    GBX_TRY(plan_rows_at_least(ctx, p, b->hub_threshold, &b->n_hub_rows));
+    std::vector<uint32_t> hrp;
+    GBX_TRY(plan_row_ptr_prefix(ctx, p, b->n_hub_rows, hrp));
+    std::vector<HubChunk> chunks;
+    for (uint32_t h = 0; h < b->n_hub_rows; h++) {
+        const uint32_t e0 = hrp[h], e1 = hrp[h + 1];
+        const uint32_t nch = (e1 - e0 + b->hub_chunk - 1) / b->hub_chunk;
+        const uint32_t first = (uint32_t)chunks.size();
+        for (uint32_t c = 0; c < nch; c++) {
+            HubChunk hc;
+            hc.row = h;
+            hc.ebeg = e0 + c * b->hub_chunk;
+            hc.eend = std::min(e1, hc.ebeg + b->hub_chunk);
+            hc.hub = h;
+            hc.first = first;
+            hc.nch = nch;
+            chunks.push_back(hc);
+        }
+    }
+    b->n_chunks = (uint32_t)chunks.size();
+    const uint32_t TPB = kBlock / V;
+    b->n_tiles = b->n_chunks + (p->N - b->n_hub_rows + TPB - 1) / TPB;
+    unsigned per_cu = 4;
+    if (const char *s = getenv("GBX_BLOCKS_PER_CU")) {
+        int v = atoi(s);
+        if (v >= 1 && v <= 64) per_cu = (unsigned)v;
+    }
+    b->grid = std::max(1u, std::min((unsigned)std::max(1, ctx->num_cus) * per_cu, b->n_tiles));
+    const size_t nv = (size_t)p->N * V * 4;
+    GBX_TRY(dev_alloc(ctx, b->R, nv));
+    GBX_TRY(dev_alloc(ctx, b->C[0], nv));
+    GBX_TRY(dev_alloc(ctx, b->C[1], nv));
+    GBX_TRY(dev_alloc(ctx, b->out, (size_t)p->N * k * 4));
+    GBX_TRY(dev_alloc(ctx, b->scal, 2 * kMaxVec * sizeof(BScal)));
+    GBX_TRY(dev_alloc(ctx, b->partials, (size_t)std::max(b->grid, 1u) * 3 * V * sizeof(double)));
+    GBX_TRY(dev_alloc(ctx, b->state, sizeof(BState)));
+    GBX_TRY(dev_alloc(ctx, b->counter, 16));
+    GBX_TRY(fill_u32(ctx, b->counter.as<uint32_t>(), 0, 4));
+    GBX_TRY(dev_alloc(ctx, b->chunks, chunks.size() * sizeof(HubChunk) + 16));
+    GBX_TRY(dev_alloc(ctx, b->hub_cnt, (size_t)b->n_hub_rows * 4 + 4));
+    GBX_TRY(dev_alloc(ctx, b->hub_part, chunks.size() * V * 4 + 16));
+    GBX_TRY(fill_u32(ctx, b->hub_cnt.as<uint32_t>(), 0, b->n_hub_rows + 1));
+    if (!chunks.empty())
+        GBX_HIP(ctx, hipMemcpyAsync(b->chunks.p, chunks.data(), chunks.size() * sizeof(HubChunk),
+                                    hipMemcpyHostToDevice, ctx->stream));
+    // teleport vectors
+    std::vector<float> isz(V, 0.0f);
+    GBX_TRY(dev_alloc(ctx, b->inv_size, V * 4));
+    b->has_mask = seed_ptr != nullptr;
+    if (b->has_mask) {
+        const uint32_t ns = seed_ptr[k];
+        std::vector<uint32_t> owner(ns);
+        for (uint32_t q = 0; q < k; q++) {
+            if (seed_ptr[q + 1] < seed_ptr[q]) return set_error(ctx, GBX_ERR_ARG, "seed_ptr not monotone");
+            const uint32_t n = seed_ptr[q + 1] - seed_ptr[q];
+            if (n == 0) return set_error(ctx, GBX_ERR_ARG, "seed set %u is empty", q);
+            isz[q] = (float)(1.0 / (double)n);
+            for (uint32_t x = seed_ptr[q]; x < seed_ptr[q + 1]; x++) {
+                if (seeds[x] >= p->N) return set_error(ctx, GBX_ERR_RANGE, "seed %u >= N", seeds[x]);
+                owner[x] = q;
+            }
+        }
+        GBX_TRY(dev_alloc(ctx, b->mask, (size_t)p->N * 8));
+        GBX_TRY(fill_u32(ctx, b->mask.as<uint32_t>(), 0, (size_t)p->N * 2));
+        DevBuf ds, dow;
+        GBX_TRY(dev_alloc(ctx, ds, (size_t)ns * 4 + 4));
+        GBX_TRY(dev_alloc(ctx, dow, (size_t)ns * 4 + 4));
+        if (ns) {
+            GBX_HIP(ctx, hipMemcpyAsync(ds.p, seeds, (size_t)ns * 4, hipMemcpyHostToDevice, ctx->stream));
+            GBX_HIP(ctx, hipMemcpyAsync(dow.p, owner.data(), (size_t)ns * 4, hipMemcpyHostToDevice, ctx->stream));
+            ppr_mask_kernel<<<grid_for(ns, 256, 4096), 256, 0, ctx->stream>>>(
+                ds.as<uint32_t>(), dow.as<uint32_t>(), ns, p->inv.as<uint32_t>(), p->N,
+                reinterpret_cast<unsigned long long *>(b->mask.p));
+            GBX_HIP(ctx, hipGetLastError());
+        }
+    }
+    GBX_HIP(ctx, hipMemcpyAsync(b->inv_size.p, isz.data(), V * 4, hipMemcpyHostToDevice, ctx->stream));
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GBX_OK;
+}
+
+}  // namespace gbx
+
+using namespace gbx;
+
+extern "C" {
+
+int gbx_ppr_create(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, uint32_t num_vectors,
+                   const uint32_t *seed_ptr, const uint32_t *seeds, gbx_pr **out)
+{
+    if (!ctx || !g || !p || !out) return set_error(ctx, GBX_ERR_ARG, "gbx_ppr_create: NULL argument");
+    *out = nullptr;
+    if (num_vectors == 0 || num_vectors > kMaxVec)
+        return set_error(ctx, GBX_ERR_ARG, "gbx_ppr_create: num_vectors %u not in 1..%u", num_vectors, kMaxVec);
+    if (seed_ptr && !seeds && seed_ptr[num_vectors] > 0)
+        return set_error(ctx, GBX_ERR_ARG, "gbx_ppr_create: seeds is NULL");
+    if (p->mode != GBX_PR_STOCHASTIC || p->precision != GBX_F32 || p->weighted || p->stop == GBX_STOP_ACTIVE)
+        return set_error(ctx, GBX_ERR_ARG,
+                         "gbx_ppr_create: batched form is STOCHASTIC, F32, unweighted, stop L1 or NONE");
+    if (!(p->damp >= 0.0f && p->damp < 1.0f))
+        return set_error(ctx, GBX_ERR_ARG, "gbx_ppr_create: damp %f not in [0,1)", (double)p->damp);
+    GBX_HIP(ctx, hipSetDevice(ctx->device));
+    gbx_plan *plan = nullptr;
+    GBX_TRY(ensure_plan(ctx, g, p->direction, &plan));
+    gbx_pr *pr = new (std::nothrow) gbx_pr();
+    if (!pr) return set_error(ctx, GBX_ERR_NOMEM, "gbx_ppr_create: out of host memory");
+    pr->ctx = ctx;
+    pr->g = g;
+    pr->plan = plan;
+    pr->p = *p;
+    pr->kind = K_PR_STOCH;
+    pr->batch = new (std::nothrow) gbx_ppr_state();
+    int rc = pr->batch ? GBX_OK : set_error(ctx, GBX_ERR_NOMEM, "gbx_ppr_create: out of host memory");
+    if (!rc) rc = build_batch(pr, num_vectors, seed_ptr, seeds);
+    if (!rc && (hipEventCreate(&pr->ev_begin) != hipSuccess || hipEventCreate(&pr->ev_end) != hipSuccess))
+        rc = set_error(ctx, GBX_ERR_HIP, "gbx_ppr_create: event creation");
+    if (rc) {
+        gbx_pr_free(pr);
+        return rc;
+    }
+    *out = pr;
+    return GBX_OK;
+}
+
+int gbx_ppr_info(const gbx_pr *pr, uint32_t *num_vectors, uint32_t *vector_width, uint32_t *hub_rows,
+                 uint32_t *hub_chunks)
+{
+    if (!pr || !pr->batch) return set_error(nullptr, GBX_ERR_ARG, "gbx_ppr_info: not a batched object");
+    if (num_vectors) *num_vectors = pr->batch->k;
+    if (vector_width) *vector_width = pr->batch->V;
+    if (hub_rows) *hub_rows = pr->batch->n_hub_rows;
+    if (hub_chunks) *hub_chunks = pr->batch->n_chunks;
+    return GBX_OK;
+}
+
+}  // extern "C"

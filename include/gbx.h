@@ -196,6 +196,23 @@ int gbx_pr_launch_timed(gbx_pr *pr, uint32_t n_iters, double *kernel_ms_avg);
 int gbx_pr_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep);
 int gbx_pr_free(gbx_pr *pr);
 
+/* ---------------------------------------------------------------- batched */
+/* Personalised PageRank for num_vectors (1..64) teleport vectors at once
+ * (no reference counterpart).  Vector q is uniform over its seed set
+ * seeds[seed_ptr[q] .. seed_ptr[q+1]) (original vertex ids); seed_ptr NULL
+ * selects the uniform teleport for every vector.  Semantics per vector are
+ * GBX_PR_STOCHASTIC; p must ask for STOCHASTIC, F32, unweighted, and stop
+ * GBX_STOP_L1 (all vectors below epsilon) or GBX_STOP_NONE.  The returned
+ * object is driven with gbx_pr_reset (teleport NULL), gbx_pr_launch,
+ * gbx_pr_launch_timed and gbx_pr_read, where ranks_out is [N][num_vectors]
+ * (vertex-major, original order) and rep points to num_vectors reports. */
+int gbx_ppr_create(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, uint32_t num_vectors,
+                   const uint32_t *seed_ptr, const uint32_t *seeds, gbx_pr **out);
+/* Shape of a batched object: vectors, lanes per row (power of two >= k),
+ * hub rows and hub chunks of its tiling. */
+int gbx_ppr_info(const gbx_pr *pr, uint32_t *num_vectors, uint32_t *vector_width,
+                 uint32_t *hub_rows, uint32_t *hub_chunks);
+
 /* Plan statistics: counts of rows per degree class and edges per class
  * (for reporting and the autotuner); arrays of length *n_classes (<= 16). */
 int gbx_pr_plan_info(const gbx_pr *pr, uint32_t *n_classes, uint32_t *class_lanes,
