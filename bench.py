@@ -72,6 +72,37 @@ def cpu_baseline(g, iterations: int, threads: int):
                        f"R-MAT graph (N={N}, E={E}), {threads} OpenMP threads, whole loop timed")
 
 
+def time_form(g, warmup, steps, **kw):
+    """Warm up, then time `steps` iterations: per-iteration HIP-event kernel
+    mean and host wall time.  Returns (kernel_ms, wall_ms, session)."""
+    ctx = g.ctx
+    k = kw.pop("num_vectors", 0)
+    seeds = kw.pop("seed_sets", None)
+    if k:
+        pr = g.ppr_session(k, seeds, iterations=warmup + steps, epsilon=0.0, stop=2)
+    else:
+        pr = g.session(iterations=warmup + steps, epsilon=0.0, stop=2, **kw)
+    pr.reset()
+    pr.launch(warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    kernel_ms = pr.launch_timed(steps)
+    ctx.sync()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    return kernel_ms, wall_ms, pr
+
+
+def form_record(name, E, N, k, kernel_ms, wall_ms, bytes_per_iter, base_gteps=None):
+    gteps = k * E / (wall_ms * 1e-3) / 1e9
+    rec = {"form": name, "num_vectors": k, "num_vertices": N, "num_edges": E,
+           "ms_per_iter": round(wall_ms, 5), "kernel_ms": round(kernel_ms, 5),
+           "gteps": round(gteps, 2), "algorithmic_bytes_per_iter": bytes_per_iter,
+           "hbm_frac": round(bytes_per_iter / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if base_gteps:
+        rec["speedup_vs_fp32_same_graph"] = round(gteps / base_gteps, 3)
+    return rec
+
+
 def calibrate_copy(gib: float):
     """A plain streaming copy of known size: y = x * 1 over gib GiB of float32
     (reads gib GiB, writes gib GiB), three launches."""
@@ -96,6 +127,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=5)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the BF16 and batched forms")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="run only a plain device copy of this many GiB (reference for PMC byte counters)")
     args = ap.parse_args()
@@ -135,6 +167,35 @@ def main():
         f"{achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of peak)")
     for c in pr.plan_info():
         log(f"  bin lanes={c['lanes']:>3} rows={c['rows']:>9} edges={c['edges']:>10}")
+    forms = None
+    if not args.no_forms:
+        import graphblox_amd as gbx
+
+        forms = {}
+        km, wm, s16 = time_form(g, args.warmup, args.steps, mode=gbx.PR_REF, precision=gbx.BF16)
+        _, rep16 = s16.read()
+        s16.free()
+        forms["bf16_gather"] = form_record("BF16 gathers, FP32 Kahan rows, same graph", E, N, 1, km, wm,
+                                           6 * E + 18 * N + 4, value)
+        log(f"BF16 form: {forms['bf16_gather']}")
+        # 32 personalised vectors on R-MAT scale 22 (seed sets of 1..1000 vertices)
+        gb = ctx.rmat_graph(min(args.scale, 22), args.edge_factor, SEED)
+        Nb, Eb = gb.N, gb.E
+        km32, wm32, s32 = time_form(gb, args.warmup, args.steps, mode=gbx.PR_STOCHASTIC)
+        s32.free()
+        base32 = Eb / (wm32 * 1e-3) / 1e9
+        forms["fp32_same_graph_as_batched"] = form_record("plain FP32 (stochastic)", Eb, Nb, 1, km32, wm32,
+                                                          8 * Eb + 16 * Nb + 4)
+        rng = np.random.default_rng(SEED)
+        sets = [rng.choice(Nb, int(rng.integers(1, 1001)), replace=False) for _ in range(32)]
+        kb, wb, sb = time_form(gb, args.warmup, args.steps, num_vectors=32, seed_sets=sets)
+        rb, repb = sb.read()
+        assert np.isfinite(rb).all()
+        sb.free()
+        forms["batched_ppr32"] = form_record("32 personalised vectors, one multi-vector launch", Eb, Nb, 32, kb, wb,
+                                             4 * (Nb + 1) + 4 * Nb + 4 * Eb + 32 * (4 * Eb + 8 * Nb), base32)
+        log(f"batched form: {forms['batched_ppr32']}")
+        gb.free()
     cpu = None
     if not args.no_cpu:
         log("CPU baseline ...")
@@ -172,6 +233,7 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": cpu,
+        "forms": forms,
     }
     print(json.dumps(out), flush=True)
     pr.free()

@@ -6,12 +6,14 @@
 
 namespace gbx {
 
+// 1024-thread blocks, one per CU, each holding the hottest contributions in
+// (almost) all of the CU's 160 KiB of LDS.
 #ifndef GBX_BLOCK
-#define GBX_BLOCK 256
+#define GBX_BLOCK 1024
 #endif
 constexpr int kBlock = GBX_BLOCK;
 #ifndef GBX_LDS_DEFAULT
-#define GBX_LDS_DEFAULT 0
+#define GBX_LDS_DEFAULT 159744
 #endif
 constexpr int kMaxClasses = 9;
 

@@ -451,7 +451,7 @@ static int build_batch(gbx_pr *pr, uint32_t k, const uint32_t *seed_ptr, const u
     b->n_chunks = (uint32_t)chunks.size();
     const uint32_t TPB = kBlock / V;
     b->n_tiles = b->n_chunks + (p->N - b->n_hub_rows + TPB - 1) / TPB;
-    unsigned per_cu = 4;
+    unsigned per_cu = 2;
     if (const char *s = getenv("GBX_BLOCKS_PER_CU")) {
         int v = atoi(s);
         if (v >= 1 && v <= 64) per_cu = (unsigned)v;

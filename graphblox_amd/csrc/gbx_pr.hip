@@ -170,17 +170,22 @@ __device__ __forceinline__ void do_hub(uint32_t tile, const PlanView &P, const G
         Acc s = red[0];
 #pragma unroll
         for (int w = 1; w < kBlock / 64; w++) s.merge(red[w]);
-        P.hub_part[tile] = s.value();
-        __threadfence();
-        uint32_t prev = atomicAdd(&P.hub_cnt[hc.hub], 1u);
-        if (prev == hc.nch - 1) {
+        if (hc.nch == 1) {
+            // the whole row in one chunk: no cross-block combine, no fences
+            epi(hc.row, epi.init(hc.row) + s.value());
+        } else {
+            P.hub_part[tile] = s.value();
             __threadfence();
-            Acc tot;
-            tot.init(0.0f);
-            const volatile float *hp = P.hub_part;
-            for (uint32_t k = 0; k < hc.nch; k++) tot.add(hp[hc.first + k]);
-            P.hub_cnt[hc.hub] = 0;
-            epi(hc.row, epi.init(hc.row) + tot.value());
+            uint32_t prev = atomicAdd(&P.hub_cnt[hc.hub], 1u);
+            if (prev == hc.nch - 1) {
+                __threadfence();
+                Acc tot;
+                tot.init(0.0f);
+                const volatile float *hp = P.hub_part;
+                for (uint32_t k = 0; k < hc.nch; k++) tot.add(hp[hc.first + k]);
+                P.hub_cnt[hc.hub] = 0;
+                epi(hc.row, epi.init(hc.row) + tot.value());
+            }
         }
     }
     __syncthreads();
@@ -573,7 +578,7 @@ __global__ void fill_f32(float *p, float v, uint32_t n)
 // GBX_BIN_THRESHOLDS="hub,t64,t32,t16,t8,t4,t2" and GBX_HUB_CHUNK.
 struct BinConfig {
     uint32_t thr[7] = {8192, 256, 128, 64, 32, 16, 8};
-    uint32_t hub_chunk = 4096;
+    uint32_t hub_chunk = 32768;
 };
 
 static BinConfig bin_config()
@@ -825,7 +830,7 @@ static PlanView plan_view(const gbx_plan *p, const float *w)
 
 static unsigned iter_grid(gbx_ctx *ctx, const gbx_plan *p)
 {
-    unsigned per_cu = 4;
+    unsigned per_cu = 1;
     if (const char *s = getenv("GBX_BLOCKS_PER_CU")) {
         int v = atoi(s);
         if (v >= 1 && v <= 64) per_cu = (unsigned)v;
@@ -858,8 +863,17 @@ template <int KIND, class T, class Acc>
 static void launch_pr_kernel(bool lds, unsigned grid, size_t shm, hipStream_t st, const PlanView &P,
                              const IterView &A)
 {
-    if (lds) pr_iter_kernel<KIND, T, Acc, true><<<grid, kBlock, shm, st>>>(P, A);
-    else pr_iter_kernel<KIND, T, Acc, false><<<grid, kBlock, 0, st>>>(P, A);
+    if (lds) {
+        // more than 64 KiB of dynamic LDS needs the per-kernel opt-in
+        static size_t granted = 64 * 1024;
+        if (shm > granted &&
+            hipFuncSetAttribute(reinterpret_cast<const void *>(&pr_iter_kernel<KIND, T, Acc, true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) == hipSuccess)
+            granted = shm;
+        pr_iter_kernel<KIND, T, Acc, true><<<grid, kBlock, shm, st>>>(P, A);
+    } else {
+        pr_iter_kernel<KIND, T, Acc, false><<<grid, kBlock, 0, st>>>(P, A);
+    }
 }
 
 static const float *plan_weights(const gbx_pr *pr)
