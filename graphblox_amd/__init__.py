@@ -92,6 +92,9 @@ def lib():
         "gbx_spmv_pull": (i32, [_vp, _vp, u32, i32, i32, _vp, _vp, _vp, C.POINTER(f64)]),
         "gbx_ppr_create": (i32, [_vp, _vp, C.POINTER(PrParams), u32, _vp, _vp, C.POINTER(_vp)]),
         "gbx_ppr_info": (i32, [_vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
+        "gbx_pr_launch_graph": (i32, [_vp, u32]),
+        "gbx_graph_batch_create": (i32, [_vp, u32, _vp, C.POINTER(_vp)]),
+        "gbx_graph_batch_info": (i32, [_vp, C.POINTER(u32), _vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -109,7 +112,8 @@ def exported_symbols():
         "gbx_rmat_edges", "gbx_graph_from_csr", "gbx_graph_info", "gbx_graph_export", "gbx_graph_free",
         "gbx_pr_params_default", "gbx_pr_pull", "gbx_pr_create", "gbx_pr_reset", "gbx_pr_launch",
         "gbx_pr_launch_timed", "gbx_pr_read", "gbx_pr_free", "gbx_pr_plan_info", "gbx_spmv_pull",
-        "gbx_ppr_create", "gbx_ppr_info",
+        "gbx_ppr_create", "gbx_ppr_info", "gbx_pr_launch_graph", "gbx_graph_batch_create",
+        "gbx_graph_batch_info",
     ]
 
 
@@ -194,6 +198,15 @@ class Context:
         self.check(self.L.gbx_graph_from_csr(self.h, N, E, *(_ptr(a) for a in arrs), C.byref(h)))
         return Graph(self, h)
 
+    def graph_batch(self, graphs) -> "Graph":
+        """Block-diagonal batch of independent graphs (copied)."""
+        arr = (_vp * len(graphs))(*[g.h for g in graphs])
+        h = _vp()
+        self.check(self.L.gbx_graph_batch_create(self.h, len(graphs), C.cast(arr, _vp), C.byref(h)))
+        g = Graph(self, h)
+        g._batch = True
+        return g
+
     def rmat_graph(self, scale, edge_factor, seed, a=0.57, b=0.19, c=0.19, weight_kind=0, weight_seed=0,
                    sort_mode=SORT_RADIX_SRC_DST, flags=0) -> "Graph":
         h = _vp()
@@ -224,6 +237,13 @@ class Graph:
         n, e = C.c_uint32(), C.c_uint32()
         ctx.check(ctx.L.gbx_graph_info(h, C.byref(n), C.byref(e)))
         self.N, self.E = n.value, e.value
+
+    def batch_offsets(self):
+        n = C.c_uint32()
+        self.ctx.check(self.ctx.L.gbx_graph_batch_info(self.h, C.byref(n), None))
+        off = np.zeros(n.value + 1, np.uint32)
+        self.ctx.check(self.ctx.L.gbx_graph_batch_info(self.h, C.byref(n), _ptr(off)))
+        return off
 
     def export(self, inverse=False):
         N, E = self.N, self.E
@@ -315,6 +335,13 @@ class PageRank:
             self.ctx.check(self.ctx.L.gbx_pr_create(self.ctx.h, graph.h, C.byref(p), C.byref(h)))
         self.h = h
         self.ctx._adopt(self)
+        self.nreports = 0 if num_vectors else len(graph.batch_offsets()) - 1
+        if self.nreports == 1 and graph.N and not self._is_batch(graph):
+            self.nreports = 0
+
+    @staticmethod
+    def _is_batch(graph):
+        return getattr(graph, "_batch", False)
 
     def reset(self, teleport=None):
         tp = _f32(teleport)
@@ -323,12 +350,20 @@ class PageRank:
     def launch(self, n):
         self.ctx.check(self.ctx.L.gbx_pr_launch(self.h, n))
 
+    def launch_graph(self, n):
+        self.ctx.check(self.ctx.L.gbx_pr_launch_graph(self.h, n))
+
     def launch_timed(self, n) -> float:
         ms = C.c_double()
         self.ctx.check(self.ctx.L.gbx_pr_launch_timed(self.h, n, C.byref(ms)))
         return ms.value
 
     def read(self):
+        if self.nreports > 1:
+            ranks = np.empty(self.g.N, np.float32)
+            reps = (PrReport * self.nreports)()
+            self.ctx.check(self.ctx.L.gbx_pr_read(self.h, _ptr(ranks), C.cast(reps, _vp)))
+            return ranks, [r.as_dict() for r in reps]
         if self.k:
             ranks = np.empty((self.g.N, self.k), np.float32)
             reps = (PrReport * self.k)()

@@ -88,6 +88,11 @@ struct gbx_graph {
     bool weighted = false;
     gbx_csr_dev fwd, inv;
     gbx_plan *plan[2] = {nullptr, nullptr};  // per direction
+    // a batch of independent graphs stored block-diagonally: graph k owns
+    // vertices [seg_off[k], seg_off[k+1]); empty for a single graph
+    std::vector<uint32_t> seg_off;
+    std::vector<uint32_t> seg_edge_off;
+    gbx::DevBuf seg_gid;  // u32[N]: graph of each vertex
 };
 
 namespace gbx {

@@ -4,6 +4,9 @@
 
 #include "gbx_internal.hpp"
 
+#include <utility>
+#include <vector>
+
 namespace gbx {
 
 // 1024-thread blocks, one per CU, each holding the hottest contributions in
@@ -36,6 +39,23 @@ struct HubChunk {
     uint32_t hub;    // hub index (counter slot)
     uint32_t first;  // first chunk of this hub
     uint32_t nch;    // chunks of this hub
+    uint32_t graph;  // graph of the row (batched graphs), else 0
+    uint32_t pad;
+};
+
+// One unit of work of a batch of graphs: rows [row_begin, row_end) of one
+// graph with `lanes` lanes per row, or (lanes == 0) hub chunk row_begin.
+struct TileDesc {
+    uint32_t row_begin, row_end;
+    uint32_t lanes;
+    uint32_t graph;
+    uint32_t slot;  // first per-wave partial slot of this tile
+};
+
+// Per-graph slot ranges of a batch of graphs.
+struct SegRange {
+    uint32_t slot_begin, slot_end;  // per-wave slots of the graph's row tiles
+    uint32_t hub_begin, hub_end;    // hub-row slots (offset by hub_slot_base)
 };
 
 struct Slot {
@@ -59,9 +79,14 @@ struct gbx_plan {
     uint64_t cls_edges[gbx::kMaxClasses];
     uint32_t n_chunks = 0, n_hubs = 0, n_tiles = 0;
     bool has_wt = false, has_wn = false, has_wn16 = false;
+    // batch of graphs: explicit graph-pure tiles and per-graph slot ranges
+    bool segmented = false;
+    uint32_t nseg = 0, n_seg_tiles = 0, n_wave_slots = 0, n_seg_hubs = 0, n_seg_chunks = 0;
+    gbx::DevBuf tiles, seg_ranges, row_gid, seg_chunks, seg_hub_cnt, seg_hub_part;
 };
 
 struct gbx_ppr_state;
+struct gbx_seg_state;
 
 struct gbx_pr {
     gbx_ctx *ctx = nullptr;
@@ -79,6 +104,9 @@ struct gbx_pr {
     double elapsed_ms = 0.0;
     gbx::Slot *h_slot = nullptr;  // pinned
     gbx_ppr_state *batch = nullptr;  // multi-vector state (gbx_ppr.hip), else null
+    gbx_seg_state *seg = nullptr;    // batch-of-graphs state (gbx_batch.hip), else null
+    // captured iteration sequences: key = (iterations << 1) | starting parity
+    std::vector<std::pair<uint64_t, hipGraphExec_t>> graph_cache;
 };
 
 namespace gbx {
@@ -175,11 +203,199 @@ __device__ __forceinline__ void store_val(uint16_t *p, uint32_t i, float v) { p[
 // Hot prefix of the contribution vector held in LDS for the whole kernel.
 extern __shared__ __align__(16) unsigned char gbx_dyn_lds[];
 
+struct PlanView {
+    const uint32_t *__restrict__ rp;
+    const uint32_t *__restrict__ col;
+    const float *__restrict__ wt;  // raw (SpMV) or row-normalised (PR) weights
+    const uint32_t *__restrict__ outdeg;
+    const HubChunk *__restrict__ chunks;
+    uint32_t *__restrict__ hub_cnt;
+    float *__restrict__ hub_part;
+    uint32_t N, n_tiles, n_chunks, nclass;
+    ClassDesc cls[kMaxClasses];
+};
+
+// ------------------------------------------------------------------ gathers
+
+// Contribution gather; with LDS the first `hot` (most gathered) vertices
+// come from the block's LDS copy.
+template <class T, bool LDS>
+struct GatherC {
+    const T *__restrict__ v;
+    uint32_t hot;
+    __device__ __forceinline__ float operator()(uint32_t, uint32_t c) const
+    {
+        if constexpr (LDS) {
+            if (c < hot) return load_val(reinterpret_cast<const T *>(gbx_dyn_lds), c);
+        }
+        return load_val(v, c);
+    }
+};
+
+// Weighted contribution gather: w[j] * v[c], weights of type T.
+template <class T, bool LDS>
+struct GatherCW {
+    const T *__restrict__ v;
+    const T *__restrict__ w;
+    uint32_t hot;
+    __device__ __forceinline__ float operator()(uint32_t j, uint32_t c) const
+    {
+        float x;
+        if constexpr (LDS) {
+            if (c < hot) x = load_val(reinterpret_cast<const T *>(gbx_dyn_lds), c);
+            else x = load_val(v, c);
+        } else {
+            x = load_val(v, c);
+        }
+        float wj;
+        if constexpr (sizeof(T) == 4) wj = ld_stream(reinterpret_cast<const float *>(w) + j);
+        else wj = load_val(w, j);
+        return wj * x;
+    }
+};
+
+struct GatherConstW {
+    const float *__restrict__ v;
+    float w;
+    __device__ __forceinline__ float operator()(uint32_t, uint32_t c) const { return w * v[c]; }
+};
+
+// Lane-strided, in-order partial sum of one row: GBX_UNROLL column loads,
+// then GBX_UNROLL gathers in flight; the tail is one predicated batch.
+template <int G, class Acc, class Gather>
+__device__ __forceinline__ void lane_sum(uint32_t j, uint32_t end, const uint32_t *__restrict__ col,
+                                         const Gather &g, Acc &acc)
+{
+    constexpr uint32_t U = GBX_UNROLL;
+    for (; j + (U - 1) * G < end; j += U * G) {
+        uint32_t c[U];
+        float t[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) c[k] = ld_stream(col + j + k * G);
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) t[k] = g(j + k * G, c[k]);
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) acc.add(t[k]);
+    }
+    if (j < end) {
+        uint32_t c[U];
+        float t[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) c[k] = (j + k * G < end) ? ld_stream(col + j + k * G) : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) t[k] = (j + k * G < end) ? g(j + k * G, c[k]) : 0.0f;
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++)
+            if (j + k * G < end) acc.add(t[k]);
+    }
+}
+
+template <int G, class Acc, class Gather, class Epi>
+__device__ __forceinline__ void do_rows(const ClassDesc &cd, uint32_t local, const PlanView &P,
+                                        const Gather &g, Epi &epi)
+{
+    constexpr uint32_t RPT = kBlock / G;
+    const uint32_t grp = threadIdx.x / G, lig = threadIdx.x % G;
+    const uint32_t row = cd.row_begin + local * RPT + grp;
+    const bool valid = row < cd.row_end;
+    Acc acc;
+    acc.init((G == 1 && valid) ? epi.init(row) : 0.0f);
+    if (valid) {
+        const uint32_t beg = ld_stream(P.rp + row), end = ld_stream(P.rp + row + 1);
+        lane_sum<G>(beg + lig, end, P.col, g, acc);
+    }
+    if constexpr (G > 1) {
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) acc.shfl_merge(off);
+    }
+    if (valid && lig == 0) epi(row, G == 1 ? acc.value() : epi.init(row) + acc.value());
+}
+
+template <class Acc, class Gather, class Epi>
+__device__ __forceinline__ void do_hub(uint32_t tile, const PlanView &P, const Gather &g, Epi &epi)
+{
+    __shared__ Acc red[kBlock / 64];
+    const HubChunk hc = P.chunks[tile];
+    Acc acc;
+    acc.init(0.0f);
+    lane_sum<kBlock>(hc.ebeg + threadIdx.x, hc.eend, P.col, g, acc);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc.shfl_merge(off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Acc s = red[0];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; w++) s.merge(red[w]);
+        if (hc.nch == 1) {
+            // the whole row in one chunk: no cross-block combine, no fences
+            epi(hc.row, epi.init(hc.row) + s.value());
+        } else {
+            P.hub_part[tile] = s.value();
+            __threadfence();
+            uint32_t prev = atomicAdd(&P.hub_cnt[hc.hub], 1u);
+            if (prev == hc.nch - 1) {
+                __threadfence();
+                Acc tot;
+                tot.init(0.0f);
+                const volatile float *hp = P.hub_part;
+                for (uint32_t k = 0; k < hc.nch; k++) tot.add(hp[hc.first + k]);
+                P.hub_cnt[hc.hub] = 0;
+                epi(hc.row, epi.init(hc.row) + tot.value());
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <class Acc, class Gather, class Epi>
+__device__ __forceinline__ void tile_loop(const PlanView &P, const Gather &g, Epi &epi)
+{
+    for (uint32_t tile = blockIdx.x; tile < P.n_tiles; tile += gridDim.x) {
+        if (tile < P.n_chunks) {
+            do_hub<Acc>(tile, P, g, epi);
+            continue;
+        }
+        uint32_t c = 1;
+        while (c + 1 < P.nclass && tile >= P.cls[c].tile_end) c++;
+        const ClassDesc &cd = P.cls[c];
+        const uint32_t local = tile - cd.tile_begin;
+        switch (cd.lanes) {
+        case 64: do_rows<64, Acc>(cd, local, P, g, epi); break;
+        case 32: do_rows<32, Acc>(cd, local, P, g, epi); break;
+        case 16: do_rows<16, Acc>(cd, local, P, g, epi); break;
+        case 8: do_rows<8, Acc>(cd, local, P, g, epi); break;
+        case 4: do_rows<4, Acc>(cd, local, P, g, epi); break;
+        case 2: do_rows<2, Acc>(cd, local, P, g, epi); break;
+        default: do_rows<1, Acc>(cd, local, P, g, epi); break;
+        }
+    }
+}
+
+// Copy the hot prefix of the contribution vector into LDS.
+template <class T>
+__device__ __forceinline__ void load_hot(const T *__restrict__ c_in, uint32_t hot)
+{
+    T *h = reinterpret_cast<T *>(gbx_dyn_lds);
+    const uint32_t words = (uint32_t)((hot * sizeof(T) + 15) / 16);
+    const uint4 *src = reinterpret_cast<const uint4 *>(c_in);
+    uint4 *dst = reinterpret_cast<uint4 *>(h);
+    for (uint32_t i = threadIdx.x; i < words; i += kBlock) dst[i] = src[i];
+    __syncthreads();
+}
+
 // Multi-vector hooks implemented in gbx_ppr.hip.
 int ppr_reset(gbx_pr *pr);
 int ppr_launch_iter(gbx_pr *pr);
 int ppr_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep);
 void ppr_state_free(gbx_ppr_state *s);
+
+// Batch-of-graphs hooks implemented in gbx_batch.hip.
+int seg_create(gbx_pr *pr);
+int seg_reset(gbx_pr *pr);
+int seg_launch_iter(gbx_pr *pr);
+int seg_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep);
+void seg_state_free(gbx_seg_state *s);
 
 // Host helpers implemented in gbx_pr.hip.
 int ensure_plan(gbx_ctx *ctx, gbx_graph *g, int direction, gbx_plan **out);

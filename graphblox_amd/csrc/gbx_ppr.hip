@@ -533,6 +533,7 @@ int gbx_ppr_create(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, uint32_t 
     GBX_HIP(ctx, hipSetDevice(ctx->device));
     gbx_plan *plan = nullptr;
     GBX_TRY(ensure_plan(ctx, g, p->direction, &plan));
+    if (plan->segmented) return set_error(ctx, GBX_ERR_STATE, "gbx_ppr_create: not available for a batch of graphs");
     gbx_pr *pr = new (std::nothrow) gbx_pr();
     if (!pr) return set_error(ctx, GBX_ERR_NOMEM, "gbx_ppr_create: out of host memory");
     pr->ctx = ctx;

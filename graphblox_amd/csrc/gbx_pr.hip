@@ -31,18 +31,6 @@
 
 namespace gbx {
 
-struct PlanView {
-    const uint32_t *__restrict__ rp;
-    const uint32_t *__restrict__ col;
-    const float *__restrict__ wt;  // raw (SpMV) or row-normalised (PR) weights
-    const uint32_t *__restrict__ outdeg;
-    const HubChunk *__restrict__ chunks;
-    uint32_t *__restrict__ hub_cnt;
-    float *__restrict__ hub_part;
-    uint32_t N, n_tiles, n_chunks, nclass;
-    ClassDesc cls[kMaxClasses];
-};
-
 struct IterView {
     float *__restrict__ r;
     const void *__restrict__ c_in;
@@ -57,175 +45,6 @@ struct IterView {
     double eps;
     int stop;
 };
-
-// ------------------------------------------------------------------ gathers
-
-// Contribution gather; with LDS the first `hot` (most gathered) vertices
-// come from the block's LDS copy.
-template <class T, bool LDS>
-struct GatherC {
-    const T *__restrict__ v;
-    uint32_t hot;
-    __device__ __forceinline__ float operator()(uint32_t, uint32_t c) const
-    {
-        if constexpr (LDS) {
-            if (c < hot) return load_val(reinterpret_cast<const T *>(gbx_dyn_lds), c);
-        }
-        return load_val(v, c);
-    }
-};
-
-// Weighted contribution gather: w[j] * v[c], weights of type T.
-template <class T, bool LDS>
-struct GatherCW {
-    const T *__restrict__ v;
-    const T *__restrict__ w;
-    uint32_t hot;
-    __device__ __forceinline__ float operator()(uint32_t j, uint32_t c) const
-    {
-        float x;
-        if constexpr (LDS) {
-            if (c < hot) x = load_val(reinterpret_cast<const T *>(gbx_dyn_lds), c);
-            else x = load_val(v, c);
-        } else {
-            x = load_val(v, c);
-        }
-        float wj;
-        if constexpr (sizeof(T) == 4) wj = ld_stream(reinterpret_cast<const float *>(w) + j);
-        else wj = load_val(w, j);
-        return wj * x;
-    }
-};
-
-struct GatherConstW {
-    const float *__restrict__ v;
-    float w;
-    __device__ __forceinline__ float operator()(uint32_t, uint32_t c) const { return w * v[c]; }
-};
-
-// Lane-strided, in-order partial sum of one row: GBX_UNROLL column loads,
-// then GBX_UNROLL gathers in flight; the tail is one predicated batch.
-template <int G, class Acc, class Gather>
-__device__ __forceinline__ void lane_sum(uint32_t j, uint32_t end, const uint32_t *__restrict__ col,
-                                         const Gather &g, Acc &acc)
-{
-    constexpr uint32_t U = GBX_UNROLL;
-    for (; j + (U - 1) * G < end; j += U * G) {
-        uint32_t c[U];
-        float t[U];
-#pragma unroll
-        for (uint32_t k = 0; k < U; k++) c[k] = ld_stream(col + j + k * G);
-#pragma unroll
-        for (uint32_t k = 0; k < U; k++) t[k] = g(j + k * G, c[k]);
-#pragma unroll
-        for (uint32_t k = 0; k < U; k++) acc.add(t[k]);
-    }
-    if (j < end) {
-        uint32_t c[U];
-        float t[U];
-#pragma unroll
-        for (uint32_t k = 0; k < U; k++) c[k] = (j + k * G < end) ? ld_stream(col + j + k * G) : 0u;
-#pragma unroll
-        for (uint32_t k = 0; k < U; k++) t[k] = (j + k * G < end) ? g(j + k * G, c[k]) : 0.0f;
-#pragma unroll
-        for (uint32_t k = 0; k < U; k++)
-            if (j + k * G < end) acc.add(t[k]);
-    }
-}
-
-template <int G, class Acc, class Gather, class Epi>
-__device__ __forceinline__ void do_rows(const ClassDesc &cd, uint32_t local, const PlanView &P,
-                                        const Gather &g, Epi &epi)
-{
-    constexpr uint32_t RPT = kBlock / G;
-    const uint32_t grp = threadIdx.x / G, lig = threadIdx.x % G;
-    const uint32_t row = cd.row_begin + local * RPT + grp;
-    const bool valid = row < cd.row_end;
-    Acc acc;
-    acc.init((G == 1 && valid) ? epi.init(row) : 0.0f);
-    if (valid) {
-        const uint32_t beg = ld_stream(P.rp + row), end = ld_stream(P.rp + row + 1);
-        lane_sum<G>(beg + lig, end, P.col, g, acc);
-    }
-    if constexpr (G > 1) {
-#pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) acc.shfl_merge(off);
-    }
-    if (valid && lig == 0) epi(row, G == 1 ? acc.value() : epi.init(row) + acc.value());
-}
-
-template <class Acc, class Gather, class Epi>
-__device__ __forceinline__ void do_hub(uint32_t tile, const PlanView &P, const Gather &g, Epi &epi)
-{
-    __shared__ Acc red[kBlock / 64];
-    const HubChunk hc = P.chunks[tile];
-    Acc acc;
-    acc.init(0.0f);
-    lane_sum<kBlock>(hc.ebeg + threadIdx.x, hc.eend, P.col, g, acc);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc.shfl_merge(off);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        Acc s = red[0];
-#pragma unroll
-        for (int w = 1; w < kBlock / 64; w++) s.merge(red[w]);
-        if (hc.nch == 1) {
-            // the whole row in one chunk: no cross-block combine, no fences
-            epi(hc.row, epi.init(hc.row) + s.value());
-        } else {
-            P.hub_part[tile] = s.value();
-            __threadfence();
-            uint32_t prev = atomicAdd(&P.hub_cnt[hc.hub], 1u);
-            if (prev == hc.nch - 1) {
-                __threadfence();
-                Acc tot;
-                tot.init(0.0f);
-                const volatile float *hp = P.hub_part;
-                for (uint32_t k = 0; k < hc.nch; k++) tot.add(hp[hc.first + k]);
-                P.hub_cnt[hc.hub] = 0;
-                epi(hc.row, epi.init(hc.row) + tot.value());
-            }
-        }
-    }
-    __syncthreads();
-}
-
-template <class Acc, class Gather, class Epi>
-__device__ __forceinline__ void tile_loop(const PlanView &P, const Gather &g, Epi &epi)
-{
-    for (uint32_t tile = blockIdx.x; tile < P.n_tiles; tile += gridDim.x) {
-        if (tile < P.n_chunks) {
-            do_hub<Acc>(tile, P, g, epi);
-            continue;
-        }
-        uint32_t c = 1;
-        while (c + 1 < P.nclass && tile >= P.cls[c].tile_end) c++;
-        const ClassDesc &cd = P.cls[c];
-        const uint32_t local = tile - cd.tile_begin;
-        switch (cd.lanes) {
-        case 64: do_rows<64, Acc>(cd, local, P, g, epi); break;
-        case 32: do_rows<32, Acc>(cd, local, P, g, epi); break;
-        case 16: do_rows<16, Acc>(cd, local, P, g, epi); break;
-        case 8: do_rows<8, Acc>(cd, local, P, g, epi); break;
-        case 4: do_rows<4, Acc>(cd, local, P, g, epi); break;
-        case 2: do_rows<2, Acc>(cd, local, P, g, epi); break;
-        default: do_rows<1, Acc>(cd, local, P, g, epi); break;
-        }
-    }
-}
-
-// Copy the hot prefix of the contribution vector into LDS.
-template <class T>
-__device__ __forceinline__ void load_hot(const T *__restrict__ c_in, uint32_t hot)
-{
-    T *h = reinterpret_cast<T *>(gbx_dyn_lds);
-    const uint32_t words = (uint32_t)((hot * sizeof(T) + 15) / 16);
-    const uint4 *src = reinterpret_cast<const uint4 *>(c_in);
-    uint4 *dst = reinterpret_cast<uint4 *>(h);
-    for (uint32_t i = threadIdx.x; i < words; i += kBlock) dst[i] = src[i];
-    __syncthreads();
-}
 
 // ------------------------------------------------------------------ reductions
 
@@ -464,17 +283,25 @@ __global__ void degree_keys(const uint32_t *__restrict__ deg, uint32_t N, uint32
     }
 }
 
-__global__ void plan_vertices(const uint32_t *__restrict__ key, const uint32_t *__restrict__ perm,
-                              uint32_t N, uint32_t maxdeg, const uint32_t *__restrict__ fwd_outdeg,
+__global__ void plan_vertices(const uint32_t *__restrict__ perm, uint32_t N,
+                              const uint32_t *__restrict__ row_deg, const uint32_t *__restrict__ fwd_outdeg,
                               uint32_t *__restrict__ inv, uint32_t *__restrict__ deg_new,
                               uint32_t *__restrict__ outdeg_new)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         const uint32_t v = perm[i];
         inv[v] = i;
-        deg_new[i] = maxdeg - key[i];
+        deg_new[i] = row_deg[v];
         outdeg_new[i] = fwd_outdeg[v];
     }
+}
+
+// key[i] = graph of the vertex now at position i (batches of graphs)
+__global__ void gid_keys(const uint32_t *__restrict__ perm, const uint32_t *__restrict__ gid, uint32_t N,
+                         uint32_t *__restrict__ key)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        key[i] = gid[perm[i]];
 }
 
 // Copy each edge of the row CSR into the relabelled plan, keeping its
@@ -599,6 +426,96 @@ static BinConfig bin_config()
     return b;
 }
 
+// Batch of graphs: graph-pure tiles.  Inside graph k (plan rows
+// [seg_off[k], seg_off[k+1]), degrees descending) hub rows become chunk tiles
+// and the other rows tiles of kBlock/lanes rows; each row tile owns one
+// partial slot per wave, each hub row one slot of its own.
+static int build_segmented_tiles(gbx_ctx *ctx, gbx_graph *g, gbx_plan *p, const DevBuf &deg_new,
+                                 const BinConfig &bc)
+{
+    const uint32_t N = p->N, nseg = (uint32_t)g->seg_off.size() - 1;
+    std::vector<uint32_t> deg(N), rph((size_t)N + 1);
+    GBX_HIP(ctx, hipMemcpyAsync(deg.data(), deg_new.p, (size_t)N * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GBX_HIP(ctx, hipMemcpyAsync(rph.data(), p->rp.p, ((size_t)N + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    static const uint32_t lanes[8] = {0, 64, 32, 16, 8, 4, 2, 1};
+    const uint32_t wpb = kBlock / 64;
+    std::vector<TileDesc> tiles;
+    std::vector<HubChunk> chunks;
+    std::vector<SegRange> ranges(nseg);
+    uint32_t slot = 0, hub = 0;
+    for (uint32_t k = 0; k < nseg; k++) {
+        const uint32_t r0 = g->seg_off[k], r1 = g->seg_off[k + 1];
+        auto first_below = [&](uint32_t t) {
+            uint32_t lo = r0, hi = r1;
+            while (lo < hi) {
+                const uint32_t mid = lo + (hi - lo) / 2;
+                if (deg[mid] >= t) lo = mid + 1;
+                else hi = mid;
+            }
+            return lo;
+        };
+        uint32_t b[7];
+        for (int c = 0; c < 7; c++) b[c] = std::max(first_below(bc.thr[c]), c ? b[c - 1] : r0);
+        ranges[k].hub_begin = hub;
+        for (uint32_t r = r0; r < b[0]; r++, hub++) {
+            const uint32_t e0 = rph[r], e1 = rph[r + 1];
+            const uint32_t nch = (e1 - e0 + bc.hub_chunk - 1) / bc.hub_chunk;
+            const uint32_t first = (uint32_t)chunks.size();
+            for (uint32_t c = 0; c < nch; c++) {
+                HubChunk hc;
+                hc.row = r;
+                hc.ebeg = e0 + c * bc.hub_chunk;
+                hc.eend = std::min(e1, hc.ebeg + bc.hub_chunk);
+                hc.hub = hub;
+                hc.first = first;
+                hc.nch = nch;
+                hc.graph = k;
+                hc.pad = 0;
+                tiles.push_back(TileDesc{(uint32_t)chunks.size(), 0u, 0u, k, 0u});
+                chunks.push_back(hc);
+            }
+        }
+        ranges[k].hub_end = hub;
+        ranges[k].slot_begin = slot;
+        for (int c = 1; c < 8; c++) {
+            const uint32_t beg = b[c - 1], end = c == 7 ? r1 : b[c];
+            const uint32_t rpt = kBlock / lanes[c];
+            for (uint32_t r = beg; r < end; r += rpt) {
+                tiles.push_back(TileDesc{r, end, lanes[c], k, slot});
+                slot += wpb;
+            }
+        }
+        ranges[k].slot_end = slot;
+    }
+    p->segmented = true;
+    p->nseg = nseg;
+    p->n_seg_tiles = (uint32_t)tiles.size();
+    p->n_wave_slots = slot;
+    p->n_seg_hubs = hub;
+    p->n_seg_chunks = (uint32_t)chunks.size();
+    GBX_TRY(dev_alloc(ctx, p->tiles, tiles.size() * sizeof(TileDesc) + 16));
+    GBX_TRY(dev_alloc(ctx, p->seg_ranges, ranges.size() * sizeof(SegRange) + 16));
+    GBX_TRY(dev_alloc(ctx, p->seg_chunks, chunks.size() * sizeof(HubChunk) + 16));
+    GBX_TRY(dev_alloc(ctx, p->seg_hub_cnt, (size_t)hub * 4 + 4));
+    GBX_TRY(dev_alloc(ctx, p->seg_hub_part, chunks.size() * 4 + 4));
+    GBX_TRY(dev_alloc(ctx, p->row_gid, (size_t)N * 4 + 4));
+    GBX_TRY(fill_u32(ctx, p->seg_hub_cnt.as<uint32_t>(), 0, hub + 1));
+    if (!tiles.empty())
+        GBX_HIP(ctx, hipMemcpyAsync(p->tiles.p, tiles.data(), tiles.size() * sizeof(TileDesc),
+                                    hipMemcpyHostToDevice, ctx->stream));
+    GBX_HIP(ctx, hipMemcpyAsync(p->seg_ranges.p, ranges.data(), ranges.size() * sizeof(SegRange),
+                                hipMemcpyHostToDevice, ctx->stream));
+    if (!chunks.empty())
+        GBX_HIP(ctx, hipMemcpyAsync(p->seg_chunks.p, chunks.data(), chunks.size() * sizeof(HubChunk),
+                                    hipMemcpyHostToDevice, ctx->stream));
+    gid_keys<<<grid_for(N, 256, 16384), 256, 0, ctx->stream>>>(p->perm.as<uint32_t>(), g->seg_gid.as<uint32_t>(),
+                                                                N, p->row_gid.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GBX_OK;
+}
+
 static int build_plan(gbx_ctx *ctx, gbx_graph *g, int direction, gbx_plan **out)
 {
     if (g->E >= 0x80000000u) return set_error(ctx, GBX_ERR_RANGE, "plan: more than 2^31 edges");
@@ -629,7 +546,17 @@ static int build_plan(gbx_ctx *ctx, gbx_graph *g, int direction, gbx_plan **out)
     if ((rc = radix_sort_pairs(ctx, key.as<uint32_t>(), p->perm.as<uint32_t>(), nullptr, N,
                                bits_for(maxdeg))))
         return fail(rc);
-    plan_vertices<<<gN, 256, 0, ctx->stream>>>(key.as<uint32_t>(), p->perm.as<uint32_t>(), N, maxdeg,
+    const bool seg = g->seg_off.size() >= 2;
+    if (seg && g->seg_off.size() > 2) {
+        // then stably by graph: graph-major, degree-descending inside each graph
+        gid_keys<<<gN, 256, 0, ctx->stream>>>(p->perm.as<uint32_t>(), g->seg_gid.as<uint32_t>(), N,
+                                               key.as<uint32_t>());
+        if (hipGetLastError() != hipSuccess) return fail(set_error(ctx, GBX_ERR_HIP, "gid_keys"));
+        if ((rc = radix_sort_pairs(ctx, key.as<uint32_t>(), p->perm.as<uint32_t>(), nullptr, N,
+                                   bits_for((uint32_t)g->seg_off.size() - 2))))
+            return fail(rc);
+    }
+    plan_vertices<<<gN, 256, 0, ctx->stream>>>(p->perm.as<uint32_t>(), N, rows.out_degree.as<uint32_t>(),
                                                 g->fwd.out_degree.as<uint32_t>(), p->inv.as<uint32_t>(),
                                                 deg_new.as<uint32_t>(), p->outdeg.as<uint32_t>());
     if (hipGetLastError() != hipSuccess) return fail(set_error(ctx, GBX_ERR_HIP, "plan_vertices"));
@@ -723,6 +650,7 @@ static int build_plan(gbx_ctx *ctx, gbx_graph *g, int direction, gbx_plan **out)
         }
         p->cls_edges[c] = (uint64_t)(e1 - e0);
     }
+    if (seg && (rc = build_segmented_tiles(ctx, g, p, deg_new, bc))) return fail(rc);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess)
         return fail(set_error(ctx, GBX_ERR_HIP, "plan build failed"));
     *out = p;
@@ -886,6 +814,7 @@ static const float *plan_weights(const gbx_pr *pr)
 static int launch_iter(gbx_pr *pr)
 {
     if (pr->batch) return ppr_launch_iter(pr);
+    if (pr->seg) return seg_launch_iter(pr);
     gbx_ctx *ctx = pr->ctx;
     PlanView P = plan_view(pr->plan, plan_weights(pr));
     IterView A = iter_view(pr);
@@ -961,6 +890,27 @@ int gbx_pr_create(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, gbx_pr **o
     GBX_HIP(ctx, hipSetDevice(ctx->device));
     gbx_plan *plan = nullptr;
     GBX_TRY(ensure_plan(ctx, g, p->direction, &plan));
+    if (plan->segmented) {
+        if (p->mode != GBX_PR_STOCHASTIC || p->precision != GBX_F32 || p->weighted || p->stop == GBX_STOP_ACTIVE)
+            return set_error(ctx, GBX_ERR_ARG,
+                             "gbx_pr_create: a batch of graphs runs STOCHASTIC, F32, unweighted, stop L1 or NONE");
+        gbx_pr *pr = new (std::nothrow) gbx_pr();
+        if (!pr) return set_error(ctx, GBX_ERR_NOMEM, "gbx_pr_create: out of host memory");
+        pr->ctx = ctx;
+        pr->g = g;
+        pr->plan = plan;
+        pr->p = *p;
+        pr->kind = K_PR_STOCH;
+        int rc = seg_create(pr);
+        if (!rc && (hipEventCreate(&pr->ev_begin) != hipSuccess || hipEventCreate(&pr->ev_end) != hipSuccess))
+            rc = set_error(ctx, GBX_ERR_HIP, "gbx_pr_create: event creation");
+        if (rc) {
+            gbx_pr_free(pr);
+            return rc;
+        }
+        *out = pr;
+        return GBX_OK;
+    }
     if (p->weighted) {
         if (p->precision == GBX_BF16) GBX_TRY(ensure_normalised_bf16(ctx, g, plan));
         else GBX_TRY(ensure_normalised(ctx, g, plan));
@@ -1005,11 +955,11 @@ int gbx_pr_reset(gbx_pr *pr, const float *teleport)
     GBX_HIP(ctx, hipSetDevice(ctx->device));
     const uint32_t N = pr->g->N;
     pr->has_tele = false;
-    if (pr->batch) {
+    if (pr->batch || pr->seg) {
         if (teleport) return set_error(ctx, GBX_ERR_ARG, "gbx_pr_reset: batched teleports are fixed at creation");
         pr->timing_started = false;
         pr->elapsed_ms = 0.0;
-        return ppr_reset(pr);
+        return pr->batch ? ppr_reset(pr) : seg_reset(pr);
     }
     if (teleport) {
         if (pr->p.mode == GBX_PR_REF)
@@ -1063,6 +1013,45 @@ int gbx_pr_launch(gbx_pr *pr, uint32_t n_iters)
     return GBX_OK;
 }
 
+int gbx_pr_launch_graph(gbx_pr *pr, uint32_t n_iters)
+{
+    if (!pr) return set_error(nullptr, GBX_ERR_ARG, "gbx_pr_launch_graph: NULL");
+    gbx_ctx *ctx = pr->ctx;
+    GBX_HIP(ctx, hipSetDevice(ctx->device));
+    if (n_iters == 0) return GBX_OK;
+    const uint64_t key = ((uint64_t)n_iters << 1) | (pr->t & 1u);
+    hipGraphExec_t exec = nullptr;
+    for (auto &e : pr->graph_cache)
+        if (e.first == key) exec = e.second;
+    if (!exec) {
+        // capture n iterations (kernel launches only: no memsets, copies or syncs)
+        const uint32_t t0 = pr->t;
+        GBX_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        int rc = GBX_OK;
+        for (uint32_t k = 0; k < n_iters && !rc; k++) rc = launch_iter(pr);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+        pr->t = t0;
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return set_error(ctx, GBX_ERR_HIP, "capture failed: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return set_error(ctx, GBX_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+        pr->graph_cache.emplace_back(key, exec);
+    }
+    if (!pr->timing_started) {
+        GBX_HIP(ctx, hipEventRecord(pr->ev_begin, ctx->stream));
+        pr->timing_started = true;
+    }
+    GBX_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+    GBX_HIP(ctx, hipEventRecord(pr->ev_end, ctx->stream));
+    pr->t += n_iters;
+    return GBX_OK;
+}
+
 int gbx_pr_launch_timed(gbx_pr *pr, uint32_t n_iters, double *kernel_ms_avg)
 {
     if (!pr) return set_error(nullptr, GBX_ERR_ARG, "gbx_pr_launch_timed: NULL");
@@ -1104,13 +1093,13 @@ int gbx_pr_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep)
     if (!pr) return set_error(nullptr, GBX_ERR_ARG, "gbx_pr_read: NULL");
     gbx_ctx *ctx = pr->ctx;
     GBX_HIP(ctx, hipSetDevice(ctx->device));
-    if (pr->batch) {
+    if (pr->batch || pr->seg) {
         GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if (pr->timing_started) {
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, pr->ev_begin, pr->ev_end) == hipSuccess) pr->elapsed_ms = ms;
         }
-        return ppr_read(pr, ranks_out, rep);
+        return pr->batch ? ppr_read(pr, ranks_out, rep) : seg_read(pr, ranks_out, rep);
     }
     const uint32_t N = pr->g->N;
     const Slot *slot = pr->slots.as<Slot>() + (pr->t & 1);
@@ -1151,6 +1140,8 @@ int gbx_pr_free(gbx_pr *pr)
     if (pr->ev_end) (void)hipEventDestroy(pr->ev_end);
     if (pr->h_slot) (void)hipHostFree(pr->h_slot);
     if (pr->batch) ppr_state_free(pr->batch);
+    if (pr->seg) seg_state_free(pr->seg);
+    for (auto &e : pr->graph_cache) (void)hipGraphExecDestroy(e.second);
     delete pr;
     return GBX_OK;
 }
@@ -1202,6 +1193,7 @@ int gbx_spmv_pull(gbx_ctx *ctx, gbx_graph *g, uint32_t iterations, int weighted,
     GBX_HIP(ctx, hipSetDevice(ctx->device));
     gbx_plan *plan = nullptr;
     GBX_TRY(ensure_plan(ctx, g, direction, &plan));
+    if (plan->segmented) return set_error(ctx, GBX_ERR_STATE, "gbx_spmv_pull: not available for a batch of graphs");
     if (weighted && !plan->has_wt)
         return set_error(ctx, GBX_ERR_STATE, "gbx_spmv_pull: weighted SpMV needs a weighted graph");
     const uint32_t N = g->N;

@@ -213,6 +213,25 @@ int gbx_ppr_create(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, uint32_t 
 int gbx_ppr_info(const gbx_pr *pr, uint32_t *num_vectors, uint32_t *vector_width,
                  uint32_t *hub_rows, uint32_t *hub_chunks);
 
+/* Replay n_iters iterations as one HIP graph (captured on first use for this
+ * (n_iters, iteration parity) and cached in the object): same results as
+ * gbx_pr_launch, one launch instead of one or two per iteration. */
+int gbx_pr_launch_graph(gbx_pr *pr, uint32_t n_iters);
+
+/* ---------------------------------------------------------------- graph batches */
+/* Many independent graphs as one device object, stored block-diagonally
+ * (graph k's vertex v becomes vertex offsets[k] + v; no reference
+ * counterpart).  The member graphs are copied; they may be freed afterwards.
+ * gbx_pr_create on the batch runs every graph's own GBX_PR_STOCHASTIC
+ * PageRank (teleport 1/N_k, its own dangling mass, normalisation and L1 stop
+ * rule) in one launch per iteration; gbx_pr_read returns the ranks in the
+ * batch's vertex order and one report per graph. */
+int gbx_graph_batch_create(gbx_ctx *ctx, uint32_t num_graphs, gbx_graph *const *graphs,
+                           gbx_graph **out);
+/* Number of member graphs and their vertex offsets [num_graphs + 1]
+ * (a single graph reports 1 and {0, N}). */
+int gbx_graph_batch_info(const gbx_graph *g, uint32_t *num_graphs, uint32_t *vertex_offsets);
+
 /* Plan statistics: counts of rows per degree class and edges per class
  * (for reporting and the autotuner); arrays of length *n_classes (<= 16). */
 int gbx_pr_plan_info(const gbx_pr *pr, uint32_t *n_classes, uint32_t *class_lanes,
