@@ -40,6 +40,18 @@ class PrParams(C.Structure):
                 ("direction", C.c_int), ("weighted", C.c_int), ("reserved", C.c_int)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("bin_thresholds", C.c_uint32 * 7), ("hub_chunk", C.c_uint32), ("lds_bytes", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32), ("ppr_hub", C.c_uint32), ("ppr_chunk", C.c_uint32),
+                ("ppr_blocks_per_cu", C.c_uint32), ("ppr_vectors_per_lane", C.c_uint32),
+                ("reserved", C.c_uint32 * 4)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("bin_thresholds", "reserved")}
+        d["bin_thresholds"] = list(self.bin_thresholds)
+        return d
+
+
 class PrReport(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("executed", C.c_uint32), ("active", C.c_uint32),
                 ("converged", C.c_uint32), ("error_total", C.c_double), ("rank_sum", C.c_double),
@@ -93,6 +105,9 @@ def lib():
         "gbx_ppr_create": (i32, [_vp, _vp, C.POINTER(PrParams), u32, _vp, _vp, C.POINTER(_vp)]),
         "gbx_ppr_info": (i32, [_vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
         "gbx_pr_launch_graph": (i32, [_vp, u32]),
+        "gbx_tuning_get": (i32, [_vp, C.POINTER(Tuning)]),
+        "gbx_tuning_set": (i32, [_vp, C.POINTER(Tuning)]),
+        "gbx_autotune": (i32, [_vp, _vp, C.POINTER(PrParams), u32, u32, C.POINTER(Tuning), C.POINTER(f64)]),
         "gbx_graph_batch_create": (i32, [_vp, u32, _vp, C.POINTER(_vp)]),
         "gbx_graph_batch_info": (i32, [_vp, C.POINTER(u32), _vp]),
     }
@@ -113,7 +128,7 @@ def exported_symbols():
         "gbx_pr_params_default", "gbx_pr_pull", "gbx_pr_create", "gbx_pr_reset", "gbx_pr_launch",
         "gbx_pr_launch_timed", "gbx_pr_read", "gbx_pr_free", "gbx_pr_plan_info", "gbx_spmv_pull",
         "gbx_ppr_create", "gbx_ppr_info", "gbx_pr_launch_graph", "gbx_graph_batch_create",
-        "gbx_graph_batch_info",
+        "gbx_graph_batch_info", "gbx_tuning_get", "gbx_tuning_set", "gbx_autotune",
     ]
 
 
@@ -168,6 +183,22 @@ class Context:
 
     def sync(self):
         self.check(self.L.gbx_sync(self.h))
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        self.check(self.L.gbx_tuning_get(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, t: Tuning):
+        self.check(self.L.gbx_tuning_set(self.h, C.byref(t)))
+
+    def autotune(self, graph, num_vectors=0, iters_per_trial=5, **kw):
+        p = params(**kw)
+        best = Tuning()
+        ms = C.c_double()
+        self.check(self.L.gbx_autotune(self.h, graph.h, C.byref(p), num_vectors, iters_per_trial, C.byref(best),
+                                       C.byref(ms)))
+        return best, ms.value
 
     # -- CSR build
     def csr_build(self, src, dst, w, num_vertices, sort_mode, move_weights=False):

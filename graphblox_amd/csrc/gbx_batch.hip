@@ -329,11 +329,7 @@ int seg_create(gbx_pr *pr)
     GBX_HIP(ctx, hipMemcpyAsync(st->stat.p, hs.data(), hs.size() * sizeof(GStat), hipMemcpyHostToDevice, ctx->stream));
     GBX_HIP(ctx, hipMemcpyAsync(st->coef.p, hc.data(), hc.size() * sizeof(GCoef), hipMemcpyHostToDevice, ctx->stream));
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    unsigned per_cu = 2;
-    if (const char *s = getenv("GBX_BLOCKS_PER_CU")) {
-        int v = atoi(s);
-        if (v >= 1 && v <= 64) per_cu = (unsigned)v;
-    }
+    const unsigned per_cu = ctx->tune.ppr_blocks_per_cu;
     st->grid = std::max(1u, std::min((unsigned)std::max(1, ctx->num_cus) * per_cu, p->n_seg_tiles));
     return GBX_OK;
 }

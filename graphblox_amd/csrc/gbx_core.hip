@@ -1,6 +1,7 @@
 // gbx_core.hip -- context, errors, device buffers and the scan/fill/max
 // primitives shared by the CSR build and the PageRank plan.
 #include <cstdarg>
+#include <cstdlib>
 #include <mutex>
 
 #include "gbx_internal.hpp"
@@ -229,6 +230,55 @@ int max_u32(gbx_ctx *ctx, const uint32_t *a, size_t n, uint32_t *h_max)
     return GBX_OK;
 }
 
+void tuning_defaults(gbx_tuning *t)
+{
+    static const uint32_t thr[7] = {8192, 256, 128, 64, 32, 16, 8};
+    memset(t, 0, sizeof(*t));
+    memcpy(t->bin_thresholds, thr, sizeof(thr));
+    t->hub_chunk = 32768;
+    t->lds_bytes = 159744;  // 156 KiB of the 160 KiB LDS of a CU (1024-thread block, one per CU)
+    t->blocks_per_cu = 1;
+    t->ppr_hub = 1024;
+    t->ppr_chunk = 2048;
+    t->ppr_blocks_per_cu = 2;
+    t->ppr_vectors_per_lane = 1;
+}
+
+bool tuning_valid(const gbx_tuning *t)
+{
+    if (t->bin_thresholds[0] == 0) return false;
+    for (int i = 1; i < 7; i++)
+        if (t->bin_thresholds[i] == 0 || t->bin_thresholds[i] > t->bin_thresholds[i - 1]) return false;
+    if (t->hub_chunk < 256 || t->hub_chunk > (1u << 24)) return false;
+    if (t->lds_bytes > 160 * 1024 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return false;
+    if (t->ppr_hub < 64 || t->ppr_chunk < 256 || t->ppr_blocks_per_cu < 1 || t->ppr_blocks_per_cu > 64)
+        return false;
+    const uint32_t v = t->ppr_vectors_per_lane;
+    return v == 1 || v == 2 || v == 4;
+}
+
+// Environment overrides, read once per context.
+static void tuning_from_env(gbx_tuning *t)
+{
+    gbx_tuning c = *t;
+    if (const char *s = getenv("GBX_BIN_THRESHOLDS"))
+        sscanf(s, "%u,%u,%u,%u,%u,%u,%u", &c.bin_thresholds[0], &c.bin_thresholds[1], &c.bin_thresholds[2],
+               &c.bin_thresholds[3], &c.bin_thresholds[4], &c.bin_thresholds[5], &c.bin_thresholds[6]);
+    struct {
+        const char *name;
+        uint32_t *field;
+    } vars[] = {{"GBX_HUB_CHUNK", &c.hub_chunk},
+                {"GBX_LDS_BYTES", &c.lds_bytes},
+                {"GBX_BLOCKS_PER_CU", &c.blocks_per_cu},
+                {"GBX_PPR_HUB", &c.ppr_hub},
+                {"GBX_PPR_CHUNK", &c.ppr_chunk},
+                {"GBX_PPR_BLOCKS_PER_CU", &c.ppr_blocks_per_cu},
+                {"GBX_PPR_VPL", &c.ppr_vectors_per_lane}};
+    for (auto &v : vars)
+        if (const char *s = getenv(v.name)) *v.field = (uint32_t)strtoul(s, nullptr, 10);
+    if (tuning_valid(&c)) *t = c;
+}
+
 }  // namespace gbx
 
 // ------------------------------------------------------------------ C ABI
@@ -260,7 +310,24 @@ int gbx_init(int device, gbx_ctx **out)
         return rc;
     }
     ctx->err = "no error";
+    gbx::tuning_defaults(&ctx->tune);
+    gbx::tuning_from_env(&ctx->tune);
     *out = ctx;
+    return GBX_OK;
+}
+
+int gbx_tuning_get(const gbx_ctx *ctx, gbx_tuning *t)
+{
+    if (!ctx || !t) return gbx::set_error(nullptr, GBX_ERR_ARG, "gbx_tuning_get: NULL argument");
+    *t = ctx->tune;
+    return GBX_OK;
+}
+
+int gbx_tuning_set(gbx_ctx *ctx, const gbx_tuning *t)
+{
+    if (!ctx || !t) return gbx::set_error(ctx, GBX_ERR_ARG, "gbx_tuning_set: NULL argument");
+    if (!gbx::tuning_valid(t)) return gbx::set_error(ctx, GBX_ERR_ARG, "gbx_tuning_set: invalid tuning");
+    ctx->tune = *t;
     return GBX_OK;
 }
 

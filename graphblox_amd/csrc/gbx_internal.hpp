@@ -70,6 +70,7 @@ struct gbx_ctx {
     hipStream_t stream = nullptr;
     int num_cus = 0;
     std::string err;
+    gbx_tuning tune;
     // scratch reused across calls (grow-only)
     gbx::DevBuf scan_tmp, sort_hist, sort_tmp[3], misc;
 };
@@ -117,6 +118,10 @@ int build_csr_device(gbx_ctx *ctx, uint32_t *src, uint32_t *dst, float *w, uint3
 int graph_from_device_edges(gbx_ctx *ctx, DevBuf &src, DevBuf &dst, DevBuf &w, uint32_t E,
                             uint32_t N, int sort_mode, unsigned flags, gbx_graph **out);
 uint32_t bits_for(uint32_t max_value);
+
+// ---- tuning (gbx_core.hip)
+void tuning_defaults(gbx_tuning *t);
+bool tuning_valid(const gbx_tuning *t);
 
 // ---- plan teardown (gbx_pr.hip)
 void plan_free(gbx_plan *p);

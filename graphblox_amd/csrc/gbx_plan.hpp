@@ -15,9 +15,6 @@ namespace gbx {
 #define GBX_BLOCK 1024
 #endif
 constexpr int kBlock = GBX_BLOCK;
-#ifndef GBX_LDS_DEFAULT
-#define GBX_LDS_DEFAULT 159744
-#endif
 constexpr int kMaxClasses = 9;
 
 enum Kind : int {
@@ -80,6 +77,7 @@ struct gbx_plan {
     uint32_t n_chunks = 0, n_hubs = 0, n_tiles = 0;
     bool has_wt = false, has_wn = false, has_wn16 = false;
     // batch of graphs: explicit graph-pure tiles and per-graph slot ranges
+    uint32_t bins_thr_chunk[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // bins the plan was cut with
     bool segmented = false;
     uint32_t nseg = 0, n_seg_tiles = 0, n_wave_slots = 0, n_seg_hubs = 0, n_seg_chunks = 0;
     gbx::DevBuf tiles, seg_ranges, row_gid, seg_chunks, seg_hub_cnt, seg_hub_part;

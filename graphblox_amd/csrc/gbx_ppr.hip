@@ -61,7 +61,7 @@ struct BatchView {
 }  // namespace gbx
 
 struct gbx_ppr_state {
-    uint32_t k = 0, V = 0;
+    uint32_t k = 0, V = 0, vpl = 1;
     uint32_t hub_threshold = 0, hub_chunk = 0, n_chunks = 0, n_hub_rows = 0, n_tiles = 0;
     uint32_t grid = 0;
     gbx::DevBuf R, C[2], mask, inv_size, scal, partials, state, counter, chunks, hub_cnt, hub_part, out;
@@ -70,64 +70,111 @@ struct gbx_ppr_state {
 
 namespace gbx {
 
-template <int V>
-__device__ __forceinline__ float team_row_sum(uint32_t j, uint32_t end, uint32_t step,
-                                              const uint32_t *__restrict__ col,
-                                              const float *__restrict__ C, uint32_t q)
+// VPL consecutive contributions of one source vertex (one 4/8/16-byte load).
+template <int VPL>
+__device__ __forceinline__ void load_vec(const float *__restrict__ p, float (&x)[VPL])
+{
+    if constexpr (VPL == 1) {
+        x[0] = p[0];
+    } else if constexpr (VPL == 2) {
+        const float2 t = *reinterpret_cast<const float2 *>(p);
+        x[0] = t.x;
+        x[1] = t.y;
+    } else {
+        const float4 t = *reinterpret_cast<const float4 *>(p);
+        x[0] = t.x;
+        x[1] = t.y;
+        x[2] = t.z;
+        x[3] = t.w;
+    }
+}
+
+// Sum of one row's contributions for this lane's VPL vectors, edges
+// j, j+step, ... in order; GBX_UNROLL gathers in flight.
+template <int V, int VPL>
+__device__ __forceinline__ void team_row_sum(uint32_t j, uint32_t end, uint32_t step,
+                                             const uint32_t *__restrict__ col,
+                                             const float *__restrict__ C, uint32_t q, float (&acc)[VPL])
 {
     constexpr uint32_t U = GBX_UNROLL;
-    float acc = 0.0f;
+#pragma unroll
+    for (int v = 0; v < VPL; v++) acc[v] = 0.0f;
+    const float *base = C + (size_t)q * VPL;
     for (; j + (U - 1) * step < end; j += U * step) {
         uint32_t c[U];
-        float x[U];
+        float x[U][VPL];
 #pragma unroll
         for (uint32_t k = 0; k < U; k++) c[k] = col[j + k * step];
 #pragma unroll
-        for (uint32_t k = 0; k < U; k++) x[k] = C[(size_t)c[k] * V + q];
+        for (uint32_t k = 0; k < U; k++) load_vec<VPL>(base + (size_t)c[k] * V, x[k]);
 #pragma unroll
-        for (uint32_t k = 0; k < U; k++) acc += x[k];
+        for (uint32_t k = 0; k < U; k++)
+#pragma unroll
+            for (int v = 0; v < VPL; v++) acc[v] += x[k][v];
     }
-    for (; j < end; j += step) acc += C[(size_t)col[j] * V + q];
-    return acc;
+    for (; j < end; j += step) {
+        float x[VPL];
+        load_vec<VPL>(base + (size_t)col[j] * V, x);
+#pragma unroll
+        for (int v = 0; v < VPL; v++) acc[v] += x[v];
+    }
 }
 
-template <int V>
+// V vectors; a team of L = V/VPL lanes walks one row, lane q holding
+// vectors q*VPL .. q*VPL+VPL-1.
+template <int V, int VPL>
 __global__ void __launch_bounds__(kBlock) ppr_iter_kernel(BPlan P, BatchView A)
 {
     if (A.state->done) return;
-    constexpr uint32_t TPB = kBlock / V;  // teams per block
-    const uint32_t q = threadIdx.x % V, team = threadIdx.x / V;
-    const BScal sc = A.scal_in[q];
-    const double S = sc.sum > 0.0 ? sc.sum : 1.0;
+    constexpr uint32_t L = V / VPL;       // lanes per team
+    constexpr uint32_t TPB = kBlock / L;  // teams per block
+    const uint32_t q = threadIdx.x % L, team = threadIdx.x / L;
     const float fa = 1.0f - A.damp;
-    const float fs = sc.sum > 0.0 ? (float)((double)A.damp / S) : 0.0f;
-    const float fd = (float)(sc.dang / (double)P.N);
     const float uni = (float)(1.0 / (double)P.N);
-    const float isz = A.mask ? A.inv_size[q] : uni;
-    double perr = 0.0, pdang = 0.0, psum = 0.0;
-    auto epi = [&](uint32_t i, float acc) {
-        const size_t o = (size_t)i * V + q;
-        const float r_old = A.R[o];
-        const float t = A.mask ? (((A.mask[i] >> q) & 1ull) ? isz : 0.0f) : uni;
-        const float r_new = fa * t + fs * (acc + fd);
-        A.R[o] = r_new;
+    float fs[VPL], fd[VPL], isz[VPL];
+    double perr[VPL], pdang[VPL], psum[VPL];
+#pragma unroll
+    for (int v = 0; v < VPL; v++) {
+        const BScal sc = A.scal_in[q * VPL + v];
+        fs[v] = sc.sum > 0.0 ? (float)((double)A.damp / sc.sum) : 0.0f;
+        fd[v] = (float)(sc.dang / (double)P.N);
+        isz[v] = A.mask ? A.inv_size[q * VPL + v] : uni;
+        perr[v] = pdang[v] = psum[v] = 0.0;
+    }
+    auto epi = [&](uint32_t i, const float (&acc)[VPL]) {
         const uint32_t od = P.outdeg[i];
-        A.C_out[o] = od ? r_new / (float)od : 0.0f;
-        perr += (double)fabsf(r_new - r_old);
-        if (!od) pdang += (double)r_new;
-        psum += (double)r_new;
+        const uint64_t m = A.mask ? A.mask[i] : 0ull;
+#pragma unroll
+        for (int v = 0; v < VPL; v++) {
+            const uint32_t qv = q * VPL + v;
+            const size_t o = (size_t)i * V + qv;
+            const float r_old = A.R[o];
+            const float t = A.mask ? (((m >> qv) & 1ull) ? isz[v] : 0.0f) : uni;
+            const float r_new = fa * t + fs[v] * (acc[v] + fd[v]);
+            A.R[o] = r_new;
+            A.C_out[o] = od ? r_new / (float)od : 0.0f;
+            perr[v] += (double)fabsf(r_new - r_old);
+            if (!od) pdang[v] += (double)r_new;
+            psum[v] += (double)r_new;
+        }
     };
-    __shared__ float red[kBlock];
+    __shared__ float red[kBlock * VPL];
     __shared__ bool last_chunk;
     for (uint32_t tile = blockIdx.x; tile < P.n_tiles; tile += gridDim.x) {
         if (tile < P.n_chunks) {
             const HubChunk hc = P.chunks[tile];
-            red[threadIdx.x] = team_row_sum<V>(hc.ebeg + team, hc.eend, TPB, P.col, A.C_in, q);
+            float acc[VPL];
+            team_row_sum<V, VPL>(hc.ebeg + team, hc.eend, TPB, P.col, A.C_in, q, acc);
+#pragma unroll
+            for (int v = 0; v < VPL; v++) red[threadIdx.x * VPL + v] = acc[v];
             __syncthreads();
             if (team == 0) {
-                float s = red[q];
-                for (uint32_t t2 = 1; t2 < TPB; t2++) s += red[t2 * V + q];
-                P.hub_part[(size_t)tile * V + q] = s;
+#pragma unroll
+                for (int v = 0; v < VPL; v++) {
+                    float s = red[q * VPL + v];
+                    for (uint32_t t2 = 1; t2 < TPB; t2++) s += red[(t2 * L + q) * VPL + v];
+                    P.hub_part[(size_t)tile * V + q * VPL + v] = s;
+                }
                 __threadfence();
             }
             __syncthreads();
@@ -139,8 +186,12 @@ __global__ void __launch_bounds__(kBlock) ppr_iter_kernel(BPlan P, BatchView A)
             if (last_chunk && team == 0) {
                 __threadfence();
                 const volatile float *hp = P.hub_part;
-                float tot = 0.0f;
-                for (uint32_t k = 0; k < hc.nch; k++) tot += hp[(size_t)(hc.first + k) * V + q];
+                float tot[VPL];
+#pragma unroll
+                for (int v = 0; v < VPL; v++) {
+                    tot[v] = 0.0f;
+                    for (uint32_t k = 0; k < hc.nch; k++) tot[v] += hp[(size_t)(hc.first + k) * V + q * VPL + v];
+                }
                 epi(hc.row, tot);
                 if (q == 0) P.hub_cnt[hc.hub] = 0;
             }
@@ -148,28 +199,46 @@ __global__ void __launch_bounds__(kBlock) ppr_iter_kernel(BPlan P, BatchView A)
         } else {
             const uint32_t row = P.row_begin + (tile - P.n_chunks) * TPB + team;
             if (row < P.N) {
-                const uint32_t beg = P.rp[row], end = P.rp[row + 1];
-                epi(row, team_row_sum<V>(beg, end, 1, P.col, A.C_in, q));
+                float acc[VPL];
+                team_row_sum<V, VPL>(P.rp[row], P.rp[row + 1], 1, P.col, A.C_in, q, acc);
+                epi(row, acc);
             }
         }
     }
-    // per-vector block partials: reduce over teams through LDS, in team order
-    __shared__ double pr[3][kBlock];
-    pr[0][threadIdx.x] = perr;
-    pr[1][threadIdx.x] = pdang;
-    pr[2][threadIdx.x] = psum;
+    // per-vector block partials: teams of a wave first (lanes L apart share
+    // their vectors), then waves through LDS, in a fixed order
+#pragma unroll
+    for (int v = 0; v < VPL; v++) {
+        for (uint32_t off = L; off < 64; off <<= 1) {
+            perr[v] += __shfl_xor(perr[v], off, 64);
+            pdang[v] += __shfl_xor(pdang[v], off, 64);
+            psum[v] += __shfl_xor(psum[v], off, 64);
+        }
+    }
+    constexpr uint32_t W = kBlock / 64;
+    __shared__ double pw[3][W][V];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane < L) {
+#pragma unroll
+        for (int v = 0; v < VPL; v++) {
+            pw[0][wave][lane * VPL + v] = perr[v];
+            pw[1][wave][lane * VPL + v] = pdang[v];
+            pw[2][wave][lane * VPL + v] = psum[v];
+        }
+    }
     __syncthreads();
-    if (team == 0) {
-        double e = pr[0][q], d = pr[1][q], s = pr[2][q];
-        for (uint32_t t2 = 1; t2 < TPB; t2++) {
-            e += pr[0][t2 * V + q];
-            d += pr[1][t2 * V + q];
-            s += pr[2][t2 * V + q];
+    if (threadIdx.x < V) {
+        const uint32_t x = threadIdx.x;
+        double e = 0.0, d = 0.0, s = 0.0;
+        for (uint32_t w = 0; w < W; w++) {
+            e += pw[0][w][x];
+            d += pw[1][w][x];
+            s += pw[2][w][x];
         }
         double *out = A.partials + (size_t)blockIdx.x * 3 * V;
-        out[q] = e;
-        out[V + q] = d;
-        out[2 * V + q] = s;
+        out[x] = e;
+        out[V + x] = d;
+        out[2 * V + x] = s;
     }
 }
 
@@ -327,10 +396,19 @@ static BatchView bview(const gbx_pr *pr, uint32_t t)
 }
 
 template <int V>
-static void launch_v(bool reset, unsigned grid, hipStream_t st, const BPlan &P, const BatchView &A)
+static void launch_v(bool reset, uint32_t vpl, unsigned grid, hipStream_t st, const BPlan &P, const BatchView &A)
 {
     if (reset) ppr_reset_kernel<V><<<grid, kBlock, 0, st>>>(P, A);
-    else ppr_iter_kernel<V><<<grid, kBlock, 0, st>>>(P, A);
+    else if constexpr (V >= 4) {
+        if (vpl == 4) ppr_iter_kernel<V, 4><<<grid, kBlock, 0, st>>>(P, A);
+        else if (vpl == 2) ppr_iter_kernel<V, 2><<<grid, kBlock, 0, st>>>(P, A);
+        else ppr_iter_kernel<V, 1><<<grid, kBlock, 0, st>>>(P, A);
+    } else if constexpr (V == 2) {
+        if (vpl >= 2) ppr_iter_kernel<V, 2><<<grid, kBlock, 0, st>>>(P, A);
+        else ppr_iter_kernel<V, 1><<<grid, kBlock, 0, st>>>(P, A);
+    } else {
+        ppr_iter_kernel<V, 1><<<grid, kBlock, 0, st>>>(P, A);
+    }
 }
 
 static int launch_any(gbx_pr *pr, bool reset)
@@ -344,13 +422,13 @@ static int launch_any(gbx_pr *pr, bool reset)
     const unsigned grid = b->grid;
     hipStream_t st = ctx->stream;
     switch (b->V) {
-    case 1: launch_v<1>(reset, grid, st, P, A); break;
-    case 2: launch_v<2>(reset, grid, st, P, A); break;
-    case 4: launch_v<4>(reset, grid, st, P, A); break;
-    case 8: launch_v<8>(reset, grid, st, P, A); break;
-    case 16: launch_v<16>(reset, grid, st, P, A); break;
-    case 32: launch_v<32>(reset, grid, st, P, A); break;
-    default: launch_v<64>(reset, grid, st, P, A); break;
+    case 1: launch_v<1>(reset, b->vpl, grid, st, P, A); break;
+    case 2: launch_v<2>(reset, b->vpl, grid, st, P, A); break;
+    case 4: launch_v<4>(reset, b->vpl, grid, st, P, A); break;
+    case 8: launch_v<8>(reset, b->vpl, grid, st, P, A); break;
+    case 16: launch_v<16>(reset, b->vpl, grid, st, P, A); break;
+    case 32: launch_v<32>(reset, b->vpl, grid, st, P, A); break;
+    default: launch_v<64>(reset, b->vpl, grid, st, P, A); break;
     }
     GBX_HIP(ctx, hipGetLastError());
     const uint32_t slot = reset ? 0u : ((pr->t + 1) & 1);
@@ -425,10 +503,9 @@ static int build_batch(gbx_pr *pr, uint32_t k, const uint32_t *seed_ptr, const u
     uint32_t V = 1;
     while (V < k) V <<= 1;
     b->V = V;
-    b->hub_threshold = 1024;
-    b->hub_chunk = 2048;
-    if (const char *s = getenv("GBX_PPR_HUB")) b->hub_threshold = std::max(64u, (uint32_t)atoi(s));
-    if (const char *s = getenv("GBX_PPR_CHUNK")) b->hub_chunk = std::max(256u, (uint32_t)atoi(s));
+    b->vpl = std::min(ctx->tune.ppr_vectors_per_lane, V);
+    b->hub_threshold = ctx->tune.ppr_hub;
+    b->hub_chunk = ctx->tune.ppr_chunk;
     GBX_TRY(plan_rows_at_least(ctx, p, b->hub_threshold, &b->n_hub_rows));
     std::vector<uint32_t> hrp;
     GBX_TRY(plan_row_ptr_prefix(ctx, p, b->n_hub_rows, hrp));
@@ -449,13 +526,9 @@ static int build_batch(gbx_pr *pr, uint32_t k, const uint32_t *seed_ptr, const u
         }
     }
     b->n_chunks = (uint32_t)chunks.size();
-    const uint32_t TPB = kBlock / V;
+    const uint32_t TPB = kBlock / (V / b->vpl);
     b->n_tiles = b->n_chunks + (p->N - b->n_hub_rows + TPB - 1) / TPB;
-    unsigned per_cu = 2;
-    if (const char *s = getenv("GBX_BLOCKS_PER_CU")) {
-        int v = atoi(s);
-        if (v >= 1 && v <= 64) per_cu = (unsigned)v;
-    }
+    const unsigned per_cu = ctx->tune.ppr_blocks_per_cu;
     b->grid = std::max(1u, std::min((unsigned)std::max(1, ctx->num_cus) * per_cu, b->n_tiles));
     const size_t nv = (size_t)p->N * V * 4;
     GBX_TRY(dev_alloc(ctx, b->R, nv));
@@ -559,7 +632,7 @@ int gbx_ppr_info(const gbx_pr *pr, uint32_t *num_vectors, uint32_t *vector_width
 {
     if (!pr || !pr->batch) return set_error(nullptr, GBX_ERR_ARG, "gbx_ppr_info: not a batched object");
     if (num_vectors) *num_vectors = pr->batch->k;
-    if (vector_width) *vector_width = pr->batch->V;
+    if (vector_width) *vector_width = pr->batch->V / pr->batch->vpl;
     if (hub_rows) *hub_rows = pr->batch->n_hub_rows;
     if (hub_chunks) *hub_chunks = pr->batch->n_chunks;
     return GBX_OK;

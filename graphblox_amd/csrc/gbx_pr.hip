@@ -400,29 +400,23 @@ __global__ void fill_f32(float *p, float v, uint32_t n)
         p[i] = v;
 }
 
-// Degree thresholds of the bins: hub, then 64, 32, 16, 8, 4, 2 lanes per row
-// (rows below the last threshold use 1 lane).  Overridable with
-// GBX_BIN_THRESHOLDS="hub,t64,t32,t16,t8,t4,t2" and GBX_HUB_CHUNK.
+// Degree thresholds of the bins (hub, then 64, 32, 16, 8, 4, 2 lanes per
+// row; rows below the last threshold use 1 lane) and the hub chunk size, from
+// the context's tuning.
 struct BinConfig {
-    uint32_t thr[7] = {8192, 256, 128, 64, 32, 16, 8};
-    uint32_t hub_chunk = 32768;
+    uint32_t thr[7];
+    uint32_t hub_chunk;
+    bool operator==(const BinConfig &o) const
+    {
+        return hub_chunk == o.hub_chunk && memcmp(thr, o.thr, sizeof(thr)) == 0;
+    }
 };
 
-static BinConfig bin_config()
+static BinConfig bin_config(const gbx_ctx *ctx)
 {
     BinConfig b;
-    if (const char *s = getenv("GBX_BIN_THRESHOLDS")) {
-        uint32_t v[7];
-        if (sscanf(s, "%u,%u,%u,%u,%u,%u,%u", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6]) == 7) {
-            bool ok = v[0] > 0;
-            for (int i = 1; i < 7; i++) ok = ok && v[i] <= v[i - 1] && v[i] > 0;
-            if (ok) memcpy(b.thr, v, sizeof(v));
-        }
-    }
-    if (const char *s = getenv("GBX_HUB_CHUNK")) {
-        uint32_t c = (uint32_t)atoi(s);
-        if (c >= 256 && c <= (1u << 20)) b.hub_chunk = c;
-    }
+    memcpy(b.thr, ctx->tune.bin_thresholds, sizeof(b.thr));
+    b.hub_chunk = ctx->tune.hub_chunk;
     return b;
 }
 
@@ -573,7 +567,9 @@ static int build_plan(gbx_ctx *ctx, gbx_graph *g, int direction, gbx_plan **out)
         if (hipGetLastError() != hipSuccess) return fail(set_error(ctx, GBX_ERR_HIP, "plan_edges"));
     }
     // bins
-    const BinConfig bc = bin_config();
+    const BinConfig bc = bin_config(ctx);
+    p->bins_thr_chunk[7] = bc.hub_chunk;
+    memcpy(p->bins_thr_chunk, bc.thr, sizeof(bc.thr));
     DevBuf dthr, dbound;
     if ((rc = dev_alloc(ctx, dthr, 64)) || (rc = dev_alloc(ctx, dbound, 64))) return fail(rc);
     uint32_t bound[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -661,7 +657,20 @@ int ensure_plan(gbx_ctx *ctx, gbx_graph *g, int direction, gbx_plan **out)
 {
     if (direction != GBX_PULL_IN && direction != GBX_PULL_FWD)
         return set_error(ctx, GBX_ERR_ARG, "direction %d not in {0,1}", direction);
-    if (!g->plan[direction]) GBX_TRY(build_plan(ctx, g, direction, &g->plan[direction]));
+    gbx_plan *&slot = g->plan[direction];
+    if (slot) {
+        // rebuild when the tuning's bins differ from the ones the plan was cut with
+        const BinConfig bc = bin_config(ctx);
+        BinConfig built;
+        memcpy(built.thr, slot->bins_thr_chunk, sizeof(built.thr));
+        built.hub_chunk = slot->bins_thr_chunk[7];
+        if (!(built == bc)) {
+            GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            plan_free(slot);
+            slot = nullptr;
+        }
+    }
+    if (!slot) GBX_TRY(build_plan(ctx, g, direction, &slot));
     *out = g->plan[direction];
     return GBX_OK;
 }
@@ -758,11 +767,7 @@ static PlanView plan_view(const gbx_plan *p, const float *w)
 
 static unsigned iter_grid(gbx_ctx *ctx, const gbx_plan *p)
 {
-    unsigned per_cu = 1;
-    if (const char *s = getenv("GBX_BLOCKS_PER_CU")) {
-        int v = atoi(s);
-        if (v >= 1 && v <= 64) per_cu = (unsigned)v;
-    }
+    const unsigned per_cu = ctx->tune.blocks_per_cu;
     unsigned g = (unsigned)std::max(1, ctx->num_cus) * per_cu;
     return std::max(1u, std::min(g, p->n_tiles));
 }
@@ -842,10 +847,9 @@ static int launch_iter(gbx_pr *pr)
 
 // Contributions kept in LDS per block: GBX_LDS_BYTES (default below), a
 // multiple of 16 bytes, never more than the vector itself.
-static uint32_t hot_count(uint32_t N, size_t elem)
+static uint32_t hot_count(const gbx_ctx *ctx, uint32_t N, size_t elem)
 {
-    size_t bytes = GBX_LDS_DEFAULT;
-    if (const char *s = getenv("GBX_LDS_BYTES")) bytes = (size_t)strtoull(s, nullptr, 10);
+    size_t bytes = ctx->tune.lds_bytes;
     if (bytes > 160 * 1024) bytes = 160 * 1024;
     size_t n = bytes / elem;
     const size_t align = 16 / elem;
@@ -924,7 +928,7 @@ int gbx_pr_create(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, gbx_pr **o
     pr->kind = p->mode == GBX_PR_REF ? K_PR_REF : (p->weighted ? K_PR_STOCH_W : K_PR_STOCH);
     pr->grid = iter_grid(ctx, plan);
     pr->elem = p->precision == GBX_BF16 ? 2 : 4;
-    pr->hot = hot_count(g->N, pr->elem);
+    pr->hot = hot_count(ctx, g->N, pr->elem);
     const size_t nb = (size_t)g->N * 4;
     const size_t cb = (size_t)g->N * pr->elem + 16;
     int rc = GBX_OK;

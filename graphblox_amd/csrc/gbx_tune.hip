@@ -1,0 +1,109 @@
+// gbx_tune.hip -- autotuner for the pull kernels' launch shapes.
+//
+// Coordinate descent from the context's current tuning: for each field in
+// turn, try its candidate values on short timed runs of the real kernel on
+// the caller's graph (HIP events around every iteration), keep a value only
+// when it is faster by more than 1 %, and repeat the sweep once.  Bin
+// thresholds and hub chunks change the plan, which is rebuilt by the next
+// object creation; the other fields only change launches.
+#include <vector>
+
+#include "gbx_plan.hpp"
+
+namespace gbx {
+
+static int trial(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, uint32_t k, uint32_t iters,
+                 const gbx_tuning &t, double *ms)
+{
+    ctx->tune = t;
+    gbx_pr_params q = *p;
+    q.stop = GBX_STOP_NONE;
+    q.max_iters = iters + 2;
+    if (k) q.mode = GBX_PR_STOCHASTIC;
+    gbx_pr *pr = nullptr;
+    int rc = k ? gbx_ppr_create(ctx, g, &q, k, nullptr, nullptr, &pr) : gbx_pr_create(ctx, g, &q, &pr);
+    if (rc) return rc;
+    rc = gbx_pr_reset(pr, nullptr);
+    if (!rc) rc = gbx_pr_launch(pr, 2);
+    if (!rc) rc = gbx_pr_launch_timed(pr, iters, ms);
+    gbx_pr_free(pr);
+    return rc;
+}
+
+}  // namespace gbx
+
+using namespace gbx;
+
+extern "C" int gbx_autotune(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, uint32_t num_vectors,
+                            uint32_t iters_per_trial, gbx_tuning *best, double *best_ms)
+{
+    if (!ctx || !g || !p) return set_error(ctx, GBX_ERR_ARG, "gbx_autotune: NULL argument");
+    if (iters_per_trial == 0) iters_per_trial = 5;
+    const gbx_tuning start = ctx->tune;
+    gbx_tuning cur = start;
+    double cur_ms = 0.0;
+    int rc = trial(ctx, g, p, num_vectors, iters_per_trial, cur, &cur_ms);
+    if (rc) {
+        ctx->tune = start;
+        return rc;
+    }
+    struct Field {
+        uint32_t *(*get)(gbx_tuning &);
+        std::vector<uint32_t> values;
+    };
+    std::vector<Field> fields;
+    std::vector<std::vector<uint32_t>> lane_sets = {
+        {256, 128, 64, 32, 16, 8}, {512, 256, 128, 64, 32, 16}, {128, 64, 32, 16, 8, 4}};
+    if (num_vectors == 0) {
+        fields.push_back({[](gbx_tuning &t) { return &t.bin_thresholds[0]; }, {2048, 4096, 8192, 16384, 32768}});
+        fields.push_back({[](gbx_tuning &t) { return &t.hub_chunk; }, {8192, 16384, 32768, 65536}});
+        fields.push_back({[](gbx_tuning &t) { return &t.lds_bytes; }, {0, 65536, 159744}});
+        fields.push_back({[](gbx_tuning &t) { return &t.blocks_per_cu; }, {1, 2}});
+    } else {
+        fields.push_back({[](gbx_tuning &t) { return &t.ppr_vectors_per_lane; }, {1, 2, 4}});
+        fields.push_back({[](gbx_tuning &t) { return &t.ppr_hub; }, {256, 1024, 4096}});
+        fields.push_back({[](gbx_tuning &t) { return &t.ppr_chunk; }, {1024, 2048, 8192}});
+        fields.push_back({[](gbx_tuning &t) { return &t.ppr_blocks_per_cu; }, {1, 2, 4}});
+    }
+    for (int pass = 0; pass < 2; pass++) {
+        if (num_vectors == 0) {
+            // the lane thresholds move together
+            for (const auto &ls : lane_sets) {
+                gbx_tuning t = cur;
+                for (int i = 0; i < 6; i++) t.bin_thresholds[i + 1] = std::min(ls[i], t.bin_thresholds[0]);
+                if (!tuning_valid(&t) || memcmp(&t, &cur, sizeof(t)) == 0) continue;
+                double ms = 0.0;
+                if ((rc = trial(ctx, g, p, num_vectors, iters_per_trial, t, &ms))) break;
+                if (ms < cur_ms * 0.99) {
+                    cur = t;
+                    cur_ms = ms;
+                }
+            }
+            if (rc) break;
+        }
+        for (auto &f : fields) {
+            for (uint32_t v : f.values) {
+                gbx_tuning t = cur;
+                if (*f.get(t) == v) continue;
+                *f.get(t) = v;
+                if (!tuning_valid(&t)) continue;
+                double ms = 0.0;
+                if ((rc = trial(ctx, g, p, num_vectors, iters_per_trial, t, &ms))) break;
+                if (ms < cur_ms * 0.99) {
+                    cur = t;
+                    cur_ms = ms;
+                }
+            }
+            if (rc) break;
+        }
+        if (rc) break;
+    }
+    if (rc) {
+        ctx->tune = start;
+        return rc;
+    }
+    ctx->tune = cur;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+    return GBX_OK;
+}

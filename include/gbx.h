@@ -208,8 +208,8 @@ int gbx_pr_free(gbx_pr *pr);
  * (vertex-major, original order) and rep points to num_vectors reports. */
 int gbx_ppr_create(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, uint32_t num_vectors,
                    const uint32_t *seed_ptr, const uint32_t *seeds, gbx_pr **out);
-/* Shape of a batched object: vectors, lanes per row (power of two >= k),
- * hub rows and hub chunks of its tiling. */
+/* Shape of a batched object: vectors, lanes per row (the power of two >= k
+ * divided by the vectors held per lane), hub rows and hub chunks. */
 int gbx_ppr_info(const gbx_pr *pr, uint32_t *num_vectors, uint32_t *vector_width,
                  uint32_t *hub_rows, uint32_t *hub_chunks);
 
@@ -231,6 +231,35 @@ int gbx_graph_batch_create(gbx_ctx *ctx, uint32_t num_graphs, gbx_graph *const *
 /* Number of member graphs and their vertex offsets [num_graphs + 1]
  * (a single graph reports 1 and {0, N}). */
 int gbx_graph_batch_info(const gbx_graph *g, uint32_t *num_graphs, uint32_t *vertex_offsets);
+
+/* ---------------------------------------------------------------- tuning */
+/* Launch shapes of the pull kernels.  Defaults come from measurements on
+ * MI355X and can be overridden per context (and at gbx_init from the
+ * environment: GBX_BIN_THRESHOLDS, GBX_HUB_CHUNK, GBX_LDS_BYTES,
+ * GBX_BLOCKS_PER_CU, GBX_PPR_HUB, GBX_PPR_CHUNK, GBX_PPR_BLOCKS_PER_CU,
+ * GBX_PPR_VPL).  A change applies to plans and objects created afterwards;
+ * results stay within the same tolerances for every setting. */
+typedef struct gbx_tuning {
+    uint32_t bin_thresholds[7];    /* degree thresholds: hub, then 64, 32, 16, 8, 4, 2 lanes per row */
+    uint32_t hub_chunk;            /* edges per hub chunk (one block each) */
+    uint32_t lds_bytes;            /* LDS bytes per block for the hottest contributions (0 = off) */
+    uint32_t blocks_per_cu;        /* persistent blocks per CU, single-vector kernels */
+    uint32_t ppr_hub;              /* batched form: rows of at least this degree are chunked */
+    uint32_t ppr_chunk;            /* batched form: edges per chunk */
+    uint32_t ppr_blocks_per_cu;    /* batched form and graph batches: blocks per CU */
+    uint32_t ppr_vectors_per_lane; /* batched form: contributions per lane (1, 2 or 4) */
+    uint32_t reserved[4];
+} gbx_tuning;
+
+int gbx_tuning_get(const gbx_ctx *ctx, gbx_tuning *t);
+int gbx_tuning_set(gbx_ctx *ctx, const gbx_tuning *t);
+
+/* Autotuner: times short runs of the pull kernel (num_vectors = 0) or of
+ * the batched form (num_vectors > 0) on graph g over a small search of the
+ * tuning fields (coordinate descent from the current setting), installs the
+ * fastest in ctx and returns it in *best with its mean iteration time. */
+int gbx_autotune(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, uint32_t num_vectors,
+                 uint32_t iters_per_trial, gbx_tuning *best, double *best_ms);
 
 /* Plan statistics: counts of rows per degree class and edges per class
  * (for reporting and the autotuner); arrays of length *n_classes (<= 16). */
