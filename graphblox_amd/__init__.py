@@ -105,6 +105,8 @@ def lib():
         "gbx_ppr_create": (i32, [_vp, _vp, C.POINTER(PrParams), u32, _vp, _vp, C.POINTER(_vp)]),
         "gbx_ppr_info": (i32, [_vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
         "gbx_pr_launch_graph": (i32, [_vp, u32]),
+        "gbx_pr_adjoint": (i32, [_vp, _vp, C.POINTER(PrParams), _vp, _vp, _vp, _vp, _vp, _vp,
+                                 C.POINTER(PrReport), C.POINTER(PrReport)]),
         "gbx_tuning_get": (i32, [_vp, C.POINTER(Tuning)]),
         "gbx_tuning_set": (i32, [_vp, C.POINTER(Tuning)]),
         "gbx_autotune": (i32, [_vp, _vp, C.POINTER(PrParams), u32, u32, C.POINTER(Tuning), C.POINTER(f64)]),
@@ -129,6 +131,7 @@ def exported_symbols():
         "gbx_pr_launch_timed", "gbx_pr_read", "gbx_pr_free", "gbx_pr_plan_info", "gbx_spmv_pull",
         "gbx_ppr_create", "gbx_ppr_info", "gbx_pr_launch_graph", "gbx_graph_batch_create",
         "gbx_graph_batch_info", "gbx_tuning_get", "gbx_tuning_set", "gbx_autotune",
+        "gbx_pr_adjoint",
     ]
 
 
@@ -307,6 +310,22 @@ class Graph:
         tp = _f32(teleport)
         self.ctx.check(self.ctx.L.gbx_pr_pull(self.ctx.h, self.h, C.byref(p), _ptr(tp), _ptr(ranks), C.byref(rep)))
         return ranks, rep.as_dict()
+
+    def adjoint(self, loss_weights, teleport=None, iterations=300, epsilon=1e-7, damp=0.85, stop=STOP_L1):
+        """Forward weighted stochastic PageRank p and its adjoint: returns
+        p, lambda, dL/dW (forward-CSR edge order), dL/dt and both reports."""
+        p = params(iterations, epsilon, PR_STOCHASTIC, stop, damp, PULL_IN, True)
+        N, E = self.N, self.E
+        out = dict(p=np.empty(N, np.float32), lam=np.empty(N, np.float32), grad_w=np.empty(E, np.float32),
+                   grad_t=np.empty(N, np.float32))
+        fr, ar = PrReport(), PrReport()
+        g = _f32(loss_weights)
+        tp = _f32(teleport)
+        self.ctx.check(self.ctx.L.gbx_pr_adjoint(self.ctx.h, self.h, C.byref(p), _ptr(tp), _ptr(g), _ptr(out["p"]),
+                                                 _ptr(out["lam"]), _ptr(out["grad_w"]), _ptr(out["grad_t"]),
+                                                 C.byref(fr), C.byref(ar)))
+        out["fwd_report"], out["adj_report"] = fr.as_dict(), ar.as_dict()
+        return out
 
     def spmv(self, iterations=1, weighted=False, direction=PULL_IN, x=None):
         xo = np.empty(self.N, np.float32)

@@ -23,6 +23,7 @@ enum Kind : int {
     K_PR_STOCH_W = 2,  // stochastic, row-normalised edge weights
     K_SPMV = 3,        // y += 0.0001f * x  (SPMV.c:177)
     K_SPMV_W = 4,      // y += w * x
+    K_ADJ = 5,         // adjoint of the weighted stochastic fixed point (forward-CSR rows)
 };
 
 struct ClassDesc {
@@ -70,12 +71,12 @@ struct Partial {
 struct gbx_plan {
     int direction = 0;
     uint32_t N = 0, E = 0;
-    gbx::DevBuf perm, inv, rp, col, wt, wn, wn16, outdeg, chunks, hub_cnt, hub_part;
+    gbx::DevBuf perm, inv, rp, col, wt, wn, wn16, wr, outdeg, chunks, hub_cnt, hub_part;
     uint32_t nclass = 0;
     gbx::ClassDesc cls[gbx::kMaxClasses];
     uint64_t cls_edges[gbx::kMaxClasses];
     uint32_t n_chunks = 0, n_hubs = 0, n_tiles = 0;
-    bool has_wt = false, has_wn = false, has_wn16 = false;
+    bool has_wt = false, has_wn = false, has_wn16 = false, has_wr = false;
     // batch of graphs: explicit graph-pure tiles and per-graph slot ranges
     uint32_t bins_thr_chunk[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // bins the plan was cut with
     bool segmented = false;

@@ -127,23 +127,28 @@ struct PrEpi {
     {
         const float r_old = ld_stream(r + i);
         float r_new;
+        const uint32_t od = ld_stream(outdeg + i);
         if constexpr (KIND == K_PR_REF) {
             r_new = base + damp * acc;  // pageRank.c:324 (no contraction: -ffp-contract=off)
+        } else if constexpr (KIND == K_ADJ) {
+            // lambda[u] = g[u] + d (sum_v P[u,v] lambda[v] + [u dangling] Lambda/N)
+            r_new = ld_stream(tele + i) + fs * (acc + (od ? 0.0f : fd));
         } else {
             const float ti = tele ? ld_stream(tele + i) : uni;
             r_new = fa * ti + fs * (acc + fd);
         }
         st_stream(r + i, r_new);
-        const uint32_t od = ld_stream(outdeg + i);
         float cn;
-        if constexpr (KIND == K_PR_STOCH_W) cn = od ? r_new : 0.0f;
+        if constexpr (KIND == K_ADJ) cn = r_new;
+        else if constexpr (KIND == K_PR_STOCH_W) cn = od ? r_new : 0.0f;
         else cn = od ? r_new / (float)od : 0.0f;  // pageRank.c:297-301
         store_val(c_out, i, cn);
         const float diff = fabsf(r_new - r_old);
         if constexpr (KIND == K_PR_REF) part.err += (double)diff * invN;
         else part.err += (double)diff;
         part.active += ((double)diff >= eps) ? 1u : 0u;
-        if (!od) part.dang += (double)r_new;
+        if constexpr (KIND == K_ADJ) part.dang += (double)fabsf(r_new);  // |lambda|_1 for the relative stop
+        else if (!od) part.dang += (double)r_new;
         part.sum += (double)r_new;
     }
 };
@@ -171,14 +176,18 @@ __global__ void __launch_bounds__(kBlock) pr_iter_kernel(PlanView P, IterView A)
     epi.invN = 1.0 / (double)P.N;
     epi.eps = A.eps;
     epi.uni = (float)(1.0 / (double)P.N);
-    if constexpr (KIND != K_PR_REF) {
+    if constexpr (KIND == K_ADJ) {
+        epi.fa = 0.0f;
+        epi.fs = A.damp;
+        epi.fd = (float)(cur->sum / (double)P.N);  // Lambda / N
+    } else if constexpr (KIND != K_PR_REF) {
         const double S = cur->sum > 0.0 ? cur->sum : 1.0;
         epi.fa = 1.0f - A.damp;
         epi.fs = (float)((double)A.damp / S);
         epi.fd = (float)(cur->dang / (double)P.N);
     }
     epi.part = {0.0, 0.0, 0.0, 0u, 0u};
-    if constexpr (KIND == K_PR_STOCH_W) {
+    if constexpr (KIND == K_PR_STOCH_W || KIND == K_ADJ) {
         GatherCW<T, LDS> g{c_in, reinterpret_cast<const T *>(P.wt), A.hot};
         tile_loop<Acc>(P, g, epi);
     } else {
@@ -197,7 +206,7 @@ __global__ void __launch_bounds__(kBlock) pr_iter_kernel(PlanView P, IterView A)
         s.iters = iters + 1;
         uint32_t done = 0;
         if (stop == GBX_STOP_ACTIVE) done = tot.active == 0;
-        else if (stop == GBX_STOP_L1) done = tot.err < eps;
+        else if (stop == GBX_STOP_L1) done = KIND == K_ADJ ? tot.err < eps * tot.dang : tot.err < eps;
         s.done = done;
         s.conv = done;
         *nxt = s;
@@ -217,10 +226,12 @@ __global__ void __launch_bounds__(kBlock) pr_reset_kernel(PlanView P, IterView A
         A.r[i] = r0;
         const uint32_t od = P.outdeg[i];
         float c0;
-        if (KIND == K_PR_STOCH_W) c0 = od ? r0 : 0.0f;
+        if (KIND == K_ADJ) c0 = r0;
+        else if (KIND == K_PR_STOCH_W) c0 = od ? r0 : 0.0f;
         else c0 = od ? r0 / (float)od : 0.0f;
         store_val(c_out, i, c0);
-        if (!od) part.dang += (double)r0;
+        if (KIND == K_ADJ) part.dang += (double)fabsf(r0);
+        else if (!od) part.dang += (double)r0;
         part.sum += (double)r0;
     }
     Slot *s0 = A.slots;
@@ -247,7 +258,7 @@ __global__ void pr_finalize_kernel(const float *__restrict__ r, const uint32_t *
     const double S = slot->sum > 0.0 ? slot->sum : 1.0;
     const float fS = (float)S;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
-        out[perm[i]] = ref ? r[i] / fN : r[i] / fS;
+        out[perm[i]] = ref == 2 ? r[i] : (ref ? r[i] / fN : r[i] / fS);
 }
 
 // ------------------------------------------------------------------ SpMV
@@ -362,6 +373,40 @@ __global__ void normalise_weights(const uint32_t *__restrict__ col, const float 
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < E; j += gridDim.x * blockDim.x) {
         const float s = S[perm[col[j]]];
         wn[j] = s > 0.0f ? (float)((double)wt[j] / (double)s) : 0.0f;
+    }
+}
+
+// Per-edge transition weight in forward-CSR order: w[e] / S[src[e]].
+__global__ void row_normalise_fwd(const uint32_t *__restrict__ src, const float *__restrict__ w,
+                                  const float *__restrict__ S, uint32_t E, float *__restrict__ out)
+{
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        const float s = S[src[e] & kIdMask];
+        out[e] = s > 0.0f ? (float)((double)w[e] / (double)s) : 0.0f;
+    }
+}
+
+// Per-edge values of the row CSR placed at their plan positions.
+__global__ void plan_edge_values(const uint32_t *__restrict__ rsrc, const uint32_t *__restrict__ ridx,
+                                 const uint32_t *__restrict__ inv, const uint32_t *__restrict__ rp, uint32_t E,
+                                 const float *__restrict__ in, float *__restrict__ out)
+{
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        const uint32_t v = rsrc[e] & kIdMask;
+        out[rp[inv[v]] + (e - ridx[v])] = in[e];
+    }
+}
+
+// dL/dW[e] for forward-CSR edge e = (u, v):  d p[u] (lambda[v] - (P lambda)[u]) / S[u]
+__global__ void adjoint_edge_grad(const uint32_t *__restrict__ src, const uint32_t *__restrict__ dst, uint32_t E,
+                                  const float *__restrict__ p, const float *__restrict__ lam,
+                                  const float *__restrict__ plam, const float *__restrict__ S, float damp,
+                                  float *__restrict__ grad)
+{
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        const uint32_t u = src[e] & kIdMask, v = dst[e] & kIdMask;
+        const double s = (double)S[u];
+        grad[e] = s > 0.0 ? (float)((double)damp * (double)p[u] * ((double)lam[v] - (double)plam[u]) / s) : 0.0f;
     }
 }
 
@@ -712,6 +757,35 @@ static int ensure_normalised_bf16(gbx_ctx *ctx, gbx_graph *g, gbx_plan *p)
     return GBX_OK;
 }
 
+// Row-normalised weights of a forward-direction plan (rows = sources): the
+// transition matrix P itself, for the adjoint's pull P lambda.
+static int ensure_row_normalised(gbx_ctx *ctx, gbx_graph *g, gbx_plan *p)
+{
+    if (p->has_wr) return GBX_OK;
+    if (p->direction != GBX_PULL_FWD) return set_error(ctx, GBX_ERR_STATE, "row weights need the forward plan");
+    if (!g->weighted) return set_error(ctx, GBX_ERR_STATE, "adjoint needs a weighted graph");
+    DevBuf S, wfn;
+    GBX_TRY(dev_alloc(ctx, S, (size_t)p->N * 4));
+    GBX_TRY(dev_alloc(ctx, wfn, (size_t)p->E * 4 + 4));
+    GBX_TRY(dev_alloc(ctx, p->wr, (size_t)p->E * 4 + 4));
+    out_weight_sums<<<grid_for((size_t)p->N * 64, 256, 16384), 256, 0, ctx->stream>>>(
+        g->fwd.out_degree.as<uint32_t>(), g->fwd.edges_idx.as<uint32_t>(), g->fwd.w.as<float>(), p->N,
+        S.as<float>());
+    GBX_HIP(ctx, hipGetLastError());
+    if (p->E) {
+        row_normalise_fwd<<<grid_for(p->E, 256, 16384), 256, 0, ctx->stream>>>(
+            g->fwd.src.as<uint32_t>(), g->fwd.w.as<float>(), S.as<float>(), p->E, wfn.as<float>());
+        GBX_HIP(ctx, hipGetLastError());
+        plan_edge_values<<<grid_for(p->E, 256, 16384), 256, 0, ctx->stream>>>(
+            g->fwd.src.as<uint32_t>(), g->fwd.edges_idx.as<uint32_t>(), p->inv.as<uint32_t>(),
+            p->rp.as<uint32_t>(), p->E, wfn.as<float>(), p->wr.as<float>());
+        GBX_HIP(ctx, hipGetLastError());
+    }
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    p->has_wr = true;
+    return GBX_OK;
+}
+
 void plan_free(gbx_plan *p) { delete p; }
 
 int plan_rows_at_least(gbx_ctx *ctx, const gbx_plan *p, uint32_t threshold, uint32_t *count)
@@ -811,6 +885,7 @@ static void launch_pr_kernel(bool lds, unsigned grid, size_t shm, hipStream_t st
 
 static const float *plan_weights(const gbx_pr *pr)
 {
+    if (pr->kind == K_ADJ) return pr->plan->wr.as<float>();
     if (pr->kind != K_PR_STOCH_W) return nullptr;
     if (pr->elem == 2) return reinterpret_cast<const float *>(pr->plan->wn16.p);
     return pr->plan->wn.as<float>();
@@ -831,6 +906,7 @@ static int launch_iter(gbx_pr *pr)
         switch (pr->kind) {
         case K_PR_REF: launch_pr_kernel<K_PR_REF, float, AccF32>(lds, g, shm, st, P, A); break;
         case K_PR_STOCH: launch_pr_kernel<K_PR_STOCH, float, AccF32>(lds, g, shm, st, P, A); break;
+        case K_ADJ: launch_pr_kernel<K_ADJ, float, AccF32>(lds, g, shm, st, P, A); break;
         default: launch_pr_kernel<K_PR_STOCH_W, float, AccF32>(lds, g, shm, st, P, A); break;
         }
     } else {
@@ -878,7 +954,14 @@ void gbx_pr_params_default(gbx_pr_params *p)
     p->weighted = 0;
 }
 
+static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, int kind, gbx_pr **out);
+
 int gbx_pr_create(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, gbx_pr **out)
+{
+    return pr_create_kind(ctx, g, p, -1, out);
+}
+
+static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, int kind_override, gbx_pr **out)
 {
     if (!ctx || !g || !p || !out) return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: NULL argument");
     *out = nullptr;
@@ -915,7 +998,9 @@ int gbx_pr_create(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, gbx_pr **o
         *out = pr;
         return GBX_OK;
     }
-    if (p->weighted) {
+    if (kind_override == K_ADJ) {
+        GBX_TRY(ensure_row_normalised(ctx, g, plan));
+    } else if (p->weighted) {
         if (p->precision == GBX_BF16) GBX_TRY(ensure_normalised_bf16(ctx, g, plan));
         else GBX_TRY(ensure_normalised(ctx, g, plan));
     }
@@ -926,6 +1011,7 @@ int gbx_pr_create(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, gbx_pr **o
     pr->plan = plan;
     pr->p = *p;
     pr->kind = p->mode == GBX_PR_REF ? K_PR_REF : (p->weighted ? K_PR_STOCH_W : K_PR_STOCH);
+    if (kind_override >= 0) pr->kind = kind_override;
     pr->grid = iter_grid(ctx, plan);
     pr->elem = p->precision == GBX_BF16 ? 2 : 4;
     pr->hot = hot_count(ctx, g->N, pr->elem);
@@ -966,7 +1052,7 @@ int gbx_pr_reset(gbx_pr *pr, const float *teleport)
         return pr->batch ? ppr_reset(pr) : seg_reset(pr);
     }
     if (teleport) {
-        if (pr->p.mode == GBX_PR_REF)
+        if (pr->p.mode == GBX_PR_REF && pr->kind != K_ADJ)
             return set_error(ctx, GBX_ERR_ARG, "gbx_pr_reset: teleport needs STOCHASTIC mode");
         DevBuf tmp;
         GBX_TRY(dev_alloc(ctx, tmp, (size_t)N * 4));
@@ -990,6 +1076,7 @@ int gbx_pr_reset(gbx_pr *pr, const float *teleport)
         switch (pr->kind) {
         case K_PR_REF: pr_reset_kernel<K_PR_REF, float><<<grid, kBlock, 0, st>>>(P, A); break;
         case K_PR_STOCH: pr_reset_kernel<K_PR_STOCH, float><<<grid, kBlock, 0, st>>>(P, A); break;
+        case K_ADJ: pr_reset_kernel<K_ADJ, float><<<grid, kBlock, 0, st>>>(P, A); break;
         default: pr_reset_kernel<K_PR_STOCH_W, float><<<grid, kBlock, 0, st>>>(P, A); break;
         }
     } else {
@@ -1108,8 +1195,8 @@ int gbx_pr_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep)
     const uint32_t N = pr->g->N;
     const Slot *slot = pr->slots.as<Slot>() + (pr->t & 1);
     pr_finalize_kernel<<<grid_for(N, 256, 16384), 256, 0, ctx->stream>>>(
-        pr->r.as<float>(), pr->plan->perm.as<uint32_t>(), N, slot, pr->kind == K_PR_REF,
-        pr->out.as<float>());
+        pr->r.as<float>(), pr->plan->perm.as<uint32_t>(), N, slot,
+        pr->kind == K_ADJ ? 2 : (pr->kind == K_PR_REF ? 1 : 0), pr->out.as<float>());
     GBX_HIP(ctx, hipGetLastError());
     GBX_HIP(ctx, hipMemcpyAsync(pr->h_slot, slot, sizeof(Slot), hipMemcpyDeviceToHost, ctx->stream));
     if (ranks_out)
@@ -1173,6 +1260,90 @@ int gbx_pr_pull(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, const float 
     }
     if (!rc) rc = gbx_pr_read(pr, ranks_out, rep);
     gbx_pr_free(pr);
+    return rc;
+}
+
+// Launch in chunks of 8 iterations until the stop rule fires or max_iters.
+static int run_to_stop(gbx_pr *pr)
+{
+    gbx_ctx *ctx = pr->ctx;
+    uint32_t launched = 0;
+    while (launched < pr->p.max_iters) {
+        const uint32_t n = std::min(8u, pr->p.max_iters - launched);
+        GBX_TRY(gbx_pr_launch(pr, n));
+        launched += n;
+        const Slot *slot = pr->slots.as<Slot>() + (pr->t & 1);
+        GBX_HIP(ctx, hipMemcpyAsync(pr->h_slot, slot, sizeof(Slot), hipMemcpyDeviceToHost, ctx->stream));
+        GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (pr->h_slot->done) break;
+    }
+    return GBX_OK;
+}
+
+int gbx_pr_adjoint(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, const float *teleport,
+                   const float *loss_weights, float *ranks_out, float *lambda_out, float *grad_w,
+                   float *grad_teleport, gbx_pr_report *fwd_rep, gbx_pr_report *adj_rep)
+{
+    if (!ctx || !g || !p || !loss_weights) return set_error(ctx, GBX_ERR_ARG, "gbx_pr_adjoint: NULL argument");
+    if (!g->weighted) return set_error(ctx, GBX_ERR_STATE, "gbx_pr_adjoint: graph has no weights");
+    if (g->seg_off.size() >= 2) return set_error(ctx, GBX_ERR_STATE, "gbx_pr_adjoint: not for a batch of graphs");
+    if (p->precision != GBX_F32 || p->stop == GBX_STOP_ACTIVE)
+        return set_error(ctx, GBX_ERR_ARG, "gbx_pr_adjoint: F32 with stop L1 or NONE");
+    GBX_HIP(ctx, hipSetDevice(ctx->device));
+    const uint32_t N = g->N, E = g->E;
+    gbx_pr_params q = *p;
+    q.mode = GBX_PR_STOCHASTIC;
+    q.weighted = 1;
+    q.direction = GBX_PULL_IN;
+    gbx_pr *fwd = nullptr, *adj = nullptr;
+    int rc = gbx_pr_create(ctx, g, &q, &fwd);
+    if (!rc) rc = gbx_pr_reset(fwd, teleport);
+    if (!rc) rc = run_to_stop(fwd);
+    if (!rc) rc = gbx_pr_read(fwd, ranks_out, fwd_rep);
+    // adjoint fixed point on the forward CSR: lambda = g + d (P lambda + delta Lambda/N)
+    gbx_pr_params qa = q;
+    qa.direction = GBX_PULL_FWD;
+    if (!rc) rc = pr_create_kind(ctx, g, &qa, K_ADJ, &adj);
+    if (!rc) rc = gbx_pr_reset(adj, loss_weights);
+    if (!rc) rc = run_to_stop(adj);
+    if (!rc) rc = gbx_pr_read(adj, lambda_out, adj_rep);
+    if (!rc && grad_w && E) {
+        gbx_plan *pf = adj->plan;
+        DevBuf y, plam, S;
+        rc = dev_alloc(ctx, y, (size_t)N * 4);
+        if (!rc) rc = dev_alloc(ctx, plam, (size_t)N * 4);
+        if (!rc) rc = dev_alloc(ctx, S, (size_t)N * 4);
+        DevBuf gw;
+        if (!rc) rc = dev_alloc(ctx, gw, (size_t)E * 4);
+        if (!rc) {
+            const unsigned gN = grid_for(N, 256, 16384);
+            fill_f32<<<gN, 256, 0, ctx->stream>>>(y.as<float>(), 0.0f, N);
+            // (P lambda)[u] in plan order, then original order
+            PlanView P = plan_view(pf, pf->wr.as<float>());
+            spmv_kernel<K_SPMV_W><<<iter_grid(ctx, pf), kBlock, 0, ctx->stream>>>(P, adj->r.as<float>(), y.as<float>());
+            scatter_perm<<<gN, 256, 0, ctx->stream>>>(y.as<float>(), pf->perm.as<uint32_t>(), N, plam.as<float>());
+            out_weight_sums<<<grid_for((size_t)N * 64, 256, 16384), 256, 0, ctx->stream>>>(
+                g->fwd.out_degree.as<uint32_t>(), g->fwd.edges_idx.as<uint32_t>(), g->fwd.w.as<float>(), N,
+                S.as<float>());
+            adjoint_edge_grad<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(
+                g->fwd.src.as<uint32_t>(), g->fwd.dst.as<uint32_t>(), E, fwd->out.as<float>(), adj->out.as<float>(),
+                plam.as<float>(), S.as<float>(), q.damp, gw.as<float>());
+            if (hipGetLastError() != hipSuccess) rc = set_error(ctx, GBX_ERR_HIP, "gbx_pr_adjoint: gradient kernels");
+        }
+        if (!rc && (hipMemcpyAsync(grad_w, gw.p, (size_t)E * 4, hipMemcpyDeviceToHost, ctx->stream) ||
+                    hipStreamSynchronize(ctx->stream)))
+            rc = set_error(ctx, GBX_ERR_HIP, "gbx_pr_adjoint: gradient copy");
+    }
+    if (!rc && grad_teleport) {
+        std::vector<float> lam(N);
+        if (hipMemcpyAsync(lam.data(), adj->out.p, (size_t)N * 4, hipMemcpyDeviceToHost, ctx->stream) ||
+            hipStreamSynchronize(ctx->stream))
+            rc = set_error(ctx, GBX_ERR_HIP, "gbx_pr_adjoint: lambda copy");
+        else
+            for (uint32_t v = 0; v < N; v++) grad_teleport[v] = (float)((1.0 - (double)q.damp) * lam[v]);
+    }
+    gbx_pr_free(fwd);
+    gbx_pr_free(adj);
     return rc;
 }
 

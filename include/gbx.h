@@ -218,6 +218,26 @@ int gbx_ppr_info(const gbx_pr *pr, uint32_t *num_vectors, uint32_t *vector_width
  * gbx_pr_launch, one launch instead of one or two per iteration. */
 int gbx_pr_launch_graph(gbx_pr *pr, uint32_t n_iters);
 
+/* ---------------------------------------------------------------- adjoint */
+/* Gradients of L = sum_v loss_weights[v] * p[v] through the weighted
+ * stochastic PageRank fixed point p = (1-d) t + d (P^T p + (delta.p)/N),
+ * P[u,v] = W[u,v] / sum_x W[u,x] (no reference counterpart).  The graph must
+ * carry weights (GBX_GRAPH_WEIGHTED).  Implicit differentiation: the forward
+ * pull runs to p's stop rule, then the adjoint fixed point
+ *   lambda = g + d (P lambda + delta (1.lambda)/N)
+ * is iterated as a pull over the forward CSR (rows = sources, row-normalised
+ * weights) to the same rule; only the converged p is kept.  Outputs
+ * (nullable): ranks_out = p [N]; lambda_out [N]; grad_w [E] = dL/dW for the
+ * forward-CSR edges in gbx_graph_export(g, 0, ...) order,
+ *   d p[u] (lambda[v] - (P lambda)[u]) / sum_x W[u,x];
+ * grad_teleport [N] = (1-d) lambda (dL/dt).  teleport: NULL (uniform) or
+ * host [N] summing to 1.  p: F32, stop GBX_STOP_L1 or NONE, max_iters per
+ * fixed point; with GBX_STOP_L1 the forward stops at sum|dp| < epsilon
+ * (sum p = 1) and the adjoint at sum|d lambda| < epsilon * sum|lambda|. */
+int gbx_pr_adjoint(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, const float *teleport,
+                   const float *loss_weights, float *ranks_out, float *lambda_out, float *grad_w,
+                   float *grad_teleport, gbx_pr_report *fwd_rep, gbx_pr_report *adj_rep);
+
 /* ---------------------------------------------------------------- graph batches */
 /* Many independent graphs as one device object, stored block-diagonally
  * (graph k's vertex v becomes vertex offsets[k] + v; no reference

@@ -261,72 +261,184 @@ int orc_pr_f64(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
     return 0;
 }
 
+/* Row sums S[u] of the raw out-weights and P[j] = wraw[j] / S[col[j]] for an
+ * in-CSR (row v lists its sources u = col[j]). */
+static int transition_f64(uint32_t N, const uint64_t *row_ptr, const uint32_t *col,
+                          const double *wraw, double **S_out, double **P_out)
+{
+    const uint64_t E = row_ptr[N];
+    double *S = calloc((size_t)N + 1, sizeof(double));
+    double *P = malloc((size_t)E * sizeof(double) + 8);
+    if (!S || !P) { free(S); free(P); return -1; }
+    for (uint64_t j = 0; j < E; j++) S[col[j]] += wraw[j];
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (uint32_t v = 0; v < N; v++)
+        for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) P[j] = wraw[j] / S[col[j]];
+    *S_out = S;
+    *P_out = P;
+    return 0;
+}
+
+/* Forward fixed point p = (1-d) t + d (P^T p + (delta.p)/N) by pull
+ * iterations from p (in/out) until sum|dp| < tol or max_iters. */
+static uint32_t forward_f64(uint32_t N, const uint64_t *row_ptr, const uint32_t *col,
+                            const double *S, const double *P, const double *teleport,
+                            double damp, uint32_t max_iters, double tol, double *p, double *tmp)
+{
+    uint32_t it = 0;
+    while (it < max_iters) {
+        double dang = 0.0, err = 0.0;
+#pragma omp parallel for reduction(+ : dang)
+        for (uint32_t u = 0; u < N; u++)
+            if (S[u] == 0.0) dang += p[u];
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : err)
+        for (uint32_t v = 0; v < N; v++) {
+            double acc = 0.0;
+            for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) acc += P[j] * p[col[j]];
+            tmp[v] = (1.0 - damp) * (teleport ? teleport[v] : 1.0 / N) + damp * (acc + dang / N);
+            err += fabs(tmp[v] - p[v]);
+        }
+        memcpy(p, tmp, (size_t)N * sizeof(double));
+        it++;
+        if (err < tol) break;
+    }
+    return it;
+}
+
 int orc_pr_adjoint_f64(uint32_t N, const uint64_t *row_ptr, const uint32_t *col,
                        const double *wraw, const double *teleport,
                        const double *g, double damp, uint32_t max_iters,
                        double tol, double *p_out, double *lambda_out,
                        double *grad_w_out, double *grad_t_out)
 {
-    double *S = calloc((size_t)N + 1, sizeof(double));
-    double *P = malloc((size_t)row_ptr[N] * sizeof(double) + 8);
+    const uint64_t E = row_ptr[N];
+    double *S = NULL, *P = NULL;
+    if (transition_f64(N, row_ptr, col, wraw, &S, &P)) return -1;
     double *Plam = malloc((size_t)N * sizeof(double) + 8);
     double *tmp = malloc((size_t)N * sizeof(double) + 8);
-    if (!S || !P || !Plam || !tmp) { free(S); free(P); free(Plam); free(tmp); return -1; }
-    uint64_t E = row_ptr[N];
-    for (uint32_t v = 0; v < N; v++)
-        for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++)
-            S[col[j]] += wraw[j];
-    for (uint32_t v = 0; v < N; v++)
-        for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++)
-            P[j] = wraw[j] / S[col[j]];
+    uint64_t *fptr = calloc((size_t)N + 2, sizeof(uint64_t));
+    uint32_t *fcol = malloc((size_t)E * sizeof(uint32_t) + 4);
+    double *fval = malloc((size_t)E * sizeof(double) + 8);
+    if (!Plam || !tmp || !fptr || !fcol || !fval) {
+        free(S); free(P); free(Plam); free(tmp); free(fptr); free(fcol); free(fval);
+        return -1;
+    }
     /* forward fixed point */
     for (uint32_t v = 0; v < N; v++) p_out[v] = teleport ? teleport[v] : 1.0 / N;
-    for (uint32_t it = 0; it < max_iters; it++) {
-        double dang = 0.0, err = 0.0;
-        for (uint32_t u = 0; u < N; u++)
-            if (S[u] == 0.0) dang += p_out[u];
-        for (uint32_t v = 0; v < N; v++) {
-            double acc = 0.0;
-            for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++)
-                acc += P[j] * p_out[col[j]];
-            tmp[v] = (1.0 - damp) * (teleport ? teleport[v] : 1.0 / N) + damp * (acc + dang / N);
-            err += fabs(tmp[v] - p_out[v]);
+    forward_f64(N, row_ptr, col, S, P, teleport, damp, max_iters, tol, p_out, tmp);
+    /* the forward CSR (row u lists P[u, v]) for a pull form of P lambda */
+    for (uint64_t j = 0; j < E; j++) fptr[col[j] + 2]++;
+    for (uint32_t u = 0; u < N; u++) fptr[u + 2] += fptr[u + 1];
+    for (uint32_t v = 0; v < N; v++)
+        for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) {
+            const uint64_t k = fptr[col[j] + 1]++;
+            fcol[k] = v;
+            fval[k] = P[j];
         }
-        memcpy(p_out, tmp, (size_t)N * sizeof(double));
-        if (err < tol) break;
-    }
-    /* adjoint fixed point lambda = g + d (P lambda + delta (1.lambda)/N) */
+    /* adjoint fixed point lambda = g + d (P lambda + delta (1.lambda)/N),
+     * stopped at sum|d lambda| < tol * sum|lambda| */
     for (uint32_t v = 0; v < N; v++) lambda_out[v] = g[v];
     for (uint32_t it = 0; it < max_iters; it++) {
-        double sum = 0.0, err = 0.0;
-        for (uint32_t v = 0; v < N; v++) { Plam[v] = 0.0; sum += lambda_out[v]; }
-        for (uint32_t v = 0; v < N; v++)
-            for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++)
-                Plam[col[j]] += P[j] * lambda_out[v];
+        double sum = 0.0, err = 0.0, mag = 0.0;
+#pragma omp parallel for reduction(+ : sum)
+        for (uint32_t v = 0; v < N; v++) sum += lambda_out[v];
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : err, mag)
         for (uint32_t u = 0; u < N; u++) {
-            double nl = g[u] + damp * (Plam[u] + (S[u] == 0.0 ? sum / N : 0.0));
+            double acc = 0.0;
+            for (uint64_t k = fptr[u]; k < fptr[u + 1]; k++) acc += fval[k] * lambda_out[fcol[k]];
+            const double nl = g[u] + damp * (acc + (S[u] == 0.0 ? sum / N : 0.0));
             err += fabs(nl - lambda_out[u]);
+            mag += fabs(nl);
             tmp[u] = nl;
         }
         memcpy(lambda_out, tmp, (size_t)N * sizeof(double));
-        if (err < tol) break;
+        if (err < tol * mag) break;
     }
     /* (P lambda)[u] for the weight gradient */
-    for (uint32_t v = 0; v < N; v++) Plam[v] = 0.0;
-    for (uint32_t v = 0; v < N; v++)
-        for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++)
-            Plam[col[j]] += P[j] * lambda_out[v];
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (uint32_t u = 0; u < N; u++) {
+        double acc = 0.0;
+        for (uint64_t k = fptr[u]; k < fptr[u + 1]; k++) acc += fval[k] * lambda_out[fcol[k]];
+        Plam[u] = acc;
+    }
     if (grad_w_out)
+#pragma omp parallel for schedule(dynamic, 1024)
         for (uint32_t v = 0; v < N; v++)
             for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) {
-                uint32_t u = col[j];
+                const uint32_t u = col[j];
                 grad_w_out[j] = damp * p_out[u] * (lambda_out[v] - Plam[u]) / S[u];
             }
     if (grad_t_out)
         for (uint32_t v = 0; v < N; v++) grad_t_out[v] = (1.0 - damp) * lambda_out[v];
-    (void)E;
-    free(S); free(P); free(Plam); free(tmp);
+    free(S); free(P); free(Plam); free(tmp); free(fptr); free(fcol); free(fval);
     return 0;
+}
+
+/* The forward fixed point in x87 extended precision (64-bit significand)
+ * for raw weights w, from p0, for exactly `iters` pull iterations. */
+static int forward_ld(uint32_t N, const uint64_t *row_ptr, const uint32_t *col, const double *w,
+                      const double *teleport, double damp, const double *p0, uint32_t iters,
+                      long double *p)
+{
+    const uint64_t E = row_ptr[N];
+    long double *S = calloc((size_t)N + 1, sizeof(long double));
+    double *P = malloc((size_t)E * sizeof(double) + 8);
+    long double *tmp = malloc((size_t)N * sizeof(long double) + 16);
+    if (!S || !P || !tmp) { free(S); free(P); free(tmp); return -1; }
+    for (uint64_t j = 0; j < E; j++) S[col[j]] += w[j];
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (uint32_t v = 0; v < N; v++)
+        for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) P[j] = (double)((long double)w[j] / S[col[j]]);
+    const long double d = damp, base = 1.0L - (long double)damp;
+    for (uint32_t v = 0; v < N; v++) p[v] = p0 ? p0[v] : (teleport ? teleport[v] : 1.0L / N);
+    for (uint32_t it = 0; it < iters; it++) {
+        long double dang = 0.0L;
+        for (uint32_t u = 0; u < N; u++)
+            if (S[u] == 0.0L) dang += p[u];
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (uint32_t v = 0; v < N; v++) {
+            long double acc = 0.0L;
+            for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) acc += (long double)P[j] * p[col[j]];
+            tmp[v] = base * (teleport ? (long double)teleport[v] : 1.0L / N) + d * (acc + dang / N);
+        }
+        memcpy(p, tmp, (size_t)N * sizeof(long double));
+    }
+    free(S); free(P); free(tmp);
+    return 0;
+}
+
+double orc_pr_loss_diff(uint32_t N, const uint64_t *row_ptr, const uint32_t *col,
+                        const double *w_plus, const double *w_minus, const double *teleport,
+                        const double *g, double damp, const double *p0, uint32_t iters)
+{
+    long double *pp = malloc((size_t)N * sizeof(long double) + 16);
+    long double *pm = malloc((size_t)N * sizeof(long double) + 16);
+    if (!pp || !pm || forward_ld(N, row_ptr, col, w_plus, teleport, damp, p0, iters, pp) ||
+        forward_ld(N, row_ptr, col, w_minus, teleport, damp, p0, iters, pm)) {
+        free(pp); free(pm);
+        return NAN;
+    }
+    long double dl = 0.0L;
+    for (uint32_t v = 0; v < N; v++) dl += (long double)g[v] * (pp[v] - pm[v]);
+    free(pp); free(pm);
+    return (double)dl;
+}
+
+double orc_pr_loss_f64(uint32_t N, const uint64_t *row_ptr, const uint32_t *col,
+                       const double *wraw, const double *teleport, const double *g,
+                       double damp, const double *p0, uint32_t max_iters, double tol,
+                       double *p_out)
+{
+    double *S = NULL, *P = NULL;
+    if (transition_f64(N, row_ptr, col, wraw, &S, &P)) return NAN;
+    double *tmp = malloc((size_t)N * sizeof(double) + 8);
+    if (!tmp) { free(S); free(P); return NAN; }
+    for (uint32_t v = 0; v < N; v++) p_out[v] = p0 ? p0[v] : (teleport ? teleport[v] : 1.0 / N);
+    forward_f64(N, row_ptr, col, S, P, teleport, damp, max_iters, tol, p_out, tmp);
+    double L = 0.0;
+    for (uint32_t v = 0; v < N; v++) L += g[v] * p_out[v];
+    free(S); free(P); free(tmp);
+    return L;
 }
 
 /* ---- R-MAT generator (counter-based; identical integer arithmetic on GPU) */

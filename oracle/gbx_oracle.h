@@ -84,12 +84,31 @@ int orc_pr_f64(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
  *   dL/dt[v]   = (1-d) lambda[v]
  *   dL/dW[e=(u,v)] = d p[u] (lambda[v] - sum_x P[u,x] lambda[x]) / S[u]
  * with S[u] = sum of u's raw out-weights.  Edges are given as an in-CSR
- * (row v lists sources u) with raw weights wraw in the same order. */
+ * (row v lists sources u) with raw weights wraw in the same order.  The
+ * forward stops at sum|dp| < tol, the adjoint at sum|d lambda| < tol *
+ * sum|lambda| (or max_iters each). */
 int orc_pr_adjoint_f64(uint32_t N, const uint64_t *row_ptr, const uint32_t *col,
                        const double *wraw, const double *teleport,
                        const double *g, double damp, uint32_t max_iters,
                        double tol, double *p_out, double *lambda_out,
                        double *grad_w_out, double *grad_t_out);
+
+/* L = sum_v g[v] p[v] at the forward fixed point for raw weights wraw
+ * (same model as orc_pr_adjoint_f64), iterating from p0 (NULL: teleport)
+ * until sum|dp| < tol or max_iters; p_out [N] receives p.  Central finite
+ * differences of this are the independent check of the adjoint. */
+double orc_pr_loss_f64(uint32_t N, const uint64_t *row_ptr, const uint32_t *col,
+                       const double *wraw, const double *teleport, const double *g,
+                       double damp, const double *p0, uint32_t max_iters, double tol,
+                       double *p_out);
+
+/* L(w_plus) - L(w_minus) for the same model, both fixed points iterated
+ * `iters` times from p0 (NULL: teleport) in x87 extended precision, so that
+ * a central difference is not swamped by FP64 rounding at hub rows (sums of
+ * 10^5 terms whose rounding changes with any perturbation). */
+double orc_pr_loss_diff(uint32_t N, const uint64_t *row_ptr, const uint32_t *col,
+                        const double *w_plus, const double *w_minus, const double *teleport,
+                        const double *g, double damp, const double *p0, uint32_t iters);
 
 /* Seeded counter-based R-MAT generator (same integer arithmetic as the GPU
  * generator).  Writes num_edges = edge_factor << scale edges, ids permuted

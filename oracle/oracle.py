@@ -58,6 +58,12 @@ def lib():
                                  C.POINTER(C.c_double)]
         L.orc_pr_adjoint_f64.argtypes = [C.c_uint32, _u64p, _u32p, _f64p, C.c_void_p, _f64p, C.c_double,
                                          C.c_uint32, C.c_double, _f64p, _f64p, _f64p, _f64p]
+        L.orc_pr_loss_f64.argtypes = [C.c_uint32, _u64p, _u32p, _f64p, C.c_void_p, _f64p, C.c_double,
+                                      C.c_void_p, C.c_uint32, C.c_double, _f64p]
+        L.orc_pr_loss_f64.restype = C.c_double
+        L.orc_pr_loss_diff.argtypes = [C.c_uint32, _u64p, _u32p, _f64p, _f64p, C.c_void_p, _f64p, C.c_double,
+                                       C.c_void_p, C.c_uint32]
+        L.orc_pr_loss_diff.restype = C.c_double
         L.orc_rmat_generate.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                         C.c_uint64, _u32p, _u32p]
         L.orc_rmat_permute.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64]
@@ -246,6 +252,50 @@ def adjoint_f64(N, row_ptr, col, wraw, g, teleport=None, damp=0.85, iterations=1
                              np.ascontiguousarray(wraw, np.float64), tp.ctypes.data if tp is not None else None,
                              np.ascontiguousarray(g, np.float64), damp, iterations, tol, p, lam, gw, gt)
     return p, lam, gw, gt
+
+
+def loss_f64(N, row_ptr, col, wraw, g, teleport=None, damp=0.85, p0=None, iterations=1000, tol=0.0):
+    """L = g.p at the weighted stochastic fixed point (FP64), and p."""
+    p = np.zeros(N)
+    tp = np.ascontiguousarray(teleport, np.float64) if teleport is not None else None
+    p0c = np.ascontiguousarray(p0, np.float64) if p0 is not None else None
+    L = lib().orc_pr_loss_f64(N, np.ascontiguousarray(row_ptr, np.uint64), np.ascontiguousarray(col, np.uint32),
+                              np.ascontiguousarray(wraw, np.float64), tp.ctypes.data if tp is not None else None,
+                              np.ascontiguousarray(g, np.float64), damp,
+                              p0c.ctypes.data if p0c is not None else None, iterations, tol, p)
+    return L, p
+
+
+def loss_diff(N, row_ptr, col, w_plus, w_minus, g, teleport=None, damp=0.85, p0=None, iterations=100):
+    """L(w_plus) - L(w_minus), both in extended precision from p0."""
+    tp = np.ascontiguousarray(teleport, np.float64) if teleport is not None else None
+    p0c = np.ascontiguousarray(p0, np.float64) if p0 is not None else None
+    return lib().orc_pr_loss_diff(N, np.ascontiguousarray(row_ptr, np.uint64), np.ascontiguousarray(col, np.uint32),
+                                  np.ascontiguousarray(w_plus, np.float64), np.ascontiguousarray(w_minus, np.float64),
+                                  tp.ctypes.data if tp is not None else None, np.ascontiguousarray(g, np.float64),
+                                  damp, p0c.ctypes.data if p0c is not None else None, iterations)
+
+
+def fd_edge_grads(N, row_ptr, col, wraw, g, edges, p0, h=1e-3, teleport=None, damp=0.85, iterations=100):
+    """FP64 central finite differences dL/dW at in-CSR edge positions."""
+    out = np.empty(len(edges))
+    for k, e in enumerate(edges):
+        wp = np.array(wraw, np.float64)
+        wm = wp.copy()
+        wp[e] *= 1.0 + h
+        wm[e] *= 1.0 - h
+        out[k] = loss_diff(N, row_ptr, col, wp, wm, g, teleport, damp, p0, iterations) / (wp[e] - wm[e])
+    return out
+
+
+def weighted_rmat_workload(scale, edge_factor=16, seed=20261203):
+    """The differentiable-PageRank workload: seeded R-MAT (permuted ids,
+    duplicates removed), FP32 edge weights lognormal(0, 1) in edge-list order
+    and loss weights g ~ N(0, 1) per vertex.  Returns src, dst, w, N, g."""
+    s, d, N = rmat_graph(scale, edge_factor, seed + scale)
+    w = np.random.default_rng(seed + 100 + scale).lognormal(0.0, 1.0, s.size).astype(np.float32)
+    g = np.random.default_rng(seed + 200 + scale).standard_normal(N)
+    return s, d, w, N, g
 
 
 def rmat(scale, edge_factor, seed, a=0.57, b=0.19, c=0.19):

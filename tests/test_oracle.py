@@ -220,6 +220,56 @@ def test_adjoint_matches_finite_differences_and_dense():
         assert gt[v] == pytest.approx(fd, rel=1e-6, abs=1e-10)
 
 
+def test_loss_diff_matches_dense_solves():
+    rng = np.random.default_rng(8)
+    n = 80
+    src = rng.integers(0, n, 600).astype(np.uint32)
+    dst = rng.integers(0, n, 600).astype(np.uint32)
+    fwd, inv = O.build_graph(src, dst, None, n, dedupe=True)
+    rp = O.row_ptr_from(inv)
+    wraw = np.exp(rng.standard_normal(inv.E))
+    g = rng.standard_normal(n)
+
+    def dense(wr):
+        S = np.zeros(n)
+        np.add.at(S, inv.dst, wr)
+        return float(g @ O.dense_stochastic(n, fwd.out_degree, rp, inv.dst, weights=wr / S[inv.dst]))
+
+    wp, wm = wraw * 1.01, wraw.copy()
+    wm[3] *= 0.5
+    got = O.loss_diff(n, rp, inv.dst, wp, wm, g, iterations=400)
+    assert got == pytest.approx(dense(wp) - dense(wm), rel=1e-9, abs=1e-15)
+    L, p = O.loss_f64(n, rp, inv.dst, wraw, g, iterations=400)
+    assert L == pytest.approx(dense(wraw), rel=1e-12)
+    assert p.sum() == pytest.approx(1.0, abs=1e-12)
+
+
+def test_adjoint_fd_fixture_pins_fp64_adjoint(golden):
+    """R-MAT scale 20 differentiable workload: the oracle's FP64 adjoint
+    against the stored FP64 central differences (forward solves only)."""
+    fx = golden("adjoint_fd_s20")
+    s, d, w, N, g = O.weighted_rmat_workload(int(fx["scale"]), int(fx["edge_factor"]), int(fx["seed"]))
+    assert (N, s.size) == (int(fx["N"]), int(fx["E"]))
+    fwd, inv = O.build_graph(s, d, w, N, mode=1, move_weights=True)
+    rp = O.row_ptr_from(inv)
+    p, lam, gw, gt = O.adjoint_f64(N, rp, inv.dst, inv.w.astype(np.float64), g, iterations=2000, tol=1e-15)
+    assert float(g @ p) == pytest.approx(float(fx["loss"]), rel=1e-12)
+    key = inv.dst.astype(np.uint64) * np.uint64(N) + inv.src.astype(np.uint64)
+    order = np.argsort(key)
+    fk = fx["u"].astype(np.uint64) * np.uint64(N) + fx["v"].astype(np.uint64)
+    at = order[np.searchsorted(key[order], fk)]
+    assert np.array_equal(key[at], fk)
+    rel = np.abs(gw[at] - fx["fd"]) / np.abs(fx["fd"])
+    assert rel.max() <= 2e-5
+    # whole-gradient directional derivative (edge-list order xi)
+    xi_list = np.random.default_rng(int(fx["seed"]) + 400 + int(fx["scale"])).standard_normal(s.size)
+    kl = s.astype(np.uint64) * np.uint64(N) + d.astype(np.uint64)
+    ol = np.argsort(kl)
+    xi = xi_list[ol][np.searchsorted(kl[ol], key)]
+    dd = float(np.sum(gw * xi * inv.w.astype(np.float64)))
+    assert dd == pytest.approx(float(fx["dir_fd"]), rel=1e-5)
+
+
 # ---------------------------------------------------------------- generator
 
 def test_rmat_permutation_is_bijection():
