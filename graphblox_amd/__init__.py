@@ -26,6 +26,7 @@ PR_REF, PR_STOCHASTIC = 0, 1
 STOP_ACTIVE, STOP_L1, STOP_NONE = 0, 1, 2
 F32, BF16 = 0, 1
 PULL_IN, PULL_FWD = 0, 1
+METHOD_AUTO, METHOD_PULL, METHOD_BINNED = 0, 1, 2
 
 
 class GbxError(RuntimeError):
@@ -37,14 +38,16 @@ class GbxError(RuntimeError):
 class PrParams(C.Structure):
     _fields_ = [("damp", C.c_float), ("max_iters", C.c_uint32), ("epsilon", C.c_double),
                 ("mode", C.c_int), ("stop", C.c_int), ("precision", C.c_int),
-                ("direction", C.c_int), ("weighted", C.c_int), ("reserved", C.c_int)]
+                ("direction", C.c_int), ("weighted", C.c_int), ("method", C.c_int)]
 
 
 class Tuning(C.Structure):
     _fields_ = [("bin_thresholds", C.c_uint32 * 7), ("hub_chunk", C.c_uint32), ("lds_bytes", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("ppr_hub", C.c_uint32), ("ppr_chunk", C.c_uint32),
                 ("ppr_blocks_per_cu", C.c_uint32), ("ppr_vectors_per_lane", C.c_uint32),
-                ("reserved", C.c_uint32 * 4)]
+                ("binned_min_vertices", C.c_uint32), ("binned_chunk_edges", C.c_uint32),
+                ("binned_bin_edges", C.c_uint32), ("binned_bin_rows", C.c_uint32),
+                ("binned_chunk_rows", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("bin_thresholds", "reserved")}
@@ -303,8 +306,8 @@ class Graph:
             pass
 
     def pagerank(self, iterations=20, epsilon=1e-4, mode=PR_REF, stop=STOP_ACTIVE, damp=0.85,
-                 direction=PULL_IN, weighted=False, precision=F32, teleport=None):
-        p = params(iterations, epsilon, mode, stop, damp, direction, weighted, precision)
+                 direction=PULL_IN, weighted=False, precision=F32, teleport=None, method=METHOD_AUTO):
+        p = params(iterations, epsilon, mode, stop, damp, direction, weighted, precision, method)
         ranks = np.empty(self.N, np.float32)
         rep = PrReport()
         tp = _f32(teleport)
@@ -348,7 +351,7 @@ class Graph:
 
 
 def params(iterations=20, epsilon=1e-4, mode=PR_REF, stop=STOP_ACTIVE, damp=0.85, direction=PULL_IN,
-           weighted=False, precision=F32) -> PrParams:
+           weighted=False, precision=F32, method=METHOD_AUTO) -> PrParams:
     p = PrParams()
     lib().gbx_pr_params_default(C.byref(p))
     p.damp = damp
@@ -359,6 +362,7 @@ def params(iterations=20, epsilon=1e-4, mode=PR_REF, stop=STOP_ACTIVE, damp=0.85
     p.precision = precision
     p.direction = direction
     p.weighted = int(weighted)
+    p.method = method
     return p
 
 

@@ -21,7 +21,8 @@ CSRC = os.path.join(PKG, "csrc")
 HOST = os.path.join(PKG, "host")
 INC = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["gbx_core.hip", "gbx_sort.hip", "gbx_rmat.hip", "gbx_pr.hip", "gbx_ppr.hip", "gbx_batch.hip", "gbx_tune.hip"]
+HIP_SOURCES = ["gbx_core.hip", "gbx_sort.hip", "gbx_rmat.hip", "gbx_pr.hip", "gbx_binned.hip", "gbx_ppr.hip",
+               "gbx_batch.hip", "gbx_tune.hip"]
 HOST_SOURCES = ["graphblox_hip.c"]
 
 LIB_HIP = os.path.join(PKG, "libgbx_hip.so")

@@ -1,0 +1,743 @@
+// gbx_binned.hip -- the binned (propagation-blocking) PageRank iteration.
+//
+// The pull iteration gathers one 4-byte contribution per edge from a vector
+// far larger than the L2; on MI355X every miss moves a 128-byte line, so at
+// R-MAT scale 24 the gathers move about three times the algorithmic bytes.
+// The binned method trades the random gather for two streaming passes over
+// a value array V with one slot per edge:
+//
+//   scatter     one block per source chunk (a contiguous range of at most
+//               32768 sources): the chunk's contributions are loaded into
+//               LDS, and for every destination bin the chunk's edges into
+//               that bin are written as one contiguous segment of V.  The
+//               chunk's source indices (src_local, u16) are stored segment
+//               by segment, each segment padded to a multiple of 8, with the
+//               V position of every group of 8 (gdst, u32): the pass streams
+//               src_local and gdst and writes V in runs, with no search.
+//   accumulate  one block per unit (a destination bin of at most 20000 rows,
+//               or a piece of a single giant row): V and the static
+//               dst_local array are read sequentially and summed per row in
+//               LDS as 64-bit fixed-point integers, then the PageRank update
+//               (the pull iteration's epilogue) writes the ranks and the
+//               next contributions.
+//
+// The fixed-point scale is a power of two chosen per iteration from the
+// previous FP64 rank sum, which bounds every row sum, so no sum can
+// overflow; integer adds are associative, so the result is independent of
+// the order of the LDS atomics (deterministic), and the per-edge rounding
+// (2^-37 at R-MAT scale 24 in REF mode, 2^-60 in stochastic mode) is below
+// FP32 accumulation error.
+//
+// Layout (built once per plan, device resident): V is bin-major, within a
+// bin chunk-major, within a (bin, chunk) segment in the plan's edge order;
+// every bin's region is padded to a multiple of 8 values (pads: value 0,
+// row 0) so the accumulate pass reads 8 values per lane.  src_local is in
+// chunk-major order (segments of one chunk contiguous, each padded to 8 with
+// the marker 0xFFFF).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "gbx_iter.hpp"
+
+namespace gbx {
+
+constexpr uint32_t kMaxBinRows = 20000;     // LDS int64 row sums of one bin
+constexpr uint32_t kMaxChunkRows = 32768;   // LDS contributions of one chunk; u16 local ids
+constexpr uint32_t kNoMulti = 0xFFFFFFFFu;
+
+struct BinUnit {
+    uint32_t row_begin, row_end;  // rows of the bin (plan order)
+    uint32_t vbeg, vend;          // V range of this unit (multiples of 8)
+    uint32_t multi;               // accumulator slot of a bin split over units, else kNoMulti
+    uint32_t nunits;              // units of the bin
+    uint32_t pad[2];
+};
+
+}  // namespace gbx
+
+struct gbx_bplan {
+    uint32_t NC = 0, NB = 0, NU = 0, nseg = 0, n_multi = 0;
+    uint32_t Vlen = 0;                // padded length of V
+    uint32_t cut[5] = {0, 0, 0, 0, 0};  // chunk edges, bin edges, bin rows, chunk rows, segment pad
+    uint32_t Plen = 0;                // padded length of src_local
+    gbx::DevBuf chunk_lo;             // u32[NC+1] first source of each chunk
+    gbx::DevBuf chunk_grp;            // u32[NC+1] first group of 8 src_local entries of each chunk
+    gbx::DevBuf src_local;            // u16[Plen] source - chunk_lo, chunk-major, segments padded (0xFFFF)
+    gbx::DevBuf gdst;                 // u32[Plen/8] V position of each group of 8 src_local entries
+    gbx::DevBuf dst_local;            // u16[Vlen] row - bin first row, V order
+    gbx::DevBuf units;                // BinUnit[NU]
+};
+
+struct gbx_binned_state {
+    std::shared_ptr<gbx_bplan> bp;  // the layout (shared with the plan)
+    gbx::DevBuf V;    // f32 or bf16 [Vlen]
+    gbx::DevBuf acc;  // i64[n_multi] sums of split bins (zero between iterations)
+    gbx::DevBuf cnt;  // u32[n_multi] arrival counters (zero between iterations)
+    size_t shm_scatter = 0, shm_accum = 0;
+};
+
+namespace gbx {
+
+// ------------------------------------------------------------------ build kernels
+
+__global__ void edge_rows(const uint32_t *__restrict__ rp, uint32_t N, uint32_t *__restrict__ rowid)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+        for (uint32_t j = rp[v]; j < rp[v + 1]; j++) rowid[j] = v;
+}
+
+__global__ void column_counts(const uint32_t *__restrict__ col, uint32_t E, uint32_t *__restrict__ cnt)
+{
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x)
+        atomicAdd(cnt + col[e], 1u);
+}
+
+// key = bin * NC + chunk (bin-major) or chunk * NB + bin; payload = plan edge index
+__global__ void bin_keys(const uint32_t *__restrict__ rowid, const uint32_t *__restrict__ col, uint32_t E,
+                         const uint32_t *__restrict__ bin_of, const uint32_t *__restrict__ chunk_of,
+                         uint32_t NB, uint32_t NC, int chunk_major, uint32_t *__restrict__ key,
+                         uint32_t *__restrict__ pay)
+{
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        const uint32_t b = bin_of[rowid[e]], k = chunk_of[col[e]];
+        key[e] = chunk_major ? k * NB + b : b * NC + k;
+        pay[e] = e;
+    }
+}
+
+// Runs of equal keys in a sorted key array: flag[i] = 1 at a run start.
+__global__ void run_flags(const uint32_t *__restrict__ key, uint32_t n, uint32_t *__restrict__ flag)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        flag[i] = (i == 0 || key[i - 1] != key[i]) ? 1u : 0u;
+}
+
+// runstart[r] = first index of run r (runidx = exclusive scan of flag)
+__global__ void run_starts(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ runidx, uint32_t n,
+                           uint32_t *__restrict__ runstart)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (flag[i]) runstart[runidx[i]] = i;
+}
+
+// Run length rounded up to a multiple of pad.
+__global__ void run_padded_lengths(const uint32_t *__restrict__ runstart, uint32_t nrun, uint32_t n, uint32_t pad,
+                                   uint32_t *__restrict__ plen)
+{
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nrun; r += gridDim.x * blockDim.x) {
+        const uint32_t len = (r + 1 < nrun ? runstart[r + 1] : n) - runstart[r];
+        plen[r] = (len + pad - 1) / pad * pad;
+    }
+}
+
+// Bin-major sorted edges -> dst_local at their padded V positions, the V
+// start of every (chunk, bin) segment in a dense table, and the padded V
+// length of every bin.
+__global__ void bin_major_fill(const uint32_t *__restrict__ key, const uint32_t *__restrict__ pay, uint32_t E,
+                               const uint32_t *__restrict__ flag, const uint32_t *__restrict__ runidx,
+                               const uint32_t *__restrict__ runstart, const uint32_t *__restrict__ vstart,
+                               const uint32_t *__restrict__ plen, const uint32_t *__restrict__ rowid,
+                               const uint32_t *__restrict__ bin_row0, uint32_t NB, uint32_t NC,
+                               uint16_t *__restrict__ dst_local, uint32_t *__restrict__ voff_table,
+                               uint32_t *__restrict__ bin_len)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < E; i += gridDim.x * blockDim.x) {
+        const uint32_t kk = key[i], b = kk / NC, k = kk % NC;
+        const uint32_t r = runidx[i] + flag[i] - 1u;
+        dst_local[vstart[r] + (i - runstart[r])] = (uint16_t)(rowid[pay[i]] - bin_row0[b]);
+        if (flag[i]) {
+            voff_table[(size_t)k * NB + b] = vstart[r];
+            atomicAdd(bin_len + b, plen[r]);
+        }
+    }
+}
+
+// Chunk-major sorted edges -> src_local at padded positions (pads keep the
+// 0xFFFF prefill) and the V position of every group of 8.
+__global__ void chunk_major_fill(const uint32_t *__restrict__ key, const uint32_t *__restrict__ pay, uint32_t E,
+                                 const uint32_t *__restrict__ flag, const uint32_t *__restrict__ runidx,
+                                 const uint32_t *__restrict__ runstart, const uint32_t *__restrict__ pstart,
+                                 const uint32_t *__restrict__ col, const uint32_t *__restrict__ chunk_lo,
+                                 uint32_t NB, uint16_t *__restrict__ src_local)
+{
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < E; j += gridDim.x * blockDim.x) {
+        const uint32_t k = key[j] / NB;
+        const uint32_t r = runidx[j] + flag[j] - 1u;
+        src_local[pstart[r] + (j - runstart[r])] = (uint16_t)(col[pay[j]] - chunk_lo[k]);
+    }
+}
+
+__global__ void group_destinations(const uint32_t *__restrict__ key, const uint32_t *__restrict__ runstart,
+                                   const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ plen,
+                                   uint32_t nrun, const uint32_t *__restrict__ voff_table,
+                                   uint32_t *__restrict__ gdst)
+{
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nrun; r += gridDim.x * blockDim.x) {
+        const uint32_t g0 = pstart[r] >> 3, ng = plen[r] >> 3, v0 = voff_table[key[runstart[r]]];
+        for (uint32_t q = 0; q < ng; q++) gdst[g0 + q] = v0 + 8u * q;
+    }
+}
+
+// chunk_grp[k] = first group of the chunk's first segment.
+__global__ void chunk_groups(const uint32_t *__restrict__ cstart, uint32_t NC, const uint32_t *__restrict__ runidx,
+                             uint32_t E, uint32_t nrun, const uint32_t *__restrict__ pstart,
+                             uint32_t *__restrict__ chunk_grp)
+{
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= NC; k += gridDim.x * blockDim.x) {
+        const uint32_t r = cstart[k] >= E ? nrun : runidx[cstart[k]];
+        chunk_grp[k] = pstart[r] >> 3;
+    }
+}
+
+// ------------------------------------------------------------------ iteration kernels
+
+struct ScatterView {
+    const uint32_t *__restrict__ chunk_lo;
+    const uint32_t *__restrict__ chunk_grp;
+    const uint16_t *__restrict__ src_local;
+    const uint32_t *__restrict__ gdst;
+    uint32_t NC;
+};
+
+constexpr int kScatterUnroll = 4;  // groups of 8 per lane in flight
+
+template <class T>
+__device__ __forceinline__ void store8(T *V, uint32_t pos, const T (&v)[8])
+{
+    if constexpr (sizeof(T) == 4) {
+        float4 *d = reinterpret_cast<float4 *>(V + pos);
+        d[0] = make_float4(v[0], v[1], v[2], v[3]);
+        d[1] = make_float4(v[4], v[5], v[6], v[7]);
+    } else {
+        uint4 w;
+        w.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+        w.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+        w.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
+        w.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
+        *reinterpret_cast<uint4 *>(V + pos) = w;
+    }
+}
+
+// One block per source chunk.  Each lane takes groups of 8 entries: one
+// 16-byte load of source indices, one V position, eight LDS reads, one
+// 32-byte (FP32) or 16-byte (BF16) store; pads (0xFFFF) store zeros, so every
+// segment is written in whole 128-byte lines.
+template <class T>
+__global__ void __launch_bounds__(kBlock) bin_scatter_kernel(ScatterView S, const Slot *slots, uint32_t t,
+                                                             const T *__restrict__ c_in, T *__restrict__ V)
+{
+    if (slots[t & 1].done) return;
+    const uint32_t k = blockIdx.x;
+    T *cl = reinterpret_cast<T *>(gbx_dyn_lds);
+    const uint32_t lo = S.chunk_lo[k], n = S.chunk_lo[k + 1] - lo;
+    for (uint32_t i = threadIdx.x; i < n; i += kBlock) cl[i] = c_in[lo + i];
+    __syncthreads();
+    const uint32_t g0 = S.chunk_grp[k], g1 = S.chunk_grp[k + 1];
+    for (uint32_t base = g0 + threadIdx.x; base < g1; base += kBlock * kScatterUnroll) {
+        uint4 src[kScatterUnroll];
+        uint32_t dst[kScatterUnroll];
+#pragma unroll
+        for (int q = 0; q < kScatterUnroll; q++) {
+            const uint32_t g = base + (uint32_t)q * kBlock;
+            if (g < g1) {
+                src[q] = reinterpret_cast<const uint4 *>(S.src_local)[g];
+                dst[q] = S.gdst[g];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kScatterUnroll; q++) {
+            const uint32_t g = base + (uint32_t)q * kBlock;
+            if (g < g1) {
+                const uint32_t w[4] = {src[q].x, src[q].y, src[q].z, src[q].w};
+                T v[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const uint32_t sidx = (i & 1) ? (w[i >> 1] >> 16) : (w[i >> 1] & 0xFFFFu);
+                    v[i] = sidx == 0xFFFFu ? T(0) : cl[sidx];
+                }
+                store8(V, dst[q], v);
+            }
+        }
+    }
+}
+
+struct AccumView {
+    const BinUnit *__restrict__ units;
+    const uint16_t *__restrict__ dst_local;
+    long long *acc;  // split-bin sums
+    uint32_t *cnt;   // split-bin arrival counters
+};
+
+template <class T>
+__device__ __forceinline__ void load8(const T *V, uint32_t e, float (&v)[8])
+{
+    if constexpr (sizeof(T) == 4) {
+        const float4 a = *reinterpret_cast<const float4 *>(V + e);
+        const float4 b = *reinterpret_cast<const float4 *>(V + e + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+        const uint4 a = *reinterpret_cast<const uint4 *>(V + e);
+        const uint32_t w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            v[2 * i] = __uint_as_float(w[i] << 16);
+            v[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
+        }
+    }
+}
+
+__device__ __forceinline__ long long to_fixed(float v, double scale)
+{
+    return __double2ll_rn((double)v * scale);
+}
+
+template <int KIND, class T>
+__global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const uint32_t *__restrict__ outdeg,
+                                                           uint32_t N, const T *__restrict__ V, IterView A)
+{
+    const Slot *cur = A.slots + (A.t & 1);
+    Slot *nxt = A.slots + ((A.t + 1) & 1);
+    if (cur->done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = *cur;
+        return;
+    }
+    const BinUnit u = B.units[blockIdx.x];
+    PrEpi<KIND, T> epi = make_epi<KIND, T>(outdeg, N, A, cur);
+    // every row sum is at most the sum of all contributions, itself at most
+    // the previous rank sum (bf16 rounding adds < 0.4 %)
+    const double bound = cur->sum * 1.01 + 1e-30;
+    int ex = 0;
+    (void)frexp(bound, &ex);
+    const double scale = ldexp(1.0, 61 - ex), inv = ldexp(1.0, ex - 61);
+    const uint32_t nrows = u.row_end - u.row_begin;
+    const uint32_t stride = kBlock * 8u;
+    if (nrows == 1) {
+        // one row: lane sums, block sum, then the split-bin combine if any
+        __shared__ long long wsum[kBlock / 64];
+        long long s = 0;
+        for (uint32_t e = u.vbeg + threadIdx.x * 8u; e < u.vend; e += stride) {
+            float v[8];
+            load8(V, e, v);
+#pragma unroll
+            for (int i = 0; i < 8; i++) s += to_fixed(v[i], scale);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if ((threadIdx.x & 63u) == 0) wsum[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long tot = 0;
+            for (int w = 0; w < kBlock / 64; w++) tot += wsum[w];
+            bool apply = true;
+            if (u.multi != kNoMulti) {
+                atomicAdd(reinterpret_cast<unsigned long long *>(B.acc + u.multi), (unsigned long long)tot);
+                __threadfence();
+                const uint32_t ticket = atomicAdd(B.cnt + u.multi, 1u);
+                apply = ticket == u.nunits - 1;
+                if (apply) {
+                    __threadfence();
+                    tot = (long long)atomicAdd(reinterpret_cast<unsigned long long *>(B.acc + u.multi), 0ull);
+                    B.acc[u.multi] = 0;
+                    B.cnt[u.multi] = 0;
+                }
+            }
+            if (apply) epi(u.row_begin, (float)((double)tot * inv));
+        }
+    } else {
+        long long *y = reinterpret_cast<long long *>(gbx_dyn_lds);
+        for (uint32_t i = threadIdx.x; i < nrows; i += kBlock) y[i] = 0;
+        __syncthreads();
+        for (uint32_t e = u.vbeg + threadIdx.x * 8u; e < u.vend; e += stride) {
+            const uint4 dw = *reinterpret_cast<const uint4 *>(B.dst_local + e);
+            const uint32_t w[4] = {dw.x, dw.y, dw.z, dw.w};
+            float v[8];
+            load8(V, e, v);
+            // runs of equal rows (segments are in row order) are summed in
+            // registers before one LDS atomic per run
+            uint32_t rd = w[0] & 0xFFFFu;
+            long long run = 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const uint32_t d = (i & 1) ? (w[i >> 1] >> 16) : (w[i >> 1] & 0xFFFFu);
+                const long long q = to_fixed(v[i], scale);
+                if (d != rd) {
+                    if (run) atomicAdd(reinterpret_cast<unsigned long long *>(y + rd), (unsigned long long)run);
+                    rd = d;
+                    run = q;
+                } else {
+                    run += q;
+                }
+            }
+            if (run) atomicAdd(reinterpret_cast<unsigned long long *>(y + rd), (unsigned long long)run);
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nrows; i += kBlock) epi(u.row_begin + i, (float)((double)y[i] * inv));
+    }
+    const uint32_t iters = cur->iters;
+    const double eps = A.eps;
+    const int stop = A.stop;
+    grid_reduce(epi.part, A.partials, A.counter,
+                [&](const Partial &tot) { *nxt = next_slot<KIND>(tot, iters, eps, stop); });
+}
+
+// ------------------------------------------------------------------ host
+
+// Runs of a sorted key array: flags, run index, starts and padded lengths,
+// padded starts (exclusive scan) and their total.
+struct Runs {
+    DevBuf flag, idx, start, plen, pstart, total;
+    uint32_t n = 0, padded = 0;
+};
+
+static int find_runs(gbx_ctx *ctx, const uint32_t *key, uint32_t E, uint32_t pad, Runs &R)
+{
+    GBX_TRY(dev_alloc(ctx, R.flag, (size_t)E * 4 + 4));
+    GBX_TRY(dev_alloc(ctx, R.idx, (size_t)E * 4 + 4));
+    run_flags<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(key, E, R.flag.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    GBX_TRY(exclusive_scan_u32(ctx, R.flag.as<uint32_t>(), R.idx.as<uint32_t>(), E, nullptr));
+    uint32_t last_idx = 0, last_flag = 0;
+    GBX_HIP(ctx, hipMemcpyAsync(&last_idx, R.idx.as<uint32_t>() + (E - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+    GBX_HIP(ctx, hipMemcpyAsync(&last_flag, R.flag.as<uint32_t>() + (E - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    R.n = last_idx + last_flag;
+    GBX_TRY(dev_alloc(ctx, R.start, (size_t)R.n * 4 + 4));
+    GBX_TRY(dev_alloc(ctx, R.plen, (size_t)R.n * 4 + 4));
+    GBX_TRY(dev_alloc(ctx, R.pstart, (size_t)(R.n + 1) * 4));
+    GBX_TRY(dev_alloc(ctx, R.total, 16));
+    run_starts<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(R.flag.as<uint32_t>(), R.idx.as<uint32_t>(), E,
+                                                                   R.start.as<uint32_t>());
+    run_padded_lengths<<<grid_for(R.n, 256, 16384), 256, 0, ctx->stream>>>(R.start.as<uint32_t>(), R.n, E, pad,
+                                                                             R.plen.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    GBX_TRY(exclusive_scan_u32(ctx, R.plen.as<uint32_t>(), R.pstart.as<uint32_t>(), R.n, R.total.as<uint32_t>()));
+    GBX_HIP(ctx, hipMemcpyAsync(&R.padded, R.total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // a padded total that wrapped around 32 bits is smaller than E
+    if ((uint64_t)R.padded < E || R.padded > 0xFFFFFF00u)
+        return set_error(ctx, GBX_ERR_RANGE, "binned layout: padded length %u for %u edges", R.padded, E);
+    GBX_HIP(ctx, hipMemcpyAsync(R.pstart.as<uint32_t>() + R.n, &R.padded, 4, hipMemcpyHostToDevice, ctx->stream));
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GBX_OK;
+}
+
+static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, gbx_bplan *bp)
+{
+    const gbx_tuning &tu = ctx->tune;
+    const uint32_t N = p->N, E = p->E;
+    const uint32_t EC = tu.binned_chunk_edges, SC = std::min(tu.binned_chunk_rows, kMaxChunkRows);
+    const uint32_t SB = std::min(tu.binned_bin_rows, kMaxBinRows);
+    const uint32_t EB = (std::max(tu.binned_bin_edges, 8u) + 7u) & ~7u;
+    std::vector<uint32_t> rp(N + 1), colcnt(N);
+    GBX_HIP(ctx, hipMemcpyAsync(rp.data(), p->rp.p, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    DevBuf dcnt;
+    GBX_TRY(dev_alloc(ctx, dcnt, (size_t)N * 4 + 4));
+    GBX_TRY(fill_u32(ctx, dcnt.as<uint32_t>(), 0, N));
+    if (E) {
+        column_counts<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(p->col.as<uint32_t>(), E,
+                                                                          dcnt.as<uint32_t>());
+        GBX_HIP(ctx, hipGetLastError());
+    }
+    GBX_HIP(ctx, hipMemcpyAsync(colcnt.data(), dcnt.p, (size_t)N * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // source chunks: <= SC sources and about EC out-edges each
+    std::vector<uint32_t> clo{0}, cstart{0};
+    {
+        uint64_t e = 0, acc = 0;
+        uint32_t start = 0;
+        for (uint32_t u = 0; u < N; u++) {
+            if (u > start && (u - start >= SC || e + colcnt[u] > EC)) {
+                clo.push_back(u);
+                cstart.push_back((uint32_t)acc);
+                start = u;
+                e = 0;
+            }
+            e += colcnt[u];
+            acc += colcnt[u];
+        }
+        clo.push_back(N);
+        cstart.push_back(E);
+    }
+    const uint32_t NC = (uint32_t)clo.size() - 1;
+    // destination bins: <= SB rows and <= EB in-edges each (a longer row is a bin of its own)
+    std::vector<uint32_t> vb{0};
+    {
+        uint64_t e = 0;
+        uint32_t start = 0;
+        for (uint32_t v = 0; v < N; v++) {
+            const uint32_t d = rp[v + 1] - rp[v];
+            if (v > start && (v - start >= SB || e + d > EB)) {
+                vb.push_back(v);
+                start = v;
+                e = 0;
+            }
+            e += d;
+        }
+        vb.push_back(N);
+    }
+    const uint32_t NB = (uint32_t)vb.size() - 1;
+    if ((uint64_t)NB * NC > (1ull << 28))
+        return set_error(ctx, GBX_ERR_RANGE, "binned layout: %u bins x %u chunks too many", NB, NC);
+    bp->NC = NC;
+    bp->NB = NB;
+    std::vector<uint32_t> bin_of(N), chunk_of(N), bin_row0(vb.begin(), vb.end() - 1);
+    for (uint32_t b = 0; b < NB; b++)
+        for (uint32_t v = vb[b]; v < vb[b + 1]; v++) bin_of[v] = b;
+    for (uint32_t k = 0; k < NC; k++)
+        for (uint32_t u = clo[k]; u < clo[k + 1]; u++) chunk_of[u] = k;
+    int rc = GBX_OK;
+    DevBuf d_bin_of, d_chunk_of, d_bin_row0, d_cstart, d_bin_len;
+    auto up = [&](DevBuf &d, const void *h, size_t bytes) {
+        if (rc) return;
+        rc = dev_alloc(ctx, d, bytes + 16);
+        if (!rc && bytes && hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+            rc = set_error(ctx, GBX_ERR_HIP, "binned layout: upload");
+    };
+    up(d_bin_of, bin_of.data(), (size_t)N * 4);
+    up(d_chunk_of, chunk_of.data(), (size_t)N * 4);
+    up(d_bin_row0, bin_row0.data(), (size_t)NB * 4);
+    up(d_cstart, cstart.data(), (size_t)(NC + 1) * 4);
+    up(bp->chunk_lo, clo.data(), (size_t)(NC + 1) * 4);
+    GBX_TRY(rc);
+    GBX_TRY(dev_alloc(ctx, d_bin_len, (size_t)NB * 4 + 4));
+    GBX_TRY(fill_u32(ctx, d_bin_len.as<uint32_t>(), 0, NB));
+    GBX_TRY(dev_alloc(ctx, bp->chunk_grp, (size_t)(NC + 1) * 4));
+    if (E) {
+        const uint32_t kbits = bits_for(NB * NC - 1);
+        DevBuf rowid, key, pay, voff_table;
+        GBX_TRY(dev_alloc(ctx, rowid, (size_t)E * 4));
+        GBX_TRY(dev_alloc(ctx, key, (size_t)E * 4));
+        GBX_TRY(dev_alloc(ctx, pay, (size_t)E * 4));
+        GBX_TRY(dev_alloc(ctx, voff_table, (size_t)NB * NC * 4));
+        edge_rows<<<grid_for(N, 256, 16384), 256, 0, ctx->stream>>>(p->rp.as<uint32_t>(), N, rowid.as<uint32_t>());
+        bin_keys<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(
+            rowid.as<uint32_t>(), p->col.as<uint32_t>(), E, d_bin_of.as<uint32_t>(), d_chunk_of.as<uint32_t>(), NB,
+            NC, 0, key.as<uint32_t>(), pay.as<uint32_t>());
+        GBX_HIP(ctx, hipGetLastError());
+        GBX_TRY(radix_sort_pairs(ctx, key.as<uint32_t>(), pay.as<uint32_t>(), nullptr, E, kbits));
+        // bin-major: V positions (segments padded to `pad` values)
+        {
+            Runs R;
+            GBX_TRY(find_runs(ctx, key.as<uint32_t>(), E, pad, R));
+            bp->nseg = R.n;
+            bp->Vlen = R.padded;
+            GBX_TRY(dev_alloc(ctx, bp->dst_local, (size_t)bp->Vlen * 2 + 16));
+            GBX_HIP(ctx, hipMemsetAsync(bp->dst_local.p, 0, (size_t)bp->Vlen * 2 + 16, ctx->stream));
+            bin_major_fill<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(
+                key.as<uint32_t>(), pay.as<uint32_t>(), E, R.flag.as<uint32_t>(), R.idx.as<uint32_t>(),
+                R.start.as<uint32_t>(), R.pstart.as<uint32_t>(), R.plen.as<uint32_t>(), rowid.as<uint32_t>(),
+                d_bin_row0.as<uint32_t>(), NB, NC, bp->dst_local.as<uint16_t>(), voff_table.as<uint32_t>(),
+                d_bin_len.as<uint32_t>());
+            GBX_HIP(ctx, hipGetLastError());
+        }
+        bin_keys<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(
+            rowid.as<uint32_t>(), p->col.as<uint32_t>(), E, d_bin_of.as<uint32_t>(), d_chunk_of.as<uint32_t>(), NB,
+            NC, 1, key.as<uint32_t>(), pay.as<uint32_t>());
+        GBX_HIP(ctx, hipGetLastError());
+        GBX_TRY(radix_sort_pairs(ctx, key.as<uint32_t>(), pay.as<uint32_t>(), nullptr, E, kbits));
+        // chunk-major: source indices and group destinations (same padding)
+        {
+            Runs R;
+            GBX_TRY(find_runs(ctx, key.as<uint32_t>(), E, pad, R));
+            if (R.n != bp->nseg) return set_error(ctx, GBX_ERR_STATE, "binned layout: segment mismatch");
+            bp->Plen = R.padded;
+            GBX_TRY(dev_alloc(ctx, bp->src_local, (size_t)bp->Plen * 2 + 16));
+            GBX_TRY(dev_alloc(ctx, bp->gdst, (size_t)(bp->Plen / 8) * 4 + 16));
+            GBX_HIP(ctx, hipMemsetAsync(bp->src_local.p, 0xFF, (size_t)bp->Plen * 2 + 16, ctx->stream));
+            chunk_major_fill<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(
+                key.as<uint32_t>(), pay.as<uint32_t>(), E, R.flag.as<uint32_t>(), R.idx.as<uint32_t>(),
+                R.start.as<uint32_t>(), R.pstart.as<uint32_t>(), p->col.as<uint32_t>(), bp->chunk_lo.as<uint32_t>(),
+                NB, bp->src_local.as<uint16_t>());
+            group_destinations<<<grid_for(R.n, 256, 16384), 256, 0, ctx->stream>>>(
+                key.as<uint32_t>(), R.start.as<uint32_t>(), R.pstart.as<uint32_t>(), R.plen.as<uint32_t>(), R.n,
+                voff_table.as<uint32_t>(), bp->gdst.as<uint32_t>());
+            chunk_groups<<<grid_for(NC + 1, 256, 1024), 256, 0, ctx->stream>>>(
+                d_cstart.as<uint32_t>(), NC, R.idx.as<uint32_t>(), E, R.n, R.pstart.as<uint32_t>(),
+                bp->chunk_grp.as<uint32_t>());
+            GBX_HIP(ctx, hipGetLastError());
+            GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        }
+    } else {
+        GBX_TRY(dev_alloc(ctx, bp->dst_local, 16));
+        GBX_TRY(dev_alloc(ctx, bp->src_local, 16));
+        GBX_TRY(dev_alloc(ctx, bp->gdst, 16));
+        GBX_TRY(fill_u32(ctx, bp->chunk_grp.as<uint32_t>(), 0, NC + 1));
+    }
+    // units: V range of each bin (a single row longer than EB is split)
+    std::vector<uint32_t> bin_len(NB);
+    GBX_HIP(ctx, hipMemcpyAsync(bin_len.data(), d_bin_len.p, (size_t)NB * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<BinUnit> units;
+    uint64_t vpos = 0;
+    uint32_t n_multi = 0;
+    for (uint32_t b = 0; b < NB; b++) {
+        const uint32_t len = bin_len[b], rows = vb[b + 1] - vb[b];
+        uint32_t nu = 1;
+        if (rows == 1 && len > EB) nu = (len + EB - 1) / EB;
+        const uint32_t multi = nu > 1 ? n_multi++ : kNoMulti;
+        for (uint32_t q = 0; q < nu; q++) {
+            BinUnit un{};
+            un.row_begin = vb[b];
+            un.row_end = vb[b + 1];
+            un.vbeg = (uint32_t)(vpos + (uint64_t)q * EB);
+            un.vend = (uint32_t)(q + 1 == nu ? vpos + len : vpos + (uint64_t)(q + 1) * EB);
+            un.multi = multi;
+            un.nunits = nu;
+            units.push_back(un);
+        }
+        vpos += len;
+    }
+    if (vpos != bp->Vlen) return set_error(ctx, GBX_ERR_STATE, "binned layout: bin lengths do not add up");
+    bp->NU = (uint32_t)units.size();
+    bp->n_multi = n_multi;
+    GBX_TRY(dev_alloc(ctx, bp->units, units.size() * sizeof(BinUnit)));
+    GBX_HIP(ctx, hipMemcpyAsync(bp->units.p, units.data(), units.size() * sizeof(BinUnit), hipMemcpyHostToDevice,
+                                ctx->stream));
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (getenv("GBX_VERBOSE"))
+        fprintf(stderr,
+                "[gbx] binned layout: N=%u E=%u pad=%u chunks=%u bins=%u units=%u split_bins=%u segments=%u "
+                "src_local=%u V=%u (+%.1f%%)\n",
+                N, E, pad, NC, NB, bp->NU, n_multi, bp->nseg, bp->Plen, bp->Vlen,
+                E ? 100.0 * ((double)bp->Vlen / E - 1.0) : 0.0);
+    return GBX_OK;
+}
+
+static int build_bplan(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, gbx_bplan **out)
+{
+    gbx_bplan *bp = new (std::nothrow) gbx_bplan();
+    if (!bp) return set_error(ctx, GBX_ERR_NOMEM, "binned layout: host memory");
+    const gbx_tuning &tu = ctx->tune;
+    bp->cut[0] = tu.binned_chunk_edges;
+    bp->cut[1] = (std::max(tu.binned_bin_edges, 8u) + 7u) & ~7u;
+    bp->cut[2] = std::min(tu.binned_bin_rows, kMaxBinRows);
+    bp->cut[3] = std::min(tu.binned_chunk_rows, kMaxChunkRows);
+    bp->cut[4] = pad;
+    const int rc = build_bplan_into(ctx, p, pad, bp);
+    if (rc) {
+        delete bp;
+        return rc;
+    }
+    *out = bp;
+    return GBX_OK;
+}
+
+template <class K>
+static void grant_lds(K *kernel, size_t shm)
+{
+    if (shm > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)shm);
+}
+
+bool binned_supported(const gbx_pr *pr)
+{
+    return !pr->batch && !pr->seg && !pr->plan->segmented && pr->g->N > 0 &&
+           (pr->kind == K_PR_REF || pr->kind == K_PR_STOCH);
+}
+
+int binned_create(gbx_pr *pr)
+{
+    gbx_ctx *ctx = pr->ctx;
+    gbx_plan *p = pr->plan;
+    const gbx_tuning &tu = ctx->tune;
+    // segments padded to whole 128-byte lines of V
+    const uint32_t pad = (uint32_t)(128 / pr->elem);
+    const uint32_t want[5] = {tu.binned_chunk_edges, (std::max(tu.binned_bin_edges, 8u) + 7u) & ~7u,
+                              std::min(tu.binned_bin_rows, kMaxBinRows), std::min(tu.binned_chunk_rows, kMaxChunkRows),
+                              pad};
+    std::shared_ptr<gbx_bplan> bp;
+    for (auto &c : p->bplans)
+        if (memcmp(c->cut, want, sizeof(want)) == 0) bp = c;
+    if (!bp) {
+        gbx_bplan *raw = nullptr;
+        GBX_TRY(build_bplan(ctx, p, pad, &raw));
+        bp = std::shared_ptr<gbx_bplan>(raw);
+        p->bplans.push_back(bp);
+    }
+    gbx_binned_state *st = new (std::nothrow) gbx_binned_state();
+    if (!st) return set_error(ctx, GBX_ERR_NOMEM, "binned state: host memory");
+    pr->bin = st;
+    st->bp = bp;
+    GBX_TRY(dev_alloc(ctx, st->V, (size_t)bp->Vlen * pr->elem + 16));
+    GBX_TRY(dev_alloc(ctx, st->acc, (size_t)bp->n_multi * 8 + 8));
+    GBX_TRY(dev_alloc(ctx, st->cnt, (size_t)bp->n_multi * 4 + 4));
+    GBX_HIP(ctx, hipMemsetAsync(st->acc.p, 0, (size_t)bp->n_multi * 8 + 8, ctx->stream));
+    GBX_HIP(ctx, hipMemsetAsync(st->cnt.p, 0, (size_t)bp->n_multi * 4 + 4, ctx->stream));
+    if (bp->NU > 16384) GBX_TRY(dev_alloc(ctx, pr->partials, (size_t)bp->NU * sizeof(Partial)));
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    st->shm_scatter = (size_t)bp->cut[3] * pr->elem;
+    st->shm_accum = (size_t)bp->cut[2] * 8;
+    if (pr->elem == 4) {
+        grant_lds(&bin_scatter_kernel<float>, st->shm_scatter);
+        grant_lds(&bin_accum_kernel<K_PR_REF, float>, st->shm_accum);
+        grant_lds(&bin_accum_kernel<K_PR_STOCH, float>, st->shm_accum);
+    } else {
+        grant_lds(&bin_scatter_kernel<uint16_t>, st->shm_scatter);
+        grant_lds(&bin_accum_kernel<K_PR_REF, uint16_t>, st->shm_accum);
+        grant_lds(&bin_accum_kernel<K_PR_STOCH, uint16_t>, st->shm_accum);
+    }
+    return GBX_OK;
+}
+
+template <class T>
+static void launch_binned(gbx_pr *pr, const IterView &A)
+{
+    const gbx_binned_state *st = pr->bin;
+    const gbx_bplan *bp = st->bp.get();
+    hipStream_t stream = pr->ctx->stream;
+    ScatterView S;
+    S.chunk_lo = bp->chunk_lo.as<uint32_t>();
+    S.chunk_grp = bp->chunk_grp.as<uint32_t>();
+    S.src_local = bp->src_local.as<uint16_t>();
+    S.gdst = bp->gdst.as<uint32_t>();
+    S.NC = bp->NC;
+    T *V = st->V.as<T>();
+    if (bp->NC)
+        bin_scatter_kernel<T><<<bp->NC, kBlock, st->shm_scatter, stream>>>(S, A.slots, A.t,
+                                                                         reinterpret_cast<const T *>(A.c_in), V);
+    AccumView B;
+    B.units = bp->units.as<BinUnit>();
+    B.dst_local = bp->dst_local.as<uint16_t>();
+    B.acc = st->acc.as<long long>();
+    B.cnt = st->cnt.as<uint32_t>();
+    const uint32_t *od = pr->plan->outdeg.as<uint32_t>();
+    if (pr->kind == K_PR_REF)
+        bin_accum_kernel<K_PR_REF, T><<<bp->NU, kBlock, st->shm_accum, stream>>>(B, od, pr->plan->N, V, A);
+    else
+        bin_accum_kernel<K_PR_STOCH, T><<<bp->NU, kBlock, st->shm_accum, stream>>>(B, od, pr->plan->N, V, A);
+}
+
+int binned_launch_iter(gbx_pr *pr)
+{
+    IterView A;
+    A.r = pr->r.as<float>();
+    A.c_in = pr->c[pr->t & 1].p;
+    A.c_out = pr->c[(pr->t + 1) & 1].p;
+    A.tele = pr->has_tele ? pr->tele.as<float>() : nullptr;
+    A.slots = pr->slots.as<Slot>();
+    A.partials = pr->partials.as<Partial>();
+    A.counter = pr->counter.as<uint32_t>();
+    A.t = pr->t;
+    A.hot = 0;
+    A.damp = pr->p.damp;
+    A.eps = pr->p.epsilon;
+    A.stop = pr->p.stop;
+    if (pr->elem == 4) launch_binned<float>(pr, A);
+    else launch_binned<uint16_t>(pr, A);
+    GBX_HIP(pr->ctx, hipGetLastError());
+    pr->t++;
+    return GBX_OK;
+}
+
+void binned_state_free(gbx_binned_state *s) { delete s; }
+
+}  // namespace gbx
+
+gbx_plan::~gbx_plan() = default;

@@ -242,6 +242,11 @@ void tuning_defaults(gbx_tuning *t)
     t->ppr_chunk = 2048;
     t->ppr_blocks_per_cu = 2;
     t->ppr_vectors_per_lane = 1;
+    t->binned_min_vertices = 1u << 21;
+    t->binned_chunk_edges = 1u << 18;
+    t->binned_bin_edges = 1u << 18;
+    t->binned_bin_rows = 20000;
+    t->binned_chunk_rows = 32768;
 }
 
 bool tuning_valid(const gbx_tuning *t)
@@ -253,6 +258,9 @@ bool tuning_valid(const gbx_tuning *t)
     if (t->lds_bytes > 160 * 1024 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return false;
     if (t->ppr_hub < 64 || t->ppr_chunk < 256 || t->ppr_blocks_per_cu < 1 || t->ppr_blocks_per_cu > 64)
         return false;
+    if (t->binned_chunk_edges < 1 || t->binned_bin_edges < 8) return false;
+    if (t->binned_bin_rows < 1 || t->binned_bin_rows > 20000) return false;
+    if (t->binned_chunk_rows < 1 || t->binned_chunk_rows > 32768) return false;
     const uint32_t v = t->ppr_vectors_per_lane;
     return v == 1 || v == 2 || v == 4;
 }
@@ -273,7 +281,12 @@ static void tuning_from_env(gbx_tuning *t)
                 {"GBX_PPR_HUB", &c.ppr_hub},
                 {"GBX_PPR_CHUNK", &c.ppr_chunk},
                 {"GBX_PPR_BLOCKS_PER_CU", &c.ppr_blocks_per_cu},
-                {"GBX_PPR_VPL", &c.ppr_vectors_per_lane}};
+                {"GBX_PPR_VPL", &c.ppr_vectors_per_lane},
+                {"GBX_BINNED_MIN_VERTICES", &c.binned_min_vertices},
+                {"GBX_BINNED_CHUNK_EDGES", &c.binned_chunk_edges},
+                {"GBX_BINNED_BIN_EDGES", &c.binned_bin_edges},
+                {"GBX_BINNED_BIN_ROWS", &c.binned_bin_rows},
+                {"GBX_BINNED_CHUNK_ROWS", &c.binned_chunk_rows}};
     for (auto &v : vars)
         if (const char *s = getenv(v.name)) *v.field = (uint32_t)strtoul(s, nullptr, 10);
     if (tuning_valid(&c)) *t = c;

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -88,7 +89,9 @@ struct gbx_graph {
     uint32_t N = 0, E = 0;
     bool weighted = false;
     gbx_csr_dev fwd, inv;
-    gbx_plan *plan[2] = {nullptr, nullptr};  // per direction
+    // per direction; PageRank objects hold a reference, so a plan rebuilt
+    // after a tuning change stays valid for the objects made before it
+    std::shared_ptr<gbx_plan> plan[2];
     // a batch of independent graphs stored block-diagonally: graph k owns
     // vertices [seg_off[k], seg_off[k+1]); empty for a single graph
     std::vector<uint32_t> seg_off;
@@ -123,8 +126,6 @@ uint32_t bits_for(uint32_t max_value);
 void tuning_defaults(gbx_tuning *t);
 bool tuning_valid(const gbx_tuning *t);
 
-// ---- plan teardown (gbx_pr.hip)
-void plan_free(gbx_plan *p);
 
 inline unsigned grid_for(size_t n, unsigned block, unsigned cap = 1u << 20) {
     size_t g = (n + block - 1) / block;

@@ -1,0 +1,179 @@
+// gbx_iter.hpp -- per-iteration state, the FP64 grid reduction and the
+// PageRank update epilogue shared by the pull iteration (gbx_pr.hip) and the
+// binned iteration (gbx_binned.hip).
+#pragma once
+
+#include "gbx_plan.hpp"
+
+namespace gbx {
+
+struct IterView {
+    float *__restrict__ r;
+    const void *__restrict__ c_in;
+    void *__restrict__ c_out;
+    const float *__restrict__ tele;
+    Slot *slots;
+    Partial *partials;
+    uint32_t *counter;
+    uint32_t t;
+    uint32_t hot;  // vertices of the contribution vector held in LDS
+    float damp;
+    double eps;
+    int stop;
+};
+
+// ------------------------------------------------------------------ reductions
+
+__device__ __forceinline__ void wave_reduce(Partial &p)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        p.err += __shfl_xor(p.err, off, 64);
+        p.dang += __shfl_xor(p.dang, off, 64);
+        p.sum += __shfl_xor(p.sum, off, 64);
+        p.active += __shfl_xor(p.active, off, 64);
+    }
+}
+
+// Block partial -> partials[blockIdx]; the last block reduces all partials
+// in block order and calls fin(total) on thread 0.
+template <class Fin>
+__device__ __forceinline__ void grid_reduce(Partial p, Partial *partials, uint32_t *counter, Fin fin)
+{
+    __shared__ Partial sp[kBlock / 64];
+    __shared__ bool last;
+    wave_reduce(p);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sp[wid] = p;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Partial q = sp[0];
+        for (int w = 1; w < kBlock / 64; w++) {
+            q.err += sp[w].err;
+            q.dang += sp[w].dang;
+            q.sum += sp[w].sum;
+            q.active += sp[w].active;
+        }
+        partials[blockIdx.x] = q;
+        __threadfence();
+        uint32_t ticket = atomicAdd(counter, 1u);
+        last = ticket == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    Partial q = {0.0, 0.0, 0.0, 0u, 0u};
+    const volatile Partial *vp = partials;
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
+        q.err += vp[b].err;
+        q.dang += vp[b].dang;
+        q.sum += vp[b].sum;
+        q.active += vp[b].active;
+    }
+    wave_reduce(q);
+    __syncthreads();
+    if (lane == 0) sp[wid] = q;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Partial tot = sp[0];
+        for (int w = 1; w < kBlock / 64; w++) {
+            tot.err += sp[w].err;
+            tot.dang += sp[w].dang;
+            tot.sum += sp[w].sum;
+            tot.active += sp[w].active;
+        }
+        *counter = 0;
+        fin(tot);
+    }
+}
+
+// ------------------------------------------------------------------ PageRank
+
+template <int KIND, class T>
+struct PrEpi {
+    float *__restrict__ r;
+    T *__restrict__ c_out;
+    const uint32_t *__restrict__ outdeg;
+    const float *__restrict__ tele;
+    float base, damp, fa, fs, fd, uni;
+    double invN, eps;
+    Partial part;
+    __device__ __forceinline__ float init(uint32_t) const { return 0.0f; }
+    __device__ __forceinline__ void operator()(uint32_t i, float acc)
+    {
+        const float r_old = ld_stream(r + i);
+        float r_new;
+        const uint32_t od = ld_stream(outdeg + i);
+        if constexpr (KIND == K_PR_REF) {
+            r_new = base + damp * acc;  // pageRank.c:324 (no contraction: -ffp-contract=off)
+        } else if constexpr (KIND == K_ADJ) {
+            // lambda[u] = g[u] + d (sum_v P[u,v] lambda[v] + [u dangling] Lambda/N)
+            r_new = ld_stream(tele + i) + fs * (acc + (od ? 0.0f : fd));
+        } else {
+            const float ti = tele ? ld_stream(tele + i) : uni;
+            r_new = fa * ti + fs * (acc + fd);
+        }
+        st_stream(r + i, r_new);
+        float cn;
+        if constexpr (KIND == K_ADJ) cn = r_new;
+        else if constexpr (KIND == K_PR_STOCH_W) cn = od ? r_new : 0.0f;
+        else cn = od ? r_new / (float)od : 0.0f;  // pageRank.c:297-301
+        store_val(c_out, i, cn);
+        const float diff = fabsf(r_new - r_old);
+        if constexpr (KIND == K_PR_REF) part.err += (double)diff * invN;
+        else part.err += (double)diff;
+        part.active += ((double)diff >= eps) ? 1u : 0u;
+        if constexpr (KIND == K_ADJ) part.dang += (double)fabsf(r_new);  // |lambda|_1 for the relative stop
+        else if (!od) part.dang += (double)r_new;
+        part.sum += (double)r_new;
+    }
+};
+
+// Update coefficients of one iteration from the previous scalar slot.
+template <int KIND, class T>
+__device__ __forceinline__ PrEpi<KIND, T> make_epi(const uint32_t *outdeg, uint32_t N, const IterView &A,
+                                                   const Slot *cur)
+{
+    PrEpi<KIND, T> epi;
+    epi.r = A.r;
+    epi.c_out = reinterpret_cast<T *>(A.c_out);
+    epi.outdeg = outdeg;
+    epi.tele = A.tele;
+    epi.base = 1.0f - A.damp;  // pageRank.c:55
+    epi.damp = A.damp;
+    epi.invN = 1.0 / (double)N;
+    epi.eps = A.eps;
+    epi.uni = (float)(1.0 / (double)N);
+    epi.fa = epi.fs = epi.fd = 0.0f;
+    if constexpr (KIND == K_ADJ) {
+        epi.fs = A.damp;
+        epi.fd = (float)(cur->sum / (double)N);  // Lambda / N
+    } else if constexpr (KIND != K_PR_REF) {
+        const double S = cur->sum > 0.0 ? cur->sum : 1.0;
+        epi.fa = 1.0f - A.damp;
+        epi.fs = (float)((double)A.damp / S);
+        epi.fd = (float)(cur->dang / (double)N);
+    }
+    epi.part = {0.0, 0.0, 0.0, 0u, 0u};
+    return epi;
+}
+
+// The next scalar slot from the reduced partials, with the stop rule.
+template <int KIND>
+__device__ __forceinline__ Slot next_slot(const Partial &tot, uint32_t iters, double eps, int stop)
+{
+    Slot s;
+    s.err = tot.err;
+    s.dang = tot.dang;
+    s.sum = tot.sum;
+    s.active = tot.active;
+    s.iters = iters + 1;
+    uint32_t done = 0;
+    if (stop == GBX_STOP_ACTIVE) done = tot.active == 0;
+    else if (stop == GBX_STOP_L1) done = KIND == K_ADJ ? tot.err < eps * tot.dang : tot.err < eps;
+    s.done = done;
+    s.conv = done;
+    return s;
+}
+
+}  // namespace gbx

@@ -4,6 +4,7 @@
 
 #include "gbx_internal.hpp"
 
+#include <memory>
 #include <utility>
 #include <vector>
 
@@ -68,6 +69,8 @@ struct Partial {
 
 }  // namespace gbx
 
+struct gbx_bplan;
+
 struct gbx_plan {
     int direction = 0;
     uint32_t N = 0, E = 0;
@@ -82,15 +85,21 @@ struct gbx_plan {
     bool segmented = false;
     uint32_t nseg = 0, n_seg_tiles = 0, n_wave_slots = 0, n_seg_hubs = 0, n_seg_chunks = 0;
     gbx::DevBuf tiles, seg_ranges, row_gid, seg_chunks, seg_hub_cnt, seg_hub_part;
+    // binned (propagation-blocking) layouts, built on first use per cut;
+    // shared with the PageRank objects that use them
+    std::vector<std::shared_ptr<gbx_bplan>> bplans;
+    ~gbx_plan();
 };
 
 struct gbx_ppr_state;
 struct gbx_seg_state;
+struct gbx_binned_state;
 
 struct gbx_pr {
     gbx_ctx *ctx = nullptr;
     gbx_graph *g = nullptr;
     gbx_plan *plan = nullptr;
+    std::shared_ptr<gbx_plan> plan_ref;  // keeps the plan alive for this object
     gbx_pr_params p;
     int kind = 0;
     size_t elem = 4;      // bytes per contribution (4: f32, 2: bf16)
@@ -104,6 +113,7 @@ struct gbx_pr {
     gbx::Slot *h_slot = nullptr;  // pinned
     gbx_ppr_state *batch = nullptr;  // multi-vector state (gbx_ppr.hip), else null
     gbx_seg_state *seg = nullptr;    // batch-of-graphs state (gbx_batch.hip), else null
+    gbx_binned_state *bin = nullptr; // binned-method state (gbx_binned.hip), else null
     // captured iteration sequences: key = (iterations << 1) | starting parity
     std::vector<std::pair<uint64_t, hipGraphExec_t>> graph_cache;
 };
@@ -395,6 +405,12 @@ int seg_reset(gbx_pr *pr);
 int seg_launch_iter(gbx_pr *pr);
 int seg_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep);
 void seg_state_free(gbx_seg_state *s);
+
+// Binned-method hooks implemented in gbx_binned.hip.
+bool binned_supported(const gbx_pr *pr);
+int binned_create(gbx_pr *pr);       // builds the plan's binned layout on first use
+int binned_launch_iter(gbx_pr *pr);  // scatter + accumulate: one iteration
+void binned_state_free(gbx_binned_state *s);
 
 // Host helpers implemented in gbx_pr.hip.
 int ensure_plan(gbx_ctx *ctx, gbx_graph *g, int direction, gbx_plan **out);

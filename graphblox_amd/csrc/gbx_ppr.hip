@@ -612,6 +612,7 @@ int gbx_ppr_create(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, uint32_t 
     pr->ctx = ctx;
     pr->g = g;
     pr->plan = plan;
+    pr->plan_ref = g->plan[p->direction];
     pr->p = *p;
     pr->kind = K_PR_STOCH;
     pr->batch = new (std::nothrow) gbx_ppr_state();

@@ -26,132 +26,10 @@
 #include <cstdlib>
 #include <vector>
 
-#include "gbx_plan.hpp"
+#include "gbx_iter.hpp"
 
 
 namespace gbx {
-
-struct IterView {
-    float *__restrict__ r;
-    const void *__restrict__ c_in;
-    void *__restrict__ c_out;
-    const float *__restrict__ tele;
-    Slot *slots;
-    Partial *partials;
-    uint32_t *counter;
-    uint32_t t;
-    uint32_t hot;  // vertices of the contribution vector held in LDS
-    float damp;
-    double eps;
-    int stop;
-};
-
-// ------------------------------------------------------------------ reductions
-
-__device__ __forceinline__ void wave_reduce(Partial &p)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        p.err += __shfl_xor(p.err, off, 64);
-        p.dang += __shfl_xor(p.dang, off, 64);
-        p.sum += __shfl_xor(p.sum, off, 64);
-        p.active += __shfl_xor(p.active, off, 64);
-    }
-}
-
-// Block partial -> partials[blockIdx]; the last block reduces all partials
-// in block order and calls fin(total) on thread 0.
-template <class Fin>
-__device__ __forceinline__ void grid_reduce(Partial p, Partial *partials, uint32_t *counter, Fin fin)
-{
-    __shared__ Partial sp[kBlock / 64];
-    __shared__ bool last;
-    wave_reduce(p);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) sp[wid] = p;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        Partial q = sp[0];
-        for (int w = 1; w < kBlock / 64; w++) {
-            q.err += sp[w].err;
-            q.dang += sp[w].dang;
-            q.sum += sp[w].sum;
-            q.active += sp[w].active;
-        }
-        partials[blockIdx.x] = q;
-        __threadfence();
-        uint32_t ticket = atomicAdd(counter, 1u);
-        last = ticket == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    Partial q = {0.0, 0.0, 0.0, 0u, 0u};
-    const volatile Partial *vp = partials;
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
-        q.err += vp[b].err;
-        q.dang += vp[b].dang;
-        q.sum += vp[b].sum;
-        q.active += vp[b].active;
-    }
-    wave_reduce(q);
-    __syncthreads();
-    if (lane == 0) sp[wid] = q;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        Partial tot = sp[0];
-        for (int w = 1; w < kBlock / 64; w++) {
-            tot.err += sp[w].err;
-            tot.dang += sp[w].dang;
-            tot.sum += sp[w].sum;
-            tot.active += sp[w].active;
-        }
-        *counter = 0;
-        fin(tot);
-    }
-}
-
-// ------------------------------------------------------------------ PageRank
-
-template <int KIND, class T>
-struct PrEpi {
-    float *__restrict__ r;
-    T *__restrict__ c_out;
-    const uint32_t *__restrict__ outdeg;
-    const float *__restrict__ tele;
-    float base, damp, fa, fs, fd, uni;
-    double invN, eps;
-    Partial part;
-    __device__ __forceinline__ float init(uint32_t) const { return 0.0f; }
-    __device__ __forceinline__ void operator()(uint32_t i, float acc)
-    {
-        const float r_old = ld_stream(r + i);
-        float r_new;
-        const uint32_t od = ld_stream(outdeg + i);
-        if constexpr (KIND == K_PR_REF) {
-            r_new = base + damp * acc;  // pageRank.c:324 (no contraction: -ffp-contract=off)
-        } else if constexpr (KIND == K_ADJ) {
-            // lambda[u] = g[u] + d (sum_v P[u,v] lambda[v] + [u dangling] Lambda/N)
-            r_new = ld_stream(tele + i) + fs * (acc + (od ? 0.0f : fd));
-        } else {
-            const float ti = tele ? ld_stream(tele + i) : uni;
-            r_new = fa * ti + fs * (acc + fd);
-        }
-        st_stream(r + i, r_new);
-        float cn;
-        if constexpr (KIND == K_ADJ) cn = r_new;
-        else if constexpr (KIND == K_PR_STOCH_W) cn = od ? r_new : 0.0f;
-        else cn = od ? r_new / (float)od : 0.0f;  // pageRank.c:297-301
-        store_val(c_out, i, cn);
-        const float diff = fabsf(r_new - r_old);
-        if constexpr (KIND == K_PR_REF) part.err += (double)diff * invN;
-        else part.err += (double)diff;
-        part.active += ((double)diff >= eps) ? 1u : 0u;
-        if constexpr (KIND == K_ADJ) part.dang += (double)fabsf(r_new);  // |lambda|_1 for the relative stop
-        else if (!od) part.dang += (double)r_new;
-        part.sum += (double)r_new;
-    }
-};
 
 // One iteration.  T: contribution element type (float or bf16 bits);
 // Acc: row accumulator; LDS: hot prefix of the contributions in LDS.
@@ -166,27 +44,7 @@ __global__ void __launch_bounds__(kBlock) pr_iter_kernel(PlanView P, IterView A)
     }
     const T *c_in = reinterpret_cast<const T *>(A.c_in);
     if constexpr (LDS) load_hot(c_in, A.hot);
-    PrEpi<KIND, T> epi;
-    epi.r = A.r;
-    epi.c_out = reinterpret_cast<T *>(A.c_out);
-    epi.outdeg = P.outdeg;
-    epi.tele = A.tele;
-    epi.base = 1.0f - A.damp;  // pageRank.c:55
-    epi.damp = A.damp;
-    epi.invN = 1.0 / (double)P.N;
-    epi.eps = A.eps;
-    epi.uni = (float)(1.0 / (double)P.N);
-    if constexpr (KIND == K_ADJ) {
-        epi.fa = 0.0f;
-        epi.fs = A.damp;
-        epi.fd = (float)(cur->sum / (double)P.N);  // Lambda / N
-    } else if constexpr (KIND != K_PR_REF) {
-        const double S = cur->sum > 0.0 ? cur->sum : 1.0;
-        epi.fa = 1.0f - A.damp;
-        epi.fs = (float)((double)A.damp / S);
-        epi.fd = (float)(cur->dang / (double)P.N);
-    }
-    epi.part = {0.0, 0.0, 0.0, 0u, 0u};
+    PrEpi<KIND, T> epi = make_epi<KIND, T>(P.outdeg, P.N, A, cur);
     if constexpr (KIND == K_PR_STOCH_W || KIND == K_ADJ) {
         GatherCW<T, LDS> g{c_in, reinterpret_cast<const T *>(P.wt), A.hot};
         tile_loop<Acc>(P, g, epi);
@@ -197,20 +55,8 @@ __global__ void __launch_bounds__(kBlock) pr_iter_kernel(PlanView P, IterView A)
     const uint32_t iters = cur->iters;
     const double eps = A.eps;
     const int stop = A.stop;
-    grid_reduce(epi.part, A.partials, A.counter, [&](const Partial &tot) {
-        Slot s;
-        s.err = tot.err;
-        s.dang = tot.dang;
-        s.sum = tot.sum;
-        s.active = tot.active;
-        s.iters = iters + 1;
-        uint32_t done = 0;
-        if (stop == GBX_STOP_ACTIVE) done = tot.active == 0;
-        else if (stop == GBX_STOP_L1) done = KIND == K_ADJ ? tot.err < eps * tot.dang : tot.err < eps;
-        s.done = done;
-        s.conv = done;
-        *nxt = s;
-    });
+    grid_reduce(epi.part, A.partials, A.counter,
+                [&](const Partial &tot) { *nxt = next_slot<KIND>(tot, iters, eps, stop); });
 }
 
 // r0, c0 and slot 0 (sum and dangling mass of r0).
@@ -702,7 +548,7 @@ int ensure_plan(gbx_ctx *ctx, gbx_graph *g, int direction, gbx_plan **out)
 {
     if (direction != GBX_PULL_IN && direction != GBX_PULL_FWD)
         return set_error(ctx, GBX_ERR_ARG, "direction %d not in {0,1}", direction);
-    gbx_plan *&slot = g->plan[direction];
+    std::shared_ptr<gbx_plan> &slot = g->plan[direction];
     if (slot) {
         // rebuild when the tuning's bins differ from the ones the plan was cut with
         const BinConfig bc = bin_config(ctx);
@@ -711,12 +557,15 @@ int ensure_plan(gbx_ctx *ctx, gbx_graph *g, int direction, gbx_plan **out)
         built.hub_chunk = slot->bins_thr_chunk[7];
         if (!(built == bc)) {
             GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            plan_free(slot);
-            slot = nullptr;
+            slot.reset();
         }
     }
-    if (!slot) GBX_TRY(build_plan(ctx, g, direction, &slot));
-    *out = g->plan[direction];
+    if (!slot) {
+        gbx_plan *raw = nullptr;
+        GBX_TRY(build_plan(ctx, g, direction, &raw));
+        slot.reset(raw);
+    }
+    *out = slot.get();
     return GBX_OK;
 }
 
@@ -786,7 +635,6 @@ static int ensure_row_normalised(gbx_ctx *ctx, gbx_graph *g, gbx_plan *p)
     return GBX_OK;
 }
 
-void plan_free(gbx_plan *p) { delete p; }
 
 int plan_rows_at_least(gbx_ctx *ctx, const gbx_plan *p, uint32_t threshold, uint32_t *count)
 {
@@ -895,6 +743,7 @@ static int launch_iter(gbx_pr *pr)
 {
     if (pr->batch) return ppr_launch_iter(pr);
     if (pr->seg) return seg_launch_iter(pr);
+    if (pr->bin) return binned_launch_iter(pr);
     gbx_ctx *ctx = pr->ctx;
     PlanView P = plan_view(pr->plan, plan_weights(pr));
     IterView A = iter_view(pr);
@@ -986,6 +835,7 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
         pr->ctx = ctx;
         pr->g = g;
         pr->plan = plan;
+        pr->plan_ref = g->plan[p->direction];
         pr->p = *p;
         pr->kind = K_PR_STOCH;
         int rc = seg_create(pr);
@@ -1009,6 +859,7 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
     pr->ctx = ctx;
     pr->g = g;
     pr->plan = plan;
+    pr->plan_ref = g->plan[p->direction];
     pr->p = *p;
     pr->kind = p->mode == GBX_PR_REF ? K_PR_REF : (p->weighted ? K_PR_STOCH_W : K_PR_STOCH);
     if (kind_override >= 0) pr->kind = kind_override;
@@ -1030,6 +881,20 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
         rc = set_error(ctx, GBX_ERR_HIP, "gbx_pr_create: event creation");
     if (!rc && hipHostMalloc((void **)&pr->h_slot, sizeof(Slot), 0) != hipSuccess)
         rc = set_error(ctx, GBX_ERR_NOMEM, "gbx_pr_create: pinned slot");
+    if (!rc) {
+        bool binned = false;
+        if (p->method == GBX_METHOD_BINNED) {
+            if (!binned_supported(pr))
+                rc = set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: the binned method runs REF or unweighted "
+                                                 "STOCHASTIC PageRank on one graph");
+            binned = true;
+        } else if (p->method == GBX_METHOD_AUTO) {
+            binned = kind_override < 0 && binned_supported(pr) && g->N >= ctx->tune.binned_min_vertices;
+        } else if (p->method != GBX_METHOD_PULL) {
+            rc = set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: method %d", p->method);
+        }
+        if (!rc && binned) rc = binned_create(pr);
+    }
     if (rc) {
         gbx_pr_free(pr);
         return rc;
@@ -1232,6 +1097,7 @@ int gbx_pr_free(gbx_pr *pr)
     if (pr->h_slot) (void)hipHostFree(pr->h_slot);
     if (pr->batch) ppr_state_free(pr->batch);
     if (pr->seg) seg_state_free(pr->seg);
+    if (pr->bin) binned_state_free(pr->bin);
     for (auto &e : pr->graph_cache) (void)hipGraphExecDestroy(e.second);
     delete pr;
     return GBX_OK;

@@ -566,7 +566,6 @@ int gbx_graph_free(gbx_graph *g)
     if (!g) return GBX_OK;
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
-    for (auto &p : g->plan) plan_free(p);
     delete g;
     return GBX_OK;
 }

@@ -149,6 +149,13 @@ int gbx_graph_free(gbx_graph *g);
 #define GBX_F32 0            /* FP32 contributions and row sums */
 #define GBX_BF16 1           /* BF16 contribution gathers, FP32 Kahan row sums */
 
+#define GBX_METHOD_AUTO 0    /* binned for large graphs (tuning.binned_min_vertices), else pull */
+#define GBX_METHOD_PULL 1    /* degree-binned pull: one gather of the source contribution per edge */
+#define GBX_METHOD_BINNED 2  /* propagation blocking: contributions scattered into destination-range
+                                bins with sequential writes, then summed per bin in LDS with exact
+                                fixed-point integer adds (deterministic); REF and unweighted
+                                STOCHASTIC only */
+
 #define GBX_PULL_IN 0        /* rows = transpose CSR (DIRECTED=1, pageRank.c:266-268) */
 #define GBX_PULL_FWD 1       /* rows = forward CSR (DIRECTED=0, pageRank.c:269-272) */
 
@@ -161,7 +168,7 @@ typedef struct gbx_pr_params {
     int precision;       /* GBX_F32 / GBX_BF16 */
     int direction;       /* GBX_PULL_IN / GBX_PULL_FWD */
     int weighted;        /* 1: transition = edge weight / sum of the source's out-weights */
-    int reserved;
+    int method;          /* GBX_METHOD_* */
 } gbx_pr_params;
 
 typedef struct gbx_pr_report {
@@ -257,7 +264,8 @@ int gbx_graph_batch_info(const gbx_graph *g, uint32_t *num_graphs, uint32_t *ver
  * MI355X and can be overridden per context (and at gbx_init from the
  * environment: GBX_BIN_THRESHOLDS, GBX_HUB_CHUNK, GBX_LDS_BYTES,
  * GBX_BLOCKS_PER_CU, GBX_PPR_HUB, GBX_PPR_CHUNK, GBX_PPR_BLOCKS_PER_CU,
- * GBX_PPR_VPL).  A change applies to plans and objects created afterwards;
+ * GBX_PPR_VPL, GBX_BINNED_MIN_VERTICES, GBX_BINNED_CHUNK_EDGES,
+ * GBX_BINNED_BIN_EDGES, GBX_BINNED_BIN_ROWS, GBX_BINNED_CHUNK_ROWS).  A change applies to plans and objects created afterwards;
  * results stay within the same tolerances for every setting. */
 typedef struct gbx_tuning {
     uint32_t bin_thresholds[7];    /* degree thresholds: hub, then 64, 32, 16, 8, 4, 2 lanes per row */
@@ -268,7 +276,12 @@ typedef struct gbx_tuning {
     uint32_t ppr_chunk;            /* batched form: edges per chunk */
     uint32_t ppr_blocks_per_cu;    /* batched form and graph batches: blocks per CU */
     uint32_t ppr_vectors_per_lane; /* batched form: contributions per lane (1, 2 or 4) */
-    uint32_t reserved[4];
+    uint32_t binned_min_vertices;  /* GBX_METHOD_AUTO picks the binned method from this many vertices */
+    uint32_t binned_chunk_edges;   /* binned: out-edges per source chunk (one scatter block each) */
+    uint32_t binned_bin_edges;     /* binned: in-edges per destination bin (one accumulate block each) */
+    uint32_t binned_bin_rows;      /* binned: destination rows per bin (<= 20000: LDS int64 sums) */
+    uint32_t binned_chunk_rows;    /* binned: sources per chunk (<= 32768: LDS contributions) */
+    uint32_t reserved[3];
 } gbx_tuning;
 
 int gbx_tuning_get(const gbx_ctx *ctx, gbx_tuning *t);
