@@ -1,10 +1,17 @@
 """Benchmark: FP32 pull PageRank on R-MAT scale 24 on one MI355X.
 
-A step is one pull PageRank iteration (gather + update + FP64 reductions,
-one kernel) over the whole device-resident graph.  Prints one JSON line:
-GTEPS (E edges per iteration / wall time per iteration), the HBM roofline of
-the iteration kernel (algorithmic bytes 8E + 16N + 4 per launch over its mean
-HIP-event duration) and the GraphBlox reference's OpenMP pull PageRank timed
+A step is one PageRank iteration (reference semantics, pageRank.c) over the
+whole device-resident graph.  The headline runs the library's default method
+for this size, the binned (propagation-blocking) iteration: a scatter kernel
+writes every edge's source contribution into destination-range bins and an
+accumulate kernel sums the bins in LDS and applies the update, with the same
+ranks as the pull iteration (within 1e-5 L1).  The plain pull kernel (one
+gather per edge) is reported beside it, with the BF16 and 32-vector forms.
+
+Prints one JSON line: GTEPS (E edges per iteration / wall time per
+iteration), the HBM roofline of the iteration (algorithmic bytes of the pull
+formulation, 8E + 16N + 4, over the mean HIP-event duration of the
+iteration's kernels) and the GraphBlox reference's OpenMP pull PageRank timed
 on this host's cores on the same graph.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--scale 24]
@@ -128,6 +135,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the BF16 and batched forms")
+    ap.add_argument("--method", choices=("binned", "pull"), default="binned",
+                    help="iteration of the headline (binned: the library default at this size)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="run only a plain device copy of this many GiB (reference for PMC byte counters)")
     args = ap.parse_args()
@@ -144,48 +153,65 @@ def main():
     ctx = gbx.Context(0)
     t0 = time.perf_counter()
     g = ctx.rmat_graph(args.scale, args.edge_factor, SEED)
-    log(f"R-MAT s{args.scale}: N={g.N} E={g.E} built on GPU in {time.perf_counter() - t0:.2f}s")
-    pr = g.session(iterations=args.warmup + args.steps, epsilon=0.0, mode=gbx.PR_REF, stop=gbx.STOP_NONE)
+    N, E = g.N, g.E
+    log(f"R-MAT s{args.scale}: N={N} E={E} built on GPU in {time.perf_counter() - t0:.2f}s")
+    method = gbx.METHOD_BINNED if args.method == "binned" else gbx.METHOD_PULL
+    t0 = time.perf_counter()
+    pr = g.session(iterations=args.warmup + args.steps, epsilon=0.0, mode=gbx.PR_REF, stop=gbx.STOP_NONE,
+                   method=method)
+    log(f"{args.method} iteration set up in {time.perf_counter() - t0:.2f}s")
     pr.reset()
     pr.launch(args.warmup)
     ctx.sync()
     t0 = time.perf_counter()
-    kernel_ms = pr.launch_timed(args.steps)  # HIP events around every iteration kernel; syncs at the end
+    kernel_ms = pr.launch_timed(args.steps)  # HIP events around every iteration's kernels; syncs at the end
     ctx.sync()
     wall = time.perf_counter() - t0
     ranks, rep = pr.read()
     assert rep["executed"] == args.warmup + args.steps
     assert np.isfinite(ranks).all()
     ms_per_step = wall * 1e3 / args.steps
-    N, E = g.N, g.E
     value = E / (ms_per_step * 1e-3) / 1e9
     bytes_per_launch = 8 * E + 16 * N + 4
     achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
-    wk = f"pr_pull_f32_rmat_s{args.scale}"
+    wk = f"pr_{args.method}_f32_rmat_s{args.scale}"
     traffic = traffic_from_profiles(wk)
-    log(f"{ms_per_step:.4f} ms/iter wall, kernel {kernel_ms:.4f} ms, {value:.1f} GTEPS, "
+    log(f"{ms_per_step:.4f} ms/iter wall, kernels {kernel_ms:.4f} ms, {value:.1f} GTEPS, "
         f"{achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of peak)")
-    for c in pr.plan_info():
-        log(f"  bin lanes={c['lanes']:>3} rows={c['rows']:>9} edges={c['edges']:>10}")
     forms = None
     if not args.no_forms:
-        import graphblox_amd as gbx
-
         forms = {}
-        km, wm, s16 = time_form(g, args.warmup, args.steps, mode=gbx.PR_REF, precision=gbx.BF16)
-        _, rep16 = s16.read()
-        s16.free()
-        forms["bf16_gather"] = form_record("BF16 gathers, FP32 Kahan rows, same graph", E, N, 1, km, wm,
-                                           6 * E + 18 * N + 4, value)
-        log(f"BF16 form: {forms['bf16_gather']}")
+        other = "pull" if args.method == "binned" else "binned"
+        other_m = gbx.METHOD_PULL if other == "pull" else gbx.METHOD_BINNED
+        km, wm, s1 = time_form(g, args.warmup, args.steps, mode=gbx.PR_REF, method=other_m)
+        r1, _ = s1.read()
+        s1.free()
+        base_pull = E / (wm * 1e-3) / 1e9 if other == "pull" else value
+        forms[f"{other}_f32"] = form_record(f"FP32 {other} iteration, same graph", E, N, 1, km, wm,
+                                            8 * E + 16 * N + 4)
+        forms[f"{other}_f32"]["l1_vs_headline"] = float(np.abs(r1.astype(np.float64) - ranks).sum())
+        log(f"{other} FP32 form: {forms[f'{other}_f32']}")
+        for mname, m in (("binned", gbx.METHOD_BINNED), ("pull", gbx.METHOD_PULL)):
+            km, wm, s16 = time_form(g, args.warmup, args.steps, mode=gbx.PR_REF, precision=gbx.BF16, method=m)
+            s16.free()
+            rec = form_record(f"BF16 contributions, {mname} iteration, same graph", E, N, 1, km, wm,
+                              6 * E + 18 * N + 4, value)
+            rec["speedup_vs_plain_pull_fp32"] = round(rec["gteps"] / base_pull, 3)
+            forms[f"bf16_{mname}"] = rec
+            log(f"BF16 {mname} form: {rec}")
         # 32 personalised vectors on R-MAT scale 22 (seed sets of 1..1000 vertices)
         gb = ctx.rmat_graph(min(args.scale, 22), args.edge_factor, SEED)
         Nb, Eb = gb.N, gb.E
-        km32, wm32, s32 = time_form(gb, args.warmup, args.steps, mode=gbx.PR_STOCHASTIC)
+        km32, wm32, s32 = time_form(gb, args.warmup, args.steps, mode=gbx.PR_STOCHASTIC, method=gbx.METHOD_PULL)
         s32.free()
         base32 = Eb / (wm32 * 1e-3) / 1e9
-        forms["fp32_same_graph_as_batched"] = form_record("plain FP32 (stochastic)", Eb, Nb, 1, km32, wm32,
-                                                          8 * Eb + 16 * Nb + 4)
+        forms["fp32_pull_same_graph_as_batched"] = form_record("plain FP32 pull (stochastic)", Eb, Nb, 1, km32,
+                                                               wm32, 8 * Eb + 16 * Nb + 4)
+        km32b, wm32b, s32b = time_form(gb, args.warmup, args.steps, mode=gbx.PR_STOCHASTIC,
+                                       method=gbx.METHOD_BINNED)
+        s32b.free()
+        forms["fp32_binned_same_graph_as_batched"] = form_record("FP32 binned (stochastic)", Eb, Nb, 1, km32b,
+                                                                 wm32b, 8 * Eb + 16 * Nb + 4)
         rng = np.random.default_rng(SEED)
         sets = [rng.choice(Nb, int(rng.integers(1, 1001)), replace=False) for _ in range(32)]
         kb, wb, sb = time_form(gb, args.warmup, args.steps, num_vectors=32, seed_sets=sets)
@@ -194,6 +220,8 @@ def main():
         sb.free()
         forms["batched_ppr32"] = form_record("32 personalised vectors, one multi-vector launch", Eb, Nb, 32, kb, wb,
                                              4 * (Nb + 1) + 4 * Nb + 4 * Eb + 32 * (4 * Eb + 8 * Nb), base32)
+        forms["batched_ppr32"]["speedup_vs_binned_fp32_same_graph"] = round(
+            forms["batched_ppr32"]["gteps"] / forms["fp32_binned_same_graph_as_batched"]["gteps"], 3)
         log(f"batched form: {forms['batched_ppr32']}")
         gb.free()
     cpu = None
@@ -217,9 +245,9 @@ def main():
         "config": {
             "workload": f"pull PageRank, reference semantics (pageRank.c), FP32, R-MAT scale {args.scale} "
                         f"edge factor {args.edge_factor} a=.57 b=.19 c=.19 d=.05, ids permuted, "
-                        f"self loops and duplicates removed",
+                        f"self loops and duplicates removed; {args.method} iteration",
             "scale": args.scale, "edge_factor": args.edge_factor, "seed": SEED,
-            "num_vertices": N, "num_edges": E, "parallelism": "single GPU",
+            "num_vertices": N, "num_edges": E, "parallelism": "single GPU", "method": args.method,
         },
         "roofline": {
             "bound": "hbm",
@@ -228,7 +256,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "pr_iter_kernel<K_PR_REF>",
+            "kernel": ("bin_scatter_kernel<float> + bin_accum_kernel<K_PR_REF, float> (one iteration)"
+                       if args.method == "binned" else "pr_iter_kernel<K_PR_REF>"),
             "kernel_ms": round(kernel_ms, 5),
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },

@@ -11,7 +11,7 @@
 //               LDS, and for every destination bin the chunk's edges into
 //               that bin are written as one contiguous segment of V.  The
 //               chunk's source indices (src_local, u16) are stored segment
-//               by segment, each segment padded to a multiple of 8, with the
+//               by segment, each segment padded to a multiple of 32, with the
 //               V position of every group of 8 (gdst, u32): the pass streams
 //               src_local and gdst and writes V in runs, with no search.
 //   accumulate  one block per unit (a destination bin of at most 20000 rows,
@@ -30,10 +30,10 @@
 //
 // Layout (built once per plan, device resident): V is bin-major, within a
 // bin chunk-major, within a (bin, chunk) segment in the plan's edge order;
-// every bin's region is padded to a multiple of 8 values (pads: value 0,
-// row 0) so the accumulate pass reads 8 values per lane.  src_local is in
-// chunk-major order (segments of one chunk contiguous, each padded to 8 with
-// the marker 0xFFFF).
+// every segment is padded to a multiple of 32 values (pads: value 0, row 0,
+// written by the scatter pass, so V is written in whole lines) and the
+// accumulate pass reads 8 values per lane.  src_local is in chunk-major
+// order, segments padded the same way with the marker 0xFFFF.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -49,6 +49,21 @@ namespace gbx {
 constexpr uint32_t kMaxBinRows = 20000;     // LDS int64 row sums of one bin
 constexpr uint32_t kMaxChunkRows = 32768;   // LDS contributions of one chunk; u16 local ids
 constexpr uint32_t kNoMulti = 0xFFFFFFFFu;
+
+// Sources per chunk: a multiple of 8 (chunks start at multiples of 8) in [8, 32768].
+inline uint32_t chunk_rows_of(const gbx_tuning &tu)
+{
+    return std::max(8u, std::min(tu.binned_chunk_rows, kMaxChunkRows) & ~7u);
+}
+
+// Edges per chunk and per bin: the tuning value, or (0) E / CUs clamped to
+// [64K, 512K] (measured on MI355X: 256K best at R-MAT s22, 512K at s24).
+inline uint32_t cut_edges_of(uint32_t v, const gbx_ctx *ctx, uint32_t E)
+{
+    if (v) return v;
+    const uint32_t cus = (uint32_t)std::max(1, ctx->num_cus);
+    return std::min(1u << 19, std::max(1u << 16, E / cus));
+}
 
 struct BinUnit {
     uint32_t row_begin, row_end;  // rows of the bin (plan order)
@@ -235,7 +250,15 @@ __global__ void __launch_bounds__(kBlock) bin_scatter_kernel(ScatterView S, cons
     const uint32_t k = blockIdx.x;
     T *cl = reinterpret_cast<T *>(gbx_dyn_lds);
     const uint32_t lo = S.chunk_lo[k], n = S.chunk_lo[k + 1] - lo;
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock) cl[i] = c_in[lo + i];
+    {
+        // chunk starts are multiples of 8 sources: 16-byte loads
+        constexpr uint32_t per = 16 / sizeof(T);
+        const uint32_t nv = n / per;
+        const uint4 *src = reinterpret_cast<const uint4 *>(c_in + lo);
+        uint4 *dst = reinterpret_cast<uint4 *>(cl);
+        for (uint32_t i = threadIdx.x; i < nv; i += kBlock) dst[i] = src[i];
+        for (uint32_t i = nv * per + threadIdx.x; i < n; i += kBlock) cl[i] = c_in[lo + i];
+    }
     __syncthreads();
     const uint32_t g0 = S.chunk_grp[k], g1 = S.chunk_grp[k + 1];
     for (uint32_t base = g0 + threadIdx.x; base < g1; base += kBlock * kScatterUnroll) {
@@ -297,6 +320,9 @@ __device__ __forceinline__ long long to_fixed(float v, double scale)
     return __double2ll_rn((double)v * scale);
 }
 
+constexpr int kAccumUnroll = 4;  // groups of 8 per lane in flight
+constexpr int kEpiBatch = 8;     // rows per thread whose update inputs load together
+
 template <int KIND, class T>
 __global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const uint32_t *__restrict__ outdeg,
                                                            uint32_t N, const T *__restrict__ V, IterView A)
@@ -321,11 +347,17 @@ __global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const ui
         // one row: lane sums, block sum, then the split-bin combine if any
         __shared__ long long wsum[kBlock / 64];
         long long s = 0;
-        for (uint32_t e = u.vbeg + threadIdx.x * 8u; e < u.vend; e += stride) {
-            float v[8];
-            load8(V, e, v);
+        for (uint32_t e0 = u.vbeg + threadIdx.x * 8u; e0 < u.vend; e0 += stride * kAccumUnroll) {
+            float v[kAccumUnroll][8];
 #pragma unroll
-            for (int i = 0; i < 8; i++) s += to_fixed(v[i], scale);
+            for (int q = 0; q < kAccumUnroll; q++)
+                if (e0 + q * stride < u.vend) load8(V, e0 + q * stride, v[q]);
+#pragma unroll
+            for (int q = 0; q < kAccumUnroll; q++)
+                if (e0 + q * stride < u.vend) {
+#pragma unroll
+                    for (int i = 0; i < 8; i++) s += to_fixed(v[q][i], scale);
+                }
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -353,31 +385,52 @@ __global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const ui
         long long *y = reinterpret_cast<long long *>(gbx_dyn_lds);
         for (uint32_t i = threadIdx.x; i < nrows; i += kBlock) y[i] = 0;
         __syncthreads();
-        for (uint32_t e = u.vbeg + threadIdx.x * 8u; e < u.vend; e += stride) {
-            const uint4 dw = *reinterpret_cast<const uint4 *>(B.dst_local + e);
-            const uint32_t w[4] = {dw.x, dw.y, dw.z, dw.w};
-            float v[8];
-            load8(V, e, v);
-            // runs of equal rows (segments are in row order) are summed in
-            // registers before one LDS atomic per run
-            uint32_t rd = w[0] & 0xFFFFu;
-            long long run = 0;
+        for (uint32_t e0 = u.vbeg + threadIdx.x * 8u; e0 < u.vend; e0 += stride * kAccumUnroll) {
+            // all loads of the unrolled groups first, then the sums
+            uint4 dw[kAccumUnroll];
+            float v[kAccumUnroll][8];
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const uint32_t d = (i & 1) ? (w[i >> 1] >> 16) : (w[i >> 1] & 0xFFFFu);
-                const long long q = to_fixed(v[i], scale);
-                if (d != rd) {
-                    if (run) atomicAdd(reinterpret_cast<unsigned long long *>(y + rd), (unsigned long long)run);
-                    rd = d;
-                    run = q;
-                } else {
-                    run += q;
+            for (int q = 0; q < kAccumUnroll; q++)
+                if (e0 + q * stride < u.vend) {
+                    dw[q] = *reinterpret_cast<const uint4 *>(B.dst_local + e0 + q * stride);
+                    load8(V, e0 + q * stride, v[q]);
                 }
+#pragma unroll
+            for (int q = 0; q < kAccumUnroll; q++) {
+                if (e0 + q * stride >= u.vend) break;
+                const uint32_t w[4] = {dw[q].x, dw[q].y, dw[q].z, dw[q].w};
+                // runs of equal rows (segments are in row order) are summed
+                // in registers before one LDS atomic per run
+                uint32_t rd = w[0] & 0xFFFFu;
+                long long run = 0;
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const uint32_t d = (i & 1) ? (w[i >> 1] >> 16) : (w[i >> 1] & 0xFFFFu);
+                    const long long x = to_fixed(v[q][i], scale);
+                    if (d != rd) {
+                        if (run) atomicAdd(reinterpret_cast<unsigned long long *>(y + rd), (unsigned long long)run);
+                        rd = d;
+                        run = x;
+                    } else {
+                        run += x;
+                    }
+                }
+                if (run) atomicAdd(reinterpret_cast<unsigned long long *>(y + rd), (unsigned long long)run);
             }
-            if (run) atomicAdd(reinterpret_cast<unsigned long long *>(y + rd), (unsigned long long)run);
         }
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nrows; i += kBlock) epi(u.row_begin + i, (float)((double)y[i] * inv));
+        // the update: inputs of kEpiBatch rows per thread loaded together
+        for (uint32_t i0 = threadIdx.x; i0 < nrows; i0 += kBlock * kEpiBatch) {
+            typename PrEpi<KIND, T>::In in[kEpiBatch];
+#pragma unroll
+            for (int q = 0; q < kEpiBatch; q++)
+                if (i0 + q * kBlock < nrows) in[q] = epi.load(u.row_begin + i0 + q * kBlock);
+#pragma unroll
+            for (int q = 0; q < kEpiBatch; q++) {
+                const uint32_t i = i0 + q * kBlock;
+                if (i < nrows) epi.apply(u.row_begin + i, (float)((double)y[i] * inv), in[q]);
+            }
+        }
     }
     const uint32_t iters = cur->iters;
     const double eps = A.eps;
@@ -431,9 +484,9 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, gbx_bplan *
 {
     const gbx_tuning &tu = ctx->tune;
     const uint32_t N = p->N, E = p->E;
-    const uint32_t EC = tu.binned_chunk_edges, SC = std::min(tu.binned_chunk_rows, kMaxChunkRows);
+    const uint32_t EC = cut_edges_of(tu.binned_chunk_edges, ctx, E), SC = chunk_rows_of(tu);
     const uint32_t SB = std::min(tu.binned_bin_rows, kMaxBinRows);
-    const uint32_t EB = (std::max(tu.binned_bin_edges, 8u) + 7u) & ~7u;
+    const uint32_t EB = (std::max(cut_edges_of(tu.binned_bin_edges, ctx, E), 8u) + 7u) & ~7u;
     std::vector<uint32_t> rp(N + 1), colcnt(N);
     GBX_HIP(ctx, hipMemcpyAsync(rp.data(), p->rp.p, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
     DevBuf dcnt;
@@ -452,7 +505,8 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, gbx_bplan *
         uint64_t e = 0, acc = 0;
         uint32_t start = 0;
         for (uint32_t u = 0; u < N; u++) {
-            if (u > start && (u - start >= SC || e + colcnt[u] > EC)) {
+            // cuts only at multiples of 8 sources (16-byte aligned contributions)
+            if ((u & 7u) == 0 && u > start && (u + 8 - start > SC || e + colcnt[u] > EC)) {
                 clo.push_back(u);
                 cstart.push_back((uint32_t)acc);
                 start = u;
@@ -614,10 +668,10 @@ static int build_bplan(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, gbx_bplan **out)
     gbx_bplan *bp = new (std::nothrow) gbx_bplan();
     if (!bp) return set_error(ctx, GBX_ERR_NOMEM, "binned layout: host memory");
     const gbx_tuning &tu = ctx->tune;
-    bp->cut[0] = tu.binned_chunk_edges;
-    bp->cut[1] = (std::max(tu.binned_bin_edges, 8u) + 7u) & ~7u;
+    bp->cut[0] = cut_edges_of(tu.binned_chunk_edges, ctx, p->E);
+    bp->cut[1] = (std::max(cut_edges_of(tu.binned_bin_edges, ctx, p->E), 8u) + 7u) & ~7u;
     bp->cut[2] = std::min(tu.binned_bin_rows, kMaxBinRows);
-    bp->cut[3] = std::min(tu.binned_chunk_rows, kMaxChunkRows);
+    bp->cut[3] = chunk_rows_of(tu);
     bp->cut[4] = pad;
     const int rc = build_bplan_into(ctx, p, pad, bp);
     if (rc) {
@@ -647,11 +701,12 @@ int binned_create(gbx_pr *pr)
     gbx_ctx *ctx = pr->ctx;
     gbx_plan *p = pr->plan;
     const gbx_tuning &tu = ctx->tune;
-    // segments padded to whole 128-byte lines of V
-    const uint32_t pad = (uint32_t)(128 / pr->elem);
-    const uint32_t want[5] = {tu.binned_chunk_edges, (std::max(tu.binned_bin_edges, 8u) + 7u) & ~7u,
-                              std::min(tu.binned_bin_rows, kMaxBinRows), std::min(tu.binned_chunk_rows, kMaxChunkRows),
-                              pad};
+    // segments padded to 32 values: whole 128-byte lines of FP32 V, 64-byte
+    // halves for BF16 (measured as fast as whole lines), one layout for both
+    const uint32_t pad = 32;
+    const uint32_t want[5] = {cut_edges_of(tu.binned_chunk_edges, ctx, p->E),
+                              (std::max(cut_edges_of(tu.binned_bin_edges, ctx, p->E), 8u) + 7u) & ~7u,
+                              std::min(tu.binned_bin_rows, kMaxBinRows), chunk_rows_of(tu), pad};
     std::shared_ptr<gbx_bplan> bp;
     for (auto &c : p->bplans)
         if (memcmp(c->cut, want, sizeof(want)) == 0) bp = c;

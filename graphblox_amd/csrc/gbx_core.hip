@@ -243,8 +243,8 @@ void tuning_defaults(gbx_tuning *t)
     t->ppr_blocks_per_cu = 2;
     t->ppr_vectors_per_lane = 1;
     t->binned_min_vertices = 1u << 21;
-    t->binned_chunk_edges = 1u << 18;
-    t->binned_bin_edges = 1u << 18;
+    t->binned_chunk_edges = 0;  // automatic: E / CUs in [64K, 512K]
+    t->binned_bin_edges = 0;
     t->binned_bin_rows = 20000;
     t->binned_chunk_rows = 32768;
 }
@@ -258,7 +258,7 @@ bool tuning_valid(const gbx_tuning *t)
     if (t->lds_bytes > 160 * 1024 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return false;
     if (t->ppr_hub < 64 || t->ppr_chunk < 256 || t->ppr_blocks_per_cu < 1 || t->ppr_blocks_per_cu > 64)
         return false;
-    if (t->binned_chunk_edges < 1 || t->binned_bin_edges < 8) return false;
+    if (t->binned_bin_edges != 0 && t->binned_bin_edges < 8) return false;
     if (t->binned_bin_rows < 1 || t->binned_bin_rows > 20000) return false;
     if (t->binned_chunk_rows < 1 || t->binned_chunk_rows > 32768) return false;
     const uint32_t v = t->ppr_vectors_per_lane;

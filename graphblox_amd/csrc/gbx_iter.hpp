@@ -99,19 +99,34 @@ struct PrEpi {
     double invN, eps;
     Partial part;
     __device__ __forceinline__ float init(uint32_t) const { return 0.0f; }
-    __device__ __forceinline__ void operator()(uint32_t i, float acc)
+    // Per-row inputs of the update, loadable ahead of the row sum.
+    struct In {
+        float r_old, ti;
+        uint32_t od;
+    };
+    __device__ __forceinline__ In load(uint32_t i) const
     {
-        const float r_old = ld_stream(r + i);
+        In x;
+        x.r_old = ld_stream(r + i);
+        x.od = ld_stream(outdeg + i);
+        if constexpr (KIND == K_ADJ) x.ti = ld_stream(tele + i);
+        else if constexpr (KIND != K_PR_REF) x.ti = tele ? ld_stream(tele + i) : uni;
+        else x.ti = 0.0f;
+        return x;
+    }
+    __device__ __forceinline__ void operator()(uint32_t i, float acc) { apply(i, acc, load(i)); }
+    __device__ __forceinline__ void apply(uint32_t i, float acc, const In &x)
+    {
+        const float r_old = x.r_old;
+        const uint32_t od = x.od;
         float r_new;
-        const uint32_t od = ld_stream(outdeg + i);
         if constexpr (KIND == K_PR_REF) {
             r_new = base + damp * acc;  // pageRank.c:324 (no contraction: -ffp-contract=off)
         } else if constexpr (KIND == K_ADJ) {
             // lambda[u] = g[u] + d (sum_v P[u,v] lambda[v] + [u dangling] Lambda/N)
-            r_new = ld_stream(tele + i) + fs * (acc + (od ? 0.0f : fd));
+            r_new = x.ti + fs * (acc + (od ? 0.0f : fd));
         } else {
-            const float ti = tele ? ld_stream(tele + i) : uni;
-            r_new = fa * ti + fs * (acc + fd);
+            r_new = fa * x.ti + fs * (acc + fd);
         }
         st_stream(r + i, r_new);
         float cn;
