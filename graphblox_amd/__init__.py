@@ -47,7 +47,8 @@ class Tuning(C.Structure):
                 ("ppr_blocks_per_cu", C.c_uint32), ("ppr_vectors_per_lane", C.c_uint32),
                 ("binned_min_vertices", C.c_uint32), ("binned_chunk_edges", C.c_uint32),
                 ("binned_bin_edges", C.c_uint32), ("binned_bin_rows", C.c_uint32),
-                ("binned_chunk_rows", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("binned_chunk_rows", C.c_uint32), ("binned_hot_sources", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("bin_thresholds", "reserved")}

@@ -21,6 +21,12 @@
 //               (the pull iteration's epilogue) writes the ranks and the
 //               next contributions.
 //
+// Hot sources: the first H sources of the plan order (the most gathered;
+// about a quarter of all edges at R-MAT scale 24 for H = 16384) keep their
+// contributions in the accumulate block's LDS instead; their edges are
+// stored per bin as static (row, source) u16 pairs, 4 bytes per edge instead
+// of 12.5 through V, and never pass through the scatter.
+//
 // The fixed-point scale is a power of two chosen per iteration from the
 // previous FP64 rank sum, which bounds every row sum, so no sum can
 // overflow; integer adds are associative, so the result is independent of
@@ -70,21 +76,28 @@ struct BinUnit {
     uint32_t vbeg, vend;          // V range of this unit (multiples of 8)
     uint32_t multi;               // accumulator slot of a bin split over units, else kNoMulti
     uint32_t nunits;              // units of the bin
-    uint32_t pad[2];
+    uint32_t hbeg, hend;          // hot (row, source) pair range of this unit (multiples of 8)
 };
+
+// LDS of an accumulate block: int64 row sums, then the hot contributions.
+constexpr uint32_t kAccumLds = 160 * 1024 - 1024;
 
 }  // namespace gbx
 
 struct gbx_bplan {
     uint32_t NC = 0, NB = 0, NU = 0, nseg = 0, n_multi = 0;
     uint32_t Vlen = 0;                // padded length of V
-    uint32_t cut[5] = {0, 0, 0, 0, 0};  // chunk edges, bin edges, bin rows, chunk rows, segment pad
+    uint32_t H = 0;                   // hot sources [0, H): their edges bypass V
+    uint32_t Hlen = 0;                // padded length of the hot pair array
+    uint32_t SB = 0;                  // rows per bin (LDS int64 sums)
+    uint32_t cut[6] = {0, 0, 0, 0, 0, 0};  // chunk edges, bin edges, bin rows, chunk rows, pad, hot
     uint32_t Plen = 0;                // padded length of src_local
     gbx::DevBuf chunk_lo;             // u32[NC+1] first source of each chunk
     gbx::DevBuf chunk_grp;            // u32[NC+1] first group of 8 src_local entries of each chunk
     gbx::DevBuf src_local;            // u16[Plen] source - chunk_lo, chunk-major, segments padded (0xFFFF)
     gbx::DevBuf gdst;                 // u32[Plen/8] V position of each group of 8 src_local entries
     gbx::DevBuf dst_local;            // u16[Vlen] row - bin first row, V order
+    gbx::DevBuf hot_pairs;            // u32[Hlen] (row - bin first row) | source << 16, per bin (pad: src 0xFFFF)
     gbx::DevBuf units;                // BinUnit[NU]
 };
 
@@ -209,6 +222,40 @@ __global__ void chunk_groups(const uint32_t *__restrict__ cstart, uint32_t NC, c
     }
 }
 
+__global__ void hot_flags(const uint32_t *__restrict__ col, uint32_t E, uint32_t H, uint32_t *__restrict__ flag)
+{
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x)
+        flag[e] = col[e] < H ? 1u : 0u;
+}
+
+__global__ void gather_u32(const uint32_t *__restrict__ a, const uint32_t *__restrict__ idx, uint32_t n,
+                           uint32_t *__restrict__ out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = a[idx[i]];
+}
+
+// Hot edges -> (row - bin first row) | source << 16 at their per-bin position;
+// cold edges -> compacted row and source arrays (plan edge order kept).
+__global__ void split_hot_cold(const uint32_t *__restrict__ col, const uint32_t *__restrict__ rowid, uint32_t E,
+                               const uint32_t *__restrict__ hflag, const uint32_t *__restrict__ hrank,
+                               const uint32_t *__restrict__ bin_of, const uint32_t *__restrict__ bin_row0,
+                               const uint32_t *__restrict__ hot_before, const uint32_t *__restrict__ hot_start,
+                               uint32_t *__restrict__ hot_pairs, uint32_t *__restrict__ rowid_c,
+                               uint32_t *__restrict__ col_c)
+{
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        const uint32_t v = rowid[e], u = col[e];
+        if (hflag[e]) {
+            const uint32_t b = bin_of[v];
+            hot_pairs[hot_start[b] + (hrank[e] - hot_before[b])] = (v - bin_row0[b]) | (u << 16);
+        } else {
+            const uint32_t c = e - hrank[e];
+            rowid_c[c] = v;
+            col_c[c] = u;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ iteration kernels
 
 struct ScatterView {
@@ -292,9 +339,28 @@ __global__ void __launch_bounds__(kBlock) bin_scatter_kernel(ScatterView S, cons
 struct AccumView {
     const BinUnit *__restrict__ units;
     const uint16_t *__restrict__ dst_local;
+    const uint32_t *__restrict__ hot_pairs;
     long long *acc;  // split-bin sums
     uint32_t *cnt;   // split-bin arrival counters
+    uint32_t H;      // hot sources
+    uint32_t hot_off;  // byte offset of the hot contributions in LDS
 };
+
+// Eight hot (row, source) pairs: rows and values from the LDS contributions.
+template <class T>
+__device__ __forceinline__ void load_hot8(const uint32_t *pairs, uint32_t e, const T *ch, uint32_t (&d)[8],
+                                          float (&v)[8])
+{
+    const uint4 a = *reinterpret_cast<const uint4 *>(pairs + e);
+    const uint4 b = *reinterpret_cast<const uint4 *>(pairs + e + 4);
+    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        d[i] = w[i] & 0xFFFFu;
+        const uint32_t src = w[i] >> 16;
+        v[i] = src == 0xFFFFu ? 0.0f : load_val(ch, src);
+    }
+}
 
 template <class T>
 __device__ __forceinline__ void load8(const T *V, uint32_t e, float (&v)[8])
@@ -343,10 +409,29 @@ __global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const ui
     const double scale = ldexp(1.0, 61 - ex), inv = ldexp(1.0, ex - 61);
     const uint32_t nrows = u.row_end - u.row_begin;
     const uint32_t stride = kBlock * 8u;
+    T *ch = reinterpret_cast<T *>(gbx_dyn_lds + B.hot_off);
+    const bool has_hot = u.hend > u.hbeg;
+    if (has_hot) {
+        // the hot contributions (B.H is a multiple of 8): 16-byte loads
+        constexpr uint32_t per = 16 / sizeof(T);
+        const uint4 *src = reinterpret_cast<const uint4 *>(A.c_in);
+        uint4 *dst = reinterpret_cast<uint4 *>(ch);
+        for (uint32_t i = threadIdx.x; i < B.H / per; i += kBlock) dst[i] = src[i];
+    }
     if (nrows == 1) {
         // one row: lane sums, block sum, then the split-bin combine if any
         __shared__ long long wsum[kBlock / 64];
         long long s = 0;
+        if (has_hot) {
+            __syncthreads();
+            for (uint32_t e = u.hbeg + threadIdx.x * 8u; e < u.hend; e += stride) {
+                uint32_t d[8];
+                float v[8];
+                load_hot8(B.hot_pairs, e, ch, d, v);
+#pragma unroll
+                for (int i = 0; i < 8; i++) s += to_fixed(v[i], scale);
+            }
+        }
         for (uint32_t e0 = u.vbeg + threadIdx.x * 8u; e0 < u.vend; e0 += stride * kAccumUnroll) {
             float v[kAccumUnroll][8];
 #pragma unroll
@@ -385,6 +470,26 @@ __global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const ui
         long long *y = reinterpret_cast<long long *>(gbx_dyn_lds);
         for (uint32_t i = threadIdx.x; i < nrows; i += kBlock) y[i] = 0;
         __syncthreads();
+        for (uint32_t e = u.hbeg + threadIdx.x * 8u; e < u.hend; e += stride) {
+            uint32_t d[8];
+            float v[8];
+            load_hot8(B.hot_pairs, e, ch, d, v);
+            // hot pairs are in row order: runs of equal rows summed in registers
+            uint32_t rd = d[0];
+            long long run = 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const long long x = to_fixed(v[i], scale);
+                if (d[i] != rd) {
+                    if (run) atomicAdd(reinterpret_cast<unsigned long long *>(y + rd), (unsigned long long)run);
+                    rd = d[i];
+                    run = x;
+                } else {
+                    run += x;
+                }
+            }
+            if (run) atomicAdd(reinterpret_cast<unsigned long long *>(y + rd), (unsigned long long)run);
+        }
         for (uint32_t e0 = u.vbeg + threadIdx.x * 8u; e0 < u.vend; e0 += stride * kAccumUnroll) {
             // all loads of the unrolled groups first, then the sums
             uint4 dw[kAccumUnroll];
@@ -480,13 +585,17 @@ static int find_runs(gbx_ctx *ctx, const uint32_t *key, uint32_t E, uint32_t pad
     return GBX_OK;
 }
 
-static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, gbx_bplan *bp)
+static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H, gbx_bplan *bp)
 {
     const gbx_tuning &tu = ctx->tune;
     const uint32_t N = p->N, E = p->E;
     const uint32_t EC = cut_edges_of(tu.binned_chunk_edges, ctx, E), SC = chunk_rows_of(tu);
-    const uint32_t SB = std::min(tu.binned_bin_rows, kMaxBinRows);
+    // rows per bin: what the LDS holds next to H hot FP32 contributions (a multiple of 8)
+    const uint32_t SB = std::min(std::min(tu.binned_bin_rows, kMaxBinRows), (kAccumLds - H * 4u) / 8u) & ~7u;
     const uint32_t EB = (std::max(cut_edges_of(tu.binned_bin_edges, ctx, E), 8u) + 7u) & ~7u;
+    if (SB < 8) return set_error(ctx, GBX_ERR_ARG, "binned layout: %u hot sources leave no room for bins", H);
+    bp->H = H;
+    bp->SB = SB;
     std::vector<uint32_t> rp(N + 1), colcnt(N);
     GBX_HIP(ctx, hipMemcpyAsync(rp.data(), p->rp.p, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
     DevBuf dcnt;
@@ -499,12 +608,15 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, gbx_bplan *
     }
     GBX_HIP(ctx, hipMemcpyAsync(colcnt.data(), dcnt.p, (size_t)N * 4, hipMemcpyDeviceToHost, ctx->stream));
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    // source chunks: <= SC sources and about EC out-edges each
-    std::vector<uint32_t> clo{0}, cstart{0};
+    uint64_t Ehot = 0;
+    for (uint32_t u = 0; u < H; u++) Ehot += colcnt[u];
+    const uint32_t Ecold = E - (uint32_t)Ehot;
+    // cold source chunks: <= SC sources and about EC out-edges each, from H on
+    std::vector<uint32_t> clo{H}, cstart{0};
     {
         uint64_t e = 0, acc = 0;
-        uint32_t start = 0;
-        for (uint32_t u = 0; u < N; u++) {
+        uint32_t start = H;
+        for (uint32_t u = H; u < N; u++) {
             // cuts only at multiples of 8 sources (16-byte aligned contributions)
             if ((u & 7u) == 0 && u > start && (u + 8 - start > SC || e + colcnt[u] > EC)) {
                 clo.push_back(u);
@@ -515,8 +627,10 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, gbx_bplan *
             e += colcnt[u];
             acc += colcnt[u];
         }
-        clo.push_back(N);
-        cstart.push_back(E);
+        if (N > H) {
+            clo.push_back(N);
+            cstart.push_back(Ecold);
+        }
     }
     const uint32_t NC = (uint32_t)clo.size() - 1;
     // destination bins: <= SB rows and <= EB in-edges each (a longer row is a bin of its own)
@@ -536,83 +650,136 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, gbx_bplan *
         vb.push_back(N);
     }
     const uint32_t NB = (uint32_t)vb.size() - 1;
-    if ((uint64_t)NB * NC > (1ull << 28))
+    if ((uint64_t)NB * std::max(NC, 1u) > (1ull << 28))
         return set_error(ctx, GBX_ERR_RANGE, "binned layout: %u bins x %u chunks too many", NB, NC);
     bp->NC = NC;
     bp->NB = NB;
-    std::vector<uint32_t> bin_of(N), chunk_of(N), bin_row0(vb.begin(), vb.end() - 1);
+    std::vector<uint32_t> bin_of(N), chunk_of(N, 0), bin_row0(vb.begin(), vb.end() - 1);
     for (uint32_t b = 0; b < NB; b++)
         for (uint32_t v = vb[b]; v < vb[b + 1]; v++) bin_of[v] = b;
     for (uint32_t k = 0; k < NC; k++)
         for (uint32_t u = clo[k]; u < clo[k + 1]; u++) chunk_of[u] = k;
     int rc = GBX_OK;
-    DevBuf d_bin_of, d_chunk_of, d_bin_row0, d_cstart, d_bin_len;
+    DevBuf d_bin_of, d_chunk_of, d_bin_row0, d_cstart, d_bin_len, d_rp_bins, d_hot_start;
     auto up = [&](DevBuf &d, const void *h, size_t bytes) {
         if (rc) return;
         rc = dev_alloc(ctx, d, bytes + 16);
         if (!rc && bytes && hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
             rc = set_error(ctx, GBX_ERR_HIP, "binned layout: upload");
     };
+    std::vector<uint32_t> rp_bins(NB + 1);
+    for (uint32_t b = 0; b <= NB; b++) rp_bins[b] = rp[vb[b]];
     up(d_bin_of, bin_of.data(), (size_t)N * 4);
     up(d_chunk_of, chunk_of.data(), (size_t)N * 4);
     up(d_bin_row0, bin_row0.data(), (size_t)NB * 4);
     up(d_cstart, cstart.data(), (size_t)(NC + 1) * 4);
+    up(d_rp_bins, rp_bins.data(), (size_t)(NB + 1) * 4);
     up(bp->chunk_lo, clo.data(), (size_t)(NC + 1) * 4);
     GBX_TRY(rc);
     GBX_TRY(dev_alloc(ctx, d_bin_len, (size_t)NB * 4 + 4));
     GBX_TRY(fill_u32(ctx, d_bin_len.as<uint32_t>(), 0, NB));
     GBX_TRY(dev_alloc(ctx, bp->chunk_grp, (size_t)(NC + 1) * 4));
+    std::vector<uint32_t> hot_start(NB + 1, 0);
+    DevBuf rowid;
     if (E) {
-        const uint32_t kbits = bits_for(NB * NC - 1);
-        DevBuf rowid, key, pay, voff_table;
         GBX_TRY(dev_alloc(ctx, rowid, (size_t)E * 4));
-        GBX_TRY(dev_alloc(ctx, key, (size_t)E * 4));
-        GBX_TRY(dev_alloc(ctx, pay, (size_t)E * 4));
-        GBX_TRY(dev_alloc(ctx, voff_table, (size_t)NB * NC * 4));
         edge_rows<<<grid_for(N, 256, 16384), 256, 0, ctx->stream>>>(p->rp.as<uint32_t>(), N, rowid.as<uint32_t>());
-        bin_keys<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(
-            rowid.as<uint32_t>(), p->col.as<uint32_t>(), E, d_bin_of.as<uint32_t>(), d_chunk_of.as<uint32_t>(), NB,
+        GBX_HIP(ctx, hipGetLastError());
+    }
+    // hot edges: per bin, in plan edge order (rows ascending), padded to 8
+    DevBuf hflag, hrank, rowid_c, col_c;
+    if (E) {
+        GBX_TRY(dev_alloc(ctx, hflag, (size_t)E * 4 + 4));
+        GBX_TRY(dev_alloc(ctx, hrank, (size_t)(E + 1) * 4));
+        DevBuf htot;
+        GBX_TRY(dev_alloc(ctx, htot, 16));
+        hot_flags<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(p->col.as<uint32_t>(), E, H, hflag.as<uint32_t>());
+        GBX_HIP(ctx, hipGetLastError());
+        GBX_TRY(exclusive_scan_u32(ctx, hflag.as<uint32_t>(), hrank.as<uint32_t>(), E, htot.as<uint32_t>()));
+        GBX_HIP(ctx, hipMemcpyAsync(hrank.as<uint32_t>() + E, htot.p, 4, hipMemcpyDeviceToDevice, ctx->stream));
+        // hot edges before each bin's first edge
+        DevBuf d_hot_before;
+        GBX_TRY(dev_alloc(ctx, d_hot_before, (size_t)(NB + 1) * 4));
+        gather_u32<<<grid_for(NB + 1, 256, 1024), 256, 0, ctx->stream>>>(hrank.as<uint32_t>(), d_rp_bins.as<uint32_t>(),
+                                                                          NB + 1, d_hot_before.as<uint32_t>());
+        GBX_HIP(ctx, hipGetLastError());
+        std::vector<uint32_t> hot_before(NB + 1);
+        GBX_HIP(ctx, hipMemcpyAsync(hot_before.data(), d_hot_before.p, (size_t)(NB + 1) * 4, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if ((uint64_t)hot_before[NB] != Ehot) return set_error(ctx, GBX_ERR_STATE, "binned layout: hot edge count");
+        uint64_t hpos = 0;
+        for (uint32_t b = 0; b < NB; b++) {
+            hot_start[b] = (uint32_t)hpos;
+            hpos += ((uint64_t)(hot_before[b + 1] - hot_before[b]) + 7u) & ~7ull;
+        }
+        hot_start[NB] = (uint32_t)hpos;
+        if (hpos >= (1ull << 32)) return set_error(ctx, GBX_ERR_RANGE, "binned layout: hot pairs too many");
+        bp->Hlen = (uint32_t)hpos;
+        GBX_TRY(dev_alloc(ctx, bp->hot_pairs, (size_t)bp->Hlen * 4 + 16));
+        GBX_TRY(fill_u32(ctx, bp->hot_pairs.as<uint32_t>(), 0xFFFF0000u, bp->Hlen));
+        up(d_hot_start, hot_start.data(), (size_t)(NB + 1) * 4);
+        GBX_TRY(rc);
+        GBX_TRY(dev_alloc(ctx, rowid_c, (size_t)Ecold * 4 + 4));
+        GBX_TRY(dev_alloc(ctx, col_c, (size_t)Ecold * 4 + 4));
+        split_hot_cold<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(
+            p->col.as<uint32_t>(), rowid.as<uint32_t>(), E, hflag.as<uint32_t>(), hrank.as<uint32_t>(),
+            d_bin_of.as<uint32_t>(), d_bin_row0.as<uint32_t>(), d_hot_before.as<uint32_t>(),
+            d_hot_start.as<uint32_t>(), bp->hot_pairs.as<uint32_t>(), rowid_c.as<uint32_t>(), col_c.as<uint32_t>());
+        GBX_HIP(ctx, hipGetLastError());
+        GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    } else {
+        GBX_TRY(dev_alloc(ctx, bp->hot_pairs, 16));
+    }
+    if (Ecold) {
+        const uint32_t kbits = bits_for(NB * NC - 1);
+        DevBuf key, pay, voff_table;
+        GBX_TRY(dev_alloc(ctx, key, (size_t)Ecold * 4));
+        GBX_TRY(dev_alloc(ctx, pay, (size_t)Ecold * 4));
+        GBX_TRY(dev_alloc(ctx, voff_table, (size_t)NB * NC * 4));
+        bin_keys<<<grid_for(Ecold, 256, 16384), 256, 0, ctx->stream>>>(
+            rowid_c.as<uint32_t>(), col_c.as<uint32_t>(), Ecold, d_bin_of.as<uint32_t>(), d_chunk_of.as<uint32_t>(), NB,
             NC, 0, key.as<uint32_t>(), pay.as<uint32_t>());
         GBX_HIP(ctx, hipGetLastError());
-        GBX_TRY(radix_sort_pairs(ctx, key.as<uint32_t>(), pay.as<uint32_t>(), nullptr, E, kbits));
+        GBX_TRY(radix_sort_pairs(ctx, key.as<uint32_t>(), pay.as<uint32_t>(), nullptr, Ecold, kbits));
         // bin-major: V positions (segments padded to `pad` values)
         {
             Runs R;
-            GBX_TRY(find_runs(ctx, key.as<uint32_t>(), E, pad, R));
+            GBX_TRY(find_runs(ctx, key.as<uint32_t>(), Ecold, pad, R));
             bp->nseg = R.n;
             bp->Vlen = R.padded;
             GBX_TRY(dev_alloc(ctx, bp->dst_local, (size_t)bp->Vlen * 2 + 16));
             GBX_HIP(ctx, hipMemsetAsync(bp->dst_local.p, 0, (size_t)bp->Vlen * 2 + 16, ctx->stream));
-            bin_major_fill<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(
-                key.as<uint32_t>(), pay.as<uint32_t>(), E, R.flag.as<uint32_t>(), R.idx.as<uint32_t>(),
-                R.start.as<uint32_t>(), R.pstart.as<uint32_t>(), R.plen.as<uint32_t>(), rowid.as<uint32_t>(),
+            bin_major_fill<<<grid_for(Ecold, 256, 16384), 256, 0, ctx->stream>>>(
+                key.as<uint32_t>(), pay.as<uint32_t>(), Ecold, R.flag.as<uint32_t>(), R.idx.as<uint32_t>(),
+                R.start.as<uint32_t>(), R.pstart.as<uint32_t>(), R.plen.as<uint32_t>(), rowid_c.as<uint32_t>(),
                 d_bin_row0.as<uint32_t>(), NB, NC, bp->dst_local.as<uint16_t>(), voff_table.as<uint32_t>(),
                 d_bin_len.as<uint32_t>());
             GBX_HIP(ctx, hipGetLastError());
         }
-        bin_keys<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(
-            rowid.as<uint32_t>(), p->col.as<uint32_t>(), E, d_bin_of.as<uint32_t>(), d_chunk_of.as<uint32_t>(), NB,
+        bin_keys<<<grid_for(Ecold, 256, 16384), 256, 0, ctx->stream>>>(
+            rowid_c.as<uint32_t>(), col_c.as<uint32_t>(), Ecold, d_bin_of.as<uint32_t>(), d_chunk_of.as<uint32_t>(), NB,
             NC, 1, key.as<uint32_t>(), pay.as<uint32_t>());
         GBX_HIP(ctx, hipGetLastError());
-        GBX_TRY(radix_sort_pairs(ctx, key.as<uint32_t>(), pay.as<uint32_t>(), nullptr, E, kbits));
+        GBX_TRY(radix_sort_pairs(ctx, key.as<uint32_t>(), pay.as<uint32_t>(), nullptr, Ecold, kbits));
         // chunk-major: source indices and group destinations (same padding)
         {
             Runs R;
-            GBX_TRY(find_runs(ctx, key.as<uint32_t>(), E, pad, R));
+            GBX_TRY(find_runs(ctx, key.as<uint32_t>(), Ecold, pad, R));
             if (R.n != bp->nseg) return set_error(ctx, GBX_ERR_STATE, "binned layout: segment mismatch");
             bp->Plen = R.padded;
             GBX_TRY(dev_alloc(ctx, bp->src_local, (size_t)bp->Plen * 2 + 16));
             GBX_TRY(dev_alloc(ctx, bp->gdst, (size_t)(bp->Plen / 8) * 4 + 16));
             GBX_HIP(ctx, hipMemsetAsync(bp->src_local.p, 0xFF, (size_t)bp->Plen * 2 + 16, ctx->stream));
-            chunk_major_fill<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(
-                key.as<uint32_t>(), pay.as<uint32_t>(), E, R.flag.as<uint32_t>(), R.idx.as<uint32_t>(),
-                R.start.as<uint32_t>(), R.pstart.as<uint32_t>(), p->col.as<uint32_t>(), bp->chunk_lo.as<uint32_t>(),
+            chunk_major_fill<<<grid_for(Ecold, 256, 16384), 256, 0, ctx->stream>>>(
+                key.as<uint32_t>(), pay.as<uint32_t>(), Ecold, R.flag.as<uint32_t>(), R.idx.as<uint32_t>(),
+                R.start.as<uint32_t>(), R.pstart.as<uint32_t>(), col_c.as<uint32_t>(), bp->chunk_lo.as<uint32_t>(),
                 NB, bp->src_local.as<uint16_t>());
             group_destinations<<<grid_for(R.n, 256, 16384), 256, 0, ctx->stream>>>(
                 key.as<uint32_t>(), R.start.as<uint32_t>(), R.pstart.as<uint32_t>(), R.plen.as<uint32_t>(), R.n,
                 voff_table.as<uint32_t>(), bp->gdst.as<uint32_t>());
             chunk_groups<<<grid_for(NC + 1, 256, 1024), 256, 0, ctx->stream>>>(
-                d_cstart.as<uint32_t>(), NC, R.idx.as<uint32_t>(), E, R.n, R.pstart.as<uint32_t>(),
+                d_cstart.as<uint32_t>(), NC, R.idx.as<uint32_t>(), Ecold, R.n, R.pstart.as<uint32_t>(),
                 bp->chunk_grp.as<uint32_t>());
             GBX_HIP(ctx, hipGetLastError());
             GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -623,7 +790,7 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, gbx_bplan *
         GBX_TRY(dev_alloc(ctx, bp->gdst, 16));
         GBX_TRY(fill_u32(ctx, bp->chunk_grp.as<uint32_t>(), 0, NC + 1));
     }
-    // units: V range of each bin (a single row longer than EB is split)
+    // units: V range and hot-pair range of each bin (a single row longer than EB is split)
     std::vector<uint32_t> bin_len(NB);
     GBX_HIP(ctx, hipMemcpyAsync(bin_len.data(), d_bin_len.p, (size_t)NB * 4, hipMemcpyDeviceToHost, ctx->stream));
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -632,15 +799,20 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, gbx_bplan *
     uint32_t n_multi = 0;
     for (uint32_t b = 0; b < NB; b++) {
         const uint32_t len = bin_len[b], rows = vb[b + 1] - vb[b];
+        const uint32_t h0 = hot_start[b], hlen = hot_start[b + 1] - hot_start[b];
         uint32_t nu = 1;
-        if (rows == 1 && len > EB) nu = (len + EB - 1) / EB;
+        if (rows == 1 && (uint64_t)len + hlen > EB) nu = (uint32_t)(((uint64_t)len + hlen + EB - 1) / EB);
         const uint32_t multi = nu > 1 ? n_multi++ : kNoMulti;
+        // each unit takes an equal share (multiple of 8) of both ranges
+        const uint32_t vstep = ((len + nu - 1) / nu + 7u) & ~7u, hstep = ((hlen + nu - 1) / nu + 7u) & ~7u;
         for (uint32_t q = 0; q < nu; q++) {
             BinUnit un{};
             un.row_begin = vb[b];
             un.row_end = vb[b + 1];
-            un.vbeg = (uint32_t)(vpos + (uint64_t)q * EB);
-            un.vend = (uint32_t)(q + 1 == nu ? vpos + len : vpos + (uint64_t)(q + 1) * EB);
+            un.vbeg = (uint32_t)(vpos + std::min<uint64_t>((uint64_t)q * vstep, len));
+            un.vend = (uint32_t)(vpos + std::min<uint64_t>((uint64_t)(q + 1) * vstep, len));
+            un.hbeg = h0 + (uint32_t)std::min<uint64_t>((uint64_t)q * hstep, hlen);
+            un.hend = h0 + (uint32_t)std::min<uint64_t>((uint64_t)(q + 1) * hstep, hlen);
             un.multi = multi;
             un.nunits = nu;
             units.push_back(un);
@@ -656,14 +828,14 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, gbx_bplan *
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (getenv("GBX_VERBOSE"))
         fprintf(stderr,
-                "[gbx] binned layout: N=%u E=%u pad=%u chunks=%u bins=%u units=%u split_bins=%u segments=%u "
-                "src_local=%u V=%u (+%.1f%%)\n",
-                N, E, pad, NC, NB, bp->NU, n_multi, bp->nseg, bp->Plen, bp->Vlen,
-                E ? 100.0 * ((double)bp->Vlen / E - 1.0) : 0.0);
+                "[gbx] binned layout: N=%u E=%u hot=%u (%.1f%% of edges) pad=%u chunks=%u bins=%u (<=%u rows) "
+                "units=%u split_bins=%u segments=%u V=%u (+%.1f%% of cold edges) hot_pairs=%u\n",
+                N, E, H, E ? 100.0 * (double)Ehot / E : 0.0, pad, NC, NB, SB, bp->NU, n_multi, bp->nseg, bp->Vlen,
+                Ecold ? 100.0 * ((double)bp->Vlen / Ecold - 1.0) : 0.0, bp->Hlen);
     return GBX_OK;
 }
 
-static int build_bplan(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, gbx_bplan **out)
+static int build_bplan(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H, gbx_bplan **out)
 {
     gbx_bplan *bp = new (std::nothrow) gbx_bplan();
     if (!bp) return set_error(ctx, GBX_ERR_NOMEM, "binned layout: host memory");
@@ -673,7 +845,8 @@ static int build_bplan(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, gbx_bplan **out)
     bp->cut[2] = std::min(tu.binned_bin_rows, kMaxBinRows);
     bp->cut[3] = chunk_rows_of(tu);
     bp->cut[4] = pad;
-    const int rc = build_bplan_into(ctx, p, pad, bp);
+    bp->cut[5] = H;
+    const int rc = build_bplan_into(ctx, p, pad, H, bp);
     if (rc) {
         delete bp;
         return rc;
@@ -704,15 +877,17 @@ int binned_create(gbx_pr *pr)
     // segments padded to 32 values: whole 128-byte lines of FP32 V, 64-byte
     // halves for BF16 (measured as fast as whole lines), one layout for both
     const uint32_t pad = 32;
-    const uint32_t want[5] = {cut_edges_of(tu.binned_chunk_edges, ctx, p->E),
+    // hot sources: a multiple of 8, at most N and 65528 (u16 source ids, 0xFFFF marks pads)
+    const uint32_t H = std::min(std::min(tu.binned_hot_sources, 65528u), p->N) & ~7u;
+    const uint32_t want[6] = {cut_edges_of(tu.binned_chunk_edges, ctx, p->E),
                               (std::max(cut_edges_of(tu.binned_bin_edges, ctx, p->E), 8u) + 7u) & ~7u,
-                              std::min(tu.binned_bin_rows, kMaxBinRows), chunk_rows_of(tu), pad};
+                              std::min(tu.binned_bin_rows, kMaxBinRows), chunk_rows_of(tu), pad, H};
     std::shared_ptr<gbx_bplan> bp;
     for (auto &c : p->bplans)
         if (memcmp(c->cut, want, sizeof(want)) == 0) bp = c;
     if (!bp) {
         gbx_bplan *raw = nullptr;
-        GBX_TRY(build_bplan(ctx, p, pad, &raw));
+        GBX_TRY(build_bplan(ctx, p, pad, H, &raw));
         bp = std::shared_ptr<gbx_bplan>(raw);
         p->bplans.push_back(bp);
     }
@@ -728,7 +903,7 @@ int binned_create(gbx_pr *pr)
     if (bp->NU > 16384) GBX_TRY(dev_alloc(ctx, pr->partials, (size_t)bp->NU * sizeof(Partial)));
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     st->shm_scatter = (size_t)bp->cut[3] * pr->elem;
-    st->shm_accum = (size_t)bp->cut[2] * 8;
+    st->shm_accum = (size_t)bp->SB * 8 + (size_t)bp->H * pr->elem;
     if (pr->elem == 4) {
         grant_lds(&bin_scatter_kernel<float>, st->shm_scatter);
         grant_lds(&bin_accum_kernel<K_PR_REF, float>, st->shm_accum);
@@ -760,6 +935,9 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
     AccumView B;
     B.units = bp->units.as<BinUnit>();
     B.dst_local = bp->dst_local.as<uint16_t>();
+    B.hot_pairs = bp->hot_pairs.as<uint32_t>();
+    B.H = bp->H;
+    B.hot_off = bp->SB * 8u;
     B.acc = st->acc.as<long long>();
     B.cnt = st->cnt.as<uint32_t>();
     const uint32_t *od = pr->plan->outdeg.as<uint32_t>();

@@ -265,7 +265,8 @@ int gbx_graph_batch_info(const gbx_graph *g, uint32_t *num_graphs, uint32_t *ver
  * environment: GBX_BIN_THRESHOLDS, GBX_HUB_CHUNK, GBX_LDS_BYTES,
  * GBX_BLOCKS_PER_CU, GBX_PPR_HUB, GBX_PPR_CHUNK, GBX_PPR_BLOCKS_PER_CU,
  * GBX_PPR_VPL, GBX_BINNED_MIN_VERTICES, GBX_BINNED_CHUNK_EDGES,
- * GBX_BINNED_BIN_EDGES, GBX_BINNED_BIN_ROWS, GBX_BINNED_CHUNK_ROWS).  A change applies to plans and objects created afterwards;
+ * GBX_BINNED_BIN_EDGES, GBX_BINNED_BIN_ROWS, GBX_BINNED_CHUNK_ROWS,
+ * GBX_BINNED_HOT_SOURCES).  A change applies to plans and objects created afterwards;
  * results stay within the same tolerances for every setting. */
 typedef struct gbx_tuning {
     uint32_t bin_thresholds[7];    /* degree thresholds: hub, then 64, 32, 16, 8, 4, 2 lanes per row */
@@ -283,7 +284,10 @@ typedef struct gbx_tuning {
                                       0 = as binned_chunk_edges */
     uint32_t binned_bin_rows;      /* binned: destination rows per bin (<= 20000: LDS int64 sums) */
     uint32_t binned_chunk_rows;    /* binned: sources per chunk (<= 32768: LDS contributions) */
-    uint32_t reserved[3];
+    uint32_t binned_hot_sources;   /* binned: the first this many sources (plan order, the most
+                                      gathered) are summed from LDS in the accumulate pass, their
+                                      edges bypassing the scatter (0 = off; <= 65528) */
+    uint32_t reserved[2];
 } gbx_tuning;
 
 int gbx_tuning_get(const gbx_ctx *ctx, gbx_tuning *t);
