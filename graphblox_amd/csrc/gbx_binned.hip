@@ -106,6 +106,8 @@ struct gbx_binned_state {
     gbx::DevBuf V;    // f32 or bf16 [Vlen]
     gbx::DevBuf acc;  // i64[n_multi] sums of split bins (zero between iterations)
     gbx::DevBuf cnt;  // u32[n_multi] arrival counters (zero between iterations)
+    gbx::DevBuf work;  // u32 next accumulate unit (zero between iterations)
+    unsigned accum_grid = 0;
     size_t shm_scatter = 0, shm_accum = 0;
 };
 
@@ -344,6 +346,8 @@ struct AccumView {
     uint32_t *cnt;   // split-bin arrival counters
     uint32_t H;      // hot sources
     uint32_t hot_off;  // byte offset of the hot contributions in LDS
+    uint32_t *work;  // next unit (zero between iterations)
+    uint32_t NU;     // units
 };
 
 // Eight hot (row, source) pairs: rows and values from the LDS contributions.
@@ -389,41 +393,22 @@ __device__ __forceinline__ long long to_fixed(float v, double scale)
 constexpr int kAccumUnroll = 4;  // groups of 8 per lane in flight
 constexpr int kEpiBatch = 8;     // rows per thread whose update inputs load together
 
+// One accumulate unit: row sums of the unit's V range and hot pairs, then the
+// update of its rows (or, for a split bin, the combine).  The hot
+// contributions are already in LDS.
 template <int KIND, class T>
-__global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const uint32_t *__restrict__ outdeg,
-                                                           uint32_t N, const T *__restrict__ V, IterView A)
+__device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B, const T *__restrict__ V,
+                                           PrEpi<KIND, T> &epi, double scale, double inv)
 {
-    const Slot *cur = A.slots + (A.t & 1);
-    Slot *nxt = A.slots + ((A.t + 1) & 1);
-    if (cur->done) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = *cur;
-        return;
-    }
-    const BinUnit u = B.units[blockIdx.x];
-    PrEpi<KIND, T> epi = make_epi<KIND, T>(outdeg, N, A, cur);
-    // every row sum is at most the sum of all contributions, itself at most
-    // the previous rank sum (bf16 rounding adds < 0.4 %)
-    const double bound = cur->sum * 1.01 + 1e-30;
-    int ex = 0;
-    (void)frexp(bound, &ex);
-    const double scale = ldexp(1.0, 61 - ex), inv = ldexp(1.0, ex - 61);
     const uint32_t nrows = u.row_end - u.row_begin;
     const uint32_t stride = kBlock * 8u;
-    T *ch = reinterpret_cast<T *>(gbx_dyn_lds + B.hot_off);
+    const T *ch = reinterpret_cast<const T *>(gbx_dyn_lds + B.hot_off);
     const bool has_hot = u.hend > u.hbeg;
-    if (has_hot) {
-        // the hot contributions (B.H is a multiple of 8): 16-byte loads
-        constexpr uint32_t per = 16 / sizeof(T);
-        const uint4 *src = reinterpret_cast<const uint4 *>(A.c_in);
-        uint4 *dst = reinterpret_cast<uint4 *>(ch);
-        for (uint32_t i = threadIdx.x; i < B.H / per; i += kBlock) dst[i] = src[i];
-    }
     if (nrows == 1) {
         // one row: lane sums, block sum, then the split-bin combine if any
         __shared__ long long wsum[kBlock / 64];
         long long s = 0;
         if (has_hot) {
-            __syncthreads();
             for (uint32_t e = u.hbeg + threadIdx.x * 8u; e < u.hend; e += stride) {
                 uint32_t d[8];
                 float v[8];
@@ -537,11 +522,53 @@ __global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const ui
             }
         }
     }
+}
+
+// Persistent: one block per CU takes units from a counter until none are
+// left, so the hot contributions are loaded once per block and one unit's
+// update overlaps the next unit's loads in other blocks.
+template <int KIND, class T>
+__global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const uint32_t *__restrict__ outdeg,
+                                                           uint32_t N, const T *__restrict__ V, IterView A)
+{
+    const Slot *cur = A.slots + (A.t & 1);
+    Slot *nxt = A.slots + ((A.t + 1) & 1);
+    if (cur->done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = *cur;
+        return;
+    }
+    PrEpi<KIND, T> epi = make_epi<KIND, T>(outdeg, N, A, cur);
+    // every row sum is at most the sum of all contributions, itself at most
+    // the previous rank sum (bf16 rounding adds < 0.4 %)
+    const double bound = cur->sum * 1.01 + 1e-30;
+    int ex = 0;
+    (void)frexp(bound, &ex);
+    const double scale = ldexp(1.0, 61 - ex), inv = ldexp(1.0, ex - 61);
+    if (B.H) {
+        // the hot contributions (B.H is a multiple of 8): 16-byte loads
+        constexpr uint32_t per = 16 / sizeof(T);
+        const uint4 *src = reinterpret_cast<const uint4 *>(A.c_in);
+        uint4 *dst = reinterpret_cast<uint4 *>(gbx_dyn_lds + B.hot_off);
+        for (uint32_t i = threadIdx.x; i < B.H / per; i += kBlock) dst[i] = src[i];
+    }
+    __shared__ uint32_t s_unit;
+    for (;;) {
+        __syncthreads();  // previous unit done with LDS (and the hot load finished)
+        if (threadIdx.x == 0) s_unit = atomicAdd(B.work, 1u);
+        __syncthreads();
+        const uint32_t w = s_unit;
+        if (w >= B.NU) break;
+        const BinUnit u = B.units[w];
+        accum_unit<KIND, T>(u, B, V, epi, scale, inv);
+    }
     const uint32_t iters = cur->iters;
     const double eps = A.eps;
     const int stop = A.stop;
-    grid_reduce(epi.part, A.partials, A.counter,
-                [&](const Partial &tot) { *nxt = next_slot<KIND>(tot, iters, eps, stop); });
+    uint32_t *work = B.work;
+    grid_reduce(epi.part, A.partials, A.counter, [&](const Partial &tot) {
+        *nxt = next_slot<KIND>(tot, iters, eps, stop);
+        *work = 0;  // every block has left its unit loop
+    });
 }
 
 // ------------------------------------------------------------------ host
@@ -900,6 +927,10 @@ int binned_create(gbx_pr *pr)
     GBX_TRY(dev_alloc(ctx, st->cnt, (size_t)bp->n_multi * 4 + 4));
     GBX_HIP(ctx, hipMemsetAsync(st->acc.p, 0, (size_t)bp->n_multi * 8 + 8, ctx->stream));
     GBX_HIP(ctx, hipMemsetAsync(st->cnt.p, 0, (size_t)bp->n_multi * 4 + 4, ctx->stream));
+    GBX_TRY(dev_alloc(ctx, st->work, 16));
+    GBX_HIP(ctx, hipMemsetAsync(st->work.p, 0, 16, ctx->stream));
+    // one accumulate block per CU (the LDS holds one), never more than the units
+    st->accum_grid = std::max(1u, std::min(bp->NU, (uint32_t)std::max(1, ctx->num_cus)));
     if (bp->NU > 16384) GBX_TRY(dev_alloc(ctx, pr->partials, (size_t)bp->NU * sizeof(Partial)));
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     st->shm_scatter = (size_t)bp->cut[3] * pr->elem;
@@ -940,11 +971,13 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
     B.hot_off = bp->SB * 8u;
     B.acc = st->acc.as<long long>();
     B.cnt = st->cnt.as<uint32_t>();
+    B.work = st->work.as<uint32_t>();
+    B.NU = bp->NU;
     const uint32_t *od = pr->plan->outdeg.as<uint32_t>();
     if (pr->kind == K_PR_REF)
-        bin_accum_kernel<K_PR_REF, T><<<bp->NU, kBlock, st->shm_accum, stream>>>(B, od, pr->plan->N, V, A);
+        bin_accum_kernel<K_PR_REF, T><<<st->accum_grid, kBlock, st->shm_accum, stream>>>(B, od, pr->plan->N, V, A);
     else
-        bin_accum_kernel<K_PR_STOCH, T><<<bp->NU, kBlock, st->shm_accum, stream>>>(B, od, pr->plan->N, V, A);
+        bin_accum_kernel<K_PR_STOCH, T><<<st->accum_grid, kBlock, st->shm_accum, stream>>>(B, od, pr->plan->N, V, A);
 }
 
 int binned_launch_iter(gbx_pr *pr)
