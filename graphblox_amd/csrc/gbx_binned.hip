@@ -12,8 +12,8 @@
 //               that bin are written as one contiguous segment of V.  The
 //               chunk's source indices (src_local, u16) are stored segment
 //               by segment, each segment padded to a multiple of 32, with the
-//               V position of every group of 8 (gdst, u32): the pass streams
-//               src_local and gdst and writes V in runs, with no search.
+//               V position of every block of 32 (bdst, u32): the pass streams
+//               src_local and bdst and writes V in runs, with no search.
 //   accumulate  one block per unit (a destination bin of at most 20000 rows,
 //               or a piece of a single giant row): V and the static
 //               dst_local array are read sequentially and summed per row in
@@ -95,7 +95,7 @@ struct gbx_bplan {
     gbx::DevBuf chunk_lo;             // u32[NC+1] first source of each chunk
     gbx::DevBuf chunk_grp;            // u32[NC+1] first group of 8 src_local entries of each chunk
     gbx::DevBuf src_local;            // u16[Plen] source - chunk_lo, chunk-major, segments padded (0xFFFF)
-    gbx::DevBuf gdst;                 // u32[Plen/8] V position of each group of 8 src_local entries
+    gbx::DevBuf bdst;                 // u32[Plen/32] V position of each block of 32 src_local entries
     gbx::DevBuf dst_local;            // u16[Vlen] row - bin first row, V order
     gbx::DevBuf hot_pairs;            // u32[Hlen] (row - bin first row) | source << 16, per bin (pad: src 0xFFFF)
     gbx::DevBuf units;                // BinUnit[NU]
@@ -202,14 +202,15 @@ __global__ void chunk_major_fill(const uint32_t *__restrict__ key, const uint32_
     }
 }
 
-__global__ void group_destinations(const uint32_t *__restrict__ key, const uint32_t *__restrict__ runstart,
+// V position of every block of 32 src_local entries (segments are padded to 32).
+__global__ void block_destinations(const uint32_t *__restrict__ key, const uint32_t *__restrict__ runstart,
                                    const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ plen,
                                    uint32_t nrun, const uint32_t *__restrict__ voff_table,
-                                   uint32_t *__restrict__ gdst)
+                                   uint32_t *__restrict__ bdst)
 {
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nrun; r += gridDim.x * blockDim.x) {
-        const uint32_t g0 = pstart[r] >> 3, ng = plen[r] >> 3, v0 = voff_table[key[runstart[r]]];
-        for (uint32_t q = 0; q < ng; q++) gdst[g0 + q] = v0 + 8u * q;
+        const uint32_t b0 = pstart[r] >> 5, nb = plen[r] >> 5, v0 = voff_table[key[runstart[r]]];
+        for (uint32_t q = 0; q < nb; q++) bdst[b0 + q] = v0 + 32u * q;
     }
 }
 
@@ -264,7 +265,7 @@ struct ScatterView {
     const uint32_t *__restrict__ chunk_lo;
     const uint32_t *__restrict__ chunk_grp;
     const uint16_t *__restrict__ src_local;
-    const uint32_t *__restrict__ gdst;
+    const uint32_t *__restrict__ bdst;
     uint32_t NC;
 };
 
@@ -288,7 +289,7 @@ __device__ __forceinline__ void store8(T *V, uint32_t pos, const T (&v)[8])
 }
 
 // One block per source chunk.  Each lane takes groups of 8 entries: one
-// 16-byte load of source indices, one V position, eight LDS reads, one
+// 16-byte load of source indices, the V position of its block of 32, eight LDS reads, one
 // 32-byte (FP32) or 16-byte (BF16) store; pads (0xFFFF) store zeros, so every
 // segment is written in whole 128-byte lines.
 template <class T>
@@ -318,7 +319,7 @@ __global__ void __launch_bounds__(kBlock) bin_scatter_kernel(ScatterView S, cons
             const uint32_t g = base + (uint32_t)q * kBlock;
             if (g < g1) {
                 src[q] = reinterpret_cast<const uint4 *>(S.src_local)[g];
-                dst[q] = S.gdst[g];
+                dst[q] = S.bdst[g >> 2] + ((g & 3u) << 3);
             }
         }
 #pragma unroll
@@ -796,15 +797,15 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
             if (R.n != bp->nseg) return set_error(ctx, GBX_ERR_STATE, "binned layout: segment mismatch");
             bp->Plen = R.padded;
             GBX_TRY(dev_alloc(ctx, bp->src_local, (size_t)bp->Plen * 2 + 16));
-            GBX_TRY(dev_alloc(ctx, bp->gdst, (size_t)(bp->Plen / 8) * 4 + 16));
+            GBX_TRY(dev_alloc(ctx, bp->bdst, (size_t)(bp->Plen / 32) * 4 + 16));
             GBX_HIP(ctx, hipMemsetAsync(bp->src_local.p, 0xFF, (size_t)bp->Plen * 2 + 16, ctx->stream));
             chunk_major_fill<<<grid_for(Ecold, 256, 16384), 256, 0, ctx->stream>>>(
                 key.as<uint32_t>(), pay.as<uint32_t>(), Ecold, R.flag.as<uint32_t>(), R.idx.as<uint32_t>(),
                 R.start.as<uint32_t>(), R.pstart.as<uint32_t>(), col_c.as<uint32_t>(), bp->chunk_lo.as<uint32_t>(),
                 NB, bp->src_local.as<uint16_t>());
-            group_destinations<<<grid_for(R.n, 256, 16384), 256, 0, ctx->stream>>>(
+            block_destinations<<<grid_for(R.n, 256, 16384), 256, 0, ctx->stream>>>(
                 key.as<uint32_t>(), R.start.as<uint32_t>(), R.pstart.as<uint32_t>(), R.plen.as<uint32_t>(), R.n,
-                voff_table.as<uint32_t>(), bp->gdst.as<uint32_t>());
+                voff_table.as<uint32_t>(), bp->bdst.as<uint32_t>());
             chunk_groups<<<grid_for(NC + 1, 256, 1024), 256, 0, ctx->stream>>>(
                 d_cstart.as<uint32_t>(), NC, R.idx.as<uint32_t>(), Ecold, R.n, R.pstart.as<uint32_t>(),
                 bp->chunk_grp.as<uint32_t>());
@@ -814,7 +815,7 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     } else {
         GBX_TRY(dev_alloc(ctx, bp->dst_local, 16));
         GBX_TRY(dev_alloc(ctx, bp->src_local, 16));
-        GBX_TRY(dev_alloc(ctx, bp->gdst, 16));
+        GBX_TRY(dev_alloc(ctx, bp->bdst, 16));
         GBX_TRY(fill_u32(ctx, bp->chunk_grp.as<uint32_t>(), 0, NC + 1));
     }
     // units: V range and hot-pair range of each bin (a single row longer than EB is split)
@@ -957,7 +958,7 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
     S.chunk_lo = bp->chunk_lo.as<uint32_t>();
     S.chunk_grp = bp->chunk_grp.as<uint32_t>();
     S.src_local = bp->src_local.as<uint16_t>();
-    S.gdst = bp->gdst.as<uint32_t>();
+    S.bdst = bp->bdst.as<uint32_t>();
     S.NC = bp->NC;
     T *V = st->V.as<T>();
     if (bp->NC)
