@@ -441,6 +441,80 @@ double orc_pr_loss_f64(uint32_t N, const uint64_t *row_ptr, const uint32_t *col,
     return L;
 }
 
+/* ---- fixed-point pull PageRank (pageRank.c:496-1030, fixedPoint.h)
+ *
+ * Per iteration: c[u] = conv_bits(r[u] / outdeg[u]) (0 for outdeg 0), an
+ * exact uint64 sum of c over the in-edges of each row, then
+ * r' = base + d * back_bits(sum).  The conversions follow fixedPoint.h:
+ *   64: DoubleToFixed64 = (u64)((double)x * 2^32 + 0.5), back = (double)s / 2^32,
+ *       update in double then rounded to float (pageRank.c:577);
+ *   32: FloatToFixed32 = (u32)(x * 65536.0f + 0.5f), back = (float)s / 65536.0f;
+ *   16: FloatToFixed16 = (u16)(x * 16384.0f + 0.5f), back = (float)s / 16384.0f;
+ *    8: FloatToFixed8  = (u8)(x * 64.0f + 0.5f),     back = (float)s / 64.0f;
+ * float to u16/u8 goes through a 32-bit signed truncation and keeps the low
+ * bits, as x86-64 code compiled from the reference does; the push form of
+ * the 64-bit variant (pageRank.c:1032-1157) sums the same integers. */
+static uint64_t fixed_conv(float x, int bits)
+{
+    switch (bits) {
+    case 64: return (uint64_t)((double)x * 4294967296.0 + 0.5);
+    case 32: return (uint64_t)(uint32_t)(int64_t)(x * 65536.0f + 0.5f);
+    case 16: return (uint64_t)(uint16_t)(int32_t)(x * 16384.0f + 0.5f);
+    default: return (uint64_t)(uint8_t)(int32_t)(x * 64.0f + 0.5f);
+    }
+}
+
+static float fixed_update(float base, float damp, uint64_t s, int bits)
+{
+    switch (bits) {
+    case 64: return (float)((double)base + (double)damp * ((double)s / 4294967296.0));
+    case 32: return base + damp * ((float)s / 65536.0f);
+    case 16: return base + damp * ((float)s / 16384.0f);
+    default: return base + damp * ((float)s / 64.0f);
+    }
+}
+
+int orc_pr_pull_fixed(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
+                      const uint32_t *col, int bits, uint32_t max_iters, double epsilon,
+                      float *ranks_out, uint32_t *iters_out, double *err_out)
+{
+    if (bits != 64 && bits != 32 && bits != 16 && bits != 8) return -1;
+    uint64_t *c = malloc((size_t)N * sizeof(uint64_t) + 8);
+    if (!c) return -1;
+    const float damp = 0.85f, base = 1.0f - damp; /* pageRank.c:55 */
+    float *r = ranks_out;
+    for (uint32_t v = 0; v < N; v++) r[v] = base;
+    uint32_t it = 0;
+    double err = 0.0;
+    for (it = 0; it < max_iters; it++) {
+#pragma omp parallel for
+        for (uint32_t v = 0; v < N; v++)
+            c[v] = outdeg[v] ? fixed_conv(r[v] / (float)outdeg[v], bits) : 0;
+        uint32_t active = 0;
+        err = 0.0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : err, active)
+        for (uint32_t v = 0; v < N; v++) {
+            uint64_t s = 0;
+            for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) s += c[col[j] & VERTEX_ID_MASK];
+            const float prev = r[v];
+            const float next = fixed_update(base, damp, s, bits);
+            r[v] = next;
+            const double e = fabs((double)next - (double)prev);
+            err += e / N;
+            if (e >= epsilon) active++;
+        }
+        if (active == 0) {
+            it++;
+            break;
+        }
+    }
+    for (uint32_t v = 0; v < N; v++) r[v] /= N;
+    if (iters_out) *iters_out = it < max_iters ? it - 1 : it;
+    if (err_out) *err_out = err;
+    free(c);
+    return 0;
+}
+
 /* ---- R-MAT generator (counter-based; identical integer arithmetic on GPU) */
 
 static uint64_t mix64(uint64_t z)

@@ -110,6 +110,15 @@ double orc_pr_loss_diff(uint32_t N, const uint64_t *row_ptr, const uint32_t *col
                         const double *w_plus, const double *w_minus, const double *teleport,
                         const double *g, double damp, const double *p0, uint32_t iters);
 
+/* The reference's fixed-point pull PageRank (pageRankPullFixedPoint{64,32,
+ * 16,8}BitGraphCSR, pageRank.c:496-1030; pushpull codes 2, 9, 10, 11, and
+ * 3 = the push form of 64): in-CSR rows, REF update, stop when no vertex
+ * changes by epsilon or more; ranks divided by N at the end.  *iters_out
+ * follows PageRankStats.iterations (the stopping iteration's index). */
+int orc_pr_pull_fixed(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
+                      const uint32_t *col, int bits, uint32_t max_iters, double epsilon,
+                      float *ranks_out, uint32_t *iters_out, double *err_out);
+
 /* Seeded counter-based R-MAT generator (same integer arithmetic as the GPU
  * generator).  Writes num_edges = edge_factor << scale edges, ids permuted
  * by the seeded bijection on [0, 2^scale). */

@@ -64,6 +64,8 @@ def lib():
         L.orc_pr_loss_diff.argtypes = [C.c_uint32, _u64p, _u32p, _f64p, _f64p, C.c_void_p, _f64p, C.c_double,
                                        C.c_void_p, C.c_uint32]
         L.orc_pr_loss_diff.restype = C.c_double
+        L.orc_pr_pull_fixed.argtypes = [C.c_uint32, _u32p, _u64p, _u32p, C.c_int, C.c_uint32, C.c_double, _f32p,
+                                        C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
         L.orc_rmat_generate.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                         C.c_uint64, _u32p, _u32p]
         L.orc_rmat_permute.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64]
@@ -96,6 +98,8 @@ def ref(directed: int = 1, weighted: int = 0):
         L.refd_free.argtypes = [C.c_void_p]
         L.refd_pagerank.argtypes = [C.c_void_p, C.c_uint32, C.c_double, C.c_int, _f32p,
                                     C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.refd_pagerank_variant.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_double, C.c_int, _f32p,
+                                            C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
         L.refd_spmv.argtypes = [C.c_void_p, C.c_uint32, C.c_int, _f32p, _f32p, C.POINTER(C.c_double)]
         if directed:
             L.refd_pagerank_on_csr.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p, _u32p, _u32p,
@@ -239,6 +243,18 @@ def pr_f64(N, outdeg, row_ptr, col, mode=0, damp=0.85, iterations=20, epsilon=0.
                      np.ascontiguousarray(col, np.uint32), wp.ctypes.data if wp is not None else None,
                      tp.ctypes.data if tp is not None else None, damp, mode, iterations, epsilon, r,
                      C.byref(it), C.byref(err))
+    return r, it.value, err.value
+
+
+def pr_pull_fixed(N, outdeg, row_ptr, col, bits, iterations=20, epsilon=0.0):
+    """The reference's fixed-point pull PageRank (64/32/16/8-bit)."""
+    r = np.zeros(N, np.float32)
+    it, err = C.c_uint32(), C.c_double()
+    rc = lib().orc_pr_pull_fixed(N, np.ascontiguousarray(outdeg, np.uint32), np.ascontiguousarray(row_ptr, np.uint64),
+                                 np.ascontiguousarray(col, np.uint32), bits, iterations, epsilon, r, C.byref(it),
+                                 C.byref(err))
+    if rc:
+        raise ValueError(f"bits {bits}")
     return r, it.value, err.value
 
 

@@ -131,6 +131,27 @@ int refd_pagerank(void *gp, uint32_t iterations, double epsilon, int nthreads,
     return 0;
 }
 
+/* pageRankGraphCSR with a variant code (arguments->pushpull, pageRank.c:177-242). */
+int refd_pagerank_variant(void *gp, int pushpull, uint32_t iterations, double epsilon, int nthreads,
+                          float *ranks_out, uint32_t *iters_out, double *err_out)
+{
+    struct Arguments *a = argumentsNew();
+    a->iterations = iterations;
+    a->epsilon = epsilon;
+    a->pushpull = pushpull;
+    a->ker_numThreads = nthreads > 0 ? nthreads : 1;
+    omp_set_num_threads(a->ker_numThreads);
+    int s = quiet_begin();
+    struct PageRankStats *st = pageRankGraphCSR(a, (struct GraphCSR *)gp);
+    quiet_end(s);
+    if (ranks_out) memcpy(ranks_out, st->pageRanks, (size_t)st->num_vertices * 4);
+    if (iters_out) *iters_out = st->iterations;
+    if (err_out) *err_out = st->error_total;
+    freePageRankStats(st);
+    free(a);
+    return 0;
+}
+
 int refd_spmv(void *gp, uint32_t iterations, int nthreads, float *y_out,
               float *x_out, double *time_out)
 {

@@ -270,6 +270,52 @@ def test_adjoint_fd_fixture_pins_fp64_adjoint(golden):
     assert dd == pytest.approx(float(fx["dir_fd"]), rel=1e-5)
 
 
+VARIANT_GRAPHS = ["fixture_v500_e500", "fixture_v1024_e4096", "fixture_v8192_e32768", "fixture_v1000_e10000",
+                  "fixture_v529_e3500", "rmat_s8", "rmat_s10", "rmat_s12"]
+
+
+@pytest.mark.parametrize("name", VARIANT_GRAPHS)
+def test_fixed_point_oracle_matches_reference_variants(golden, name):
+    """pageRankPullFixedPoint{64,32,16,8}BitGraphCSR (pushpull 2, 9, 10, 11):
+    bit-exact ranks and identical stopping iterations."""
+    z, v = golden(name), golden(f"variants_{name}")
+    fwd, inv = O.build_graph(z["edges_src"], z["edges_dst"], None, int(z["m1_N"]), mode=1)
+    rp = O.row_ptr_from(inv)
+    for bits, code in ((64, 2), (32, 9), (16, 10), (8, 11)):
+        r, _, err = O.pr_pull_fixed(fwd.N, fwd.out_degree, rp, inv.dst, bits, 20, 0.0)
+        np.testing.assert_array_equal(r, v[f"pp{code}_r20"])
+        assert err == pytest.approx(float(v[f"pp{code}_err20"]), rel=1e-9)
+        r, it, _ = O.pr_pull_fixed(fwd.N, fwd.out_degree, rp, inv.dst, bits, 100, 1e-4)
+        assert it == int(v[f"pp{code}_eps_iters"])
+        np.testing.assert_array_equal(r, v[f"pp{code}_eps_ranks"])
+
+
+@pytest.mark.parametrize("name", VARIANT_GRAPHS)
+def test_reference_push_variants_equal_pull_sums(golden, name):
+    """Push (1) sums the same contributions as pull; push fixed point (3)
+    the same integers as pull fixed point 64 (2): the reference's own
+    outputs agree (one thread, so float atomics run in edge order)."""
+    z, v = golden(name), golden(f"variants_{name}")
+    assert O_l1(v["pp1_r20"], z["pr_r20"]) <= 1e-6
+    np.testing.assert_array_equal(v["pp3_r20"], v["pp2_r20"])
+    np.testing.assert_array_equal(v["pp3_eps_ranks"], v["pp2_eps_ranks"])
+
+
+def O_l1(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)).sum())
+
+
+def test_fixed_point_goldens_tell_widths_and_iterations_apart(golden):
+    # a width mixed up, or an iteration too few, would not pass the bit-exact checks
+    z, v = golden("rmat_s10"), golden("variants_rmat_s10")
+    fwd, inv = O.build_graph(z["edges_src"], z["edges_dst"], None, int(z["m1_N"]), mode=1)
+    rp = O.row_ptr_from(inv)
+    r16, _, _ = O.pr_pull_fixed(fwd.N, fwd.out_degree, rp, inv.dst, 16, 20, 0.0)
+    assert not np.array_equal(r16, v["pp9_r20"])  # a 16-bit run is not the 32-bit variant
+    r32, _, _ = O.pr_pull_fixed(fwd.N, fwd.out_degree, rp, inv.dst, 32, 19, 0.0)
+    assert not np.array_equal(r32, v["pp9_r20"])  # nor one iteration fewer
+
+
 # ---------------------------------------------------------------- generator
 
 def test_rmat_permutation_is_bijection():
