@@ -23,6 +23,7 @@ GBX_OK = 0
 SORT_RADIX_SRC, SORT_RADIX_SRC_DST, SORT_COUNT_SRC, SORT_COUNT_SRC_DST = 0, 1, 2, 3
 GRAPH_DEDUPE, GRAPH_WEIGHTED, GRAPH_SYMMETRIC = 1, 2, 4
 PR_REF, PR_STOCHASTIC = 0, 1
+PR_FIXED64, PR_FIXED32, PR_FIXED16, PR_FIXED8 = 2, 3, 4, 5
 STOP_ACTIVE, STOP_L1, STOP_NONE = 0, 1, 2
 F32, BF16 = 0, 1
 PULL_IN, PULL_FWD = 0, 1

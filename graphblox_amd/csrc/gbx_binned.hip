@@ -46,6 +46,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "gbx_iter.hpp"
@@ -56,10 +57,11 @@ constexpr uint32_t kMaxBinRows = 20000;     // LDS int64 row sums of one bin
 constexpr uint32_t kMaxChunkRows = 32768;   // LDS contributions of one chunk; u16 local ids
 constexpr uint32_t kNoMulti = 0xFFFFFFFFu;
 
-// Sources per chunk: a multiple of 8 (chunks start at multiples of 8) in [8, 32768].
-inline uint32_t chunk_rows_of(const gbx_tuning &tu)
+// Sources per chunk: a multiple of 8 (chunks start at multiples of 8) in [8, 32768],
+// and at most 128 KiB of contributions of `elem` bytes in the scatter's LDS.
+inline uint32_t chunk_rows_of(const gbx_tuning &tu, uint32_t elem)
 {
-    return std::max(8u, std::min(tu.binned_chunk_rows, kMaxChunkRows) & ~7u);
+    return std::max(8u, std::min(std::min(tu.binned_chunk_rows, kMaxChunkRows), (128u * 1024u) / elem) & ~7u);
 }
 
 // Edges per chunk and per bin: the tuning value, or (0) E / CUs clamped to
@@ -90,7 +92,7 @@ struct gbx_bplan {
     uint32_t H = 0;                   // hot sources [0, H): their edges bypass V
     uint32_t Hlen = 0;                // padded length of the hot pair array
     uint32_t SB = 0;                  // rows per bin (LDS int64 sums)
-    uint32_t cut[6] = {0, 0, 0, 0, 0, 0};  // chunk edges, bin edges, bin rows, chunk rows, pad, hot
+    uint32_t cut[7] = {0, 0, 0, 0, 0, 0, 0};  // chunk edges, bin edges, bin rows, chunk rows, pad, hot, hot bytes
     uint32_t Plen = 0;                // padded length of src_local
     gbx::DevBuf chunk_lo;             // u32[NC+1] first source of each chunk
     gbx::DevBuf chunk_grp;            // u32[NC+1] first group of 8 src_local entries of each chunk
@@ -271,21 +273,30 @@ struct ScatterView {
 
 constexpr int kScatterUnroll = 4;  // groups of 8 per lane in flight
 
+// Eight consecutive values moved as one wide access (8, 16, 32 or 64 bytes).
+template <class T>
+struct alignas(8 * sizeof(T)) Vec8 {
+    T v[8];
+};
+
 template <class T>
 __device__ __forceinline__ void store8(T *V, uint32_t pos, const T (&v)[8])
 {
-    if constexpr (sizeof(T) == 4) {
-        float4 *d = reinterpret_cast<float4 *>(V + pos);
-        d[0] = make_float4(v[0], v[1], v[2], v[3]);
-        d[1] = make_float4(v[4], v[5], v[6], v[7]);
-    } else {
-        uint4 w;
-        w.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
-        w.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
-        w.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
-        w.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
-        *reinterpret_cast<uint4 *>(V + pos) = w;
-    }
+    Vec8<T> w;
+#pragma unroll
+    for (int i = 0; i < 8; i++) w.v[i] = v[i];
+    *reinterpret_cast<Vec8<T> *>(V + pos) = w;
+}
+
+// n values (whole groups of 8 at 8-value aligned addresses, then a tail)
+// from global memory into LDS.
+template <class T>
+__device__ __forceinline__ void copy_to_lds(T *dst, const T *src, uint32_t n)
+{
+    const uint32_t nv = n / 8;
+    for (uint32_t i = threadIdx.x; i < nv; i += kBlock)
+        reinterpret_cast<Vec8<T> *>(dst)[i] = reinterpret_cast<const Vec8<T> *>(src)[i];
+    for (uint32_t i = nv * 8 + threadIdx.x; i < n; i += kBlock) dst[i] = src[i];
 }
 
 // One block per source chunk.  Each lane takes groups of 8 entries: one
@@ -300,15 +311,7 @@ __global__ void __launch_bounds__(kBlock) bin_scatter_kernel(ScatterView S, cons
     const uint32_t k = blockIdx.x;
     T *cl = reinterpret_cast<T *>(gbx_dyn_lds);
     const uint32_t lo = S.chunk_lo[k], n = S.chunk_lo[k + 1] - lo;
-    {
-        // chunk starts are multiples of 8 sources: 16-byte loads
-        constexpr uint32_t per = 16 / sizeof(T);
-        const uint32_t nv = n / per;
-        const uint4 *src = reinterpret_cast<const uint4 *>(c_in + lo);
-        uint4 *dst = reinterpret_cast<uint4 *>(cl);
-        for (uint32_t i = threadIdx.x; i < nv; i += kBlock) dst[i] = src[i];
-        for (uint32_t i = nv * per + threadIdx.x; i < n; i += kBlock) cl[i] = c_in[lo + i];
-    }
+    copy_to_lds(cl, c_in + lo, n);  // chunk starts are multiples of 8 sources
     __syncthreads();
     const uint32_t g0 = S.chunk_grp[k], g1 = S.chunk_grp[k + 1];
     for (uint32_t base = g0 + threadIdx.x; base < g1; base += kBlock * kScatterUnroll) {
@@ -351,10 +354,43 @@ struct AccumView {
     uint32_t NU;     // units
 };
 
+// Values as loaded (raw) and as 64-bit fixed point.  Float kinds keep FP32
+// (BF16 widened) and scale by 2^scale; fixed-point kinds are integers already.
+template <int KIND>
+using RawVal = std::conditional_t<is_fixed_kind(KIND), uint64_t, float>;
+
+template <int KIND, class T>
+__device__ __forceinline__ RawVal<KIND> raw_of(T x)
+{
+    if constexpr (is_fixed_kind(KIND)) return (uint64_t)x;
+    else if constexpr (sizeof(T) == 2) return __uint_as_float((uint32_t)x << 16);  // bf16
+    else return x;
+}
+
+__device__ __forceinline__ long long to_fixed(float v, double scale)
+{
+    return __double2ll_rn((double)v * scale);
+}
+
+template <int KIND>
+__device__ __forceinline__ long long fixed_of(RawVal<KIND> v, double scale)
+{
+    if constexpr (is_fixed_kind(KIND)) return (long long)v;
+    else return to_fixed(v, scale);
+}
+
+template <int KIND, class T>
+__device__ __forceinline__ void load8(const T *V, uint32_t e, RawVal<KIND> (&v)[8])
+{
+    const Vec8<T> w = *reinterpret_cast<const Vec8<T> *>(V + e);
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = raw_of<KIND, T>(w.v[i]);
+}
+
 // Eight hot (row, source) pairs: rows and values from the LDS contributions.
-template <class T>
+template <int KIND, class T>
 __device__ __forceinline__ void load_hot8(const uint32_t *pairs, uint32_t e, const T *ch, uint32_t (&d)[8],
-                                          float (&v)[8])
+                                          RawVal<KIND> (&v)[8])
 {
     const uint4 a = *reinterpret_cast<const uint4 *>(pairs + e);
     const uint4 b = *reinterpret_cast<const uint4 *>(pairs + e + 4);
@@ -363,33 +399,12 @@ __device__ __forceinline__ void load_hot8(const uint32_t *pairs, uint32_t e, con
     for (int i = 0; i < 8; i++) {
         d[i] = w[i] & 0xFFFFu;
         const uint32_t src = w[i] >> 16;
-        v[i] = src == 0xFFFFu ? 0.0f : load_val(ch, src);
+        v[i] = src == 0xFFFFu ? RawVal<KIND>(0) : raw_of<KIND, T>(ch[src]);
     }
 }
 
-template <class T>
-__device__ __forceinline__ void load8(const T *V, uint32_t e, float (&v)[8])
-{
-    if constexpr (sizeof(T) == 4) {
-        const float4 a = *reinterpret_cast<const float4 *>(V + e);
-        const float4 b = *reinterpret_cast<const float4 *>(V + e + 4);
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-    } else {
-        const uint4 a = *reinterpret_cast<const uint4 *>(V + e);
-        const uint32_t w[4] = {a.x, a.y, a.z, a.w};
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            v[2 * i] = __uint_as_float(w[i] << 16);
-            v[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
-        }
-    }
-}
-
-__device__ __forceinline__ long long to_fixed(float v, double scale)
-{
-    return __double2ll_rn((double)v * scale);
-}
+template <int KIND, class T>
+using EpiOf = std::conditional_t<is_fixed_kind(KIND), FixEpi<KIND, T>, PrEpi<KIND, T>>;
 
 constexpr int kAccumUnroll = 4;  // groups of 8 per lane in flight
 constexpr int kEpiBatch = 8;     // rows per thread whose update inputs load together
@@ -399,7 +414,7 @@ constexpr int kEpiBatch = 8;     // rows per thread whose update inputs load tog
 // contributions are already in LDS.
 template <int KIND, class T>
 __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B, const T *__restrict__ V,
-                                           PrEpi<KIND, T> &epi, double scale, double inv)
+                                           EpiOf<KIND, T> &epi, double scale, double inv)
 {
     const uint32_t nrows = u.row_end - u.row_begin;
     const uint32_t stride = kBlock * 8u;
@@ -412,22 +427,22 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
         if (has_hot) {
             for (uint32_t e = u.hbeg + threadIdx.x * 8u; e < u.hend; e += stride) {
                 uint32_t d[8];
-                float v[8];
-                load_hot8(B.hot_pairs, e, ch, d, v);
+                RawVal<KIND> v[8];
+                load_hot8<KIND, T>(B.hot_pairs, e, ch, d, v);
 #pragma unroll
-                for (int i = 0; i < 8; i++) s += to_fixed(v[i], scale);
+                for (int i = 0; i < 8; i++) s += fixed_of<KIND>(v[i], scale);
             }
         }
         for (uint32_t e0 = u.vbeg + threadIdx.x * 8u; e0 < u.vend; e0 += stride * kAccumUnroll) {
-            float v[kAccumUnroll][8];
+            RawVal<KIND> v[kAccumUnroll][8];
 #pragma unroll
             for (int q = 0; q < kAccumUnroll; q++)
-                if (e0 + q * stride < u.vend) load8(V, e0 + q * stride, v[q]);
+                if (e0 + q * stride < u.vend) load8<KIND, T>(V, e0 + q * stride, v[q]);
 #pragma unroll
             for (int q = 0; q < kAccumUnroll; q++)
                 if (e0 + q * stride < u.vend) {
 #pragma unroll
-                    for (int i = 0; i < 8; i++) s += to_fixed(v[q][i], scale);
+                    for (int i = 0; i < 8; i++) s += fixed_of<KIND>(v[q][i], scale);
                 }
         }
 #pragma unroll
@@ -450,7 +465,10 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
                     B.cnt[u.multi] = 0;
                 }
             }
-            if (apply) epi(u.row_begin, (float)((double)tot * inv));
+            if (apply) {
+                if constexpr (is_fixed_kind(KIND)) epi(u.row_begin, (uint64_t)tot);
+                else epi(u.row_begin, (float)((double)tot * inv));
+            }
         }
     } else {
         long long *y = reinterpret_cast<long long *>(gbx_dyn_lds);
@@ -458,14 +476,14 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
         __syncthreads();
         for (uint32_t e = u.hbeg + threadIdx.x * 8u; e < u.hend; e += stride) {
             uint32_t d[8];
-            float v[8];
-            load_hot8(B.hot_pairs, e, ch, d, v);
+            RawVal<KIND> v[8];
+            load_hot8<KIND, T>(B.hot_pairs, e, ch, d, v);
             // hot pairs are in row order: runs of equal rows summed in registers
             uint32_t rd = d[0];
             long long run = 0;
 #pragma unroll
             for (int i = 0; i < 8; i++) {
-                const long long x = to_fixed(v[i], scale);
+                const long long x = fixed_of<KIND>(v[i], scale);
                 if (d[i] != rd) {
                     if (run) atomicAdd(reinterpret_cast<unsigned long long *>(y + rd), (unsigned long long)run);
                     rd = d[i];
@@ -479,12 +497,12 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
         for (uint32_t e0 = u.vbeg + threadIdx.x * 8u; e0 < u.vend; e0 += stride * kAccumUnroll) {
             // all loads of the unrolled groups first, then the sums
             uint4 dw[kAccumUnroll];
-            float v[kAccumUnroll][8];
+            RawVal<KIND> v[kAccumUnroll][8];
 #pragma unroll
             for (int q = 0; q < kAccumUnroll; q++)
                 if (e0 + q * stride < u.vend) {
                     dw[q] = *reinterpret_cast<const uint4 *>(B.dst_local + e0 + q * stride);
-                    load8(V, e0 + q * stride, v[q]);
+                    load8<KIND, T>(V, e0 + q * stride, v[q]);
                 }
 #pragma unroll
             for (int q = 0; q < kAccumUnroll; q++) {
@@ -497,7 +515,7 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
 #pragma unroll
                 for (int i = 0; i < 8; i++) {
                     const uint32_t d = (i & 1) ? (w[i >> 1] >> 16) : (w[i >> 1] & 0xFFFFu);
-                    const long long x = to_fixed(v[q][i], scale);
+                    const long long x = fixed_of<KIND>(v[q][i], scale);
                     if (d != rd) {
                         if (run) atomicAdd(reinterpret_cast<unsigned long long *>(y + rd), (unsigned long long)run);
                         rd = d;
@@ -512,14 +530,17 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
         __syncthreads();
         // the update: inputs of kEpiBatch rows per thread loaded together
         for (uint32_t i0 = threadIdx.x; i0 < nrows; i0 += kBlock * kEpiBatch) {
-            typename PrEpi<KIND, T>::In in[kEpiBatch];
+            typename EpiOf<KIND, T>::In in[kEpiBatch];
 #pragma unroll
             for (int q = 0; q < kEpiBatch; q++)
                 if (i0 + q * kBlock < nrows) in[q] = epi.load(u.row_begin + i0 + q * kBlock);
 #pragma unroll
             for (int q = 0; q < kEpiBatch; q++) {
                 const uint32_t i = i0 + q * kBlock;
-                if (i < nrows) epi.apply(u.row_begin + i, (float)((double)y[i] * inv), in[q]);
+                if (i < nrows) {
+                    if constexpr (is_fixed_kind(KIND)) epi.apply(u.row_begin + i, (uint64_t)y[i], in[q]);
+                    else epi.apply(u.row_begin + i, (float)((double)y[i] * inv), in[q]);
+                }
             }
         }
     }
@@ -538,20 +559,18 @@ __global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const ui
         if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = *cur;
         return;
     }
-    PrEpi<KIND, T> epi = make_epi<KIND, T>(outdeg, N, A, cur);
-    // every row sum is at most the sum of all contributions, itself at most
-    // the previous rank sum (bf16 rounding adds < 0.4 %)
+    EpiOf<KIND, T> epi;
+    if constexpr (is_fixed_kind(KIND)) epi = make_fix_epi<KIND, T>(outdeg, N, A);
+    else epi = make_epi<KIND, T>(outdeg, N, A, cur);
+    // float kinds: every row sum is at most the sum of all contributions,
+    // itself at most the previous rank sum (bf16 rounding adds < 0.4 %);
+    // fixed-point kinds sum their integers as they are (uint64 wrap-around,
+    // as the reference's uint64 sums)
     const double bound = cur->sum * 1.01 + 1e-30;
     int ex = 0;
     (void)frexp(bound, &ex);
     const double scale = ldexp(1.0, 61 - ex), inv = ldexp(1.0, ex - 61);
-    if (B.H) {
-        // the hot contributions (B.H is a multiple of 8): 16-byte loads
-        constexpr uint32_t per = 16 / sizeof(T);
-        const uint4 *src = reinterpret_cast<const uint4 *>(A.c_in);
-        uint4 *dst = reinterpret_cast<uint4 *>(gbx_dyn_lds + B.hot_off);
-        for (uint32_t i = threadIdx.x; i < B.H / per; i += kBlock) dst[i] = src[i];
-    }
+    if (B.H) copy_to_lds(reinterpret_cast<T *>(gbx_dyn_lds + B.hot_off), reinterpret_cast<const T *>(A.c_in), B.H);
     __shared__ uint32_t s_unit;
     for (;;) {
         __syncthreads();  // previous unit done with LDS (and the hot load finished)
@@ -613,13 +632,15 @@ static int find_runs(gbx_ctx *ctx, const uint32_t *key, uint32_t E, uint32_t pad
     return GBX_OK;
 }
 
-static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H, gbx_bplan *bp)
+static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H, uint32_t hot_elem, gbx_bplan *bp)
 {
     const gbx_tuning &tu = ctx->tune;
     const uint32_t N = p->N, E = p->E;
-    const uint32_t EC = cut_edges_of(tu.binned_chunk_edges, ctx, E), SC = chunk_rows_of(tu);
-    // rows per bin: what the LDS holds next to H hot FP32 contributions (a multiple of 8)
-    const uint32_t SB = std::min(std::min(tu.binned_bin_rows, kMaxBinRows), (kAccumLds - H * 4u) / 8u) & ~7u;
+    const uint32_t EC = cut_edges_of(tu.binned_chunk_edges, ctx, E), SC = chunk_rows_of(tu, hot_elem);
+    // rows per bin: what the LDS holds next to H hot contributions of hot_elem
+    // bytes (a multiple of 8)
+    const uint32_t SB =
+        std::min(std::min(tu.binned_bin_rows, kMaxBinRows), (kAccumLds - H * hot_elem) / 8u) & ~7u;
     const uint32_t EB = (std::max(cut_edges_of(tu.binned_bin_edges, ctx, E), 8u) + 7u) & ~7u;
     if (SB < 8) return set_error(ctx, GBX_ERR_ARG, "binned layout: %u hot sources leave no room for bins", H);
     bp->H = H;
@@ -863,7 +884,7 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     return GBX_OK;
 }
 
-static int build_bplan(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H, gbx_bplan **out)
+static int build_bplan(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H, uint32_t hot_elem, gbx_bplan **out)
 {
     gbx_bplan *bp = new (std::nothrow) gbx_bplan();
     if (!bp) return set_error(ctx, GBX_ERR_NOMEM, "binned layout: host memory");
@@ -871,10 +892,11 @@ static int build_bplan(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H, gbx_
     bp->cut[0] = cut_edges_of(tu.binned_chunk_edges, ctx, p->E);
     bp->cut[1] = (std::max(cut_edges_of(tu.binned_bin_edges, ctx, p->E), 8u) + 7u) & ~7u;
     bp->cut[2] = std::min(tu.binned_bin_rows, kMaxBinRows);
-    bp->cut[3] = chunk_rows_of(tu);
+    bp->cut[3] = chunk_rows_of(tu, hot_elem);
     bp->cut[4] = pad;
     bp->cut[5] = H;
-    const int rc = build_bplan_into(ctx, p, pad, H, bp);
+    bp->cut[6] = hot_elem;
+    const int rc = build_bplan_into(ctx, p, pad, H, hot_elem, bp);
     if (rc) {
         delete bp;
         return rc;
@@ -894,7 +916,19 @@ static void grant_lds(K *kernel, size_t shm)
 bool binned_supported(const gbx_pr *pr)
 {
     return !pr->batch && !pr->seg && !pr->plan->segmented && pr->g->N > 0 &&
-           (pr->kind == K_PR_REF || pr->kind == K_PR_STOCH);
+           (pr->kind == K_PR_REF || pr->kind == K_PR_STOCH || is_fixed_kind(pr->kind));
+}
+
+// The scatter moves values without looking at them: one instance per width.
+template <size_t W>
+using UintOf = std::conditional_t<W == 1, uint8_t,
+               std::conditional_t<W == 2, uint16_t, std::conditional_t<W == 4, uint32_t, uint64_t>>>;
+
+template <int KIND, class T>
+static void grant_kind(const gbx_binned_state *st)
+{
+    grant_lds(&bin_scatter_kernel<UintOf<sizeof(T)>>, st->shm_scatter);
+    grant_lds(&bin_accum_kernel<KIND, T>, st->shm_accum);
 }
 
 int binned_create(gbx_pr *pr)
@@ -902,20 +936,25 @@ int binned_create(gbx_pr *pr)
     gbx_ctx *ctx = pr->ctx;
     gbx_plan *p = pr->plan;
     const gbx_tuning &tu = ctx->tune;
-    // segments padded to 32 values: whole 128-byte lines of FP32 V, 64-byte
-    // halves for BF16 (measured as fast as whole lines), one layout for both
+    // segments padded to 32 values (a multiple of 32 is required by the
+    // per-block V positions): whole 128-byte lines of FP32 V, 64-byte halves
+    // for BF16 (measured as fast as whole lines)
     const uint32_t pad = 32;
-    // hot sources: a multiple of 8, at most N and 65528 (u16 source ids, 0xFFFF marks pads)
-    const uint32_t H = std::min(std::min(tu.binned_hot_sources, 65528u), p->N) & ~7u;
-    const uint32_t want[6] = {cut_edges_of(tu.binned_chunk_edges, ctx, p->E),
+    // hot sources: the tuning value for contributions of up to 4 bytes (one
+    // layout for FP32, BF16 and the 32/16/8-bit fixed-point kinds), half as
+    // many 8-byte ones in the same LDS; a multiple of 8, at most N and 65528
+    // (u16 source ids, 0xFFFF marks pads)
+    const uint32_t hot_elem = std::max<uint32_t>((uint32_t)pr->elem, 4u);
+    const uint32_t H = std::min(std::min(tu.binned_hot_sources * 4u / hot_elem, 65528u), p->N) & ~7u;
+    const uint32_t want[7] = {cut_edges_of(tu.binned_chunk_edges, ctx, p->E),
                               (std::max(cut_edges_of(tu.binned_bin_edges, ctx, p->E), 8u) + 7u) & ~7u,
-                              std::min(tu.binned_bin_rows, kMaxBinRows), chunk_rows_of(tu), pad, H};
+                              std::min(tu.binned_bin_rows, kMaxBinRows), chunk_rows_of(tu, hot_elem), pad, H, hot_elem};
     std::shared_ptr<gbx_bplan> bp;
     for (auto &c : p->bplans)
         if (memcmp(c->cut, want, sizeof(want)) == 0) bp = c;
     if (!bp) {
         gbx_bplan *raw = nullptr;
-        GBX_TRY(build_bplan(ctx, p, pad, H, &raw));
+        GBX_TRY(build_bplan(ctx, p, pad, H, hot_elem, &raw));
         bp = std::shared_ptr<gbx_bplan>(raw);
         p->bplans.push_back(bp);
     }
@@ -923,7 +962,7 @@ int binned_create(gbx_pr *pr)
     if (!st) return set_error(ctx, GBX_ERR_NOMEM, "binned state: host memory");
     pr->bin = st;
     st->bp = bp;
-    GBX_TRY(dev_alloc(ctx, st->V, (size_t)bp->Vlen * pr->elem + 16));
+    GBX_TRY(dev_alloc(ctx, st->V, (size_t)bp->Vlen * pr->elem + 64));
     GBX_TRY(dev_alloc(ctx, st->acc, (size_t)bp->n_multi * 8 + 8));
     GBX_TRY(dev_alloc(ctx, st->cnt, (size_t)bp->n_multi * 4 + 4));
     GBX_HIP(ctx, hipMemsetAsync(st->acc.p, 0, (size_t)bp->n_multi * 8 + 8, ctx->stream));
@@ -936,21 +975,21 @@ int binned_create(gbx_pr *pr)
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     st->shm_scatter = (size_t)bp->cut[3] * pr->elem;
     st->shm_accum = (size_t)bp->SB * 8 + (size_t)bp->H * pr->elem;
-    if (pr->elem == 4) {
-        grant_lds(&bin_scatter_kernel<float>, st->shm_scatter);
-        grant_lds(&bin_accum_kernel<K_PR_REF, float>, st->shm_accum);
-        grant_lds(&bin_accum_kernel<K_PR_STOCH, float>, st->shm_accum);
-    } else {
-        grant_lds(&bin_scatter_kernel<uint16_t>, st->shm_scatter);
-        grant_lds(&bin_accum_kernel<K_PR_REF, uint16_t>, st->shm_accum);
-        grant_lds(&bin_accum_kernel<K_PR_STOCH, uint16_t>, st->shm_accum);
+    switch (pr->kind) {
+    case K_PR_REF: pr->elem == 4 ? grant_kind<K_PR_REF, float>(st) : grant_kind<K_PR_REF, uint16_t>(st); break;
+    case K_PR_STOCH: pr->elem == 4 ? grant_kind<K_PR_STOCH, float>(st) : grant_kind<K_PR_STOCH, uint16_t>(st); break;
+    case K_FIX64: grant_kind<K_FIX64, uint64_t>(st); break;
+    case K_FIX32: grant_kind<K_FIX32, uint32_t>(st); break;
+    case K_FIX16: grant_kind<K_FIX16, uint16_t>(st); break;
+    default: grant_kind<K_FIX8, uint8_t>(st); break;
     }
     return GBX_OK;
 }
 
-template <class T>
+template <int KIND, class T>
 static void launch_binned(gbx_pr *pr, const IterView &A)
 {
+    using U = UintOf<sizeof(T)>;
     const gbx_binned_state *st = pr->bin;
     const gbx_bplan *bp = st->bp.get();
     hipStream_t stream = pr->ctx->stream;
@@ -960,10 +999,9 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
     S.src_local = bp->src_local.as<uint16_t>();
     S.bdst = bp->bdst.as<uint32_t>();
     S.NC = bp->NC;
-    T *V = st->V.as<T>();
     if (bp->NC)
-        bin_scatter_kernel<T><<<bp->NC, kBlock, st->shm_scatter, stream>>>(S, A.slots, A.t,
-                                                                         reinterpret_cast<const T *>(A.c_in), V);
+        bin_scatter_kernel<U><<<bp->NC, kBlock, st->shm_scatter, stream>>>(
+            S, A.slots, A.t, reinterpret_cast<const U *>(A.c_in), st->V.as<U>());
     AccumView B;
     B.units = bp->units.as<BinUnit>();
     B.dst_local = bp->dst_local.as<uint16_t>();
@@ -974,11 +1012,8 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
     B.cnt = st->cnt.as<uint32_t>();
     B.work = st->work.as<uint32_t>();
     B.NU = bp->NU;
-    const uint32_t *od = pr->plan->outdeg.as<uint32_t>();
-    if (pr->kind == K_PR_REF)
-        bin_accum_kernel<K_PR_REF, T><<<st->accum_grid, kBlock, st->shm_accum, stream>>>(B, od, pr->plan->N, V, A);
-    else
-        bin_accum_kernel<K_PR_STOCH, T><<<st->accum_grid, kBlock, st->shm_accum, stream>>>(B, od, pr->plan->N, V, A);
+    bin_accum_kernel<KIND, T><<<st->accum_grid, kBlock, st->shm_accum, stream>>>(
+        B, pr->plan->outdeg.as<uint32_t>(), pr->plan->N, st->V.as<T>(), A);
 }
 
 int binned_launch_iter(gbx_pr *pr)
@@ -996,8 +1031,20 @@ int binned_launch_iter(gbx_pr *pr)
     A.damp = pr->p.damp;
     A.eps = pr->p.epsilon;
     A.stop = pr->p.stop;
-    if (pr->elem == 4) launch_binned<float>(pr, A);
-    else launch_binned<uint16_t>(pr, A);
+    switch (pr->kind) {
+    case K_PR_REF:
+        if (pr->elem == 4) launch_binned<K_PR_REF, float>(pr, A);
+        else launch_binned<K_PR_REF, uint16_t>(pr, A);
+        break;
+    case K_PR_STOCH:
+        if (pr->elem == 4) launch_binned<K_PR_STOCH, float>(pr, A);
+        else launch_binned<K_PR_STOCH, uint16_t>(pr, A);
+        break;
+    case K_FIX64: launch_binned<K_FIX64, uint64_t>(pr, A); break;
+    case K_FIX32: launch_binned<K_FIX32, uint32_t>(pr, A); break;
+    case K_FIX16: launch_binned<K_FIX16, uint16_t>(pr, A); break;
+    default: launch_binned<K_FIX8, uint8_t>(pr, A); break;
+    }
     GBX_HIP(pr->ctx, hipGetLastError());
     pr->t++;
     return GBX_OK;

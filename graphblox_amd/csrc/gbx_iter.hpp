@@ -144,6 +144,76 @@ struct PrEpi {
     }
 };
 
+// ------------------------------------------------------------------ fixed point
+// The conversions of fixedPoint.h as pageRank.c:496-1030 applies them (REF
+// update).  c = r / outdeg in float; to fixed: 64-bit (u64)((double)c * 2^32
+// + 0.5); 32-bit (u32)(c * 65536.0f + 0.5f); 16-bit (u16)(c * 16384.0f +
+// 0.5f); 8-bit (u8)(c * 64.0f + 0.5f); float to u16/u8 through a 32-bit signed
+// truncation keeping the low bits (what x86-64 code of the reference does).
+// Back: 64-bit r' = (float)((double)base + (double)d * ((double)s / 2^32));
+// others r' = base + d * ((float)s / 2^k) in float (k = 16, 14, 6).
+// -ffp-contract=off keeps every multiply and add separately rounded.
+template <int KIND>
+__device__ __forceinline__ uint64_t fixed_conv(float x)
+{
+    if constexpr (KIND == K_FIX64) return (uint64_t)((double)x * 4294967296.0 + 0.5);
+    else if constexpr (KIND == K_FIX32) return (uint64_t)(uint32_t)(long long)(x * 65536.0f + 0.5f);
+    else if constexpr (KIND == K_FIX16) return (uint64_t)(uint16_t)(int)(x * 16384.0f + 0.5f);
+    else return (uint64_t)(uint8_t)(int)(x * 64.0f + 0.5f);
+}
+
+template <int KIND>
+__device__ __forceinline__ float fixed_update(float base, float damp, uint64_t s)
+{
+    if constexpr (KIND == K_FIX64) return (float)((double)base + (double)damp * ((double)s / 4294967296.0));
+    else if constexpr (KIND == K_FIX32) return base + damp * ((float)s / 65536.0f);
+    else if constexpr (KIND == K_FIX16) return base + damp * ((float)s / 16384.0f);
+    else return base + damp * ((float)s / 64.0f);
+}
+
+// Update of a fixed-point kind from the exact integer row sum.
+template <int KIND, class T>
+struct FixEpi {
+    float *__restrict__ r;
+    T *__restrict__ c_out;
+    const uint32_t *__restrict__ outdeg;
+    float base, damp;
+    double invN, eps;
+    Partial part;
+    struct In {
+        float r_old;
+        uint32_t od;
+    };
+    __device__ __forceinline__ In load(uint32_t i) const { return In{ld_stream(r + i), ld_stream(outdeg + i)}; }
+    __device__ __forceinline__ void apply(uint32_t i, uint64_t s, const In &x)
+    {
+        const float r_new = fixed_update<KIND>(base, damp, s);  // pageRank.c:577,711,843,976
+        st_stream(r + i, r_new);
+        c_out[i] = x.od ? (T)fixed_conv<KIND>(r_new / (float)x.od) : (T)0;  // pageRank.c:543-548
+        const float diff = fabsf(r_new - x.r_old);
+        part.err += (double)diff * invN;
+        part.active += ((double)diff >= eps) ? 1u : 0u;
+        if (!x.od) part.dang += (double)r_new;
+        part.sum += (double)r_new;
+    }
+    __device__ __forceinline__ void operator()(uint32_t i, uint64_t s) { apply(i, s, load(i)); }
+};
+
+template <int KIND, class T>
+__device__ __forceinline__ FixEpi<KIND, T> make_fix_epi(const uint32_t *outdeg, uint32_t N, const IterView &A)
+{
+    FixEpi<KIND, T> epi;
+    epi.r = A.r;
+    epi.c_out = reinterpret_cast<T *>(A.c_out);
+    epi.outdeg = outdeg;
+    epi.base = 1.0f - A.damp;  // pageRank.c:55
+    epi.damp = A.damp;
+    epi.invN = 1.0 / (double)N;
+    epi.eps = A.eps;
+    epi.part = {0.0, 0.0, 0.0, 0u, 0u};
+    return epi;
+}
+
 // Update coefficients of one iteration from the previous scalar slot.
 template <int KIND, class T>
 __device__ __forceinline__ PrEpi<KIND, T> make_epi(const uint32_t *outdeg, uint32_t N, const IterView &A,

@@ -25,7 +25,15 @@ enum Kind : int {
     K_SPMV = 3,        // y += 0.0001f * x  (SPMV.c:177)
     K_SPMV_W = 4,      // y += w * x
     K_ADJ = 5,         // adjoint of the weighted stochastic fixed point (forward-CSR rows)
+    // the reference's fixed-point pull PageRank (pageRank.c:496-1030): integer
+    // contributions of 64/32/16/8 bits, exact integer row sums
+    K_FIX64 = 6,
+    K_FIX32 = 7,
+    K_FIX16 = 8,
+    K_FIX8 = 9,
 };
+
+constexpr bool is_fixed_kind(int k) { return k >= K_FIX64 && k <= K_FIX8; }
 
 struct ClassDesc {
     uint32_t lanes;       // lanes per row (1..64); 256 = hub chunks

@@ -68,14 +68,18 @@ __global__ void __launch_bounds__(kBlock) pr_reset_kernel(PlanView P, IterView A
     const float uni = (float)(1.0 / (double)P.N);
     T *c_out = reinterpret_cast<T *>(A.c_out);
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < P.N; i += gridDim.x * kBlock) {
-        float r0 = KIND == K_PR_REF ? base : (A.tele ? A.tele[i] : uni);
+        float r0 = (KIND == K_PR_REF || is_fixed_kind(KIND)) ? base : (A.tele ? A.tele[i] : uni);
         A.r[i] = r0;
         const uint32_t od = P.outdeg[i];
-        float c0;
-        if (KIND == K_ADJ) c0 = r0;
-        else if (KIND == K_PR_STOCH_W) c0 = od ? r0 : 0.0f;
-        else c0 = od ? r0 / (float)od : 0.0f;
-        store_val(c_out, i, c0);
+        if constexpr (is_fixed_kind(KIND)) {
+            c_out[i] = od ? (T)fixed_conv<KIND>(r0 / (float)od) : (T)0;  // pageRank.c:543-548
+        } else {
+            float c0;
+            if (KIND == K_ADJ) c0 = r0;
+            else if (KIND == K_PR_STOCH_W) c0 = od ? r0 : 0.0f;
+            else c0 = od ? r0 / (float)od : 0.0f;
+            store_val(c_out, i, c0);
+        }
         if (KIND == K_ADJ) part.dang += (double)fabsf(r0);
         else if (!od) part.dang += (double)r0;
         part.sum += (double)r0;
@@ -814,12 +818,16 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
 {
     if (!ctx || !g || !p || !out) return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: NULL argument");
     *out = nullptr;
-    if (p->mode != GBX_PR_REF && p->mode != GBX_PR_STOCHASTIC)
+    if (p->mode < GBX_PR_REF || p->mode > GBX_PR_FIXED8)
         return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: mode %d", p->mode);
+    const bool fixed = p->mode >= GBX_PR_FIXED64;
+    if (fixed && (p->weighted || p->method == GBX_METHOD_PULL))
+        return set_error(ctx, GBX_ERR_ARG,
+                         "gbx_pr_create: the fixed-point modes are unweighted and run on the binned method");
     if (p->stop < 0 || p->stop > 2) return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: stop %d", p->stop);
     if (p->precision != GBX_F32 && p->precision != GBX_BF16)
         return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: precision %d not supported", p->precision);
-    if (p->weighted && p->mode == GBX_PR_REF)
+    if (p->weighted && p->mode != GBX_PR_STOCHASTIC)
         return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: weighted PageRank needs STOCHASTIC mode");
     if (!(p->damp >= 0.0f && p->damp < 1.0f))
         return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: damp %f not in [0,1)", (double)p->damp);
@@ -862,9 +870,11 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
     pr->plan_ref = g->plan[p->direction];
     pr->p = *p;
     pr->kind = p->mode == GBX_PR_REF ? K_PR_REF : (p->weighted ? K_PR_STOCH_W : K_PR_STOCH);
+    if (fixed) pr->kind = K_FIX64 + (p->mode - GBX_PR_FIXED64);
     if (kind_override >= 0) pr->kind = kind_override;
     pr->grid = iter_grid(ctx, plan);
     pr->elem = p->precision == GBX_BF16 ? 2 : 4;
+    if (fixed) pr->elem = (size_t)8 >> (p->mode - GBX_PR_FIXED64);  // 8, 4, 2, 1 bytes
     pr->hot = hot_count(ctx, g->N, pr->elem);
     const size_t nb = (size_t)g->N * 4;
     const size_t cb = (size_t)g->N * pr->elem + 16;
@@ -889,7 +899,7 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
                                                  "STOCHASTIC PageRank on one graph");
             binned = true;
         } else if (p->method == GBX_METHOD_AUTO) {
-            binned = kind_override < 0 && binned_supported(pr) && g->N >= ctx->tune.binned_min_vertices;
+            binned = kind_override < 0 && binned_supported(pr) && (fixed || g->N >= ctx->tune.binned_min_vertices);
         } else if (p->method != GBX_METHOD_PULL) {
             rc = set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: method %d", p->method);
         }
@@ -917,7 +927,7 @@ int gbx_pr_reset(gbx_pr *pr, const float *teleport)
         return pr->batch ? ppr_reset(pr) : seg_reset(pr);
     }
     if (teleport) {
-        if (pr->p.mode == GBX_PR_REF && pr->kind != K_ADJ)
+        if (pr->p.mode != GBX_PR_STOCHASTIC && pr->kind != K_ADJ)
             return set_error(ctx, GBX_ERR_ARG, "gbx_pr_reset: teleport needs STOCHASTIC mode");
         DevBuf tmp;
         GBX_TRY(dev_alloc(ctx, tmp, (size_t)N * 4));
@@ -937,7 +947,14 @@ int gbx_pr_reset(gbx_pr *pr, const float *teleport)
     A.c_out = pr->c[0].p;
     const unsigned grid = grid_for(N, kBlock, (unsigned)std::max(1, ctx->num_cus) * 8);
     hipStream_t st = ctx->stream;
-    if (pr->elem == 4) {
+    if (is_fixed_kind(pr->kind)) {
+        switch (pr->kind) {
+        case K_FIX64: pr_reset_kernel<K_FIX64, uint64_t><<<grid, kBlock, 0, st>>>(P, A); break;
+        case K_FIX32: pr_reset_kernel<K_FIX32, uint32_t><<<grid, kBlock, 0, st>>>(P, A); break;
+        case K_FIX16: pr_reset_kernel<K_FIX16, uint16_t><<<grid, kBlock, 0, st>>>(P, A); break;
+        default: pr_reset_kernel<K_FIX8, uint8_t><<<grid, kBlock, 0, st>>>(P, A); break;
+        }
+    } else if (pr->elem == 4) {
         switch (pr->kind) {
         case K_PR_REF: pr_reset_kernel<K_PR_REF, float><<<grid, kBlock, 0, st>>>(P, A); break;
         case K_PR_STOCH: pr_reset_kernel<K_PR_STOCH, float><<<grid, kBlock, 0, st>>>(P, A); break;
@@ -1061,7 +1078,7 @@ int gbx_pr_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep)
     const Slot *slot = pr->slots.as<Slot>() + (pr->t & 1);
     pr_finalize_kernel<<<grid_for(N, 256, 16384), 256, 0, ctx->stream>>>(
         pr->r.as<float>(), pr->plan->perm.as<uint32_t>(), N, slot,
-        pr->kind == K_ADJ ? 2 : (pr->kind == K_PR_REF ? 1 : 0), pr->out.as<float>());
+        pr->kind == K_ADJ ? 2 : ((pr->kind == K_PR_REF || is_fixed_kind(pr->kind)) ? 1 : 0), pr->out.as<float>());
     GBX_HIP(ctx, hipGetLastError());
     GBX_HIP(ctx, hipMemcpyAsync(pr->h_slot, slot, sizeof(Slot), hipMemcpyDeviceToHost, ctx->stream));
     if (ranks_out)

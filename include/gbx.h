@@ -141,6 +141,14 @@ int gbx_graph_free(gbx_graph *g);
                                 redistribution, output r/N */
 #define GBX_PR_STOCHASTIC 1  /* r0 = t, r' = (1-d) t + d (sum + D/N), D = dangling mass,
                                 normalised by the previous sum in FP64 */
+/* The reference's fixed-point pull PageRank, bit-exact (REF update, contributions
+ * r/outdeg converted to fixed point and summed as integers; binned method only):
+ * pageRankPullFixedPoint{64,32,16,8}BitGraphCSR (pageRank.c:496-1030), pushpull
+ * codes 2 (also 3, the push form), 9, 10, 11. */
+#define GBX_PR_FIXED64 2
+#define GBX_PR_FIXED32 3
+#define GBX_PR_FIXED16 4
+#define GBX_PR_FIXED8 5
 
 #define GBX_STOP_ACTIVE 0    /* stop when no vertex has |dr| >= epsilon (pageRank.c:327-339) */
 #define GBX_STOP_L1 1        /* stop when sum |dr| < epsilon (normalised scale) */
@@ -163,7 +171,7 @@ typedef struct gbx_pr_params {
     float damp;          /* damping factor, 0.85f in the reference (pageRank.h:13) */
     uint32_t max_iters;  /* arguments->iterations */
     double epsilon;      /* threshold for the stop rule */
-    int mode;            /* GBX_PR_REF / GBX_PR_STOCHASTIC */
+    int mode;            /* GBX_PR_REF / GBX_PR_STOCHASTIC / GBX_PR_FIXED* */
     int stop;            /* GBX_STOP_* */
     int precision;       /* GBX_F32 / GBX_BF16 */
     int direction;       /* GBX_PULL_IN / GBX_PULL_FWD */
