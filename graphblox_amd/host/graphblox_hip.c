@@ -382,8 +382,10 @@ void freePageRankStats(struct PageRankStats *stats)
     free(stats);
 }
 
-/* pageRankPullGraphCSR, pageRank.c:245-368 (DIRECTED=1 rows, FP32) */
-struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+/* One PageRank run on the device in a given mode; stats as the reference
+ * fills them (pageRank.c:339-365). */
+static struct PageRankStats *run_pagerank(struct Arguments *arguments, struct GraphCSR *graph, int mode,
+                                          const char *name)
 {
     gbx_graph *g = device_graph(graph);
     if (!g) return NULL;
@@ -393,14 +395,14 @@ struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct G
     p.damp = stats->damp;
     p.max_iters = arguments->iterations;
     p.epsilon = arguments->epsilon;
-    p.mode = GBX_PR_REF;
+    p.mode = mode;
     p.stop = GBX_STOP_ACTIVE;
     p.precision = GBX_F32;
     p.direction = GBX_PULL_IN;
     gbx_pr_report rep;
     int rc = gbx_pr_pull(g_ctx, g, &p, NULL, stats->pageRanks, &rep);
     if (rc != GBX_OK) {
-        fprintf(stderr, "graphblox_hip: pageRankPullGraphCSR failed: %s\n", gbx_last_error(g_ctx));
+        fprintf(stderr, "graphblox_hip: %s failed: %s\n", name, gbx_last_error(g_ctx));
         freePageRankStats(stats);
         return NULL;
     }
@@ -408,6 +410,8 @@ struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct G
     stats->activeVertices = rep.active;
     stats->error_total = rep.error_total;
     stats->time_total = rep.time_total_s;
+    printf(" -----------------------------------------------------\n");
+    printf("| %-51s | \n", name);
     printf(" -----------------------------------------------------\n");
     printf("| %-10s | %-8s | %-15s | %-9s | \n", "Iterations", "PR Sum", "Error", "Time (S)");
     printf(" -----------------------------------------------------\n");
@@ -417,17 +421,69 @@ struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct G
     return stats;
 }
 
-/* pageRankGraphCSR, pageRank.c:177-242: variant 0 (and unknown codes) is
- * the pull kernel; the other reference variants (push, fixed point,
- * quantised, data driven) are not provided by this backend -> NULL. */
+/* pageRankPullGraphCSR, pageRank.c:245-368 (DIRECTED=1 rows, FP32) */
+struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_pagerank(arguments, graph, GBX_PR_REF, "Page Rank Pull (HIP)");
+}
+
+/* pageRankPushGraphCSR, pageRank.c:370-492: each vertex pushes r/outdeg along
+ * its out-edges, which gives every vertex the sum over its in-edges that the
+ * pull iteration gathers; the device runs that iteration. */
+struct PageRankStats *pageRankPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_pagerank(arguments, graph, GBX_PR_REF, "Page Rank Push (HIP, pull sums)");
+}
+
+/* pageRankPullFixedPoint64BitGraphCSR, pageRank.c:496-630 (bit-exact) */
+struct PageRankStats *pageRankPullFixedPoint64BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_pagerank(arguments, graph, GBX_PR_FIXED64, "Page Rank Pull Fixed Point 64-bit (HIP)");
+}
+
+/* pageRankPushFixedPointGraphCSR, pageRank.c:1032-1157: the push form sums the
+ * same 64-bit integers as pageRankPullFixedPoint64BitGraphCSR (bit-exact). */
+struct PageRankStats *pageRankPushFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_pagerank(arguments, graph, GBX_PR_FIXED64, "Page Rank Push Fixed Point (HIP, pull sums)");
+}
+
+/* pageRankPullFixedPoint32BitGraphCSR, pageRank.c:632-765 (bit-exact) */
+struct PageRankStats *pageRankPullFixedPoint32BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_pagerank(arguments, graph, GBX_PR_FIXED32, "Page Rank Pull Fixed Point 32-bit (HIP)");
+}
+
+/* pageRankPullFixedPoint16BitGraphCSR, pageRank.c:767-897 (bit-exact) */
+struct PageRankStats *pageRankPullFixedPoint16BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_pagerank(arguments, graph, GBX_PR_FIXED16, "Page Rank Pull Fixed Point 16-bit (HIP)");
+}
+
+/* pageRankPullFixedPoint8BitGraphCSR, pageRank.c:899-1030 (bit-exact) */
+struct PageRankStats *pageRankPullFixedPoint8BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_pagerank(arguments, graph, GBX_PR_FIXED8, "Page Rank Pull Fixed Point 8-bit (HIP)");
+}
+
+/* pageRankGraphCSR, pageRank.c:177-242: the variant selected by
+ * arguments->pushpull; codes this backend does not provide (quantised 4, 5,
+ * 12, 13 and data driven 6, 7, 8) return NULL. */
 struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
-    if (arguments->pushpull >= 1 && arguments->pushpull <= 13) {
+    switch (arguments->pushpull) {
+    case 1: return pageRankPushGraphCSR(arguments, graph);
+    case 2: return pageRankPullFixedPoint64BitGraphCSR(arguments, graph);
+    case 3: return pageRankPushFixedPointGraphCSR(arguments, graph);
+    case 9: return pageRankPullFixedPoint32BitGraphCSR(arguments, graph);
+    case 10: return pageRankPullFixedPoint16BitGraphCSR(arguments, graph);
+    case 11: return pageRankPullFixedPoint8BitGraphCSR(arguments, graph);
+    case 4: case 5: case 6: case 7: case 8: case 12: case 13:
         fprintf(stderr, "graphblox_hip: PageRank variant %u is not provided by the HIP backend\n",
                 arguments->pushpull);
         return NULL;
+    default: return pageRankPullGraphCSR(arguments, graph);
     }
-    return pageRankPullGraphCSR(arguments, graph);
 }
 
 /* ------------------------------------------------------------ SpMV */

@@ -157,6 +157,14 @@ struct PageRankStats *newPageRankStatsGraphCSR(struct GraphCSR *graph);
 void freePageRankStats(struct PageRankStats *stats);
 struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:370-492 (push; same sums as the pull iteration) */
+struct PageRankStats *pageRankPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:496-1030 and 1032-1157: the fixed-point variants, bit-exact */
+struct PageRankStats *pageRankPullFixedPoint64BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct PageRankStats *pageRankPushFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct PageRankStats *pageRankPullFixedPoint32BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct PageRankStats *pageRankPullFixedPoint16BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct PageRankStats *pageRankPullFixedPoint8BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 
 /* SPMV.c:27-215 */
 struct SPMVStats *newSPMVStatsGraphCSR(struct GraphCSR *graph);

@@ -123,8 +123,24 @@ def test_preprocessing_and_pagerank_through_reference_api(golden, tmp_path, name
         y = _arr(sp.contents.vector_output, N, np.float32)
         assert l1(y, z["spmv_y3"]) <= 1e-6 * float(np.abs(z["spmv_y3"]).sum())
         L.freeSPMVStats(sp)
-        a.contents.pushpull = 1  # push variant: not provided by this backend
-        assert not L.pageRankGraphCSR(a, g)
+        # the reference's variants by pushpull code: push sums as pull; fixed point bit-exact
+        v = golden(f"variants_{name}")
+        a.contents.iterations = 20
+        a.contents.epsilon = 0.0
+        for code in (1, 2, 3, 9, 10, 11):
+            a.contents.pushpull = code
+            st = L.pageRankGraphCSR(a, g)
+            assert st, code
+            r = _arr(st.contents.pageRanks, N, np.float32)
+            if code == 1:
+                assert l1(r, v["pp1_r20"]) <= 1e-5
+            else:
+                np.testing.assert_array_equal(r, v[f"pp{code}_r20"])
+            L.freePageRankStats(st)
+        for code in (4, 5, 6, 7, 8, 12, 13):  # quantised and data-driven variants: not provided
+            a.contents.pushpull = code
+            assert not L.pageRankGraphCSR(a, g)
+        a.contents.pushpull = 0
     L.graphCSRFree(g)
     L.argumentsFree(a)
 
