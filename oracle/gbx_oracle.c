@@ -515,6 +515,107 @@ int orc_pr_pull_fixed(uint32_t N, const uint32_t *outdeg, const uint64_t *row_pt
     return 0;
 }
 
+/* ---- quantized pull PageRank (pageRank.c:1159-1577, quantization.h)
+ *
+ * Per iteration: c[u] = r[u] / outdeg[u] in float (0 for outdeg 0); the
+ * exact min and max of c over all vertices (getMinMax_k); the scale
+ * GetScale_k = (max - min in double) / RANGE_MAX_k (1 when min == max),
+ * kept in double for 32 bits and rounded to float for 16 and 8 bits; the
+ * zero point 0; q[u] = quantize_k(c[u]) = ROUND(c / scale) clamped to the
+ * width; an exact uint64 sum of q over each row's in-edges, and the update
+ * r' = base + d * next with next = (float)(scale * (double)sum) (32 bits)
+ * or scale * (float)sum in float (16 and 8 bits).
+ *
+ * ROUND(x) is (uint32_t)(x + 0.5): x86-64 code converts a double to a u32
+ * through a 64-bit signed truncation and keeps the low 32 bits, so a value
+ * of 2^32 or more (c / scale exceeds RANGE_MAX_k when min > 0) wraps, and
+ * one of 2^63 or more (or NaN) gives the indefinite integer, low bits 0;
+ * CLAMP_16/CLAMP_8 then saturate the wrapped u32 at the width's maximum. */
+static double quant_scale(float mn, float mx, int bits)
+{
+    if (mn == mx) return 1.0;
+    const double d = fabs((double)mx - (double)mn);
+    switch (bits) {
+    case 32: return d / 4294967295.0;
+    case 16: return (double)(float)(d / 65535.0);
+    default: return (double)(float)(d / 255.0);
+    }
+}
+
+static uint32_t quant_round(double x)
+{
+    const double y = x + 0.5;
+    if (!(y < 9223372036854775808.0)) return 0u;
+    return (uint32_t)(uint64_t)(int64_t)y;
+}
+
+static uint64_t quant_value(float c, double scale, int bits)
+{
+    const uint32_t q = quant_round((double)c / scale);
+    switch (bits) {
+    case 32: return q;
+    case 16: return q > 65535u ? 65535u : q;
+    default: return q > 255u ? 255u : q;
+    }
+}
+
+static float quant_update(float base, float damp, double scale, uint64_t s, int bits)
+{
+    float next;
+    if (bits == 32) next = (float)(scale * (double)s);
+    else next = (float)scale * (float)s;
+    return base + damp * next;
+}
+
+int orc_pr_pull_quant(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
+                      const uint32_t *col, int bits, uint32_t max_iters, double epsilon,
+                      float *ranks_out, uint32_t *iters_out, double *err_out)
+{
+    if ((bits != 32 && bits != 16 && bits != 8) || N == 0) return -1;
+    float *c = malloc((size_t)N * sizeof(float) + 8);
+    uint64_t *q = malloc((size_t)N * sizeof(uint64_t) + 8);
+    if (!c || !q) { free(c); free(q); return -1; }
+    const float damp = 0.85f, base = 1.0f - damp; /* pageRank.c:55 */
+    float *r = ranks_out;
+    for (uint32_t v = 0; v < N; v++) r[v] = base;
+    uint32_t it = 0;
+    double err = 0.0;
+    for (it = 0; it < max_iters; it++) {
+        float mn = INFINITY, mx = -INFINITY;
+        for (uint32_t v = 0; v < N; v++) {
+            c[v] = outdeg[v] ? r[v] / (float)outdeg[v] : 0.0f;
+            if (c[v] < mn) mn = c[v];
+            if (c[v] > mx) mx = c[v];
+        }
+        const double scale = quant_scale(mn, mx, bits);
+#pragma omp parallel for
+        for (uint32_t v = 0; v < N; v++) q[v] = quant_value(c[v], scale, bits);
+        uint32_t active = 0;
+        err = 0.0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : err, active)
+        for (uint32_t v = 0; v < N; v++) {
+            uint64_t s = 0;
+            for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) s += q[col[j] & VERTEX_ID_MASK];
+            const float prev = r[v];
+            const float next = quant_update(base, damp, scale, s, bits);
+            r[v] = next;
+            const double e = fabs((double)next - (double)prev);
+            err += e / N;
+            if (e >= epsilon) active++;
+        }
+        if (active == 0) {
+            it++;
+            break;
+        }
+    }
+    for (uint32_t v = 0; v < N; v++) r[v] /= N;
+    if (iters_out) *iters_out = it < max_iters ? it - 1 : it;
+    if (err_out) *err_out = err;
+    free(c);
+    free(q);
+    return 0;
+}
+
 /* ---- R-MAT generator (counter-based; identical integer arithmetic on GPU) */
 
 static uint64_t mix64(uint64_t z)

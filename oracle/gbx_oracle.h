@@ -119,6 +119,14 @@ int orc_pr_pull_fixed(uint32_t N, const uint32_t *outdeg, const uint64_t *row_pt
                       const uint32_t *col, int bits, uint32_t max_iters, double epsilon,
                       float *ranks_out, uint32_t *iters_out, double *err_out);
 
+/* The reference's quantized pull PageRank (pageRankPullQuant{32,16,8}Bit
+ * GraphCSR, pageRank.c:1159-1577; pushpull codes 4, 12, 13): per-iteration
+ * min/max scale, zero point 0, exact uint64 sums of the quantized
+ * contributions; same update, stop rule and outputs as orc_pr_pull_fixed. */
+int orc_pr_pull_quant(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
+                      const uint32_t *col, int bits, uint32_t max_iters, double epsilon,
+                      float *ranks_out, uint32_t *iters_out, double *err_out);
+
 /* Seeded counter-based R-MAT generator (same integer arithmetic as the GPU
  * generator).  Writes num_edges = edge_factor << scale edges, ids permuted
  * by the seeded bijection on [0, 2^scale). */

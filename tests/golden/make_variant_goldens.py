@@ -10,7 +10,11 @@ Codes: 1 push, 2 pull fixed point 64-bit, 3 push fixed point, 4 pull
 quantised 32-bit, 5 push quantised, 9/10/11 pull fixed point 32/16/8-bit,
 12/13 pull quantised 16/8-bit.  One OpenMP thread, so the float-atomic push
 variants are deterministic.  The graphs are those of make_goldens.py (the
-shipped fixtures and the seeded R-MAT scale 8-12 graphs).
+shipped fixtures and the seeded R-MAT scale 8-12 graphs), plus two graphs
+without dangling vertices, whose edges are stored with them: `ring_rmat_s10`
+(the R-MAT scale 10 graph and the ring v -> v+1 mod N: the quantised
+variants see min > 0) and `ring_64` (a 64-vertex ring: every contribution
+equal, min == max).
 
     make -C oracle ref && python tests/golden/make_variant_goldens.py
 """
@@ -34,7 +38,7 @@ RMAT_SEED = 20261203
 CODES = (1, 2, 3, 4, 5, 9, 10, 11, 12, 13)
 
 
-def variants_for(path: str, name: str):
+def variants_for(path: str, name: str, edges=None):
     L = O.ref(1, 0)
     g = L.refd_build(path.encode(), 1, 0, 0, 0, 1)
     N = L.refd_num_vertices(g)
@@ -49,23 +53,40 @@ def variants_for(path: str, name: str):
             else:
                 out[f"pp{c}_eps_ranks"], out[f"pp{c}_eps_iters"], out[f"pp{c}_eps_err"] = r, it.value, err.value
     L.refd_free(g)
+    if edges is not None:
+        out["edges_src"], out["edges_dst"] = edges
     dest = os.path.join(HERE, f"variants_{name}.npz")
     np.savez_compressed(dest, **{k: np.asarray(v) for k, v in out.items()})
     print("wrote", dest, os.path.getsize(dest), "bytes")
 
 
-def main():
+def main(only=None):
+    """Writes every file, or only the graphs named on the command line."""
     if not O.ref_available(1, 0):
         raise SystemExit("reference build missing: run `make -C oracle ref` first")
+    want = (lambda name: not only or name in only)
     for fx in FIXTURES:
-        variants_for(os.path.join(REF_TEST, fx, "graph.bin"), f"fixture_{fx}")
+        if want(f"fixture_{fx}"):
+            variants_for(os.path.join(REF_TEST, fx, "graph.bin"), f"fixture_{fx}")
     with tempfile.TemporaryDirectory() as tmp:
         for scale in (8, 10, 12):
             s, d, N = O.rmat_graph(scale, 16, RMAT_SEED + scale)
             p = os.path.join(tmp, f"rmat{scale}.bin")
             O.write_bin(p, s, d)
-            variants_for(p, f"rmat_s{scale}")
+            if want(f"rmat_s{scale}"):
+                variants_for(p, f"rmat_s{scale}")
+        s, d, N = O.rmat_graph(10, 16, RMAT_SEED + 10)
+        v = np.arange(N, dtype=np.uint32)
+        rs, rd, _ = O.dedupe_keep_first(np.concatenate([s, v]), np.concatenate([d, (v + 1) % N]))
+        rings = {"ring_rmat_s10": (rs, rd),
+                 "ring_64": (np.arange(64, dtype=np.uint32), (np.arange(64, dtype=np.uint32) + 1) % 64)}
+        for name, (rs, rd) in rings.items():
+            if not want(name):
+                continue
+            p = os.path.join(tmp, f"{name}.bin")
+            O.write_bin(p, rs, rd)
+            variants_for(p, name, (rs.astype(np.uint32), rd.astype(np.uint32)))
 
 
 if __name__ == "__main__":
-    main()
+    main(set(sys.argv[1:]))

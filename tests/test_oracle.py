@@ -6,7 +6,7 @@ import re
 import numpy as np
 import pytest
 
-from conftest import FIXTURES, RMAT_SMALL, ROOT, l1
+from conftest import FIXTURES, GOLDEN, RMAT_SMALL, ROOT, l1
 from oracle import oracle as O
 
 GRAPHS = [f"fixture_{f}" for f in FIXTURES] + [f"rmat_s{s}" for s in RMAT_SMALL]
@@ -299,6 +299,76 @@ def test_reference_push_variants_equal_pull_sums(golden, name):
     assert O_l1(v["pp1_r20"], z["pr_r20"]) <= 1e-6
     np.testing.assert_array_equal(v["pp3_r20"], v["pp2_r20"])
     np.testing.assert_array_equal(v["pp3_eps_ranks"], v["pp2_eps_ranks"])
+
+
+RING_GRAPHS = ["ring_rmat_s10", "ring_64"]  # no dangling vertex: quantisation min > 0, and min == max
+
+
+def _variant_csr(golden, name):
+    """In-CSR of a golden graph; the ring graphs carry their own edges."""
+    v = golden(f"variants_{name}")
+    if name in RING_GRAPHS:
+        s, d = v["edges_src"], v["edges_dst"]
+        N = int(max(s.max(), d.max())) + 1
+    else:
+        z = golden(name)
+        s, d, N = z["edges_src"], z["edges_dst"], int(z["m1_N"])
+    fwd, inv = O.build_graph(s, d, None, N, mode=1)
+    return v, fwd, inv, O.row_ptr_from(inv)
+
+
+@pytest.mark.parametrize("name", VARIANT_GRAPHS + RING_GRAPHS)
+def test_quantized_oracle_matches_reference_variants(golden, name):
+    """pageRankPullQuant{32,16,8}BitGraphCSR (pushpull 4, 12, 13): bit-exact
+    ranks, error and stopping iterations, including the graphs without
+    dangling vertices where the 32-bit quantisation wraps past 2^32 and the
+    16/8-bit one saturates (ring_rmat_s10), and where every contribution is
+    equal (ring_64: scale 1)."""
+    v, fwd, inv, rp = _variant_csr(golden, name)
+    for bits, code in ((32, 4), (16, 12), (8, 13)):
+        r, _, err = O.pr_pull_quant(fwd.N, fwd.out_degree, rp, inv.dst, bits, 20, 0.0)
+        np.testing.assert_array_equal(r, v[f"pp{code}_r20"])
+        assert err == pytest.approx(float(v[f"pp{code}_err20"]), rel=1e-9, abs=1e-300)
+        r, it, _ = O.pr_pull_quant(fwd.N, fwd.out_degree, rp, inv.dst, bits, 100, 1e-4)
+        assert it == int(v[f"pp{code}_eps_iters"])
+        np.testing.assert_array_equal(r, v[f"pp{code}_eps_ranks"])
+
+
+@pytest.mark.parametrize("name", RING_GRAPHS)
+def test_fixed_point_oracle_on_graphs_without_dangling_vertices(golden, name):
+    v, fwd, inv, rp = _variant_csr(golden, name)
+    for bits, code in ((64, 2), (32, 9), (16, 10), (8, 11)):
+        r, _, _ = O.pr_pull_fixed(fwd.N, fwd.out_degree, rp, inv.dst, bits, 20, 0.0)
+        np.testing.assert_array_equal(r, v[f"pp{code}_r20"])
+
+
+def test_ring_golden_reaches_the_wrap_and_saturation():
+    """The first iteration of ring_rmat_s10 quantises values past the
+    width's range: c / scale + 0.5 >= 2^32 for the 32-bit variant (the u32
+    wrap) and above 65535 / 255 for 16 / 8 bits (CLAMP), so the bit-exact
+    match above pins both rules."""
+    with np.load(os.path.join(GOLDEN, "variants_ring_rmat_s10.npz"), allow_pickle=False) as z:
+        s, d = z["edges_src"], z["edges_dst"]
+    N = int(max(s.max(), d.max())) + 1
+    fwd, _ = O.build_graph(s, d, None, N, mode=1)
+    assert fwd.out_degree.min() > 0
+    c = (np.float32(1) - np.float32(0.85)) / fwd.out_degree.astype(np.float32)
+    spread = np.float64(c.max()) - np.float64(c.min())
+    for rng, as_float in ((4294967295.0, False), (65535.0, True), (255.0, True)):
+        scale = spread / rng
+        if as_float:
+            scale = np.float64(np.float32(scale))
+        x = c.astype(np.float64) / scale + 0.5
+        assert (x >= rng + 1).sum() > 100
+    assert (c.astype(np.float64) / (spread / 4294967295.0) + 0.5 >= 2.0**32).sum() > 100
+
+
+def test_quantized_goldens_tell_widths_apart(golden):
+    v, fwd, inv, rp = _variant_csr(golden, "rmat_s10")
+    r16, _, _ = O.pr_pull_quant(fwd.N, fwd.out_degree, rp, inv.dst, 16, 20, 0.0)
+    assert not np.array_equal(r16, v["pp13_r20"]) and not np.array_equal(r16, v["pp4_r20"])
+    r8, _, _ = O.pr_pull_quant(fwd.N, fwd.out_degree, rp, inv.dst, 8, 19, 0.0)
+    assert not np.array_equal(r8, v["pp13_r20"])
 
 
 def O_l1(a, b):
