@@ -299,19 +299,39 @@ __device__ __forceinline__ void copy_to_lds(T *dst, const T *src, uint32_t n)
     for (uint32_t i = nv * 8 + threadIdx.x; i < n; i += kBlock) dst[i] = src[i];
 }
 
+// n FP32 contributions quantized into LDS (the quantized kinds): whole
+// groups of 8 at 8-value aligned addresses, then a tail.
+template <int KIND, class U>
+__device__ __forceinline__ void quant_to_lds(U *dst, const float *src, uint32_t n, double scale)
+{
+    const uint32_t nv = n / 8;
+    for (uint32_t i = threadIdx.x; i < nv; i += kBlock) {
+        const Vec8<float> w = reinterpret_cast<const Vec8<float> *>(src)[i];
+        Vec8<U> q;
+#pragma unroll
+        for (int k = 0; k < 8; k++) q.v[k] = (U)quantize<KIND>(w.v[k], scale);
+        reinterpret_cast<Vec8<U> *>(dst)[i] = q;
+    }
+    for (uint32_t i = nv * 8 + threadIdx.x; i < n; i += kBlock) dst[i] = (U)quantize<KIND>(src[i], scale);
+}
+
 // One block per source chunk.  Each lane takes groups of 8 entries: one
 // 16-byte load of source indices, the V position of its block of 32, eight LDS reads, one
 // 32-byte (FP32) or 16-byte (BF16) store; pads (0xFFFF) store zeros, so every
-// segment is written in whole 128-byte lines.
-template <class T>
+// segment is written in whole 128-byte lines.  QK: a quantized kind, whose
+// FP32 contributions are quantized on the way into LDS (0: values as stored).
+template <class T, int QK>
 __global__ void __launch_bounds__(kBlock) bin_scatter_kernel(ScatterView S, const Slot *slots, uint32_t t,
-                                                             const T *__restrict__ c_in, T *__restrict__ V)
+                                                             const void *__restrict__ c_in, T *__restrict__ V)
 {
     if (slots[t & 1].done) return;
     const uint32_t k = blockIdx.x;
     T *cl = reinterpret_cast<T *>(gbx_dyn_lds);
     const uint32_t lo = S.chunk_lo[k], n = S.chunk_lo[k + 1] - lo;
-    copy_to_lds(cl, c_in + lo, n);  // chunk starts are multiples of 8 sources
+    // chunk starts are multiples of 8 sources
+    if constexpr (QK != 0) quant_to_lds<QK, T>(cl, reinterpret_cast<const float *>(c_in) + lo, n,
+                                               quant_scale<QK>(slots[t & 1]));
+    else copy_to_lds(cl, reinterpret_cast<const T *>(c_in) + lo, n);
     __syncthreads();
     const uint32_t g0 = S.chunk_grp[k], g1 = S.chunk_grp[k + 1];
     for (uint32_t base = g0 + threadIdx.x; base < g1; base += kBlock * kScatterUnroll) {
@@ -355,14 +375,15 @@ struct AccumView {
 };
 
 // Values as loaded (raw) and as 64-bit fixed point.  Float kinds keep FP32
-// (BF16 widened) and scale by 2^scale; fixed-point kinds are integers already.
+// (BF16 widened) and scale by 2^scale; fixed-point and quantized kinds are
+// integers already.
 template <int KIND>
-using RawVal = std::conditional_t<is_fixed_kind(KIND), uint64_t, float>;
+using RawVal = std::conditional_t<is_int_kind(KIND), uint64_t, float>;
 
 template <int KIND, class T>
 __device__ __forceinline__ RawVal<KIND> raw_of(T x)
 {
-    if constexpr (is_fixed_kind(KIND)) return (uint64_t)x;
+    if constexpr (is_int_kind(KIND)) return (uint64_t)x;
     else if constexpr (sizeof(T) == 2) return __uint_as_float((uint32_t)x << 16);  // bf16
     else return x;
 }
@@ -375,7 +396,7 @@ __device__ __forceinline__ long long to_fixed(float v, double scale)
 template <int KIND>
 __device__ __forceinline__ long long fixed_of(RawVal<KIND> v, double scale)
 {
-    if constexpr (is_fixed_kind(KIND)) return (long long)v;
+    if constexpr (is_int_kind(KIND)) return (long long)v;
     else return to_fixed(v, scale);
 }
 
@@ -404,7 +425,8 @@ __device__ __forceinline__ void load_hot8(const uint32_t *pairs, uint32_t e, con
 }
 
 template <int KIND, class T>
-using EpiOf = std::conditional_t<is_fixed_kind(KIND), FixEpi<KIND, T>, PrEpi<KIND, T>>;
+using EpiOf = std::conditional_t<is_fixed_kind(KIND), FixEpi<KIND, T>,
+                                 std::conditional_t<is_quant_kind(KIND), QuantEpi<KIND>, PrEpi<KIND, T>>>;
 
 constexpr int kAccumUnroll = 4;  // groups of 8 per lane in flight
 constexpr int kEpiBatch = 8;     // rows per thread whose update inputs load together
@@ -466,7 +488,7 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
                 }
             }
             if (apply) {
-                if constexpr (is_fixed_kind(KIND)) epi(u.row_begin, (uint64_t)tot);
+                if constexpr (is_int_kind(KIND)) epi(u.row_begin, (uint64_t)tot);
                 else epi(u.row_begin, (float)((double)tot * inv));
             }
         }
@@ -538,7 +560,7 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
             for (int q = 0; q < kEpiBatch; q++) {
                 const uint32_t i = i0 + q * kBlock;
                 if (i < nrows) {
-                    if constexpr (is_fixed_kind(KIND)) epi.apply(u.row_begin + i, (uint64_t)y[i], in[q]);
+                    if constexpr (is_int_kind(KIND)) epi.apply(u.row_begin + i, (uint64_t)y[i], in[q]);
                     else epi.apply(u.row_begin + i, (float)((double)y[i] * inv), in[q]);
                 }
             }
@@ -561,16 +583,22 @@ __global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const ui
     }
     EpiOf<KIND, T> epi;
     if constexpr (is_fixed_kind(KIND)) epi = make_fix_epi<KIND, T>(outdeg, N, A);
+    else if constexpr (is_quant_kind(KIND)) epi = make_quant_epi<KIND>(outdeg, N, A, cur);
     else epi = make_epi<KIND, T>(outdeg, N, A, cur);
     // float kinds: every row sum is at most the sum of all contributions,
     // itself at most the previous rank sum (bf16 rounding adds < 0.4 %);
-    // fixed-point kinds sum their integers as they are (uint64 wrap-around,
-    // as the reference's uint64 sums)
+    // fixed-point and quantized kinds sum their integers as they are (uint64
+    // wrap-around, as the reference's uint64 sums)
     const double bound = cur->sum * 1.01 + 1e-30;
     int ex = 0;
     (void)frexp(bound, &ex);
     const double scale = ldexp(1.0, 61 - ex), inv = ldexp(1.0, ex - 61);
-    if (B.H) copy_to_lds(reinterpret_cast<T *>(gbx_dyn_lds + B.hot_off), reinterpret_cast<const T *>(A.c_in), B.H);
+    if (B.H) {
+        T *hot = reinterpret_cast<T *>(gbx_dyn_lds + B.hot_off);
+        if constexpr (is_quant_kind(KIND))
+            quant_to_lds<KIND, T>(hot, reinterpret_cast<const float *>(A.c_in), B.H, epi.scale);
+        else copy_to_lds(hot, reinterpret_cast<const T *>(A.c_in), B.H);
+    }
     __shared__ uint32_t s_unit;
     for (;;) {
         __syncthreads();  // previous unit done with LDS (and the hot load finished)
@@ -916,7 +944,7 @@ static void grant_lds(K *kernel, size_t shm)
 bool binned_supported(const gbx_pr *pr)
 {
     return !pr->batch && !pr->seg && !pr->plan->segmented && pr->g->N > 0 &&
-           (pr->kind == K_PR_REF || pr->kind == K_PR_STOCH || is_fixed_kind(pr->kind));
+           (pr->kind == K_PR_REF || pr->kind == K_PR_STOCH || is_int_kind(pr->kind));
 }
 
 // The scatter moves values without looking at them: one instance per width.
@@ -924,11 +952,27 @@ template <size_t W>
 using UintOf = std::conditional_t<W == 1, uint8_t,
                std::conditional_t<W == 2, uint16_t, std::conditional_t<W == 4, uint32_t, uint64_t>>>;
 
+// The scatter's instance: values moved as stored, or quantized on loading.
+template <int KIND>
+constexpr int scatter_qk() { return is_quant_kind(KIND) ? KIND : 0; }
+
 template <int KIND, class T>
 static void grant_kind(const gbx_binned_state *st)
 {
-    grant_lds(&bin_scatter_kernel<UintOf<sizeof(T)>>, st->shm_scatter);
+    grant_lds(&bin_scatter_kernel<UintOf<sizeof(T)>, scatter_qk<KIND>()>, st->shm_scatter);
     grant_lds(&bin_accum_kernel<KIND, T>, st->shm_accum);
+}
+
+// Bytes of one value in V and in the LDS copies: the contribution's own
+// width, or the quantized width.
+static size_t value_bytes(const gbx_pr *pr)
+{
+    switch (pr->kind) {
+    case K_Q32: return 4;
+    case K_Q16: return 2;
+    case K_Q8: return 1;
+    default: return pr->elem;
+    }
 }
 
 int binned_create(gbx_pr *pr)
@@ -944,7 +988,8 @@ int binned_create(gbx_pr *pr)
     // layout for FP32, BF16 and the 32/16/8-bit fixed-point kinds), half as
     // many 8-byte ones in the same LDS; a multiple of 8, at most N and 65528
     // (u16 source ids, 0xFFFF marks pads)
-    const uint32_t hot_elem = std::max<uint32_t>((uint32_t)pr->elem, 4u);
+    const size_t vel = value_bytes(pr);
+    const uint32_t hot_elem = std::max<uint32_t>((uint32_t)vel, 4u);
     const uint32_t H = std::min(std::min(tu.binned_hot_sources * 4u / hot_elem, 65528u), p->N) & ~7u;
     const uint32_t want[7] = {cut_edges_of(tu.binned_chunk_edges, ctx, p->E),
                               (std::max(cut_edges_of(tu.binned_bin_edges, ctx, p->E), 8u) + 7u) & ~7u,
@@ -962,7 +1007,7 @@ int binned_create(gbx_pr *pr)
     if (!st) return set_error(ctx, GBX_ERR_NOMEM, "binned state: host memory");
     pr->bin = st;
     st->bp = bp;
-    GBX_TRY(dev_alloc(ctx, st->V, (size_t)bp->Vlen * pr->elem + 64));
+    GBX_TRY(dev_alloc(ctx, st->V, (size_t)bp->Vlen * vel + 64));
     GBX_TRY(dev_alloc(ctx, st->acc, (size_t)bp->n_multi * 8 + 8));
     GBX_TRY(dev_alloc(ctx, st->cnt, (size_t)bp->n_multi * 4 + 4));
     GBX_HIP(ctx, hipMemsetAsync(st->acc.p, 0, (size_t)bp->n_multi * 8 + 8, ctx->stream));
@@ -973,15 +1018,18 @@ int binned_create(gbx_pr *pr)
     st->accum_grid = std::max(1u, std::min(bp->NU, (uint32_t)std::max(1, ctx->num_cus)));
     if (bp->NU > 16384) GBX_TRY(dev_alloc(ctx, pr->partials, (size_t)bp->NU * sizeof(Partial)));
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    st->shm_scatter = (size_t)bp->cut[3] * pr->elem;
-    st->shm_accum = (size_t)bp->SB * 8 + (size_t)bp->H * pr->elem;
+    st->shm_scatter = (size_t)bp->cut[3] * vel;
+    st->shm_accum = (size_t)bp->SB * 8 + (size_t)bp->H * vel;
     switch (pr->kind) {
     case K_PR_REF: pr->elem == 4 ? grant_kind<K_PR_REF, float>(st) : grant_kind<K_PR_REF, uint16_t>(st); break;
     case K_PR_STOCH: pr->elem == 4 ? grant_kind<K_PR_STOCH, float>(st) : grant_kind<K_PR_STOCH, uint16_t>(st); break;
     case K_FIX64: grant_kind<K_FIX64, uint64_t>(st); break;
     case K_FIX32: grant_kind<K_FIX32, uint32_t>(st); break;
     case K_FIX16: grant_kind<K_FIX16, uint16_t>(st); break;
-    default: grant_kind<K_FIX8, uint8_t>(st); break;
+    case K_FIX8: grant_kind<K_FIX8, uint8_t>(st); break;
+    case K_Q32: grant_kind<K_Q32, uint32_t>(st); break;
+    case K_Q16: grant_kind<K_Q16, uint16_t>(st); break;
+    default: grant_kind<K_Q8, uint8_t>(st); break;
     }
     return GBX_OK;
 }
@@ -1000,8 +1048,8 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
     S.bdst = bp->bdst.as<uint32_t>();
     S.NC = bp->NC;
     if (bp->NC)
-        bin_scatter_kernel<U><<<bp->NC, kBlock, st->shm_scatter, stream>>>(
-            S, A.slots, A.t, reinterpret_cast<const U *>(A.c_in), st->V.as<U>());
+        bin_scatter_kernel<U, scatter_qk<KIND>()><<<bp->NC, kBlock, st->shm_scatter, stream>>>(
+            S, A.slots, A.t, A.c_in, st->V.as<U>());
     AccumView B;
     B.units = bp->units.as<BinUnit>();
     B.dst_local = bp->dst_local.as<uint16_t>();
@@ -1043,7 +1091,10 @@ int binned_launch_iter(gbx_pr *pr)
     case K_FIX64: launch_binned<K_FIX64, uint64_t>(pr, A); break;
     case K_FIX32: launch_binned<K_FIX32, uint32_t>(pr, A); break;
     case K_FIX16: launch_binned<K_FIX16, uint16_t>(pr, A); break;
-    default: launch_binned<K_FIX8, uint8_t>(pr, A); break;
+    case K_FIX8: launch_binned<K_FIX8, uint8_t>(pr, A); break;
+    case K_Q32: launch_binned<K_Q32, uint32_t>(pr, A); break;
+    case K_Q16: launch_binned<K_Q16, uint16_t>(pr, A); break;
+    default: launch_binned<K_Q8, uint8_t>(pr, A); break;
     }
     GBX_HIP(pr->ctx, hipGetLastError());
     pr->t++;

@@ -32,7 +32,19 @@ __device__ __forceinline__ void wave_reduce(Partial &p)
         p.dang += __shfl_xor(p.dang, off, 64);
         p.sum += __shfl_xor(p.sum, off, 64);
         p.active += __shfl_xor(p.active, off, 64);
+        p.cmax = max(p.cmax, (uint32_t)__shfl_xor(p.cmax, off, 64));
+        p.cnmin = max(p.cnmin, (uint32_t)__shfl_xor(p.cnmin, off, 64));
     }
+}
+
+__device__ __forceinline__ void add_partial(Partial &q, const Partial &b)
+{
+    q.err += b.err;
+    q.dang += b.dang;
+    q.sum += b.sum;
+    q.active += b.active;
+    q.cmax = max(q.cmax, b.cmax);
+    q.cnmin = max(q.cnmin, b.cnmin);
 }
 
 // Block partial -> partials[blockIdx]; the last block reduces all partials
@@ -48,12 +60,7 @@ __device__ __forceinline__ void grid_reduce(Partial p, Partial *partials, uint32
     __syncthreads();
     if (threadIdx.x == 0) {
         Partial q = sp[0];
-        for (int w = 1; w < kBlock / 64; w++) {
-            q.err += sp[w].err;
-            q.dang += sp[w].dang;
-            q.sum += sp[w].sum;
-            q.active += sp[w].active;
-        }
+        for (int w = 1; w < kBlock / 64; w++) add_partial(q, sp[w]);
         partials[blockIdx.x] = q;
         __threadfence();
         uint32_t ticket = atomicAdd(counter, 1u);
@@ -69,6 +76,8 @@ __device__ __forceinline__ void grid_reduce(Partial p, Partial *partials, uint32
         q.dang += vp[b].dang;
         q.sum += vp[b].sum;
         q.active += vp[b].active;
+        q.cmax = max(q.cmax, (uint32_t)vp[b].cmax);
+        q.cnmin = max(q.cnmin, (uint32_t)vp[b].cnmin);
     }
     wave_reduce(q);
     __syncthreads();
@@ -76,12 +85,7 @@ __device__ __forceinline__ void grid_reduce(Partial p, Partial *partials, uint32
     __syncthreads();
     if (threadIdx.x == 0) {
         Partial tot = sp[0];
-        for (int w = 1; w < kBlock / 64; w++) {
-            tot.err += sp[w].err;
-            tot.dang += sp[w].dang;
-            tot.sum += sp[w].sum;
-            tot.active += sp[w].active;
-        }
+        for (int w = 1; w < kBlock / 64; w++) add_partial(tot, sp[w]);
         *counter = 0;
         fin(tot);
     }
@@ -214,6 +218,100 @@ __device__ __forceinline__ FixEpi<KIND, T> make_fix_epi(const uint32_t *outdeg, 
     return epi;
 }
 
+// ------------------------------------------------------------------ quantized
+// The quantization of pageRank.c:1216-1231 with quantization.h's GetScale_k
+// and quantize_k: scale = (max - min in double) / RANGE_MAX_k (1 when
+// min == max), kept in double for 32 bits and rounded to float for 16 and 8
+// bits; zero point 0; q = (u32)(c / scale + 0.5) in double, where the u32
+// conversion is x86-64's (a 64-bit signed truncation keeping the low 32
+// bits: 2^32 and above wrap, 2^63 and above give 0), then CLAMP to the
+// width.  Back (pageRank.c:1245): next = (float)(scale * (double)sum) for 32
+// bits, scale * (float)sum in float for 16 and 8 bits; r' = base + d * next.
+// Contributions are >= 0, so min and max come from the slot's float bits.
+template <int KIND>
+__device__ __forceinline__ double quant_scale(const Slot &s)
+{
+    const float mn = __uint_as_float(~s.cnmin), mx = __uint_as_float(s.cmax);
+    if (mn == mx) return 1.0;
+    const double d = (double)mx - (double)mn;
+    if constexpr (KIND == K_Q32) return d / 4294967295.0;
+    else if constexpr (KIND == K_Q16) return (double)(float)(d / 65535.0);
+    else return (double)(float)(d / 255.0);
+}
+
+template <int KIND>
+__device__ __forceinline__ uint32_t quantize(float c, double scale)
+{
+    const double y = (double)c / scale + 0.5;
+    const uint32_t q = y < 9223372036854775808.0 ? (uint32_t)(unsigned long long)(long long)y : 0u;
+    if constexpr (KIND == K_Q32) return q;
+    else if constexpr (KIND == K_Q16) return q > 65535u ? 65535u : q;
+    else return q > 255u ? 255u : q;
+}
+
+template <int KIND>
+__device__ __forceinline__ float quant_update(float base, float damp, double scale, uint64_t s)
+{
+    float next;
+    if constexpr (KIND == K_Q32) next = (float)(scale * (double)s);
+    else next = (float)scale * (float)s;
+    return base + damp * next;
+}
+
+__device__ __forceinline__ void note_contribution(Partial &p, float c)
+{
+    p.cmax = max(p.cmax, __float_as_uint(c));
+    p.cnmin = max(p.cnmin, ~__float_as_uint(c));
+}
+
+// Update of a quantized kind from the exact integer row sum; the next
+// contributions are FP32 (quantized when the next iteration loads them).
+template <int KIND>
+struct QuantEpi {
+    float *__restrict__ r;
+    float *__restrict__ c_out;
+    const uint32_t *__restrict__ outdeg;
+    float base, damp;
+    double scale, invN, eps;
+    Partial part;
+    struct In {
+        float r_old;
+        uint32_t od;
+    };
+    __device__ __forceinline__ In load(uint32_t i) const { return In{ld_stream(r + i), ld_stream(outdeg + i)}; }
+    __device__ __forceinline__ void apply(uint32_t i, uint64_t s, const In &x)
+    {
+        const float r_new = quant_update<KIND>(base, damp, scale, s);  // pageRank.c:1245,1253
+        st_stream(r + i, r_new);
+        const float c = x.od ? r_new / (float)x.od : 0.0f;  // pageRank.c:1205-1211
+        c_out[i] = c;
+        note_contribution(part, c);
+        const float diff = fabsf(r_new - x.r_old);
+        part.err += (double)diff * invN;
+        part.active += ((double)diff >= eps) ? 1u : 0u;
+        if (!x.od) part.dang += (double)r_new;
+        part.sum += (double)r_new;
+    }
+    __device__ __forceinline__ void operator()(uint32_t i, uint64_t s) { apply(i, s, load(i)); }
+};
+
+template <int KIND>
+__device__ __forceinline__ QuantEpi<KIND> make_quant_epi(const uint32_t *outdeg, uint32_t N, const IterView &A,
+                                                         const Slot *cur)
+{
+    QuantEpi<KIND> epi;
+    epi.r = A.r;
+    epi.c_out = reinterpret_cast<float *>(A.c_out);
+    epi.outdeg = outdeg;
+    epi.base = 1.0f - A.damp;  // pageRank.c:55
+    epi.damp = A.damp;
+    epi.scale = quant_scale<KIND>(*cur);
+    epi.invN = 1.0 / (double)N;
+    epi.eps = A.eps;
+    epi.part = {0.0, 0.0, 0.0, 0u, 0u};
+    return epi;
+}
+
 // Update coefficients of one iteration from the previous scalar slot.
 template <int KIND, class T>
 __device__ __forceinline__ PrEpi<KIND, T> make_epi(const uint32_t *outdeg, uint32_t N, const IterView &A,
@@ -253,6 +351,8 @@ __device__ __forceinline__ Slot next_slot(const Partial &tot, uint32_t iters, do
     s.sum = tot.sum;
     s.active = tot.active;
     s.iters = iters + 1;
+    s.cmax = tot.cmax;
+    s.cnmin = tot.cnmin;
     uint32_t done = 0;
     if (stop == GBX_STOP_ACTIVE) done = tot.active == 0;
     else if (stop == GBX_STOP_L1) done = KIND == K_ADJ ? tot.err < eps * tot.dang : tot.err < eps;

@@ -31,9 +31,18 @@ enum Kind : int {
     K_FIX32 = 7,
     K_FIX16 = 8,
     K_FIX8 = 9,
+    // the reference's quantized pull PageRank (pageRank.c:1159-1577): FP32
+    // contributions quantized per iteration to 32/16/8 bits with a min/max
+    // scale, exact integer row sums
+    K_Q32 = 10,
+    K_Q16 = 11,
+    K_Q8 = 12,
 };
 
 constexpr bool is_fixed_kind(int k) { return k >= K_FIX64 && k <= K_FIX8; }
+constexpr bool is_quant_kind(int k) { return k >= K_Q32 && k <= K_Q8; }
+// kinds whose binned row sums are the reference's own uint64 integer sums
+constexpr bool is_int_kind(int k) { return is_fixed_kind(k) || is_quant_kind(k); }
 
 struct ClassDesc {
     uint32_t lanes;       // lanes per row (1..64); 256 = hub chunks
@@ -65,14 +74,19 @@ struct SegRange {
     uint32_t hub_begin, hub_end;    // hub-row slots (offset by hub_slot_base)
 };
 
+// cmax / cnmin: the largest contribution and the complement of the smallest
+// as float bits (contributions are >= 0, so unsigned order is float order and
+// both reduce with max from 0); the quantized kinds take their scale from them.
 struct Slot {
     double err, dang, sum;
     uint32_t active, done, iters, conv;
+    uint32_t cmax, cnmin;
 };
 
 struct Partial {
     double err, dang, sum;
     uint32_t active, pad;
+    uint32_t cmax, cnmin;
 };
 
 }  // namespace gbx

@@ -68,11 +68,15 @@ __global__ void __launch_bounds__(kBlock) pr_reset_kernel(PlanView P, IterView A
     const float uni = (float)(1.0 / (double)P.N);
     T *c_out = reinterpret_cast<T *>(A.c_out);
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < P.N; i += gridDim.x * kBlock) {
-        float r0 = (KIND == K_PR_REF || is_fixed_kind(KIND)) ? base : (A.tele ? A.tele[i] : uni);
+        float r0 = (KIND == K_PR_REF || is_int_kind(KIND)) ? base : (A.tele ? A.tele[i] : uni);
         A.r[i] = r0;
         const uint32_t od = P.outdeg[i];
         if constexpr (is_fixed_kind(KIND)) {
             c_out[i] = od ? (T)fixed_conv<KIND>(r0 / (float)od) : (T)0;  // pageRank.c:543-548
+        } else if constexpr (is_quant_kind(KIND)) {
+            const float c0 = od ? r0 / (float)od : 0.0f;  // pageRank.c:1205-1211
+            c_out[i] = c0;
+            note_contribution(part, c0);
         } else {
             float c0;
             if (KIND == K_ADJ) c0 = r0;
@@ -95,6 +99,8 @@ __global__ void __launch_bounds__(kBlock) pr_reset_kernel(PlanView P, IterView A
         s.iters = 0;
         s.done = 0;
         s.conv = 0;
+        s.cmax = tot.cmax;
+        s.cnmin = tot.cnmin;
         *s0 = s;
     });
 }
@@ -818,12 +824,13 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
 {
     if (!ctx || !g || !p || !out) return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: NULL argument");
     *out = nullptr;
-    if (p->mode < GBX_PR_REF || p->mode > GBX_PR_FIXED8)
+    if (p->mode < GBX_PR_REF || p->mode > GBX_PR_QUANT8)
         return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: mode %d", p->mode);
-    const bool fixed = p->mode >= GBX_PR_FIXED64;
-    if (fixed && (p->weighted || p->method == GBX_METHOD_PULL))
-        return set_error(ctx, GBX_ERR_ARG,
-                         "gbx_pr_create: the fixed-point modes are unweighted and run on the binned method");
+    const bool fixed = p->mode >= GBX_PR_FIXED64 && p->mode <= GBX_PR_FIXED8;
+    const bool quant = p->mode >= GBX_PR_QUANT32;
+    if ((fixed || quant) && (p->weighted || p->method == GBX_METHOD_PULL))
+        return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: the fixed-point and quantized modes are unweighted "
+                                           "and run on the binned method");
     if (p->stop < 0 || p->stop > 2) return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: stop %d", p->stop);
     if (p->precision != GBX_F32 && p->precision != GBX_BF16)
         return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: precision %d not supported", p->precision);
@@ -871,10 +878,12 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
     pr->p = *p;
     pr->kind = p->mode == GBX_PR_REF ? K_PR_REF : (p->weighted ? K_PR_STOCH_W : K_PR_STOCH);
     if (fixed) pr->kind = K_FIX64 + (p->mode - GBX_PR_FIXED64);
+    if (quant) pr->kind = K_Q32 + (p->mode - GBX_PR_QUANT32);
     if (kind_override >= 0) pr->kind = kind_override;
     pr->grid = iter_grid(ctx, plan);
     pr->elem = p->precision == GBX_BF16 ? 2 : 4;
     if (fixed) pr->elem = (size_t)8 >> (p->mode - GBX_PR_FIXED64);  // 8, 4, 2, 1 bytes
+    if (quant) pr->elem = 4;  // FP32 contributions; quantized when loaded
     pr->hot = hot_count(ctx, g->N, pr->elem);
     const size_t nb = (size_t)g->N * 4;
     const size_t cb = (size_t)g->N * pr->elem + 16;
@@ -899,10 +908,14 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
                                                  "STOCHASTIC PageRank on one graph");
             binned = true;
         } else if (p->method == GBX_METHOD_AUTO) {
-            binned = kind_override < 0 && binned_supported(pr) && (fixed || g->N >= ctx->tune.binned_min_vertices);
+            binned = kind_override < 0 && binned_supported(pr) &&
+                     (fixed || quant || g->N >= ctx->tune.binned_min_vertices);
         } else if (p->method != GBX_METHOD_PULL) {
             rc = set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: method %d", p->method);
         }
+        if (!rc && !binned && (fixed || quant))
+            rc = set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: the fixed-point and quantized modes run on the binned "
+                                             "method, which this graph does not support");
         if (!rc && binned) rc = binned_create(pr);
     }
     if (rc) {
@@ -954,6 +967,9 @@ int gbx_pr_reset(gbx_pr *pr, const float *teleport)
         case K_FIX16: pr_reset_kernel<K_FIX16, uint16_t><<<grid, kBlock, 0, st>>>(P, A); break;
         default: pr_reset_kernel<K_FIX8, uint8_t><<<grid, kBlock, 0, st>>>(P, A); break;
         }
+    } else if (is_quant_kind(pr->kind)) {
+        // contributions and their min/max do not depend on the width
+        pr_reset_kernel<K_Q32, float><<<grid, kBlock, 0, st>>>(P, A);
     } else if (pr->elem == 4) {
         switch (pr->kind) {
         case K_PR_REF: pr_reset_kernel<K_PR_REF, float><<<grid, kBlock, 0, st>>>(P, A); break;
@@ -1078,7 +1094,7 @@ int gbx_pr_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep)
     const Slot *slot = pr->slots.as<Slot>() + (pr->t & 1);
     pr_finalize_kernel<<<grid_for(N, 256, 16384), 256, 0, ctx->stream>>>(
         pr->r.as<float>(), pr->plan->perm.as<uint32_t>(), N, slot,
-        pr->kind == K_ADJ ? 2 : ((pr->kind == K_PR_REF || is_fixed_kind(pr->kind)) ? 1 : 0), pr->out.as<float>());
+        pr->kind == K_ADJ ? 2 : ((pr->kind == K_PR_REF || is_int_kind(pr->kind)) ? 1 : 0), pr->out.as<float>());
     GBX_HIP(ctx, hipGetLastError());
     GBX_HIP(ctx, hipMemcpyAsync(pr->h_slot, slot, sizeof(Slot), hipMemcpyDeviceToHost, ctx->stream));
     if (ranks_out)

@@ -466,9 +466,27 @@ struct PageRankStats *pageRankPullFixedPoint8BitGraphCSR(struct Arguments *argum
     return run_pagerank(arguments, graph, GBX_PR_FIXED8, "Page Rank Pull Fixed Point 8-bit (HIP)");
 }
 
+/* pageRankPullQuant32BitGraphCSR, pageRank.c:1159-1297 (bit-exact) */
+struct PageRankStats *pageRankPullQuant32BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_pagerank(arguments, graph, GBX_PR_QUANT32, "Page Rank Pull Quant_32 (HIP)");
+}
+
+/* pageRankPullQuant16BitGraphCSR, pageRank.c:1299-1436 (bit-exact) */
+struct PageRankStats *pageRankPullQuant16BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_pagerank(arguments, graph, GBX_PR_QUANT16, "Page Rank Pull Quant_16 (HIP)");
+}
+
+/* pageRankPullQuant8BitGraphCSR, pageRank.c:1438-1577 (bit-exact) */
+struct PageRankStats *pageRankPullQuant8BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_pagerank(arguments, graph, GBX_PR_QUANT8, "Page Rank Pull Quant_8 (HIP)");
+}
+
 /* pageRankGraphCSR, pageRank.c:177-242: the variant selected by
- * arguments->pushpull; codes this backend does not provide (quantised 4, 5,
- * 12, 13 and data driven 6, 7, 8) return NULL. */
+ * arguments->pushpull; codes this backend does not provide (push quantised
+ * 5 and data driven 6, 7, 8) return NULL. */
 struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
     switch (arguments->pushpull) {
@@ -478,7 +496,10 @@ struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct Graph
     case 9: return pageRankPullFixedPoint32BitGraphCSR(arguments, graph);
     case 10: return pageRankPullFixedPoint16BitGraphCSR(arguments, graph);
     case 11: return pageRankPullFixedPoint8BitGraphCSR(arguments, graph);
-    case 4: case 5: case 6: case 7: case 8: case 12: case 13:
+    case 4: return pageRankPullQuant32BitGraphCSR(arguments, graph);
+    case 12: return pageRankPullQuant16BitGraphCSR(arguments, graph);
+    case 13: return pageRankPullQuant8BitGraphCSR(arguments, graph);
+    case 5: case 6: case 7: case 8:
         fprintf(stderr, "graphblox_hip: PageRank variant %u is not provided by the HIP backend\n",
                 arguments->pushpull);
         return NULL;

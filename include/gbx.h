@@ -149,6 +149,14 @@ int gbx_graph_free(gbx_graph *g);
 #define GBX_PR_FIXED32 3
 #define GBX_PR_FIXED16 4
 #define GBX_PR_FIXED8 5
+/* The reference's quantized pull PageRank, bit-exact (REF update; contributions
+ * r/outdeg quantized each iteration with the scale (max - min) / (2^k - 1) of
+ * their exact min and max, summed as integers; binned method only):
+ * pageRankPullQuant{32,16,8}BitGraphCSR (pageRank.c:1159-1577), pushpull
+ * codes 4, 12, 13. */
+#define GBX_PR_QUANT32 6
+#define GBX_PR_QUANT16 7
+#define GBX_PR_QUANT8 8
 
 #define GBX_STOP_ACTIVE 0    /* stop when no vertex has |dr| >= epsilon (pageRank.c:327-339) */
 #define GBX_STOP_L1 1        /* stop when sum |dr| < epsilon (normalised scale) */
@@ -171,7 +179,7 @@ typedef struct gbx_pr_params {
     float damp;          /* damping factor, 0.85f in the reference (pageRank.h:13) */
     uint32_t max_iters;  /* arguments->iterations */
     double epsilon;      /* threshold for the stop rule */
-    int mode;            /* GBX_PR_REF / GBX_PR_STOCHASTIC / GBX_PR_FIXED* */
+    int mode;            /* GBX_PR_REF / GBX_PR_STOCHASTIC / GBX_PR_FIXED* / GBX_PR_QUANT* */
     int stop;            /* GBX_STOP_* */
     int precision;       /* GBX_F32 / GBX_BF16 */
     int direction;       /* GBX_PULL_IN / GBX_PULL_FWD */

@@ -165,6 +165,10 @@ struct PageRankStats *pageRankPushFixedPointGraphCSR(struct Arguments *arguments
 struct PageRankStats *pageRankPullFixedPoint32BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct PageRankStats *pageRankPullFixedPoint16BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct PageRankStats *pageRankPullFixedPoint8BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:1159-1577: the quantized pull variants, bit-exact */
+struct PageRankStats *pageRankPullQuant32BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct PageRankStats *pageRankPullQuant16BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct PageRankStats *pageRankPullQuant8BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 
 /* SPMV.c:27-215 */
 struct SPMVStats *newSPMVStatsGraphCSR(struct GraphCSR *graph);

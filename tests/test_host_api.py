@@ -127,7 +127,7 @@ def test_preprocessing_and_pagerank_through_reference_api(golden, tmp_path, name
         v = golden(f"variants_{name}")
         a.contents.iterations = 20
         a.contents.epsilon = 0.0
-        for code in (1, 2, 3, 9, 10, 11):
+        for code in (1, 2, 3, 4, 9, 10, 11, 12, 13):
             a.contents.pushpull = code
             st = L.pageRankGraphCSR(a, g)
             assert st, code
@@ -137,7 +137,7 @@ def test_preprocessing_and_pagerank_through_reference_api(golden, tmp_path, name
             else:
                 np.testing.assert_array_equal(r, v[f"pp{code}_r20"])
             L.freePageRankStats(st)
-        for code in (4, 5, 6, 7, 8, 12, 13):  # quantised and data-driven variants: not provided
+        for code in (5, 6, 7, 8):  # push quantised and data-driven variants: not provided
             a.contents.pushpull = code
             assert not L.pageRankGraphCSR(a, g)
         a.contents.pushpull = 0
