@@ -312,6 +312,64 @@ __device__ __forceinline__ QuantEpi<KIND> make_quant_epi(const uint32_t *outdeg,
     return epi;
 }
 
+// ------------------------------------------------------------------ data-driven pull
+// pageRankDataDrivenPullGraphCSR (pageRank.c:1711-1870): every row is on the
+// worklist in the first iteration, afterwards the rows with an in-neighbour
+// updated in the previous iteration.  A worklist row computes r' = base +
+// d * sum, adds |r' - r| / N to the error, and only when |r' - r| >= epsilon
+// stores r', counts as active and puts its out-neighbours on the next
+// worklist.  Its next contribution carries that flag in the sign bit; the row
+// accumulator (AccDD) hands each row its sum with the OR of its in-edges'
+// flags as the sign, so a row learns from its own in-edges whether it is on
+// the worklist.
+struct DdEpi {
+    float *__restrict__ r;
+    float *__restrict__ c_out;
+    const uint32_t *__restrict__ outdeg;
+    float base, damp;
+    double dN, eps;
+    bool first;
+    Partial part;
+    __device__ __forceinline__ float init(uint32_t) const { return 0.0f; }
+    __device__ __forceinline__ void operator()(uint32_t i, float acc)
+    {
+        const float r_old = ld_stream(r + i);
+        const uint32_t od = ld_stream(outdeg + i);
+        float r_now = r_old;
+        uint32_t upd = 0;
+        if (first || (__float_as_uint(acc) >> 31)) {
+            const float r_new = base + damp * fabsf(acc);  // pageRank.c:1800
+            const double e = (double)fabsf(r_new - r_old);
+            part.err += e / dN;
+            if (e >= eps) {  // pageRank.c:1803-1818
+                r_now = r_new;
+                upd = 0x80000000u;
+                part.active += 1u;
+                st_stream(r + i, r_new);
+            }
+        }
+        const float c = od ? r_now / (float)od : 0.0f;  // pageRank.c:1772-1778
+        st_stream(c_out + i, __uint_as_float(__float_as_uint(c) | upd));
+        if (!od) part.dang += (double)r_now;
+        part.sum += (double)r_now;
+    }
+};
+
+__device__ __forceinline__ DdEpi make_dd_epi(const uint32_t *outdeg, uint32_t N, const IterView &A, const Slot *cur)
+{
+    DdEpi epi;
+    epi.r = A.r;
+    epi.c_out = reinterpret_cast<float *>(A.c_out);
+    epi.outdeg = outdeg;
+    epi.base = 1.0f - A.damp;  // pageRank.c:55
+    epi.damp = A.damp;
+    epi.dN = (double)N;
+    epi.eps = A.eps;
+    epi.first = cur->iters == 0;
+    epi.part = {0.0, 0.0, 0.0, 0u, 0u};
+    return epi;
+}
+
 // Update coefficients of one iteration from the previous scalar slot.
 template <int KIND, class T>
 __device__ __forceinline__ PrEpi<KIND, T> make_epi(const uint32_t *outdeg, uint32_t N, const IterView &A,

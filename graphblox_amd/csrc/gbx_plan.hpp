@@ -37,6 +37,9 @@ enum Kind : int {
     K_Q32 = 10,
     K_Q16 = 11,
     K_Q8 = 12,
+    // the reference's data-driven pull PageRank (pageRank.c:1711-1870):
+    // worklist rows only, conditional update (pull iteration)
+    K_DD = 13,
 };
 
 constexpr bool is_fixed_kind(int k) { return k >= K_FIX64 && k <= K_FIX8; }
@@ -212,6 +215,37 @@ struct AccKahan {
         merge(o);
     }
     __device__ __forceinline__ float value() const { return s - c; }
+};
+
+// Data-driven pull: contributions carry "updated in the previous iteration"
+// in their sign bit (they are >= 0 otherwise).  Sums the magnitudes in the
+// same order as AccF32 and ORs the sign bits; value() returns the sum with
+// the OR as its sign (a flagged row's sum is > 0: its flagged in-neighbour
+// has an out-edge, so a positive contribution).
+struct AccDD {
+    float s;
+    uint32_t f;
+    __device__ __forceinline__ void init(float v)
+    {
+        s = fabsf(v);
+        f = __float_as_uint(v) & 0x80000000u;
+    }
+    __device__ __forceinline__ void add(float x)
+    {
+        s += fabsf(x);
+        f |= __float_as_uint(x) & 0x80000000u;
+    }
+    __device__ __forceinline__ void merge(const AccDD &o)
+    {
+        s += o.s;
+        f |= o.f;
+    }
+    __device__ __forceinline__ void shfl_merge(int off)
+    {
+        s += __shfl_xor(s, off, 64);
+        f |= (uint32_t)__shfl_xor((int)f, off, 64);
+    }
+    __device__ __forceinline__ float value() const { return __uint_as_float(__float_as_uint(s) | f); }
 };
 
 // ------------------------------------------------------------------ element types

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 #include "gbx_iter.hpp"
@@ -44,7 +45,9 @@ __global__ void __launch_bounds__(kBlock) pr_iter_kernel(PlanView P, IterView A)
     }
     const T *c_in = reinterpret_cast<const T *>(A.c_in);
     if constexpr (LDS) load_hot(c_in, A.hot);
-    PrEpi<KIND, T> epi = make_epi<KIND, T>(P.outdeg, P.N, A, cur);
+    std::conditional_t<KIND == K_DD, DdEpi, PrEpi<KIND, T>> epi;
+    if constexpr (KIND == K_DD) epi = make_dd_epi(P.outdeg, P.N, A, cur);
+    else epi = make_epi<KIND, T>(P.outdeg, P.N, A, cur);
     if constexpr (KIND == K_PR_STOCH_W || KIND == K_ADJ) {
         GatherCW<T, LDS> g{c_in, reinterpret_cast<const T *>(P.wt), A.hot};
         tile_loop<Acc>(P, g, epi);
@@ -766,6 +769,7 @@ static int launch_iter(gbx_pr *pr)
         case K_PR_REF: launch_pr_kernel<K_PR_REF, float, AccF32>(lds, g, shm, st, P, A); break;
         case K_PR_STOCH: launch_pr_kernel<K_PR_STOCH, float, AccF32>(lds, g, shm, st, P, A); break;
         case K_ADJ: launch_pr_kernel<K_ADJ, float, AccF32>(lds, g, shm, st, P, A); break;
+        case K_DD: launch_pr_kernel<K_DD, float, AccDD>(lds, g, shm, st, P, A); break;
         default: launch_pr_kernel<K_PR_STOCH_W, float, AccF32>(lds, g, shm, st, P, A); break;
         }
     } else {
@@ -824,10 +828,14 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
 {
     if (!ctx || !g || !p || !out) return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: NULL argument");
     *out = nullptr;
-    if (p->mode < GBX_PR_REF || p->mode > GBX_PR_QUANT8)
+    if (p->mode < GBX_PR_REF || p->mode > GBX_PR_DATA_DRIVEN)
         return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: mode %d", p->mode);
+    const bool dd = p->mode == GBX_PR_DATA_DRIVEN;
+    if (dd && (p->weighted || p->precision != GBX_F32 || p->method == GBX_METHOD_BINNED))
+        return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: the data-driven mode is unweighted FP32 on the pull "
+                                           "method");
     const bool fixed = p->mode >= GBX_PR_FIXED64 && p->mode <= GBX_PR_FIXED8;
-    const bool quant = p->mode >= GBX_PR_QUANT32;
+    const bool quant = p->mode >= GBX_PR_QUANT32 && p->mode <= GBX_PR_QUANT8;
     if ((fixed || quant) && (p->weighted || p->method == GBX_METHOD_PULL))
         return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: the fixed-point and quantized modes are unweighted "
                                            "and run on the binned method");
@@ -879,6 +887,7 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
     pr->kind = p->mode == GBX_PR_REF ? K_PR_REF : (p->weighted ? K_PR_STOCH_W : K_PR_STOCH);
     if (fixed) pr->kind = K_FIX64 + (p->mode - GBX_PR_FIXED64);
     if (quant) pr->kind = K_Q32 + (p->mode - GBX_PR_QUANT32);
+    if (dd) pr->kind = K_DD;
     if (kind_override >= 0) pr->kind = kind_override;
     pr->grid = iter_grid(ctx, plan);
     pr->elem = p->precision == GBX_BF16 ? 2 : 4;
@@ -972,7 +981,8 @@ int gbx_pr_reset(gbx_pr *pr, const float *teleport)
         pr_reset_kernel<K_Q32, float><<<grid, kBlock, 0, st>>>(P, A);
     } else if (pr->elem == 4) {
         switch (pr->kind) {
-        case K_PR_REF: pr_reset_kernel<K_PR_REF, float><<<grid, kBlock, 0, st>>>(P, A); break;
+        case K_PR_REF:
+        case K_DD: pr_reset_kernel<K_PR_REF, float><<<grid, kBlock, 0, st>>>(P, A); break;
         case K_PR_STOCH: pr_reset_kernel<K_PR_STOCH, float><<<grid, kBlock, 0, st>>>(P, A); break;
         case K_ADJ: pr_reset_kernel<K_ADJ, float><<<grid, kBlock, 0, st>>>(P, A); break;
         default: pr_reset_kernel<K_PR_STOCH_W, float><<<grid, kBlock, 0, st>>>(P, A); break;
@@ -1092,9 +1102,11 @@ int gbx_pr_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep)
     }
     const uint32_t N = pr->g->N;
     const Slot *slot = pr->slots.as<Slot>() + (pr->t & 1);
+    // REF-update kinds divide by N, the adjoint is raw, the stochastic forms divide by the rank sum
+    const bool ref_update = pr->kind == K_PR_REF || pr->kind == K_DD || is_int_kind(pr->kind);
     pr_finalize_kernel<<<grid_for(N, 256, 16384), 256, 0, ctx->stream>>>(
-        pr->r.as<float>(), pr->plan->perm.as<uint32_t>(), N, slot,
-        pr->kind == K_ADJ ? 2 : ((pr->kind == K_PR_REF || is_int_kind(pr->kind)) ? 1 : 0), pr->out.as<float>());
+        pr->r.as<float>(), pr->plan->perm.as<uint32_t>(), N, slot, pr->kind == K_ADJ ? 2 : (ref_update ? 1 : 0),
+        pr->out.as<float>());
     GBX_HIP(ctx, hipGetLastError());
     GBX_HIP(ctx, hipMemcpyAsync(pr->h_slot, slot, sizeof(Slot), hipMemcpyDeviceToHost, ctx->stream));
     if (ranks_out)

@@ -484,9 +484,15 @@ struct PageRankStats *pageRankPullQuant8BitGraphCSR(struct Arguments *arguments,
     return run_pagerank(arguments, graph, GBX_PR_QUANT8, "Page Rank Pull Quant_8 (HIP)");
 }
 
+/* pageRankDataDrivenPullGraphCSR, pageRank.c:1711-1870 */
+struct PageRankStats *pageRankDataDrivenPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_pagerank(arguments, graph, GBX_PR_DATA_DRIVEN, "Page Rank Pull DD (HIP)");
+}
+
 /* pageRankGraphCSR, pageRank.c:177-242: the variant selected by
  * arguments->pushpull; codes this backend does not provide (push quantised
- * 5 and data driven 6, 7, 8) return NULL. */
+ * 5, data-driven push 7 and pull-push 8) return NULL. */
 struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
     switch (arguments->pushpull) {
@@ -499,7 +505,8 @@ struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct Graph
     case 4: return pageRankPullQuant32BitGraphCSR(arguments, graph);
     case 12: return pageRankPullQuant16BitGraphCSR(arguments, graph);
     case 13: return pageRankPullQuant8BitGraphCSR(arguments, graph);
-    case 5: case 6: case 7: case 8:
+    case 6: return pageRankDataDrivenPullGraphCSR(arguments, graph);
+    case 5: case 7: case 8:
         fprintf(stderr, "graphblox_hip: PageRank variant %u is not provided by the HIP backend\n",
                 arguments->pushpull);
         return NULL;

@@ -157,6 +157,12 @@ int gbx_graph_free(gbx_graph *g);
 #define GBX_PR_QUANT32 6
 #define GBX_PR_QUANT16 7
 #define GBX_PR_QUANT8 8
+/* The reference's data-driven pull PageRank (pageRankDataDrivenPullGraphCSR,
+ * pageRank.c:1711-1870; pushpull code 6): worklist rows only (all in the
+ * first iteration, then those with an in-neighbour updated in the previous
+ * one); a row stores its new rank only when it changed by epsilon or more.
+ * Unweighted FP32, pull method. */
+#define GBX_PR_DATA_DRIVEN 9
 
 #define GBX_STOP_ACTIVE 0    /* stop when no vertex has |dr| >= epsilon (pageRank.c:327-339) */
 #define GBX_STOP_L1 1        /* stop when sum |dr| < epsilon (normalised scale) */
@@ -179,7 +185,8 @@ typedef struct gbx_pr_params {
     float damp;          /* damping factor, 0.85f in the reference (pageRank.h:13) */
     uint32_t max_iters;  /* arguments->iterations */
     double epsilon;      /* threshold for the stop rule */
-    int mode;            /* GBX_PR_REF / GBX_PR_STOCHASTIC / GBX_PR_FIXED* / GBX_PR_QUANT* */
+    int mode;            /* GBX_PR_REF / GBX_PR_STOCHASTIC / GBX_PR_FIXED* / GBX_PR_QUANT* /
+                            GBX_PR_DATA_DRIVEN */
     int stop;            /* GBX_STOP_* */
     int precision;       /* GBX_F32 / GBX_BF16 */
     int direction;       /* GBX_PULL_IN / GBX_PULL_FWD */

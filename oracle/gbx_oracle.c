@@ -616,6 +616,58 @@ int orc_pr_pull_quant(uint32_t N, const uint32_t *outdeg, const uint64_t *row_pt
     return 0;
 }
 
+/* ---- data-driven pull PageRank (pageRank.c:1711-1870)
+ *
+ * Every vertex is on the worklist in the first iteration.  Per iteration:
+ * c = r / outdeg snapshot (float, 0 for outdeg 0); for each worklist row, a
+ * sequential float sum of c over its in-edges, r' = base + d * sum,
+ * error = |r' - r| (float difference) added as error / N; only when
+ * error >= epsilon is r' stored, the row counted active and its
+ * out-neighbours put on the next worklist.  Stop when no row is active. */
+int orc_pr_pull_dd(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
+                   const uint32_t *col, const uint64_t *fwd_row_ptr, const uint32_t *fwd_col,
+                   uint32_t max_iters, double epsilon, float *ranks_out, uint32_t *iters_out,
+                   double *err_out)
+{
+    float *c = malloc((size_t)N * sizeof(float) + 8);
+    uint8_t *cur = malloc((size_t)N + 8), *nxt = malloc((size_t)N + 8);
+    if (!c || !cur || !nxt) { free(c); free(cur); free(nxt); return -1; }
+    const float damp = 0.85f, base = 1.0f - damp; /* pageRank.c:55 */
+    float *r = ranks_out;
+    for (uint32_t v = 0; v < N; v++) { r[v] = base; cur[v] = 1; nxt[v] = 0; }
+    uint32_t it = 0;
+    double err = 0.0;
+    for (it = 0; it < max_iters; it++) {
+        for (uint32_t v = 0; v < N; v++) c[v] = outdeg[v] ? r[v] / (float)outdeg[v] : 0.0f;
+        uint32_t active = 0;
+        err = 0.0;
+        for (uint32_t v = 0; v < N; v++) {
+            if (!cur[v]) continue;
+            float s = 0.0f;
+            for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) s += c[col[j] & VERTEX_ID_MASK];
+            const float next = base + damp * s;
+            const double e = fabs(next - r[v]);
+            err += e / N;
+            if (e >= epsilon) {
+                r[v] = next;
+                for (uint64_t j = fwd_row_ptr[v]; j < fwd_row_ptr[v + 1]; j++) nxt[fwd_col[j] & VERTEX_ID_MASK] = 1;
+                active++;
+            }
+        }
+        uint8_t *t = cur; cur = nxt; nxt = t;
+        memset(nxt, 0, N);
+        if (active == 0) {
+            it++;
+            break;
+        }
+    }
+    for (uint32_t v = 0; v < N; v++) r[v] /= N;
+    if (iters_out) *iters_out = it < max_iters ? it - 1 : it;
+    if (err_out) *err_out = err;
+    free(c); free(cur); free(nxt);
+    return 0;
+}
+
 /* ---- R-MAT generator (counter-based; identical integer arithmetic on GPU) */
 
 static uint64_t mix64(uint64_t z)

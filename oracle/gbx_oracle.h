@@ -16,6 +16,9 @@
  *   - FP32 OpenMP pull SpMV                   01_host/src/algorithms/openmp/SPMV.c:117-215
  *   - FP64 PageRank oracles (reference-semantics and stochastic/personalised)
  *   - FP64 adjoint of the stochastic weighted PageRank fixed point
+ *   - the reference's PageRank variants: fixed-point pull 64/32/16/8-bit,
+ *     quantized pull 32/16/8-bit, data-driven pull
+ *                                             01_host/src/algorithms/openmp/pageRank.c:496-1870
  *   - the seeded counter-based R-MAT generator (no reference counterpart)
  */
 #ifndef GBX_ORACLE_H
@@ -126,6 +129,15 @@ int orc_pr_pull_fixed(uint32_t N, const uint32_t *outdeg, const uint64_t *row_pt
 int orc_pr_pull_quant(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
                       const uint32_t *col, int bits, uint32_t max_iters, double epsilon,
                       float *ranks_out, uint32_t *iters_out, double *err_out);
+
+/* The reference's data-driven pull PageRank (pageRankDataDrivenPullGraphCSR,
+ * pageRank.c:1711-1870; pushpull code 6): worklist rows only, conditional
+ * update, the out-neighbours of updated rows form the next worklist.  row_ptr
+ * and col: in-CSR; fwd_row_ptr and fwd_col: forward CSR (out-neighbours). */
+int orc_pr_pull_dd(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
+                   const uint32_t *col, const uint64_t *fwd_row_ptr, const uint32_t *fwd_col,
+                   uint32_t max_iters, double epsilon, float *ranks_out, uint32_t *iters_out,
+                   double *err_out);
 
 /* Seeded counter-based R-MAT generator (same integer arithmetic as the GPU
  * generator).  Writes num_edges = edge_factor << scale edges, ids permuted

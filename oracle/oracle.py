@@ -67,6 +67,8 @@ def lib():
         L.orc_pr_pull_fixed.argtypes = [C.c_uint32, _u32p, _u64p, _u32p, C.c_int, C.c_uint32, C.c_double, _f32p,
                                         C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
         L.orc_pr_pull_quant.argtypes = L.orc_pr_pull_fixed.argtypes
+        L.orc_pr_pull_dd.argtypes = [C.c_uint32, _u32p, _u64p, _u32p, _u64p, _u32p, C.c_uint32, C.c_double, _f32p,
+                                     C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
         L.orc_rmat_generate.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                         C.c_uint64, _u32p, _u32p]
         L.orc_rmat_permute.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64]
@@ -256,6 +258,19 @@ def pr_pull_fixed(N, outdeg, row_ptr, col, bits, iterations=20, epsilon=0.0):
                                  C.byref(err))
     if rc:
         raise ValueError(f"bits {bits}")
+    return r, it.value, err.value
+
+
+def pr_pull_dd(N, outdeg, row_ptr, col, fwd_row_ptr, fwd_col, iterations=20, epsilon=0.0):
+    """The reference's data-driven pull PageRank (worklist, conditional update)."""
+    r = np.zeros(N, np.float32)
+    it, err = C.c_uint32(), C.c_double()
+    rc = lib().orc_pr_pull_dd(N, np.ascontiguousarray(outdeg, np.uint32), np.ascontiguousarray(row_ptr, np.uint64),
+                              np.ascontiguousarray(col, np.uint32), np.ascontiguousarray(fwd_row_ptr, np.uint64),
+                              np.ascontiguousarray(fwd_col, np.uint32), iterations, epsilon, r, C.byref(it),
+                              C.byref(err))
+    if rc:
+        raise MemoryError("orc_pr_pull_dd")
     return r, it.value, err.value
 
 

@@ -7,9 +7,10 @@ after 20 iterations with epsilon 0 (`pp<c>_r20`, `pp<c>_err20`) and, with
 epsilon 1e-4 and up to 100 iterations, the ranks, the stopping iteration
 and the error (`pp<c>_eps_ranks`, `pp<c>_eps_iters`, `pp<c>_eps_err`).
 Codes: 1 push, 2 pull fixed point 64-bit, 3 push fixed point, 4 pull
-quantised 32-bit, 5 push quantised, 9/10/11 pull fixed point 32/16/8-bit,
-12/13 pull quantised 16/8-bit.  One OpenMP thread, so the float-atomic push
-variants are deterministic.  The graphs are those of make_goldens.py (the
+quantised 32-bit, 5 push quantised, 6 data-driven pull, 7 data-driven push,
+8 data-driven pull-push, 9/10/11 pull fixed point 32/16/8-bit, 12/13 pull
+quantised 16/8-bit.  One OpenMP thread, so the float-atomic push variants
+and the in-place data-driven push variants are deterministic.  The graphs are those of make_goldens.py (the
 shipped fixtures and the seeded R-MAT scale 8-12 graphs), plus two graphs
 without dangling vertices, whose edges are stored with them: `ring_rmat_s10`
 (the R-MAT scale 10 graph and the ring v -> v+1 mod N: the quantised
@@ -35,7 +36,7 @@ from oracle import oracle as O  # noqa: E402
 REF_TEST = "/root/reference/03_data/TEST"
 FIXTURES = ["v500_e500", "v1024_e4096", "v8192_e32768", "v1000_e10000", "v529_e3500"]
 RMAT_SEED = 20261203
-CODES = (1, 2, 3, 4, 5, 9, 10, 11, 12, 13)
+CODES = (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13)
 
 
 def variants_for(path: str, name: str, edges=None):

@@ -363,6 +363,30 @@ def test_ring_golden_reaches_the_wrap_and_saturation():
     assert (c.astype(np.float64) / (spread / 4294967295.0) + 0.5 >= 2.0**32).sum() > 100
 
 
+@pytest.mark.parametrize("name", VARIANT_GRAPHS + RING_GRAPHS)
+def test_data_driven_pull_oracle_matches_reference(golden, name):
+    """pageRankDataDrivenPullGraphCSR (pushpull 6): bit-exact ranks, error
+    and stopping iteration (sequential float row sums, as the reference)."""
+    v, fwd, inv, rp = _variant_csr(golden, name)
+    frp = O.row_ptr_from(fwd)
+    r, _, err = O.pr_pull_dd(fwd.N, fwd.out_degree, rp, inv.dst, frp, fwd.dst, 20, 0.0)
+    np.testing.assert_array_equal(r, v["pp6_r20"])
+    assert err == float(v["pp6_err20"])
+    r, it, err = O.pr_pull_dd(fwd.N, fwd.out_degree, rp, inv.dst, frp, fwd.dst, 100, 1e-4)
+    assert it == int(v["pp6_eps_iters"]) and err == float(v["pp6_eps_err"])
+    np.testing.assert_array_equal(r, v["pp6_eps_ranks"])
+
+
+def test_data_driven_pull_is_ref_at_epsilon_zero_and_differs_above(golden):
+    """With epsilon 0 every worklist row is updated and a row leaves the
+    worklist only when its in-neighbours stop changing, so the data-driven
+    pull is the plain pull; with epsilon > 0 rows below the threshold keep
+    their rank and the outputs differ."""
+    v, fwd, inv, rp = _variant_csr(golden, "rmat_s12")
+    np.testing.assert_array_equal(v["pp6_r20"], golden("rmat_s12")["pr_r20"])
+    assert not np.array_equal(v["pp6_eps_ranks"], golden("rmat_s12")["pr_eps_ranks"])
+
+
 def test_quantized_goldens_tell_widths_apart(golden):
     v, fwd, inv, rp = _variant_csr(golden, "rmat_s10")
     r16, _, _ = O.pr_pull_quant(fwd.N, fwd.out_degree, rp, inv.dst, 16, 20, 0.0)
