@@ -25,7 +25,7 @@ GRAPH_DEDUPE, GRAPH_WEIGHTED, GRAPH_SYMMETRIC = 1, 2, 4
 PR_REF, PR_STOCHASTIC = 0, 1
 PR_FIXED64, PR_FIXED32, PR_FIXED16, PR_FIXED8 = 2, 3, 4, 5
 PR_QUANT32, PR_QUANT16, PR_QUANT8 = 6, 7, 8
-PR_DATA_DRIVEN = 9
+PR_DATA_DRIVEN, PR_DD_PUSH, PR_DD_PULL_PUSH = 9, 10, 11
 STOP_ACTIVE, STOP_L1, STOP_NONE = 0, 1, 2
 F32, BF16 = 0, 1
 PULL_IN, PULL_FWD = 0, 1

@@ -40,6 +40,10 @@ enum Kind : int {
     // the reference's data-driven pull PageRank (pageRank.c:1711-1870):
     // worklist rows only, conditional update (pull iteration)
     K_DD = 13,
+    // the reference's data-driven push and pull-push PageRank
+    // (pageRank.c:1872-2202): residual pushes over a frontier (gbx_ddpush.hip)
+    K_DDP = 14,
+    K_DDPP = 15,
 };
 
 constexpr bool is_fixed_kind(int k) { return k >= K_FIX64 && k <= K_FIX8; }
@@ -119,6 +123,7 @@ struct gbx_plan {
 struct gbx_ppr_state;
 struct gbx_seg_state;
 struct gbx_binned_state;
+struct gbx_ddp_state;
 
 struct gbx_pr {
     gbx_ctx *ctx = nullptr;
@@ -139,6 +144,7 @@ struct gbx_pr {
     gbx_ppr_state *batch = nullptr;  // multi-vector state (gbx_ppr.hip), else null
     gbx_seg_state *seg = nullptr;    // batch-of-graphs state (gbx_batch.hip), else null
     gbx_binned_state *bin = nullptr; // binned-method state (gbx_binned.hip), else null
+    gbx_ddp_state *ddp = nullptr;    // data-driven push state (gbx_ddpush.hip), else null
     // captured iteration sequences: key = (iterations << 1) | starting parity
     std::vector<std::pair<uint64_t, hipGraphExec_t>> graph_cache;
 };
@@ -467,6 +473,12 @@ bool binned_supported(const gbx_pr *pr);
 int binned_create(gbx_pr *pr);       // builds the plan's binned layout on first use
 int binned_launch_iter(gbx_pr *pr);  // scatter + accumulate: one iteration
 void binned_state_free(gbx_binned_state *s);
+
+// Data-driven push hooks implemented in gbx_ddpush.hip.
+int ddp_create(gbx_pr *pr);
+int ddp_reset(gbx_pr *pr);
+int ddp_launch_iter(gbx_pr *pr);
+void ddp_state_free(gbx_ddp_state *s);
 
 // Host helpers implemented in gbx_pr.hip.
 int ensure_plan(gbx_ctx *ctx, gbx_graph *g, int direction, gbx_plan **out);

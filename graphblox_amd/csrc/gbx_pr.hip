@@ -117,7 +117,7 @@ __global__ void pr_finalize_kernel(const float *__restrict__ r, const uint32_t *
     const double S = slot->sum > 0.0 ? slot->sum : 1.0;
     const float fS = (float)S;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
-        out[perm[i]] = ref == 2 ? r[i] : (ref ? r[i] / fN : r[i] / fS);
+        out[perm ? perm[i] : i] = ref == 2 ? r[i] : (ref ? r[i] / fN : r[i] / fS);
 }
 
 // ------------------------------------------------------------------ SpMV
@@ -757,6 +757,7 @@ static int launch_iter(gbx_pr *pr)
     if (pr->batch) return ppr_launch_iter(pr);
     if (pr->seg) return seg_launch_iter(pr);
     if (pr->bin) return binned_launch_iter(pr);
+    if (pr->ddp) return ddp_launch_iter(pr);
     gbx_ctx *ctx = pr->ctx;
     PlanView P = plan_view(pr->plan, plan_weights(pr));
     IterView A = iter_view(pr);
@@ -828,12 +829,14 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
 {
     if (!ctx || !g || !p || !out) return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: NULL argument");
     *out = nullptr;
-    if (p->mode < GBX_PR_REF || p->mode > GBX_PR_DATA_DRIVEN)
+    if (p->mode < GBX_PR_REF || p->mode > GBX_PR_DD_PULL_PUSH)
         return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: mode %d", p->mode);
     const bool dd = p->mode == GBX_PR_DATA_DRIVEN;
-    if (dd && (p->weighted || p->precision != GBX_F32 || p->method == GBX_METHOD_BINNED))
-        return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: the data-driven mode is unweighted FP32 on the pull "
-                                           "method");
+    const bool ddp = p->mode == GBX_PR_DD_PUSH || p->mode == GBX_PR_DD_PULL_PUSH;
+    if ((dd || ddp) && (p->weighted || p->precision != GBX_F32 || p->method == GBX_METHOD_BINNED))
+        return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: the data-driven modes are unweighted FP32 and do not "
+                                           "run on the binned method");
+    if (ddp && g->N == 0) return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: empty graph");
     const bool fixed = p->mode >= GBX_PR_FIXED64 && p->mode <= GBX_PR_FIXED8;
     const bool quant = p->mode >= GBX_PR_QUANT32 && p->mode <= GBX_PR_QUANT8;
     if ((fixed || quant) && (p->weighted || p->method == GBX_METHOD_PULL))
@@ -888,6 +891,7 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
     if (fixed) pr->kind = K_FIX64 + (p->mode - GBX_PR_FIXED64);
     if (quant) pr->kind = K_Q32 + (p->mode - GBX_PR_QUANT32);
     if (dd) pr->kind = K_DD;
+    if (ddp) pr->kind = p->mode == GBX_PR_DD_PUSH ? K_DDP : K_DDPP;
     if (kind_override >= 0) pr->kind = kind_override;
     pr->grid = iter_grid(ctx, plan);
     pr->elem = p->precision == GBX_BF16 ? 2 : 4;
@@ -926,6 +930,7 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
             rc = set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: the fixed-point and quantized modes run on the binned "
                                              "method, which this graph does not support");
         if (!rc && binned) rc = binned_create(pr);
+        if (!rc && ddp) rc = ddp_create(pr);
     }
     if (rc) {
         gbx_pr_free(pr);
@@ -964,6 +969,7 @@ int gbx_pr_reset(gbx_pr *pr, const float *teleport)
     pr->t = 0;
     pr->timing_started = false;
     pr->elapsed_ms = 0.0;
+    if (pr->ddp) return ddp_reset(pr);
     PlanView P = plan_view(pr->plan, plan_weights(pr));
     IterView A = iter_view(pr);
     A.c_out = pr->c[0].p;
@@ -1103,10 +1109,12 @@ int gbx_pr_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep)
     const uint32_t N = pr->g->N;
     const Slot *slot = pr->slots.as<Slot>() + (pr->t & 1);
     // REF-update kinds divide by N, the adjoint is raw, the stochastic forms divide by the rank sum
-    const bool ref_update = pr->kind == K_PR_REF || pr->kind == K_DD || is_int_kind(pr->kind);
+    const bool ref_update =
+        pr->kind == K_PR_REF || pr->kind == K_DD || pr->kind == K_DDP || pr->kind == K_DDPP || is_int_kind(pr->kind);
+    // the data-driven push ranks are kept in vertex order, the others in plan order
+    const uint32_t *perm = pr->ddp ? nullptr : pr->plan->perm.as<uint32_t>();
     pr_finalize_kernel<<<grid_for(N, 256, 16384), 256, 0, ctx->stream>>>(
-        pr->r.as<float>(), pr->plan->perm.as<uint32_t>(), N, slot, pr->kind == K_ADJ ? 2 : (ref_update ? 1 : 0),
-        pr->out.as<float>());
+        pr->r.as<float>(), perm, N, slot, pr->kind == K_ADJ ? 2 : (ref_update ? 1 : 0), pr->out.as<float>());
     GBX_HIP(ctx, hipGetLastError());
     GBX_HIP(ctx, hipMemcpyAsync(pr->h_slot, slot, sizeof(Slot), hipMemcpyDeviceToHost, ctx->stream));
     if (ranks_out)
@@ -1143,6 +1151,7 @@ int gbx_pr_free(gbx_pr *pr)
     if (pr->batch) ppr_state_free(pr->batch);
     if (pr->seg) seg_state_free(pr->seg);
     if (pr->bin) binned_state_free(pr->bin);
+    if (pr->ddp) ddp_state_free(pr->ddp);
     for (auto &e : pr->graph_cache) (void)hipGraphExecDestroy(e.second);
     delete pr;
     return GBX_OK;

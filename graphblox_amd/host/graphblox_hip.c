@@ -484,15 +484,38 @@ struct PageRankStats *pageRankPullQuant8BitGraphCSR(struct Arguments *arguments,
     return run_pagerank(arguments, graph, GBX_PR_QUANT8, "Page Rank Pull Quant_8 (HIP)");
 }
 
+/* pageRankPushQuantGraphCSR, pageRank.c:1579-1709: its float atomics add
+ * scale * (q - zero) per edge, and (q - zero) mod 2^32 is the 32-bit pull
+ * quantization of the same contribution, so it sums the terms of
+ * pageRankPullQuant32BitGraphCSR in float; this runs the exact-sum form */
+struct PageRankStats *pageRankPushQuantGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_pagerank(arguments, graph, GBX_PR_QUANT32, "Page Rank Push Quant_32 (HIP, pull sums)");
+}
+
 /* pageRankDataDrivenPullGraphCSR, pageRank.c:1711-1870 */
 struct PageRankStats *pageRankDataDrivenPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
     return run_pagerank(arguments, graph, GBX_PR_DATA_DRIVEN, "Page Rank Pull DD (HIP)");
 }
 
+/* pageRankDataDrivenPushGraphCSR, pageRank.c:1872-2027 (frontier iteration:
+ * the same fixed point, not the same order of in-place updates) */
+struct PageRankStats *pageRankDataDrivenPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_pagerank(arguments, graph, GBX_PR_DD_PUSH, "Page Rank Push DD (HIP)");
+}
+
+/* pageRankDataDrivenPullPushGraphCSR, pageRank.c:2029-2202 (frontier iteration) */
+struct PageRankStats *pageRankDataDrivenPullPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_pagerank(arguments, graph, GBX_PR_DD_PULL_PUSH, "Page Rank Pull-Push DD (HIP)");
+}
+
 /* pageRankGraphCSR, pageRank.c:177-242: the variant selected by
- * arguments->pushpull; codes this backend does not provide (push quantised
- * 5, data-driven push 7 and pull-push 8) return NULL. */
+ * arguments->pushpull; push quantised (5) runs the 32-bit quantized pull
+ * (pageRankPushQuantGraphCSR adds the same scale * q terms with float
+ * atomics). */
 struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
     switch (arguments->pushpull) {
@@ -505,11 +528,10 @@ struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct Graph
     case 4: return pageRankPullQuant32BitGraphCSR(arguments, graph);
     case 12: return pageRankPullQuant16BitGraphCSR(arguments, graph);
     case 13: return pageRankPullQuant8BitGraphCSR(arguments, graph);
+    case 5: return pageRankPushQuantGraphCSR(arguments, graph);
     case 6: return pageRankDataDrivenPullGraphCSR(arguments, graph);
-    case 5: case 7: case 8:
-        fprintf(stderr, "graphblox_hip: PageRank variant %u is not provided by the HIP backend\n",
-                arguments->pushpull);
-        return NULL;
+    case 7: return pageRankDataDrivenPushGraphCSR(arguments, graph);
+    case 8: return pageRankDataDrivenPullPushGraphCSR(arguments, graph);
     default: return pageRankPullGraphCSR(arguments, graph);
     }
 }

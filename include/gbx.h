@@ -163,6 +163,16 @@ int gbx_graph_free(gbx_graph *g);
  * one); a row stores its new rank only when it changed by epsilon or more.
  * Unweighted FP32, pull method. */
 #define GBX_PR_DATA_DRIVEN 9
+/* The reference's data-driven push (pageRankDataDrivenPushGraphCSR,
+ * pageRank.c:1872-2027; pushpull code 7) and pull-push (pageRankDataDriven
+ * PullPushGraphCSR, pageRank.c:2029-2202; code 8) PageRank: residuals pushed
+ * along out-edges over a frontier of vertices whose residual crossed epsilon.
+ * The reference updates in place in vertex order (its result depends on the
+ * thread schedule); here each iteration processes its frontier concurrently
+ * with float atomics, so results are reproducible to the fixed point the
+ * method converges to, not bit for bit.  Unweighted FP32. */
+#define GBX_PR_DD_PUSH 10
+#define GBX_PR_DD_PULL_PUSH 11
 
 #define GBX_STOP_ACTIVE 0    /* stop when no vertex has |dr| >= epsilon (pageRank.c:327-339) */
 #define GBX_STOP_L1 1        /* stop when sum |dr| < epsilon (normalised scale) */
@@ -186,7 +196,7 @@ typedef struct gbx_pr_params {
     uint32_t max_iters;  /* arguments->iterations */
     double epsilon;      /* threshold for the stop rule */
     int mode;            /* GBX_PR_REF / GBX_PR_STOCHASTIC / GBX_PR_FIXED* / GBX_PR_QUANT* /
-                            GBX_PR_DATA_DRIVEN */
+                            GBX_PR_DATA_DRIVEN / GBX_PR_DD_PUSH / GBX_PR_DD_PULL_PUSH */
     int stop;            /* GBX_STOP_* */
     int precision;       /* GBX_F32 / GBX_BF16 */
     int direction;       /* GBX_PULL_IN / GBX_PULL_FWD */

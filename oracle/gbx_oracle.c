@@ -668,6 +668,88 @@ int orc_pr_pull_dd(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
     return 0;
 }
 
+/* ---- data-driven push and pull-push PageRank (pageRank.c:1872-2202)
+ *
+ * Sequential, in place, in vertex order (one OpenMP thread of the
+ * reference).  Residuals start at (1-d) * d * sum over in-edges of
+ * 1.0f / outdeg[u] (float, in-edge order); every vertex is on the first
+ * worklist.  A worklist vertex v takes its new rank (push: res[v] + r[v];
+ * pull-push: base + d * the float sum of r[u] / outdeg[u] over its in-edges,
+ * reading ranks already updated in this sweep), adds |r'/N - r/N| to the
+ * error, pushes delta = d * (res[v] / outdeg[v]) onto each out-neighbour's
+ * residual and then zeroes its own (so a self loop's push is lost).  A push
+ * that lifts a residual from <= epsilon to >= epsilon counts as active and
+ * puts the neighbour on the next worklist; the pull-push form adds delta a
+ * second time on that event.  Stop when nothing became active. */
+int orc_pr_dd_push(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
+                   const uint32_t *col, const uint64_t *fwd_row_ptr, const uint32_t *fwd_col,
+                   int pull_push, uint32_t max_iters, double epsilon, float *ranks_out,
+                   uint32_t *iters_out, double *err_out)
+{
+    float *res = malloc((size_t)N * sizeof(float) + 8);
+    uint8_t *cur = malloc((size_t)N + 8), *nxt = malloc((size_t)N + 8);
+    if (!res || !cur || !nxt) { free(res); free(cur); free(nxt); return -1; }
+    const float damp = 0.85f, base = 1.0f - damp; /* pageRank.c:55 */
+    float *r = ranks_out;
+    for (uint32_t v = 0; v < N; v++) {
+        r[v] = base;
+        cur[v] = 1;
+        nxt[v] = 0;
+        float s = 0.0f;
+        for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) {
+            const uint32_t u = col[j] & VERTEX_ID_MASK;
+            if (outdeg[u]) s += 1.0f / (float)outdeg[u];
+        }
+        res[v] = (1.0f - damp) * damp * s;
+    }
+    uint32_t it = 0;
+    double err = 0.0;
+    for (it = 0; it < max_iters; it++) {
+        uint32_t active = 0;
+        err = 0.0;
+        for (uint32_t v = 0; v < N; v++) {
+            if (!cur[v]) continue;
+            const float old = r[v];
+            float next;
+            if (pull_push) {
+                float s = 0.0f;
+                for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++) {
+                    const uint32_t u = col[j] & VERTEX_ID_MASK;
+                    s += r[u] / (float)outdeg[u];
+                }
+                next = base + damp * s;
+            } else {
+                next = res[v] + r[v];
+            }
+            err += fabs(next / (float)N - old / (float)N);
+            r[v] = next;
+            const float delta = damp * (res[v] / (float)outdeg[v]);
+            for (uint64_t j = fwd_row_ptr[v]; j < fwd_row_ptr[v + 1]; j++) {
+                const uint32_t u = fwd_col[j] & VERTEX_ID_MASK;
+                const float prev = res[u];
+                res[u] += delta;
+                if (fabs(prev + delta) >= epsilon && prev <= epsilon) {
+                    active++;
+                    if (pull_push) res[u] += delta;
+                    nxt[u] = 1;
+                }
+            }
+            res[v] = 0.0f;
+        }
+        uint8_t *t = cur; cur = nxt; nxt = t;
+        memset(nxt, 0, N);
+        if (active == 0) {
+            it++;
+            break;
+        }
+    }
+    for (uint32_t v = 0; v < N; v++) r[v] /= N;
+    if (iters_out) *iters_out = it < max_iters ? it - 1 : it;
+    if (err_out) *err_out = err;
+    free(res); free(cur); free(nxt);
+    return 0;
+}
+
 /* ---- R-MAT generator (counter-based; identical integer arithmetic on GPU) */
 
 static uint64_t mix64(uint64_t z)

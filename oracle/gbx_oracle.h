@@ -17,8 +17,8 @@
  *   - FP64 PageRank oracles (reference-semantics and stochastic/personalised)
  *   - FP64 adjoint of the stochastic weighted PageRank fixed point
  *   - the reference's PageRank variants: fixed-point pull 64/32/16/8-bit,
- *     quantized pull 32/16/8-bit, data-driven pull
- *                                             01_host/src/algorithms/openmp/pageRank.c:496-1870
+ *     quantized pull 32/16/8-bit, data-driven pull, push and pull-push
+ *                                             01_host/src/algorithms/openmp/pageRank.c:496-2202
  *   - the seeded counter-based R-MAT generator (no reference counterpart)
  */
 #ifndef GBX_ORACLE_H
@@ -138,6 +138,16 @@ int orc_pr_pull_dd(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
                    const uint32_t *col, const uint64_t *fwd_row_ptr, const uint32_t *fwd_col,
                    uint32_t max_iters, double epsilon, float *ranks_out, uint32_t *iters_out,
                    double *err_out);
+
+/* The reference's data-driven push (pull_push 0, pageRankDataDrivenPush
+ * GraphCSR, pageRank.c:1872-2027, pushpull code 7) and pull-push (pull_push
+ * 1, pageRankDataDrivenPullPushGraphCSR, pageRank.c:2029-2202, code 8)
+ * PageRank, sequential and in place as one reference thread runs them:
+ * residual pushes along out-edges drive the worklist. */
+int orc_pr_dd_push(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
+                   const uint32_t *col, const uint64_t *fwd_row_ptr, const uint32_t *fwd_col,
+                   int pull_push, uint32_t max_iters, double epsilon, float *ranks_out,
+                   uint32_t *iters_out, double *err_out);
 
 /* Seeded counter-based R-MAT generator (same integer arithmetic as the GPU
  * generator).  Writes num_edges = edge_factor << scale edges, ids permuted

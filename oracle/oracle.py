@@ -69,6 +69,8 @@ def lib():
         L.orc_pr_pull_quant.argtypes = L.orc_pr_pull_fixed.argtypes
         L.orc_pr_pull_dd.argtypes = [C.c_uint32, _u32p, _u64p, _u32p, _u64p, _u32p, C.c_uint32, C.c_double, _f32p,
                                      C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
+        L.orc_pr_dd_push.argtypes = [C.c_uint32, _u32p, _u64p, _u32p, _u64p, _u32p, C.c_int, C.c_uint32, C.c_double,
+                                     _f32p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
         L.orc_rmat_generate.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                         C.c_uint64, _u32p, _u32p]
         L.orc_rmat_permute.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64]
@@ -272,6 +274,45 @@ def pr_pull_dd(N, outdeg, row_ptr, col, fwd_row_ptr, fwd_col, iterations=20, eps
     if rc:
         raise MemoryError("orc_pr_pull_dd")
     return r, it.value, err.value
+
+
+def pr_dd_push(N, outdeg, row_ptr, col, fwd_row_ptr, fwd_col, pull_push=False, iterations=20, epsilon=0.0):
+    """The reference's data-driven push (or pull-push) PageRank, sequential."""
+    r = np.zeros(N, np.float32)
+    it, err = C.c_uint32(), C.c_double()
+    rc = lib().orc_pr_dd_push(N, np.ascontiguousarray(outdeg, np.uint32), np.ascontiguousarray(row_ptr, np.uint64),
+                              np.ascontiguousarray(col, np.uint32), np.ascontiguousarray(fwd_row_ptr, np.uint64),
+                              np.ascontiguousarray(fwd_col, np.uint32), int(pull_push), iterations, epsilon, r,
+                              C.byref(it), C.byref(err))
+    if rc:
+        raise MemoryError("orc_pr_dd_push")
+    return r, it.value, err.value
+
+
+def dd_push_limit(N, outdeg, row_ptr, col, damp=0.85, tol=1e-15, max_iters=5000):
+    """FP64 limit of the reference's data-driven push PageRank
+    (pageRank.c:1872-2027), normalised by N: ranks start at 1-d with residual
+    res0 = (1-d) d P^T 1 (self loops included, pageRank.c:1908-1923); every
+    later push moves d * res / outdeg to each out-neighbour except the vertex
+    itself (its own residual is zeroed after the push loop), so
+    r = (1-d) 1 + (I - d P'^T)^-1 res0 with P' the transition matrix without
+    self loops (out-degrees unchanged)."""
+    od = np.asarray(outdeg, np.int64)
+    rp = np.asarray(row_ptr, np.int64)
+    c = np.asarray(col, np.int64) & 0x3FFFFFFF
+    rows = np.repeat(np.arange(N), np.diff(rp))
+    inv_od = np.where(od > 0, 1.0 / np.maximum(od, 1), 0.0)
+    res0 = (1.0 - damp) * damp * np.bincount(rows, weights=inv_od[c], minlength=N)
+    keep = rows != c
+    kr, kc = rows[keep], c[keep]
+    y = res0.copy()
+    for _ in range(max_iters):
+        y2 = res0 + damp * np.bincount(kr, weights=y[kc] * inv_od[kc], minlength=N)
+        done = np.abs(y2 - y).sum() < tol * max(1.0, np.abs(y2).sum())
+        y = y2
+        if done:
+            break
+    return ((1.0 - damp) + y) / N
 
 
 def pr_pull_quant(N, outdeg, row_ptr, col, bits, iterations=20, epsilon=0.0):

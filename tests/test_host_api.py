@@ -127,19 +127,31 @@ def test_preprocessing_and_pagerank_through_reference_api(golden, tmp_path, name
         v = golden(f"variants_{name}")
         a.contents.iterations = 20
         a.contents.epsilon = 0.0
-        for code in (1, 2, 3, 4, 6, 9, 10, 11, 12, 13):
+        for code in (1, 2, 3, 4, 5, 6, 9, 10, 11, 12, 13):
             a.contents.pushpull = code
             st = L.pageRankGraphCSR(a, g)
             assert st, code
             r = _arr(st.contents.pageRanks, N, np.float32)
             if code in (1, 6):  # float sums: the pull iteration's tolerance
                 assert l1(r, v[f"pp{code}_r20"]) <= 1e-5
+            elif code == 5:  # float-atomic sums of the 32-bit quantized terms (7e-8 from code 4 in the reference)
+                assert l1(r, v["pp5_r20"]) <= 1e-6
             else:
                 np.testing.assert_array_equal(r, v[f"pp{code}_r20"])
             L.freePageRankStats(st)
-        for code in (5, 7, 8):  # push quantised, data-driven push and pull-push: not provided
+        # data-driven push and pull-push: their limits as epsilon shrinks
+        fwd, inv = O.build_graph(z["edges_src"], z["edges_dst"], None, N, mode=1)
+        rp = O.row_ptr_from(inv)
+        lim = {7: O.dd_push_limit(N, fwd.out_degree, rp, inv.dst),
+               8: O.pr_f64(N, fwd.out_degree, rp, inv.dst, mode=0, iterations=3000, epsilon=1e-15)[0]}
+        a.contents.iterations = 3000
+        a.contents.epsilon = 1e-7
+        for code in (7, 8):
             a.contents.pushpull = code
-            assert not L.pageRankGraphCSR(a, g)
+            st = L.pageRankGraphCSR(a, g)
+            assert st, code
+            assert l1(_arr(st.contents.pageRanks, N, np.float32), lim[code]) <= 1e-7 / 0.15 + 1e-6
+            L.freePageRankStats(st)
         a.contents.pushpull = 0
     L.graphCSRFree(g)
     L.argumentsFree(a)

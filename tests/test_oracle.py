@@ -387,6 +387,40 @@ def test_data_driven_pull_is_ref_at_epsilon_zero_and_differs_above(golden):
     assert not np.array_equal(v["pp6_eps_ranks"], golden("rmat_s12")["pr_eps_ranks"])
 
 
+@pytest.mark.parametrize("name", VARIANT_GRAPHS + RING_GRAPHS)
+def test_data_driven_push_oracles_match_reference(golden, name):
+    """pageRankDataDriven{Push,PullPush}GraphCSR (pushpull 7, 8) as one
+    reference thread runs them: bit-exact ranks, error and iterations."""
+    v, fwd, inv, rp = _variant_csr(golden, name)
+    frp = O.row_ptr_from(fwd)
+    for code, pp in ((7, False), (8, True)):
+        r, _, err = O.pr_dd_push(fwd.N, fwd.out_degree, rp, inv.dst, frp, fwd.dst, pp, 20, 0.0)
+        np.testing.assert_array_equal(r, v[f"pp{code}_r20"])
+        assert err == float(v[f"pp{code}_err20"])
+        r, it, err = O.pr_dd_push(fwd.N, fwd.out_degree, rp, inv.dst, frp, fwd.dst, pp, 100, 1e-4)
+        assert it == int(v[f"pp{code}_eps_iters"]) and err == float(v[f"pp{code}_eps_err"])
+        np.testing.assert_array_equal(r, v[f"pp{code}_eps_ranks"])
+
+
+@pytest.mark.parametrize("name", ["fixture_v1024_e4096", "fixture_v529_e3500", "rmat_s12"])
+def test_data_driven_push_limits(golden, name):
+    """The fixed points the reference's own algorithms approach as epsilon
+    shrinks: the push form (7) the self-loop-free limit of dd_push_limit (the
+    fixtures hold self loops, whose later pushes the reference drops; the
+    ordinary fixed point is 1e-3 away there), the pull-push form (8) the
+    ordinary REF fixed point.  The GPU forms are held to the same limits."""
+    v, fwd, inv, rp = _variant_csr(golden, name)
+    frp = O.row_ptr_from(fwd)
+    lim = O.dd_push_limit(fwd.N, fwd.out_degree, rp, inv.dst)
+    fix, _, _ = O.pr_f64(fwd.N, fwd.out_degree, rp, inv.dst, mode=0, iterations=2000, epsilon=1e-15)
+    r7, _, _ = O.pr_dd_push(fwd.N, fwd.out_degree, rp, inv.dst, frp, fwd.dst, False, 3000, 1e-7)
+    r8, _, _ = O.pr_dd_push(fwd.N, fwd.out_degree, rp, inv.dst, frp, fwd.dst, True, 3000, 1e-7)
+    assert O_l1(r7, lim) <= 1e-6
+    assert O_l1(r8, fix) <= 1e-6
+    if name.startswith("fixture"):
+        assert O_l1(lim, fix) > 1e-4  # the self-loop rule matters on these graphs
+
+
 def test_quantized_goldens_tell_widths_apart(golden):
     v, fwd, inv, rp = _variant_csr(golden, "rmat_s10")
     r16, _, _ = O.pr_pull_quant(fwd.N, fwd.out_degree, rp, inv.dst, 16, 20, 0.0)
