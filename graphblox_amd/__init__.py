@@ -90,6 +90,7 @@ def lib():
         "gbx_stream": (_vp, [_vp]),
         "gbx_version": (C.c_char_p, []),
         "gbx_csr_build": (i32, [_vp, _vp, _vp, _vp, u32, u32, i32, i32, _vp, _vp, _vp, _vp, _vp]),
+        "gbx_rank_sort": (i32, [_vp, _vp, u32, _vp]),
         "gbx_graph_create": (i32, [_vp, _vp, _vp, _vp, u32, u32, i32, C.c_uint, C.POINTER(_vp)]),
         "gbx_graph_create_rmat": (i32, [_vp, u32, u32, f64, f64, f64, u64, i32, u64, i32, C.c_uint,
                                         C.POINTER(_vp)]),
@@ -132,7 +133,7 @@ def exported_symbols():
     """Names declared in include/gbx.h that this binding expects."""
     return [
         "gbx_init", "gbx_free", "gbx_last_error", "gbx_status_string", "gbx_sync", "gbx_stream",
-        "gbx_version", "gbx_csr_build", "gbx_graph_create", "gbx_graph_create_rmat",
+        "gbx_version", "gbx_csr_build", "gbx_rank_sort", "gbx_graph_create", "gbx_graph_create_rmat",
         "gbx_rmat_edges", "gbx_graph_from_csr", "gbx_graph_info", "gbx_graph_export", "gbx_graph_free",
         "gbx_pr_params_default", "gbx_pr_pull", "gbx_pr_create", "gbx_pr_reset", "gbx_pr_launch",
         "gbx_pr_launch_timed", "gbx_pr_read", "gbx_pr_free", "gbx_pr_plan_info", "gbx_spmv_pull",
@@ -220,6 +221,14 @@ class Context:
                                         int(move_weights), _ptr(out["src"]), _ptr(out["dst"]), _ptr(out["w"]),
                                         _ptr(out["out_degree"]), _ptr(out["edges_idx"])))
         return out
+
+    def rank_sort(self, ranks, labels=None):
+        """radixSortEdgesByPageRank on the GPU: labels (default: the identity)
+        stably sorted by the ascending fixed-point key of ranks."""
+        ranks = np.ascontiguousarray(ranks, np.float32)
+        lab = np.arange(ranks.size, dtype=np.uint32) if labels is None else _u32(labels).copy()
+        self.check(self.L.gbx_rank_sort(self.h, _ptr(ranks), ranks.size, _ptr(lab)))
+        return lab
 
     def graph(self, src, dst, w=None, num_vertices=0, sort_mode=SORT_RADIX_SRC_DST, flags=0) -> "Graph":
         src, dst, w = _u32(src), _u32(dst), _f32(w)

@@ -23,7 +23,7 @@ INC = os.path.join(ROOT, "include")
 
 HIP_SOURCES = ["gbx_core.hip", "gbx_sort.hip", "gbx_rmat.hip", "gbx_pr.hip", "gbx_binned.hip", "gbx_ppr.hip",
                "gbx_batch.hip", "gbx_tune.hip", "gbx_ddpush.hip"]
-HOST_SOURCES = ["graphblox_hip.c"]
+HOST_SOURCES = ["graphblox_hip.c", "graphstats_hip.c"]
 
 LIB_HIP = os.path.join(PKG, "libgbx_hip.so")
 LIB_HOST = os.path.join(PKG, "libgraphblox_hip.so")        # WEIGHTED=0 layout

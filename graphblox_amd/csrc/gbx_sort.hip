@@ -390,7 +390,43 @@ static int check_ids(gbx_ctx *ctx, const DevBuf &ds, const DevBuf &dd, uint32_t 
 
 using namespace gbx;
 
+// FloatToFixed32SORT (fixedPoint.h:38): (u32)(r * 2^20 + 0.5f) in float,
+// -0.5f for negative r; the u32 conversion is x86-64's (64-bit signed
+// truncation keeping the low 32 bits; 2^63 and beyond, or NaN, give 0).
+__global__ void rank_sort_keys(const float *__restrict__ ranks, uint32_t n, uint32_t *__restrict__ key)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float r = ranks[i];
+        const float x = r * 1048576.0f + (r >= 0.0f ? 0.5f : -0.5f);
+        key[i] = (x < 9223372036854775808.0f && x >= -9223372036854775808.0f)
+                     ? (uint32_t)(unsigned long long)(long long)x
+                     : 0u;
+    }
+}
+
 extern "C" {
+
+int gbx_rank_sort(gbx_ctx *ctx, const float *ranks, uint32_t num_vertices, uint32_t *labels)
+{
+    if (!ctx) return set_error(nullptr, GBX_ERR_ARG, "gbx_rank_sort: ctx is NULL");
+    if (num_vertices && (!ranks || !labels)) return set_error(ctx, GBX_ERR_ARG, "gbx_rank_sort: NULL array");
+    if (num_vertices <= 1) return GBX_OK;
+    GBX_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t nb = (size_t)num_vertices * 4;
+    DevBuf r, key, lab;
+    GBX_TRY(dev_alloc(ctx, r, nb));
+    GBX_TRY(dev_alloc(ctx, key, nb));
+    GBX_TRY(dev_alloc(ctx, lab, nb));
+    GBX_HIP(ctx, hipMemcpyAsync(r.p, ranks, nb, hipMemcpyHostToDevice, ctx->stream));
+    GBX_HIP(ctx, hipMemcpyAsync(lab.p, labels, nb, hipMemcpyHostToDevice, ctx->stream));
+    rank_sort_keys<<<grid_for(num_vertices, 256, 16384), 256, 0, ctx->stream>>>(r.as<float>(), num_vertices,
+                                                                                 key.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    GBX_TRY(radix_sort_pairs(ctx, key.as<uint32_t>(), lab.as<uint32_t>(), nullptr, num_vertices, 32));
+    GBX_HIP(ctx, hipMemcpyAsync(labels, lab.p, nb, hipMemcpyDeviceToHost, ctx->stream));
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GBX_OK;
+}
 
 int gbx_csr_build(gbx_ctx *ctx, const uint32_t *src, const uint32_t *dst, const float *w,
                   uint32_t num_edges, uint32_t num_vertices, int sort_mode, int move_weights,

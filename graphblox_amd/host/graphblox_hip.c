@@ -352,6 +352,197 @@ void graphCSRFree(struct GraphCSR *graph)
     free(graph);
 }
 
+/* ------------------------------------------------------------ CSR files */
+
+static char *path_with(const char *fname, const char *ext)
+{
+    const size_t n = strlen(fname), m = strlen(ext);
+    char *p = host_malloc(n + m + 1);
+    memcpy(p, fname, n);
+    memcpy(p + n, ext, m + 1);
+    return p;
+}
+
+static void write_all(FILE *f, const void *buf, size_t words, const char *path)
+{
+    if (words && fwrite(buf, 4, words, f) != words) die_io("write", path);
+}
+
+static void read_all(FILE *f, void *buf, size_t words, const char *path)
+{
+    if (words && fread(buf, 4, words, f) != words) die_io("read", path);
+}
+
+enum { CSR_IO_CHUNK = 1 << 16 }; /* records staged per fwrite / fread */
+
+/* writeToBinFileGraphCSR, graphCSR.c:457-527: "<fname>.csr" holds num_edges
+ * and num_vertices (u32) [and max_weight (f32) with WEIGHTED]; then for every
+ * vertex the forward (out_degree, in_degree, edges_idx) followed by the
+ * inverse triple when the graph has an inverse; then for every edge the
+ * forward (src, dest [, weight]) followed by the inverse record. */
+void writeToBinFileGraphCSR(const char *fname, struct GraphCSR *graph)
+{
+    char *path = path_with(fname, ".csr");
+    FILE *f = fopen(path, "wb");
+    if (!f) die_io("open", path);
+    const uint32_t N = graph->num_vertices, E = graph->num_edges;
+    const int inv = graph->inverse_vertices != NULL;
+    write_all(f, &graph->num_edges, 1, path);
+    write_all(f, &graph->num_vertices, 1, path);
+#if GBX_WEIGHTED
+    write_all(f, &graph->max_weight, 1, path);
+#endif
+    uint32_t *rec = host_malloc((size_t)CSR_IO_CHUNK * 6 * 4);
+    for (uint32_t b = 0; b < N; b += CSR_IO_CHUNK) {
+        const uint32_t n = N - b < CSR_IO_CHUNK ? N - b : CSR_IO_CHUNK;
+        size_t k = 0;
+        for (uint32_t i = b; i < b + n; i++) {
+            const struct Vertex *v = graph->vertices;
+            rec[k++] = v->out_degree[i];
+            rec[k++] = v->in_degree[i];
+            rec[k++] = v->edges_idx[i];
+            if (inv) {
+                const struct Vertex *w = graph->inverse_vertices;
+                rec[k++] = w->out_degree[i];
+                rec[k++] = w->in_degree[i];
+                rec[k++] = w->edges_idx[i];
+            }
+        }
+        write_all(f, rec, k, path);
+    }
+    for (uint32_t b = 0; b < E; b += CSR_IO_CHUNK) {
+        const uint32_t n = E - b < CSR_IO_CHUNK ? E - b : CSR_IO_CHUNK;
+        size_t k = 0;
+        for (uint32_t i = b; i < b + n; i++) {
+            for (int side = 0; side < 1 + inv; side++) {
+                const struct EdgeList *e = side ? graph->inverse_sorted_edges_array : graph->sorted_edges_array;
+                rec[k++] = e->edges_array_src[i];
+                rec[k++] = e->edges_array_dest[i];
+#if GBX_WEIGHTED
+                memcpy(&rec[k++], &e->edges_array_weight[i], 4);
+#endif
+            }
+        }
+        write_all(f, rec, k, path);
+    }
+    free(rec);
+    if (fclose(f) != 0) die_io("close", path);
+    free(path);
+}
+
+/* readFromBinFileGraphCSR, graphCSR.c:638-744: the file written above (with
+ * the inverse records: this layout is DIRECTED=1), as a host GraphCSR; its
+ * device copy is made when an algorithm first runs on it. */
+struct GraphCSR *readFromBinFileGraphCSR(const char *fname)
+{
+    FILE *f = fopen(fname, "rb");
+    if (!f) die_io("open", fname);
+    uint32_t hdr[2];
+    read_all(f, hdr, 2, fname);
+    const uint32_t E = hdr[0], N = hdr[1];
+    struct GraphCSR *csr = host_malloc(sizeof(*csr));
+    memset(csr, 0, sizeof(*csr));
+    csr->num_edges = E;
+    csr->num_vertices = N;
+    csr->avg_degree = N ? E / N : 0;
+    float max_weight = 1.0f;
+#if GBX_WEIGHTED
+    read_all(f, &max_weight, 1, fname);
+    csr->max_weight = max_weight;
+#endif
+    csr->vertices = new_vertex_array(N);
+    csr->inverse_vertices = new_vertex_array(N);
+    for (int side = 0; side < 2; side++) {
+        struct EdgeList *el = new_edge_list(E);
+        el->num_vertices = N;
+        el->avg_degree = csr->avg_degree;
+        el->max_weight = max_weight;
+        label_arrays(el, N);
+        if (side) csr->inverse_sorted_edges_array = el;
+        else csr->sorted_edges_array = el;
+    }
+    const size_t per_edge = GBX_WEIGHTED ? 6 : 4;
+    uint32_t *rec = host_malloc((size_t)CSR_IO_CHUNK * 6 * 4);
+    for (uint32_t b = 0; b < N; b += CSR_IO_CHUNK) {
+        const uint32_t n = N - b < CSR_IO_CHUNK ? N - b : CSR_IO_CHUNK;
+        read_all(f, rec, (size_t)n * 6, fname);
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t *r = rec + (size_t)i * 6;
+            csr->vertices->out_degree[b + i] = r[0];
+            csr->vertices->in_degree[b + i] = r[1];
+            csr->vertices->edges_idx[b + i] = r[2];
+            csr->inverse_vertices->out_degree[b + i] = r[3];
+            csr->inverse_vertices->in_degree[b + i] = r[4];
+            csr->inverse_vertices->edges_idx[b + i] = r[5];
+        }
+    }
+    for (uint32_t b = 0; b < E; b += CSR_IO_CHUNK) {
+        const uint32_t n = E - b < CSR_IO_CHUNK ? E - b : CSR_IO_CHUNK;
+        read_all(f, rec, (size_t)n * per_edge, fname);
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t *r = rec + (size_t)i * per_edge;
+            for (int side = 0; side < 2; side++) {
+                struct EdgeList *e = side ? csr->inverse_sorted_edges_array : csr->sorted_edges_array;
+                const uint32_t *q = r + side * (per_edge / 2);
+                e->edges_array_src[b + i] = q[0];
+                e->edges_array_dest[b + i] = q[1];
+#if GBX_WEIGHTED
+                memcpy(&e->edges_array_weight[b + i], &q[2], 4);
+#endif
+            }
+        }
+    }
+    free(rec);
+    fclose(f);
+    printf(" -----------------------------------------------------\n");
+    printf("| %-51s | \n", "GraphCSR (read from .csr)");
+    printf(" -----------------------------------------------------\n");
+    printf("| %-30s | %-18u | \n", "Number of Vertices (V)", N);
+    printf("| %-30s | %-18u | \n", "Number of Edges (E)", E);
+    printf(" -----------------------------------------------------\n");
+    return csr;
+}
+
+static FILE *open_text(const char *fname, const char *ext, char **path)
+{
+    *path = path_with(fname, ext);
+    FILE *f = fopen(*path, "w");
+    if (!f) die_io("open", *path);
+    return f;
+}
+
+/* writetoTextFilesGraphCSR, graphCSR.c:529-635: the forward CSR as text, one
+ * value per line: <fname>.out_degree (preceded by num_vertices), .in_degree,
+ * .edges_idx, .edges_array_src (preceded by num_edges), .edges_array_dest
+ * [and .edges_array_weight, "%f", with WEIGHTED]. */
+void writetoTextFilesGraphCSR(const char *fname, struct GraphCSR *graph)
+{
+    static const char *ext[] = {".out_degree", ".in_degree", ".edges_idx", ".edges_array_src",
+                                ".edges_array_dest", ".edges_array_weight"};
+    const int nfiles = GBX_WEIGHTED ? 6 : 5;
+    FILE *f[6] = {NULL};
+    char *path[6] = {NULL};
+    for (int k = 0; k < nfiles; k++) f[k] = open_text(fname, ext[k], &path[k]);
+    const struct Vertex *v = graph->vertices;
+    const struct EdgeList *e = graph->sorted_edges_array;
+    fprintf(f[0], "%u\n", graph->num_vertices);
+    for (uint32_t i = 0; i < graph->num_vertices; i++) {
+        fprintf(f[0], "%u\n", v->out_degree[i]);
+        fprintf(f[1], "%u\n", v->in_degree[i]);
+        fprintf(f[2], "%u\n", v->edges_idx[i]);
+    }
+    fprintf(f[3], "%u\n", graph->num_edges);
+    for (uint32_t i = 0; i < graph->num_edges; i++) {
+        fprintf(f[3], "%u\n", e->edges_array_src[i]);
+        fprintf(f[4], "%u\n", e->edges_array_dest[i]);
+        if (GBX_WEIGHTED) fprintf(f[5], "%f\n", e->edges_array_weight[i]);
+    }
+    for (int k = 0; k < nfiles; k++) {
+        if (fclose(f[k]) != 0) die_io("close", path[k]);
+        free(path[k]);
+    }
+}
+
 /* ------------------------------------------------------------ PageRank */
 
 /* newPageRankStatsGraphCSR / freePageRankStats, pageRank.c:47-88 */
@@ -534,6 +725,38 @@ struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct Graph
     case 8: return pageRankDataDrivenPullPushGraphCSR(arguments, graph);
     default: return pageRankPullGraphCSR(arguments, graph);
     }
+}
+
+/* radixSortEdgesByPageRank, reorder.c:165-221: labels stably sorted by the
+ * ascending fixed-point key of pageRanks, on the GPU; in place, returned. */
+uint32_t *radixSortEdgesByPageRank(float *pageRanks, uint32_t *labels, uint32_t num_vertices)
+{
+    gbx_ctx *ctx = backend();
+    if (!ctx) exit(EXIT_FAILURE);
+    if (gbx_rank_sort(ctx, pageRanks, num_vertices, labels) != GBX_OK) {
+        fprintf(stderr, "graphblox_hip: rank sort failed: %s\n", gbx_last_error(ctx));
+        exit(EXIT_FAILURE);
+    }
+    return labels;
+}
+
+/* runPageRankAlgorithm, graphRun.c:933-975: the CSR data structures (0, 4 and
+ * unknown codes) run pageRankGraphCSR; segments (1, 5) and 6 are not
+ * implemented, as in the reference.  The ranking realRanks is then sorted by
+ * rank. */
+struct PageRankStats *runPageRankAlgorithm(struct Arguments *arguments, void *graph)
+{
+    struct PageRankStats *stats = NULL;
+    switch (arguments->datastructure) {
+    case 1: case 5: case 6:
+        printf(" -----------------------------------------------------\n");
+        printf("| %-51s | \n", "PageRank NOT YET IMPLEMENTED");
+        printf(" -----------------------------------------------------\n");
+        break;
+    default: stats = pageRankGraphCSR(arguments, (struct GraphCSR *)graph); break;
+    }
+    if (stats) stats->realRanks = radixSortEdgesByPageRank(stats->pageRanks, stats->realRanks, stats->num_vertices);
+    return stats;
 }
 
 /* ------------------------------------------------------------ SpMV */

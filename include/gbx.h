@@ -81,6 +81,13 @@ int gbx_csr_build(gbx_ctx *ctx, const uint32_t *src, const uint32_t *dst,
                   uint32_t *sorted_dst, float *sorted_w, uint32_t *out_degree,
                   uint32_t *edges_idx);
 
+/* radixSortEdgesByPageRank (reorder.c:165-221): labels[0..n) reordered by a
+ * stable ascending sort on the fixed-point key FloatToFixed32SORT(rank of the
+ * label's position) = (u32)(ranks[i] * 2^20 + 0.5f) (fixedPoint.h:38), on the
+ * GPU.  ranks[i] is the key of the label found at labels[i] on entry (the
+ * reference passes the identity realRanks).  Host arrays, in place. */
+int gbx_rank_sort(gbx_ctx *ctx, const float *ranks, uint32_t num_vertices, uint32_t *labels);
+
 /* ---------------------------------------------------------------- graphs */
 #define GBX_GRAPH_DEDUPE 1u     /* -k: (src,dst) sort + removeDulpicatesSelfLoopEdges */
 #define GBX_GRAPH_WEIGHTED 2u   /* WEIGHTED=1: weights move through the sorts */

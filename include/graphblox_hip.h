@@ -25,6 +25,7 @@
 #define GRAPHBLOX_HIP_H
 
 #include <stdint.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -175,6 +176,35 @@ struct PageRankStats *pageRankPushQuantGraphCSR(struct Arguments *arguments, str
 struct PageRankStats *pageRankDataDrivenPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct PageRankStats *pageRankDataDrivenPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct PageRankStats *pageRankDataDrivenPullPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+
+/* graphCSR.c:457-744: the .csr binary file (writer and reader, this
+ * library's DIRECTED=1 layout) and the text-fixture writer */
+void writeToBinFileGraphCSR(const char *fname, struct GraphCSR *graph);
+struct GraphCSR *readFromBinFileGraphCSR(const char *fname);
+void writetoTextFilesGraphCSR(const char *fname, struct GraphCSR *graph);
+
+/* reorder.c:165-221: labels sorted by rank (GPU), and graphRun.c:933-975 */
+uint32_t *radixSortEdgesByPageRank(float *pageRanks, uint32_t *labels, uint32_t num_vertices);
+struct PageRankStats *runPageRankAlgorithm(struct Arguments *arguments, void *graph);
+
+/* graphStats.h:32-41, graphStats.c:589-1010: top-k similarity of two
+ * PageRank runs' rankings */
+struct PageRankCorrelationStats {
+    uint32_t levenshtein_distance;
+    float float_Kendall;
+    float real_Kendall;
+    uint32_t intersection;
+    uint32_t mismatch;
+    double avg_error_float;
+    double avg_error_relative;
+};
+struct PageRankCorrelationStats collectStatsPageRank_topK(struct PageRankStats *ref_stats, struct PageRankStats *stats,
+                                                          uint32_t *ref_rankedVertices_total,
+                                                          uint32_t *ref_rankedVertices_inverse,
+                                                          uint32_t *rankedVertices_inverse, uint32_t topk,
+                                                          uint32_t num_vertices, FILE *fptr, uint32_t verbose);
+void collectStatsPageRank(struct Arguments *arguments, struct PageRankStats *ref_stats, struct PageRankStats *stats,
+                          uint32_t trial);
 
 /* SPMV.c:27-215 */
 struct SPMVStats *newSPMVStatsGraphCSR(struct GraphCSR *graph);
