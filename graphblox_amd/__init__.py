@@ -59,6 +59,10 @@ class Tuning(C.Structure):
         return d
 
 
+class Relabel(C.Structure):
+    _fields_ = [("lmode", C.c_uint32 * 3), ("file_labels", C.c_void_p), ("num_file_labels", C.c_uint32)]
+
+
 class PrReport(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("executed", C.c_uint32), ("active", C.c_uint32),
                 ("converged", C.c_uint32), ("error_total", C.c_double), ("rank_sum", C.c_double),
@@ -92,6 +96,8 @@ def lib():
         "gbx_csr_build": (i32, [_vp, _vp, _vp, _vp, u32, u32, i32, i32, _vp, _vp, _vp, _vp, _vp]),
         "gbx_rank_sort": (i32, [_vp, _vp, u32, _vp]),
         "gbx_graph_create": (i32, [_vp, _vp, _vp, _vp, u32, u32, i32, C.c_uint, C.POINTER(_vp)]),
+        "gbx_graph_create_relabel": (i32, [_vp, _vp, _vp, _vp, u32, u32, i32, C.c_uint, C.POINTER(Relabel), _vp,
+                                           C.POINTER(_vp)]),
         "gbx_graph_create_rmat": (i32, [_vp, u32, u32, f64, f64, f64, u64, i32, u64, i32, C.c_uint,
                                         C.POINTER(_vp)]),
         "gbx_rmat_edges": (i32, [_vp, u32, u32, f64, f64, f64, u64, i32, i32, u64, _vp, _vp, _vp,
@@ -133,7 +139,8 @@ def exported_symbols():
     """Names declared in include/gbx.h that this binding expects."""
     return [
         "gbx_init", "gbx_free", "gbx_last_error", "gbx_status_string", "gbx_sync", "gbx_stream",
-        "gbx_version", "gbx_csr_build", "gbx_rank_sort", "gbx_graph_create", "gbx_graph_create_rmat",
+        "gbx_version", "gbx_csr_build", "gbx_rank_sort", "gbx_graph_create", "gbx_graph_create_relabel",
+        "gbx_graph_create_rmat",
         "gbx_rmat_edges", "gbx_graph_from_csr", "gbx_graph_info", "gbx_graph_export", "gbx_graph_free",
         "gbx_pr_params_default", "gbx_pr_pull", "gbx_pr_create", "gbx_pr_reset", "gbx_pr_launch",
         "gbx_pr_launch_timed", "gbx_pr_read", "gbx_pr_free", "gbx_pr_plan_info", "gbx_spmv_pull",
@@ -230,12 +237,28 @@ class Context:
         self.check(self.L.gbx_rank_sort(self.h, _ptr(ranks), ranks.size, _ptr(lab)))
         return lab
 
-    def graph(self, src, dst, w=None, num_vertices=0, sort_mode=SORT_RADIX_SRC_DST, flags=0) -> "Graph":
+    def graph(self, src, dst, w=None, num_vertices=0, sort_mode=SORT_RADIX_SRC_DST, flags=0, lmode=(0, 0, 0),
+              file_labels=None) -> "Graph":
+        """A device graph from an edge list; lmode: the relabel levels of
+        graphCSRPreProcessingStep (graph.labels: new id of each vertex)."""
         src, dst, w = _u32(src), _u32(dst), _f32(w)
         h = _vp()
-        self.check(self.L.gbx_graph_create(self.h, _ptr(src), _ptr(dst), _ptr(w), src.size, num_vertices,
-                                           sort_mode, flags, C.byref(h)))
-        return Graph(self, h)
+        if not any(lmode) and file_labels is None:
+            self.check(self.L.gbx_graph_create(self.h, _ptr(src), _ptr(dst), _ptr(w), src.size, num_vertices,
+                                               sort_mode, flags, C.byref(h)))
+            return Graph(self, h)
+        rl = Relabel()
+        rl.lmode[:] = list(lmode) + [0] * (3 - len(lmode))
+        fl = _u32(file_labels) if file_labels is not None else None
+        rl.file_labels = _ptr(fl)
+        rl.num_file_labels = fl.size if fl is not None else 0
+        n = num_vertices or (int(max(src.max(), dst.max())) + 1 if src.size else 0)
+        labels = np.zeros(max(n, 1), np.uint32)
+        self.check(self.L.gbx_graph_create_relabel(self.h, _ptr(src), _ptr(dst), _ptr(w), src.size, num_vertices,
+                                                   sort_mode, flags, C.byref(rl), _ptr(labels), C.byref(h)))
+        g = Graph(self, h)
+        g.labels = labels[:g.N]
+        return g
 
     def graph_from_csr(self, fwd: dict, inv: dict) -> "Graph":
         """Upload CSR arrays in the reference layout (keys out_degree,

@@ -119,7 +119,16 @@ int build_csr_device(gbx_ctx *ctx, uint32_t *src, uint32_t *dst, float *w, uint3
                      uint32_t N, int sort_mode, bool move_weights, uint32_t *out_degree,
                      uint32_t *edges_idx);
 int graph_from_device_edges(gbx_ctx *ctx, DevBuf &src, DevBuf &dst, DevBuf &w, uint32_t E,
-                            uint32_t N, int sort_mode, unsigned flags, gbx_graph **out);
+                            uint32_t N, int sort_mode, unsigned flags, gbx_graph **out,
+                            const gbx_relabel *relabel = nullptr, uint32_t *labels_out = nullptr);
+// ---- relabel levels (gbx_relabel.hip)
+bool relabel_requested(const gbx_relabel *rl);
+int relabel_validate(gbx_ctx *ctx, const gbx_relabel *rl);
+// Renames the (sorted) edges level by level, sorting again after each; total
+// [N] receives the composed new id of every original vertex.
+int relabel_edges(gbx_ctx *ctx, DevBuf &src, DevBuf &dst, DevBuf &w, uint32_t E, uint32_t N, uint32_t avg,
+                  int sort_mode, bool weighted, uint32_t *out_degree, uint32_t *edges_idx,
+                  const gbx_relabel *rl, DevBuf &total);
 uint32_t bits_for(uint32_t max_value);
 
 // ---- tuning (gbx_core.hip)

@@ -258,7 +258,8 @@ __global__ void symmetric_append(uint32_t *src, uint32_t *dst, float *w, uint32_
 }
 
 int graph_from_device_edges(gbx_ctx *ctx, DevBuf &src, DevBuf &dst, DevBuf &w, uint32_t E,
-                            uint32_t N, int sort_mode, unsigned flags, gbx_graph **out)
+                            uint32_t N, int sort_mode, unsigned flags, gbx_graph **out,
+                            const gbx_relabel *relabel, uint32_t *labels_out)
 {
     const bool weighted = (flags & GBX_GRAPH_WEIGHTED) != 0;
     if (flags & GBX_GRAPH_DEDUPE) sort_mode = GBX_SORT_RADIX_SRC_DST;  // GraphBlox.c:358-359
@@ -317,6 +318,20 @@ int graph_from_device_edges(gbx_ctx *ctx, DevBuf &src, DevBuf &dst, DevBuf &w, u
         vertex_degrees<<<grid_for(N, 256, 16384), 256, 0, ctx->stream>>>(
             N, g->fwd.out_degree.as<uint32_t>(), g->fwd.edges_idx.as<uint32_t>());
         if (hipGetLastError() != hipSuccess) return fail(set_error(ctx, GBX_ERR_HIP, "vertex_degrees"));
+    }
+    if (relabel_requested(relabel)) {
+        // EdgeList.avg_degree: E / N as read, recomputed by the dedupe (edgeList.c:305,681)
+        const uint32_t avg = N ? E / N : 0;
+        DevBuf total;
+        if ((rc = relabel_edges(ctx, src, dst, w, E, N, avg, sort_mode, weighted, g->fwd.out_degree.as<uint32_t>(),
+                                g->fwd.edges_idx.as<uint32_t>(), relabel, total)))
+            return fail(rc);
+        if (labels_out &&
+            (hipMemcpyAsync(labels_out, total.p, (size_t)N * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+             hipStreamSynchronize(ctx->stream) != hipSuccess))
+            return fail(set_error(ctx, GBX_ERR_HIP, "relabel: label copy failed"));
+    } else if (labels_out) {
+        for (uint32_t v = 0; v < N; v++) labels_out[v] = v;
     }
     g->E = E;
     g->fwd.src = std::move(src);
@@ -462,7 +477,16 @@ int gbx_graph_create(gbx_ctx *ctx, const uint32_t *src, const uint32_t *dst, con
                      uint32_t num_edges, uint32_t num_vertices, int sort_mode, unsigned flags,
                      gbx_graph **out)
 {
+    return gbx_graph_create_relabel(ctx, src, dst, w, num_edges, num_vertices, sort_mode, flags, nullptr, nullptr,
+                                    out);
+}
+
+int gbx_graph_create_relabel(gbx_ctx *ctx, const uint32_t *src, const uint32_t *dst, const float *w,
+                             uint32_t num_edges, uint32_t num_vertices, int sort_mode, unsigned flags,
+                             const gbx_relabel *relabel, uint32_t *labels_out, gbx_graph **out)
+{
     if (!ctx || !out) return set_error(ctx, GBX_ERR_ARG, "gbx_graph_create: NULL argument");
+    GBX_TRY(relabel_validate(ctx, relabel));
     *out = nullptr;
     if (num_edges && (!src || !dst))
         return set_error(ctx, GBX_ERR_ARG, "gbx_graph_create: src/dst is NULL");
@@ -482,7 +506,7 @@ int gbx_graph_create(gbx_ctx *ctx, const uint32_t *src, const uint32_t *dst, con
     }
     uint32_t N = num_vertices;
     GBX_TRY(check_ids(ctx, ds, dd, E, &N));
-    return graph_from_device_edges(ctx, ds, dd, dw, E, N, sort_mode, flags, out);
+    return graph_from_device_edges(ctx, ds, dd, dw, E, N, sort_mode, flags, out, relabel, labels_out);
 }
 
 int gbx_graph_from_csr(gbx_ctx *ctx, uint32_t num_vertices, uint32_t num_edges,

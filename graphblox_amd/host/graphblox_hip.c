@@ -274,16 +274,44 @@ static void free_vertex_array(struct Vertex *v)
     free(v);
 }
 
+/* relabelEdgeListFromFile, reorder.c:1662-1716: the new id of each vertex,
+ * one "%u" per line, num_vertices of them (a shorter file is an error here;
+ * the reference leaves the rest of its array unset). */
+static uint32_t *read_label_file(const char *fname, uint32_t num_vertices)
+{
+    if (!fname) die_io("open", "(no label file)");
+    FILE *f = fopen(fname, "r");
+    if (!f) die_io("open", fname);
+    uint32_t *labels = host_malloc(((size_t)num_vertices + 1) * 4);
+    for (uint32_t x = 0; x < num_vertices; x++)
+        if (fscanf(f, "%u", &labels[x]) != 1) die_io("too few labels in", fname);
+    fclose(f);
+    return labels;
+}
+
 /* graphCSRPreProcessingStep, graphCSR.c:243-327 (DIRECTED=1): the reader
- * runs on the host; both count sorts, the -k dedupe and the vertex mapping
- * run on the GPU. */
+ * runs on the host; both count sorts, the -k dedupe, the relabel levels
+ * (lmode, lmode_l2, lmode_l3) and the vertex mapping run on the GPU. */
 struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
 {
     gbx_ctx *ctx = backend();
     if (!ctx) exit(EXIT_FAILURE);
-    if (arguments->lmode || arguments->lmode_l2 || arguments->lmode_l3 || arguments->mmode)
-        fprintf(stderr, "graphblox_hip: relabelling/mask modes are not provided by this backend; ignored\n");
+    if (arguments->mmode)
+        fprintf(stderr, "graphblox_hip: mask modes (mmode) are not provided by this backend; ignored\n");
     struct EdgeList *in = readEdgeListsbin(arguments->fnameb, 0, arguments->symmetric, arguments->weighted);
+    /* the relabel levels (graphCSR.c:268-290) run on the GPU inside the build */
+    gbx_relabel rl;
+    memset(&rl, 0, sizeof(rl));
+    rl.lmode[0] = arguments->lmode;
+    rl.lmode[1] = arguments->lmode_l2;
+    rl.lmode[2] = arguments->lmode_l3;
+    const int relabel = rl.lmode[0] || rl.lmode[1] || rl.lmode[2];
+    uint32_t *file_labels = NULL;
+    if (rl.lmode[0] == 11 || rl.lmode[1] == 11 || rl.lmode[2] == 11)
+        file_labels = read_label_file(arguments->fnamel, in->num_vertices);
+    rl.file_labels = file_labels;
+    rl.num_file_labels = file_labels ? in->num_vertices : 0;
+    uint32_t *labels = relabel ? host_malloc(((size_t)in->num_vertices + 1) * 4) : NULL;
     unsigned flags = 0;
     uint32_t sort = arguments->sort;
     if (arguments->dflag) {
@@ -293,8 +321,10 @@ struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
     if (GBX_WEIGHTED) flags |= GBX_GRAPH_WEIGHTED;
     if (sort > 3) sort = 0; /* sortRun.c:59-64: unknown modes fall back to radix by source */
     gbx_graph *g = NULL;
-    int rc = gbx_graph_create(ctx, in->edges_array_src, in->edges_array_dest, in->edges_array_weight,
-                              in->num_edges, in->num_vertices, (int)sort, flags, &g);
+    int rc = gbx_graph_create_relabel(ctx, in->edges_array_src, in->edges_array_dest, in->edges_array_weight,
+                                      in->num_edges, in->num_vertices, (int)sort, flags, relabel ? &rl : NULL,
+                                      labels, &g);
+    free(file_labels);
     if (rc != GBX_OK) {
         fprintf(stderr, "graphblox_hip: CSR build failed: %s\n", gbx_last_error(ctx));
         exit(EXIT_FAILURE);
@@ -316,6 +346,11 @@ struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
         el->avg_degree = E / N;
         el->max_weight = in->max_weight;
         label_arrays(el, N);
+        if (labels) /* reorder.c:376-380: old id -> new id and back */
+            for (uint32_t x = 0; x < N; x++) {
+                el->label_array[x] = labels[x];
+                el->inverse_label_array[labels[x]] = x;
+            }
         rc = gbx_graph_export(g, inv, v->out_degree, v->in_degree, v->edges_idx, el->edges_array_src,
                               el->edges_array_dest, el->edges_array_weight);
         if (rc != GBX_OK) {
@@ -331,6 +366,7 @@ struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
         }
     }
     freeEdgeList(in);
+    free(labels);
     register_graph(csr, g);
     printf(" -----------------------------------------------------\n");
     printf("| %-51s | \n", "GraphCSR (HIP backend)");

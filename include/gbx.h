@@ -101,6 +101,27 @@ int gbx_graph_create(gbx_ctx *ctx, const uint32_t *src, const uint32_t *dst,
                      const float *w, uint32_t num_edges, uint32_t num_vertices,
                      int sort_mode, unsigned flags, gbx_graph **out);
 
+/* The relabel levels of graphCSRPreProcessingStep (arguments lmode, lmode_l2,
+ * lmode_l3; graphCSR.c:268-290): after the sort (and -k dedupe) each nonzero
+ * level renames every vertex as reorderGraphProcess (reorder.c:1556-1616)
+ * orders them, on the GPU, and sorts the edges again: 1 out-degree, 2
+ * in-degree, 3 in+out degree, 10 random degree (descending degree); 4/5 DBG
+ * and 12/13 Corder (out/in degree buckets); 6/7 HUBSort; 8/9 HUBCluster; 11
+ * the labels in file_labels (new id of each vertex, num_file_labels >= N,
+ * each below N); other codes order by out-degree.  labels_out [N]
+ * (nullable) receives the composed new id of every original vertex
+ * (EdgeList.label_array).  With all levels 0 this is gbx_graph_create. */
+typedef struct gbx_relabel {
+    uint32_t lmode[3];
+    const uint32_t *file_labels;
+    uint32_t num_file_labels;
+} gbx_relabel;
+
+int gbx_graph_create_relabel(gbx_ctx *ctx, const uint32_t *src, const uint32_t *dst,
+                             const float *w, uint32_t num_edges, uint32_t num_vertices,
+                             int sort_mode, unsigned flags, const gbx_relabel *relabel,
+                             uint32_t *labels_out, gbx_graph **out);
+
 /* Seeded R-MAT graph generated on the GPU: (edge_factor << scale) edges,
  * quadrant probabilities a, b, c, d = 1-a-b-c, ids relabelled by a seeded
  * bijection of [0, 2^scale), self loops and duplicates removed (first

@@ -95,6 +95,10 @@ def ref(directed: int = 1, weighted: int = 0):
         L = C.CDLL(path)
         L.refd_build.argtypes = [C.c_char_p, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int]
         L.refd_build.restype = C.c_void_p
+        L.refd_build_relabel.argtypes = [C.c_char_p, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.c_uint32, C.c_uint32, C.c_uint32, C.c_char_p]
+        L.refd_build_relabel.restype = C.c_void_p
+        L.refd_labels.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.refd_num_vertices.argtypes = [C.c_void_p]
         L.refd_num_vertices.restype = C.c_uint32
         L.refd_num_edges.argtypes = [C.c_void_p]
@@ -313,6 +317,85 @@ def dd_push_limit(N, outdeg, row_ptr, col, damp=0.85, tol=1e-15, max_iters=5000)
         if done:
             break
     return ((1.0 - damp) + y) / N
+
+
+def mt19937_uint32(seed, count):
+    """MT19937 (Matsumoto & Nishimura 1998) with the 1999 reference code's
+    seeding (each state word from two steps of x <- 69069 x + 1, upper 16 bits
+    each), as mt19937.c:46-57 initialises it; `count` tempered 32-bit outputs."""
+    n, m = 624, 397
+    mt = [0] * n
+    x = seed & 0xFFFFFFFF
+    for i in range(n):
+        hi = x & 0xFFFF0000
+        x = (69069 * x + 1) & 0xFFFFFFFF
+        mt[i] = hi | ((x & 0xFFFF0000) >> 16)
+        x = (69069 * x + 1) & 0xFFFFFFFF
+    out, idx = [], n
+    for _ in range(count):
+        if idx >= n:
+            for k in range(n):
+                y = (mt[k] & 0x80000000) | (mt[(k + 1) % n] & 0x7FFFFFFF)
+                mt[k] = mt[(k + m) % n] ^ (y >> 1) ^ (0x9908B0DF if y & 1 else 0)
+            idx = 0
+        y = mt[idx]
+        idx += 1
+        y ^= y >> 11
+        y ^= (y << 7) & 0x9D2C5680
+        y ^= (y << 15) & 0xEFC60000
+        y ^= y >> 18
+        out.append(y & 0xFFFFFFFF)
+    return out
+
+
+def relabel_labels(src, dst, N, avg_degree, lmode, file_labels=None):
+    """New id of every vertex for one relabel level of reorderGraphProcess
+    (reorder.c:1556-1616) on an edge list, one OpenMP thread:
+      degrees (reorder.c:1483-1552): out (lmode 1, 4, 6, 8, 12, 13 and any
+        other), in (2, 5, 7, 9), in + out (3), MT19937(27491095) % N (10);
+      degree order (1, 2, 3, 10 and other codes, reorder.c:296-385): by
+        degree descending, equal degrees by id descending;
+      DBG and Corder (4, 5, 12, 13; Corder runs the DBG list function,
+        reorder.c:574-624): 11 buckets, thresholds t0 = avg_degree / 2 (1 when
+        avg_degree <= 1) doubling, the last unbounded; bucket = first with
+        degree <= threshold; buckets from high to low, ids ascending within;
+      HUBSort (6, 7): hubs (degree > t0) by degree descending, equal degrees by
+        id descending, then the others by id ascending (reorder.c:814-982);
+      HUBCluster (8, 9): hubs (degree > avg_degree, reorder.c:1003) then the
+        others, ids ascending within each;
+      file (11): the given labels."""
+    src = np.asarray(src, np.int64)
+    dst = np.asarray(dst, np.int64)
+    if lmode == 11:
+        return np.asarray(file_labels, np.uint32)[:N].copy()
+    if lmode == 10:
+        deg = np.array(mt19937_uint32(27491095, N), np.uint64) % np.uint64(N)
+    else:
+        if lmode in (2, 5, 7, 9):
+            deg = np.bincount(dst, minlength=N)
+        elif lmode == 3:
+            deg = np.bincount(dst, minlength=N) + np.bincount(src, minlength=N)
+        else:
+            deg = np.bincount(src, minlength=N)
+    deg = np.asarray(deg, np.int64)
+    ids = np.arange(N, dtype=np.int64)
+    t0 = 1 if avg_degree <= 1 else avg_degree // 2
+    if lmode in (4, 5, 12, 13, 8, 9):
+        nb = 2 if lmode in (8, 9) else 11
+        th = [avg_degree] if lmode in (8, 9) else [t0 * (1 << i) for i in range(nb - 1)]
+        th = th + [0xFFFFFFFF]
+        bucket = np.searchsorted(np.array(th, np.int64), deg, side="left")
+        order = ids[np.lexsort((ids, -bucket))]
+    elif lmode in (6, 7):
+        hub = deg > t0
+        h = ids[hub]
+        hubs = h[np.lexsort((-h, -deg[hub]))]
+        order = np.concatenate([hubs, ids[~hub]])
+    else:
+        order = ids[np.lexsort((-ids, -deg))]
+    labels = np.empty(N, np.uint32)
+    labels[order] = np.arange(N, dtype=np.uint32)
+    return labels
 
 
 def pr_pull_quant(N, outdeg, row_ptr, col, bits, iterations=20, epsilon=0.0):

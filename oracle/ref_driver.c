@@ -5,7 +5,8 @@
  *
  * It exposes a flat C interface (plain pointers and sizes) so the Python
  * tests and the golden-vector generator can drive:
- *   - graphCSRPreProcessingStep  (01_host/src/structures/graphCSR.c:243-327)
+ *   - graphCSRPreProcessingStep  (01_host/src/structures/graphCSR.c:243-327),
+ *     also with the relabel levels  (01_host/src/preprocess/reorder.c:1556-1716)
  *   - pageRankPullGraphCSR       (01_host/src/algorithms/openmp/pageRank.c:245-368)
  *   - SPMVPullGraphCSR           (01_host/src/algorithms/openmp/SPMV.c:117-215)
  * and to run pageRankPullGraphCSR on CSR arrays supplied by the caller
@@ -72,6 +73,44 @@ void *refd_build(const char *fname, uint32_t sort, int dflag, int symmetric,
     quiet_end(s);
     free(a);
     return g;
+}
+
+/* graphCSRPreProcessingStep with the relabel levels (lmode, lmode_l2,
+ * lmode_l3; graphCSR.c:268-290, reorderGraphProcess reorder.c:1556-1616);
+ * fnamel: the label file of lmode 11 (may be NULL). */
+void *refd_build_relabel(const char *fname, uint32_t sort, int dflag, int symmetric, int weighted,
+                         int nthreads, uint32_t lmode, uint32_t lmode_l2, uint32_t lmode_l3,
+                         const char *fnamel)
+{
+    struct Arguments *a = argumentsNew();
+    a->fnameb = (char *)fname;
+    a->fnamel = (char *)fnamel;
+    a->sort = sort;
+    a->dflag = dflag;
+    if (dflag)
+        a->sort = 1; /* GraphBlox.c:358-359 */
+    a->symmetric = (uint32_t)symmetric;
+    a->weighted = (uint32_t)weighted;
+    a->lmode = lmode;
+    a->lmode_l2 = lmode_l2;
+    a->lmode_l3 = lmode_l3;
+    omp_set_num_threads(nthreads > 0 ? nthreads : 1);
+    int s = quiet_begin();
+    struct GraphCSR *g = graphCSRPreProcessingStep(a);
+    quiet_end(s);
+    free(a);
+    return g;
+}
+
+/* The forward edge list's label arrays (old id -> new id and back). */
+int refd_labels(void *gp, uint32_t *label, uint32_t *inverse_label)
+{
+    struct GraphCSR *g = (struct GraphCSR *)gp;
+    struct EdgeList *e = g->sorted_edges_array;
+    uint32_t N = g->num_vertices;
+    if (label) memcpy(label, e->label_array, (size_t)N * 4);
+    if (inverse_label) memcpy(inverse_label, e->inverse_label_array, (size_t)N * 4);
+    return 0;
 }
 
 uint32_t refd_num_vertices(void *gp) { return ((struct GraphCSR *)gp)->num_vertices; }

@@ -444,6 +444,79 @@ def test_fixed_point_goldens_tell_widths_and_iterations_apart(golden):
     assert not np.array_equal(r32, v["pp9_r20"])  # nor one iteration fewer
 
 
+# ---------------------------------------------------------------- relabel
+
+LMODES = (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 13, 14)
+
+
+def _ref_relabel(path, dflag, levels, fnamel=None, sort=1):
+    """Labels and forward CSR arrays of the reference's pipeline with relabel levels."""
+    R = O.ref(1, 0)
+    g = R.refd_build_relabel(path.encode(), sort, dflag, 0, 0, 1, *levels, fnamel.encode() if fnamel else None)
+    N, E = R.refd_num_vertices(g), R.refd_num_edges(g)
+    lab = np.zeros(N, np.uint32)
+    R.refd_labels(g, lab.ctypes.data, None)
+    src, dst = np.zeros(E, np.uint32), np.zeros(E, np.uint32)
+    R.refd_export(g, 0, None, None, None, src.ctypes.data, dst.ctypes.data, None)
+    R.refd_free(g)
+    return lab, src, dst
+
+
+@pytest.mark.parametrize("name", ["rmat_s10", "fixture_v1024_e4096"])
+@pytest.mark.parametrize("dflag", [0, 1])
+def test_relabel_restatement_matches_reference(golden, tmp_path, name, dflag):
+    """reorderGraphProcess's orders (reorder.c:296-1164, 1483-1616) for every
+    lmode, and the pipeline around them: the renamed edges sorted again
+    stably (graphCSR.c:268-290)."""
+    z = golden(name)
+    path = str(tmp_path / "g.bin")
+    O.write_bin(path, z["edges_src"], z["edges_dst"])
+    lab0, src, dst = _ref_relabel(path, dflag, (0, 0, 0))
+    N = lab0.size
+    avg = (src.size if dflag else len(z["edges_src"])) // N
+    for lm in LMODES:
+        lab, rs, rd = _ref_relabel(path, dflag, (lm, 0, 0))
+        mine = O.relabel_labels(src, dst, N, avg, lm)
+        np.testing.assert_array_equal(mine, lab, err_msg=f"lmode {lm}")
+        ns, nd = mine[src].astype(np.uint64), mine[dst].astype(np.uint64)
+        order = np.argsort(ns * np.uint64(N) + nd, kind="stable")
+        np.testing.assert_array_equal(ns[order], rs, err_msg=f"lmode {lm}")
+        np.testing.assert_array_equal(nd[order], rd, err_msg=f"lmode {lm}")
+
+
+def test_relabel_levels_compose_and_file_labels(golden, tmp_path):
+    z = golden("rmat_s10")
+    path = str(tmp_path / "g.bin")
+    O.write_bin(path, z["edges_src"], z["edges_dst"])
+    lab0, src, dst = _ref_relabel(path, 0, (0, 0, 0))
+    N = lab0.size
+    avg = len(z["edges_src"]) // N
+    total = np.arange(N, dtype=np.uint32)
+    s, d = src, dst
+    for lm in (4, 1, 8):
+        step = O.relabel_labels(s, d, N, avg, lm)
+        s, d = step[s], step[d]
+        total = step[total]
+    lab, _, _ = _ref_relabel(path, 0, (4, 1, 8))
+    np.testing.assert_array_equal(total, lab)
+    perm = np.random.default_rng(5).permutation(N).astype(np.uint32)
+    fl = str(tmp_path / "labels.txt")
+    np.savetxt(fl, perm, fmt="%u")
+    lab, rs, _ = _ref_relabel(path, 0, (11, 0, 0), fnamel=fl)
+    np.testing.assert_array_equal(lab, perm)
+    np.testing.assert_array_equal(np.sort(perm[src]), rs)
+
+
+def test_mt19937_stream():
+    """The MT19937 stream of the random-degree order: the reference's random
+    relabel (pinned above) starts from it; the generator is the published one
+    (equidistribution: outputs cover the 32 bits evenly)."""
+    x = np.array(O.mt19937_uint32(27491095, 20000), np.uint64)
+    assert len(set(x.tolist())) == x.size
+    bits = np.array([(x >> np.uint64(b)) & np.uint64(1) for b in range(32)]).mean(axis=1)
+    assert np.all(np.abs(bits - 0.5) < 0.02)
+
+
 # ---------------------------------------------------------------- generator
 
 def test_rmat_permutation_is_bijection():
