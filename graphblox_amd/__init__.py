@@ -116,6 +116,7 @@ def lib():
         "gbx_pr_free": (i32, [_vp]),
         "gbx_pr_plan_info": (i32, [_vp, C.POINTER(u32), _vp, _vp, _vp]),
         "gbx_spmv_pull": (i32, [_vp, _vp, u32, i32, i32, _vp, _vp, _vp, C.POINTER(f64)]),
+        "gbx_spmv_fixed": (i32, [_vp, _vp, u32, i32, i32, i32, _vp, _vp, C.POINTER(f64)]),
         "gbx_ppr_create": (i32, [_vp, _vp, C.POINTER(PrParams), u32, _vp, _vp, C.POINTER(_vp)]),
         "gbx_ppr_info": (i32, [_vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
         "gbx_pr_launch_graph": (i32, [_vp, u32]),
@@ -143,7 +144,7 @@ def exported_symbols():
         "gbx_graph_create_rmat",
         "gbx_rmat_edges", "gbx_graph_from_csr", "gbx_graph_info", "gbx_graph_export", "gbx_graph_free",
         "gbx_pr_params_default", "gbx_pr_pull", "gbx_pr_create", "gbx_pr_reset", "gbx_pr_launch",
-        "gbx_pr_launch_timed", "gbx_pr_read", "gbx_pr_free", "gbx_pr_plan_info", "gbx_spmv_pull",
+        "gbx_pr_launch_timed", "gbx_pr_read", "gbx_pr_free", "gbx_pr_plan_info", "gbx_spmv_pull", "gbx_spmv_fixed",
         "gbx_ppr_create", "gbx_ppr_info", "gbx_pr_launch_graph", "gbx_graph_batch_create",
         "gbx_graph_batch_info", "gbx_tuning_get", "gbx_tuning_set", "gbx_autotune",
         "gbx_pr_adjoint",
@@ -373,6 +374,15 @@ class Graph:
         xi = _f32(x)
         self.ctx.check(self.ctx.L.gbx_spmv_pull(self.ctx.h, self.h, iterations, int(weighted), direction, _ptr(xi),
                                                 _ptr(xo), _ptr(yo), C.byref(sec)))
+        return xo, yo, sec.value
+
+    def spmv_fixed(self, bits, iterations=1, weighted=False, direction=PULL_IN):
+        """The reference's fixed-point SpMV variants (bits 32: pull, 64: push)."""
+        xo = np.empty(self.N, np.float32)
+        yo = np.empty(self.N, np.float32)
+        sec = C.c_double()
+        self.ctx.check(self.ctx.L.gbx_spmv_fixed(self.ctx.h, self.h, iterations, int(weighted), direction, bits,
+                                                 _ptr(xo), _ptr(yo), C.byref(sec)))
         return xo, yo, sec.value
 
     def session(self, **kw) -> "PageRank":

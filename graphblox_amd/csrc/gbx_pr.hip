@@ -798,6 +798,82 @@ static uint32_t hot_count(const gbx_ctx *ctx, uint32_t N, size_t elem)
     return (uint32_t)n;
 }
 
+// ------------------------------------------------------------------ fixed-point SpMV
+// SPMVPullFixedPointGraphCSR (SPMV.c:313-449, 32 bits) and
+// SPMVPushFixedPointGraphCSR (SPMV.c:451-590, 64 bits; it pushes the same
+// integer terms, so a pull gives its sums): per edge an integer product of
+// fixed-point weight and input, added in uint64 (order-free, so bit-exact).
+// 32: weight = (float)FloatToFixed32(w), x = FloatToFixed32(x), term =
+// ((u64)weight * x) >> 16; 64: DoubleToFixed64 of both, term = ((u128) w * x)
+// >> 32 mod 2^64.  Float to integer conversions are x86-64's: a 64-bit
+// truncation (low 32 bits for u32).
+__device__ __forceinline__ uint32_t fix32_of(float v)
+{
+    return (uint32_t)(unsigned long long)(long long)(v * 65536.0f + 0.5f);
+}
+
+__device__ __forceinline__ uint64_t fix64_of(float v)
+{
+    const double d = (double)v * 4294967296.0 + 0.5;
+    return d >= 9223372036854775808.0 ? (uint64_t)(d - 9223372036854775808.0) + (1ull << 63) : (uint64_t)d;
+}
+
+__global__ void spmv_fixed_input(const uint32_t *__restrict__ outdeg, uint32_t N, int bits, float *__restrict__ x,
+                                 uint64_t *__restrict__ xq, uint64_t *__restrict__ y)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        const uint32_t od = outdeg[v];
+        const float xv = od ? 1.0f / (float)od : 0.001f;  // SPMV.c:156-163
+        x[v] = xv;
+        xq[v] = bits == 32 ? (uint64_t)fix32_of(xv) : fix64_of(xv);
+        y[v] = 0;
+    }
+}
+
+// Edge-parallel over a CSR whose edges are sorted by row: each wave takes 64
+// consecutive edges, adds the terms of equal rows with a segmented scan, and
+// the last lane of each row run adds its sum with one atomic.
+template <int BITS>
+__global__ void spmv_fixed_kernel(const uint32_t *__restrict__ row, const uint32_t *__restrict__ col,
+                                  const float *__restrict__ w, uint32_t E, const uint64_t *__restrict__ xq,
+                                  uint64_t wconst, uint64_t *__restrict__ y)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t nwaves = gridDim.x * (blockDim.x / 64);
+    for (uint32_t base = (blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * 64u; base < E;
+         base += nwaves * 64u) {
+        const uint32_t j = base + lane;
+        const bool valid = j < E;
+        const uint32_t r = valid ? row[j] : 0xFFFFFFFFu;
+        uint64_t t = 0;
+        if (valid) {
+            const uint64_t x = xq[col[j] & kIdMask];
+            if constexpr (BITS == 32) {
+                const float wf = w ? (float)fix32_of(w[j]) : (float)wconst;
+                t = ((uint64_t)wf * x) >> 16;
+            } else {
+                const uint64_t wq = w ? fix64_of(w[j]) : wconst;
+                t = (__umul64hi(wq, x) << 32) | ((wq * x) >> 32);
+            }
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint64_t o = __shfl_up(t, off, 64);
+            const uint32_t ro = __shfl_up(r, off, 64);
+            if ((int)lane >= off && ro == r) t += o;
+        }
+        const uint32_t rn = __shfl_down(r, 1, 64);
+        if (valid && (lane == 63 || rn != r))
+            atomicAdd(reinterpret_cast<unsigned long long *>(y + r), (unsigned long long)t);
+    }
+}
+
+__global__ void spmv_fixed_output(const uint64_t *__restrict__ y, uint32_t N, int bits, float *__restrict__ out)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+        out[v] = bits == 32 ? (float)y[v] / 65536.0f : (float)((double)y[v] / 4294967296.0);
+}
+
 }  // namespace gbx
 
 using namespace gbx;
@@ -1329,6 +1405,63 @@ int gbx_spmv_pull(gbx_ctx *ctx, gbx_graph *g, uint32_t iterations, int weighted,
         GBX_HIP(ctx, hipGetLastError());
         GBX_HIP(ctx, hipMemcpyAsync(y_out, tmp.p, (size_t)N * 4, hipMemcpyDeviceToHost, ctx->stream));
     }
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (seconds) *seconds = ms * 1e-3;
+    return GBX_OK;
+}
+
+int gbx_spmv_fixed(gbx_ctx *ctx, gbx_graph *g, uint32_t iterations, int weighted, int direction, int bits,
+                   float *x_out, float *y_out, double *seconds)
+{
+    if (!ctx || !g) return set_error(ctx, GBX_ERR_ARG, "gbx_spmv_fixed: NULL argument");
+    if (bits != 32 && bits != 64) return set_error(ctx, GBX_ERR_ARG, "gbx_spmv_fixed: bits %d (32 or 64)", bits);
+    if (direction != GBX_PULL_IN && direction != GBX_PULL_FWD)
+        return set_error(ctx, GBX_ERR_ARG, "gbx_spmv_fixed: direction %d", direction);
+    if (!g->seg_off.empty()) return set_error(ctx, GBX_ERR_STATE, "gbx_spmv_fixed: not available for a batch of graphs");
+    if (weighted && !g->weighted)
+        return set_error(ctx, GBX_ERR_STATE, "gbx_spmv_fixed: weighted SpMV needs a weighted graph");
+    GBX_HIP(ctx, hipSetDevice(ctx->device));
+    const uint32_t N = g->N, E = g->E;
+    // the pull runs over the transpose rows (DIRECTED) or the forward rows;
+    // the push form always sums over each destination's in-edges
+    const gbx_csr_dev &rows = (bits == 64 || direction == GBX_PULL_IN) ? g->inv : g->fwd;
+    DevBuf x, xq, y, out;
+    GBX_TRY(dev_alloc(ctx, x, (size_t)N * 4 + 4));
+    GBX_TRY(dev_alloc(ctx, xq, (size_t)N * 8 + 8));
+    GBX_TRY(dev_alloc(ctx, y, (size_t)N * 8 + 8));
+    GBX_TRY(dev_alloc(ctx, out, (size_t)N * 4 + 4));
+    const unsigned gN = grid_for(N, 256, 16384);
+    spmv_fixed_input<<<gN, 256, 0, ctx->stream>>>(g->fwd.out_degree.as<uint32_t>(), N, bits, x.as<float>(),
+                                                 xq.as<uint64_t>(), y.as<uint64_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    const float *w = weighted ? rows.w.as<float>() : nullptr;
+    // SPMV.c:354 and :497: the constant weight 0.0001f, converted
+    const float wc_f = 0.0001f;
+    const uint64_t wconst = bits == 32 ? (uint64_t)(uint32_t)(long long)(wc_f * 65536.0f + 0.5f)
+                                       : (uint64_t)((double)wc_f * 4294967296.0 + 0.5);
+    const unsigned gE = grid_for(((size_t)E + 63) / 64 * 64, 256, (unsigned)std::max(1, ctx->num_cus) * 16);
+    hipEvent_t e0, e1;
+    GBX_HIP(ctx, hipEventCreate(&e0));
+    GBX_HIP(ctx, hipEventCreate(&e1));
+    GBX_HIP(ctx, hipEventRecord(e0, ctx->stream));
+    for (uint32_t it = 0; it < iterations && E; it++) {
+        if (bits == 32)
+            spmv_fixed_kernel<32><<<gE, 256, 0, ctx->stream>>>(rows.src.as<uint32_t>(), rows.dst.as<uint32_t>(), w, E,
+                                                               xq.as<uint64_t>(), wconst, y.as<uint64_t>());
+        else
+            spmv_fixed_kernel<64><<<gE, 256, 0, ctx->stream>>>(rows.src.as<uint32_t>(), rows.dst.as<uint32_t>(), w, E,
+                                                               xq.as<uint64_t>(), wconst, y.as<uint64_t>());
+        GBX_HIP(ctx, hipGetLastError());
+    }
+    GBX_HIP(ctx, hipEventRecord(e1, ctx->stream));
+    spmv_fixed_output<<<gN, 256, 0, ctx->stream>>>(y.as<uint64_t>(), N, bits, out.as<float>());
+    GBX_HIP(ctx, hipGetLastError());
+    if (x_out) GBX_HIP(ctx, hipMemcpyAsync(x_out, x.p, (size_t)N * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (y_out) GBX_HIP(ctx, hipMemcpyAsync(y_out, out.p, (size_t)N * 4, hipMemcpyDeviceToHost, ctx->stream));
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, e0, e1);

@@ -820,17 +820,21 @@ void freeSPMVStats(struct SPMVStats *stats)
 }
 
 /* SPMVPullGraphCSR, SPMV.c:117-215 (DIRECTED=1 rows; weighted when GBX_WEIGHTED) */
-struct SPMVStats *SPMVPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+/* One SpMV variant on the device copy of graph: bits 0 = the float pull,
+ * 32 / 64 = the fixed-point forms. */
+static struct SPMVStats *run_spmv(struct Arguments *arguments, struct GraphCSR *graph, int bits, const char *name)
 {
     gbx_graph *g = device_graph(graph);
     if (!g) return NULL;
     struct SPMVStats *stats = newSPMVStatsGraphCSR(graph);
     stats->arguments = arguments;
     double secs = 0.0;
-    int rc = gbx_spmv_pull(g_ctx, g, arguments->iterations, GBX_WEIGHTED, GBX_PULL_IN, NULL,
-                           stats->vector_input, stats->vector_output, &secs);
+    int rc = bits ? gbx_spmv_fixed(g_ctx, g, arguments->iterations, GBX_WEIGHTED, GBX_PULL_IN, bits,
+                                   stats->vector_input, stats->vector_output, &secs)
+                  : gbx_spmv_pull(g_ctx, g, arguments->iterations, GBX_WEIGHTED, GBX_PULL_IN, NULL,
+                                  stats->vector_input, stats->vector_output, &secs);
     if (rc != GBX_OK) {
-        fprintf(stderr, "graphblox_hip: SPMVPullGraphCSR failed: %s\n", gbx_last_error(g_ctx));
+        fprintf(stderr, "graphblox_hip: %s failed: %s\n", name, gbx_last_error(g_ctx));
         freeSPMVStats(stats);
         return NULL;
     }
@@ -847,14 +851,37 @@ struct SPMVStats *SPMVPullGraphCSR(struct Arguments *arguments, struct GraphCSR 
     return stats;
 }
 
-/* SPMVGraphCSR, SPMV.c:89-116: variant 0 (and unknown codes) is the pull
- * kernel; push and fixed-point variants are not provided -> NULL. */
+/* SPMVPullGraphCSR, SPMV.c:117-215 */
+struct SPMVStats *SPMVPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_spmv(arguments, graph, 0, "SPMVPullGraphCSR");
+}
+
+/* SPMVPushGraphCSR, SPMV.c:216-311: the same float terms as the pull */
+struct SPMVStats *SPMVPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_spmv(arguments, graph, 0, "SPMVPushGraphCSR");
+}
+
+/* SPMVPullFixedPointGraphCSR, SPMV.c:313-449 (bit-exact) */
+struct SPMVStats *SPMVPullFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_spmv(arguments, graph, 32, "SPMVPullFixedPointGraphCSR");
+}
+
+/* SPMVPushFixedPointGraphCSR, SPMV.c:451-590 (bit-exact) */
+struct SPMVStats *SPMVPushFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_spmv(arguments, graph, 64, "SPMVPushFixedPointGraphCSR");
+}
+
+/* SPMVGraphCSR, SPMV.c:89-116: the variant selected by arguments->pushpull */
 struct SPMVStats *SPMVGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
-    if (arguments->pushpull >= 1 && arguments->pushpull <= 3) {
-        fprintf(stderr, "graphblox_hip: SpMV variant %u is not provided by the HIP backend\n",
-                arguments->pushpull);
-        return NULL;
+    switch (arguments->pushpull) {
+    case 1: return SPMVPushGraphCSR(arguments, graph);
+    case 2: return SPMVPullFixedPointGraphCSR(arguments, graph);
+    case 3: return SPMVPushFixedPointGraphCSR(arguments, graph);
+    default: return SPMVPullGraphCSR(arguments, graph);
     }
-    return SPMVPullGraphCSR(arguments, graph);
 }

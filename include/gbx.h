@@ -18,7 +18,8 @@
  *                                     (01_host/src/structures/graphCSR.c:243-327)
  *   gbx_pr_*                          pageRankPullGraphCSR (01_host/src/algorithms/openmp/pageRank.c:245-368)
  *                                     and its stats (pageRank.h:19-30)
- *   gbx_spmv_pull                     SPMVPullGraphCSR (01_host/src/algorithms/openmp/SPMV.c:117-215)
+ *   gbx_spmv_pull, gbx_spmv_fixed     SPMVPullGraphCSR (01_host/src/algorithms/openmp/SPMV.c:117-215)
+ *                                     and its variants (SPMV.c:216-590)
  *   gbx_ppr_batched, gbx_graph_batch_*, gbx_pr_adjoint, gbx_rmat_*
  *                                     no reference counterpart (batched, differentiable
  *                                     and generator forms of the same path)
@@ -375,6 +376,15 @@ int gbx_pr_plan_info(const gbx_pr *pr, uint32_t *n_classes, uint32_t *class_lane
 int gbx_spmv_pull(gbx_ctx *ctx, gbx_graph *g, uint32_t iterations, int weighted,
                   int direction, const float *x_in, float *x_out, float *y_out,
                   double *seconds);
+
+/* The reference's fixed-point SpMV variants, bit-exact: bits 32 =
+ * SPMVPullFixedPointGraphCSR (SPMV.c:313-449, rows as `direction`), bits 64
+ * = SPMVPushFixedPointGraphCSR (SPMV.c:451-590; it sums the same integer
+ * terms as a pull over the transpose rows).  Same input vector and
+ * accumulation over `iterations` passes as gbx_spmv_pull; y_out receives
+ * the reference's float conversion of the fixed-point sums. */
+int gbx_spmv_fixed(gbx_ctx *ctx, gbx_graph *g, uint32_t iterations, int weighted,
+                   int direction, int bits, float *x_out, float *y_out, double *seconds);
 
 #ifdef __cplusplus
 }

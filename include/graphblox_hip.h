@@ -211,6 +211,10 @@ struct SPMVStats *newSPMVStatsGraphCSR(struct GraphCSR *graph);
 void freeSPMVStats(struct SPMVStats *stats);
 struct SPMVStats *SPMVGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct SPMVStats *SPMVPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* SPMV.c:216-590: push (the pull's terms) and the fixed-point variants (bit-exact) */
+struct SPMVStats *SPMVPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct SPMVStats *SPMVPullFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct SPMVStats *SPMVPushFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 
 /* Backend housekeeping (no reference counterpart): release the process-wide
  * HIP context and every cached device graph. */

@@ -110,6 +110,7 @@ def ref(directed: int = 1, weighted: int = 0):
         L.refd_pagerank_variant.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_double, C.c_int, _f32p,
                                             C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
         L.refd_spmv.argtypes = [C.c_void_p, C.c_uint32, C.c_int, _f32p, _f32p, C.POINTER(C.c_double)]
+        L.refd_spmv_variant.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_int, _f32p, _f32p]
         if directed:
             L.refd_pagerank_on_csr.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p, _u32p, _u32p,
                                                C.c_uint32, C.c_double, C.c_int, _f32p,
@@ -396,6 +397,58 @@ def relabel_labels(src, dst, N, avg_degree, lmode, file_labels=None):
     labels = np.empty(N, np.uint32)
     labels[order] = np.arange(N, dtype=np.uint32)
     return labels
+
+
+def _x86_to_u64(x):
+    """(uint64_t) of non-negative doubles below 2^64, as x86-64 code converts them."""
+    x = np.asarray(x, np.float64)
+    big = x >= 2.0**63
+    out = np.where(big, x - 2.0**63, x).astype(np.int64).astype(np.uint64)
+    return np.where(big, out + np.uint64(1 << 63), out)
+
+
+def spmv_fixed(N, outdeg, row_ptr, col, w, bits, iterations=1):
+    """The reference's fixed-point SpMV (SPMV.c:313-590): x = 1/outdeg (0.001f
+    when 0) as float.  bits 32 (SPMVPullFixedPointGraphCSR): x32 =
+    FloatToFixed32(x), weight = (float)FloatToFixed32(w or 0.0001f), term =
+    ((u64)weight * x32) >> 16; bits 64 (SPMVPushFixedPointGraphCSR, the same
+    integer terms pushed): x64 = DoubleToFixed64(x), w64 = DoubleToFixed64(w or
+    0.0001f), term = ((u128)w64 * x64) >> 32 mod 2^64.  y (u64) accumulates
+    `iterations` passes; out: float y = Fixed32ToFloat / (float)Fixed64ToDouble.
+    row_ptr/col/w: the rows the pull runs over (in-CSR for DIRECTED)."""
+    od = np.asarray(outdeg, np.uint32)
+    x = np.where(od > 0, np.float32(1.0) / np.maximum(od, 1).astype(np.float32), np.float32(0.001)).astype(np.float32)
+    rp = np.asarray(row_ptr, np.int64)
+    c = np.asarray(col, np.int64) & 0x3FFFFFFF
+    rows = np.repeat(np.arange(N), np.diff(rp))
+    if bits == 32:
+        def f2x32(v):
+            t = (np.asarray(v, np.float32) * np.float32(65536.0) + np.float32(0.5)).astype(np.float32)
+            return t.astype(np.int64).astype(np.uint32)
+        x32 = f2x32(x).astype(np.uint64)
+        wq = (f2x32(w) if w is not None else np.full(c.size, f2x32(np.float32(0.0001)), np.uint32))
+        wq = wq.astype(np.float32).astype(np.uint64)
+        term = (wq * x32[c]) >> np.uint64(16)
+    else:
+        def d2x64(v):
+            return _x86_to_u64(np.asarray(v, np.float32).astype(np.float64) * 4294967296.0 + 0.5)
+        x64 = d2x64(x)
+        wq = d2x64(w) if w is not None else np.full(c.size, d2x64(np.float32(0.0001)), np.uint64)
+        a, b = wq, x64[c]
+        a_hi, a_lo = a >> np.uint64(32), a & np.uint64(0xFFFFFFFF)
+        b_hi, b_lo = b >> np.uint64(32), b & np.uint64(0xFFFFFFFF)
+        # ((a * b) >> 32) mod 2^64 from 32-bit limbs
+        lo = a_lo * b_lo
+        mid = a_hi * b_lo + a_lo * b_hi + (lo >> np.uint64(32))  # may wrap: only its low 64 bits matter below
+        term = (a_hi * b_hi << np.uint64(32)) + mid
+    y = np.zeros(N, np.uint64)
+    np.add.at(y, rows, term)
+    y = y * np.uint64(iterations)
+    if bits == 32:
+        out = y.astype(np.float32) / np.float32(65536.0)
+    else:
+        out = (y.astype(np.float64) / 4294967296.0).astype(np.float32)
+    return x, out.astype(np.float32)
 
 
 def pr_pull_quant(N, outdeg, row_ptr, col, bits, iterations=20, epsilon=0.0):

@@ -209,6 +209,26 @@ int refd_spmv(void *gp, uint32_t iterations, int nthreads, float *y_out,
     return 0;
 }
 
+/* SPMVGraphCSR (SPMV.c:89-116) by variant code: 0 pull, 1 push, 2 pull fixed
+ * point, 3 push fixed point. */
+int refd_spmv_variant(void *gp, int pushpull, uint32_t iterations, int nthreads, float *y_out,
+                      float *x_out)
+{
+    struct Arguments *a = argumentsNew();
+    a->iterations = iterations;
+    a->pushpull = (uint32_t)pushpull;
+    a->ker_numThreads = nthreads > 0 ? nthreads : 1;
+    omp_set_num_threads(a->ker_numThreads);
+    int s = quiet_begin();
+    struct SPMVStats *st = SPMVGraphCSR(a, (struct GraphCSR *)gp);
+    quiet_end(s);
+    if (y_out) memcpy(y_out, st->vector_output, (size_t)st->num_vertices * 4);
+    if (x_out) memcpy(x_out, st->vector_input, (size_t)st->num_vertices * 4);
+    freeSPMVStats(st);
+    free(a);
+    return 0;
+}
+
 #if DIRECTED
 /*
  * Run the reference pageRankPullGraphCSR on caller-owned CSR arrays (no file,

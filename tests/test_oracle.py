@@ -444,6 +444,59 @@ def test_fixed_point_goldens_tell_widths_and_iterations_apart(golden):
     assert not np.array_equal(r32, v["pp9_r20"])  # nor one iteration fewer
 
 
+# ---------------------------------------------------------------- SpMV variants
+
+def _ref_spmv_graph(tmp_path, z, weighted):
+    R = O.ref(1, int(weighted))
+    path = str(tmp_path / ("g.wbin" if weighted else "g.bin"))
+    if weighted:
+        O.write_bin(path, z["wedges_src"], z["wedges_dst"], z["wedges_w"])
+    else:
+        O.write_bin(path, z["edges_src"], z["edges_dst"])
+    g = R.refd_build(path.encode(), 1, 0, 0, int(weighted), 1)
+    N, E = R.refd_num_vertices(g), R.refd_num_edges(g)
+    a = [np.zeros(N, np.uint32) for _ in range(3)] + [np.zeros(E, np.uint32) for _ in range(2)] + \
+        [np.zeros(E, np.float32)]
+    R.refd_export(g, 1, *[x.ctypes.data for x in a])
+    fod = np.zeros(N, np.uint32)
+    R.refd_export(g, 0, fod.ctypes.data, None, None, None, None, None)
+    rp = np.zeros(N + 1, np.uint64)
+    np.cumsum(a[0], out=rp[1:])
+    return R, g, N, fod, rp, a[4], a[5]
+
+
+@pytest.mark.parametrize("name", ["rmat_s10", "rmat_s12"])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_spmv_fixed_oracle_matches_reference(golden, tmp_path, name, weighted):
+    """SPMVPullFixedPointGraphCSR (2) and SPMVPushFixedPointGraphCSR (3):
+    bit-exact inputs and outputs after 1 and 3 accumulating passes."""
+    R, g, N, fod, rp, col, w = _ref_spmv_graph(tmp_path, golden(name), weighted)
+    for code, bits in ((2, 32), (3, 64)):
+        for it in (1, 3):
+            y, x = np.zeros(N, np.float32), np.zeros(N, np.float32)
+            R.refd_spmv_variant(g, code, it, 1, y, x)
+            xo, yo = O.spmv_fixed(N, fod, rp, col, w if weighted else None, bits, it)
+            np.testing.assert_array_equal(xo, x)
+            np.testing.assert_array_equal(yo, y, err_msg=f"code {code} iterations {it}")
+    # the push forms sum the pull's terms: one reference thread gives identical floats
+    y0, y1, xd = np.zeros(N, np.float32), np.zeros(N, np.float32), np.zeros(N, np.float32)
+    R.refd_spmv_variant(g, 0, 3, 1, y0, xd)
+    R.refd_spmv_variant(g, 1, 3, 1, y1, xd)
+    np.testing.assert_array_equal(y0, y1)
+    R.refd_free(g)
+
+
+def test_spmv_fixed_64_product_uses_the_high_bits():
+    """((u128) w * x) >> 32 needs the high half: weights and inputs near 2^32
+    (a weight of ~1 times an input of ~1) tell a 64-bit-only product apart."""
+    N = 3
+    rp = np.array([0, 1, 2, 3], np.uint64)
+    col = np.array([1, 2, 0], np.uint32)
+    w = np.array([0.9, 1.0, 0.5], np.float32)
+    _, y = O.spmv_fixed(N, np.array([1, 1, 1], np.uint32), rp, col, w, 64, 1)
+    np.testing.assert_allclose(y, [0.9, 1.0, 0.5], rtol=1e-6)
+
+
 # ---------------------------------------------------------------- relabel
 
 LMODES = (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 13, 14)
