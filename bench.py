@@ -47,9 +47,11 @@ def traffic_from_profiles(workload_key: str):
     return t.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(g, iterations: int, threads: int):
+def cpu_baseline(g, iterations: int, threads: int, gpu_ranks=None):
     """Reference pageRankPullGraphCSR (oracle/_ref) on this graph's CSR arrays,
-    or the oracle's restatement of it when the reference build is absent."""
+    or the oracle's restatement of it when the reference build is absent.
+    With gpu_ranks (the measured path's ranks after the same iterations) the
+    record also holds their L1 distance from the reference's ranks."""
     import ctypes as C
     from oracle import oracle as O
 
@@ -71,12 +73,15 @@ def cpu_baseline(g, iterations: int, threads: int):
         fo = O.OracleCSR(N, E, fwd["out_degree"], fwd["in_degree"], fwd["edges_idx"], fwd["src"], fwd["dst"], fwd["w"])
         io = O.OracleCSR(N, E, inv["out_degree"], inv["in_degree"], inv["edges_idx"], inv["src"], inv["dst"], inv["w"])
         t0 = time.perf_counter()
-        O.pr_pull_f32(fo, io, iterations, 0.0, nthreads=threads)
+        ranks = O.pr_pull_f32(fo, io, iterations, 0.0, nthreads=threads)["ranks"]
         secs = time.perf_counter() - t0
     gteps = E * iterations / secs / 1e9
-    return dict(value=round(gteps, 4), unit="GTEPS", cores=threads, kind=kind,
-                sample=f"{iterations} iterations of pull PageRank (FP32, reference semantics) on the same "
-                       f"R-MAT graph (N={N}, E={E}), {threads} OpenMP threads, whole loop timed")
+    rec = dict(value=round(gteps, 4), unit="GTEPS", cores=threads, kind=kind,
+               sample=f"{iterations} iterations of pull PageRank (FP32, reference semantics) on the same "
+                      f"R-MAT graph (N={N}, E={E}), {threads} OpenMP threads, whole loop timed")
+    if gpu_ranks is not None:
+        rec["l1_gpu_vs_reference"] = float(np.abs(np.asarray(gpu_ranks, np.float64) - ranks).sum())
+    return rec
 
 
 def time_form(g, warmup, steps, **kw):
@@ -224,10 +229,29 @@ def main():
             forms["batched_ppr32"]["gteps"] / forms["fp32_binned_same_graph_as_batched"]["gteps"], 3)
         log(f"batched form: {forms['batched_ppr32']}")
         gb.free()
+    # wall time to convergence: reference semantics, stop when no vertex changes by 1e-4 or more
+    # (pageRank.c:327-339), at most 100 iterations; reset, iterations and read-back timed
+    conv_s = g.session(iterations=100, epsilon=1e-4, mode=gbx.PR_REF, stop=gbx.STOP_ACTIVE, method=method)
+    ctx.sync()
+    t0 = time.perf_counter()
+    conv_s.reset()
+    conv_s.launch(100)
+    _, conv_rep = conv_s.read()
+    conv_ms = (time.perf_counter() - t0) * 1e3
+    conv_s.free()
+    convergence = {"epsilon": 1e-4, "stop": "no vertex with |dr| >= epsilon", "iterations": conv_rep["iterations"],
+                   "executed": conv_rep["executed"], "converged": bool(conv_rep["converged"]),
+                   "wall_ms": round(conv_ms, 3)}
+    log(f"convergence: {convergence}")
     cpu = None
     if not args.no_cpu:
         log("CPU baseline ...")
-        cpu = cpu_baseline(g, args.cpu_iters, args.cpu_threads)
+        cs = g.session(iterations=args.cpu_iters, epsilon=0.0, mode=gbx.PR_REF, stop=gbx.STOP_NONE, method=method)
+        cs.reset()
+        cs.launch(args.cpu_iters)
+        gpu_ranks, _ = cs.read()
+        cs.free()
+        cpu = cpu_baseline(g, args.cpu_iters, args.cpu_threads, gpu_ranks)
         log(f"CPU baseline {cpu['value']} GTEPS ({cpu['kind']}, {cpu['cores']} threads)")
     out = {
         "metric": "GTEPS & % HBM roofline, pull PageRank R-MAT s24, 1 MI355X, vs plain FP32",
@@ -256,12 +280,13 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": ("bin_scatter_kernel<float> + bin_accum_kernel<K_PR_REF, float> (one iteration)"
+            "kernel": ("bin_scatter_kernel<unsigned int, 0> + bin_accum_kernel<K_PR_REF, float> (one iteration)"
                        if args.method == "binned" else "pr_iter_kernel<K_PR_REF>"),
             "kernel_ms": round(kernel_ms, 5),
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": cpu,
+        "convergence": convergence,
         "forms": forms,
     }
     print(json.dumps(out), flush=True)
