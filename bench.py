@@ -280,7 +280,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": ("bin_scatter_kernel<unsigned int, 0> + bin_accum_kernel<K_PR_REF, float> (one iteration)"
+            "kernel": ("bin_scatter_runs_kernel<float> + bin_accum_kernel<K_PR_REF, float> (one iteration)"
                        if args.method == "binned" else "pr_iter_kernel<K_PR_REF>"),
             "kernel_ms": round(kernel_ms, 5),
             "algorithmic_bytes_per_launch": bytes_per_launch,

@@ -51,7 +51,7 @@ class Tuning(C.Structure):
                 ("binned_min_vertices", C.c_uint32), ("binned_chunk_edges", C.c_uint32),
                 ("binned_bin_edges", C.c_uint32), ("binned_bin_rows", C.c_uint32),
                 ("binned_chunk_rows", C.c_uint32), ("binned_hot_sources", C.c_uint32),
-                ("reserved", C.c_uint32 * 2)]
+                ("binned_edge_values", C.c_uint32), ("reserved", C.c_uint32 * 1)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("bin_thresholds", "reserved")}

@@ -40,6 +40,17 @@
 // written by the scatter pass, so V is written in whole lines) and the
 // accumulate pass reads 8 values per lane.  src_local is in chunk-major
 // order, segments padded the same way with the marker 0xFFFF.
+//
+// Run layout (the FP32/BF16 kinds, unless tuned off): the edges of one
+// chunk into one row are consecutive in their segment, and the scatter sums
+// each such run (cut at every group of 8 src_local entries) into one V
+// value, so V, dst_local and both passes over V shrink to the runs (0.64 of
+// the cold edges at R-MAT scale 24).  src_local flags each run's last entry
+// (bit 15; 15-bit sources, chunks of at most 32736), vgrp gives the V
+// position of each group's first run, and V segments are padded to 8
+// values (zeros, written once).  The run sums are FP32, so the result
+// depends on the cut (not on the launch): deterministic for a plan, within
+// FP32 rounding of the one-value-per-edge layout.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -62,6 +73,17 @@ constexpr uint32_t kNoMulti = 0xFFFFFFFFu;
 inline uint32_t chunk_rows_of(const gbx_tuning &tu, uint32_t elem)
 {
     return std::max(8u, std::min(std::min(tu.binned_chunk_rows, kMaxChunkRows), (128u * 1024u) / elem) & ~7u);
+}
+
+// The run layout keeps 15 bits of source index (the top bit flags a run's
+// end) and one index past the chunk for the pads, and its scatter's LDS
+// holds the staging next to the contributions: at most 32736 sources.
+constexpr uint32_t kMaxRunChunkRows = 32736;
+
+inline uint32_t chunk_sources_of(const gbx_tuning &tu, uint32_t elem, bool runs)
+{
+    const uint32_t sc = chunk_rows_of(tu, elem);
+    return runs ? std::min(sc, kMaxRunChunkRows) : sc;
 }
 
 // Edges per chunk and per bin: the tuning value, or (0) E / CUs clamped to
@@ -92,12 +114,16 @@ struct gbx_bplan {
     uint32_t H = 0;                   // hot sources [0, H): their edges bypass V
     uint32_t Hlen = 0;                // padded length of the hot pair array
     uint32_t SB = 0;                  // rows per bin (LDS int64 sums)
-    uint32_t cut[7] = {0, 0, 0, 0, 0, 0, 0};  // chunk edges, bin edges, bin rows, chunk rows, pad, hot, hot bytes
+    uint32_t cut[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // chunk edges, bin edges, bin rows, chunk rows, pad, hot,
+                                                  // hot bytes, run layout
     uint32_t Plen = 0;                // padded length of src_local
+    uint32_t SC = 0;                  // sources per chunk (at most)
+    bool runs = false;                // run layout: one V value per run of equal rows of a segment
     gbx::DevBuf chunk_lo;             // u32[NC+1] first source of each chunk
     gbx::DevBuf chunk_grp;            // u32[NC+1] first group of 8 src_local entries of each chunk
     gbx::DevBuf src_local;            // u16[Plen] source - chunk_lo, chunk-major, segments padded (0xFFFF)
     gbx::DevBuf bdst;                 // u32[Plen/32] V position of each block of 32 src_local entries
+    gbx::DevBuf vgrp;                 // run layout: u32[Plen/8] V position of each group's first run
     gbx::DevBuf dst_local;            // u16[Vlen] row - bin first row, V order
     gbx::DevBuf hot_pairs;            // u32[Hlen] (row - bin first row) | source << 16, per bin (pad: src 0xFFFF)
     gbx::DevBuf units;                // BinUnit[NU]
@@ -110,6 +136,8 @@ struct gbx_binned_state {
     gbx::DevBuf cnt;  // u32[n_multi] arrival counters (zero between iterations)
     gbx::DevBuf work;  // u32 next accumulate unit (zero between iterations)
     unsigned accum_grid = 0;
+    uint32_t dummy_off = 0;  // run layout: byte offset of the scatter's dummy areas in V
+    uint32_t stage_off = 0;  // run layout: byte offset of the scatter's staging in LDS
     size_t shm_scatter = 0, shm_accum = 0;
 };
 
@@ -216,14 +244,14 @@ __global__ void block_destinations(const uint32_t *__restrict__ key, const uint3
     }
 }
 
-// chunk_grp[k] = first group of the chunk's first segment.
+// chunk_grp[k] = first group (of 2^shift entries) of the chunk's first segment.
 __global__ void chunk_groups(const uint32_t *__restrict__ cstart, uint32_t NC, const uint32_t *__restrict__ runidx,
-                             uint32_t E, uint32_t nrun, const uint32_t *__restrict__ pstart,
+                             uint32_t E, uint32_t nrun, const uint32_t *__restrict__ pstart, uint32_t shift,
                              uint32_t *__restrict__ chunk_grp)
 {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= NC; k += gridDim.x * blockDim.x) {
         const uint32_t r = cstart[k] >= E ? nrun : runidx[cstart[k]];
-        chunk_grp[k] = pstart[r] >> 3;
+        chunk_grp[k] = pstart[r] >> shift;
     }
 }
 
@@ -261,6 +289,88 @@ __global__ void split_hot_cold(const uint32_t *__restrict__ col, const uint32_t 
     }
 }
 
+// ---- run layout: one V value per run of equal rows in a (chunk, bin) segment
+
+// A lane of the scatter takes a group of 8 src_local entries; runs never
+// cross a group (segments are padded to 8 entries and a run is cut at every
+// group boundary), so every lane sums its own runs.
+constexpr uint32_t kRunGroup = 8;
+constexpr uint32_t kRunEnd = 0x8000u;  // src_local flag: the entry ends its run
+
+// head[j] = 1 where a run starts: at a new row and at every group boundary
+// (segment starts are group boundaries).  j indexes the chunk-major sorted
+// cold edges; segment r = runidx + flag - 1 sits at src_local position
+// pstart[r].
+__global__ void run_heads(const uint32_t *__restrict__ pay, uint32_t n, const uint32_t *__restrict__ flag,
+                          const uint32_t *__restrict__ runidx, const uint32_t *__restrict__ runstart,
+                          const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ rowid_c,
+                          uint32_t *__restrict__ head)
+{
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t r = runidx[j] + flag[j] - 1u;
+        const uint32_t p = pstart[r] + (j - runstart[r]);
+        head[j] = (p % kRunGroup == 0 || rowid_c[pay[j]] != rowid_c[pay[j - 1]]) ? 1u : 0u;
+    }
+}
+
+// Per segment: V length (runs, padded to 8), the bin-major sort key
+// (bin * NC + chunk) with the segment as payload, and the bin's V length.
+__global__ void run_segments(const uint32_t *__restrict__ key, const uint32_t *__restrict__ runstart, uint32_t nseg,
+                             uint32_t n, const uint32_t *__restrict__ hx, uint32_t NB, uint32_t NC,
+                             uint32_t *__restrict__ vlen, uint32_t *__restrict__ skey, uint32_t *__restrict__ spay,
+                             uint32_t *__restrict__ bin_len)
+{
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nseg; r += gridDim.x * blockDim.x) {
+        const uint32_t s0 = runstart[r], s1 = r + 1 < nseg ? runstart[r + 1] : n;
+        const uint32_t len = (hx[s1] - hx[s0] + 7u) & ~7u;
+        const uint32_t kk = key[s0], k = kk / NB, b = kk % NB;
+        vlen[r] = len;
+        skey[r] = b * NC + k;
+        spay[r] = r;
+        atomicAdd(bin_len + b, len);
+    }
+}
+
+__global__ void scatter_u32(const uint32_t *__restrict__ a, const uint32_t *__restrict__ idx, uint32_t n,
+                            uint32_t *__restrict__ out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[idx[i]] = a[i];
+}
+
+// Every cold edge: its source (| kRunEnd on a run's last entry) in
+// src_local, its run's row in dst_local, and for the first entry of each
+// group the V position of its run (the group's first run).  A segment's
+// last run takes the pads of its group (zeros): its end flag sits on the
+// group's last entry, so every group ends with a run end.
+__global__ void run_fill(const uint32_t *__restrict__ key, const uint32_t *__restrict__ pay, uint32_t n,
+                         const uint32_t *__restrict__ flag, const uint32_t *__restrict__ runidx,
+                         const uint32_t *__restrict__ runstart, const uint32_t *__restrict__ pstart,
+                         const uint32_t *__restrict__ head, const uint32_t *__restrict__ hx,
+                         const uint32_t *__restrict__ voff, const uint32_t *__restrict__ rowid_c,
+                         const uint32_t *__restrict__ col_c, const uint32_t *__restrict__ chunk_lo,
+                         const uint32_t *__restrict__ bin_row0, uint32_t NB, uint32_t pad,
+                         uint16_t *__restrict__ src_local, uint16_t *__restrict__ dst_local,
+                         uint32_t *__restrict__ vgrp)
+{
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t r = runidx[j] + flag[j] - 1u, s0 = runstart[r];
+        const uint32_t p = pstart[r] + (j - s0);
+        const uint32_t kk = key[j], k = kk / NB, b = kk % NB;
+        const uint32_t vpos = voff[r] + (hx[j] + head[j] - 1u - hx[s0]);
+        const bool end = j + 1 == n || head[j + 1];
+        const bool seg_end = j + 1 == n || flag[j + 1];
+        const uint32_t e = pay[j];
+        bool flag_here = end;
+        if (seg_end && p % kRunGroup != kRunGroup - 1) {
+            flag_here = false;
+            src_local[p | (kRunGroup - 1)] = (uint16_t)(pad | kRunEnd);
+        }
+        src_local[p] = (uint16_t)((col_c[e] - chunk_lo[k]) | (flag_here ? kRunEnd : 0u));
+        if (end) dst_local[vpos] = (uint16_t)(rowid_c[e] - bin_row0[b]);
+        if (p % kRunGroup == 0) vgrp[p / kRunGroup] = vpos;
+    }
+}
+
 // ------------------------------------------------------------------ iteration kernels
 
 struct ScatterView {
@@ -290,13 +400,13 @@ __device__ __forceinline__ void store8(T *V, uint32_t pos, const T (&v)[8])
 
 // n values (whole groups of 8 at 8-value aligned addresses, then a tail)
 // from global memory into LDS.
-template <class T>
+template <class T, int NT = kBlock>
 __device__ __forceinline__ void copy_to_lds(T *dst, const T *src, uint32_t n)
 {
     const uint32_t nv = n / 8;
-    for (uint32_t i = threadIdx.x; i < nv; i += kBlock)
+    for (uint32_t i = threadIdx.x; i < nv; i += NT)
         reinterpret_cast<Vec8<T> *>(dst)[i] = reinterpret_cast<const Vec8<T> *>(src)[i];
-    for (uint32_t i = nv * 8 + threadIdx.x; i < n; i += kBlock) dst[i] = src[i];
+    for (uint32_t i = nv * 8 + threadIdx.x; i < n; i += NT) dst[i] = src[i];
 }
 
 // n FP32 contributions quantized into LDS (the quantized kinds): whole
@@ -360,6 +470,217 @@ __global__ void __launch_bounds__(kBlock) bin_scatter_kernel(ScatterView S, cons
             }
         }
     }
+}
+
+struct RunsView {
+    const uint32_t *__restrict__ chunk_lo;
+    const uint32_t *__restrict__ chunk_grp;
+    const uint16_t *__restrict__ src_local;  // local source | kRunEnd; pads: `pad`
+    const uint32_t *__restrict__ vgrp;       // V position of each group's first run
+    uint32_t pad;        // LDS index of a zero (the plan's chunk size)
+    uint32_t stage_off;  // byte offset of the per-wave staging in LDS
+    uint32_t dummy;      // byte offset in V of the dummy areas past the last run (written by idle store
+                         // lanes, never read): kRunDummyAreas areas of 64 values
+};
+
+// Values summed in FP32 (BF16 widened; a run's sum rounded to BF16 once).
+__device__ __forceinline__ float f32_of(float x) { return x; }
+__device__ __forceinline__ float f32_of(uint16_t x) { return __uint_as_float((uint32_t)x << 16); }
+__device__ __forceinline__ void put_f32(float *V, uint32_t i, float v) { V[i] = v; }
+__device__ __forceinline__ void put_f32(uint16_t *V, uint32_t i, float v) { V[i] = f32_to_bf16(v); }
+
+// LDS written by one lane of a wave and read by another lane of the same
+// wave: the LDS performs one wavefront's operations in issue order, so only
+// the compiler has to keep the order (no memory access moves across this).
+__device__ __forceinline__ void wave_lds_order() { asm volatile("" ::: "memory"); }
+
+constexpr int kRunsBlock = 1024;  // 16 waves: 128 KiB of contributions + 16 x 2 KiB of staging fit the LDS
+constexpr uint32_t kRunStage = 64 * kRunGroup;  // staged run sums of a wave
+constexpr int kRunsAhead = 4;     // steps per wave whose loads are in flight
+// Idle store lanes write a dummy area of their own wave (64 values, 256
+// bytes apart): one common dummy address would funnel a large share of
+// all stores into one cache line of one L2 channel.
+constexpr uint32_t kRunDummyAreas = 4096;
+constexpr uint32_t kRunDummyStride = 256;
+
+// Cross-lane moves of the wave (DPP, no LDS): row_shr:n within rows of 16
+// lanes, the last lane of a row broadcast to the next row(s), the whole
+// wave shifted up one lane.  Lanes without a source read 0.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROW_MASK, 0xF, true);
+}
+
+// Inclusive sum over lanes 0..lane.
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v)
+{
+    v += dpp_u32<0x111>(v);  // row_shr:1
+    v += dpp_u32<0x112>(v);  // row_shr:2
+    v += dpp_u32<0x114>(v);  // row_shr:4
+    v += dpp_u32<0x118>(v);  // row_shr:8
+    v += dpp_u32<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_u32<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
+// One step of the run layout, in three parts so that consecutive steps
+// overlap (sum step k + 1, write out step k, stage step k + 1): each lane
+// sums the runs of its group of 8 src_local entries (runs never cross a
+// group, and every group ends with a run end) and stages each run sum in
+// LDS at the run's rank in the wave; the wave then writes the staged sums
+// in rank order, 64 consecutive ranks per store.  The runs of the lanes of
+// one (chunk, bin) segment have consecutive ranks and consecutive V
+// positions, so V position = rank + delta with delta constant over the
+// segment; a lane whose delta differs from the lane below leads a segment
+// (about 3 per step), and the write-out takes each rank's delta from the
+// last leader at or below it.  The write-out always issues the same 8
+// stores (ranks past the wave's runs store to the dummy offset), so the
+// loads of later steps stay in flight across them.
+struct RunStep {
+    float v[8];      // running sum after each entry (a run's sum at its end)
+    uint32_t endm;   // entries that end a run
+    uint32_t pre;    // rank of the lane's first run
+    uint32_t delta;  // V index - rank, constant over a segment
+    uint32_t total;  // runs of the wave
+    uint64_t lead;   // lanes that lead a segment
+};
+
+template <class T>
+__device__ __forceinline__ RunStep runs_sum(const T *cl, uint4 w, uint32_t vbase, uint32_t lane)
+{
+    RunStep st;
+    const uint32_t ew[4] = {w.x, w.y, w.z, w.w};
+    float x[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint32_t h = (i & 1) ? __builtin_amdgcn_ubfe(ew[i >> 1], 16, 15) : (ew[i >> 1] & (kRunEnd - 1u));
+        x[i] = f32_of(cl[h]);
+    }
+    st.endm = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) st.endm |= ((ew[i >> 1] >> ((i & 1) ? 31 : 15)) & 1u) << i;
+    const uint32_t cnt = (uint32_t)__popc(st.endm);
+    const uint32_t inc = wave_inclusive_sum(cnt);
+    st.pre = inc - cnt;
+    st.total = __builtin_amdgcn_readlane(inc, 63);
+    st.delta = vbase - st.pre;
+    st.lead = __ballot(lane == 0 || st.delta != dpp_u32<0x138>(st.delta));  // wave_shr:1
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        s += x[i];
+        st.v[i] = s;
+        s = ((st.endm >> i) & 1u) ? 0.f : s;
+    }
+    return st;
+}
+
+__device__ __forceinline__ void runs_stage(float *stage, const RunStep &st)
+{
+    uint32_t o = st.pre;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+        if ((st.endm >> i) & 1u) stage[o++] = st.v[i];
+}
+
+template <class T>
+__device__ __forceinline__ void runs_out(const float *stage, const RunStep &st, uint32_t dummy_off, uint32_t lane,
+                                         T *__restrict__ V)
+{
+    char *Vb = reinterpret_cast<char *>(V);
+    uint64_t lead = st.lead & (st.lead - 1);  // lane 0 leads the first segment
+    uint32_t cur = __builtin_amdgcn_readlane(st.delta, 0);
+    uint32_t nl = lead ? (uint32_t)__builtin_ctzll(lead) : 0u;
+    uint32_t np = lead ? __builtin_amdgcn_readlane(st.pre, nl) : 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t r = 0; r < kRunGroup; r++) {
+        const uint32_t q = lane + 64u * r;
+        uint32_t d = cur;
+        while (np < 64u * (r + 1)) {  // segments whose first rank falls in this store
+            const uint32_t dl = __builtin_amdgcn_readlane(st.delta, nl);
+            d = q >= np ? dl : d;
+            cur = dl;
+            lead &= lead - 1;
+            nl = lead ? (uint32_t)__builtin_ctzll(lead) : 0u;
+            np = lead ? __builtin_amdgcn_readlane(st.pre, nl) : 0xFFFFFFFFu;
+        }
+        const float v = stage[q];
+        put_f32(reinterpret_cast<T *>(Vb + (q < st.total ? (q + d) * (uint32_t)sizeof(T) : dummy_off)), 0, v);
+    }
+}
+
+// Run layout (float kinds): one block per source chunk (the chunks come in
+// plan order, the heaviest first); each wave takes steps of 64 consecutive
+// groups of 8 src_local entries.  The loads of the next kRunsAhead steps
+// are in flight while a step is summed; a wave always runs whole rounds of
+// kRunsAhead steps (steps past the chunk's last group load a valid group
+// and stage nothing).
+template <class T>
+__global__ void __launch_bounds__(kRunsBlock) bin_scatter_runs_kernel(RunsView S, const Slot *slots, uint32_t t,
+                                                                      const T *__restrict__ c_in, T *__restrict__ V)
+{
+    if (slots[t & 1].done) return;
+    const uint32_t k = blockIdx.x;
+    T *cl = reinterpret_cast<T *>(gbx_dyn_lds);
+    const uint32_t c0 = S.chunk_lo[k], n = S.chunk_lo[k + 1] - c0;
+    copy_to_lds<T, kRunsBlock>(cl, c_in + c0, n);
+    float *stage = reinterpret_cast<float *>(gbx_dyn_lds + S.stage_off) + (threadIdx.x >> 6) * kRunStage;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t dummy_off = S.dummy + ((blockIdx.x * (kRunsBlock / 64) + (threadIdx.x >> 6)) % kRunDummyAreas) *
+                                             kRunDummyStride + lane * (uint32_t)sizeof(T);
+    if (threadIdx.x == 0) cl[S.pad] = T(0);
+    __syncthreads();
+    const uint32_t g0 = S.chunk_grp[k], g1 = S.chunk_grp[k + 1];
+    if (g0 >= g1) return;
+    const uint32_t padw = S.pad | (S.pad << 16);
+    const uint4 padv = make_uint4(padw, padw, padw, padw);
+    const uint4 *src = reinterpret_cast<const uint4 *>(S.src_local);
+    uint4 w[kRunsAhead];
+    uint32_t vb[kRunsAhead];
+    // loads are always issued (clamped to the chunk's last group); lanes
+    // past it take pads when the step is summed
+    auto load = [&](int d, uint32_t first) {
+        const uint32_t g = min(first + lane, g1 - 1u);
+        w[d] = src[g];
+        vb[d] = S.vgrp[g];
+    };
+    // sum the step that starts at group `first` from slot d, then refill d
+    auto sum = [&](int d, uint32_t first) -> RunStep {
+        __builtin_amdgcn_sched_barrier(0);
+        const bool act = first + lane < g1;
+        const uint4 cw = act ? w[d] : padv;
+        const uint32_t cv = act ? vb[d] : 0u;  // both slot registers are free for the next load
+        __builtin_amdgcn_sched_barrier(0);
+        load(d, first + kRunsAhead * kRunsBlock);
+        return runs_sum<T>(cl, cw, cv, lane);
+    };
+    uint32_t r = g0 + (threadIdx.x & ~63u);
+    if (r >= g1) return;
+#pragma unroll
+    for (int d = 0; d < kRunsAhead; d++) {
+        load(d, r + (uint32_t)d * kRunsBlock);  // in slot order: slot 0 is waited for first
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    RunStep cur = sum(0, r);
+    runs_stage(stage, cur);
+    // a round: for each of its steps, sum the next step, write out this
+    // one, stage the next
+    auto round = [&](uint32_t rr) {
+#pragma unroll
+        for (int d = 0; d < kRunsAhead; d++) {
+            const RunStep nxt = sum((d + 1) % kRunsAhead, rr + (uint32_t)(d + 1) * kRunsBlock);
+            wave_lds_order();
+            runs_out<T>(stage, cur, dummy_off, lane, V);
+            wave_lds_order();
+            runs_stage(stage, nxt);
+            cur = nxt;
+        }
+    };
+    // the first round apart: every entry to the loop then follows a round of
+    // the same loads and stores, and the waits inside it stay as short
+    round(r);
+    for (r += kRunsAhead * kRunsBlock; r < g1; r += kRunsAhead * kRunsBlock) round(r);
 }
 
 struct AccumView {
@@ -660,11 +981,88 @@ static int find_runs(gbx_ctx *ctx, const uint32_t *key, uint32_t E, uint32_t pad
     return GBX_OK;
 }
 
-static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H, uint32_t hot_elem, gbx_bplan *bp)
+// Run layout of the cold edges (rowid_c, col_c: plan edge order, so the
+// edges of one row are consecutive in every segment): src_local chunk-major
+// with segments padded to 8 entries (pads: the LDS index bp->SC), V
+// bin-major, within a bin chunk-major, one value per run, segments padded
+// to 8 values (zeros, never written by the scatter).
+static int build_runs(gbx_ctx *ctx, gbx_bplan *bp, uint32_t Ecold, const uint32_t *rowid_c, const uint32_t *col_c,
+                      const uint32_t *bin_of, const uint32_t *chunk_of, const uint32_t *bin_row0,
+                      const uint32_t *cstart, uint32_t *bin_len)
+{
+    const uint32_t NB = bp->NB, NC = bp->NC;
+    const uint32_t kbits = bits_for(NB * NC - 1);
+    const unsigned gE = grid_for(Ecold, 256, 16384);
+    DevBuf key, pay;
+    GBX_TRY(dev_alloc(ctx, key, (size_t)Ecold * 4));
+    GBX_TRY(dev_alloc(ctx, pay, (size_t)Ecold * 4));
+    bin_keys<<<gE, 256, 0, ctx->stream>>>(rowid_c, col_c, Ecold, bin_of, chunk_of, NB, NC, 1, key.as<uint32_t>(),
+                                          pay.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    GBX_TRY(radix_sort_pairs(ctx, key.as<uint32_t>(), pay.as<uint32_t>(), nullptr, Ecold, kbits));
+    Runs R;
+    GBX_TRY(find_runs(ctx, key.as<uint32_t>(), Ecold, kRunGroup, R));
+    bp->nseg = R.n;
+    bp->Plen = R.padded;
+    chunk_groups<<<grid_for(NC + 1, 256, 1024), 256, 0, ctx->stream>>>(
+        cstart, NC, R.idx.as<uint32_t>(), Ecold, R.n, R.pstart.as<uint32_t>(), 3u, bp->chunk_grp.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    // run starts and their running count
+    DevBuf head, hx, htot;
+    GBX_TRY(dev_alloc(ctx, head, (size_t)Ecold * 4 + 4));
+    GBX_TRY(dev_alloc(ctx, hx, (size_t)(Ecold + 1) * 4));
+    GBX_TRY(dev_alloc(ctx, htot, 16));
+    run_heads<<<gE, 256, 0, ctx->stream>>>(pay.as<uint32_t>(), Ecold, R.flag.as<uint32_t>(), R.idx.as<uint32_t>(),
+                                           R.start.as<uint32_t>(), R.pstart.as<uint32_t>(), rowid_c,
+                                           head.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    GBX_TRY(exclusive_scan_u32(ctx, head.as<uint32_t>(), hx.as<uint32_t>(), Ecold, htot.as<uint32_t>()));
+    GBX_HIP(ctx, hipMemcpyAsync(hx.as<uint32_t>() + Ecold, htot.p, 4, hipMemcpyDeviceToDevice, ctx->stream));
+    // segments: V lengths, then V offsets in bin-major order
+    const uint32_t ns = R.n;
+    DevBuf vlen, skey, spay, vsorted, osorted, voff, vtot;
+    for (DevBuf *d : {&vlen, &skey, &spay, &vsorted, &osorted, &voff}) GBX_TRY(dev_alloc(ctx, *d, (size_t)ns * 4 + 4));
+    GBX_TRY(dev_alloc(ctx, vtot, 16));
+    const unsigned gS = grid_for(ns, 256, 16384);
+    run_segments<<<gS, 256, 0, ctx->stream>>>(key.as<uint32_t>(), R.start.as<uint32_t>(), ns, Ecold,
+                                              hx.as<uint32_t>(), NB, NC, vlen.as<uint32_t>(), skey.as<uint32_t>(),
+                                              spay.as<uint32_t>(), bin_len);
+    GBX_HIP(ctx, hipGetLastError());
+    GBX_TRY(radix_sort_pairs(ctx, skey.as<uint32_t>(), spay.as<uint32_t>(), nullptr, ns, kbits));
+    gather_u32<<<gS, 256, 0, ctx->stream>>>(vlen.as<uint32_t>(), spay.as<uint32_t>(), ns, vsorted.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    GBX_TRY(exclusive_scan_u32(ctx, vsorted.as<uint32_t>(), osorted.as<uint32_t>(), ns, vtot.as<uint32_t>()));
+    scatter_u32<<<gS, 256, 0, ctx->stream>>>(osorted.as<uint32_t>(), spay.as<uint32_t>(), ns, voff.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    uint32_t Vlen = 0;
+    GBX_HIP(ctx, hipMemcpyAsync(&Vlen, vtot.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // runs <= entries in every segment, both padded to 8
+    if (Vlen > bp->Plen) return set_error(ctx, GBX_ERR_STATE, "binned layout: %u runs for %u entries", Vlen, bp->Plen);
+    bp->Vlen = Vlen;
+    GBX_TRY(dev_alloc(ctx, bp->src_local, (size_t)bp->Plen * 2 + 16));
+    GBX_TRY(dev_alloc(ctx, bp->dst_local, (size_t)Vlen * 2 + 16));
+    GBX_TRY(dev_alloc(ctx, bp->vgrp, (size_t)(bp->Plen / kRunGroup) * 4 + 16));
+    GBX_TRY(dev_alloc(ctx, bp->bdst, 16));
+    GBX_TRY(fill_u32(ctx, bp->src_local.as<uint32_t>(), bp->SC | (bp->SC << 16), bp->Plen / 2 + 4));
+    GBX_HIP(ctx, hipMemsetAsync(bp->dst_local.p, 0, (size_t)Vlen * 2 + 16, ctx->stream));
+    GBX_HIP(ctx, hipMemsetAsync(bp->vgrp.p, 0, (size_t)(bp->Plen / kRunGroup) * 4 + 16, ctx->stream));
+    run_fill<<<gE, 256, 0, ctx->stream>>>(key.as<uint32_t>(), pay.as<uint32_t>(), Ecold, R.flag.as<uint32_t>(),
+                                          R.idx.as<uint32_t>(), R.start.as<uint32_t>(), R.pstart.as<uint32_t>(),
+                                          head.as<uint32_t>(), hx.as<uint32_t>(), voff.as<uint32_t>(), rowid_c, col_c,
+                                          bp->chunk_lo.as<uint32_t>(), bin_row0, NB, bp->SC, bp->src_local.as<uint16_t>(),
+                                          bp->dst_local.as<uint16_t>(), bp->vgrp.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GBX_OK;
+}
+
+static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H, uint32_t hot_elem, bool runs,
+                            gbx_bplan *bp)
 {
     const gbx_tuning &tu = ctx->tune;
     const uint32_t N = p->N, E = p->E;
-    const uint32_t EC = cut_edges_of(tu.binned_chunk_edges, ctx, E), SC = chunk_rows_of(tu, hot_elem);
+    const uint32_t EC = cut_edges_of(tu.binned_chunk_edges, ctx, E), SC = chunk_sources_of(tu, hot_elem, runs);
     // rows per bin: what the LDS holds next to H hot contributions of hot_elem
     // bytes (a multiple of 8)
     const uint32_t SB =
@@ -673,6 +1071,8 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     if (SB < 8) return set_error(ctx, GBX_ERR_ARG, "binned layout: %u hot sources leave no room for bins", H);
     bp->H = H;
     bp->SB = SB;
+    bp->SC = SC;
+    bp->runs = runs;
     std::vector<uint32_t> rp(N + 1), colcnt(N);
     GBX_HIP(ctx, hipMemcpyAsync(rp.data(), p->rp.p, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
     DevBuf dcnt;
@@ -808,7 +1208,11 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     } else {
         GBX_TRY(dev_alloc(ctx, bp->hot_pairs, 16));
     }
-    if (Ecold) {
+    if (Ecold && runs) {
+        GBX_TRY(build_runs(ctx, bp, Ecold, rowid_c.as<uint32_t>(), col_c.as<uint32_t>(), d_bin_of.as<uint32_t>(),
+                           d_chunk_of.as<uint32_t>(), d_bin_row0.as<uint32_t>(), d_cstart.as<uint32_t>(),
+                           d_bin_len.as<uint32_t>()));
+    } else if (Ecold) {
         const uint32_t kbits = bits_for(NB * NC - 1);
         DevBuf key, pay, voff_table;
         GBX_TRY(dev_alloc(ctx, key, (size_t)Ecold * 4));
@@ -856,7 +1260,7 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
                 key.as<uint32_t>(), R.start.as<uint32_t>(), R.pstart.as<uint32_t>(), R.plen.as<uint32_t>(), R.n,
                 voff_table.as<uint32_t>(), bp->bdst.as<uint32_t>());
             chunk_groups<<<grid_for(NC + 1, 256, 1024), 256, 0, ctx->stream>>>(
-                d_cstart.as<uint32_t>(), NC, R.idx.as<uint32_t>(), Ecold, R.n, R.pstart.as<uint32_t>(),
+                d_cstart.as<uint32_t>(), NC, R.idx.as<uint32_t>(), Ecold, R.n, R.pstart.as<uint32_t>(), 3u,
                 bp->chunk_grp.as<uint32_t>());
             GBX_HIP(ctx, hipGetLastError());
             GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -865,6 +1269,7 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
         GBX_TRY(dev_alloc(ctx, bp->dst_local, 16));
         GBX_TRY(dev_alloc(ctx, bp->src_local, 16));
         GBX_TRY(dev_alloc(ctx, bp->bdst, 16));
+        GBX_TRY(dev_alloc(ctx, bp->vgrp, 16));
         GBX_TRY(fill_u32(ctx, bp->chunk_grp.as<uint32_t>(), 0, NC + 1));
     }
     // units: V range and hot-pair range of each bin (a single row longer than EB is split)
@@ -912,19 +1317,12 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     return GBX_OK;
 }
 
-static int build_bplan(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H, uint32_t hot_elem, gbx_bplan **out)
+static int build_bplan(gbx_ctx *ctx, gbx_plan *p, const uint32_t (&cut)[8], gbx_bplan **out)
 {
     gbx_bplan *bp = new (std::nothrow) gbx_bplan();
     if (!bp) return set_error(ctx, GBX_ERR_NOMEM, "binned layout: host memory");
-    const gbx_tuning &tu = ctx->tune;
-    bp->cut[0] = cut_edges_of(tu.binned_chunk_edges, ctx, p->E);
-    bp->cut[1] = (std::max(cut_edges_of(tu.binned_bin_edges, ctx, p->E), 8u) + 7u) & ~7u;
-    bp->cut[2] = std::min(tu.binned_bin_rows, kMaxBinRows);
-    bp->cut[3] = chunk_rows_of(tu, hot_elem);
-    bp->cut[4] = pad;
-    bp->cut[5] = H;
-    bp->cut[6] = hot_elem;
-    const int rc = build_bplan_into(ctx, p, pad, H, hot_elem, bp);
+    memcpy(bp->cut, cut, sizeof(bp->cut));
+    const int rc = build_bplan_into(ctx, p, cut[4], cut[5], cut[6], cut[7] != 0, bp);
     if (rc) {
         delete bp;
         return rc;
@@ -959,6 +1357,7 @@ constexpr int scatter_qk() { return is_quant_kind(KIND) ? KIND : 0; }
 template <int KIND, class T>
 static void grant_kind(const gbx_binned_state *st)
 {
+    if constexpr (KIND == K_PR_REF || KIND == K_PR_STOCH) grant_lds(&bin_scatter_runs_kernel<T>, st->shm_scatter);
     grant_lds(&bin_scatter_kernel<UintOf<sizeof(T)>, scatter_qk<KIND>()>, st->shm_scatter);
     grant_lds(&bin_accum_kernel<KIND, T>, st->shm_accum);
 }
@@ -980,10 +1379,13 @@ int binned_create(gbx_pr *pr)
     gbx_ctx *ctx = pr->ctx;
     gbx_plan *p = pr->plan;
     const gbx_tuning &tu = ctx->tune;
-    // segments padded to 32 values (a multiple of 32 is required by the
-    // per-block V positions): whole 128-byte lines of FP32 V, 64-byte halves
-    // for BF16 (measured as fast as whole lines)
-    const uint32_t pad = 32;
+    // The float kinds sum the cold edges of one chunk into one row in the
+    // scatter (run layout; segments padded to 8).  The integer kinds keep
+    // one V value per edge: segments padded to 32 values (a multiple of 32 is
+    // required by the per-block V positions), whole 128-byte lines of FP32
+    // V, 64-byte halves of 16-bit values (measured as fast as whole lines).
+    const bool runs = (pr->kind == K_PR_REF || pr->kind == K_PR_STOCH) && tu.binned_edge_values == 0;
+    const uint32_t pad = runs ? 8 : 32;
     // hot sources: the tuning value for contributions of up to 4 bytes (one
     // layout for FP32, BF16 and the 32/16/8-bit fixed-point kinds), half as
     // many 8-byte ones in the same LDS; a multiple of 8, at most N and 65528
@@ -991,15 +1393,20 @@ int binned_create(gbx_pr *pr)
     const size_t vel = value_bytes(pr);
     const uint32_t hot_elem = std::max<uint32_t>((uint32_t)vel, 4u);
     const uint32_t H = std::min(std::min(tu.binned_hot_sources * 4u / hot_elem, 65528u), p->N) & ~7u;
-    const uint32_t want[7] = {cut_edges_of(tu.binned_chunk_edges, ctx, p->E),
+    const uint32_t want[8] = {cut_edges_of(tu.binned_chunk_edges, ctx, p->E),
                               (std::max(cut_edges_of(tu.binned_bin_edges, ctx, p->E), 8u) + 7u) & ~7u,
-                              std::min(tu.binned_bin_rows, kMaxBinRows), chunk_rows_of(tu, hot_elem), pad, H, hot_elem};
+                              std::min(tu.binned_bin_rows, kMaxBinRows),
+                              chunk_sources_of(tu, hot_elem, runs),
+                              pad,
+                              H,
+                              hot_elem,
+                              runs ? 1u : 0u};
     std::shared_ptr<gbx_bplan> bp;
     for (auto &c : p->bplans)
         if (memcmp(c->cut, want, sizeof(want)) == 0) bp = c;
     if (!bp) {
         gbx_bplan *raw = nullptr;
-        GBX_TRY(build_bplan(ctx, p, pad, H, hot_elem, &raw));
+        GBX_TRY(build_bplan(ctx, p, want, &raw));
         bp = std::shared_ptr<gbx_bplan>(raw);
         p->bplans.push_back(bp);
     }
@@ -1007,7 +1414,14 @@ int binned_create(gbx_pr *pr)
     if (!st) return set_error(ctx, GBX_ERR_NOMEM, "binned state: host memory");
     pr->bin = st;
     st->bp = bp;
-    GBX_TRY(dev_alloc(ctx, st->V, (size_t)bp->Vlen * vel + 64));
+    // run layout: the dummy areas of the scatter's idle store lanes follow V
+    const size_t vbytes = ((size_t)bp->Vlen * vel + 64 + kRunDummyStride - 1) / kRunDummyStride * kRunDummyStride;
+    if (bp->runs && vbytes + (size_t)kRunDummyAreas * kRunDummyStride > 0xFFFFFFFFull)
+        return set_error(ctx, GBX_ERR_RANGE, "binned layout: V of %zu bytes beyond 32-bit offsets", vbytes);
+    st->dummy_off = (uint32_t)vbytes;
+    GBX_TRY(dev_alloc(ctx, st->V, vbytes + (bp->runs ? (size_t)kRunDummyAreas * kRunDummyStride : 0)));
+    // the run layout's pads are never written: zero once
+    if (bp->runs) GBX_HIP(ctx, hipMemsetAsync(st->V.p, 0, vbytes, ctx->stream));
     GBX_TRY(dev_alloc(ctx, st->acc, (size_t)bp->n_multi * 8 + 8));
     GBX_TRY(dev_alloc(ctx, st->cnt, (size_t)bp->n_multi * 4 + 4));
     GBX_HIP(ctx, hipMemsetAsync(st->acc.p, 0, (size_t)bp->n_multi * 8 + 8, ctx->stream));
@@ -1018,7 +1432,11 @@ int binned_create(gbx_pr *pr)
     st->accum_grid = std::max(1u, std::min(bp->NU, (uint32_t)std::max(1, ctx->num_cus)));
     if (bp->NU > 16384) GBX_TRY(dev_alloc(ctx, pr->partials, (size_t)bp->NU * sizeof(Partial)));
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    st->shm_scatter = (size_t)bp->cut[3] * vel;
+    // run layout: the chunk's contributions, a zero at index SC, then the
+    // per-wave staging of run sums and V positions
+    st->stage_off = bp->runs ? (uint32_t)((((size_t)bp->SC + 1) * vel + 15) & ~(size_t)15) : 0;
+    st->shm_scatter = bp->runs ? st->stage_off + (size_t)(kRunsBlock / 64) * kRunStage * sizeof(float)
+                               : (size_t)bp->cut[3] * vel;
     st->shm_accum = (size_t)bp->SB * 8 + (size_t)bp->H * vel;
     switch (pr->kind) {
     case K_PR_REF: pr->elem == 4 ? grant_kind<K_PR_REF, float>(st) : grant_kind<K_PR_REF, uint16_t>(st); break;
@@ -1047,7 +1465,23 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
     S.src_local = bp->src_local.as<uint16_t>();
     S.bdst = bp->bdst.as<uint32_t>();
     S.NC = bp->NC;
-    if (bp->NC)
+    if constexpr (KIND == K_PR_REF || KIND == K_PR_STOCH) {
+        if (bp->runs) {
+            RunsView RV;
+            RV.chunk_lo = S.chunk_lo;
+            RV.chunk_grp = S.chunk_grp;
+            RV.src_local = S.src_local;
+            RV.vgrp = bp->vgrp.as<uint32_t>();
+            RV.pad = bp->SC;
+            RV.stage_off = st->stage_off;
+            RV.dummy = st->dummy_off;
+            if (bp->NC)
+                bin_scatter_runs_kernel<T><<<bp->NC, kRunsBlock, st->shm_scatter, stream>>>(
+                    RV, A.slots, A.t, reinterpret_cast<const T *>(A.c_in), st->V.as<T>());
+            S.NC = 0;
+        }
+    }
+    if (S.NC)
         bin_scatter_kernel<U, scatter_qk<KIND>()><<<bp->NC, kBlock, st->shm_scatter, stream>>>(
             S, A.slots, A.t, A.c_in, st->V.as<U>());
     AccumView B;

@@ -262,7 +262,7 @@ bool tuning_valid(const gbx_tuning *t)
     if (t->binned_bin_edges != 0 && t->binned_bin_edges < 8) return false;
     if (t->binned_bin_rows < 1 || t->binned_bin_rows > 20000) return false;
     if (t->binned_chunk_rows < 1 || t->binned_chunk_rows > 32768) return false;
-    if (t->binned_hot_sources > 65528) return false;
+    if (t->binned_hot_sources > 65528 || t->binned_edge_values > 1) return false;
     const uint32_t v = t->ppr_vectors_per_lane;
     return v == 1 || v == 2 || v == 4;
 }
@@ -289,7 +289,8 @@ static void tuning_from_env(gbx_tuning *t)
                 {"GBX_BINNED_BIN_EDGES", &c.binned_bin_edges},
                 {"GBX_BINNED_BIN_ROWS", &c.binned_bin_rows},
                 {"GBX_BINNED_CHUNK_ROWS", &c.binned_chunk_rows},
-                {"GBX_BINNED_HOT_SOURCES", &c.binned_hot_sources}};
+                {"GBX_BINNED_HOT_SOURCES", &c.binned_hot_sources},
+                {"GBX_BINNED_EDGE_VALUES", &c.binned_edge_values}};
     for (auto &v : vars)
         if (const char *s = getenv(v.name)) *v.field = (uint32_t)strtoul(s, nullptr, 10);
     if (tuning_valid(&c)) *t = c;

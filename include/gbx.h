@@ -328,7 +328,7 @@ int gbx_graph_batch_info(const gbx_graph *g, uint32_t *num_graphs, uint32_t *ver
  * GBX_BLOCKS_PER_CU, GBX_PPR_HUB, GBX_PPR_CHUNK, GBX_PPR_BLOCKS_PER_CU,
  * GBX_PPR_VPL, GBX_BINNED_MIN_VERTICES, GBX_BINNED_CHUNK_EDGES,
  * GBX_BINNED_BIN_EDGES, GBX_BINNED_BIN_ROWS, GBX_BINNED_CHUNK_ROWS,
- * GBX_BINNED_HOT_SOURCES).  A change applies to plans and objects created afterwards;
+ * GBX_BINNED_HOT_SOURCES, GBX_BINNED_EDGE_VALUES).  A change applies to plans and objects created afterwards;
  * results stay within the same tolerances for every setting. */
 typedef struct gbx_tuning {
     uint32_t bin_thresholds[7];    /* degree thresholds: hub, then 64, 32, 16, 8, 4, 2 lanes per row */
@@ -345,11 +345,14 @@ typedef struct gbx_tuning {
     uint32_t binned_bin_edges;     /* binned: in-edges per destination bin (one accumulate block each);
                                       0 = as binned_chunk_edges */
     uint32_t binned_bin_rows;      /* binned: destination rows per bin (<= 20000: LDS int64 sums) */
-    uint32_t binned_chunk_rows;    /* binned: sources per chunk (<= 32768: LDS contributions) */
+    uint32_t binned_chunk_rows;    /* binned: sources per chunk (<= 32768: LDS contributions; 32736 with the run layout) */
     uint32_t binned_hot_sources;   /* binned: the first this many sources (plan order, the most
                                       gathered) are summed from LDS in the accumulate pass, their
                                       edges bypassing the scatter (0 = off; <= 65528) */
-    uint32_t reserved[2];
+    uint32_t binned_edge_values;   /* binned, FP32/BF16 kinds: 0 = the scatter sums the edges of one
+                                      source chunk into one row and stores one value per such run
+                                      (default); 1 = one value per edge */
+    uint32_t reserved[1];
 } gbx_tuning;
 
 int gbx_tuning_get(const gbx_ctx *ctx, gbx_tuning *t);

@@ -1,8 +1,10 @@
 """Binned (propagation-blocking) PageRank iteration: same parity bars as the
 pull iteration (<= 1e-5 L1 vs the reference's FP32 ranks and vs FP64; BF16
-<= 2e-3), deterministic (exact fixed-point row sums), and the same results
-under every layout cut (tiny chunks and bins force thousands of segments
-and giant rows split over several accumulate blocks)."""
+<= 2e-3), deterministic (exact fixed-point row sums of a fixed layout), and
+the same results under every layout cut (tiny chunks and bins force
+thousands of segments and giant rows split over several accumulate blocks):
+bit-identical with one V value per edge, within FP32 rounding of the run
+sums (<= 1e-6 L1) with the run layout."""
 import numpy as np
 import pytest
 
@@ -96,22 +98,56 @@ def test_binned_stochastic_matches_fp64(gpu_ctx, golden, name):
     assert l1(ranks, r64) <= TOL_F32
 
 
+def _edge_values(ctx, on):
+    t = ctx.tuning()
+    t.binned_edge_values = int(on)
+    ctx.set_tuning(t)
+
+
 @pytest.mark.parametrize("name", ["rmat_s12", "fixture_v8192_e32768", "fixture_v500_e500"])
 def test_binned_small_cut_layout(small_cut_ctx, golden, name):
     """Thousands of segments, bins of <= 37 rows, rows split over several
-    accumulate blocks: same ranks as the default cut (exact integer sums
-    make them bit-identical whenever no sum differs, which is always)."""
+    accumulate blocks.  Run layout: the cut changes which edges one FP32 run
+    sum takes, so ranks agree with the default cut's to rounding.  One V
+    value per edge: exact integer sums make them bit-identical."""
     z = golden(name)
     print(f"[gpu] binned small cut {name}", flush=True)
     g = _graph(small_cut_ctx, z)
     ranks, _ = g.pagerank(iterations=20, epsilon=0.0, method=BINNED)
     assert l1(ranks, z["pr_r20"]) <= TOL_F32
+    _edge_values(small_cut_ctx, True)
+    ranks_e, _ = g.pagerank(iterations=20, epsilon=0.0, method=BINNED)
+    assert l1(ranks_e, z["pr_r20"]) <= TOL_F32
     g2 = gbx.Context(0)
     try:
-        ref, _ = _graph(g2, z).pagerank(iterations=20, epsilon=0.0, method=BINNED)
+        g2g = _graph(g2, z)
+        ref, _ = g2g.pagerank(iterations=20, epsilon=0.0, method=BINNED)
+        _edge_values(g2, True)
+        ref_e, _ = g2g.pagerank(iterations=20, epsilon=0.0, method=BINNED)
     finally:
         g2.close()
-    np.testing.assert_array_equal(ranks, ref)
+    assert l1(ranks, ref) <= 1e-6
+    np.testing.assert_array_equal(ranks_e, ref_e)
+
+
+@pytest.mark.parametrize("precision, tol", [(gbx.F32, 1e-6), (gbx.BF16, 2e-3)])
+def test_binned_run_layout_matches_edge_values(gpu_ctx, golden, precision, tol):
+    """The run layout (one V value per run of edges from one chunk into one
+    row, summed in the scatter) against one V value per edge, same graph:
+    REF and stochastic ranks agree to the rounding of the run sums."""
+    z = golden("rmat_s12")
+    g = _graph(gpu_ctx, z)
+    out = {}
+    for ev in (0, 1):
+        _edge_values(gpu_ctx, ev)
+        for mode in (gbx.PR_REF, gbx.PR_STOCHASTIC):
+            out[ev, mode], _ = g.pagerank(iterations=20, epsilon=0.0, mode=mode, precision=precision,
+                                          method=BINNED)
+    _edge_values(gpu_ctx, 0)
+    for mode in (gbx.PR_REF, gbx.PR_STOCHASTIC):
+        e = l1(out[0, mode], out[1, mode])
+        print(f"[gpu] run layout vs edge values, precision {precision} mode {mode}: L1 {e:.3e}")
+        assert e <= tol
 
 
 def test_binned_deterministic_and_graph_replay(gpu_ctx, golden):
