@@ -86,13 +86,18 @@ inline uint32_t chunk_sources_of(const gbx_tuning &tu, uint32_t elem, bool runs)
     return runs ? std::min(sc, kMaxRunChunkRows) : sc;
 }
 
-// Edges per chunk and per bin: the tuning value, or (0) E / CUs clamped to
-// [64K, 512K] (measured on MI355X: 256K best at R-MAT s22, 512K at s24).
-inline uint32_t cut_edges_of(uint32_t v, const gbx_ctx *ctx, uint32_t E)
+// Edges per chunk and per bin: the tuning value, or (0) automatic, measured
+// on MI355X.  One value per edge: E / CUs clamped to [64K, 512K] (256K best
+// at R-MAT s22, 512K at s24).  Run layout: 7/8 of E / CUs clamped to
+// [64K, 1M] (bigger chunks merge more edges into runs; 0.53 ms at s24 for
+// 768K-896K against 0.59 for 512K and 0.55 for 1M; 192K-256K at s22, 64K at
+// s20).
+inline uint32_t cut_edges_of(uint32_t v, const gbx_ctx *ctx, uint32_t E, bool runs)
 {
     if (v) return v;
-    const uint32_t cus = (uint32_t)std::max(1, ctx->num_cus);
-    return std::min(1u << 19, std::max(1u << 16, E / cus));
+    const uint32_t per_cu = E / (uint32_t)std::max(1, ctx->num_cus);
+    if (runs) return std::min(1u << 20, std::max(1u << 16, (uint32_t)((uint64_t)per_cu * 7 / 8)));
+    return std::min(1u << 19, std::max(1u << 16, per_cu));
 }
 
 struct BinUnit {
@@ -1062,12 +1067,12 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
 {
     const gbx_tuning &tu = ctx->tune;
     const uint32_t N = p->N, E = p->E;
-    const uint32_t EC = cut_edges_of(tu.binned_chunk_edges, ctx, E), SC = chunk_sources_of(tu, hot_elem, runs);
+    const uint32_t EC = cut_edges_of(tu.binned_chunk_edges, ctx, E, runs), SC = chunk_sources_of(tu, hot_elem, runs);
     // rows per bin: what the LDS holds next to H hot contributions of hot_elem
     // bytes (a multiple of 8)
     const uint32_t SB =
         std::min(std::min(tu.binned_bin_rows, kMaxBinRows), (kAccumLds - H * hot_elem) / 8u) & ~7u;
-    const uint32_t EB = (std::max(cut_edges_of(tu.binned_bin_edges, ctx, E), 8u) + 7u) & ~7u;
+    const uint32_t EB = (std::max(cut_edges_of(tu.binned_bin_edges, ctx, E, runs), 8u) + 7u) & ~7u;
     if (SB < 8) return set_error(ctx, GBX_ERR_ARG, "binned layout: %u hot sources leave no room for bins", H);
     bp->H = H;
     bp->SB = SB;
@@ -1393,8 +1398,8 @@ int binned_create(gbx_pr *pr)
     const size_t vel = value_bytes(pr);
     const uint32_t hot_elem = std::max<uint32_t>((uint32_t)vel, 4u);
     const uint32_t H = std::min(std::min(tu.binned_hot_sources * 4u / hot_elem, 65528u), p->N) & ~7u;
-    const uint32_t want[8] = {cut_edges_of(tu.binned_chunk_edges, ctx, p->E),
-                              (std::max(cut_edges_of(tu.binned_bin_edges, ctx, p->E), 8u) + 7u) & ~7u,
+    const uint32_t want[8] = {cut_edges_of(tu.binned_chunk_edges, ctx, p->E, runs),
+                              (std::max(cut_edges_of(tu.binned_bin_edges, ctx, p->E, runs), 8u) + 7u) & ~7u,
                               std::min(tu.binned_bin_rows, kMaxBinRows),
                               chunk_sources_of(tu, hot_elem, runs),
                               pad,

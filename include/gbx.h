@@ -341,7 +341,8 @@ typedef struct gbx_tuning {
     uint32_t ppr_vectors_per_lane; /* batched form: contributions per lane (1, 2 or 4) */
     uint32_t binned_min_vertices;  /* GBX_METHOD_AUTO picks the binned method from this many vertices */
     uint32_t binned_chunk_edges;   /* binned: out-edges per source chunk (one scatter block each);
-                                      0 = E / CUs clamped to [65536, 524288] */
+                                      0 = E / CUs clamped to [65536, 524288]; with the run
+                                      layout 7/8 of E / CUs clamped to [65536, 1048576] */
     uint32_t binned_bin_edges;     /* binned: in-edges per destination bin (one accumulate block each);
                                       0 = as binned_chunk_edges */
     uint32_t binned_bin_rows;      /* binned: destination rows per bin (<= 20000: LDS int64 sums) */
