@@ -10,9 +10,10 @@ gather per edge) is reported beside it, with the BF16 and 32-vector forms.
 
 Prints one JSON line: GTEPS (E edges per iteration / wall time per
 iteration), the HBM roofline of the iteration (algorithmic bytes of the pull
-formulation, 8E + 16N + 4, over the mean HIP-event duration of the
-iteration's kernels) and the GraphBlox reference's OpenMP pull PageRank timed
-on this host's cores on the same graph.
+formulation, 8E + 16N + 4, over the device time per iteration between the
+HIP events that bracket the timed iterations on the library's stream) and
+the GraphBlox reference's OpenMP pull PageRank timed on this host's cores on
+the same graph.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--scale 24]
 """
@@ -84,10 +85,27 @@ def cpu_baseline(g, iterations: int, threads: int, gpu_ranks=None):
     return rec
 
 
+def timed_steps(pr, steps):
+    """The timed region, after a warm-up: reset the session (its start event
+    is recorded at the next launch), launch `steps` iterations back to back
+    as a user runs them.  Returns (device ms per iteration between the HIP
+    events the library records on its stream before the first and after the
+    last launch, host wall ms per iteration, ranks, report)."""
+    ctx = pr.ctx
+    pr.reset()
+    ctx.sync()
+    t0 = time.perf_counter()
+    pr.launch(steps)
+    ctx.sync()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    ranks, rep = pr.read()
+    rep0 = rep[0] if isinstance(rep, list) else rep
+    return rep0["time_total_s"] * 1e3 / steps, wall_ms, ranks, rep
+
+
 def time_form(g, warmup, steps, **kw):
-    """Warm up, then time `steps` iterations: per-iteration HIP-event kernel
-    mean and host wall time.  Returns (kernel_ms, wall_ms, session)."""
-    ctx = g.ctx
+    """Warm up, then time `steps` iterations (timed_steps).  Returns
+    (kernel_ms, wall_ms, session)."""
     k = kw.pop("num_vectors", 0)
     seeds = kw.pop("seed_sets", None)
     if k:
@@ -96,11 +114,7 @@ def time_form(g, warmup, steps, **kw):
         pr = g.session(iterations=warmup + steps, epsilon=0.0, stop=2, **kw)
     pr.reset()
     pr.launch(warmup)
-    ctx.sync()
-    t0 = time.perf_counter()
-    kernel_ms = pr.launch_timed(steps)
-    ctx.sync()
-    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    kernel_ms, wall_ms, _, _ = timed_steps(pr, steps)
     return kernel_ms, wall_ms, pr
 
 
@@ -167,15 +181,9 @@ def main():
     log(f"{args.method} iteration set up in {time.perf_counter() - t0:.2f}s")
     pr.reset()
     pr.launch(args.warmup)
-    ctx.sync()
-    t0 = time.perf_counter()
-    kernel_ms = pr.launch_timed(args.steps)  # HIP events around every iteration's kernels; syncs at the end
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    ranks, rep = pr.read()
-    assert rep["executed"] == args.warmup + args.steps
+    kernel_ms, ms_per_step, ranks, rep = timed_steps(pr, args.steps)
+    assert rep["executed"] == args.steps
     assert np.isfinite(ranks).all()
-    ms_per_step = wall * 1e3 / args.steps
     value = E / (ms_per_step * 1e-3) / 1e9
     bytes_per_launch = 8 * E + 16 * N + 4
     achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9

@@ -502,6 +502,10 @@ __device__ __forceinline__ void wave_lds_order() { asm volatile("" ::: "memory")
 constexpr int kRunsBlock = 1024;  // 16 waves: 128 KiB of contributions + 16 x 2 KiB of staging fit the LDS
 constexpr uint32_t kRunStage = 64 * kRunGroup;  // staged run sums of a wave
 constexpr int kRunsAhead = 4;     // steps per wave whose loads are in flight
+// Stores every step issues (384 ranks; a step has about 300 runs at R-MAT
+// s24).  Measured: 6 always and the rest on demand beats 8 always by 4 %,
+// 3-5 by 2 %.
+constexpr uint32_t kRunStores = 6;
 // Idle store lanes write a dummy area of their own wave (64 values, 256
 // bytes apart): one common dummy address would funnel a large share of
 // all stores into one cache line of one L2 channel.
@@ -539,9 +543,9 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v)
 // positions, so V position = rank + delta with delta constant over the
 // segment; a lane whose delta differs from the lane below leads a segment
 // (about 3 per step), and the write-out takes each rank's delta from the
-// last leader at or below it.  The write-out always issues the same 8
-// stores (ranks past the wave's runs store to the dummy offset), so the
-// loads of later steps stay in flight across them.
+// last leader at or below it.  The write-out always issues at least
+// kRunStores stores (ranks past the wave's runs store to the wave's dummy
+// area), so the loads of later steps stay in flight across them.
 struct RunStep {
     float v[8];      // running sum after each entry (a run's sum at its end)
     uint32_t endm;   // entries that end a run
@@ -600,6 +604,9 @@ __device__ __forceinline__ void runs_out(const float *stage, const RunStep &st, 
     uint32_t np = lead ? __builtin_amdgcn_readlane(st.pre, nl) : 0xFFFFFFFFu;
 #pragma unroll
     for (uint32_t r = 0; r < kRunGroup; r++) {
+        // the first kRunStores stores always (their count is what the loads'
+        // waits assume), more only while ranks are left
+        if (r >= kRunStores && 64u * r >= st.total) break;
         const uint32_t q = lane + 64u * r;
         uint32_t d = cur;
         while (np < 64u * (r + 1)) {  // segments whose first rank falls in this store
