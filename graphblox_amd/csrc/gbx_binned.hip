@@ -25,7 +25,8 @@
 // about a quarter of all edges at R-MAT scale 24 for H = 16384) keep their
 // contributions in the accumulate block's LDS instead; their edges are
 // stored per bin as static (row, source) u16 pairs, 4 bytes per edge instead
-// of 12.5 through V, and never pass through the scatter.
+// of 12.5 through V, and never pass through the scatter (with the run
+// layout packed into 16 bits per edge, see hot_fill16).
 //
 // The fixed-point scale is a power of two chosen per iteration from the
 // previous FP64 rank sum, which bounds every row sum, so no sum can
@@ -124,13 +125,16 @@ struct gbx_bplan {
     uint32_t Plen = 0;                // padded length of src_local
     uint32_t SC = 0;                  // sources per chunk (at most)
     bool runs = false;                // run layout: one V value per run of equal rows of a segment
+    bool hot16 = false;               // packed hot pairs (with the run layout)
     gbx::DevBuf chunk_lo;             // u32[NC+1] first source of each chunk
     gbx::DevBuf chunk_grp;            // u32[NC+1] first group of 8 src_local entries of each chunk
     gbx::DevBuf src_local;            // u16[Plen] source - chunk_lo, chunk-major, segments padded (0xFFFF)
     gbx::DevBuf bdst;                 // u32[Plen/32] V position of each block of 32 src_local entries
     gbx::DevBuf vgrp;                 // run layout: u32[Plen/8] V position of each group's first run
     gbx::DevBuf dst_local;            // u16[Vlen] row - bin first row, V order
-    gbx::DevBuf hot_pairs;            // u32[Hlen] (row - bin first row) | source << 16, per bin (pad: src 0xFFFF)
+    gbx::DevBuf hot_pairs;            // u32[Hlen] (row - bin first row) | source << 16, per bin (pad: src 0xFFFF);
+                                      // packed (hot16): u16[Hlen] source | row step << 14
+    gbx::DevBuf hot_base;             // packed: u16[Hlen/8] row - bin first row of each group's first pair
     gbx::DevBuf units;                // BinUnit[NU]
 };
 
@@ -284,8 +288,10 @@ __global__ void split_hot_cold(const uint32_t *__restrict__ col, const uint32_t 
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
         const uint32_t v = rowid[e], u = col[e];
         if (hflag[e]) {
-            const uint32_t b = bin_of[v];
-            hot_pairs[hot_start[b] + (hrank[e] - hot_before[b])] = (v - bin_row0[b]) | (u << 16);
+            if (hot_pairs) {
+                const uint32_t b = bin_of[v];
+                hot_pairs[hot_start[b] + (hrank[e] - hot_before[b])] = (v - bin_row0[b]) | (u << 16);
+            }
         } else {
             const uint32_t c = e - hrank[e];
             rowid_c[c] = v;
@@ -373,6 +379,66 @@ __global__ void run_fill(const uint32_t *__restrict__ key, const uint32_t *__res
         src_local[p] = (uint16_t)((col_c[e] - chunk_lo[k]) | (flag_here ? kRunEnd : 0u));
         if (end) dst_local[vpos] = (uint16_t)(rowid_c[e] - bin_row0[b]);
         if (p % kRunGroup == 0) vgrp[p / kRunGroup] = vpos;
+    }
+}
+
+// ---- packed hot pairs (the run layout's plans): 16 bits per pair
+
+// A hot pair is source (14 bits) | row step (2 bits) << 14: the row of a
+// pair is the row of the previous pair of its group of 8 plus the step, the
+// first pair of a group takes its row from hot_base (u16 per group).  A gap
+// of more than 3 rows between consecutive pairs of a bin gets filler pairs
+// (source kHotPad, a zero in LDS, step 3), about 0.4 % of the pairs at
+// R-MAT scale 24.
+constexpr uint32_t kHotPad = 0x3FFFu;
+constexpr uint32_t kHot16Max = 16376;   // hot sources of the packed form
+constexpr uint32_t kHot16Slots = 16384; // LDS hot slots (kHotPad included)
+
+// Hot edges (plan order) -> compacted rows and sources.
+__global__ void hot_compact(const uint32_t *__restrict__ col, const uint32_t *__restrict__ rowid, uint32_t E,
+                            const uint32_t *__restrict__ hflag, const uint32_t *__restrict__ hrank,
+                            uint32_t *__restrict__ rowid_h, uint32_t *__restrict__ col_h)
+{
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x)
+        if (hflag[e]) {
+            rowid_h[hrank[e]] = rowid[e];
+            col_h[hrank[e]] = col[e];
+        }
+}
+
+// Filler pairs before hot pair i: a gap of d > 3 rows to the previous pair
+// of the bin takes (d - 1) / 3 fillers of step 3; n[i] = fillers + 1.
+__global__ void hot_counts(const uint32_t *__restrict__ rowid_h, uint32_t Eh, const uint32_t *__restrict__ bin_of,
+                           const uint32_t *__restrict__ hot_before, uint32_t *__restrict__ n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < Eh; i += gridDim.x * blockDim.x) {
+        const uint32_t b = bin_of[rowid_h[i]];
+        const uint32_t d = i == hot_before[b] ? 0u : rowid_h[i] - rowid_h[i - 1];
+        n[i] = 1u + (d > 3u ? (d - 1u) / 3u : 0u);
+    }
+}
+
+// Every hot pair and its fillers at their packed positions (bin b starts at
+// hot_start[b]; hpos = exclusive scan of n), and the row of every group's
+// first pair.
+__global__ void hot_fill16(const uint32_t *__restrict__ rowid_h, const uint32_t *__restrict__ col_h, uint32_t Eh,
+                           const uint32_t *__restrict__ n, const uint32_t *__restrict__ hpos,
+                           const uint32_t *__restrict__ bin_of, const uint32_t *__restrict__ hot_before,
+                           const uint32_t *__restrict__ hot_start, const uint32_t *__restrict__ bin_row0,
+                           uint16_t *__restrict__ pairs, uint16_t *__restrict__ hbase)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < Eh; i += gridDim.x * blockDim.x) {
+        const uint32_t row = rowid_h[i], b = bin_of[row], fill = n[i] - 1u;
+        const uint32_t base = hot_start[b] + (hpos[i] - hpos[hot_before[b]]);
+        const uint32_t d = i == hot_before[b] ? 0u : row - rowid_h[i - 1];
+        for (uint32_t k = 1; k <= fill; k++) {
+            const uint32_t pos = base + k - 1u;
+            pairs[pos] = (uint16_t)(kHotPad | (3u << 14));
+            if ((pos & 7u) == 0) hbase[pos >> 3] = (uint16_t)(row - d + 3u * k - bin_row0[b]);
+        }
+        const uint32_t pos = base + fill;
+        pairs[pos] = (uint16_t)(col_h[i] | ((d - 3u * fill) << 14));
+        if ((pos & 7u) == 0) hbase[pos >> 3] = (uint16_t)(row - bin_row0[b]);
     }
 }
 
@@ -699,6 +765,7 @@ struct AccumView {
     const BinUnit *__restrict__ units;
     const uint16_t *__restrict__ dst_local;
     const uint32_t *__restrict__ hot_pairs;
+    const uint16_t *__restrict__ hot_base;  // packed pairs: row of each group's first pair (else null)
     long long *acc;  // split-bin sums
     uint32_t *cnt;   // split-bin arrival counters
     uint32_t H;      // hot sources
@@ -757,6 +824,34 @@ __device__ __forceinline__ void load_hot8(const uint32_t *pairs, uint32_t e, con
     }
 }
 
+// Eight packed hot pairs: rows from the group's base row and the 2-bit
+// steps, values from the LDS contributions (kHotPad holds 0).
+template <int KIND, class T>
+__device__ __forceinline__ void load_hot8_packed(const uint16_t *pairs, const uint16_t *base, uint32_t e,
+                                                 const T *ch, uint32_t (&d)[8], RawVal<KIND> (&v)[8])
+{
+    const uint4 a = *reinterpret_cast<const uint4 *>(pairs + e);
+    const uint32_t w[4] = {a.x, a.y, a.z, a.w};
+    uint32_t r = base[e >> 3];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint32_t p = (i & 1) ? (w[i >> 1] >> 16) : (w[i >> 1] & 0xFFFFu);
+        if (i) r += p >> 14;
+        d[i] = r;
+        v[i] = raw_of<KIND, T>(ch[p & kHotPad]);
+    }
+}
+
+template <int KIND, class T>
+__device__ __forceinline__ void load_hot8_any(const AccumView &B, uint32_t e, const T *ch, uint32_t (&d)[8],
+                                              RawVal<KIND> (&v)[8])
+{
+    if (B.hot_base)
+        load_hot8_packed<KIND, T>(reinterpret_cast<const uint16_t *>(B.hot_pairs), B.hot_base, e, ch, d, v);
+    else
+        load_hot8<KIND, T>(B.hot_pairs, e, ch, d, v);
+}
+
 template <int KIND, class T>
 using EpiOf = std::conditional_t<is_fixed_kind(KIND), FixEpi<KIND, T>,
                                  std::conditional_t<is_quant_kind(KIND), QuantEpi<KIND>, PrEpi<KIND, T>>>;
@@ -783,7 +878,7 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
             for (uint32_t e = u.hbeg + threadIdx.x * 8u; e < u.hend; e += stride) {
                 uint32_t d[8];
                 RawVal<KIND> v[8];
-                load_hot8<KIND, T>(B.hot_pairs, e, ch, d, v);
+                load_hot8_any<KIND, T>(B, e, ch, d, v);
 #pragma unroll
                 for (int i = 0; i < 8; i++) s += fixed_of<KIND>(v[i], scale);
             }
@@ -832,7 +927,7 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
         for (uint32_t e = u.hbeg + threadIdx.x * 8u; e < u.hend; e += stride) {
             uint32_t d[8];
             RawVal<KIND> v[8];
-            load_hot8<KIND, T>(B.hot_pairs, e, ch, d, v);
+            load_hot8_any<KIND, T>(B, e, ch, d, v);
             // hot pairs are in row order: runs of equal rows summed in registers
             uint32_t rd = d[0];
             long long run = 0;
@@ -931,6 +1026,7 @@ __global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const ui
         if constexpr (is_quant_kind(KIND))
             quant_to_lds<KIND, T>(hot, reinterpret_cast<const float *>(A.c_in), B.H, epi.scale);
         else copy_to_lds(hot, reinterpret_cast<const T *>(A.c_in), B.H);
+        if (B.hot_base && threadIdx.x == 0) hot[kHotPad] = T(0);  // packed pairs' pads and fillers
     }
     __shared__ uint32_t s_unit;
     for (;;) {
@@ -1077,14 +1173,18 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     const uint32_t EC = cut_edges_of(tu.binned_chunk_edges, ctx, E, runs), SC = chunk_sources_of(tu, hot_elem, runs);
     // rows per bin: what the LDS holds next to H hot contributions of hot_elem
     // bytes (a multiple of 8)
+    const bool hot16 = runs && H > 0;
+    const uint32_t hot_slots = hot16 ? kHot16Slots : H;
     const uint32_t SB =
-        std::min(std::min(tu.binned_bin_rows, kMaxBinRows), (kAccumLds - H * hot_elem) / 8u) & ~7u;
+        std::min(std::min(tu.binned_bin_rows, kMaxBinRows), (kAccumLds - hot_slots * hot_elem) / 8u) & ~7u;
     const uint32_t EB = (std::max(cut_edges_of(tu.binned_bin_edges, ctx, E, runs), 8u) + 7u) & ~7u;
     if (SB < 8) return set_error(ctx, GBX_ERR_ARG, "binned layout: %u hot sources leave no room for bins", H);
     bp->H = H;
     bp->SB = SB;
     bp->SC = SC;
     bp->runs = runs;
+    bp->hot16 = hot16;
+    if (hot16 && H > kHot16Max) return set_error(ctx, GBX_ERR_ARG, "binned layout: %u hot sources, packed pairs take %u", H, kHot16Max);
     std::vector<uint32_t> rp(N + 1), colcnt(N);
     GBX_HIP(ctx, hipMemcpyAsync(rp.data(), p->rp.p, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
     DevBuf dcnt;
@@ -1197,28 +1297,74 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
                                     ctx->stream));
         GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if ((uint64_t)hot_before[NB] != Ehot) return set_error(ctx, GBX_ERR_STATE, "binned layout: hot edge count");
+        // packed pairs: fillers per pair, their running count and each bin's packed length
+        DevBuf rowid_h, col_h, hn, hp;
+        std::vector<uint32_t> packed_before(NB + 1);
+        const uint32_t Eh = (uint32_t)Ehot;
+        if (hot16) {
+            DevBuf hptot, d_pb;
+            GBX_TRY(dev_alloc(ctx, rowid_h, (size_t)Eh * 4 + 4));
+            GBX_TRY(dev_alloc(ctx, col_h, (size_t)Eh * 4 + 4));
+            GBX_TRY(dev_alloc(ctx, hn, (size_t)Eh * 4 + 4));
+            GBX_TRY(dev_alloc(ctx, hp, (size_t)(Eh + 1) * 4));
+            GBX_TRY(dev_alloc(ctx, hptot, 16));
+            GBX_TRY(dev_alloc(ctx, d_pb, (size_t)(NB + 1) * 4));
+            hot_compact<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(
+                p->col.as<uint32_t>(), rowid.as<uint32_t>(), E, hflag.as<uint32_t>(), hrank.as<uint32_t>(),
+                rowid_h.as<uint32_t>(), col_h.as<uint32_t>());
+            hot_counts<<<grid_for(Eh, 256, 16384), 256, 0, ctx->stream>>>(
+                rowid_h.as<uint32_t>(), Eh, d_bin_of.as<uint32_t>(), d_hot_before.as<uint32_t>(), hn.as<uint32_t>());
+            GBX_HIP(ctx, hipGetLastError());
+            GBX_TRY(exclusive_scan_u32(ctx, hn.as<uint32_t>(), hp.as<uint32_t>(), Eh, hptot.as<uint32_t>()));
+            GBX_HIP(ctx, hipMemcpyAsync(hp.as<uint32_t>() + Eh, hptot.p, 4, hipMemcpyDeviceToDevice, ctx->stream));
+            gather_u32<<<grid_for(NB + 1, 256, 1024), 256, 0, ctx->stream>>>(hp.as<uint32_t>(), d_hot_before.as<uint32_t>(),
+                                                                              NB + 1, d_pb.as<uint32_t>());
+            GBX_HIP(ctx, hipGetLastError());
+            GBX_HIP(ctx, hipMemcpyAsync(packed_before.data(), d_pb.p, (size_t)(NB + 1) * 4, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+            GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        } else {
+            packed_before = hot_before;
+        }
         uint64_t hpos = 0;
         for (uint32_t b = 0; b < NB; b++) {
             hot_start[b] = (uint32_t)hpos;
-            hpos += ((uint64_t)(hot_before[b + 1] - hot_before[b]) + 7u) & ~7ull;
+            hpos += ((uint64_t)(packed_before[b + 1] - packed_before[b]) + 7u) & ~7ull;
         }
         hot_start[NB] = (uint32_t)hpos;
         if (hpos >= (1ull << 32)) return set_error(ctx, GBX_ERR_RANGE, "binned layout: hot pairs too many");
         bp->Hlen = (uint32_t)hpos;
-        GBX_TRY(dev_alloc(ctx, bp->hot_pairs, (size_t)bp->Hlen * 4 + 16));
-        GBX_TRY(fill_u32(ctx, bp->hot_pairs.as<uint32_t>(), 0xFFFF0000u, bp->Hlen));
+        if (hot16) {
+            GBX_TRY(dev_alloc(ctx, bp->hot_pairs, (size_t)bp->Hlen * 2 + 16));
+            GBX_TRY(fill_u32(ctx, bp->hot_pairs.as<uint32_t>(), kHotPad | (kHotPad << 16), bp->Hlen / 2 + 4));
+            GBX_TRY(dev_alloc(ctx, bp->hot_base, (size_t)(bp->Hlen / 8) * 2 + 16));
+            GBX_HIP(ctx, hipMemsetAsync(bp->hot_base.p, 0, (size_t)(bp->Hlen / 8) * 2 + 16, ctx->stream));
+        } else {
+            GBX_TRY(dev_alloc(ctx, bp->hot_pairs, (size_t)bp->Hlen * 4 + 16));
+            GBX_TRY(fill_u32(ctx, bp->hot_pairs.as<uint32_t>(), 0xFFFF0000u, bp->Hlen));
+            GBX_TRY(dev_alloc(ctx, bp->hot_base, 16));
+        }
         up(d_hot_start, hot_start.data(), (size_t)(NB + 1) * 4);
         GBX_TRY(rc);
+        if (hot16 && Eh) {
+            hot_fill16<<<grid_for(Eh, 256, 16384), 256, 0, ctx->stream>>>(
+                rowid_h.as<uint32_t>(), col_h.as<uint32_t>(), Eh, hn.as<uint32_t>(), hp.as<uint32_t>(),
+                d_bin_of.as<uint32_t>(), d_hot_before.as<uint32_t>(), d_hot_start.as<uint32_t>(),
+                d_bin_row0.as<uint32_t>(), bp->hot_pairs.as<uint16_t>(), bp->hot_base.as<uint16_t>());
+            GBX_HIP(ctx, hipGetLastError());
+        }
         GBX_TRY(dev_alloc(ctx, rowid_c, (size_t)Ecold * 4 + 4));
         GBX_TRY(dev_alloc(ctx, col_c, (size_t)Ecold * 4 + 4));
         split_hot_cold<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(
             p->col.as<uint32_t>(), rowid.as<uint32_t>(), E, hflag.as<uint32_t>(), hrank.as<uint32_t>(),
             d_bin_of.as<uint32_t>(), d_bin_row0.as<uint32_t>(), d_hot_before.as<uint32_t>(),
-            d_hot_start.as<uint32_t>(), bp->hot_pairs.as<uint32_t>(), rowid_c.as<uint32_t>(), col_c.as<uint32_t>());
+            d_hot_start.as<uint32_t>(), hot16 ? nullptr : bp->hot_pairs.as<uint32_t>(), rowid_c.as<uint32_t>(),
+            col_c.as<uint32_t>());
         GBX_HIP(ctx, hipGetLastError());
         GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     } else {
         GBX_TRY(dev_alloc(ctx, bp->hot_pairs, 16));
+        GBX_TRY(dev_alloc(ctx, bp->hot_base, 16));
     }
     if (Ecold && runs) {
         GBX_TRY(build_runs(ctx, bp, Ecold, rowid_c.as<uint32_t>(), col_c.as<uint32_t>(), d_bin_of.as<uint32_t>(),
@@ -1404,7 +1550,8 @@ int binned_create(gbx_pr *pr)
     // (u16 source ids, 0xFFFF marks pads)
     const size_t vel = value_bytes(pr);
     const uint32_t hot_elem = std::max<uint32_t>((uint32_t)vel, 4u);
-    const uint32_t H = std::min(std::min(tu.binned_hot_sources * 4u / hot_elem, 65528u), p->N) & ~7u;
+    const uint32_t H = std::min(std::min(tu.binned_hot_sources * 4u / hot_elem, runs ? kHot16Max : 65528u), p->N) &
+                       ~7u;
     const uint32_t want[8] = {cut_edges_of(tu.binned_chunk_edges, ctx, p->E, runs),
                               (std::max(cut_edges_of(tu.binned_bin_edges, ctx, p->E, runs), 8u) + 7u) & ~7u,
                               std::min(tu.binned_bin_rows, kMaxBinRows),
@@ -1449,7 +1596,7 @@ int binned_create(gbx_pr *pr)
     st->stage_off = bp->runs ? (uint32_t)((((size_t)bp->SC + 1) * vel + 15) & ~(size_t)15) : 0;
     st->shm_scatter = bp->runs ? st->stage_off + (size_t)(kRunsBlock / 64) * kRunStage * sizeof(float)
                                : (size_t)bp->cut[3] * vel;
-    st->shm_accum = (size_t)bp->SB * 8 + (size_t)bp->H * vel;
+    st->shm_accum = (size_t)bp->SB * 8 + (size_t)(bp->hot16 ? kHot16Slots : bp->H) * vel;
     switch (pr->kind) {
     case K_PR_REF: pr->elem == 4 ? grant_kind<K_PR_REF, float>(st) : grant_kind<K_PR_REF, uint16_t>(st); break;
     case K_PR_STOCH: pr->elem == 4 ? grant_kind<K_PR_STOCH, float>(st) : grant_kind<K_PR_STOCH, uint16_t>(st); break;
@@ -1500,6 +1647,7 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
     B.units = bp->units.as<BinUnit>();
     B.dst_local = bp->dst_local.as<uint16_t>();
     B.hot_pairs = bp->hot_pairs.as<uint32_t>();
+    B.hot_base = bp->hot16 ? bp->hot_base.as<uint16_t>() : nullptr;
     B.H = bp->H;
     B.hot_off = bp->SB * 8u;
     B.acc = st->acc.as<long long>();
