@@ -47,9 +47,9 @@
 // each such run (cut at every group of 8 src_local entries) into one V
 // value, so V, dst_local and both passes over V shrink to the runs (0.64 of
 // the cold edges at R-MAT scale 24).  src_local flags each run's last entry
-// (bit 15; 15-bit sources, chunks of at most 32736), vgrp gives the V
-// position of each group's first run, and V segments are padded to 8
-// values (zeros, written once).  The run sums are FP32, so the result
+// (bit 15; 15-bit sources, chunks of at most 32736); a record per block of
+// 64 groups and each segment's V offset give the runs' V positions; V
+// segments are padded to 8 values (zeros, written once).  The run sums are FP32, so the result
 // depends on the cut (not on the launch): deterministic for a plan, within
 // FP32 rounding of the one-value-per-edge layout.
 #include <algorithm>
@@ -130,7 +130,9 @@ struct gbx_bplan {
     gbx::DevBuf chunk_grp;            // u32[NC+1] first group of 8 src_local entries of each chunk
     gbx::DevBuf src_local;            // u16[Plen] source - chunk_lo, chunk-major, segments padded (0xFFFF)
     gbx::DevBuf bdst;                 // u32[Plen/32] V position of each block of 32 src_local entries
-    gbx::DevBuf vgrp;                 // run layout: u32[Plen/8] V position of each group's first run
+    gbx::DevBuf blk_rec;              // run layout: uint4 per block of 64 groups (V position of its first run,
+                                      // segments before it, segment-start bits)
+    gbx::DevBuf seg_v;                // run layout: u32[nseg] V offset of each segment (chunk-major order)
     gbx::DevBuf dst_local;            // u16[Vlen] row - bin first row, V order
     gbx::DevBuf hot_pairs;            // u32[Hlen] (row - bin first row) | source << 16, per bin (pad: src 0xFFFF);
                                       // packed (hot16): u16[Hlen] source | row step << 14
@@ -306,6 +308,10 @@ __global__ void split_hot_cold(const uint32_t *__restrict__ col, const uint32_t 
 // cross a group (segments are padded to 8 entries and a run is cut at every
 // group boundary), so every lane sums its own runs.
 constexpr uint32_t kRunGroup = 8;
+// A wave's step: 64 groups, aligned to blocks of 64 groups of the whole
+// chunk-major src_local; each block has a record (blk_rec) and every
+// segment its V offset (seg_v), which give the V positions of a step's runs.
+constexpr uint32_t kRunBlockGroups = 64;
 constexpr uint32_t kRunEnd = 0x8000u;  // src_local flag: the entry ends its run
 
 // head[j] = 1 where a run starts: at a new row and at every group boundary
@@ -350,9 +356,9 @@ __global__ void scatter_u32(const uint32_t *__restrict__ a, const uint32_t *__re
 
 // Every cold edge: its source (| kRunEnd on a run's last entry) in
 // src_local, its run's row in dst_local, and for the first entry of each
-// group the V position of its run (the group's first run).  A segment's
-// last run takes the pads of its group (zeros): its end flag sits on the
-// group's last entry, so every group ends with a run end.
+// block of 64 groups the V position of its run (the block's first run).  A
+// segment's last run takes the pads of its group (zeros): its end flag sits
+// on the group's last entry, so every group ends with a run end.
 __global__ void run_fill(const uint32_t *__restrict__ key, const uint32_t *__restrict__ pay, uint32_t n,
                          const uint32_t *__restrict__ flag, const uint32_t *__restrict__ runidx,
                          const uint32_t *__restrict__ runstart, const uint32_t *__restrict__ pstart,
@@ -361,7 +367,7 @@ __global__ void run_fill(const uint32_t *__restrict__ key, const uint32_t *__res
                          const uint32_t *__restrict__ col_c, const uint32_t *__restrict__ chunk_lo,
                          const uint32_t *__restrict__ bin_row0, uint32_t NB, uint32_t pad,
                          uint16_t *__restrict__ src_local, uint16_t *__restrict__ dst_local,
-                         uint32_t *__restrict__ vgrp)
+                         uint32_t *__restrict__ blk_v)
 {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         const uint32_t r = runidx[j] + flag[j] - 1u, s0 = runstart[r];
@@ -378,8 +384,33 @@ __global__ void run_fill(const uint32_t *__restrict__ key, const uint32_t *__res
         }
         src_local[p] = (uint16_t)((col_c[e] - chunk_lo[k]) | (flag_here ? kRunEnd : 0u));
         if (end) dst_local[vpos] = (uint16_t)(rowid_c[e] - bin_row0[b]);
-        if (p % kRunGroup == 0) vgrp[p / kRunGroup] = vpos;
+        if (p % (kRunGroup * kRunBlockGroups) == 0) blk_v[p / (kRunGroup * kRunBlockGroups)] = vpos;
     }
+}
+
+// Bit g of `bits` (u32 words) for every group g that starts a segment.
+__global__ void run_segment_bits(const uint32_t *__restrict__ pstart, uint32_t nseg, uint32_t *__restrict__ bits)
+{
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nseg; r += gridDim.x * blockDim.x) {
+        const uint32_t g = pstart[r] / kRunGroup;
+        atomicOr(bits + g / 32, 1u << (g % 32));
+    }
+}
+
+// Segment starts per block of 64 groups.
+__global__ void run_block_counts(const uint32_t *__restrict__ bits, uint32_t nblk, uint32_t *__restrict__ cnt)
+{
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nblk; b += gridDim.x * blockDim.x)
+        cnt[b] = (uint32_t)(__popc(bits[2 * b]) + __popc(bits[2 * b + 1]));
+}
+
+// The record of every block of 64 groups: V position of its first run,
+// segments starting before it, its segment-start bits.
+__global__ void run_block_records(const uint32_t *__restrict__ blk_v, const uint32_t *__restrict__ blk_seg,
+                                  const uint32_t *__restrict__ bits, uint32_t nblk, uint4 *__restrict__ rec)
+{
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nblk; b += gridDim.x * blockDim.x)
+        rec[b] = make_uint4(blk_v[b], blk_seg[b], bits[2 * b], bits[2 * b + 1]);
 }
 
 // ---- packed hot pairs (the run layout's plans): 16 bits per pair
@@ -547,7 +578,10 @@ struct RunsView {
     const uint32_t *__restrict__ chunk_lo;
     const uint32_t *__restrict__ chunk_grp;
     const uint16_t *__restrict__ src_local;  // local source | kRunEnd; pads: `pad`
-    const uint32_t *__restrict__ vgrp;       // V position of each group's first run
+    const uint4 *__restrict__ blk_rec;       // per block of 64 groups: first run's V position, segments
+                                             // before it, segment-start bits (lo, hi)
+    const uint32_t *__restrict__ seg_v;      // V offset of each segment
+    uint32_t nseg;
     uint32_t pad;        // LDS index of a zero (the plan's chunk size)
     uint32_t stage_off;  // byte offset of the per-wave staging in LDS
     uint32_t dummy;      // byte offset in V of the dummy areas past the last run (written by idle store
@@ -607,9 +641,10 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v)
 // in rank order, 64 consecutive ranks per store.  The runs of the lanes of
 // one (chunk, bin) segment have consecutive ranks and consecutive V
 // positions, so V position = rank + delta with delta constant over the
-// segment; a lane whose delta differs from the lane below leads a segment
-// (about 3 per step), and the write-out takes each rank's delta from the
-// last leader at or below it.  The write-out always issues at least
+// segment; the lanes that start a segment lead it (about 3 per step; their
+// V offsets come from seg_v), lane 0 leads the segment the step starts in
+// (its V position from the block record), and the write-out takes each
+// rank's delta from the last leader at or below it.  The write-out always issues at least
 // kRunStores stores (ranks past the wave's runs store to the wave's dummy
 // area), so the loads of later steps stay in flight across them.
 struct RunStep {
@@ -622,7 +657,7 @@ struct RunStep {
 };
 
 template <class T>
-__device__ __forceinline__ RunStep runs_sum(const T *cl, uint4 w, uint32_t vbase, uint32_t lane)
+__device__ __forceinline__ RunStep runs_sum(const T *cl, uint4 w, uint32_t vbase, bool leader)
 {
     RunStep st;
     const uint32_t ew[4] = {w.x, w.y, w.z, w.w};
@@ -639,8 +674,8 @@ __device__ __forceinline__ RunStep runs_sum(const T *cl, uint4 w, uint32_t vbase
     const uint32_t inc = wave_inclusive_sum(cnt);
     st.pre = inc - cnt;
     st.total = __builtin_amdgcn_readlane(inc, 63);
-    st.delta = vbase - st.pre;
-    st.lead = __ballot(lane == 0 || st.delta != dpp_u32<0x138>(st.delta));  // wave_shr:1
+    st.delta = vbase - st.pre;  // meaningful on leaders
+    st.lead = __ballot(leader);
     float s = 0.f;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
@@ -664,8 +699,8 @@ __device__ __forceinline__ void runs_out(const float *stage, const RunStep &st, 
                                          T *__restrict__ V)
 {
     char *Vb = reinterpret_cast<char *>(V);
-    uint64_t lead = st.lead & (st.lead - 1);  // lane 0 leads the first segment
-    uint32_t cur = __builtin_amdgcn_readlane(st.delta, 0);
+    uint64_t lead = st.lead;  // the lowest leader has rank 0 when the step has runs
+    uint32_t cur = 0;
     uint32_t nl = lead ? (uint32_t)__builtin_ctzll(lead) : 0u;
     uint32_t np = lead ? __builtin_amdgcn_readlane(st.pre, nl) : 0xFFFFFFFFu;
 #pragma unroll
@@ -714,40 +749,58 @@ __global__ void __launch_bounds__(kRunsBlock) bin_scatter_runs_kernel(RunsView S
     const uint32_t padw = S.pad | (S.pad << 16);
     const uint4 padv = make_uint4(padw, padw, padw, padw);
     const uint4 *src = reinterpret_cast<const uint4 *>(S.src_local);
-    uint4 w[kRunsAhead];
-    uint32_t vb[kRunsAhead];
-    // loads are always issued (clamped to the chunk's last group); lanes
-    // past it take pads when the step is summed
+    const uint32_t blk_last = (g1 - 1u) / kRunBlockGroups;
+    uint4 w[kRunsAhead], rec[kRunsAhead];
+    uint32_t sv[2];  // the V offset of the segment each lane would start, two steps ahead
+    // a step's group data and block record; loads are always issued
+    // (clamped into the chunk), lanes outside it take pads when summed
     auto load = [&](int d, uint32_t first) {
-        const uint32_t g = min(first + lane, g1 - 1u);
+        const uint32_t g = min(max(first + lane, g0), g1 - 1u);
         w[d] = src[g];
-        vb[d] = S.vgrp[g];
+        rec[d] = S.blk_rec[min(first / kRunBlockGroups, blk_last)];
     };
-    // sum the step that starts at group `first` from slot d, then refill d
-    auto sum = [&](int d, uint32_t first) -> RunStep {
+    // seg_v of the segment starting at each lane's group (used by segment
+    // starts only), from the step's block record in slot d
+    auto load_sv = [&](int d, int par) {
+        const uint64_t bits = ((uint64_t)rec[d].w << 32) | rec[d].z;
+        const uint32_t idx = rec[d].y + (uint32_t)__popcll(bits & ((1ull << lane) - 1ull));
+        sv[par] = S.seg_v[min(idx, S.nseg - 1u)];
+    };
+    // sum the step that starts at group `first` from slot d (and sv[par]),
+    // then refill d with the step kRunsAhead later and sv[par] with the
+    // step two later
+    auto sum = [&](int d, int par, uint32_t first) -> RunStep {
         __builtin_amdgcn_sched_barrier(0);
-        const bool act = first + lane < g1;
+        const uint32_t g = first + lane;
+        const bool act = g >= g0 && g < g1;
         const uint4 cw = act ? w[d] : padv;
-        const uint32_t cv = act ? vb[d] : 0u;  // both slot registers are free for the next load
+        const uint64_t bits = ((uint64_t)rec[d].w << 32) | rec[d].z;
+        const bool start = act && ((bits >> lane) & 1u);
+        const bool leader = start || (act && lane == 0);
+        const uint32_t vbase = start ? sv[par] : rec[d].x;  // the slots are free for the next loads
         __builtin_amdgcn_sched_barrier(0);
         load(d, first + kRunsAhead * kRunsBlock);
-        return runs_sum<T>(cl, cw, cv, lane);
+        load_sv((d + 2) % kRunsAhead, par);
+        return runs_sum<T>(cl, cw, vbase, leader);
     };
-    uint32_t r = g0 + (threadIdx.x & ~63u);
+    // steps start at multiples of 64 groups of the whole src_local
+    uint32_t r = (g0 & ~(kRunBlockGroups - 1u)) + (threadIdx.x & ~63u);
     if (r >= g1) return;
 #pragma unroll
     for (int d = 0; d < kRunsAhead; d++) {
         load(d, r + (uint32_t)d * kRunsBlock);  // in slot order: slot 0 is waited for first
         __builtin_amdgcn_sched_barrier(0);
     }
-    RunStep cur = sum(0, r);
+    load_sv(0, 0);
+    load_sv(1, 1);
+    RunStep cur = sum(0, 0, r);
     runs_stage(stage, cur);
     // a round: for each of its steps, sum the next step, write out this
     // one, stage the next
     auto round = [&](uint32_t rr) {
 #pragma unroll
         for (int d = 0; d < kRunsAhead; d++) {
-            const RunStep nxt = sum((d + 1) % kRunsAhead, rr + (uint32_t)(d + 1) * kRunsBlock);
+            const RunStep nxt = sum((d + 1) % kRunsAhead, (d + 1) & 1, rr + (uint32_t)(d + 1) * kRunsBlock);
             wave_lds_order();
             runs_out<T>(stage, cur, dummy_off, lane, V);
             wave_lds_order();
@@ -1150,17 +1203,36 @@ static int build_runs(gbx_ctx *ctx, gbx_bplan *bp, uint32_t Ecold, const uint32_
     bp->Vlen = Vlen;
     GBX_TRY(dev_alloc(ctx, bp->src_local, (size_t)bp->Plen * 2 + 16));
     GBX_TRY(dev_alloc(ctx, bp->dst_local, (size_t)Vlen * 2 + 16));
-    GBX_TRY(dev_alloc(ctx, bp->vgrp, (size_t)(bp->Plen / kRunGroup) * 4 + 16));
+    const uint32_t ngrp = bp->Plen / kRunGroup, nblk = (ngrp + kRunBlockGroups - 1) / kRunBlockGroups;
+    DevBuf blk_v, bits, bcnt, bseg, btot;
+    GBX_TRY(dev_alloc(ctx, blk_v, (size_t)nblk * 4 + 4));
+    GBX_TRY(dev_alloc(ctx, bits, (size_t)nblk * 8 + 8));
+    GBX_TRY(dev_alloc(ctx, bcnt, (size_t)nblk * 4 + 4));
+    GBX_TRY(dev_alloc(ctx, bseg, (size_t)nblk * 4 + 4));
+    GBX_TRY(dev_alloc(ctx, btot, 16));
+    GBX_TRY(dev_alloc(ctx, bp->blk_rec, (size_t)nblk * 16 + 16));
+    GBX_TRY(fill_u32(ctx, bits.as<uint32_t>(), 0, (size_t)nblk * 2 + 2));
+    GBX_TRY(fill_u32(ctx, blk_v.as<uint32_t>(), 0, nblk + 1));
     GBX_TRY(dev_alloc(ctx, bp->bdst, 16));
     GBX_TRY(fill_u32(ctx, bp->src_local.as<uint32_t>(), bp->SC | (bp->SC << 16), bp->Plen / 2 + 4));
     GBX_HIP(ctx, hipMemsetAsync(bp->dst_local.p, 0, (size_t)Vlen * 2 + 16, ctx->stream));
-    GBX_HIP(ctx, hipMemsetAsync(bp->vgrp.p, 0, (size_t)(bp->Plen / kRunGroup) * 4 + 16, ctx->stream));
+
     run_fill<<<gE, 256, 0, ctx->stream>>>(key.as<uint32_t>(), pay.as<uint32_t>(), Ecold, R.flag.as<uint32_t>(),
                                           R.idx.as<uint32_t>(), R.start.as<uint32_t>(), R.pstart.as<uint32_t>(),
                                           head.as<uint32_t>(), hx.as<uint32_t>(), voff.as<uint32_t>(), rowid_c, col_c,
                                           bp->chunk_lo.as<uint32_t>(), bin_row0, NB, bp->SC, bp->src_local.as<uint16_t>(),
-                                          bp->dst_local.as<uint16_t>(), bp->vgrp.as<uint32_t>());
+                                          bp->dst_local.as<uint16_t>(), blk_v.as<uint32_t>());
     GBX_HIP(ctx, hipGetLastError());
+    // block records: segment-start bits, their running count, first V positions
+    run_segment_bits<<<gS, 256, 0, ctx->stream>>>(R.pstart.as<uint32_t>(), ns, bits.as<uint32_t>());
+    run_block_counts<<<grid_for(nblk, 256, 16384), 256, 0, ctx->stream>>>(bits.as<uint32_t>(), nblk,
+                                                                          bcnt.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    GBX_TRY(exclusive_scan_u32(ctx, bcnt.as<uint32_t>(), bseg.as<uint32_t>(), nblk, btot.as<uint32_t>()));
+    run_block_records<<<grid_for(nblk, 256, 16384), 256, 0, ctx->stream>>>(
+        blk_v.as<uint32_t>(), bseg.as<uint32_t>(), bits.as<uint32_t>(), nblk, bp->blk_rec.as<uint4>());
+    GBX_HIP(ctx, hipGetLastError());
+    bp->seg_v = std::move(voff);
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GBX_OK;
 }
@@ -1427,7 +1499,8 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
         GBX_TRY(dev_alloc(ctx, bp->dst_local, 16));
         GBX_TRY(dev_alloc(ctx, bp->src_local, 16));
         GBX_TRY(dev_alloc(ctx, bp->bdst, 16));
-        GBX_TRY(dev_alloc(ctx, bp->vgrp, 16));
+        GBX_TRY(dev_alloc(ctx, bp->blk_rec, 16));
+        GBX_TRY(dev_alloc(ctx, bp->seg_v, 16));
         GBX_TRY(fill_u32(ctx, bp->chunk_grp.as<uint32_t>(), 0, NC + 1));
     }
     // units: V range and hot-pair range of each bin (a single row longer than EB is split)
@@ -1630,7 +1703,9 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
             RV.chunk_lo = S.chunk_lo;
             RV.chunk_grp = S.chunk_grp;
             RV.src_local = S.src_local;
-            RV.vgrp = bp->vgrp.as<uint32_t>();
+            RV.blk_rec = bp->blk_rec.as<uint4>();
+            RV.seg_v = bp->seg_v.as<uint32_t>();
+            RV.nseg = bp->nseg;
             RV.pad = bp->SC;
             RV.stage_off = st->stage_off;
             RV.dummy = st->dummy_off;
