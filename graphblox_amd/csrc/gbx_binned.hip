@@ -910,14 +910,29 @@ using EpiOf = std::conditional_t<is_fixed_kind(KIND), FixEpi<KIND, T>,
                                  std::conditional_t<is_quant_kind(KIND), QuantEpi<KIND>, PrEpi<KIND, T>>>;
 
 constexpr int kAccumUnroll = 4;  // groups of 8 per lane in flight
+constexpr uint32_t kAccumSlab = 64u * 8u * kAccumUnroll;  // V entries a wave takes at a time
+constexpr int kHotUnroll = 2;                               // groups of 8 hot pairs per lane in flight
+constexpr uint32_t kHotSlab = 64u * 8u * kHotUnroll;        // hot pairs a wave takes at a time
 constexpr int kEpiBatch = 8;     // rows per thread whose update inputs load together
 
+// Thread 0 takes the next unit from the counter and loads its record.
+__device__ __forceinline__ void claim_unit(const AccumView &B, uint32_t *s_w, BinUnit *s_u)
+{
+    const uint32_t w = atomicAdd(B.work, 1u);
+    *s_w = w;
+    if (w < B.NU) *s_u = B.units[w];
+}
+
 // One accumulate unit: row sums of the unit's V range and hot pairs, then the
-// update of its rows (or, for a split bin, the combine).  The hot
-// contributions are already in LDS.
+// update of its rows (or, for a split bin, the combine), while thread 0
+// claims the next unit into s_w, s_u.  The hot contributions are already in
+// LDS, the row sums are zero on entry and the update leaves them zero;
+// s_slab holds the two slab counters (zero on entry, zero again when the
+// update starts).
 template <int KIND, class T>
 __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B, const T *__restrict__ V,
-                                           EpiOf<KIND, T> &epi, double scale, double inv)
+                                           EpiOf<KIND, T> &epi, double scale, double inv, uint32_t *s_slab,
+                                           uint32_t *s_w, BinUnit *s_u)
 {
     const uint32_t nrows = u.row_end - u.row_begin;
     const uint32_t stride = kBlock * 8u;
@@ -972,44 +987,74 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
                 if constexpr (is_int_kind(KIND)) epi(u.row_begin, (uint64_t)tot);
                 else epi(u.row_begin, (float)((double)tot * inv));
             }
+            claim_unit(B, s_w, s_u);
         }
     } else {
         long long *y = reinterpret_cast<long long *>(gbx_dyn_lds);
-        for (uint32_t i = threadIdx.x; i < nrows; i += kBlock) y[i] = 0;
-        __syncthreads();
-        for (uint32_t e = u.hbeg + threadIdx.x * 8u; e < u.hend; e += stride) {
-            uint32_t d[8];
-            RawVal<KIND> v[8];
-            load_hot8_any<KIND, T>(B, e, ch, d, v);
-            // hot pairs are in row order: runs of equal rows summed in registers
-            uint32_t rd = d[0];
-            long long run = 0;
+        // The waves take slabs (kHotSlab hot pairs first, then kAccumSlab V
+        // entries) from an LDS counter: waves that run ahead take more, so the
+        // block reaches the update barrier together.
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t nh = (u.hend - u.hbeg + kHotSlab - 1) / kHotSlab;
+        const uint32_t nv = (u.vend - u.vbeg + kAccumSlab - 1) / kAccumSlab;
+        constexpr uint32_t kq = 64u * 8u;  // entries of one group of 8 per lane
+        for (;;) {
+            uint32_t k = 0;
+            if (lane == 0) k = atomicAdd(&s_slab[0], 1u);
+            k = __builtin_amdgcn_readfirstlane(k);
+            if (k >= nh) break;
+            {
+                const uint32_t e0 = u.hbeg + k * kHotSlab + lane * 8u;
+                // groups past the range load the range's last group and add zeros
+                uint32_t d[kHotUnroll][8];
+                RawVal<KIND> v[kHotUnroll][8];
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const long long x = fixed_of<KIND>(v[i], scale);
-                if (d[i] != rd) {
+                for (int q = 0; q < kHotUnroll; q++) {
+                    const uint32_t e = e0 + q * kq;
+                    const bool in = e < u.hend;
+                    load_hot8_any<KIND, T>(B, in ? e : u.hend - 8u, ch, d[q], v[q]);
+                    if (!in)
+#pragma unroll
+                        for (int i = 0; i < 8; i++) v[q][i] = RawVal<KIND>(0);
+                }
+#pragma unroll
+                for (int q = 0; q < kHotUnroll; q++) {
+                    // hot pairs are in row order: runs of equal rows summed in registers
+                    uint32_t rd = d[q][0];
+                    long long run = 0;
+#pragma unroll
+                    for (int i = 0; i < 8; i++) {
+                        const long long x = fixed_of<KIND>(v[q][i], scale);
+                        if (d[q][i] != rd) {
+                            if (run) atomicAdd(reinterpret_cast<unsigned long long *>(y + rd), (unsigned long long)run);
+                            rd = d[q][i];
+                            run = x;
+                        } else {
+                            run += x;
+                        }
+                    }
                     if (run) atomicAdd(reinterpret_cast<unsigned long long *>(y + rd), (unsigned long long)run);
-                    rd = d[i];
-                    run = x;
-                } else {
-                    run += x;
                 }
             }
-            if (run) atomicAdd(reinterpret_cast<unsigned long long *>(y + rd), (unsigned long long)run);
         }
-        for (uint32_t e0 = u.vbeg + threadIdx.x * 8u; e0 < u.vend; e0 += stride * kAccumUnroll) {
+        for (;;) {
+            uint32_t k = 0;
+            if (lane == 0) k = atomicAdd(&s_slab[1], 1u);
+            k = __builtin_amdgcn_readfirstlane(k);
+            if (k >= nv) break;
+            const uint32_t e0 = u.vbeg + k * kAccumSlab + lane * 8u;
             // all loads of the unrolled groups first, then the sums
             uint4 dw[kAccumUnroll];
             RawVal<KIND> v[kAccumUnroll][8];
 #pragma unroll
             for (int q = 0; q < kAccumUnroll; q++)
-                if (e0 + q * stride < u.vend) {
-                    dw[q] = *reinterpret_cast<const uint4 *>(B.dst_local + e0 + q * stride);
-                    load8<KIND, T>(V, e0 + q * stride, v[q]);
+                if (e0 + q * kq < u.vend) {
+                    dw[q] = *reinterpret_cast<const uint4 *>(B.dst_local + e0 + q * kq);
+                    load8<KIND, T>(V, e0 + q * kq, v[q]);
                 }
 #pragma unroll
             for (int q = 0; q < kAccumUnroll; q++) {
-                if (e0 + q * stride >= u.vend) break;
+                if (e0 + q * kq >= u.vend) break;
                 const uint32_t w[4] = {dw[q].x, dw[q].y, dw[q].z, dw[q].w};
                 // runs of equal rows (segments are in row order) are summed
                 // in registers before one LDS atomic per run
@@ -1031,18 +1076,29 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
             }
         }
         __syncthreads();
-        // the update: inputs of kEpiBatch rows per thread loaded together
-        for (uint32_t i0 = threadIdx.x; i0 < nrows; i0 += kBlock * kEpiBatch) {
+        if (threadIdx.x < 64) {
+            // wave 0 resets the slab counters (every wave has taken its last
+            // slab) and claims the next unit while the other waves update
+            if (threadIdx.x < 2) s_slab[threadIdx.x] = 0;
+            if (threadIdx.x == 0) claim_unit(B, s_w, s_u);
+            return;
+        }
+        // the update on waves 1..: inputs of kEpiBatch rows per thread loaded
+        // together; each row sum is cleared once read
+        constexpr uint32_t kEpiThreads = kBlock - 64;
+        for (uint32_t i0 = threadIdx.x - 64; i0 < nrows; i0 += kEpiThreads * kEpiBatch) {
             typename EpiOf<KIND, T>::In in[kEpiBatch];
 #pragma unroll
             for (int q = 0; q < kEpiBatch; q++)
-                if (i0 + q * kBlock < nrows) in[q] = epi.load(u.row_begin + i0 + q * kBlock);
+                if (i0 + q * kEpiThreads < nrows) in[q] = epi.load(u.row_begin + i0 + q * kEpiThreads);
 #pragma unroll
             for (int q = 0; q < kEpiBatch; q++) {
-                const uint32_t i = i0 + q * kBlock;
+                const uint32_t i = i0 + q * kEpiThreads;
                 if (i < nrows) {
-                    if constexpr (is_int_kind(KIND)) epi.apply(u.row_begin + i, (uint64_t)y[i], in[q]);
-                    else epi.apply(u.row_begin + i, (float)((double)y[i] * inv), in[q]);
+                    const long long yi = y[i];
+                    y[i] = 0;
+                    if constexpr (is_int_kind(KIND)) epi.apply(u.row_begin + i, (uint64_t)yi, in[q]);
+                    else epi.apply(u.row_begin + i, (float)((double)yi * inv), in[q]);
                 }
             }
         }
@@ -1081,15 +1137,24 @@ __global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const ui
         else copy_to_lds(hot, reinterpret_cast<const T *>(A.c_in), B.H);
         if (B.hot_base && threadIdx.x == 0) hot[kHotPad] = T(0);  // packed pairs' pads and fillers
     }
-    __shared__ uint32_t s_unit;
+    // Thread 0 claims the next unit while the other waves update the rows
+    // of the current one, so neither the claim nor the record load waits at
+    // a barrier of its own.
+    __shared__ uint32_t s_w, s_slab[2];
+    __shared__ BinUnit s_u;
+    {
+        long long *y = reinterpret_cast<long long *>(gbx_dyn_lds);
+        for (uint32_t i = threadIdx.x; i < B.hot_off / 8u; i += kBlock) y[i] = 0;
+        if (threadIdx.x < 2) s_slab[threadIdx.x] = 0;
+    }
+    if (threadIdx.x == 0) claim_unit(B, &s_w, &s_u);
     for (;;) {
-        __syncthreads();  // previous unit done with LDS (and the hot load finished)
-        if (threadIdx.x == 0) s_unit = atomicAdd(B.work, 1u);
-        __syncthreads();
-        const uint32_t w = s_unit;
-        if (w >= B.NU) break;
-        const BinUnit u = B.units[w];
-        accum_unit<KIND, T>(u, B, V, epi, scale, inv);
+        __syncthreads();  // next unit claimed; the previous unit done with LDS (and the hot load finished)
+        if (s_w >= B.NU) break;
+        // thread 0 claims again only after the unit's sums barrier, when
+        // every thread has its copy
+        const BinUnit u = s_u;
+        accum_unit<KIND, T>(u, B, V, epi, scale, inv, s_slab, &s_w, &s_u);
     }
     const uint32_t iters = cur->iters;
     const double eps = A.eps;
