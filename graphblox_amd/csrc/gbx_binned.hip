@@ -1598,6 +1598,15 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
         vpos += len;
     }
     if (vpos != bp->Vlen) return set_error(ctx, GBX_ERR_STATE, "binned layout: bin lengths do not add up");
+    // the accumulate blocks take units in this order: the largest (bytes
+    // moved) first, so the last units taken are the smallest
+    const uint32_t vel = std::max(hot_elem, 1u);
+    auto unit_bytes = [&](const BinUnit &un) {
+        return (uint64_t)(un.vend - un.vbeg) * (vel + 2u) + (uint64_t)(un.hend - un.hbeg) * 3u +
+               (uint64_t)(un.row_end - un.row_begin) * 16u;
+    };
+    std::stable_sort(units.begin(), units.end(),
+                     [&](const BinUnit &a, const BinUnit &b) { return unit_bytes(a) > unit_bytes(b); });
     bp->NU = (uint32_t)units.size();
     bp->n_multi = n_multi;
     GBX_TRY(dev_alloc(ctx, bp->units, units.size() * sizeof(BinUnit)));
