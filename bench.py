@@ -163,8 +163,8 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
-                                   "(see DESIGN.md), so --gpus must be 1"}))
+        print(json.dumps({"error": "this library runs on one GPU: sharding over several GPUs is "
+                                   "designed, not built, so --gpus must be 1"}))
         sys.exit(2)
 
     import graphblox_amd as gbx

@@ -10,7 +10,7 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 GRAPHS = [f"fixture_{f}" for f in FIXTURES] + [f"rmat_s{s}" for s in RMAT_SMALL]
-TOL_F32 = 1e-5  # L1 on normalised ranks vs the reference / FP64
+TOL_F32 = 1e-5  # L1 on normalised ranks vs the reference / FP64 (the FP32 parity bar)
 
 
 def _graph(ctx, z, **kw):
