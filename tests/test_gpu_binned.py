@@ -223,3 +223,26 @@ def test_binned_rmat_s20_matches_pull(gpu_ctx):
     e_b, e_p = l1(binned, r64), l1(pull, r64)
     print(f"[gpu] s20 REF L1 vs FP64: binned {e_b:.3e}, pull {e_p:.3e}")
     assert e_b <= TOL_F32 and e_p <= TOL_F32
+
+
+def test_binned_rmat_s24_matches_pull(gpu_ctx):
+    """The benchmark's graph at its full size: R-MAT s24 (16.7M vertices,
+    263M edges), seed 24, the default cut (the largest accumulate units, rows
+    split over several units, units handed out largest first).  The binned
+    iteration against the pull kernel after 5 iterations: FP32 within FP32
+    rounding of the run sums, BF16 within its bar, and the same rank sum
+    (below 1 under the reference's update, which drops dangling mass)."""
+    g = gpu_ctx.rmat_graph(24, 16, 24)
+    print(f"[gpu] binned R-MAT s24 N={g.N} E={g.E}", flush=True)
+    pull, rp = g.pagerank(iterations=5, epsilon=0.0, method=gbx.METHOD_PULL)
+    binned, rb = g.pagerank(iterations=5, epsilon=0.0, method=BINNED)
+    bf16, r16 = g.pagerank(iterations=5, epsilon=0.0, method=BINNED, precision=gbx.BF16)
+    e32, e16 = l1(binned, pull), l1(bf16, pull)
+    print(f"[gpu] s24 binned vs pull: FP32 {e32:.3e}, BF16 {e16:.3e}", flush=True)
+    assert e32 <= 1e-6
+    assert e16 <= 2e-3
+    for rep in (rp, rb, r16):
+        assert rep["executed"] == 5
+    assert abs(rb["rank_sum"] - rp["rank_sum"]) < 1e-6
+    assert abs(r16["rank_sum"] - rp["rank_sum"]) < 2e-3
+    g.free()
