@@ -733,10 +733,17 @@ template <class T>
 __global__ void __launch_bounds__(kRunsBlock) bin_scatter_runs_kernel(RunsView S, const Slot *slots, uint32_t t,
                                                                       const T *__restrict__ c_in, T *__restrict__ V)
 {
-    if (slots[t & 1].done) return;
+    // the stop flag and the chunk's bounds are loaded together (one round
+    // trip before the contributions' loads instead of two); the empty asm
+    // keeps the bound loads ahead of the flag's branch
     const uint32_t k = blockIdx.x;
+    const auto done = slots[t & 1].done;
+    const uint32_t c0 = S.chunk_lo[k], c1 = S.chunk_lo[k + 1];
+    const uint32_t g0 = S.chunk_grp[k], g1 = S.chunk_grp[k + 1];
+    asm volatile("" ::"s"(c0), "s"(c1), "s"(g0), "s"(g1));
+    if (done) return;
     T *cl = reinterpret_cast<T *>(gbx_dyn_lds);
-    const uint32_t c0 = S.chunk_lo[k], n = S.chunk_lo[k + 1] - c0;
+    const uint32_t n = c1 - c0;
     copy_to_lds<T, kRunsBlock>(cl, c_in + c0, n);
     float *stage = reinterpret_cast<float *>(gbx_dyn_lds + S.stage_off) + (threadIdx.x >> 6) * kRunStage;
     const uint32_t lane = threadIdx.x & 63u;
@@ -744,7 +751,6 @@ __global__ void __launch_bounds__(kRunsBlock) bin_scatter_runs_kernel(RunsView S
                                              kRunDummyStride + lane * (uint32_t)sizeof(T);
     if (threadIdx.x == 0) cl[S.pad] = T(0);
     __syncthreads();
-    const uint32_t g0 = S.chunk_grp[k], g1 = S.chunk_grp[k + 1];
     if (g0 >= g1) return;
     const uint32_t padw = S.pad | (S.pad << 16);
     const uint4 padv = make_uint4(padw, padw, padw, padw);
