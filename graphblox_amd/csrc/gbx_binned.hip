@@ -501,13 +501,24 @@ __device__ __forceinline__ void store8(T *V, uint32_t pos, const T (&v)[8])
 }
 
 // n values (whole groups of 8 at 8-value aligned addresses, then a tail)
-// from global memory into LDS.
+// from global memory into LDS.  A thread's loads of kCopyAhead groups are
+// issued before any of them is written, so a fill of a chunk's
+// contributions (4 groups a thread) costs one memory round trip, not four.
+constexpr int kCopyAhead = 4;
 template <class T, int NT = kBlock>
 __device__ __forceinline__ void copy_to_lds(T *dst, const T *src, uint32_t n)
 {
     const uint32_t nv = n / 8;
-    for (uint32_t i = threadIdx.x; i < nv; i += NT)
-        reinterpret_cast<Vec8<T> *>(dst)[i] = reinterpret_cast<const Vec8<T> *>(src)[i];
+    for (uint32_t i0 = threadIdx.x; i0 < nv; i0 += NT * kCopyAhead) {
+        Vec8<T> w[kCopyAhead];
+        // no branches: indexes past the last group are clamped to it, so a
+        // lane past the end loads and writes the last group again (the same
+        // value to the same address)
+#pragma unroll
+        for (int u = 0; u < kCopyAhead; u++) w[u] = reinterpret_cast<const Vec8<T> *>(src)[min(i0 + u * NT, nv - 1u)];
+#pragma unroll
+        for (int u = 0; u < kCopyAhead; u++) reinterpret_cast<Vec8<T> *>(dst)[min(i0 + u * NT, nv - 1u)] = w[u];
+    }
     for (uint32_t i = nv * 8 + threadIdx.x; i < n; i += NT) dst[i] = src[i];
 }
 
