@@ -25,6 +25,10 @@ import os
 import sys
 import time
 
+# the CPU baseline's OpenMP threads spread over the cores (read when the
+# oracle library's OpenMP runtime starts)
+os.environ.setdefault("OMP_PROC_BIND", "spread")
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -48,40 +52,82 @@ def traffic_from_profiles(workload_key: str):
     return t.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(g, iterations: int, threads: int, gpu_ranks=None):
-    """Reference pageRankPullGraphCSR (oracle/_ref) on this graph's CSR arrays,
-    or the oracle's restatement of it when the reference build is absent.
-    With gpu_ranks (the measured path's ranks after the same iterations) the
-    record also holds their L1 distance from the reference's ranks."""
-    import ctypes as C
+def host_cores():
+    """Logical CPUs of the machine, CPUs this process may run on, physical cores."""
+    logical = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = logical
+    phys = set()
+    try:
+        pid = cid = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("physical id"):
+                    pid = line.split(":")[1].strip()
+                elif line.startswith("core id"):
+                    cid = line.split(":")[1].strip()
+                elif not line.strip() and pid is not None:
+                    phys.add((pid, cid))
+                    pid = cid = None
+    except OSError:
+        pass
+    return logical, allowed, len(phys) or logical
+
+
+def baseline_threads():
+    """The OpenMP threads of the CPU baseline: OMP_NUM_THREADS when set (the
+    host's CPU share for this GPU), else every CPU this process may use."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.strip().isdigit() and int(env) > 0:
+        return int(env)
+    return host_cores()[1]
+
+
+def csr_for_oracle(g):
     from oracle import oracle as O
 
     fwd = g.export(inverse=False)
     inv = g.export(inverse=True)
-    N, E = g.N, g.E
-    ranks = np.zeros(N, np.float32)
-    it, err, t = C.c_uint32(), C.c_double(), C.c_double()
-    kind = "reference"
-    if O.ref_available(1, 0):
-        L = O.ref(1, 0)
-        t0 = time.perf_counter()
-        L.refd_pagerank_on_csr(N, E, fwd["out_degree"], inv["out_degree"], inv["edges_idx"], inv["dst"],
-                               iterations, 0.0, threads, ranks, C.byref(it), C.byref(err), C.byref(t))
-        wall = time.perf_counter() - t0
-        secs = t.value if t.value > 0 else wall
-    else:
-        kind = "port"
-        fo = O.OracleCSR(N, E, fwd["out_degree"], fwd["in_degree"], fwd["edges_idx"], fwd["src"], fwd["dst"], fwd["w"])
-        io = O.OracleCSR(N, E, inv["out_degree"], inv["in_degree"], inv["edges_idx"], inv["src"], inv["dst"], inv["w"])
-        t0 = time.perf_counter()
-        ranks = O.pr_pull_f32(fo, io, iterations, 0.0, nthreads=threads)["ranks"]
-        secs = time.perf_counter() - t0
-    gteps = E * iterations / secs / 1e9
-    rec = dict(value=round(gteps, 4), unit="GTEPS", cores=threads, kind=kind,
-               sample=f"{iterations} iterations of pull PageRank (FP32, reference semantics) on the same "
-                      f"R-MAT graph (N={N}, E={E}), {threads} OpenMP threads, whole loop timed")
+    mk = lambda c: O.OracleCSR(g.N, g.E, c["out_degree"], c["in_degree"], c["edges_idx"], c["src"], c["dst"], c["w"])
+    return mk(fwd), mk(inv)
+
+
+def fp64_errors(ranks, ref64):
+    """L1 and largest relative error of ranks against an FP64 result."""
+    r = np.asarray(ranks, np.float64)
+    d = np.abs(r - ref64)
+    den = np.maximum(np.abs(ref64), 1e-300)
+    return {"l1_vs_fp64": float(d.sum()), "max_rel_vs_fp64": float((d / den).max())}
+
+
+def cpu_baseline(fo, io, iterations: int, threads: int, gpu_ranks=None):
+    """The oracle's restatement of pageRankPullGraphCSR (pageRank.c:245-368;
+    its ranks, error and stopping iteration are bit-exact with the reference's
+    own build, tests/test_oracle.py) on this graph's CSR arrays: `iterations`
+    iterations with `threads` OpenMP threads, each iteration timed; the value
+    is E / (median time of iterations 2..n).  With gpu_ranks (the measured
+    GPU path's ranks after the same iterations) the record holds their L1
+    distance from the port's ranks."""
+    from oracle import oracle as O
+
+    N, E = fo.N, fo.E
+    t0 = time.perf_counter()
+    out = O.pr_pull_f32(fo, io, iterations, 0.0, nthreads=threads, timings=True)
+    wall = time.perf_counter() - t0
+    ts = np.asarray(out["iter_seconds"], np.float64)
+    med = float(np.median(ts[1:])) if ts.size > 1 else float(ts[0])
+    logical, allowed, physical = host_cores()
+    rec = dict(value=round(E / med / 1e9, 4), unit="GTEPS", cores=threads, kind="port",
+               sample=f"{iterations} iterations of pull PageRank (FP32, reference semantics, the oracle's "
+                      f"OpenMP restatement of pageRankPullGraphCSR) on the same R-MAT graph (N={N}, E={E}), "
+                      f"{threads} OpenMP threads, OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND', '')}; "
+                      f"value = E / median of iterations 2..{iterations}",
+               median_iter_ms=round(med * 1e3, 3), first_iter_ms=round(float(ts[0]) * 1e3, 3),
+               wall_s=round(wall, 3), logical_cpus=logical, allowed_cpus=allowed, physical_cores=physical)
     if gpu_ranks is not None:
-        rec["l1_gpu_vs_reference"] = float(np.abs(np.asarray(gpu_ranks, np.float64) - ranks).sum())
+        rec["l1_gpu_vs_port"] = float(np.abs(np.asarray(gpu_ranks, np.float64) - out["ranks"]).sum())
     return rec
 
 
@@ -150,8 +196,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--edge-factor", type=int, default=16)
-    ap.add_argument("--cpu-iters", type=int, default=5)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: OMP_NUM_THREADS, else every allowed CPU")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the BF16 and batched forms")
     ap.add_argument("--method", choices=("binned", "pull"), default="binned",
@@ -253,14 +299,22 @@ def main():
     log(f"convergence: {convergence}")
     cpu = None
     if not args.no_cpu:
+        from oracle import oracle as O
+
         log("CPU baseline ...")
         cs = g.session(iterations=args.cpu_iters, epsilon=0.0, mode=gbx.PR_REF, stop=gbx.STOP_NONE, method=method)
         cs.reset()
         cs.launch(args.cpu_iters)
         gpu_ranks, _ = cs.read()
         cs.free()
-        cpu = cpu_baseline(g, args.cpu_iters, args.cpu_threads, gpu_ranks)
+        fo, io = csr_for_oracle(g)
+        threads = args.cpu_threads or baseline_threads()
+        cpu = cpu_baseline(fo, io, args.cpu_iters, threads, gpu_ranks)
         log(f"CPU baseline {cpu['value']} GTEPS ({cpu['kind']}, {cpu['cores']} threads)")
+        # the headline's ranks (after `steps` iterations) against the FP64 oracle
+        r64, _, _ = O.pr_f64(N, fo.out_degree, O.row_ptr_from(io), io.dst, mode=0, iterations=args.steps)
+        headline_err = fp64_errors(ranks, r64)
+        log(f"headline vs FP64: {headline_err}")
     out = {
         "metric": "GTEPS & % HBM roofline, pull PageRank R-MAT s24, 1 MI355X, vs plain FP32",
         "value": round(value, 3),
@@ -294,6 +348,7 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": cpu,
+        "error_vs_fp64": headline_err if cpu is not None else None,
         "convergence": convergence,
         "forms": forms,
     }

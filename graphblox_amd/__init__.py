@@ -117,6 +117,7 @@ def lib():
         "gbx_pr_plan_info": (i32, [_vp, C.POINTER(u32), _vp, _vp, _vp]),
         "gbx_spmv_pull": (i32, [_vp, _vp, u32, i32, i32, _vp, _vp, _vp, C.POINTER(f64)]),
         "gbx_spmv_fixed": (i32, [_vp, _vp, u32, i32, i32, i32, _vp, _vp, C.POINTER(f64)]),
+        "gbx_gather_accumulate_u32": (i32, [_vp, _vp, i32, _vp, _vp]),
         "gbx_ppr_create": (i32, [_vp, _vp, C.POINTER(PrParams), u32, _vp, _vp, C.POINTER(_vp)]),
         "gbx_ppr_info": (i32, [_vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
         "gbx_pr_launch_graph": (i32, [_vp, u32]),
@@ -147,7 +148,7 @@ def exported_symbols():
         "gbx_pr_launch_timed", "gbx_pr_read", "gbx_pr_free", "gbx_pr_plan_info", "gbx_spmv_pull", "gbx_spmv_fixed",
         "gbx_ppr_create", "gbx_ppr_info", "gbx_pr_launch_graph", "gbx_graph_batch_create",
         "gbx_graph_batch_info", "gbx_tuning_get", "gbx_tuning_set", "gbx_autotune",
-        "gbx_pr_adjoint",
+        "gbx_pr_adjoint", "gbx_gather_accumulate_u32",
     ]
 
 
@@ -375,6 +376,16 @@ class Graph:
         self.ctx.check(self.ctx.L.gbx_spmv_pull(self.ctx.h, self.h, iterations, int(weighted), direction, _ptr(xi),
                                                 _ptr(xo), _ptr(yo), C.byref(sec)))
         return xo, yo, sec.value
+
+    def gather_accumulate_u32(self, aux1, aux2=None, direction=PULL_IN):
+        """The FPGA overlay's Single.PR integer pull on u32[2N] buffers:
+        aux2[N + v] = sum of aux1[N + u] over row v (mod 2^32)."""
+        a1 = _u32(aux1)
+        if a1.size != 2 * self.N:
+            raise ValueError("aux1 must hold 2N words")
+        a2 = np.zeros(2 * self.N, np.uint32) if aux2 is None else _u32(aux2).copy()
+        self.ctx.check(self.ctx.L.gbx_gather_accumulate_u32(self.ctx.h, self.h, direction, _ptr(a1), _ptr(a2)))
+        return a2
 
     def spmv_fixed(self, bits, iterations=1, weighted=False, direction=PULL_IN):
         """The reference's fixed-point SpMV variants (bits 32: pull, 64: push)."""

@@ -26,8 +26,12 @@ HIP_SOURCES = ["gbx_core.hip", "gbx_sort.hip", "gbx_rmat.hip", "gbx_pr.hip", "gb
 HOST_SOURCES = ["graphblox_hip.c", "graphstats_hip.c"]
 
 LIB_HIP = os.path.join(PKG, "libgbx_hip.so")
-LIB_HOST = os.path.join(PKG, "libgraphblox_hip.so")        # WEIGHTED=0 layout
-LIB_HOST_W = os.path.join(PKG, "libgraphblox_hip_w.so")    # WEIGHTED=1 layout
+LIB_HOST = os.path.join(PKG, "libgraphblox_hip.so")        # DIRECTED=1, WEIGHTED=0 layout
+LIB_HOST_W = os.path.join(PKG, "libgraphblox_hip_w.so")    # DIRECTED=1, WEIGHTED=1 layout
+# the adapter in each (DIRECTED, WEIGHTED) layout of the reference's compile-time switches
+HOST_VARIANTS = {(1, 0): LIB_HOST, (1, 1): LIB_HOST_W,
+                 (0, 0): os.path.join(PKG, "libgraphblox_hip_d0.so"),
+                 (0, 1): os.path.join(PKG, "libgraphblox_hip_d0_w.so")}
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -56,17 +60,19 @@ def _run(cmd, cwd=None):
 
 
 def build_hip(verbose: bool = False) -> str:
+    from concurrent.futures import ThreadPoolExecutor
+
     hipcc = _hipcc()
     objdir = os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
-    for src in HIP_SOURCES:
-        obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = [hipcc, *HIPCC_FLAGS, "-I", INC, "-c", os.path.join(CSRC, src), "-o", obj]
-        out = _run(cmd)
-        if verbose and out.strip():
-            print(out)
-        objs.append(obj)
+    objs = [os.path.join(objdir, src.replace(".hip", ".o")) for src in HIP_SOURCES]
+    cmds = [[hipcc, *HIPCC_FLAGS, "-I", INC, "-c", os.path.join(CSRC, src), "-o", obj]
+            for src, obj in zip(HIP_SOURCES, objs)]
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", 0)) or (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        for out in pool.map(_run, cmds):
+            if verbose and out.strip():
+                print(out)
     tmp = LIB_HIP + ".tmp"
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB_HIP)
@@ -75,10 +81,10 @@ def build_hip(verbose: bool = False) -> str:
 
 def build_host(verbose: bool = False) -> str:
     cc = shutil.which("gcc") or "cc"
-    for weighted, lib in ((0, LIB_HOST), (1, LIB_HOST_W)):
+    for (directed, weighted), lib in HOST_VARIANTS.items():
         tmp = lib + ".tmp"
         cmd = [cc, "-O2", "-fPIC", "-shared", "-std=gnu11", "-Wall", "-Wextra", f"-DGBX_WEIGHTED={weighted}",
-               "-I", INC, *[os.path.join(HOST, s) for s in HOST_SOURCES],
+               f"-DGBX_DIRECTED={directed}", "-I", INC, *[os.path.join(HOST, s) for s in HOST_SOURCES],
                "-o", tmp, "-L", PKG, "-lgbx_hip", "-Wl,-rpath,$ORIGIN", "-lm"]
         out = _run(cmd)
         if verbose and out.strip():

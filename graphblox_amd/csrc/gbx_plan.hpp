@@ -5,6 +5,7 @@
 #include "gbx_internal.hpp"
 
 #include <memory>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -145,7 +146,7 @@ struct gbx_pr {
     gbx_seg_state *seg = nullptr;    // batch-of-graphs state (gbx_batch.hip), else null
     gbx_binned_state *bin = nullptr; // binned-method state (gbx_binned.hip), else null
     gbx_ddp_state *ddp = nullptr;    // data-driven push state (gbx_ddpush.hip), else null
-    // captured iteration sequences: key = (iterations << 1) | starting parity
+    // captured iteration sequences: key = (iterations << 2) | (teleport set << 1) | starting parity
     std::vector<std::pair<uint64_t, hipGraphExec_t>> graph_cache;
 };
 
@@ -221,6 +222,17 @@ struct AccKahan {
         merge(o);
     }
     __device__ __forceinline__ float value() const { return s - c; }
+};
+
+// Unsigned 32-bit sum with wrap-around: the FPGA overlay's ALU_ACC
+// (engine_alu_ops_kernel.sv:104-112), exact in any order.
+struct AccU32 {
+    uint32_t s;
+    __device__ __forceinline__ void init(uint32_t v) { s = v; }
+    __device__ __forceinline__ void add(uint32_t x) { s += x; }
+    __device__ __forceinline__ void merge(const AccU32 &o) { s += o.s; }
+    __device__ __forceinline__ void shfl_merge(int off) { s += (uint32_t)__shfl_xor((int)s, off, 64); }
+    __device__ __forceinline__ uint32_t value() const { return s; }
 };
 
 // Data-driven pull: contributions carry "updated in the previous iteration"
@@ -332,15 +344,17 @@ struct GatherConstW {
 };
 
 // Lane-strided, in-order partial sum of one row: GBX_UNROLL column loads,
-// then GBX_UNROLL gathers in flight; the tail is one predicated batch.
+// then GBX_UNROLL gathers in flight; the tail is one predicated batch.  The
+// gathered values are float, or uint32_t for the integer accumulator.
 template <int G, class Acc, class Gather>
 __device__ __forceinline__ void lane_sum(uint32_t j, uint32_t end, const uint32_t *__restrict__ col,
                                          const Gather &g, Acc &acc)
 {
     constexpr uint32_t U = GBX_UNROLL;
+    using Val = decltype(g(0u, 0u));
     for (; j + (U - 1) * G < end; j += U * G) {
         uint32_t c[U];
-        float t[U];
+        Val t[U];
 #pragma unroll
         for (uint32_t k = 0; k < U; k++) c[k] = ld_stream(col + j + k * G);
 #pragma unroll
@@ -350,11 +364,11 @@ __device__ __forceinline__ void lane_sum(uint32_t j, uint32_t end, const uint32_
     }
     if (j < end) {
         uint32_t c[U];
-        float t[U];
+        Val t[U];
 #pragma unroll
         for (uint32_t k = 0; k < U; k++) c[k] = (j + k * G < end) ? ld_stream(col + j + k * G) : 0u;
 #pragma unroll
-        for (uint32_t k = 0; k < U; k++) t[k] = (j + k * G < end) ? g(j + k * G, c[k]) : 0.0f;
+        for (uint32_t k = 0; k < U; k++) t[k] = (j + k * G < end) ? g(j + k * G, c[k]) : Val(0);
 #pragma unroll
         for (uint32_t k = 0; k < U; k++)
             if (j + k * G < end) acc.add(t[k]);
@@ -369,8 +383,9 @@ __device__ __forceinline__ void do_rows(const ClassDesc &cd, uint32_t local, con
     const uint32_t grp = threadIdx.x / G, lig = threadIdx.x % G;
     const uint32_t row = cd.row_begin + local * RPT + grp;
     const bool valid = row < cd.row_end;
+    using Val = decltype(epi.init(0u));
     Acc acc;
-    acc.init((G == 1 && valid) ? epi.init(row) : 0.0f);
+    acc.init((G == 1 && valid) ? epi.init(row) : Val(0));
     if (valid) {
         const uint32_t beg = ld_stream(P.rp + row), end = ld_stream(P.rp + row + 1);
         lane_sum<G>(beg + lig, end, P.col, g, acc);
@@ -386,9 +401,12 @@ template <class Acc, class Gather, class Epi>
 __device__ __forceinline__ void do_hub(uint32_t tile, const PlanView &P, const Gather &g, Epi &epi)
 {
     __shared__ Acc red[kBlock / 64];
+    using Val = decltype(epi.init(0u));
+    // chunk partials: float bits, or the integer accumulator's own value
+    using Part = std::conditional_t<std::is_same_v<Val, uint32_t>, uint32_t, float>;
     const HubChunk hc = P.chunks[tile];
     Acc acc;
-    acc.init(0.0f);
+    acc.init(Val(0));
     lane_sum<kBlock>(hc.ebeg + threadIdx.x, hc.eend, P.col, g, acc);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc.shfl_merge(off);
@@ -402,14 +420,14 @@ __device__ __forceinline__ void do_hub(uint32_t tile, const PlanView &P, const G
             // the whole row in one chunk: no cross-block combine, no fences
             epi(hc.row, epi.init(hc.row) + s.value());
         } else {
-            P.hub_part[tile] = s.value();
+            reinterpret_cast<Part *>(P.hub_part)[tile] = s.value();
             __threadfence();
             uint32_t prev = atomicAdd(&P.hub_cnt[hc.hub], 1u);
             if (prev == hc.nch - 1) {
                 __threadfence();
                 Acc tot;
-                tot.init(0.0f);
-                const volatile float *hp = P.hub_part;
+                tot.init(Val(0));
+                const volatile Part *hp = reinterpret_cast<const volatile Part *>(P.hub_part);
                 for (uint32_t k = 0; k < hc.nch; k++) tot.add(hp[hc.first + k]);
                 P.hub_cnt[hc.hub] = 0;
                 epi(hc.row, epi.init(hc.row) + tot.value());
@@ -487,5 +505,10 @@ int ensure_plan(gbx_ctx *ctx, gbx_graph *g, int direction, gbx_plan **out);
 int plan_rows_at_least(gbx_ctx *ctx, const gbx_plan *p, uint32_t threshold, uint32_t *count);
 // Row pointers rp[0..nrows] of the plan, copied to host.
 int plan_row_ptr_prefix(gbx_ctx *ctx, const gbx_plan *p, uint32_t nrows, std::vector<uint32_t> &out);
+// Kernel view of a plan with per-edge weights w (nullable), and the grid of
+// its pull kernels on this context.
+PlanView plan_view(const gbx_plan *p, const float *w);
+unsigned iter_grid(const gbx_ctx *ctx, const gbx_plan *p);
+
 
 }  // namespace gbx

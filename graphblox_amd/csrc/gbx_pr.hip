@@ -142,6 +142,44 @@ spmv_kernel(PlanView P, const float *__restrict__ x, float *__restrict__ y)
     }
 }
 
+// ------------------------------------------------------------------ integer gather-accumulate
+// The FPGA overlay's Single.PR program (02_device/ol/GraphBlox/Single/
+// topology.json:162-201, host side buffer_mapping.PR.cpp:2-68): for each row
+// v, read its edge range, gather the u32 value of every column u and add
+// them with ALU_ACC's unsigned 32-bit add (engine_alu_ops_kernel.sv:104-112).
+struct GatherU32 {
+    const uint32_t *__restrict__ v;
+    __device__ __forceinline__ uint32_t operator()(uint32_t, uint32_t c) const { return v[c]; }
+};
+
+struct U32Epi {
+    uint32_t *__restrict__ out;
+    __device__ __forceinline__ uint32_t init(uint32_t) const { return 0u; }
+    __device__ __forceinline__ void operator()(uint32_t i, uint32_t acc) { out[i] = acc; }
+};
+
+__global__ void __launch_bounds__(kBlock)
+gather_acc_u32_kernel(PlanView P, const uint32_t *__restrict__ in, uint32_t *__restrict__ out)
+{
+    U32Epi epi{out};
+    GatherU32 g{in};
+    tile_loop<AccU32>(P, g, epi);
+}
+
+__global__ void gather_perm_u32(const uint32_t *__restrict__ in, const uint32_t *__restrict__ perm, uint32_t N,
+                                uint32_t *__restrict__ out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        out[i] = in[perm[i]];
+}
+
+__global__ void scatter_perm_u32(const uint32_t *__restrict__ in, const uint32_t *__restrict__ perm, uint32_t N,
+                                 uint32_t *__restrict__ out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        out[perm[i]] = in[i];
+}
+
 // ------------------------------------------------------------------ plan build
 
 __global__ void degree_keys(const uint32_t *__restrict__ deg, uint32_t N, uint32_t maxdeg,
@@ -682,7 +720,7 @@ int plan_row_ptr_prefix(gbx_ctx *ctx, const gbx_plan *p, uint32_t nrows, std::ve
     return GBX_OK;
 }
 
-static PlanView plan_view(const gbx_plan *p, const float *w)
+PlanView plan_view(const gbx_plan *p, const float *w)
 {
     PlanView v;
     v.rp = p->rp.as<uint32_t>();
@@ -700,7 +738,7 @@ static PlanView plan_view(const gbx_plan *p, const float *w)
     return v;
 }
 
-static unsigned iter_grid(gbx_ctx *ctx, const gbx_plan *p)
+unsigned iter_grid(const gbx_ctx *ctx, const gbx_plan *p)
 {
     const unsigned per_cu = ctx->tune.blocks_per_cu;
     unsigned g = (unsigned)std::max(1, ctx->num_cus) * per_cu;
@@ -1100,7 +1138,9 @@ int gbx_pr_launch_graph(gbx_pr *pr, uint32_t n_iters)
     gbx_ctx *ctx = pr->ctx;
     GBX_HIP(ctx, hipSetDevice(ctx->device));
     if (n_iters == 0) return GBX_OK;
-    const uint64_t key = ((uint64_t)n_iters << 1) | (pr->t & 1u);
+    // the captured launches hold the teleport pointer (or none) and the
+    // ping-pong parity they start from
+    const uint64_t key = ((uint64_t)n_iters << 2) | ((pr->has_tele ? 1u : 0u) << 1) | (pr->t & 1u);
     hipGraphExec_t exec = nullptr;
     for (auto &e : pr->graph_cache)
         if (e.first == key) exec = e.second;
@@ -1417,6 +1457,39 @@ int gbx_spmv_pull(gbx_ctx *ctx, gbx_graph *g, uint32_t iterations, int weighted,
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (seconds) *seconds = ms * 1e-3;
+    return GBX_OK;
+}
+
+int gbx_gather_accumulate_u32(gbx_ctx *ctx, gbx_graph *g, int direction, const uint32_t *aux1, uint32_t *aux2)
+{
+    if (!ctx || !g || !aux1 || !aux2) return set_error(ctx, GBX_ERR_ARG, "gbx_gather_accumulate_u32: NULL argument");
+    GBX_HIP(ctx, hipSetDevice(ctx->device));
+    gbx_plan *plan = nullptr;
+    GBX_TRY(ensure_plan(ctx, g, direction, &plan));
+    if (plan->segmented)
+        return set_error(ctx, GBX_ERR_STATE, "gbx_gather_accumulate_u32: not available for a batch of graphs");
+    const uint32_t N = g->N;
+    const size_t nb = (size_t)N * 4;
+    DevBuf in, in_plan, out_plan, out;
+    GBX_TRY(dev_alloc(ctx, in, nb));
+    GBX_TRY(dev_alloc(ctx, in_plan, nb));
+    GBX_TRY(dev_alloc(ctx, out_plan, nb));
+    GBX_TRY(dev_alloc(ctx, out, nb));
+    // the second halves of the two 2N-word buffers hold the vertex values
+    // (the READ_WRITE engines' Index_Start = N, buffer_mapping.PR.cpp:30,58)
+    GBX_HIP(ctx, hipMemcpyAsync(in.p, aux1 + N, nb, hipMemcpyHostToDevice, ctx->stream));
+    const unsigned gN = grid_for(N, 256, 16384);
+    const uint32_t *perm = plan->perm.as<uint32_t>();
+    gather_perm_u32<<<gN, 256, 0, ctx->stream>>>(in.as<uint32_t>(), perm, N, in_plan.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    PlanView P = plan_view(plan, nullptr);
+    gather_acc_u32_kernel<<<iter_grid(ctx, plan), kBlock, 0, ctx->stream>>>(P, in_plan.as<uint32_t>(),
+                                                                           out_plan.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    scatter_perm_u32<<<gN, 256, 0, ctx->stream>>>(out_plan.as<uint32_t>(), perm, N, out.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    GBX_HIP(ctx, hipMemcpyAsync(aux2 + N, out.p, nb, hipMemcpyDeviceToHost, ctx->stream));
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GBX_OK;
 }
 

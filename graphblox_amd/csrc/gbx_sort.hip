@@ -9,6 +9,8 @@
 // sort is an LSD radix sort with 8-bit digits; the stable in-tile rank of
 // each element comes from a wavefront multisplit (64-bit ballots over the
 // digit bits), so no atomics decide the order.
+#include <algorithm>
+
 #include "gbx_internal.hpp"
 
 namespace gbx {
@@ -239,6 +241,32 @@ __global__ void check_sorted(const uint32_t *__restrict__ src, const uint32_t *_
         if ((src[i] & kIdMask) >= N || (dst[i] & kIdMask) >= N) f |= 2u;
         if (f) atomicOr(flag, f);
     }
+}
+
+// The vertex arrays of a CSR against its sorted edges: flag[0] |= 4 if an
+// edge e of source v = src[e] lies outside [edges_idx[v], edges_idx[v] +
+// out_degree[v]); *degsum += sum of out_degree.  With src sorted, every
+// edge inside its source's range and the degrees summing to E, each range
+// holds exactly its vertex's edges (no range can hold more edges than its
+// length), which is what the plan build indexes with.
+__global__ void check_csr_index(const uint32_t *__restrict__ src, uint32_t E, const uint32_t *__restrict__ od,
+                                const uint32_t *__restrict__ idx, uint32_t N, uint32_t *__restrict__ flag,
+                                unsigned long long *__restrict__ degsum)
+{
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t bad = 0;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
+        const uint32_t v = src[e] & kIdMask;
+        if (v >= N) continue;  // reported by check_sorted
+        const uint64_t b = idx[v];
+        if ((uint64_t)e < b || (uint64_t)e >= b + od[v]) bad = 4u;
+    }
+    if (bad) atomicOr(flag, bad);
+    unsigned long long s = 0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += stride) s += od[v];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(degsum, s);
 }
 
 __global__ void fill_f32_kernel(float *p, float v, size_t n)
@@ -520,8 +548,9 @@ int gbx_graph_from_csr(gbx_ctx *ctx, uint32_t num_vertices, uint32_t num_edges,
     *out = nullptr;
     if (num_vertices == 0 || num_vertices > kMaxVertices)
         return set_error(ctx, GBX_ERR_ARG, "gbx_graph_from_csr: num_vertices %u", num_vertices);
-    if (!fwd_out_degree || !fwd_edges_idx || !inv_out_degree || !inv_edges_idx ||
-        (num_edges && (!fwd_src || !fwd_dst || !inv_src || !inv_dst)))
+    const bool fwd_only = !inv_out_degree && !inv_edges_idx && !inv_src && !inv_dst && !inv_w;
+    if (!fwd_out_degree || !fwd_edges_idx || (num_edges && (!fwd_src || !fwd_dst)) ||
+        (!fwd_only && (!inv_out_degree || !inv_edges_idx || (num_edges && (!inv_src || !inv_dst)))))
         return set_error(ctx, GBX_ERR_ARG, "gbx_graph_from_csr: NULL array");
     GBX_HIP(ctx, hipSetDevice(ctx->device));
     gbx_graph *g = new (std::nothrow) gbx_graph();
@@ -529,7 +558,7 @@ int gbx_graph_from_csr(gbx_ctx *ctx, uint32_t num_vertices, uint32_t num_edges,
     g->ctx = ctx;
     g->N = num_vertices;
     g->E = num_edges;
-    g->weighted = fwd_w && inv_w;
+    g->weighted = fwd_w && (fwd_only || inv_w);
     const size_t nb = (size_t)num_vertices * 4, eb = (size_t)num_edges * 4;
     struct Side {
         gbx_csr_dev *d;
@@ -538,7 +567,8 @@ int gbx_graph_from_csr(gbx_ctx *ctx, uint32_t num_vertices, uint32_t num_edges,
     } sides[2] = {{&g->fwd, fwd_out_degree, fwd_edges_idx, fwd_src, fwd_dst, fwd_w},
                   {&g->inv, inv_out_degree, inv_edges_idx, inv_src, inv_dst, inv_w}};
     int rc = GBX_OK;
-    for (auto &sd : sides) {
+    for (int k = 0; k < 2 && !rc; k++) {
+        Side &sd = sides[k];
         if (!rc) rc = dev_alloc(ctx, sd.d->out_degree, nb);
         if (!rc) rc = dev_alloc(ctx, sd.d->in_degree, nb);
         if (!rc) rc = dev_alloc(ctx, sd.d->edges_idx, nb);
@@ -546,6 +576,17 @@ int gbx_graph_from_csr(gbx_ctx *ctx, uint32_t num_vertices, uint32_t num_edges,
         if (!rc) rc = dev_alloc(ctx, sd.d->dst, eb);
         if (!rc) rc = dev_alloc(ctx, sd.d->w, eb);
         if (rc) break;
+        if (k == 1 && fwd_only) {
+            // the transpose of the forward CSR: swapped edges, stable sort by the
+            // new source (edgeList.c:799-801, then countsort.c:28-140)
+            if (eb && (hipMemcpyAsync(sd.d->src.p, g->fwd.dst.p, eb, hipMemcpyDeviceToDevice, ctx->stream) ||
+                       hipMemcpyAsync(sd.d->dst.p, g->fwd.src.p, eb, hipMemcpyDeviceToDevice, ctx->stream) ||
+                       hipMemcpyAsync(sd.d->w.p, g->fwd.w.p, eb, hipMemcpyDeviceToDevice, ctx->stream))) {
+                rc = set_error(ctx, GBX_ERR_HIP, "gbx_graph_from_csr: transpose copy failed");
+                break;
+            }
+            continue;  // sorted after the forward side has been checked
+        }
         if (hipMemcpyAsync(sd.d->out_degree.p, sd.od, nb, hipMemcpyHostToDevice, ctx->stream) ||
             hipMemcpyAsync(sd.d->edges_idx.p, sd.idx, nb, hipMemcpyHostToDevice, ctx->stream) ||
             (eb && hipMemcpyAsync(sd.d->src.p, sd.src, eb, hipMemcpyHostToDevice, ctx->stream)) ||
@@ -561,27 +602,48 @@ int gbx_graph_from_csr(gbx_ctx *ctx, uint32_t num_vertices, uint32_t num_edges,
                                                                                    num_edges);
         }
     }
+    // every uploaded side: sorted sources, ids below N, vertex arrays that
+    // agree with the edges (checked before any kernel indexes with them)
+    const int nchecked = fwd_only ? 1 : 2;
+    if (!rc) {
+        DevBuf flag;
+        uint32_t hflag[4] = {0, 0, 0, 0};
+        rc = dev_alloc(ctx, flag, 32);
+        if (!rc) rc = fill_u32(ctx, flag.as<uint32_t>(), 0, 8);
+        for (int k = 0; k < nchecked && !rc; k++) {
+            const Side &sd = sides[k];
+            if (num_edges)
+                check_sorted<<<grid_for(num_edges, 256, 16384), 256, 0, ctx->stream>>>(
+                    sd.d->src.as<uint32_t>(), sd.d->dst.as<uint32_t>(), num_edges, num_vertices,
+                    flag.as<uint32_t>());
+            check_csr_index<<<grid_for(std::max(num_edges, num_vertices), 256, 16384), 256, 0, ctx->stream>>>(
+                sd.d->src.as<uint32_t>(), num_edges, sd.d->out_degree.as<uint32_t>(), sd.d->edges_idx.as<uint32_t>(),
+                num_vertices, flag.as<uint32_t>(), reinterpret_cast<unsigned long long *>(flag.as<uint32_t>() + 2 + 2 * k));
+            if (hipGetLastError() != hipSuccess) rc = set_error(ctx, GBX_ERR_HIP, "gbx_graph_from_csr: check launch");
+        }
+        uint64_t degsum[2] = {0, 0};
+        if (!rc && (hipMemcpyAsync(hflag, flag.p, 16, hipMemcpyDeviceToHost, ctx->stream) ||
+                    hipMemcpyAsync(degsum, flag.as<uint32_t>() + 2, 16, hipMemcpyDeviceToHost, ctx->stream) ||
+                    hipStreamSynchronize(ctx->stream)))
+            rc = set_error(ctx, GBX_ERR_HIP, "gbx_graph_from_csr: check failed");
+        for (int k = 0; k < nchecked; k++)
+            if (degsum[k] != num_edges) hflag[0] |= 4u;
+        if (!rc && (hflag[0] & 2)) rc = set_error(ctx, GBX_ERR_RANGE, "gbx_graph_from_csr: vertex id >= num_vertices");
+        if (!rc && (hflag[0] & 1)) rc = set_error(ctx, GBX_ERR_ARG, "gbx_graph_from_csr: edges not sorted by source");
+        if (!rc && (hflag[0] & 4))
+            rc = set_error(ctx, GBX_ERR_ARG, "gbx_graph_from_csr: out_degree / edges_idx do not match the sorted "
+                                             "edges (an edge outside its source's range, or degrees not summing "
+                                             "to num_edges)");
+    }
+    if (!rc && fwd_only)
+        rc = build_csr_device(ctx, g->inv.src.as<uint32_t>(), g->inv.dst.as<uint32_t>(), g->inv.w.as<float>(),
+                              num_edges, num_vertices, GBX_SORT_COUNT_SRC, g->weighted,
+                              g->inv.out_degree.as<uint32_t>(), g->inv.edges_idx.as<uint32_t>());
     if (!rc) {
         // in_degree of each side = out_degree of the other (vertex.c:291-311)
         if (hipMemcpyAsync(g->fwd.in_degree.p, g->inv.out_degree.p, nb, hipMemcpyDeviceToDevice, ctx->stream) ||
             hipMemcpyAsync(g->inv.in_degree.p, g->fwd.out_degree.p, nb, hipMemcpyDeviceToDevice, ctx->stream))
             rc = set_error(ctx, GBX_ERR_HIP, "gbx_graph_from_csr: copy failed");
-    }
-    if (!rc && num_edges) {
-        DevBuf flag;
-        uint32_t hflag = 0;
-        rc = dev_alloc(ctx, flag, 4);
-        if (!rc) rc = fill_u32(ctx, flag.as<uint32_t>(), 0, 1);
-        for (int k = 0; k < 2 && !rc; k++) {
-            check_sorted<<<grid_for(num_edges, 256, 16384), 256, 0, ctx->stream>>>(
-                sides[k].d->src.as<uint32_t>(), sides[k].d->dst.as<uint32_t>(), num_edges, num_vertices,
-                flag.as<uint32_t>());
-        }
-        if (!rc && (hipMemcpyAsync(&hflag, flag.p, 4, hipMemcpyDeviceToHost, ctx->stream) ||
-                    hipStreamSynchronize(ctx->stream)))
-            rc = set_error(ctx, GBX_ERR_HIP, "gbx_graph_from_csr: check failed");
-        if (!rc && (hflag & 2)) rc = set_error(ctx, GBX_ERR_RANGE, "gbx_graph_from_csr: vertex id >= num_vertices");
-        if (!rc && (hflag & 1)) rc = set_error(ctx, GBX_ERR_ARG, "gbx_graph_from_csr: edges not sorted by source");
     }
     if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess)
         rc = set_error(ctx, GBX_ERR_HIP, "gbx_graph_from_csr: sync failed");

@@ -42,12 +42,88 @@ static void die_io(const char *what, const char *fname)
 
 static gbx_ctx *g_ctx = NULL;
 
+/* The pull walks the transpose rows with DIRECTED=1 (pageRank.c:266-268),
+ * the forward rows with DIRECTED=0 (pageRank.c:269-272). */
+#define PULL_DIRECTION (GBX_DIRECTED ? GBX_PULL_IN : GBX_PULL_FWD)
+
+/* What a cached device graph was made from: sizes, the addresses of the
+ * host structs and arrays, and a hash of 64 sampled entries of every array.
+ * A graph freed by other code and a new one built at the same address
+ * differ in at least one of them (unless every sampled entry agrees). */
+enum { PRINT_PTRS = 14, PRINT_SAMPLES = 64 };
+struct graph_print {
+    uint32_t N, E;
+    const void *ptr[PRINT_PTRS];
+    uint64_t hash;
+};
+
 struct graph_entry {
     const struct GraphCSR *host;
+    struct graph_print print;
     gbx_graph *dev;
     struct graph_entry *next;
 };
 static struct graph_entry *g_graphs = NULL;
+
+static uint64_t hash_samples(uint64_t h, const void *arr, uint32_t n)
+{
+    const uint32_t *a = (const uint32_t *)arr;
+    if (!a || n == 0) return h ^ 0x9e3779b97f4a7c15ull;
+    const uint32_t k = n < PRINT_SAMPLES ? n : PRINT_SAMPLES;
+    for (uint32_t i = 0; i < k; i++) {
+        const uint32_t pos = k == 1 ? 0 : (uint32_t)(((uint64_t)i * (n - 1)) / (k - 1));
+        h = (h ^ a[pos]) * 0x100000001b3ull; /* FNV-1a, one word at a time */
+    }
+    return h;
+}
+
+static void fingerprint(const struct GraphCSR *g, struct graph_print *fp)
+{
+    memset(fp, 0, sizeof(*fp));
+    fp->N = g->num_vertices;
+    fp->E = g->num_edges;
+    const struct Vertex *vs[2] = {g->vertices, NULL};
+    const struct EdgeList *es[2] = {g->sorted_edges_array, NULL};
+#if GBX_DIRECTED
+    vs[1] = g->inverse_vertices;
+    es[1] = g->inverse_sorted_edges_array;
+#endif
+    uint64_t h = 0xcbf29ce484222325ull;
+    int k = 0;
+    for (int side = 0; side < 2; side++) {
+        const struct Vertex *v = vs[side];
+        const struct EdgeList *e = es[side];
+        fp->ptr[k++] = v;
+        fp->ptr[k++] = e;
+        if (v) {
+            fp->ptr[k++] = v->out_degree;
+            fp->ptr[k++] = v->edges_idx;
+            h = hash_samples(h, v->out_degree, g->num_vertices);
+            h = hash_samples(h, v->edges_idx, g->num_vertices);
+        } else {
+            k += 2;
+        }
+        if (e) {
+            fp->ptr[k++] = e->edges_array_src;
+            fp->ptr[k++] = e->edges_array_dest;
+            fp->ptr[k++] = GBX_WEIGHTED ? (const void *)e->edges_array_weight : NULL;
+            h = hash_samples(h, e->edges_array_src, g->num_edges);
+            h = hash_samples(h, e->edges_array_dest, g->num_edges);
+            if (GBX_WEIGHTED) h = hash_samples(h, e->edges_array_weight, g->num_edges);
+        } else {
+            k += 3;
+        }
+    }
+    fp->hash = h;
+}
+
+static int same_print(const struct graph_print *a, const struct graph_print *b)
+{
+    if (a->N != b->N || a->E != b->E || a->hash != b->hash) return 0;
+    for (int i = 0; i < PRINT_PTRS; i++)
+        if (a->ptr[i] != b->ptr[i]) return 0;
+    return 1;
+}
 
 static gbx_ctx *backend(void)
 {
@@ -62,22 +138,6 @@ static gbx_ctx *backend(void)
         }
     }
     return g_ctx;
-}
-
-static void register_graph(const struct GraphCSR *host, gbx_graph *dev)
-{
-    struct graph_entry *e = host_malloc(sizeof(*e));
-    e->host = host;
-    e->dev = dev;
-    e->next = g_graphs;
-    g_graphs = e;
-}
-
-static gbx_graph *lookup_graph(const struct GraphCSR *host)
-{
-    for (struct graph_entry *e = g_graphs; e; e = e->next)
-        if (e->host == host) return e->dev;
-    return NULL;
 }
 
 static void unregister_graph(const struct GraphCSR *host)
@@ -95,6 +155,37 @@ static void unregister_graph(const struct GraphCSR *host)
     }
 }
 
+static void register_graph(const struct GraphCSR *host, gbx_graph *dev)
+{
+    unregister_graph(host);
+    struct graph_entry *e = host_malloc(sizeof(*e));
+    e->host = host;
+    fingerprint(host, &e->print);
+    e->dev = dev;
+    e->next = g_graphs;
+    g_graphs = e;
+}
+
+/* The cached device copy of host, or NULL; a copy whose fingerprint no longer
+ * matches the host graph is released. */
+static gbx_graph *lookup_graph(const struct GraphCSR *host)
+{
+    for (struct graph_entry *e = g_graphs; e; e = e->next)
+        if (e->host == host) {
+            struct graph_print now;
+            fingerprint(host, &now);
+            if (same_print(&now, &e->print)) return e->dev;
+            unregister_graph(host);
+            return NULL;
+        }
+    return NULL;
+}
+
+void graphbloxHipForget(const struct GraphCSR *graph)
+{
+    if (graph) unregister_graph(graph);
+}
+
 void graphbloxHipShutdown(void)
 {
     while (g_graphs) {
@@ -108,24 +199,36 @@ void graphbloxHipShutdown(void)
 }
 
 /* Device graph for a host GraphCSR: the one built by graphCSRPreProcessingStep,
- * or an upload of arrays built elsewhere (gbx_graph_from_csr). */
+ * or an upload of arrays built elsewhere (gbx_graph_from_csr; with DIRECTED=0
+ * the forward CSR only, whose transpose the device builds itself). */
 static gbx_graph *device_graph(struct GraphCSR *graph)
 {
     gbx_graph *g = lookup_graph(graph);
     if (g) return g;
     gbx_ctx *ctx = backend();
     if (!ctx) return NULL;
-    struct Vertex *fv = graph->vertices, *iv = graph->inverse_vertices;
-    struct EdgeList *fe = graph->sorted_edges_array, *ie = graph->inverse_sorted_edges_array;
-    if (!fv || !iv || !fe || !ie) {
-        fprintf(stderr, "graphblox_hip: graph has no transpose CSR (DIRECTED=1 layout required)\n");
+    struct Vertex *fv = graph->vertices;
+    struct EdgeList *fe = graph->sorted_edges_array;
+    if (!fv || !fe) {
+        fprintf(stderr, "graphblox_hip: graph has no forward CSR\n");
         return NULL;
     }
     const float *fw = GBX_WEIGHTED ? fe->edges_array_weight : NULL;
+#if GBX_DIRECTED
+    struct Vertex *iv = graph->inverse_vertices;
+    struct EdgeList *ie = graph->inverse_sorted_edges_array;
+    if (!iv || !ie) {
+        fprintf(stderr, "graphblox_hip: graph has no transpose CSR (DIRECTED=1 layout)\n");
+        return NULL;
+    }
     const float *iw = GBX_WEIGHTED ? ie->edges_array_weight : NULL;
     int rc = gbx_graph_from_csr(ctx, graph->num_vertices, graph->num_edges, fv->out_degree, fv->edges_idx,
                                 fe->edges_array_src, fe->edges_array_dest, fw, iv->out_degree, iv->edges_idx,
                                 ie->edges_array_src, ie->edges_array_dest, iw, &g);
+#else
+    int rc = gbx_graph_from_csr(ctx, graph->num_vertices, graph->num_edges, fv->out_degree, fv->edges_idx,
+                                fe->edges_array_src, fe->edges_array_dest, fw, NULL, NULL, NULL, NULL, NULL, &g);
+#endif
     if (rc != GBX_OK) {
         fprintf(stderr, "graphblox_hip: upload failed: %s\n", gbx_last_error(ctx));
         return NULL;
@@ -289,9 +392,12 @@ static uint32_t *read_label_file(const char *fname, uint32_t num_vertices)
     return labels;
 }
 
-/* graphCSRPreProcessingStep, graphCSR.c:243-327 (DIRECTED=1): the reader
- * runs on the host; both count sorts, the -k dedupe, the relabel levels
- * (lmode, lmode_l2, lmode_l3) and the vertex mapping run on the GPU. */
+/* graphCSRPreProcessingStep, graphCSR.c:243-327: the reader runs on the
+ * host; both count sorts, the -k dedupe, the relabel levels (lmode,
+ * lmode_l2, lmode_l3) and the vertex mapping run on the GPU.  With
+ * DIRECTED=0 the GraphCSR holds the forward CSR only, whose in_degree the
+ * reference leaves zero (vertex.c:39-45, written only under DIRECTED at
+ * vertex.c:291-311); the device keeps its transpose as well. */
 struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
 {
     gbx_ctx *ctx = backend();
@@ -339,7 +445,7 @@ struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
 #if GBX_WEIGHTED
     csr->max_weight = in->max_weight;
 #endif
-    for (int inv = 0; inv < 2; inv++) {
+    for (int inv = 0; inv < 1 + GBX_DIRECTED; inv++) {
         struct Vertex *v = new_vertex_array(N);
         struct EdgeList *el = new_edge_list(E);
         el->num_vertices = N;
@@ -357,13 +463,16 @@ struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
             fprintf(stderr, "graphblox_hip: CSR export failed: %s\n", gbx_last_error(ctx));
             exit(EXIT_FAILURE);
         }
+        if (!GBX_DIRECTED) memset(v->in_degree, 0, (size_t)N * 4);
+#if GBX_DIRECTED
         if (inv) {
             csr->inverse_vertices = v;
             csr->inverse_sorted_edges_array = el;
-        } else {
-            csr->vertices = v;
-            csr->sorted_edges_array = el;
+            continue;
         }
+#endif
+        csr->vertices = v;
+        csr->sorted_edges_array = el;
     }
     freeEdgeList(in);
     free(labels);
@@ -382,9 +491,11 @@ void graphCSRFree(struct GraphCSR *graph)
     if (!graph) return;
     unregister_graph(graph);
     free_vertex_array(graph->vertices);
-    free_vertex_array(graph->inverse_vertices);
     freeEdgeList(graph->sorted_edges_array);
+#if GBX_DIRECTED
+    free_vertex_array(graph->inverse_vertices);
     freeEdgeList(graph->inverse_sorted_edges_array);
+#endif
     free(graph);
 }
 
@@ -422,7 +533,11 @@ void writeToBinFileGraphCSR(const char *fname, struct GraphCSR *graph)
     FILE *f = fopen(path, "wb");
     if (!f) die_io("open", path);
     const uint32_t N = graph->num_vertices, E = graph->num_edges;
+#if GBX_DIRECTED
     const int inv = graph->inverse_vertices != NULL;
+#else
+    const int inv = 0;
+#endif
     write_all(f, &graph->num_edges, 1, path);
     write_all(f, &graph->num_vertices, 1, path);
 #if GBX_WEIGHTED
@@ -437,12 +552,14 @@ void writeToBinFileGraphCSR(const char *fname, struct GraphCSR *graph)
             rec[k++] = v->out_degree[i];
             rec[k++] = v->in_degree[i];
             rec[k++] = v->edges_idx[i];
+#if GBX_DIRECTED
             if (inv) {
                 const struct Vertex *w = graph->inverse_vertices;
                 rec[k++] = w->out_degree[i];
                 rec[k++] = w->in_degree[i];
                 rec[k++] = w->edges_idx[i];
             }
+#endif
         }
         write_all(f, rec, k, path);
     }
@@ -451,7 +568,11 @@ void writeToBinFileGraphCSR(const char *fname, struct GraphCSR *graph)
         size_t k = 0;
         for (uint32_t i = b; i < b + n; i++) {
             for (int side = 0; side < 1 + inv; side++) {
+#if GBX_DIRECTED
                 const struct EdgeList *e = side ? graph->inverse_sorted_edges_array : graph->sorted_edges_array;
+#else
+                const struct EdgeList *e = graph->sorted_edges_array;
+#endif
                 rec[k++] = e->edges_array_src[i];
                 rec[k++] = e->edges_array_dest[i];
 #if GBX_WEIGHTED
@@ -467,8 +588,9 @@ void writeToBinFileGraphCSR(const char *fname, struct GraphCSR *graph)
 }
 
 /* readFromBinFileGraphCSR, graphCSR.c:638-744: the file written above (with
- * the inverse records: this layout is DIRECTED=1), as a host GraphCSR; its
- * device copy is made when an algorithm first runs on it. */
+ * the inverse records in the DIRECTED=1 layout, without them in the
+ * DIRECTED=0 layout), as a host GraphCSR; its device copy is made when an
+ * algorithm first runs on it. */
 struct GraphCSR *readFromBinFileGraphCSR(const char *fname)
 {
     FILE *f = fopen(fname, "rb");
@@ -486,40 +608,46 @@ struct GraphCSR *readFromBinFileGraphCSR(const char *fname)
     read_all(f, &max_weight, 1, fname);
     csr->max_weight = max_weight;
 #endif
-    csr->vertices = new_vertex_array(N);
-    csr->inverse_vertices = new_vertex_array(N);
-    for (int side = 0; side < 2; side++) {
+    const int sides = 1 + GBX_DIRECTED;
+    struct Vertex *vs[2] = {NULL, NULL};
+    struct EdgeList *es[2] = {NULL, NULL};
+    for (int side = 0; side < sides; side++) {
+        vs[side] = new_vertex_array(N);
         struct EdgeList *el = new_edge_list(E);
         el->num_vertices = N;
         el->avg_degree = csr->avg_degree;
         el->max_weight = max_weight;
         label_arrays(el, N);
-        if (side) csr->inverse_sorted_edges_array = el;
-        else csr->sorted_edges_array = el;
+        es[side] = el;
     }
-    const size_t per_edge = GBX_WEIGHTED ? 6 : 4;
+    csr->vertices = vs[0];
+    csr->sorted_edges_array = es[0];
+#if GBX_DIRECTED
+    csr->inverse_vertices = vs[1];
+    csr->inverse_sorted_edges_array = es[1];
+#endif
+    const size_t per_vertex = 3 * (size_t)sides, per_side = GBX_WEIGHTED ? 3 : 2;
+    const size_t per_edge = per_side * (size_t)sides;
     uint32_t *rec = host_malloc((size_t)CSR_IO_CHUNK * 6 * 4);
     for (uint32_t b = 0; b < N; b += CSR_IO_CHUNK) {
         const uint32_t n = N - b < CSR_IO_CHUNK ? N - b : CSR_IO_CHUNK;
-        read_all(f, rec, (size_t)n * 6, fname);
-        for (uint32_t i = 0; i < n; i++) {
-            const uint32_t *r = rec + (size_t)i * 6;
-            csr->vertices->out_degree[b + i] = r[0];
-            csr->vertices->in_degree[b + i] = r[1];
-            csr->vertices->edges_idx[b + i] = r[2];
-            csr->inverse_vertices->out_degree[b + i] = r[3];
-            csr->inverse_vertices->in_degree[b + i] = r[4];
-            csr->inverse_vertices->edges_idx[b + i] = r[5];
-        }
+        read_all(f, rec, (size_t)n * per_vertex, fname);
+        for (uint32_t i = 0; i < n; i++)
+            for (int side = 0; side < sides; side++) {
+                const uint32_t *r = rec + (size_t)i * per_vertex + 3 * side;
+                vs[side]->out_degree[b + i] = r[0];
+                vs[side]->in_degree[b + i] = r[1];
+                vs[side]->edges_idx[b + i] = r[2];
+            }
     }
     for (uint32_t b = 0; b < E; b += CSR_IO_CHUNK) {
         const uint32_t n = E - b < CSR_IO_CHUNK ? E - b : CSR_IO_CHUNK;
         read_all(f, rec, (size_t)n * per_edge, fname);
         for (uint32_t i = 0; i < n; i++) {
             const uint32_t *r = rec + (size_t)i * per_edge;
-            for (int side = 0; side < 2; side++) {
-                struct EdgeList *e = side ? csr->inverse_sorted_edges_array : csr->sorted_edges_array;
-                const uint32_t *q = r + side * (per_edge / 2);
+            for (int side = 0; side < sides; side++) {
+                struct EdgeList *e = es[side];
+                const uint32_t *q = r + side * per_side;
                 e->edges_array_src[b + i] = q[0];
                 e->edges_array_dest[b + i] = q[1];
 #if GBX_WEIGHTED
@@ -625,7 +753,7 @@ static struct PageRankStats *run_pagerank(struct Arguments *arguments, struct Gr
     p.mode = mode;
     p.stop = GBX_STOP_ACTIVE;
     p.precision = GBX_F32;
-    p.direction = GBX_PULL_IN;
+    p.direction = PULL_DIRECTION;
     gbx_pr_report rep;
     int rc = gbx_pr_pull(g_ctx, g, &p, NULL, stats->pageRanks, &rep);
     if (rc != GBX_OK) {
@@ -648,7 +776,8 @@ static struct PageRankStats *run_pagerank(struct Arguments *arguments, struct Gr
     return stats;
 }
 
-/* pageRankPullGraphCSR, pageRank.c:245-368 (DIRECTED=1 rows, FP32) */
+/* pageRankPullGraphCSR, pageRank.c:245-368 (FP32; transpose rows with
+ * DIRECTED=1, forward rows with DIRECTED=0) */
 struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
     return run_pagerank(arguments, graph, GBX_PR_REF, "Page Rank Pull (HIP)");
@@ -720,23 +849,39 @@ struct PageRankStats *pageRankPushQuantGraphCSR(struct Arguments *arguments, str
     return run_pagerank(arguments, graph, GBX_PR_QUANT32, "Page Rank Push Quant_32 (HIP, pull sums)");
 }
 
+/* The data-driven variants take their worklist from the out-neighbours of
+ * the transpose rows; with DIRECTED=0 the reference runs them over the
+ * forward arrays for both (pageRank.c:1742-1748), which this backend does
+ * not provide: NULL, the reference's "not available" (graphRun.c:948-957). */
+static struct PageRankStats *run_data_driven(struct Arguments *arguments, struct GraphCSR *graph, int mode,
+                                             const char *name)
+{
+    if (!GBX_DIRECTED) {
+        printf(" -----------------------------------------------------\n");
+        printf("| %-51s | \n", "data-driven PageRank: DIRECTED=1 builds only");
+        printf(" -----------------------------------------------------\n");
+        return NULL;
+    }
+    return run_pagerank(arguments, graph, mode, name);
+}
+
 /* pageRankDataDrivenPullGraphCSR, pageRank.c:1711-1870 */
 struct PageRankStats *pageRankDataDrivenPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
-    return run_pagerank(arguments, graph, GBX_PR_DATA_DRIVEN, "Page Rank Pull DD (HIP)");
+    return run_data_driven(arguments, graph, GBX_PR_DATA_DRIVEN, "Page Rank Pull DD (HIP)");
 }
 
 /* pageRankDataDrivenPushGraphCSR, pageRank.c:1872-2027 (frontier iteration:
  * the same fixed point, not the same order of in-place updates) */
 struct PageRankStats *pageRankDataDrivenPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
-    return run_pagerank(arguments, graph, GBX_PR_DD_PUSH, "Page Rank Push DD (HIP)");
+    return run_data_driven(arguments, graph, GBX_PR_DD_PUSH, "Page Rank Push DD (HIP)");
 }
 
 /* pageRankDataDrivenPullPushGraphCSR, pageRank.c:2029-2202 (frontier iteration) */
 struct PageRankStats *pageRankDataDrivenPullPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
-    return run_pagerank(arguments, graph, GBX_PR_DD_PULL_PUSH, "Page Rank Pull-Push DD (HIP)");
+    return run_data_driven(arguments, graph, GBX_PR_DD_PULL_PUSH, "Page Rank Pull-Push DD (HIP)");
 }
 
 /* pageRankGraphCSR, pageRank.c:177-242: the variant selected by
@@ -819,7 +964,6 @@ void freeSPMVStats(struct SPMVStats *stats)
     free(stats);
 }
 
-/* SPMVPullGraphCSR, SPMV.c:117-215 (DIRECTED=1 rows; weighted when GBX_WEIGHTED) */
 /* One SpMV variant on the device copy of graph: bits 0 = the float pull,
  * 32 / 64 = the fixed-point forms. */
 static struct SPMVStats *run_spmv(struct Arguments *arguments, struct GraphCSR *graph, int bits, const char *name)
@@ -829,9 +973,9 @@ static struct SPMVStats *run_spmv(struct Arguments *arguments, struct GraphCSR *
     struct SPMVStats *stats = newSPMVStatsGraphCSR(graph);
     stats->arguments = arguments;
     double secs = 0.0;
-    int rc = bits ? gbx_spmv_fixed(g_ctx, g, arguments->iterations, GBX_WEIGHTED, GBX_PULL_IN, bits,
+    int rc = bits ? gbx_spmv_fixed(g_ctx, g, arguments->iterations, GBX_WEIGHTED, PULL_DIRECTION, bits,
                                    stats->vector_input, stats->vector_output, &secs)
-                  : gbx_spmv_pull(g_ctx, g, arguments->iterations, GBX_WEIGHTED, GBX_PULL_IN, NULL,
+                  : gbx_spmv_pull(g_ctx, g, arguments->iterations, GBX_WEIGHTED, PULL_DIRECTION, NULL,
                                   stats->vector_input, stats->vector_output, &secs);
     if (rc != GBX_OK) {
         fprintf(stderr, "graphblox_hip: %s failed: %s\n", name, gbx_last_error(g_ctx));
@@ -851,7 +995,8 @@ static struct SPMVStats *run_spmv(struct Arguments *arguments, struct GraphCSR *
     return stats;
 }
 
-/* SPMVPullGraphCSR, SPMV.c:117-215 */
+/* SPMVPullGraphCSR, SPMV.c:117-215 (transpose rows and weights with
+ * DIRECTED=1, forward ones with DIRECTED=0; weighted when GBX_WEIGHTED) */
 struct SPMVStats *SPMVPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
     return run_spmv(arguments, graph, 0, "SPMVPullGraphCSR");
@@ -883,5 +1028,20 @@ struct SPMVStats *SPMVGraphCSR(struct Arguments *arguments, struct GraphCSR *gra
     case 2: return SPMVPullFixedPointGraphCSR(arguments, graph);
     case 3: return SPMVPushFixedPointGraphCSR(arguments, graph);
     default: return SPMVPullGraphCSR(arguments, graph);
+    }
+}
+
+/* runSPMVAlgorithm, graphRun.c:859-893: the CSR data structures (0, 4 and
+ * unknown codes) run SPMVGraphCSR; segments (1, 5) and 6 are not
+ * implemented, as in the reference. */
+struct SPMVStats *runSPMVAlgorithm(struct Arguments *arguments, void *graph)
+{
+    switch (arguments->datastructure) {
+    case 1: case 5: case 6:
+        printf(" -----------------------------------------------------\n");
+        printf("| %-51s | \n", "SPMV NOT YET IMPLEMENTED");
+        printf(" -----------------------------------------------------\n");
+        return NULL;
+    default: return SPMVGraphCSR(arguments, (struct GraphCSR *)graph);
     }
 }

@@ -20,6 +20,9 @@
  *                                     and its stats (pageRank.h:19-30)
  *   gbx_spmv_pull, gbx_spmv_fixed     SPMVPullGraphCSR (01_host/src/algorithms/openmp/SPMV.c:117-215)
  *                                     and its variants (SPMV.c:216-590)
+ *   gbx_gather_accumulate_u32         the FPGA overlay's Single.PR integer pull
+ *                                     (02_device/ol/GraphBlox/Single/topology.json:162-201,
+ *                                     01_host/src/utils_fpga_cpp/buffer_mapping.PR.cpp:2-68)
  *   gbx_ppr_batched, gbx_graph_batch_*, gbx_pr_adjoint, gbx_rmat_*
  *                                     no reference counterpart (batched, differentiable
  *                                     and generator forms of the same path)
@@ -145,8 +148,14 @@ int gbx_rmat_edges(gbx_ctx *ctx, uint32_t scale, uint32_t edge_factor, double a,
 /* Device-resident graph from CSR arrays already built on the host in the
  * reference's layout (e.g. by graphCSRPreProcessingStep): forward and
  * transpose out_degree/edges_idx [num_vertices], src/dst [num_edges],
- * w [num_edges] (nullable).  Arrays are copied; nothing is re-sorted.
- * Sorted order is checked (src non-decreasing on both sides). */
+ * w [num_edges] (nullable).  Arrays are copied; nothing is re-sorted.  All
+ * five transpose arguments NULL (the DIRECTED=0 GraphCSR, graphCSR.h:26-30):
+ * the device builds the transpose from the forward arrays (swapped edges,
+ * stable sort by the new source).  Checked before anything else uses them,
+ * on every side given (GBX_ERR_ARG / GBX_ERR_RANGE otherwise): src
+ * non-decreasing, every id below num_vertices, every edge e of source v
+ * inside [edges_idx[v], edges_idx[v] + out_degree[v]) and the out-degrees
+ * summing to num_edges. */
 int gbx_graph_from_csr(gbx_ctx *ctx, uint32_t num_vertices, uint32_t num_edges,
                        const uint32_t *fwd_out_degree, const uint32_t *fwd_edges_idx,
                        const uint32_t *fwd_src, const uint32_t *fwd_dst, const float *fwd_w,
@@ -283,8 +292,9 @@ int gbx_ppr_info(const gbx_pr *pr, uint32_t *num_vectors, uint32_t *vector_width
                  uint32_t *hub_rows, uint32_t *hub_chunks);
 
 /* Replay n_iters iterations as one HIP graph (captured on first use for this
- * (n_iters, iteration parity) and cached in the object): same results as
- * gbx_pr_launch, one launch instead of one or two per iteration. */
+ * (n_iters, iteration parity, teleport given to the last gbx_pr_reset or
+ * not) and cached in the object): same results as gbx_pr_launch, one launch
+ * instead of one or two per iteration. */
 int gbx_pr_launch_graph(gbx_pr *pr, uint32_t n_iters);
 
 /* ---------------------------------------------------------------- adjoint */
@@ -389,6 +399,20 @@ int gbx_spmv_pull(gbx_ctx *ctx, gbx_graph *g, uint32_t iterations, int weighted,
  * the reference's float conversion of the fixed-point sums. */
 int gbx_spmv_fixed(gbx_ctx *ctx, gbx_graph *g, uint32_t iterations, int weighted,
                    int direction, int bits, float *x_out, float *y_out, double *seconds);
+
+/* ---------------------------------------------------------------- integer gather */
+/* The integer pull of the FPGA overlay's Single.PR program (topology.json:
+ * 162-201, PR.ol; host mapping buffer_mapping.PR.cpp:2-68): for every row v
+ * (the transpose CSR for GBX_PULL_IN, as the DIRECTED host maps it,
+ * graphbloxenv.cpp:346-352; the forward CSR for GBX_PULL_FWD),
+ *   aux2[N + v] = sum over v's edges (v, u) of aux1[N + u]   (mod 2^32),
+ * ALU_ACC's unsigned 32-bit add (engine_alu_ops_kernel.sv:104-112).  aux1,
+ * aux2: host u32[2N], the overlay's buffers 7 and 8
+ * (test_GraphBlox_USER_MANAGED.cpp:775-804); aux2[0, N) is left unchanged.
+ * With aux1[N + u] = 1 the result is every row's degree, the check the
+ * reference's host runs (test_GraphBlox_USER_MANAGED.cpp:518-528). */
+int gbx_gather_accumulate_u32(gbx_ctx *ctx, gbx_graph *g, int direction, const uint32_t *aux1,
+                              uint32_t *aux2);
 
 #ifdef __cplusplus
 }

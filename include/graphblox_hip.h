@@ -8,11 +8,16 @@
  *   struct Arguments      01_host/include/config/graphConfig.h:18-63
  *   struct EdgeList       01_host/include/structures/edgeList.h:12-25
  *   struct Vertex         01_host/include/structures/vertex.h:17-23
- *   struct GraphCSR       01_host/include/structures/graphCSR.h:14-31 (DIRECTED=1)
+ *   struct GraphCSR       01_host/include/structures/graphCSR.h:14-31
  *   struct PageRankStats  01_host/include/algorithms/openmp/pageRank.h:19-30
  *   struct SPMVStats      01_host/include/algorithms/openmp/SPMV.h:17-26
  * GBX_WEIGHTED selects the WEIGHTED=1 layout (GraphCSR.max_weight) and the
- * weighted SpMV, as the reference's compile-time WEIGHTED switch does.
+ * weighted SpMV, as the reference's compile-time WEIGHTED switch does
+ * (graphConfig.h:13).  GBX_DIRECTED (default 1) selects the layout and the
+ * pull direction of the reference's DIRECTED switch (graphConfig.h:14):
+ * with 1 the GraphCSR carries the transpose (inverse_*) and the pull walks
+ * it (pageRank.c:266-268); with 0 the GraphCSR has the forward CSR only and
+ * the pull walks the forward rows (pageRank.c:269-272).
  *
  * Ownership follows the reference: the callee allocates stats and arrays
  * (4096-byte aligned, myMalloc.c:24-39); the caller frees them with
@@ -33,6 +38,9 @@ extern "C" {
 
 #ifndef GBX_WEIGHTED
 #define GBX_WEIGHTED 0
+#endif
+#ifndef GBX_DIRECTED
+#define GBX_DIRECTED 1
 #endif
 
 /* layout of the reference's mt19937 state (mt19937.h): 624 words + index */
@@ -114,8 +122,10 @@ struct GraphCSR {
 #endif
     struct Vertex *vertices;
     struct EdgeList *sorted_edges_array;
+#if GBX_DIRECTED
     struct Vertex *inverse_vertices;
     struct EdgeList *inverse_sorted_edges_array;
+#endif
 };
 
 struct PageRankStats {
@@ -172,13 +182,14 @@ struct PageRankStats *pageRankPullQuant16BitGraphCSR(struct Arguments *arguments
 struct PageRankStats *pageRankPullQuant8BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 /* pageRank.c:1579-1709: push quantised (the 32-bit quantized terms, exact sums) */
 struct PageRankStats *pageRankPushQuantGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
-/* pageRank.c:1711-2202: the data-driven pull, push and pull-push variants */
+/* pageRank.c:1711-2202: the data-driven pull, push and pull-push variants
+ * (GBX_DIRECTED=1 builds; the GBX_DIRECTED=0 build returns NULL for them) */
 struct PageRankStats *pageRankDataDrivenPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct PageRankStats *pageRankDataDrivenPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct PageRankStats *pageRankDataDrivenPullPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 
-/* graphCSR.c:457-744: the .csr binary file (writer and reader, this
- * library's DIRECTED=1 layout) and the text-fixture writer */
+/* graphCSR.c:457-744: the .csr binary file (writer and reader, in this
+ * build's DIRECTED layout) and the text-fixture writer */
 void writeToBinFileGraphCSR(const char *fname, struct GraphCSR *graph);
 struct GraphCSR *readFromBinFileGraphCSR(const char *fname);
 void writetoTextFilesGraphCSR(const char *fname, struct GraphCSR *graph);
@@ -215,9 +226,22 @@ struct SPMVStats *SPMVPullGraphCSR(struct Arguments *arguments, struct GraphCSR 
 struct SPMVStats *SPMVPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct SPMVStats *SPMVPullFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct SPMVStats *SPMVPushFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* graphRun.c:859-893: SPMVGraphCSR for the CSR data structures (0, 4 and
+ * unknown codes); segments (1, 5) and 6 are not implemented (NULL) */
+struct SPMVStats *runSPMVAlgorithm(struct Arguments *arguments, void *graph);
 
-/* Backend housekeeping (no reference counterpart): release the process-wide
- * HIP context and every cached device graph. */
+/* Backend housekeeping (no reference counterpart).
+ * The device copy of a GraphCSR that this library did not build (one made
+ * by the reference's own graphCSRPreProcessingStep, or read from a file) is
+ * made on first use and cached; every call checks the cached copy against a
+ * fingerprint of the host graph (sizes, the struct and array addresses and
+ * 64 sampled entries of every array) and uploads it again on a mismatch, so
+ * a graph freed by other code and a new one at the same address are told
+ * apart.  After editing a graph's arrays in place, call
+ * graphbloxHipForget(graph): an edit that misses every sampled entry keeps
+ * the fingerprint.  graphbloxHipShutdown releases the process-wide HIP
+ * context and every cached device graph. */
+void graphbloxHipForget(const struct GraphCSR *graph);
 void graphbloxHipShutdown(void);
 
 #ifdef __cplusplus

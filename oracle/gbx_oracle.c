@@ -207,6 +207,9 @@ int orc_pr_f64(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
                double damp, int mode, uint32_t max_iters, double epsilon,
                double *ranks_out, uint32_t *iters_out, double *err_out)
 {
+    /* OpenMP over vertices; every row sum is sequential in edge order, so a
+     * vertex's value does not depend on the thread count (only the FP64
+     * dangling-mass and error reductions are reassociated) */
     double *r = ranks_out;
     double *c = malloc((size_t)N * sizeof(double) + 8);
     double *nx = malloc((size_t)N * sizeof(double) + 8);
@@ -214,10 +217,12 @@ int orc_pr_f64(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
     const double base = 1.0 - damp;
     uint32_t it = 0;
     double err = 0.0;
+#pragma omp parallel for schedule(static)
     for (uint32_t v = 0; v < N; v++)
         r[v] = mode == 0 ? base : (teleport ? teleport[v] : 1.0 / N);
     for (it = 0; it < max_iters; it++) {
         double dang = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : dang)
         for (uint32_t v = 0; v < N; v++) {
             if (outdeg[v]) {
                 c[v] = w ? r[v] : r[v] / outdeg[v];
@@ -228,6 +233,7 @@ int orc_pr_f64(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
         }
         uint32_t active = 0;
         err = 0.0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : err, active)
         for (uint32_t v = 0; v < N; v++) {
             double acc = 0.0;
             for (uint64_t j = row_ptr[v]; j < row_ptr[v + 1]; j++)
@@ -253,6 +259,7 @@ int orc_pr_f64(uint32_t N, const uint32_t *outdeg, const uint64_t *row_ptr,
         }
     }
     if (mode == 0)
+#pragma omp parallel for schedule(static)
         for (uint32_t v = 0; v < N; v++) r[v] /= N;
     if (iters_out) *iters_out = it;
     if (err_out) *err_out = err;

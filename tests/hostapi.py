@@ -2,6 +2,8 @@
 import ctypes as C
 import os
 
+import numpy as np
+
 from conftest import ROOT
 
 u32 = C.c_uint32
@@ -38,13 +40,16 @@ class Vertex(C.Structure):
                 ("edges_idx", C.POINTER(u32))]
 
 
-def graph_csr_type(weighted: bool):
+def graph_csr_type(weighted: bool, directed: bool = True):
+    """struct GraphCSR in the (DIRECTED, WEIGHTED) layout (graphCSR.h:14-31)."""
     fields = [("num_edges", u32), ("num_vertices", u32), ("avg_degree", u32)]
     if weighted:
         fields.append(("max_weight", C.c_float))
-    fields += [("vertices", C.POINTER(Vertex)), ("sorted_edges_array", C.POINTER(EdgeList)),
-               ("inverse_vertices", C.POINTER(Vertex)), ("inverse_sorted_edges_array", C.POINTER(EdgeList))]
-    return type("GraphCSRW" if weighted else "GraphCSR", (C.Structure,), {"_fields_": fields})
+    fields += [("vertices", C.POINTER(Vertex)), ("sorted_edges_array", C.POINTER(EdgeList))]
+    if directed:
+        fields += [("inverse_vertices", C.POINTER(Vertex)), ("inverse_sorted_edges_array", C.POINTER(EdgeList))]
+    name = "GraphCSR" + ("W" if weighted else "") + ("" if directed else "D0")
+    return type(name, (C.Structure,), {"_fields_": fields})
 
 
 class PageRankStats(C.Structure):
@@ -59,10 +64,13 @@ class SPMVStats(C.Structure):
                 ("time_total", C.c_double)]
 
 
-def load(weighted: bool = False):
-    name = "libgraphblox_hip_w.so" if weighted else "libgraphblox_hip.so"
-    L = C.CDLL(os.path.join(ROOT, "graphblox_amd", name))
-    G = graph_csr_type(weighted)
+def library_name(weighted: bool = False, directed: bool = True) -> str:
+    return "libgraphblox_hip" + ("" if directed else "_d0") + ("_w" if weighted else "") + ".so"
+
+
+def load(weighted: bool = False, directed: bool = True):
+    L = C.CDLL(os.path.join(ROOT, "graphblox_amd", library_name(weighted, directed)))
+    G = graph_csr_type(weighted, directed)
     pA, pG = C.POINTER(Arguments), C.POINTER(G)
     L.argumentsNew.restype = pA
     L.argumentsFree.argtypes = [pA]
@@ -83,5 +91,58 @@ def load(weighted: bool = False):
     L.SPMVPullGraphCSR.argtypes = [pA, C.c_void_p]
     L.freeSPMVStats.argtypes = [C.POINTER(SPMVStats)]
     L.graphbloxHipShutdown.argtypes = []
+    L.graphbloxHipForget.argtypes = [C.c_void_p]
+    L.runSPMVAlgorithm.restype = C.POINTER(SPMVStats)
+    L.runSPMVAlgorithm.argtypes = [pA, C.c_void_p]
     L.GraphCSR = G
     return L
+
+
+class HostGraph:
+    """A GraphCSR assembled in Python (numpy arrays, the reference's struct
+    layout), as code other than this library's graphCSRPreProcessingStep
+    would hand one over.  `keep` holds every array the struct points to."""
+
+    def __init__(self, L, N, E, fwd, inv=None, weighted=False):
+        self.L = L
+        self.keep = []
+        self.g = L.GraphCSR()
+        self.set(N, E, fwd, inv, weighted)
+
+    def _vertex(self, N, a):
+        v = Vertex()
+        v.num_vertices = N
+        for nm in ("out_degree", "in_degree", "edges_idx"):
+            arr = np.ascontiguousarray(a.get(nm, np.zeros(N, np.uint32)), np.uint32)
+            self.keep.append(arr)
+            setattr(v, nm, arr.ctypes.data_as(C.POINTER(u32)))
+        self.keep.append(v)
+        return C.pointer(v)
+
+    def _edges(self, N, E, a):
+        e = EdgeList()
+        e.num_edges, e.num_vertices, e.avg_degree, e.max_weight = E, N, E // max(N, 1), 1.0
+        for nm, key, dt in (("edges_array_src", "src", np.uint32), ("edges_array_dest", "dst", np.uint32),
+                            ("edges_array_weight", "w", np.float32)):
+            arr = np.ascontiguousarray(a.get(key, np.ones(E, dt)), dt)
+            self.keep.append(arr)
+            ct = C.c_float if dt == np.float32 else u32
+            setattr(e, nm, arr.ctypes.data_as(C.POINTER(ct)))
+        self.keep.append(e)
+        return C.pointer(e)
+
+    def set(self, N, E, fwd, inv=None, weighted=False):
+        """Point the same struct (same address) at new arrays."""
+        g = self.g
+        g.num_edges, g.num_vertices, g.avg_degree = E, N, E // max(N, 1)
+        if weighted:
+            g.max_weight = 1.0
+        g.vertices = self._vertex(N, fwd)
+        g.sorted_edges_array = self._edges(N, E, fwd)
+        if inv is not None:
+            g.inverse_vertices = self._vertex(N, inv)
+            g.inverse_sorted_edges_array = self._edges(N, E, inv)
+
+    @property
+    def ptr(self):
+        return C.addressof(self.g)

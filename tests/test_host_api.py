@@ -180,28 +180,3 @@ def test_weighted_build_through_reference_api(golden, tmp_path):
     assert l1(y, z["wspmv_y1"]) <= 1e-6 * float(np.abs(z["wspmv_y1"]).sum())
     L.freeSPMVStats(sp)
     L.graphCSRFree(g)
-
-
-@pytest.mark.gpu
-@pytest.mark.skipif(not O.ref_available(1, 0), reason="reference checker build (oracle/_ref) not present")
-def test_adapter_runs_on_a_graph_built_by_the_reference(golden, tmp_path):
-    """A GraphCSR produced by the reference's own graphCSRPreProcessingStep is
-    handed to the adapter's pageRankPullGraphCSR unchanged: the struct layouts
-    must agree for this to give the reference's ranks."""
-    import hostapi
-
-    L = hostapi.load(False)
-    R = O.ref(1, 0)
-    z = golden("rmat_s12")
-    path = _write(tmp_path, z)
-    gref = R.refd_build(path.encode(), 1, 0, 0, 0, 2)
-    a = L.argumentsNew()
-    a.contents.iterations = 20
-    a.contents.epsilon = 0.0
-    st = L.pageRankPullGraphCSR(a, gref)
-    assert st
-    N = int(z["m1_N"])
-    assert l1(_arr(st.contents.pageRanks, N, np.float32), z["pr_r20"]) <= 1e-5
-    L.freePageRankStats(st)
-    L.graphbloxHipShutdown()
-    R.refd_free(gref)

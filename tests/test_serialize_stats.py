@@ -47,9 +47,9 @@ def _edges_file(tmp_path, z, weighted):
     return p
 
 
-def _export(R, g, N, E):
+def _export(R, g, N, E, sides=(0, 1)):
     out = {}
-    for side in (0, 1):
+    for side in sides:
         a = [np.zeros(N, np.uint32) for _ in range(3)] + [np.zeros(E, np.uint32) for _ in range(2)] + \
             [np.zeros(E, np.float32)]
         R.refd_export(g, side, *[x.ctypes.data for x in a])
@@ -61,13 +61,18 @@ def _arr(ptr, n, dt):
     return np.ctypeslib.as_array(ptr, (n,)).astype(dt, copy=True)
 
 
+@pytest.mark.parametrize("directed", [True, False])
 @pytest.mark.parametrize("weighted", [False, True])
 @pytest.mark.parametrize("name", ["rmat_s10", "rmat_s12"])
-def test_csr_files_match_the_reference(golden, tmp_path, name, weighted):
+def test_csr_files_match_the_reference(golden, tmp_path, name, weighted, directed):
+    """In each (DIRECTED, WEIGHTED) layout: the reference's own build of that
+    layout writes the GraphCSR, the adapter of the same layout writes it too,
+    and each reads the other's file."""
     z = golden(name)
-    R = _bind_io(O.ref(1, int(weighted)), C.c_void_p)
-    L = hostapi.load(weighted)
+    R = _bind_io(O.ref(int(directed), int(weighted)), C.c_void_p)
+    L = hostapi.load(weighted, directed)
     _bind_io(L, C.POINTER(L.GraphCSR))
+    sides = (0, 1) if directed else (0,)
     g = R.refd_build(_edges_file(tmp_path, z, weighted).encode(), 1, 0, 0, int(weighted), 1)
     N, E = R.refd_num_vertices(g), R.refd_num_edges(g)
     ref_base, mine_base = str(tmp_path / "ref"), str(tmp_path / "mine")
@@ -75,19 +80,21 @@ def test_csr_files_match_the_reference(golden, tmp_path, name, weighted):
     L.writeToBinFileGraphCSR(mine_base.encode(), g)
     ref_bytes = open(ref_base + ".csr", "rb").read()
     assert open(mine_base + ".csr", "rb").read() == ref_bytes
-    assert len(ref_bytes) == 4 * (2 + int(weighted) + 6 * N + (6 if weighted else 4) * E)
+    assert len(ref_bytes) == 4 * (2 + int(weighted) + len(sides) * (3 * N + (3 if weighted else 2) * E))
     R.writetoTextFilesGraphCSR(ref_base.encode(), g)
     L.writetoTextFilesGraphCSR(mine_base.encode(), g)
     exts = [".out_degree", ".in_degree", ".edges_idx", ".edges_array_src", ".edges_array_dest"]
     for ext in exts + ([".edges_array_weight"] if weighted else []):
         assert open(mine_base + ext).read() == open(ref_base + ext).read(), ext
     # read back by the adapter: the same arrays, and the same file when written again
-    ref_arrays = _export(R, g, N, E)
+    ref_arrays = _export(R, g, N, E, sides)
     gm = L.readFromBinFileGraphCSR((mine_base + ".csr").encode())
     c = gm.contents
     assert (c.num_vertices, c.num_edges) == (N, E)
-    for side, (v, e) in enumerate(((c.vertices, c.sorted_edges_array),
-                                   (c.inverse_vertices, c.inverse_sorted_edges_array))):
+    pairs = [(c.vertices, c.sorted_edges_array)]
+    if directed:
+        pairs.append((c.inverse_vertices, c.inverse_sorted_edges_array))
+    for side, (v, e) in enumerate(pairs):
         od, ind, idx, src, dst, w = ref_arrays[side]
         np.testing.assert_array_equal(_arr(v.contents.out_degree, N, np.uint32), od)
         np.testing.assert_array_equal(_arr(v.contents.in_degree, N, np.uint32), ind)
@@ -102,8 +109,8 @@ def test_csr_files_match_the_reference(golden, tmp_path, name, weighted):
     L.graphCSRFree(gm)
     # and the reference's reader takes the adapter's file
     gr = R.readFromBinFileGraphCSR((mine_base + ".csr").encode())
-    back = _export(R, gr, N, E)
-    for side in (0, 1):
+    back = _export(R, gr, N, E, sides)
+    for side in sides:
         for a, b in zip(back[side], ref_arrays[side]):
             np.testing.assert_array_equal(a, b)
     R.refd_free(gr)
