@@ -8,10 +8,17 @@ output depends on the order (with several threads it changes from run to run:
 method determines is its limit as epsilon shrinks, which the reference's own
 algorithm (the oracle, bit-exact with its one-thread outputs) reaches: the
 push form's self-loop-free limit (oracle.dd_push_limit) and the pull-push
-form's ordinary REF fixed point.  Bars: within epsilon / (1 - d) + 1e-6 L1 of
-the limit (at the stop every residual is below epsilon, and (I - d P^T)^-1
+form's ordinary REF fixed point.  Bars, push form: within epsilon / (1 - d)
++ 1e-6 L1 of the limit (its residual is exactly the change of the rank still
+to come: at the stop every residual is below epsilon, and (I - d P^T)^-1
 grows an L1 norm by at most 1 / (1 - d)); and within twice that of the
-reference's own output at epsilon 1e-4."""
+reference's own output at epsilon 1e-4.  The pull-push form recomputes a rank
+by a pull but pushes the residual, not the pull's change (pageRank.c:
+2114-2143), so its stop rule bounds the pushed residuals only: a vertex
+whose in-neighbours changed by more than their residuals can stay stale.
+Which vertices that hits depends on the processing order (25 runs per graph
+and epsilon on MI355X: 3e-8 to 6e-8 L1 from the fixed point in 49 of 50 runs
+on fixture_v8192_e32768, 1.2e-5 in one); its bar is PULL_PUSH_BAR."""
 import numpy as np
 import pytest
 
@@ -24,6 +31,11 @@ pytestmark = pytest.mark.gpu
 GRAPHS = [f"fixture_{f}" for f in FIXTURES] + [f"rmat_s{s}" for s in RMAT_SMALL] + ["ring_rmat_s10"]
 MODES = ((gbx.PR_DD_PUSH, 7), (gbx.PR_DD_PULL_PUSH, 8))
 D = 0.85
+PULL_PUSH_BAR = 1e-4
+
+
+def _bar(mode, eps):
+    return eps / (1 - D) + 1e-6 if mode == gbx.PR_DD_PUSH else max(PULL_PUSH_BAR, eps / (1 - D) + 1e-6)
 
 
 def _edges(golden, name):
@@ -55,11 +67,11 @@ def test_dd_push_reaches_the_methods_limits(gpu_ctx, golden, name):
         ranks, rep = g.pagerank(iterations=3000, epsilon=eps, mode=mode)
         e = l1(ranks, lim[mode])
         print(f"[gpu] {name} code {code}: {rep['iterations']} iterations, L1 vs limit {e:.2e}", flush=True)
-        assert e <= eps / (1 - D) + 1e-6
+        assert e <= _bar(mode, eps)
         eps = 1e-4
         ranks, rep = g.pagerank(iterations=3000, epsilon=eps, mode=mode)
-        assert l1(ranks, lim[mode]) <= eps / (1 - D) + 1e-6
-        assert l1(ranks, v[f"pp{code}_eps_ranks"]) <= 2 * (eps / (1 - D) + 1e-6)
+        assert l1(ranks, lim[mode]) <= _bar(mode, eps)
+        assert l1(ranks, v[f"pp{code}_eps_ranks"]) <= 2 * _bar(mode, eps)
         if code == 7:
             assert rep["converged"] == 1
 
@@ -72,7 +84,7 @@ def test_dd_push_rmat_s18(gpu_ctx):
         ranks, rep = g.pagerank(iterations=3000, epsilon=1e-7, mode=mode)
         e = l1(ranks, lim[mode])
         print(f"[gpu] s18 code {code}: {rep['iterations']} iterations, L1 vs limit {e:.2e}", flush=True)
-        assert e <= 1e-7 / (1 - D) + 1e-6
+        assert e <= _bar(mode, 1e-7)
         assert rep["rank_sum"] == pytest.approx(float(lim[mode].sum()), rel=1e-5)
 
 
@@ -89,7 +101,7 @@ def test_dd_push_self_loops_and_ring(gpu_ctx):
     assert l1(lim[gbx.PR_DD_PUSH], lim[gbx.PR_DD_PULL_PUSH]) > 1e-3
     for mode, _ in MODES:
         ranks, _ = g.pagerank(iterations=5000, epsilon=1e-8, mode=mode)
-        assert l1(ranks, lim[mode]) <= 1e-8 / (1 - D) + 1e-6
+        assert l1(ranks, lim[mode]) <= _bar(mode, 1e-8)
 
 
 def test_dd_push_graph_replay_and_errors(gpu_ctx, golden):

@@ -150,7 +150,9 @@ def test_preprocessing_and_pagerank_through_reference_api(golden, tmp_path, name
             a.contents.pushpull = code
             st = L.pageRankGraphCSR(a, g)
             assert st, code
-            assert l1(_arr(st.contents.pageRanks, N, np.float32), lim[code]) <= 1e-7 / 0.15 + 1e-6
+            # the pull-push form's stop rule bounds its pushed residuals only (tests/test_gpu_dd_push.py)
+            assert l1(_arr(st.contents.pageRanks, N, np.float32), lim[code]) <= (1e-7 / 0.15 + 1e-6 if code == 7
+                                                                                  else 1e-4)
             L.freePageRankStats(st)
         a.contents.pushpull = 0
     L.graphCSRFree(g)
