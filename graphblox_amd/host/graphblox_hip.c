@@ -43,8 +43,12 @@ static void die_io(const char *what, const char *fname)
 static gbx_ctx *g_ctx = NULL;
 
 /* The pull walks the transpose rows with DIRECTED=1 (pageRank.c:266-268),
- * the forward rows with DIRECTED=0 (pageRank.c:269-272). */
+ * the forward rows with DIRECTED=0 (pageRank.c:269-272).  The push forms
+ * push along the forward out-edges under either setting (pageRank.c:424-437,
+ * SPMV.c:232-281), so each vertex receives the sum over its in-edges: the
+ * transpose rows' pull. */
 #define PULL_DIRECTION (GBX_DIRECTED ? GBX_PULL_IN : GBX_PULL_FWD)
+#define PUSH_DIRECTION GBX_PULL_IN
 
 /* What a cached device graph was made from: sizes, the addresses of the
  * host structs and arrays, and a hash of 64 sampled entries of every array.
@@ -739,8 +743,8 @@ void freePageRankStats(struct PageRankStats *stats)
 
 /* One PageRank run on the device in a given mode; stats as the reference
  * fills them (pageRank.c:339-365). */
-static struct PageRankStats *run_pagerank(struct Arguments *arguments, struct GraphCSR *graph, int mode,
-                                          const char *name)
+static struct PageRankStats *run_pagerank_dir(struct Arguments *arguments, struct GraphCSR *graph, int mode,
+                                              int direction, const char *name)
 {
     gbx_graph *g = device_graph(graph);
     if (!g) return NULL;
@@ -753,7 +757,7 @@ static struct PageRankStats *run_pagerank(struct Arguments *arguments, struct Gr
     p.mode = mode;
     p.stop = GBX_STOP_ACTIVE;
     p.precision = GBX_F32;
-    p.direction = PULL_DIRECTION;
+    p.direction = direction;
     gbx_pr_report rep;
     int rc = gbx_pr_pull(g_ctx, g, &p, NULL, stats->pageRanks, &rep);
     if (rc != GBX_OK) {
@@ -776,6 +780,18 @@ static struct PageRankStats *run_pagerank(struct Arguments *arguments, struct Gr
     return stats;
 }
 
+static struct PageRankStats *run_pagerank(struct Arguments *arguments, struct GraphCSR *graph, int mode,
+                                          const char *name)
+{
+    return run_pagerank_dir(arguments, graph, mode, PULL_DIRECTION, name);
+}
+
+static struct PageRankStats *run_pagerank_push(struct Arguments *arguments, struct GraphCSR *graph, int mode,
+                                               const char *name)
+{
+    return run_pagerank_dir(arguments, graph, mode, PUSH_DIRECTION, name);
+}
+
 /* pageRankPullGraphCSR, pageRank.c:245-368 (FP32; transpose rows with
  * DIRECTED=1, forward rows with DIRECTED=0) */
 struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
@@ -788,7 +804,7 @@ struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct G
  * pull iteration gathers; the device runs that iteration. */
 struct PageRankStats *pageRankPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
-    return run_pagerank(arguments, graph, GBX_PR_REF, "Page Rank Push (HIP, pull sums)");
+    return run_pagerank_push(arguments, graph, GBX_PR_REF, "Page Rank Push (HIP, pull sums)");
 }
 
 /* pageRankPullFixedPoint64BitGraphCSR, pageRank.c:496-630 (bit-exact) */
@@ -801,7 +817,7 @@ struct PageRankStats *pageRankPullFixedPoint64BitGraphCSR(struct Arguments *argu
  * same 64-bit integers as pageRankPullFixedPoint64BitGraphCSR (bit-exact). */
 struct PageRankStats *pageRankPushFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
-    return run_pagerank(arguments, graph, GBX_PR_FIXED64, "Page Rank Push Fixed Point (HIP, pull sums)");
+    return run_pagerank_push(arguments, graph, GBX_PR_FIXED64, "Page Rank Push Fixed Point (HIP, pull sums)");
 }
 
 /* pageRankPullFixedPoint32BitGraphCSR, pageRank.c:632-765 (bit-exact) */
@@ -846,7 +862,7 @@ struct PageRankStats *pageRankPullQuant8BitGraphCSR(struct Arguments *arguments,
  * pageRankPullQuant32BitGraphCSR in float; this runs the exact-sum form */
 struct PageRankStats *pageRankPushQuantGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
-    return run_pagerank(arguments, graph, GBX_PR_QUANT32, "Page Rank Push Quant_32 (HIP, pull sums)");
+    return run_pagerank_push(arguments, graph, GBX_PR_QUANT32, "Page Rank Push Quant_32 (HIP, pull sums)");
 }
 
 /* The data-driven variants take their worklist from the out-neighbours of
@@ -964,18 +980,19 @@ void freeSPMVStats(struct SPMVStats *stats)
     free(stats);
 }
 
-/* One SpMV variant on the device copy of graph: bits 0 = the float pull,
- * 32 / 64 = the fixed-point forms. */
-static struct SPMVStats *run_spmv(struct Arguments *arguments, struct GraphCSR *graph, int bits, const char *name)
+/* One SpMV variant on the device copy of graph: bits 0 = the float sums,
+ * 32 / 64 = the fixed-point forms; rows in `direction`. */
+static struct SPMVStats *run_spmv(struct Arguments *arguments, struct GraphCSR *graph, int bits, int direction,
+                                  const char *name)
 {
     gbx_graph *g = device_graph(graph);
     if (!g) return NULL;
     struct SPMVStats *stats = newSPMVStatsGraphCSR(graph);
     stats->arguments = arguments;
     double secs = 0.0;
-    int rc = bits ? gbx_spmv_fixed(g_ctx, g, arguments->iterations, GBX_WEIGHTED, PULL_DIRECTION, bits,
+    int rc = bits ? gbx_spmv_fixed(g_ctx, g, arguments->iterations, GBX_WEIGHTED, direction, bits,
                                    stats->vector_input, stats->vector_output, &secs)
-                  : gbx_spmv_pull(g_ctx, g, arguments->iterations, GBX_WEIGHTED, PULL_DIRECTION, NULL,
+                  : gbx_spmv_pull(g_ctx, g, arguments->iterations, GBX_WEIGHTED, direction, NULL,
                                   stats->vector_input, stats->vector_output, &secs);
     if (rc != GBX_OK) {
         fprintf(stderr, "graphblox_hip: %s failed: %s\n", name, gbx_last_error(g_ctx));
@@ -999,25 +1016,26 @@ static struct SPMVStats *run_spmv(struct Arguments *arguments, struct GraphCSR *
  * DIRECTED=1, forward ones with DIRECTED=0; weighted when GBX_WEIGHTED) */
 struct SPMVStats *SPMVPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
-    return run_spmv(arguments, graph, 0, "SPMVPullGraphCSR");
+    return run_spmv(arguments, graph, 0, PULL_DIRECTION, "SPMVPullGraphCSR");
 }
 
-/* SPMVPushGraphCSR, SPMV.c:216-311: the same float terms as the pull */
+/* SPMVPushGraphCSR, SPMV.c:216-311: the terms of the transpose rows' pull
+ * (it pushes along the forward out-edges under either DIRECTED setting) */
 struct SPMVStats *SPMVPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
-    return run_spmv(arguments, graph, 0, "SPMVPushGraphCSR");
+    return run_spmv(arguments, graph, 0, PUSH_DIRECTION, "SPMVPushGraphCSR");
 }
 
 /* SPMVPullFixedPointGraphCSR, SPMV.c:313-449 (bit-exact) */
 struct SPMVStats *SPMVPullFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
-    return run_spmv(arguments, graph, 32, "SPMVPullFixedPointGraphCSR");
+    return run_spmv(arguments, graph, 32, PULL_DIRECTION, "SPMVPullFixedPointGraphCSR");
 }
 
 /* SPMVPushFixedPointGraphCSR, SPMV.c:451-590 (bit-exact) */
 struct SPMVStats *SPMVPushFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
-    return run_spmv(arguments, graph, 64, "SPMVPushFixedPointGraphCSR");
+    return run_spmv(arguments, graph, 64, PUSH_DIRECTION, "SPMVPushFixedPointGraphCSR");
 }
 
 /* SPMVGraphCSR, SPMV.c:89-116: the variant selected by arguments->pushpull */

@@ -22,8 +22,9 @@ in full):
     R-MAT s12, unweighted (WEIGHTED=0 build) and weighted (WEIGHTED=1 build);
   * DIRECTED=0 builds (graphConfig.h:14 as shipped): on R-MAT s12, sort mode
     1, SPMVGraphCSR codes 0-3 (3 passes, unweighted and weighted) and the
-    pageRankGraphCSR variants 0, 2, 4 and 9 after 20 iterations (epsilon 0),
-    all over the forward rows (pageRank.c:269-272, SPMV.c:141-147).
+    pageRankGraphCSR variants 0-5 and 9 after 20 iterations (epsilon 0): the
+    pull forms over the forward rows (pageRank.c:269-272, SPMV.c:141-147),
+    the push forms along the forward out-edges (in-edge sums).
 The inputs are regenerated by the tests from the same goldens and seeds.
 """
 from __future__ import annotations
@@ -157,7 +158,7 @@ def main():
                 R.refd_spmv_variant(g, code, 3, 1, y, x)
                 out[f"d0/spmv/w{weighted}/code{code}"] = y
             if not weighted:
-                for code in (0, 2, 4, 9):
+                for code in (0, 1, 2, 3, 4, 5, 9):
                     r = np.zeros(N, np.float32)
                     it, err = C.c_uint32(), C.c_double()
                     R.refd_pagerank_variant(g, code, 20, 0.0, 1, r, C.byref(it), C.byref(err))
