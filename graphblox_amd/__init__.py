@@ -117,6 +117,7 @@ def lib():
         "gbx_pr_plan_info": (i32, [_vp, C.POINTER(u32), _vp, _vp, _vp]),
         "gbx_spmv_pull": (i32, [_vp, _vp, u32, i32, i32, _vp, _vp, _vp, C.POINTER(f64)]),
         "gbx_spmv_fixed": (i32, [_vp, _vp, u32, i32, i32, i32, _vp, _vp, C.POINTER(f64)]),
+        "gbx_spmv_pull_noalloc": (i32, [_vp, _vp, u32, i32, i32, _vp, _vp]),
         "gbx_gather_accumulate_u32": (i32, [_vp, _vp, i32, _vp, _vp]),
         "gbx_ppr_create": (i32, [_vp, _vp, C.POINTER(PrParams), u32, _vp, _vp, C.POINTER(_vp)]),
         "gbx_ppr_info": (i32, [_vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
@@ -148,7 +149,7 @@ def exported_symbols():
         "gbx_pr_launch_timed", "gbx_pr_read", "gbx_pr_free", "gbx_pr_plan_info", "gbx_spmv_pull", "gbx_spmv_fixed",
         "gbx_ppr_create", "gbx_ppr_info", "gbx_pr_launch_graph", "gbx_graph_batch_create",
         "gbx_graph_batch_info", "gbx_tuning_get", "gbx_tuning_set", "gbx_autotune",
-        "gbx_pr_adjoint", "gbx_gather_accumulate_u32",
+        "gbx_pr_adjoint", "gbx_spmv_pull_noalloc", "gbx_gather_accumulate_u32",
     ]
 
 
@@ -376,6 +377,12 @@ class Graph:
         self.ctx.check(self.ctx.L.gbx_spmv_pull(self.ctx.h, self.h, iterations, int(weighted), direction, _ptr(xi),
                                                 _ptr(xo), _ptr(yo), C.byref(sec)))
         return xo, yo, sec.value
+
+    def spmv_noalloc(self, x_ptr: int, y_ptr: int, iterations=1, weighted=False, direction=PULL_IN):
+        """gbx_spmv_pull_noalloc on device arrays x, y [N] (vertex order):
+        launched on the context's stream, not waited for."""
+        self.ctx.check(self.ctx.L.gbx_spmv_pull_noalloc(self.ctx.h, self.h, iterations, int(weighted), direction,
+                                                         x_ptr, y_ptr))
 
     def gather_accumulate_u32(self, aux1, aux2=None, direction=PULL_IN):
         """The FPGA overlay's Single.PR integer pull on u32[2N] buffers:

@@ -4,7 +4,7 @@
 #include <cstdlib>
 #include <mutex>
 
-#include "gbx_internal.hpp"
+#include "gbx_plan.hpp"
 
 namespace gbx {
 
@@ -329,6 +329,13 @@ int gbx_init(int device, gbx_ctx **out)
     ctx->err = "no error";
     gbx::tuning_defaults(&ctx->tune);
     gbx::tuning_from_env(&ctx->tune);
+    // the device address of the allocation-free entry point's scratch array
+    if (int src = gbx::noalloc_setup(ctx)) {
+        int rc = gbx::set_error(nullptr, src, "%s", ctx->err.c_str());
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return rc;
+    }
     *out = ctx;
     return GBX_OK;
 }

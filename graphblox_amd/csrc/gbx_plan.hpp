@@ -510,5 +510,8 @@ int plan_row_ptr_prefix(gbx_ctx *ctx, const gbx_plan *p, uint32_t nrows, std::ve
 PlanView plan_view(const gbx_plan *p, const float *w);
 unsigned iter_grid(const gbx_ctx *ctx, const gbx_plan *p);
 
+// Per-device set-up of the allocation-free entry point (gbx_noalloc.hip),
+// run by gbx_init with the context's device current.
+int noalloc_setup(gbx_ctx *ctx);
 
 }  // namespace gbx

@@ -23,9 +23,9 @@
  *   gbx_gather_accumulate_u32         the FPGA overlay's Single.PR integer pull
  *                                     (02_device/ol/GraphBlox/Single/topology.json:162-201,
  *                                     01_host/src/utils_fpga_cpp/buffer_mapping.PR.cpp:2-68)
- *   gbx_ppr_batched, gbx_graph_batch_*, gbx_pr_adjoint, gbx_rmat_*
- *                                     no reference counterpart (batched, differentiable
- *                                     and generator forms of the same path)
+ *   gbx_ppr_batched, gbx_graph_batch_*, gbx_pr_adjoint, gbx_rmat_*, gbx_spmv_pull_noalloc
+ *                                     no reference counterpart (batched, differentiable,
+ *                                     generator and allocation-free forms of the same path)
  */
 #ifndef GBX_H
 #define GBX_H
@@ -390,6 +390,22 @@ int gbx_pr_plan_info(const gbx_pr *pr, uint32_t *n_classes, uint32_t *class_lane
 int gbx_spmv_pull(gbx_ctx *ctx, gbx_graph *g, uint32_t iterations, int weighted,
                   int direction, const float *x_in, float *x_out, float *y_out,
                   double *seconds);
+
+/* gbx_spmv_pull on caller-owned device vectors, allocating nothing (no
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory, such as code under HIP graph capture or a serving loop).  x, y:
+ * device arrays [N] in the graph's vertex order; y = the sum of
+ * `iterations` passes of y += w[j] x[col[j]] from y = 0 (w = 0.0001f unless
+ * weighted), the same values gbx_spmv_pull returns for that x.  Launched
+ * on the context's stream and not waited for.  Needs the degree-binned
+ * plan of `direction` to exist already (built by gbx_spmv_pull or
+ * gbx_pr_create on this graph; GBX_ERR_STATE otherwise).  Hub chunks are
+ * combined in a device array of the library (512 KiB per device, held
+ * until the process ends): plans of more than 65536 hub chunks or hub rows
+ * are refused (GBX_ERR_RANGE).  Calls that overlap on one GPU race on that
+ * array. */
+int gbx_spmv_pull_noalloc(gbx_ctx *ctx, gbx_graph *g, uint32_t iterations, int weighted,
+                          int direction, const float *x, float *y);
 
 /* The reference's fixed-point SpMV variants, bit-exact: bits 32 =
  * SPMVPullFixedPointGraphCSR (SPMV.c:313-449, rows as `direction`), bits 64
