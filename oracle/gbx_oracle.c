@@ -121,7 +121,7 @@ int orc_pr_pull_f32(uint32_t N, const uint32_t *fwd_out_degree,
     float *next = malloc((size_t)N * sizeof(float) + 4);
     float *contrib = malloc((size_t)N * sizeof(float) + 4);
     if (!next || !contrib) { free(next); free(contrib); return -1; }
-    if (nthreads < 1) nthreads = 1;
+    if (nthreads < 1) nthreads = omp_get_max_threads(); /* 0: every OpenMP thread */
     double error_total = 0.0;
     uint32_t it;
 
@@ -179,7 +179,7 @@ int orc_spmv_pull_f32(uint32_t N, const uint32_t *fwd_out_degree,
                       uint32_t iterations, int nthreads, float *x_out,
                       float *y_out, double *iter_seconds)
 {
-    if (nthreads < 1) nthreads = 1;
+    if (nthreads < 1) nthreads = omp_get_max_threads(); /* 0: every OpenMP thread */
     /* SPMV.c:156-163 */
 #pragma omp parallel for num_threads(nthreads)
     for (uint32_t v = 0; v < N; v++) {

@@ -51,6 +51,7 @@ uint32_t orc_remove_duplicates(uint32_t *src, uint32_t *dst, float *w,
  * fwd_out_degree: forward out-degree (divisor); rows come from
  * (row_degree,row_idx,row_col).  ranks_out receives the normalised ranks.
  * iter_seconds (nullable, length max_iters) receives per-iteration wall time.
+ * nthreads < 1 runs on every OpenMP thread (omp_get_max_threads()).
  */
 int orc_pr_pull_f32(uint32_t N, const uint32_t *fwd_out_degree,
                     const uint32_t *row_degree, const uint32_t *row_idx,
