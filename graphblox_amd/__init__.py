@@ -353,10 +353,11 @@ class Graph:
         self.ctx.check(self.ctx.L.gbx_pr_pull(self.ctx.h, self.h, C.byref(p), _ptr(tp), _ptr(ranks), C.byref(rep)))
         return ranks, rep.as_dict()
 
-    def adjoint(self, loss_weights, teleport=None, iterations=300, epsilon=1e-7, damp=0.85, stop=STOP_L1):
+    def adjoint(self, loss_weights, teleport=None, iterations=300, epsilon=1e-7, damp=0.85, stop=STOP_L1,
+                method=METHOD_AUTO):
         """Forward weighted stochastic PageRank p and its adjoint: returns
         p, lambda, dL/dW (forward-CSR edge order), dL/dt and both reports."""
-        p = params(iterations, epsilon, PR_STOCHASTIC, stop, damp, PULL_IN, True)
+        p = params(iterations, epsilon, PR_STOCHASTIC, stop, damp, PULL_IN, True, method=method)
         N, E = self.N, self.E
         out = dict(p=np.empty(N, np.float32), lam=np.empty(N, np.float32), grad_w=np.empty(E, np.float32),
                    grad_t=np.empty(N, np.float32))

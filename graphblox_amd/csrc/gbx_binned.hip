@@ -66,6 +66,7 @@
 namespace gbx {
 
 constexpr uint32_t kMaxBinRows = 20000;     // LDS int64 row sums of one bin
+constexpr uint32_t kNoEdge = 0xFFFFFFFFu;   // no plan edge behind a layout entry
 constexpr uint32_t kMaxChunkRows = 32768;   // LDS contributions of one chunk; u16 local ids
 constexpr uint32_t kNoMulti = 0xFFFFFFFFu;
 
@@ -138,6 +139,17 @@ struct gbx_bplan {
                                       // packed (hot16): u16[Hlen] source | row step << 14
     gbx::DevBuf hot_base;             // packed: u16[Hlen/8] row - bin first row of each group's first pair
     gbx::DevBuf units;                // BinUnit[NU]
+    // run layout: the plan edge behind every src_local entry and every packed
+    // hot pair (kNoEdge for pads and fillers), from which a per-edge weight
+    // array of the plan is laid out beside them (weighted kinds)
+    gbx::DevBuf cold_eidx;            // u32[Plen]
+    gbx::DevBuf hot_eidx;             // u32[Hlen]
+    struct EdgeWeights {
+        const void *src = nullptr;    // the plan's weight array (plan edge order) they were made from
+        gbx::DevBuf cold;             // f32[Plen] weight of every src_local entry (0 for pads)
+        gbx::DevBuf hot;              // f32[Hlen] weight of every hot pair (0 for fillers)
+    };
+    std::vector<std::unique_ptr<EdgeWeights>> weights;
 };
 
 struct gbx_binned_state {
@@ -150,6 +162,7 @@ struct gbx_binned_state {
     uint32_t dummy_off = 0;  // run layout: byte offset of the scatter's dummy areas in V
     uint32_t stage_off = 0;  // run layout: byte offset of the scatter's staging in LDS
     size_t shm_scatter = 0, shm_accum = 0;
+    const float *wcold = nullptr, *whot = nullptr;  // weighted kinds: the layout's edge weights
 };
 
 namespace gbx {
@@ -285,7 +298,7 @@ __global__ void split_hot_cold(const uint32_t *__restrict__ col, const uint32_t 
                                const uint32_t *__restrict__ bin_of, const uint32_t *__restrict__ bin_row0,
                                const uint32_t *__restrict__ hot_before, const uint32_t *__restrict__ hot_start,
                                uint32_t *__restrict__ hot_pairs, uint32_t *__restrict__ rowid_c,
-                               uint32_t *__restrict__ col_c)
+                               uint32_t *__restrict__ col_c, uint32_t *__restrict__ eidx_c)
 {
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
         const uint32_t v = rowid[e], u = col[e];
@@ -298,6 +311,7 @@ __global__ void split_hot_cold(const uint32_t *__restrict__ col, const uint32_t 
             const uint32_t c = e - hrank[e];
             rowid_c[c] = v;
             col_c[c] = u;
+            eidx_c[c] = e;
         }
     }
 }
@@ -366,8 +380,9 @@ __global__ void run_fill(const uint32_t *__restrict__ key, const uint32_t *__res
                          const uint32_t *__restrict__ voff, const uint32_t *__restrict__ rowid_c,
                          const uint32_t *__restrict__ col_c, const uint32_t *__restrict__ chunk_lo,
                          const uint32_t *__restrict__ bin_row0, uint32_t NB, uint32_t pad,
-                         uint16_t *__restrict__ src_local, uint16_t *__restrict__ dst_local,
-                         uint32_t *__restrict__ blk_v)
+                         const uint32_t *__restrict__ eidx_c, uint16_t *__restrict__ src_local,
+                         uint16_t *__restrict__ dst_local, uint32_t *__restrict__ blk_v,
+                         uint32_t *__restrict__ cold_eidx)
 {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         const uint32_t r = runidx[j] + flag[j] - 1u, s0 = runstart[r];
@@ -383,6 +398,7 @@ __global__ void run_fill(const uint32_t *__restrict__ key, const uint32_t *__res
             src_local[p | (kRunGroup - 1)] = (uint16_t)(pad | kRunEnd);
         }
         src_local[p] = (uint16_t)((col_c[e] - chunk_lo[k]) | (flag_here ? kRunEnd : 0u));
+        cold_eidx[p] = eidx_c[e];
         if (end) dst_local[vpos] = (uint16_t)(rowid_c[e] - bin_row0[b]);
         if (p % (kRunGroup * kRunBlockGroups) == 0) blk_v[p / (kRunGroup * kRunBlockGroups)] = vpos;
     }
@@ -428,12 +444,14 @@ constexpr uint32_t kHot16Slots = 16384; // LDS hot slots (kHotPad included)
 // Hot edges (plan order) -> compacted rows and sources.
 __global__ void hot_compact(const uint32_t *__restrict__ col, const uint32_t *__restrict__ rowid, uint32_t E,
                             const uint32_t *__restrict__ hflag, const uint32_t *__restrict__ hrank,
-                            uint32_t *__restrict__ rowid_h, uint32_t *__restrict__ col_h)
+                            uint32_t *__restrict__ rowid_h, uint32_t *__restrict__ col_h,
+                            uint32_t *__restrict__ eidx_h)
 {
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x)
         if (hflag[e]) {
             rowid_h[hrank[e]] = rowid[e];
             col_h[hrank[e]] = col[e];
+            eidx_h[hrank[e]] = e;
         }
 }
 
@@ -456,7 +474,8 @@ __global__ void hot_fill16(const uint32_t *__restrict__ rowid_h, const uint32_t 
                            const uint32_t *__restrict__ n, const uint32_t *__restrict__ hpos,
                            const uint32_t *__restrict__ bin_of, const uint32_t *__restrict__ hot_before,
                            const uint32_t *__restrict__ hot_start, const uint32_t *__restrict__ bin_row0,
-                           uint16_t *__restrict__ pairs, uint16_t *__restrict__ hbase)
+                           const uint32_t *__restrict__ eidx_h, uint16_t *__restrict__ pairs,
+                           uint16_t *__restrict__ hbase, uint32_t *__restrict__ hot_eidx)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < Eh; i += gridDim.x * blockDim.x) {
         const uint32_t row = rowid_h[i], b = bin_of[row], fill = n[i] - 1u;
@@ -469,6 +488,7 @@ __global__ void hot_fill16(const uint32_t *__restrict__ rowid_h, const uint32_t 
         }
         const uint32_t pos = base + fill;
         pairs[pos] = (uint16_t)(col_h[i] | ((d - 3u * fill) << 14));
+        hot_eidx[pos] = eidx_h[i];
         if ((pos & 7u) == 0) hbase[pos >> 3] = (uint16_t)(row - bin_row0[b]);
     }
 }
@@ -597,6 +617,7 @@ struct RunsView {
     uint32_t stage_off;  // byte offset of the per-wave staging in LDS
     uint32_t dummy;      // byte offset in V of the dummy areas past the last run (written by idle store
                          // lanes, never read): kRunDummyAreas areas of 64 values
+    const float *__restrict__ wcold;  // weighted kinds: the weight of every src_local entry (0 for pads)
 };
 
 // Values summed in FP32 (BF16 widened; a run's sum rounded to BF16 once).
@@ -667,8 +688,13 @@ struct RunStep {
     uint64_t lead;   // lanes that lead a segment
 };
 
-template <class T>
-__device__ __forceinline__ RunStep runs_sum(const T *cl, uint4 w, uint32_t vbase, bool leader)
+// Weights of a group of 8 entries (weighted kinds), two 16-byte loads.
+struct W8 {
+    uint4 a, b;
+};
+
+template <class T, bool W>
+__device__ __forceinline__ RunStep runs_sum(const T *cl, uint4 w, const W8 &wt, uint32_t vbase, bool leader)
 {
     RunStep st;
     const uint32_t ew[4] = {w.x, w.y, w.z, w.w};
@@ -677,6 +703,11 @@ __device__ __forceinline__ RunStep runs_sum(const T *cl, uint4 w, uint32_t vbase
     for (int i = 0; i < 8; i++) {
         const uint32_t h = (i & 1) ? __builtin_amdgcn_ubfe(ew[i >> 1], 16, 15) : (ew[i >> 1] & (kRunEnd - 1u));
         x[i] = f32_of(cl[h]);
+    }
+    if constexpr (W) {
+        const uint32_t wb[8] = {wt.a.x, wt.a.y, wt.a.z, wt.a.w, wt.b.x, wt.b.y, wt.b.z, wt.b.w};
+#pragma unroll
+        for (int i = 0; i < 8; i++) x[i] = __uint_as_float(wb[i]) * x[i];  // the edge's weight times its source's value
     }
     st.endm = 0;
 #pragma unroll
@@ -740,7 +771,7 @@ __device__ __forceinline__ void runs_out(const float *stage, const RunStep &st, 
 // are in flight while a step is summed; a wave always runs whole rounds of
 // kRunsAhead steps (steps past the chunk's last group load a valid group
 // and stage nothing).
-template <class T>
+template <class T, bool W = false>
 __global__ void __launch_bounds__(kRunsBlock) bin_scatter_runs_kernel(RunsView S, const Slot *slots, uint32_t t,
                                                                       const T *__restrict__ c_in, T *__restrict__ V)
 {
@@ -766,14 +797,20 @@ __global__ void __launch_bounds__(kRunsBlock) bin_scatter_runs_kernel(RunsView S
     const uint32_t padw = S.pad | (S.pad << 16);
     const uint4 padv = make_uint4(padw, padw, padw, padw);
     const uint4 *src = reinterpret_cast<const uint4 *>(S.src_local);
+    const uint4 *wsrc = reinterpret_cast<const uint4 *>(S.wcold);
     const uint32_t blk_last = (g1 - 1u) / kRunBlockGroups;
     uint4 w[kRunsAhead], rec[kRunsAhead];
+    W8 wt[kRunsAhead];
     uint32_t sv[2];  // the V offset of the segment each lane would start, two steps ahead
     // a step's group data and block record; loads are always issued
     // (clamped into the chunk), lanes outside it take pads when summed
     auto load = [&](int d, uint32_t first) {
         const uint32_t g = min(max(first + lane, g0), g1 - 1u);
         w[d] = src[g];
+        if constexpr (W) {
+            wt[d].a = wsrc[2u * g];
+            wt[d].b = wsrc[2u * g + 1u];
+        }
         rec[d] = S.blk_rec[min(first / kRunBlockGroups, blk_last)];
     };
     // seg_v of the segment starting at each lane's group (used by segment
@@ -791,6 +828,8 @@ __global__ void __launch_bounds__(kRunsBlock) bin_scatter_runs_kernel(RunsView S
         const uint32_t g = first + lane;
         const bool act = g >= g0 && g < g1;
         const uint4 cw = act ? w[d] : padv;
+        W8 cwt{};  // pads read the LDS zero: any finite weight gives 0
+        if constexpr (W) cwt = wt[d];
         const uint64_t bits = ((uint64_t)rec[d].w << 32) | rec[d].z;
         const bool start = act && ((bits >> lane) & 1u);
         const bool leader = start || (act && lane == 0);
@@ -798,7 +837,7 @@ __global__ void __launch_bounds__(kRunsBlock) bin_scatter_runs_kernel(RunsView S
         __builtin_amdgcn_sched_barrier(0);
         load(d, first + kRunsAhead * kRunsBlock);
         load_sv((d + 2) % kRunsAhead, par);
-        return runs_sum<T>(cl, cw, vbase, leader);
+        return runs_sum<T, W>(cl, cw, cwt, vbase, leader);
     };
     // steps start at multiples of 64 groups of the whole src_local
     uint32_t r = (g0 & ~(kRunBlockGroups - 1u)) + (threadIdx.x & ~63u);
@@ -842,6 +881,7 @@ struct AccumView {
     uint32_t hot_off;  // byte offset of the hot contributions in LDS
     uint32_t *work;  // next unit (zero between iterations)
     uint32_t NU;     // units
+    const float *__restrict__ whot;  // weighted kinds: the weight of every hot pair (0 for fillers)
 };
 
 // Values as loaded (raw) and as 64-bit fixed point.  Float kinds keep FP32
@@ -920,16 +960,28 @@ __device__ __forceinline__ void load_hot8_any(const AccumView &B, uint32_t e, co
         load_hot8_packed<KIND, T>(reinterpret_cast<const uint16_t *>(B.hot_pairs), B.hot_base, e, ch, d, v);
     else
         load_hot8<KIND, T>(B.hot_pairs, e, ch, d, v);
+    if constexpr (is_weighted_kind(KIND)) {
+        // the pair's edge weight times its source's value (weights of fillers are 0)
+        const Vec8<float> w = *reinterpret_cast<const Vec8<float> *>(B.whot + e);
+#pragma unroll
+        for (int i = 0; i < 8; i++) v[i] = w.v[i] * v[i];
+    }
 }
 
 template <int KIND, class T>
-using EpiOf = std::conditional_t<is_fixed_kind(KIND), FixEpi<KIND, T>,
-                                 std::conditional_t<is_quant_kind(KIND), QuantEpi<KIND>, PrEpi<KIND, T>>>;
+using EpiOf = std::conditional_t<
+    KIND == K_SPMV_W, SpmvBinEpi,
+    std::conditional_t<is_fixed_kind(KIND), FixEpi<KIND, T>,
+                       std::conditional_t<is_quant_kind(KIND), QuantEpi<KIND>, PrEpi<KIND, T>>>>;
 
 constexpr int kAccumUnroll = 4;  // groups of 8 per lane in flight
 constexpr uint32_t kAccumSlab = 64u * 8u * kAccumUnroll;  // V entries a wave takes at a time
-constexpr int kHotUnroll = 2;                               // groups of 8 hot pairs per lane in flight
-constexpr uint32_t kHotSlab = 64u * 8u * kHotUnroll;        // hot pairs a wave takes at a time
+// groups of 8 hot pairs per lane in flight (weighted kinds: one, for the
+// registers their weights take)
+template <int KIND>
+constexpr int hot_unroll() { return is_weighted_kind(KIND) ? 1 : 2; }
+template <int KIND>
+constexpr uint32_t hot_slab() { return 64u * 8u * hot_unroll<KIND>(); }  // hot pairs a wave takes at a time
 constexpr int kEpiBatch = 8;     // rows per thread whose update inputs load together
 
 // Thread 0 takes the next unit from the counter and loads its record.
@@ -1012,6 +1064,8 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
         // entries) from an LDS counter: waves that run ahead take more, so the
         // block reaches the update barrier together.
         const uint32_t lane = threadIdx.x & 63u;
+        constexpr int kHotUnroll = hot_unroll<KIND>();
+        constexpr uint32_t kHotSlab = hot_slab<KIND>();
         const uint32_t nh = (u.hend - u.hbeg + kHotSlab - 1) / kHotSlab;
         const uint32_t nv = (u.vend - u.vbeg + kAccumSlab - 1) / kAccumSlab;
         constexpr uint32_t kq = 64u * 8u;  // entries of one group of 8 per lane
@@ -1138,12 +1192,17 @@ __global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const ui
     EpiOf<KIND, T> epi;
     if constexpr (is_fixed_kind(KIND)) epi = make_fix_epi<KIND, T>(outdeg, N, A);
     else if constexpr (is_quant_kind(KIND)) epi = make_quant_epi<KIND>(outdeg, N, A, cur);
+    else if constexpr (KIND == K_SPMV_W) epi = make_spmv_epi(A);
     else epi = make_epi<KIND, T>(outdeg, N, A, cur);
-    // float kinds: every row sum is at most the sum of all contributions,
-    // itself at most the previous rank sum (bf16 rounding adds < 0.4 %);
+    // float kinds: every row sum is at most the sum of the magnitudes of its
+    // terms, and those of all rows together are at most: the previous rank
+    // sum (each source's transition weights sum to at most 1; bf16 rounding
+    // adds < 0.4 %), for the adjoint the previous sum of |lambda| (each row
+    // of P sums to 1, so a row sum is at most max |lambda|), for the SpMV the
+    // sum over all edges of |w x| that its reset stores in the slot;
     // fixed-point and quantized kinds sum their integers as they are (uint64
     // wrap-around, as the reference's uint64 sums)
-    const double bound = cur->sum * 1.01 + 1e-30;
+    const double bound = (KIND == K_ADJ ? cur->dang : cur->sum) * 1.01 + 1e-30;
     int ex = 0;
     (void)frexp(bound, &ex);
     const double scale = ldexp(1.0, 61 - ex), inv = ldexp(1.0, ex - 61);
@@ -1178,7 +1237,13 @@ __global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const ui
     const int stop = A.stop;
     uint32_t *work = B.work;
     grid_reduce(epi.part, A.partials, A.counter, [&](const Partial &tot) {
-        *nxt = next_slot<KIND>(tot, iters, eps, stop);
+        if constexpr (KIND == K_SPMV_W) {
+            Slot s = *cur;  // the bound of the (unchanging) input vector carries over
+            s.iters = iters + 1;
+            *nxt = s;
+        } else {
+            *nxt = next_slot<KIND>(tot, iters, eps, stop);
+        }
         *work = 0;  // every block has left its unit loop
     });
 }
@@ -1230,8 +1295,8 @@ static int find_runs(gbx_ctx *ctx, const uint32_t *key, uint32_t E, uint32_t pad
 // bin-major, within a bin chunk-major, one value per run, segments padded
 // to 8 values (zeros, never written by the scatter).
 static int build_runs(gbx_ctx *ctx, gbx_bplan *bp, uint32_t Ecold, const uint32_t *rowid_c, const uint32_t *col_c,
-                      const uint32_t *bin_of, const uint32_t *chunk_of, const uint32_t *bin_row0,
-                      const uint32_t *cstart, uint32_t *bin_len)
+                      const uint32_t *eidx_c, const uint32_t *bin_of, const uint32_t *chunk_of,
+                      const uint32_t *bin_row0, const uint32_t *cstart, uint32_t *bin_len)
 {
     const uint32_t NB = bp->NB, NC = bp->NC;
     const uint32_t kbits = bits_for(NB * NC - 1);
@@ -1298,12 +1363,15 @@ static int build_runs(gbx_ctx *ctx, gbx_bplan *bp, uint32_t Ecold, const uint32_
     GBX_TRY(dev_alloc(ctx, bp->bdst, 16));
     GBX_TRY(fill_u32(ctx, bp->src_local.as<uint32_t>(), bp->SC | (bp->SC << 16), bp->Plen / 2 + 4));
     GBX_HIP(ctx, hipMemsetAsync(bp->dst_local.p, 0, (size_t)Vlen * 2 + 16, ctx->stream));
+    GBX_TRY(dev_alloc(ctx, bp->cold_eidx, (size_t)bp->Plen * 4 + 16));
+    GBX_TRY(fill_u32(ctx, bp->cold_eidx.as<uint32_t>(), kNoEdge, bp->Plen + 4));
 
     run_fill<<<gE, 256, 0, ctx->stream>>>(key.as<uint32_t>(), pay.as<uint32_t>(), Ecold, R.flag.as<uint32_t>(),
                                           R.idx.as<uint32_t>(), R.start.as<uint32_t>(), R.pstart.as<uint32_t>(),
                                           head.as<uint32_t>(), hx.as<uint32_t>(), voff.as<uint32_t>(), rowid_c, col_c,
-                                          bp->chunk_lo.as<uint32_t>(), bin_row0, NB, bp->SC, bp->src_local.as<uint16_t>(),
-                                          bp->dst_local.as<uint16_t>(), blk_v.as<uint32_t>());
+                                          bp->chunk_lo.as<uint32_t>(), bin_row0, NB, bp->SC, eidx_c,
+                                          bp->src_local.as<uint16_t>(), bp->dst_local.as<uint16_t>(),
+                                          blk_v.as<uint32_t>(), bp->cold_eidx.as<uint32_t>());
     GBX_HIP(ctx, hipGetLastError());
     // block records: segment-start bits, their running count, first V positions
     run_segment_bits<<<gS, 256, 0, ctx->stream>>>(R.pstart.as<uint32_t>(), ns, bits.as<uint32_t>());
@@ -1430,7 +1498,7 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
         GBX_HIP(ctx, hipGetLastError());
     }
     // hot edges: per bin, in plan edge order (rows ascending), padded to 8
-    DevBuf hflag, hrank, rowid_c, col_c;
+    DevBuf hflag, hrank, rowid_c, col_c, eidx_c;
     if (E) {
         GBX_TRY(dev_alloc(ctx, hflag, (size_t)E * 4 + 4));
         GBX_TRY(dev_alloc(ctx, hrank, (size_t)(E + 1) * 4));
@@ -1452,20 +1520,21 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
         GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if ((uint64_t)hot_before[NB] != Ehot) return set_error(ctx, GBX_ERR_STATE, "binned layout: hot edge count");
         // packed pairs: fillers per pair, their running count and each bin's packed length
-        DevBuf rowid_h, col_h, hn, hp;
+        DevBuf rowid_h, col_h, eidx_h, hn, hp;
         std::vector<uint32_t> packed_before(NB + 1);
         const uint32_t Eh = (uint32_t)Ehot;
         if (hot16) {
             DevBuf hptot, d_pb;
             GBX_TRY(dev_alloc(ctx, rowid_h, (size_t)Eh * 4 + 4));
             GBX_TRY(dev_alloc(ctx, col_h, (size_t)Eh * 4 + 4));
+            GBX_TRY(dev_alloc(ctx, eidx_h, (size_t)Eh * 4 + 4));
             GBX_TRY(dev_alloc(ctx, hn, (size_t)Eh * 4 + 4));
             GBX_TRY(dev_alloc(ctx, hp, (size_t)(Eh + 1) * 4));
             GBX_TRY(dev_alloc(ctx, hptot, 16));
             GBX_TRY(dev_alloc(ctx, d_pb, (size_t)(NB + 1) * 4));
             hot_compact<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(
                 p->col.as<uint32_t>(), rowid.as<uint32_t>(), E, hflag.as<uint32_t>(), hrank.as<uint32_t>(),
-                rowid_h.as<uint32_t>(), col_h.as<uint32_t>());
+                rowid_h.as<uint32_t>(), col_h.as<uint32_t>(), eidx_h.as<uint32_t>());
             hot_counts<<<grid_for(Eh, 256, 16384), 256, 0, ctx->stream>>>(
                 rowid_h.as<uint32_t>(), Eh, d_bin_of.as<uint32_t>(), d_hot_before.as<uint32_t>(), hn.as<uint32_t>());
             GBX_HIP(ctx, hipGetLastError());
@@ -1493,6 +1562,8 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
             GBX_TRY(fill_u32(ctx, bp->hot_pairs.as<uint32_t>(), kHotPad | (kHotPad << 16), bp->Hlen / 2 + 4));
             GBX_TRY(dev_alloc(ctx, bp->hot_base, (size_t)(bp->Hlen / 8) * 2 + 16));
             GBX_HIP(ctx, hipMemsetAsync(bp->hot_base.p, 0, (size_t)(bp->Hlen / 8) * 2 + 16, ctx->stream));
+            GBX_TRY(dev_alloc(ctx, bp->hot_eidx, (size_t)bp->Hlen * 4 + 16));
+            GBX_TRY(fill_u32(ctx, bp->hot_eidx.as<uint32_t>(), kNoEdge, bp->Hlen + 4));
         } else {
             GBX_TRY(dev_alloc(ctx, bp->hot_pairs, (size_t)bp->Hlen * 4 + 16));
             GBX_TRY(fill_u32(ctx, bp->hot_pairs.as<uint32_t>(), 0xFFFF0000u, bp->Hlen));
@@ -1504,16 +1575,18 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
             hot_fill16<<<grid_for(Eh, 256, 16384), 256, 0, ctx->stream>>>(
                 rowid_h.as<uint32_t>(), col_h.as<uint32_t>(), Eh, hn.as<uint32_t>(), hp.as<uint32_t>(),
                 d_bin_of.as<uint32_t>(), d_hot_before.as<uint32_t>(), d_hot_start.as<uint32_t>(),
-                d_bin_row0.as<uint32_t>(), bp->hot_pairs.as<uint16_t>(), bp->hot_base.as<uint16_t>());
+                d_bin_row0.as<uint32_t>(), eidx_h.as<uint32_t>(), bp->hot_pairs.as<uint16_t>(),
+                bp->hot_base.as<uint16_t>(), bp->hot_eidx.as<uint32_t>());
             GBX_HIP(ctx, hipGetLastError());
         }
         GBX_TRY(dev_alloc(ctx, rowid_c, (size_t)Ecold * 4 + 4));
         GBX_TRY(dev_alloc(ctx, col_c, (size_t)Ecold * 4 + 4));
+        GBX_TRY(dev_alloc(ctx, eidx_c, (size_t)Ecold * 4 + 4));
         split_hot_cold<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(
             p->col.as<uint32_t>(), rowid.as<uint32_t>(), E, hflag.as<uint32_t>(), hrank.as<uint32_t>(),
             d_bin_of.as<uint32_t>(), d_bin_row0.as<uint32_t>(), d_hot_before.as<uint32_t>(),
             d_hot_start.as<uint32_t>(), hot16 ? nullptr : bp->hot_pairs.as<uint32_t>(), rowid_c.as<uint32_t>(),
-            col_c.as<uint32_t>());
+            col_c.as<uint32_t>(), eidx_c.as<uint32_t>());
         GBX_HIP(ctx, hipGetLastError());
         GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     } else {
@@ -1521,9 +1594,9 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
         GBX_TRY(dev_alloc(ctx, bp->hot_base, 16));
     }
     if (Ecold && runs) {
-        GBX_TRY(build_runs(ctx, bp, Ecold, rowid_c.as<uint32_t>(), col_c.as<uint32_t>(), d_bin_of.as<uint32_t>(),
-                           d_chunk_of.as<uint32_t>(), d_bin_row0.as<uint32_t>(), d_cstart.as<uint32_t>(),
-                           d_bin_len.as<uint32_t>()));
+        GBX_TRY(build_runs(ctx, bp, Ecold, rowid_c.as<uint32_t>(), col_c.as<uint32_t>(), eidx_c.as<uint32_t>(),
+                           d_bin_of.as<uint32_t>(), d_chunk_of.as<uint32_t>(), d_bin_row0.as<uint32_t>(),
+                           d_cstart.as<uint32_t>(), d_bin_len.as<uint32_t>()));
     } else if (Ecold) {
         const uint32_t kbits = bits_for(NB * NC - 1);
         DevBuf key, pay, voff_table;
@@ -1663,8 +1736,60 @@ static void grant_lds(K *kernel, size_t shm)
 
 bool binned_supported(const gbx_pr *pr)
 {
-    return !pr->batch && !pr->seg && !pr->plan->segmented && pr->g->N > 0 &&
-           (pr->kind == K_PR_REF || pr->kind == K_PR_STOCH || is_int_kind(pr->kind));
+    if (pr->batch || pr->seg || pr->plan->segmented || pr->g->N == 0) return false;
+    if (is_weighted_kind(pr->kind)) return pr->elem == 4;  // FP32 contributions and weights
+    return pr->kind == K_PR_REF || pr->kind == K_PR_STOCH || is_int_kind(pr->kind);
+}
+
+// The plan's per-edge weights a weighted kind multiplies its values by.
+static const float *kind_weights(const gbx_pr *pr)
+{
+    switch (pr->kind) {
+    case K_PR_STOCH_W: return pr->plan->wn.as<float>();  // row-normalised transition weights
+    case K_ADJ: return pr->plan->wr.as<float>();         // P itself, forward-plan rows
+    case K_SPMV_W: return pr->plan->wt.as<float>();      // raw weights
+    default: return nullptr;
+    }
+}
+
+__global__ void gather_edge_weights(const uint32_t *__restrict__ eidx, uint32_t n, const float *__restrict__ w,
+                                    float *__restrict__ out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t e = eidx[i];
+        out[i] = e == kNoEdge ? 0.0f : w[e];
+    }
+}
+
+// The weights of a layout's src_local entries and hot pairs taken from the
+// plan's weight array w (built once per layout and array).
+static int layout_weights(gbx_ctx *ctx, gbx_bplan *bp, const float *w, const gbx_bplan::EdgeWeights **out)
+{
+    for (auto &lw : bp->weights)
+        if (lw->src == w) {
+            *out = lw.get();
+            return GBX_OK;
+        }
+    if (!bp->runs || (bp->H && !bp->hot16))
+        return set_error(ctx, GBX_ERR_STATE, "binned layout: weighted kinds need the run layout");
+    auto lw = std::make_unique<gbx_bplan::EdgeWeights>();
+    lw->src = w;
+    GBX_TRY(dev_alloc(ctx, lw->cold, (size_t)bp->Plen * 4 + 64));
+    GBX_TRY(dev_alloc(ctx, lw->hot, (size_t)bp->Hlen * 4 + 64));
+    if (bp->Plen) {
+        gather_edge_weights<<<grid_for(bp->Plen, 256, 16384), 256, 0, ctx->stream>>>(
+            bp->cold_eidx.as<uint32_t>(), bp->Plen, w, lw->cold.as<float>());
+        GBX_HIP(ctx, hipGetLastError());
+    }
+    if (bp->Hlen) {
+        gather_edge_weights<<<grid_for(bp->Hlen, 256, 16384), 256, 0, ctx->stream>>>(
+            bp->hot_eidx.as<uint32_t>(), bp->Hlen, w, lw->hot.as<float>());
+        GBX_HIP(ctx, hipGetLastError());
+    }
+    GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *out = lw.get();
+    bp->weights.push_back(std::move(lw));
+    return GBX_OK;
 }
 
 // The scatter moves values without looking at them: one instance per width.
@@ -1680,7 +1805,8 @@ template <int KIND, class T>
 static void grant_kind(const gbx_binned_state *st)
 {
     if constexpr (KIND == K_PR_REF || KIND == K_PR_STOCH) grant_lds(&bin_scatter_runs_kernel<T>, st->shm_scatter);
-    grant_lds(&bin_scatter_kernel<UintOf<sizeof(T)>, scatter_qk<KIND>()>, st->shm_scatter);
+    if constexpr (is_weighted_kind(KIND)) grant_lds(&bin_scatter_runs_kernel<T, true>, st->shm_scatter);
+    else grant_lds(&bin_scatter_kernel<UintOf<sizeof(T)>, scatter_qk<KIND>()>, st->shm_scatter);
     grant_lds(&bin_accum_kernel<KIND, T>, st->shm_accum);
 }
 
@@ -1702,11 +1828,13 @@ int binned_create(gbx_pr *pr)
     gbx_plan *p = pr->plan;
     const gbx_tuning &tu = ctx->tune;
     // The float kinds sum the cold edges of one chunk into one row in the
-    // scatter (run layout; segments padded to 8).  The integer kinds keep
-    // one V value per edge: segments padded to 32 values (a multiple of 32 is
-    // required by the per-block V positions), whole 128-byte lines of FP32
-    // V, 64-byte halves of 16-bit values (measured as fast as whole lines).
-    const bool runs = (pr->kind == K_PR_REF || pr->kind == K_PR_STOCH) && tu.binned_edge_values == 0;
+    // scatter (run layout; segments padded to 8; the weighted kinds always).
+    // The integer kinds keep one V value per edge: segments padded to 32
+    // values (a multiple of 32 is required by the per-block V positions),
+    // whole 128-byte lines of FP32 V, 64-byte halves of 16-bit values
+    // (measured as fast as whole lines).
+    const bool runs = is_weighted_kind(pr->kind) ||
+                      ((pr->kind == K_PR_REF || pr->kind == K_PR_STOCH) && tu.binned_edge_values == 0);
     const uint32_t pad = runs ? 8 : 32;
     // hot sources: the tuning value for contributions of up to 4 bytes (one
     // layout for FP32, BF16 and the 32/16/8-bit fixed-point kinds), half as
@@ -1737,6 +1865,14 @@ int binned_create(gbx_pr *pr)
     if (!st) return set_error(ctx, GBX_ERR_NOMEM, "binned state: host memory");
     pr->bin = st;
     st->bp = bp;
+    if (is_weighted_kind(pr->kind)) {
+        const float *w = kind_weights(pr);
+        if (!w) return set_error(ctx, GBX_ERR_STATE, "binned: the plan has no weights for this kind");
+        const gbx_bplan::EdgeWeights *lw = nullptr;
+        GBX_TRY(layout_weights(ctx, bp.get(), w, &lw));
+        st->wcold = lw->cold.as<float>();
+        st->whot = lw->hot.as<float>();
+    }
     // run layout: the dummy areas of the scatter's idle store lanes follow V
     const size_t vbytes = ((size_t)bp->Vlen * vel + 64 + kRunDummyStride - 1) / kRunDummyStride * kRunDummyStride;
     if (bp->runs && vbytes + (size_t)kRunDummyAreas * kRunDummyStride > 0xFFFFFFFFull)
@@ -1764,6 +1900,9 @@ int binned_create(gbx_pr *pr)
     switch (pr->kind) {
     case K_PR_REF: pr->elem == 4 ? grant_kind<K_PR_REF, float>(st) : grant_kind<K_PR_REF, uint16_t>(st); break;
     case K_PR_STOCH: pr->elem == 4 ? grant_kind<K_PR_STOCH, float>(st) : grant_kind<K_PR_STOCH, uint16_t>(st); break;
+    case K_PR_STOCH_W: grant_kind<K_PR_STOCH_W, float>(st); break;
+    case K_ADJ: grant_kind<K_ADJ, float>(st); break;
+    case K_SPMV_W: grant_kind<K_SPMV_W, float>(st); break;
     case K_FIX64: grant_kind<K_FIX64, uint64_t>(st); break;
     case K_FIX32: grant_kind<K_FIX32, uint32_t>(st); break;
     case K_FIX16: grant_kind<K_FIX16, uint16_t>(st); break;
@@ -1788,7 +1927,7 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
     S.src_local = bp->src_local.as<uint16_t>();
     S.bdst = bp->bdst.as<uint32_t>();
     S.NC = bp->NC;
-    if constexpr (KIND == K_PR_REF || KIND == K_PR_STOCH) {
+    if constexpr (KIND == K_PR_REF || KIND == K_PR_STOCH || is_weighted_kind(KIND)) {
         if (bp->runs) {
             RunsView RV;
             RV.chunk_lo = S.chunk_lo;
@@ -1800,15 +1939,18 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
             RV.pad = bp->SC;
             RV.stage_off = st->stage_off;
             RV.dummy = st->dummy_off;
+            RV.wcold = st->wcold;
             if (bp->NC)
-                bin_scatter_runs_kernel<T><<<bp->NC, kRunsBlock, st->shm_scatter, stream>>>(
+                bin_scatter_runs_kernel<T, is_weighted_kind(KIND)><<<bp->NC, kRunsBlock, st->shm_scatter, stream>>>(
                     RV, A.slots, A.t, reinterpret_cast<const T *>(A.c_in), st->V.as<T>());
             S.NC = 0;
         }
     }
-    if (S.NC)
-        bin_scatter_kernel<U, scatter_qk<KIND>()><<<bp->NC, kBlock, st->shm_scatter, stream>>>(
-            S, A.slots, A.t, A.c_in, st->V.as<U>());
+    if constexpr (!is_weighted_kind(KIND)) {
+        if (S.NC)
+            bin_scatter_kernel<U, scatter_qk<KIND>()><<<bp->NC, kBlock, st->shm_scatter, stream>>>(
+                S, A.slots, A.t, A.c_in, st->V.as<U>());
+    }
     AccumView B;
     B.units = bp->units.as<BinUnit>();
     B.dst_local = bp->dst_local.as<uint16_t>();
@@ -1820,6 +1962,7 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
     B.cnt = st->cnt.as<uint32_t>();
     B.work = st->work.as<uint32_t>();
     B.NU = bp->NU;
+    B.whot = st->whot;
     bin_accum_kernel<KIND, T><<<st->accum_grid, kBlock, st->shm_accum, stream>>>(
         B, pr->plan->outdeg.as<uint32_t>(), pr->plan->N, st->V.as<T>(), A);
 }
@@ -1847,6 +1990,14 @@ int binned_launch_iter(gbx_pr *pr)
     case K_PR_STOCH:
         if (pr->elem == 4) launch_binned<K_PR_STOCH, float>(pr, A);
         else launch_binned<K_PR_STOCH, uint16_t>(pr, A);
+        break;
+    case K_PR_STOCH_W: launch_binned<K_PR_STOCH_W, float>(pr, A); break;
+    case K_ADJ: launch_binned<K_ADJ, float>(pr, A); break;
+    case K_SPMV_W:
+        // the input vector stays in c[0]: every pass reads it
+        A.c_in = pr->c[0].p;
+        A.c_out = pr->c[1].p;
+        launch_binned<K_SPMV_W, float>(pr, A);
         break;
     case K_FIX64: launch_binned<K_FIX64, uint64_t>(pr, A); break;
     case K_FIX32: launch_binned<K_FIX32, uint32_t>(pr, A); break;

@@ -148,6 +148,27 @@ struct PrEpi {
     }
 };
 
+// SpMV on the binned iteration: y[i] += the row sum of w x (SPMV.c:171-189;
+// y accumulates over the passes).  y is the state's rank array.
+struct SpmvBinEpi {
+    float *__restrict__ y;
+    Partial part;
+    struct In {
+        float y;
+    };
+    __device__ __forceinline__ In load(uint32_t i) const { return In{ld_stream(y + i)}; }
+    __device__ __forceinline__ void apply(uint32_t i, float acc, const In &x) { st_stream(y + i, x.y + acc); }
+    __device__ __forceinline__ void operator()(uint32_t i, float acc) { apply(i, acc, load(i)); }
+};
+
+__device__ __forceinline__ SpmvBinEpi make_spmv_epi(const IterView &A)
+{
+    SpmvBinEpi epi;
+    epi.y = A.r;
+    epi.part = {0.0, 0.0, 0.0, 0u, 0u};
+    return epi;
+}
+
 // ------------------------------------------------------------------ fixed point
 // The conversions of fixedPoint.h as pageRank.c:496-1030 applies them (REF
 // update).  c = r / outdeg in float; to fixed: 64-bit (u64)((double)c * 2^32

@@ -51,6 +51,8 @@ constexpr bool is_fixed_kind(int k) { return k >= K_FIX64 && k <= K_FIX8; }
 constexpr bool is_quant_kind(int k) { return k >= K_Q32 && k <= K_Q8; }
 // kinds whose binned row sums are the reference's own uint64 integer sums
 constexpr bool is_int_kind(int k) { return is_fixed_kind(k) || is_quant_kind(k); }
+// kinds whose gathered values are edge weight times source value
+constexpr bool is_weighted_kind(int k) { return k == K_PR_STOCH_W || k == K_ADJ || k == K_SPMV_W; }
 
 struct ClassDesc {
     uint32_t lanes;       // lanes per row (1..64); 256 = hub chunks

@@ -988,7 +988,9 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
         *out = pr;
         return GBX_OK;
     }
-    if (kind_override == K_ADJ) {
+    if (kind_override == K_SPMV_W) {
+        if (!plan->has_wt) return set_error(ctx, GBX_ERR_STATE, "weighted SpMV needs a weighted graph");
+    } else if (kind_override == K_ADJ) {
         GBX_TRY(ensure_row_normalised(ctx, g, plan));
     } else if (p->weighted) {
         if (p->precision == GBX_BF16) GBX_TRY(ensure_normalised_bf16(ctx, g, plan));
@@ -1031,11 +1033,11 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
         bool binned = false;
         if (p->method == GBX_METHOD_BINNED) {
             if (!binned_supported(pr))
-                rc = set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: the binned method runs REF or unweighted "
-                                                 "STOCHASTIC PageRank on one graph");
+                rc = set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: the binned method runs REF, STOCHASTIC (FP32 or "
+                                                 "BF16) and weighted STOCHASTIC (FP32) PageRank on one graph");
             binned = true;
         } else if (p->method == GBX_METHOD_AUTO) {
-            binned = kind_override < 0 && binned_supported(pr) &&
+            binned = (kind_override < 0 || kind_override == K_ADJ) && binned_supported(pr) &&
                      (fixed || quant || g->N >= ctx->tune.binned_min_vertices);
         } else if (p->method != GBX_METHOD_PULL) {
             rc = set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: method %d", p->method);
@@ -1403,6 +1405,102 @@ int gbx_pr_plan_info(const gbx_pr *pr, uint32_t *n_classes, uint32_t *class_lane
     return GBX_OK;
 }
 
+// Sum over the plan's edges of |w[j] x[col[j]]| in FP64, in a fixed order:
+// per-block partials, then one block.  Bounds every row sum of a weighted
+// SpMV pass (the scale of the binned iteration's fixed-point row sums).
+__global__ void __launch_bounds__(256) spmv_bound_partial(const uint32_t *__restrict__ col,
+                                                          const float *__restrict__ w, uint32_t E,
+                                                          const float *__restrict__ x, double *__restrict__ part)
+{
+    double s = 0.0;
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < E; j += gridDim.x * 256)
+        s += fabs((double)w[j] * (double)x[col[j]]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    __shared__ double ws[4];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+}
+
+__global__ void __launch_bounds__(256) spmv_bound_final(const double *__restrict__ part, uint32_t n, Slot *slot)
+{
+    double s = 0.0;
+    for (uint32_t b = threadIdx.x; b < n; b += 256) s += part[b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    __shared__ double ws[4];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Slot z = {};
+        z.sum = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+        *slot = z;
+    }
+}
+
+// Weighted SpMV on the binned (propagation-blocking) iteration: a gbx_pr of
+// kind K_SPMV_W whose input vector stays in c[0] and whose rank array is y.
+static int spmv_binned(gbx_ctx *ctx, gbx_graph *g, uint32_t iterations, int direction, const float *x_in,
+                       float *x_out, float *y_out, double *seconds)
+{
+    gbx_pr_params q;
+    gbx_pr_params_default(&q);
+    q.mode = GBX_PR_STOCHASTIC;
+    q.weighted = 1;
+    q.stop = GBX_STOP_NONE;
+    q.direction = direction;
+    q.method = GBX_METHOD_BINNED;
+    q.max_iters = iterations;
+    gbx_pr *pr = nullptr;
+    GBX_TRY(pr_create_kind(ctx, g, &q, K_SPMV_W, &pr));
+    const gbx_plan *plan = pr->plan;
+    const uint32_t N = g->N, E = g->E;
+    const unsigned gN = grid_for(N, 256, 16384);
+    hipStream_t st = ctx->stream;
+    int rc = GBX_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DevBuf part;
+    const unsigned gb = std::max(1u, std::min(grid_for(E, 256, 4096), 4096u));
+    rc = dev_alloc(ctx, part, (size_t)gb * 8);
+    if (!rc) {
+        spmv_init_x<<<gN, 256, 0, st>>>(plan->outdeg.as<uint32_t>(), N, pr->c[0].as<float>(), pr->r.as<float>());
+        if (x_in && (hipMemcpyAsync(pr->out.p, x_in, (size_t)N * 4, hipMemcpyHostToDevice, st) != hipSuccess))
+            rc = set_error(ctx, GBX_ERR_HIP, "gbx_spmv_pull: input copy");
+        if (!rc && x_in)
+            gather_perm<<<gN, 256, 0, st>>>(pr->out.as<float>(), plan->perm.as<uint32_t>(), N, pr->c[0].as<float>());
+        spmv_bound_partial<<<gb, 256, 0, st>>>(plan->col.as<uint32_t>(), plan->wt.as<float>(), E,
+                                               pr->c[0].as<float>(), part.as<double>());
+        spmv_bound_final<<<1, 256, 0, st>>>(part.as<double>(), gb, pr->slots.as<Slot>());
+        if (hipGetLastError() != hipSuccess) rc = set_error(ctx, GBX_ERR_HIP, "gbx_spmv_pull: set-up kernels");
+    }
+    pr->t = 0;
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = set_error(ctx, GBX_ERR_HIP, "gbx_spmv_pull: events");
+    if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = set_error(ctx, GBX_ERR_HIP, "gbx_spmv_pull: event");
+    for (uint32_t it = 0; it < iterations && !rc; it++) rc = binned_launch_iter(pr);
+    if (!rc && hipEventRecord(e1, st) != hipSuccess) rc = set_error(ctx, GBX_ERR_HIP, "gbx_spmv_pull: event");
+    if (!rc && y_out) {
+        scatter_perm<<<gN, 256, 0, st>>>(pr->r.as<float>(), plan->perm.as<uint32_t>(), N, pr->out.as<float>());
+        if (hipMemcpyAsync(y_out, pr->out.p, (size_t)N * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = set_error(ctx, GBX_ERR_HIP, "gbx_spmv_pull: output copy");
+    }
+    if (!rc && x_out) {
+        scatter_perm<<<gN, 256, 0, st>>>(pr->c[0].as<float>(), plan->perm.as<uint32_t>(), N, pr->out.as<float>());
+        if (hipMemcpyAsync(x_out, pr->out.p, (size_t)N * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+            rc = set_error(ctx, GBX_ERR_HIP, "gbx_spmv_pull: output copy");
+    }
+    if (!rc && hipStreamSynchronize(st) != hipSuccess)
+        rc = set_error(ctx, GBX_ERR_HIP, "gbx_spmv_pull: %s", hipGetErrorString(hipGetLastError()));
+    float ms = 0.0f;
+    if (!rc && seconds && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) *seconds = ms * 1e-3;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    gbx_pr_free(pr);
+    return rc;
+}
+
 int gbx_spmv_pull(gbx_ctx *ctx, gbx_graph *g, uint32_t iterations, int weighted, int direction,
                   const float *x_in, float *x_out, float *y_out, double *seconds)
 {
@@ -1413,6 +1511,9 @@ int gbx_spmv_pull(gbx_ctx *ctx, gbx_graph *g, uint32_t iterations, int weighted,
     if (plan->segmented) return set_error(ctx, GBX_ERR_STATE, "gbx_spmv_pull: not available for a batch of graphs");
     if (weighted && !plan->has_wt)
         return set_error(ctx, GBX_ERR_STATE, "gbx_spmv_pull: weighted SpMV needs a weighted graph");
+    // large weighted graphs: the binned iteration, as GBX_METHOD_AUTO picks for PageRank
+    if (weighted && g->N >= ctx->tune.binned_min_vertices && g->E)
+        return spmv_binned(ctx, g, iterations, direction, x_in, x_out, y_out, seconds);
     const uint32_t N = g->N;
     DevBuf x, y, tmp;
     GBX_TRY(dev_alloc(ctx, x, (size_t)N * 4));

@@ -200,10 +200,15 @@ def test_binned_auto_selection_and_errors(golden):
         assert l1(binned, pull) <= 1e-6
         w = np.ones(len(z["edges_src"]), np.float32)
         gw = ctx.graph(z["edges_src"], z["edges_dst"], w, sort_mode=1, flags=gbx.GRAPH_WEIGHTED)
+        # weighted stochastic runs on the binned method too (FP32); BF16 weighted on the pull method only
+        rw_b, _ = gw.pagerank(iterations=5, mode=gbx.PR_STOCHASTIC, weighted=True, method=BINNED)
+        rw_p, _ = gw.pagerank(iterations=5, mode=gbx.PR_STOCHASTIC, weighted=True, method=gbx.METHOD_PULL)
+        rw_a, _ = gw.pagerank(iterations=5, mode=gbx.PR_STOCHASTIC, weighted=True)  # AUTO from 1 vertex: binned
+        assert l1(rw_b, rw_p) <= 1e-6
+        np.testing.assert_array_equal(rw_a, rw_b)
         with pytest.raises(gbx.GbxError, match="binned"):
-            gw.pagerank(iterations=5, mode=gbx.PR_STOCHASTIC, weighted=True, method=BINNED)
-        # weighted under AUTO runs the pull method
-        gw.pagerank(iterations=5, mode=gbx.PR_STOCHASTIC, weighted=True)
+            gw.pagerank(iterations=5, mode=gbx.PR_STOCHASTIC, weighted=True, precision=gbx.BF16, method=BINNED)
+        gw.pagerank(iterations=5, mode=gbx.PR_STOCHASTIC, weighted=True, precision=gbx.BF16)  # AUTO: pull
     finally:
         ctx.close()
 
