@@ -1,19 +1,23 @@
-"""Benchmark: FP32 pull PageRank on R-MAT scale 24 on one MI355X.
+"""Benchmark: pull PageRank on R-MAT scale 24 on one MI355X, and the other
+workloads of the path beside it.
 
 A step is one PageRank iteration (reference semantics, pageRank.c) over the
 whole device-resident graph.  The headline runs the library's default method
-for this size, the binned (propagation-blocking) iteration: a scatter kernel
+for this size, the binned (propagation-blocking) iteration -- a scatter kernel
 writes every edge's source contribution into destination-range bins and an
-accumulate kernel sums the bins in LDS and applies the update, with the same
-ranks as the pull iteration (within 1e-5 L1).  The plain pull kernel (one
-gather per edge) is reported beside it, with the BF16 and 32-vector forms.
+accumulate kernel sums the bins in LDS and applies the update -- with its
+layout cut chosen by the library's autotuner on this graph.
 
 Prints one JSON line: GTEPS (E edges per iteration / wall time per
 iteration), the HBM roofline of the iteration (algorithmic bytes of the pull
-formulation, 8E + 16N + 4, over the device time per iteration between the
-HIP events that bracket the timed iterations on the library's stream) and
-the GraphBlox reference's OpenMP pull PageRank timed on this host's cores on
-the same graph.
+formulation, 8E + 16N + 4, over the device time per iteration between the HIP
+events that bracket the timed iterations on the library's stream), the error
+of its ranks against the FP64 oracle, the oracle's OpenMP port of
+pageRankPullGraphCSR timed on this host's cores on the same graph, wall time
+to convergence, and `forms`: the pull kernel, BF16, the 32-vector batched
+form, weighted PageRank and weighted SpMV on R-MAT s22, stochastic PageRank
+to L1 < 1e-6 on s22, s16 beside the CPU port, the batch of 256 graphs and the
+adjoint on s20 -- each with its error against FP64 (or the port).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--scale 24]
 """
@@ -128,20 +132,24 @@ def cpu_baseline(fo, io, iterations: int, threads: int, gpu_ranks=None):
                wall_s=round(wall, 3), logical_cpus=logical, allowed_cpus=allowed, physical_cores=physical)
     if gpu_ranks is not None:
         rec["l1_gpu_vs_port"] = float(np.abs(np.asarray(gpu_ranks, np.float64) - out["ranks"]).sum())
-    return rec
+    return rec, out["ranks"]
 
 
-def timed_steps(pr, steps):
+def timed_steps(pr, steps, graph_replay=False):
     """The timed region, after a warm-up: reset the session (its start event
     is recorded at the next launch), launch `steps` iterations back to back
-    as a user runs them.  Returns (device ms per iteration between the HIP
-    events the library records on its stream before the first and after the
-    last launch, host wall ms per iteration, ranks, report)."""
+    as a user runs them (or as one captured HIP graph).  Returns (device ms
+    per iteration between the HIP events the library records on its stream
+    before the first and after the last launch, host wall ms per iteration,
+    ranks, report)."""
     ctx = pr.ctx
     pr.reset()
     ctx.sync()
     t0 = time.perf_counter()
-    pr.launch(steps)
+    if graph_replay:
+        pr.launch_graph(steps)
+    else:
+        pr.launch(steps)
     ctx.sync()
     wall_ms = (time.perf_counter() - t0) * 1e3 / steps
     ranks, rep = pr.read()
@@ -149,22 +157,31 @@ def timed_steps(pr, steps):
     return rep0["time_total_s"] * 1e3 / steps, wall_ms, ranks, rep
 
 
+def time_session(pr, warmup, steps, graph_replay=False):
+    pr.reset()
+    if graph_replay:
+        pr.launch_graph(steps)  # captured here, replayed in the timed region
+    else:
+        pr.launch(warmup)
+    return timed_steps(pr, steps, graph_replay)
+
+
 def time_form(g, warmup, steps, **kw):
     """Warm up, then time `steps` iterations (timed_steps).  Returns
-    (kernel_ms, wall_ms, session)."""
+    (kernel_ms, wall_ms, ranks, session)."""
+    import graphblox_amd as gbx
+
     k = kw.pop("num_vectors", 0)
     seeds = kw.pop("seed_sets", None)
     if k:
-        pr = g.ppr_session(k, seeds, iterations=warmup + steps, epsilon=0.0, stop=2)
+        pr = g.ppr_session(k, seeds, iterations=warmup + steps, epsilon=0.0, stop=gbx.STOP_NONE)
     else:
-        pr = g.session(iterations=warmup + steps, epsilon=0.0, stop=2, **kw)
-    pr.reset()
-    pr.launch(warmup)
-    kernel_ms, wall_ms, _, _ = timed_steps(pr, steps)
-    return kernel_ms, wall_ms, pr
+        pr = g.session(iterations=warmup + steps, epsilon=0.0, stop=gbx.STOP_NONE, **kw)
+    kernel_ms, wall_ms, ranks, _ = time_session(pr, warmup, steps)
+    return kernel_ms, wall_ms, ranks, pr
 
 
-def form_record(name, E, N, k, kernel_ms, wall_ms, bytes_per_iter, base_gteps=None):
+def form_record(name, E, N, k, kernel_ms, wall_ms, bytes_per_iter, base_gteps=None, **extra):
     gteps = k * E / (wall_ms * 1e-3) / 1e9
     rec = {"form": name, "num_vectors": k, "num_vertices": N, "num_edges": E,
            "ms_per_iter": round(wall_ms, 5), "kernel_ms": round(kernel_ms, 5),
@@ -172,6 +189,8 @@ def form_record(name, E, N, k, kernel_ms, wall_ms, bytes_per_iter, base_gteps=No
            "hbm_frac": round(bytes_per_iter / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if base_gteps:
         rec["speedup_vs_fp32_same_graph"] = round(gteps / base_gteps, 3)
+    rec.update(extra)
+    log(f"{name}: {rec}")
     return rec
 
 
@@ -189,6 +208,259 @@ def calibrate_copy(gib: float):
     print(json.dumps({"calibrate_copy_bytes_read": n * 4, "calibrate_copy_bytes_written": n * 4}))
 
 
+# ---------------------------------------------------------------- forms
+
+
+def forms_s24(gbx, O, g, r64_ref, args, headline_gteps, method):
+    """The pull kernel (or the binned iteration) and BF16 with both methods on
+    the headline graph, after the same iterations as the headline."""
+    N, E = g.N, g.E
+    out = {}
+    other = "pull" if method == "binned" else "binned"
+    other_m = gbx.METHOD_PULL if other == "pull" else gbx.METHOD_BINNED
+    km, wm, r1, s1 = time_form(g, args.warmup, args.steps, mode=gbx.PR_REF, method=other_m)
+    s1.free()
+    base_pull = E / (wm * 1e-3) / 1e9 if other == "pull" else headline_gteps
+    out[f"{other}_f32"] = form_record(f"FP32 {other} iteration, same graph", E, N, 1, km, wm, 8 * E + 16 * N + 4,
+                                      **fp64_errors(r1, r64_ref))
+    for mname, m in (("binned", gbx.METHOD_BINNED), ("pull", gbx.METHOD_PULL)):
+        km, wm, r16, s16 = time_form(g, args.warmup, args.steps, mode=gbx.PR_REF, precision=gbx.BF16, method=m)
+        s16.free()
+        rec = form_record(f"BF16 contributions, {mname} iteration, same graph", E, N, 1, km, wm,
+                          6 * E + 18 * N + 4, headline_gteps if mname == "binned" else base_pull,
+                          **fp64_errors(r16, r64_ref))
+        rec["speedup_vs_plain_pull_fp32"] = round(rec["gteps"] / base_pull, 3)
+        out[f"bf16_{mname}"] = rec
+    return out
+
+
+def forms_s22(gbx, O, ctx, args):
+    """R-MAT s22: plain stochastic PageRank (pull, binned), the 32-vector
+    batched form, stochastic PageRank to L1 < 1e-6, and uniform(0,1)-weighted
+    PageRank and SpMV."""
+    out = {}
+    gb = ctx.rmat_graph(min(args.scale, 22), args.edge_factor, SEED)
+    Nb, Eb = gb.N, gb.E
+    fo, io = csr_for_oracle(gb)
+    rp = O.row_ptr_from(io)
+    r64s, _, _ = O.pr_f64(Nb, fo.out_degree, rp, io.dst, mode=1, iterations=args.steps)
+    km, wm, rr, s = time_form(gb, args.warmup, args.steps, mode=gbx.PR_STOCHASTIC, method=gbx.METHOD_PULL)
+    s.free()
+    base32 = Eb / (wm * 1e-3) / 1e9
+    out["fp32_pull_same_graph_as_batched"] = form_record("plain FP32 pull (stochastic)", Eb, Nb, 1, km, wm,
+                                                         8 * Eb + 16 * Nb + 4, **fp64_errors(rr, r64s))
+    km, wm, rr, s = time_form(gb, args.warmup, args.steps, mode=gbx.PR_STOCHASTIC, method=gbx.METHOD_BINNED)
+    s.free()
+    out["fp32_binned_same_graph_as_batched"] = form_record("FP32 binned (stochastic)", Eb, Nb, 1, km, wm,
+                                                           8 * Eb + 16 * Nb + 4, **fp64_errors(rr, r64s))
+    rng = np.random.default_rng(SEED)
+    sets = [rng.choice(Nb, int(rng.integers(1, 1001)), replace=False) for _ in range(32)]
+    kb, wb, rb, sb = time_form(gb, args.warmup, args.steps, num_vectors=32, seed_sets=sets)
+    sb.free()
+    errs = []
+    for q in range(0, 32, 8):
+        t = np.zeros(Nb)
+        t[sets[q]] = 1.0 / len(sets[q])
+        r64q, _, _ = O.pr_f64(Nb, fo.out_degree, rp, io.dst, mode=1, iterations=args.steps, teleport=t)
+        errs.append(fp64_errors(rb[:, q], r64q))
+    out["batched_ppr32"] = form_record(
+        "32 personalised vectors, one multi-vector launch", Eb, Nb, 32, kb, wb,
+        4 * (Nb + 1) + 4 * Nb + 4 * Eb + 32 * (4 * Eb + 8 * Nb), base32,
+        l1_vs_fp64=max(e["l1_vs_fp64"] for e in errs), max_rel_vs_fp64=max(e["max_rel_vs_fp64"] for e in errs),
+        fp64_checked_vectors="4 of 32 (0, 8, 16, 24)")
+    out["batched_ppr32"]["speedup_vs_binned_fp32_same_graph"] = round(
+        out["batched_ppr32"]["gteps"] / out["fp32_binned_same_graph_as_batched"]["gteps"], 3)
+    # stochastic PageRank to L1 < 1e-6 (at most 100 iterations), wall time from reset to read-back
+    conv = gb.session(iterations=100, epsilon=1e-6, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_L1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    conv.reset()
+    conv.launch(100)
+    rc, repc = conv.read()
+    wall = (time.perf_counter() - t0) * 1e3
+    conv.free()
+    r64c, it64, _ = O.pr_f64(Nb, fo.out_degree, rp, io.dst, mode=1, iterations=100, epsilon=1e-6)
+    out["stochastic_to_l1_1e-6"] = {"form": "stochastic PageRank to L1 < 1e-6, binned", "num_vertices": Nb,
+                                    "num_edges": Eb, "iterations": repc["executed"], "fp64_iterations": it64,
+                                    "wall_ms": round(wall, 3), "rank_sum": repc["rank_sum"], **fp64_errors(rc, r64c)}
+    log(f"s22 to L1<1e-6: {out['stochastic_to_l1_1e-6']}")
+    gb.free()
+    # uniform(0,1) weights: weighted stochastic PageRank and weighted SpMV, both methods
+    gw = ctx.rmat_graph(min(args.scale, 22), args.edge_factor, SEED, weight_kind=1, weight_seed=SEED,
+                        flags=gbx.GRAPH_WEIGHTED)
+    Nw, Ew = gw.N, gw.E
+    fo, io = csr_for_oracle(gw)
+    rp = O.row_ptr_from(io)
+    S = np.bincount(fo.src, weights=fo.w.astype(np.float64), minlength=Nw)
+    wn = np.where(S[io.dst] > 0, io.w.astype(np.float64) / np.maximum(S[io.dst], 1e-300), 0.0)
+    r64w, _, _ = O.pr_f64(Nw, fo.out_degree, rp, io.dst, mode=1, iterations=args.steps, weights=wn)
+    for mname, m in (("binned", gbx.METHOD_BINNED), ("pull", gbx.METHOD_PULL)):
+        km, wm, rw, s = time_form(gw, args.warmup, args.steps, mode=gbx.PR_STOCHASTIC, weighted=True, method=m)
+        s.free()
+        out[f"weighted_pr_{mname}"] = form_record(f"weighted stochastic PageRank, uniform(0,1) weights, {mname}",
+                                                  Ew, Nw, 1, km, wm, 12 * Ew + 16 * Nw + 4, **fp64_errors(rw, r64w))
+    xo, yo = O.spmv_pull_f32(fo, io, iterations=1, weights=io.w, nthreads=0)
+    tune0 = ctx.tuning()
+    for mname, thr in (("binned", 0), ("pull", 1 << 30)):
+        t = ctx.tuning()
+        t.binned_min_vertices = thr  # gbx_spmv_pull takes the binned iteration from this many vertices
+        ctx.set_tuning(t)
+        gw.spmv(iterations=args.warmup, weighted=True)
+        _, y, sec = gw.spmv(iterations=args.steps, weighted=True)
+        ms = sec * 1e3 / args.steps
+        _, y1, _ = gw.spmv(iterations=1, weighted=True)
+        rel = float(np.abs(y1.astype(np.float64) - yo).sum() / np.abs(yo.astype(np.float64)).sum())
+        out[f"weighted_spmv_{mname}"] = form_record(f"weighted SpMV y = A^T x, uniform(0,1) weights, {mname}",
+                                                    Ew, Nw, 1, ms, ms, 12 * Ew + 8 * Nw + 4,
+                                                    rel_l1_vs_port=rel, timing="device events around the passes")
+    ctx.set_tuning(tune0)
+    gw.free()
+    return out
+
+
+def forms_s16(gbx, O, ctx, args):
+    """R-MAT s16 (65,536 vertices): the GPU beside the CPU port, 20 iterations."""
+    g = ctx.rmat_graph(16, args.edge_factor, SEED)
+    N, E = g.N, g.E
+    fo, io = csr_for_oracle(g)
+    km, wm, r, s = time_form(g, args.warmup, args.steps, mode=gbx.PR_REF)
+    s.free()
+    rec_cpu, port = cpu_baseline(fo, io, args.steps, baseline_threads(), r)
+    r64, _, _ = O.pr_f64(N, fo.out_degree, O.row_ptr_from(io), io.dst, mode=0, iterations=args.steps)
+    rec = form_record("R-MAT s16, reference semantics, GPU (default method) beside the CPU port", E, N, 1, km, wm,
+                      8 * E + 16 * N + 4, **fp64_errors(r, r64))
+    rec["cpu_port"] = rec_cpu
+    rec["gpu_vs_cpu_port_speedup"] = round(rec["gteps"] / rec_cpu["value"], 2)
+    ks, ws, rs, ss = time_form(g, args.warmup, args.steps, mode=gbx.PR_STOCHASTIC)
+    ss.free()
+    r64s, _, _ = O.pr_f64(N, fo.out_degree, O.row_ptr_from(io), io.dst, mode=1, iterations=args.steps)
+    rec["stochastic"] = form_record("R-MAT s16, stochastic (uniform dangling mass)", E, N, 1, ks, ws,
+                                    8 * E + 16 * N + 4, **fp64_errors(rs, r64s))
+    g.free()
+    return rec
+
+
+def forms_batch(gbx, O, ctx, args):
+    """256 independent R-MAT graphs, 2^10..2^18 vertices (log-uniform), edge
+    factor 8..32: every graph's stochastic PageRank in one launch per
+    iteration, the iterations replayed as one captured HIP graph, launch
+    shape autotuned on the batch."""
+    rng = np.random.default_rng(256)
+    scales = rng.integers(10, 19, 256)
+    efs = rng.integers(8, 33, 256)
+    members = [ctx.rmat_graph(int(s), int(f), 5000 + k) for k, (s, f) in enumerate(zip(scales, efs))]
+    b = ctx.graph_batch(members)
+    off = b.batch_offsets()
+    N, E = b.N, b.E
+    tune0 = ctx.tuning()
+    best, tuned_ms = ctx.autotune(b, 0, 5, iterations=10, epsilon=0.0, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_NONE)
+    s = b.session(iterations=args.steps, epsilon=0.0, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_NONE)
+    km, wm, ranks, reps = time_session(s, args.warmup, args.steps, graph_replay=True)
+    s.free()
+    errs = []
+    for k in (0, 85, 170, 255):
+        m = members[k]
+        fo, io = csr_for_oracle(m)
+        r64, _, _ = O.pr_f64(m.N, fo.out_degree, O.row_ptr_from(io), io.dst, mode=1, iterations=args.steps)
+        errs.append(fp64_errors(ranks[off[k]:off[k + 1]], r64))
+    # to convergence (each graph its own L1 < 1e-7 stop), one replayed graph of 100 iterations
+    c = b.session(iterations=100, epsilon=1e-7, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_L1)
+    c.reset()
+    c.launch_graph(100)
+    c.read()
+    ctx.sync()
+    t0 = time.perf_counter()
+    c.reset()
+    c.launch_graph(100)
+    _, creps = c.read()
+    conv_ms = (time.perf_counter() - t0) * 1e3
+    c.free()
+    rec = form_record("256 R-MAT graphs (2^10..2^18 vertices, edge factor 8..32), one launch per iteration, "
+                      "replayed as a HIP graph", E, N, 1, km, wm, 8 * E + 16 * N + 4 * 256,
+                      l1_vs_fp64=max(e["l1_vs_fp64"] for e in errs),
+                      max_rel_vs_fp64=max(e["max_rel_vs_fp64"] for e in errs),
+                      fp64_checked_graphs="4 of 256 (0, 85, 170, 255)",
+                      autotuned={"bin_thresholds": list(best.bin_thresholds), "hub_chunk": best.hub_chunk,
+                                 "ppr_blocks_per_cu": best.ppr_blocks_per_cu, "trial_ms": round(tuned_ms, 5)},
+                      convergence={"epsilon": 1e-7, "wall_ms": round(conv_ms, 3),
+                                   "iterations_max": max(r["executed"] for r in creps),
+                                   "all_converged": all(r["converged"] for r in creps)})
+    ctx.set_tuning(tune0)
+    b.free()
+    for m in members:
+        m.free()
+    return rec
+
+
+def forms_adjoint(gbx, O, ctx, args):
+    """Weighted PageRank on R-MAT s20 with lognormal(0,1) weights and its
+    adjoint (loss weights N(0,1)): forward and adjoint fixed points to
+    1e-7, then dL/dW; ms per iteration of each and wall time to the gradient,
+    both methods; the gradient against the FP64 adjoint."""
+    g = ctx.rmat_graph(20, args.edge_factor, SEED, weight_kind=2, weight_seed=SEED, flags=gbx.GRAPH_WEIGHTED)
+    N, E = g.N, g.E
+    gl = np.random.default_rng(SEED).standard_normal(N).astype(np.float32)
+    inv = g.export(inverse=True)
+    rp = np.zeros(N + 1, np.uint64)
+    np.cumsum(inv["out_degree"], out=rp[1:])
+    p, lam, gw64, _ = O.adjoint_f64(N, rp, inv["dst"], inv["w"].astype(np.float64), gl, iterations=2000, tol=1e-14)
+    f = g.export()
+    # the forward-CSR position of every inverse-CSR edge (u -> v is (src u, dst v) forward, (v, u) inverse)
+    key_fwd = f["src"].astype(np.uint64) * np.uint64(N) + f["dst"].astype(np.uint64)
+    key = inv["dst"].astype(np.uint64) * np.uint64(N) + inv["src"].astype(np.uint64)
+    order = np.argsort(key_fwd)
+    perm = order[np.searchsorted(key_fwd[order], key)]
+    assert np.array_equal(key_fwd[perm], key)
+    del key_fwd, key, order
+    out = {}
+    for mname, m in (("binned", gbx.METHOD_BINNED), ("pull", gbx.METHOD_PULL)):
+        g.adjoint(gl, iterations=300, epsilon=1e-7, method=m)  # plans, layouts and weights built
+        t0 = time.perf_counter()
+        o = g.adjoint(gl, iterations=300, epsilon=1e-7, method=m)
+        wall = (time.perf_counter() - t0) * 1e3
+        fr, ar = o["fwd_report"], o["adj_report"]
+        gw = o["grad_w"][perm].astype(np.float64)
+        out[mname] = {"forward_iterations": fr["executed"], "adjoint_iterations": ar["executed"],
+                      "ms_per_forward_iteration": round(fr["time_total_s"] * 1e3 / max(fr["executed"], 1), 5),
+                      "ms_per_adjoint_iteration": round(ar["time_total_s"] * 1e3 / max(ar["executed"], 1), 5),
+                      "wall_ms_to_gradient": round(wall, 3),
+                      "grad_w_rel_l2_vs_fp64": float(np.linalg.norm(gw - gw64) / np.linalg.norm(gw64)),
+                      "lambda_rel_l1_vs_fp64": float(np.abs(o["lam"] - lam).sum() / np.abs(lam).sum())}
+        log(f"adjoint s20 {mname}: {out[mname]}")
+    g.free()
+    return {"form": "adjoint of weighted stochastic PageRank, R-MAT s20, lognormal weights", "num_vertices": N,
+            "num_edges": E, "algorithmic_bytes_per_iteration": 12 * E + 16 * N + 4, **out}
+
+
+def convergence(gbx, ctx, args, g, method, m):
+    """Reference semantics, stop when no vertex changes by 1e-4 or more
+    (pageRank.c:327-339), at most 100 iterations; wall time from reset to
+    read-back of the ranks, on the graph whose plan and binned layout exist
+    (warm), and one-shot on a graph just built, which pays both (cold)."""
+    conv_s = g.session(iterations=100, epsilon=1e-4, mode=gbx.PR_REF, stop=gbx.STOP_ACTIVE, method=m)
+    ctx.sync()
+    t0 = time.perf_counter()
+    conv_s.reset()
+    conv_s.launch(100)
+    _, conv_rep = conv_s.read()
+    warm_ms = (time.perf_counter() - t0) * 1e3
+    conv_s.free()
+    g2 = ctx.rmat_graph(args.scale, args.edge_factor, SEED + 1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    _, rep2 = g2.pagerank(iterations=100, epsilon=1e-4, method=m)  # gbx_pr_pull: plan, layout, iterations, read
+    cold_ms = (time.perf_counter() - t0) * 1e3
+    g2.free()
+    rec = {"epsilon": 1e-4, "stop": "no vertex with |dr| >= epsilon", "iterations": conv_rep["iterations"],
+           "executed": conv_rep["executed"], "converged": bool(conv_rep["converged"]),
+           "wall_ms": round(warm_ms, 3),
+           "wall_ms_one_shot_with_plan_and_layout_build": round(cold_ms, 3),
+           "one_shot_graph": f"R-MAT s{args.scale} seed {SEED + 1} (same shape, a new graph object)",
+           "one_shot_iterations": rep2["iterations"]}
+    log(f"convergence ({method}): {rec}")
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,7 +471,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: OMP_NUM_THREADS, else every allowed CPU")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-forms", action="store_true", help="skip the BF16 and batched forms")
+    ap.add_argument("--no-forms", action="store_true", help="the headline only")
+    ap.add_argument("--no-autotune", action="store_true", help="the library's default layout cut")
     ap.add_argument("--method", choices=("binned", "pull"), default="binned",
                     help="iteration of the headline (binned: the library default at this size)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
@@ -214,6 +487,7 @@ def main():
         sys.exit(2)
 
     import graphblox_amd as gbx
+    from oracle import oracle as O
 
     ctx = gbx.Context(0)
     t0 = time.perf_counter()
@@ -221,13 +495,23 @@ def main():
     N, E = g.N, g.E
     log(f"R-MAT s{args.scale}: N={N} E={E} built on GPU in {time.perf_counter() - t0:.2f}s")
     method = gbx.METHOD_BINNED if args.method == "binned" else gbx.METHOD_PULL
+    tuned = None
+    tune_default = ctx.tuning()
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        best, best_ms = ctx.autotune(g, 0, 5, iterations=10, epsilon=0.0, mode=gbx.PR_REF, stop=gbx.STOP_NONE,
+                                     method=method)
+        d = best.as_dict()
+        tuned = {k: d[k] for k in d if (k.startswith("binned") if args.method == "binned" else
+                                        not k.startswith(("binned", "ppr")))}
+        tuned["trial_ms"] = round(best_ms, 5)
+        tuned["search_s"] = round(time.perf_counter() - t0, 2)
+        log(f"autotuned: {tuned}")
     t0 = time.perf_counter()
     pr = g.session(iterations=args.warmup + args.steps, epsilon=0.0, mode=gbx.PR_REF, stop=gbx.STOP_NONE,
                    method=method)
     log(f"{args.method} iteration set up in {time.perf_counter() - t0:.2f}s")
-    pr.reset()
-    pr.launch(args.warmup)
-    kernel_ms, ms_per_step, ranks, rep = timed_steps(pr, args.steps)
+    kernel_ms, ms_per_step, ranks, rep = time_session(pr, args.warmup, args.steps)
     assert rep["executed"] == args.steps
     assert np.isfinite(ranks).all()
     value = E / (ms_per_step * 1e-3) / 1e9
@@ -237,84 +521,28 @@ def main():
     traffic = traffic_from_profiles(wk)
     log(f"{ms_per_step:.4f} ms/iter wall, kernels {kernel_ms:.4f} ms, {value:.1f} GTEPS, "
         f"{achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of peak)")
-    forms = None
-    if not args.no_forms:
-        forms = {}
-        other = "pull" if args.method == "binned" else "binned"
-        other_m = gbx.METHOD_PULL if other == "pull" else gbx.METHOD_BINNED
-        km, wm, s1 = time_form(g, args.warmup, args.steps, mode=gbx.PR_REF, method=other_m)
-        r1, _ = s1.read()
-        s1.free()
-        base_pull = E / (wm * 1e-3) / 1e9 if other == "pull" else value
-        forms[f"{other}_f32"] = form_record(f"FP32 {other} iteration, same graph", E, N, 1, km, wm,
-                                            8 * E + 16 * N + 4)
-        forms[f"{other}_f32"]["l1_vs_headline"] = float(np.abs(r1.astype(np.float64) - ranks).sum())
-        log(f"{other} FP32 form: {forms[f'{other}_f32']}")
-        for mname, m in (("binned", gbx.METHOD_BINNED), ("pull", gbx.METHOD_PULL)):
-            km, wm, s16 = time_form(g, args.warmup, args.steps, mode=gbx.PR_REF, precision=gbx.BF16, method=m)
-            s16.free()
-            rec = form_record(f"BF16 contributions, {mname} iteration, same graph", E, N, 1, km, wm,
-                              6 * E + 18 * N + 4, value)
-            rec["speedup_vs_plain_pull_fp32"] = round(rec["gteps"] / base_pull, 3)
-            forms[f"bf16_{mname}"] = rec
-            log(f"BF16 {mname} form: {rec}")
-        # 32 personalised vectors on R-MAT scale 22 (seed sets of 1..1000 vertices)
-        gb = ctx.rmat_graph(min(args.scale, 22), args.edge_factor, SEED)
-        Nb, Eb = gb.N, gb.E
-        km32, wm32, s32 = time_form(gb, args.warmup, args.steps, mode=gbx.PR_STOCHASTIC, method=gbx.METHOD_PULL)
-        s32.free()
-        base32 = Eb / (wm32 * 1e-3) / 1e9
-        forms["fp32_pull_same_graph_as_batched"] = form_record("plain FP32 pull (stochastic)", Eb, Nb, 1, km32,
-                                                               wm32, 8 * Eb + 16 * Nb + 4)
-        km32b, wm32b, s32b = time_form(gb, args.warmup, args.steps, mode=gbx.PR_STOCHASTIC,
-                                       method=gbx.METHOD_BINNED)
-        s32b.free()
-        forms["fp32_binned_same_graph_as_batched"] = form_record("FP32 binned (stochastic)", Eb, Nb, 1, km32b,
-                                                                 wm32b, 8 * Eb + 16 * Nb + 4)
-        rng = np.random.default_rng(SEED)
-        sets = [rng.choice(Nb, int(rng.integers(1, 1001)), replace=False) for _ in range(32)]
-        kb, wb, sb = time_form(gb, args.warmup, args.steps, num_vectors=32, seed_sets=sets)
-        rb, repb = sb.read()
-        assert np.isfinite(rb).all()
-        sb.free()
-        forms["batched_ppr32"] = form_record("32 personalised vectors, one multi-vector launch", Eb, Nb, 32, kb, wb,
-                                             4 * (Nb + 1) + 4 * Nb + 4 * Eb + 32 * (4 * Eb + 8 * Nb), base32)
-        forms["batched_ppr32"]["speedup_vs_binned_fp32_same_graph"] = round(
-            forms["batched_ppr32"]["gteps"] / forms["fp32_binned_same_graph_as_batched"]["gteps"], 3)
-        log(f"batched form: {forms['batched_ppr32']}")
-        gb.free()
-    # wall time to convergence: reference semantics, stop when no vertex changes by 1e-4 or more
-    # (pageRank.c:327-339), at most 100 iterations; reset, iterations and read-back timed
-    conv_s = g.session(iterations=100, epsilon=1e-4, mode=gbx.PR_REF, stop=gbx.STOP_ACTIVE, method=method)
-    ctx.sync()
-    t0 = time.perf_counter()
-    conv_s.reset()
-    conv_s.launch(100)
-    _, conv_rep = conv_s.read()
-    conv_ms = (time.perf_counter() - t0) * 1e3
-    conv_s.free()
-    convergence = {"epsilon": 1e-4, "stop": "no vertex with |dr| >= epsilon", "iterations": conv_rep["iterations"],
-                   "executed": conv_rep["executed"], "converged": bool(conv_rep["converged"]),
-                   "wall_ms": round(conv_ms, 3)}
-    log(f"convergence: {convergence}")
+    fo, io = csr_for_oracle(g)
+    r64, _, _ = O.pr_f64(N, fo.out_degree, O.row_ptr_from(io), io.dst, mode=0, iterations=args.steps)
+    headline_err = fp64_errors(ranks, r64)
+    log(f"headline vs FP64: {headline_err}")
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as O
-
         log("CPU baseline ...")
-        cs = g.session(iterations=args.cpu_iters, epsilon=0.0, mode=gbx.PR_REF, stop=gbx.STOP_NONE, method=method)
-        cs.reset()
-        cs.launch(args.cpu_iters)
-        gpu_ranks, _ = cs.read()
-        cs.free()
-        fo, io = csr_for_oracle(g)
-        threads = args.cpu_threads or baseline_threads()
-        cpu = cpu_baseline(fo, io, args.cpu_iters, threads, gpu_ranks)
+        cpu, port = cpu_baseline(fo, io, args.cpu_iters, args.cpu_threads or baseline_threads(),
+                                 ranks if args.cpu_iters == args.steps else None)
         log(f"CPU baseline {cpu['value']} GTEPS ({cpu['kind']}, {cpu['cores']} threads)")
-        # the headline's ranks (after `steps` iterations) against the FP64 oracle
-        r64, _, _ = O.pr_f64(N, fo.out_degree, O.row_ptr_from(io), io.dst, mode=0, iterations=args.steps)
-        headline_err = fp64_errors(ranks, r64)
-        log(f"headline vs FP64: {headline_err}")
+    conv = convergence(gbx, ctx, args, g, args.method, method)
+    forms = None
+    if not args.no_forms:
+        forms = forms_s24(gbx, O, g, r64, args, value, args.method)
+        del fo, io
+        pr.free()
+        g.free()
+        ctx.set_tuning(tune_default)  # the cut tuned for s24 is not the other graphs' cut
+        forms.update(forms_s22(gbx, O, ctx, args))
+        forms["rmat_s16_gpu_and_cpu"] = forms_s16(gbx, O, ctx, args)
+        forms["graph_batch_256"] = forms_batch(gbx, O, ctx, args)
+        forms["adjoint_s20"] = forms_adjoint(gbx, O, ctx, args)
     out = {
         "metric": "GTEPS & % HBM roofline, pull PageRank R-MAT s24, 1 MI355X, vs plain FP32",
         "value": round(value, 3),
@@ -334,6 +562,7 @@ def main():
                         f"self loops and duplicates removed; {args.method} iteration",
             "scale": args.scale, "edge_factor": args.edge_factor, "seed": SEED,
             "num_vertices": N, "num_edges": E, "parallelism": "single GPU", "method": args.method,
+            "autotuned": tuned,
         },
         "roofline": {
             "bound": "hbm",
@@ -348,13 +577,14 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": cpu,
-        "error_vs_fp64": headline_err if cpu is not None else None,
-        "convergence": convergence,
+        "error_vs_fp64": headline_err,
+        "convergence": conv,
         "forms": forms,
     }
     print(json.dumps(out), flush=True)
-    pr.free()
-    g.free()
+    if args.no_forms:
+        pr.free()
+        g.free()
     ctx.close()
 
 

@@ -2014,6 +2014,13 @@ int binned_launch_iter(gbx_pr *pr)
 
 void binned_state_free(gbx_binned_state *s) { delete s; }
 
+void binned_drop_unused(gbx_plan *p)
+{
+    auto &v = p->bplans;
+    v.erase(std::remove_if(v.begin(), v.end(), [](const std::shared_ptr<gbx_bplan> &b) { return b.use_count() == 1; }),
+            v.end());
+}
+
 }  // namespace gbx
 
 gbx_plan::~gbx_plan() = default;

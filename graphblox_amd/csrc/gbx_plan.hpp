@@ -492,6 +492,7 @@ void seg_state_free(gbx_seg_state *s);
 bool binned_supported(const gbx_pr *pr);
 int binned_create(gbx_pr *pr);       // builds the plan's binned layout on first use
 int binned_launch_iter(gbx_pr *pr);  // scatter + accumulate: one iteration
+void binned_drop_unused(gbx_plan *p);  // drops the plan's layouts no object holds
 void binned_state_free(gbx_binned_state *s);
 
 // Data-driven push hooks implemented in gbx_ddpush.hip.

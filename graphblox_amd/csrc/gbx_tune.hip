@@ -1,11 +1,21 @@
-// gbx_tune.hip -- autotuner for the pull kernels' launch shapes.
+// gbx_tune.hip -- autotuner for the pull kernels' launch shapes and the
+// binned iteration's layout cut.
 //
 // Coordinate descent from the context's current tuning: for each field in
-// turn, try its candidate values on short timed runs of the real kernel on
+// turn, try its candidate values on short timed runs of the real kernels on
 // the caller's graph (HIP events around every iteration), keep a value only
-// when it is faster by more than 1 %, and repeat the sweep once.  Bin
-// thresholds and hub chunks change the plan, which is rebuilt by the next
-// object creation; the other fields only change launches.
+// when it is faster by more than 1 %, and repeat the sweep once.  Which
+// fields are searched follows the method the parameters select on this
+// graph: for the pull iteration the degree-bin thresholds, hub chunk, LDS
+// prefix and blocks per CU (bin thresholds and hub chunks change the plan,
+// rebuilt by the next object creation); for the binned iteration the edges
+// per source chunk and per destination bin, the rows per bin and the hot
+// sources (each cut builds a layout; those not chosen are dropped at the
+// end); for a batch of graphs the bin thresholds, hub chunk and blocks per
+// CU of its tile kernel; for the batched form its lanes, hub cut and blocks
+// per CU.
+#include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "gbx_plan.hpp"
@@ -13,7 +23,7 @@
 namespace gbx {
 
 static int trial(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, uint32_t k, uint32_t iters,
-                 const gbx_tuning &t, double *ms)
+                 const gbx_tuning &t, double *ms, int *method = nullptr)
 {
     ctx->tune = t;
     gbx_pr_params q = *p;
@@ -23,6 +33,7 @@ static int trial(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, uint32_t k,
     gbx_pr *pr = nullptr;
     int rc = k ? gbx_ppr_create(ctx, g, &q, k, nullptr, nullptr, &pr) : gbx_pr_create(ctx, g, &q, &pr);
     if (rc) return rc;
+    if (method) *method = pr->bin ? 1 : pr->seg ? 2 : 0;  // binned, batch of graphs, pull
     rc = gbx_pr_reset(pr, nullptr);
     if (!rc) rc = gbx_pr_launch(pr, 2);
     if (!rc) rc = gbx_pr_launch_timed(pr, iters, ms);
@@ -42,7 +53,9 @@ extern "C" int gbx_autotune(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, 
     const gbx_tuning start = ctx->tune;
     gbx_tuning cur = start;
     double cur_ms = 0.0;
-    int rc = trial(ctx, g, p, num_vectors, iters_per_trial, cur, &cur_ms);
+    int method = 0;
+    int rc = trial(ctx, g, p, num_vectors, iters_per_trial, cur, &cur_ms, &method);
+    const bool binned = method == 1, seg = method == 2;
     if (rc) {
         ctx->tune = start;
         return rc;
@@ -54,7 +67,20 @@ extern "C" int gbx_autotune(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, 
     std::vector<Field> fields;
     std::vector<std::vector<uint32_t>> lane_sets = {
         {256, 128, 64, 32, 16, 8}, {512, 256, 128, 64, 32, 16}, {128, 64, 32, 16, 8, 4}};
-    if (num_vectors == 0) {
+    if (num_vectors == 0 && binned) {
+        // 0: the automatic cut (7/8 of E / CUs in [64K, 1M] for the run layout)
+        const std::vector<uint32_t> cuts = {0, 393216, 524288, 786432, 1048576, 1310720};
+        fields.push_back({[](gbx_tuning &t) { return &t.binned_chunk_edges; }, cuts});
+        fields.push_back({[](gbx_tuning &t) { return &t.binned_bin_edges; }, cuts});
+        fields.push_back({[](gbx_tuning &t) { return &t.binned_hot_sources; }, {8192, 12288, 16384}});
+        fields.push_back({[](gbx_tuning &t) { return &t.binned_bin_rows; }, {4096, 8192, 20000}});
+    } else if (num_vectors == 0 && seg) {
+        // a batch of graphs: its graph-pure tiles follow the bin thresholds
+        // and hub chunk; its grid, the batched form's blocks per CU
+        fields.push_back({[](gbx_tuning &t) { return &t.bin_thresholds[0]; }, {2048, 4096, 8192, 16384, 32768}});
+        fields.push_back({[](gbx_tuning &t) { return &t.hub_chunk; }, {8192, 16384, 32768, 65536}});
+        fields.push_back({[](gbx_tuning &t) { return &t.ppr_blocks_per_cu; }, {1, 2, 4, 8}});
+    } else if (num_vectors == 0) {
         fields.push_back({[](gbx_tuning &t) { return &t.bin_thresholds[0]; }, {2048, 4096, 8192, 16384, 32768}});
         fields.push_back({[](gbx_tuning &t) { return &t.hub_chunk; }, {8192, 16384, 32768, 65536}});
         fields.push_back({[](gbx_tuning &t) { return &t.lds_bytes; }, {0, 65536, 159744}});
@@ -66,7 +92,7 @@ extern "C" int gbx_autotune(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, 
         fields.push_back({[](gbx_tuning &t) { return &t.ppr_blocks_per_cu; }, {1, 2, 4}});
     }
     for (int pass = 0; pass < 2; pass++) {
-        if (num_vectors == 0) {
+        if (num_vectors == 0 && !binned) {
             // the lane thresholds move together
             for (const auto &ls : lane_sets) {
                 gbx_tuning t = cur;
@@ -98,6 +124,9 @@ extern "C" int gbx_autotune(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, 
         }
         if (rc) break;
     }
+    // layouts of the cuts not chosen (no object holds them) are dropped;
+    // the next object on the chosen cut reuses or rebuilds its own
+    if (binned && (unsigned)p->direction < 2 && g->plan[p->direction]) binned_drop_unused(g->plan[p->direction].get());
     if (rc) {
         ctx->tune = start;
         return rc;

@@ -369,10 +369,15 @@ typedef struct gbx_tuning {
 int gbx_tuning_get(const gbx_ctx *ctx, gbx_tuning *t);
 int gbx_tuning_set(gbx_ctx *ctx, const gbx_tuning *t);
 
-/* Autotuner: times short runs of the pull kernel (num_vectors = 0) or of
- * the batched form (num_vectors > 0) on graph g over a small search of the
- * tuning fields (coordinate descent from the current setting), installs the
- * fastest in ctx and returns it in *best with its mean iteration time. */
+/* Autotuner: times short runs of the iteration p selects on graph g (the
+ * pull kernel or the binned iteration, num_vectors = 0) or of the batched
+ * form (num_vectors > 0) over a small search of the tuning fields of that
+ * iteration (coordinate descent from the current setting: bin thresholds,
+ * hub chunk, LDS prefix and blocks per CU for the pull kernel; edges per
+ * chunk and per bin, rows per bin and hot sources for the binned one; bin
+ * thresholds, hub chunk and ppr_blocks_per_cu for a batch of graphs; lanes,
+ * hub cut and blocks per CU for the batched form), installs the fastest in
+ * ctx and returns it in *best with its mean iteration time. */
 int gbx_autotune(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, uint32_t num_vectors,
                  uint32_t iters_per_trial, gbx_tuning *best, double *best_ms);
 

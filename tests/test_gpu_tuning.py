@@ -95,3 +95,51 @@ def test_autotune_single_and_batched(fresh_ctx):
     bestb, msb = fresh_ctx.autotune(g, 8, 3, iterations=10, epsilon=0.0, mode=gbx.PR_STOCHASTIC)
     assert msb > 0 and bestb.ppr_vectors_per_lane in (1, 2, 4)
     print("[gpu] autotuned:", best.as_dict(), ms, bestb.as_dict(), msb)
+
+
+def test_autotune_binned_layout_cut(fresh_ctx):
+    """With the binned iteration the autotuner searches its layout cut and hot
+    sources (not the pull kernel's fields), and the ranks stay within FP32
+    rounding of the default cut's (the run sums follow the cut)."""
+    g = fresh_ctx.rmat_graph(17, 16, 12)
+    start = fresh_ctx.tuning().as_dict()
+    ref, _ = g.pagerank(iterations=20, epsilon=0.0, method=gbx.METHOD_BINNED)
+    best, ms = fresh_ctx.autotune(g, 0, 3, iterations=10, epsilon=0.0, method=gbx.METHOD_BINNED)
+    b = best.as_dict()
+    assert ms > 0 and fresh_ctx.tuning().as_dict() == b
+    for k in ("bin_thresholds", "hub_chunk", "lds_bytes", "blocks_per_cu"):
+        assert b[k] == start[k], k
+    r, _ = g.pagerank(iterations=20, epsilon=0.0, method=gbx.METHOD_BINNED)
+    assert l1(r, ref) <= 1e-6
+    print("[gpu] autotuned binned:", {k: b[k] for k in b if k.startswith("binned")}, ms)
+
+
+def test_autotune_graph_batch(fresh_ctx):
+    """A batch of graphs tunes its tile cut and grid; every member's ranks
+    stay within 1e-6 of those under the starting setting."""
+    members = [fresh_ctx.rmat_graph(s, 8, 70 + s) for s in (10, 12, 13, 11)]
+    b = fresh_ctx.graph_batch(members)
+    kw = dict(iterations=15, epsilon=0.0, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_NONE)
+    s = b.session(**kw)
+    s.reset()
+    s.launch(15)
+    ref, _ = s.read()
+    s.free()
+    start = fresh_ctx.tuning().as_dict()
+    best, ms = fresh_ctx.autotune(b, 0, 3, **kw)
+    d = best.as_dict()
+    assert ms > 0 and fresh_ctx.tuning().as_dict() == d
+    for k in ("lds_bytes", "blocks_per_cu", "binned_chunk_edges", "ppr_vectors_per_lane"):
+        assert d[k] == start[k], k
+    s = b.session(**kw)
+    s.reset()
+    s.launch(15)
+    r, reps = s.read()
+    s.free()
+    off = b.batch_offsets()
+    for k in range(len(members)):
+        assert l1(r[off[k]:off[k + 1]], ref[off[k]:off[k + 1]]) <= 1e-6, k
+        assert abs(reps[k]["rank_sum"] - 1.0) < 1e-5
+    b.free()
+    for m in members:
+        m.free()
