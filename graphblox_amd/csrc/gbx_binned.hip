@@ -89,18 +89,30 @@ inline uint32_t chunk_sources_of(const gbx_tuning &tu, uint32_t elem, bool runs)
 }
 
 // Edges per chunk and per bin: the tuning value, or (0) automatic, measured
-// on MI355X.  One value per edge: E / CUs clamped to [64K, 512K] (256K best
-// at R-MAT s22, 512K at s24).  Run layout: 7/8 of E / CUs clamped to
-// [64K, 1M] (bigger chunks merge more edges into runs; 0.53 ms at s24 for
-// 768K-896K against 0.59 for 512K and 0.55 for 1M; 192K-256K at s22, 64K at
-// s20).
-inline uint32_t cut_edges_of(uint32_t v, const gbx_ctx *ctx, uint32_t E, bool runs)
+// on MI355X.  One value per edge (one scatter block per chunk): E / CUs
+// clamped to [64K, 512K] (256K best at R-MAT s22, 512K at s24).  Run
+// layout: chunks are cut by sources only (the scatter's blocks take equal
+// shares of all groups, whatever the chunks: the larger a chunk, the more
+// of its edges into one row merge into one run -- 0.386 V values per cold
+// edge at R-MAT s24 against 0.578 with 900K edges per chunk); bins 7/8 of
+// E / CUs clamped to [64K, 1M].
+inline uint32_t cut_edges_of(uint32_t v, const gbx_ctx *ctx, uint32_t E, bool runs, bool chunk)
 {
     if (v) return v;
+    if (runs && chunk) return 0xFFFFFFFFu;
     const uint32_t per_cu = E / (uint32_t)std::max(1, ctx->num_cus);
     if (runs) return std::min(1u << 20, std::max(1u << 16, (uint32_t)((uint64_t)per_cu * 7 / 8)));
     return std::min(1u << 19, std::max(1u << 16, per_cu));
 }
+
+// Run layout: one scatter block per piece -- a chunk, or an equal part of a
+// chunk of more than a piece's groups: 1/(2 CUs) of all groups, at least
+// kPieceGroups, cut at multiples of 64 groups (a step's start) -- launched
+// the largest first.
+// Pieces of at least this many groups: the LDS copy of a chunk's
+// contributions (128 KiB) stays a small share of a piece's work (measured:
+// 23040-30720 best at R-MAT s22, 46208 = 1/(2 CUs) at s24).
+constexpr uint32_t kPieceGroups = 24576;
 
 struct BinUnit {
     uint32_t row_begin, row_end;  // rows of the bin (plan order)
@@ -129,6 +141,8 @@ struct gbx_bplan {
     bool hot16 = false;               // packed hot pairs (with the run layout)
     gbx::DevBuf chunk_lo;             // u32[NC+1] first source of each chunk
     gbx::DevBuf chunk_grp;            // u32[NC+1] first group of 8 src_local entries of each chunk
+    uint32_t NP = 0;                  // run layout: scatter blocks (pieces)
+    gbx::DevBuf pieces;               // run layout: uint4[NP] first and end source, first and end group
     gbx::DevBuf src_local;            // u16[Plen] source - chunk_lo, chunk-major, segments padded (0xFFFF)
     gbx::DevBuf bdst;                 // u32[Plen/32] V position of each block of 32 src_local entries
     gbx::DevBuf blk_rec;              // run layout: uint4 per block of 64 groups (V position of its first run,
@@ -608,6 +622,7 @@ __global__ void __launch_bounds__(kBlock) bin_scatter_kernel(ScatterView S, cons
 struct RunsView {
     const uint32_t *__restrict__ chunk_lo;
     const uint32_t *__restrict__ chunk_grp;
+    const uint4 *__restrict__ pieces;  // per block: sources [x, y) of its chunk, groups [z, w)
     const uint16_t *__restrict__ src_local;  // local source | kRunEnd; pads: `pad`
     const uint4 *__restrict__ blk_rec;       // per block of 64 groups: first run's V position, segments
                                              // before it, segment-start bits (lo, hi)
@@ -634,10 +649,10 @@ __device__ __forceinline__ void wave_lds_order() { asm volatile("" ::: "memory")
 constexpr int kRunsBlock = 1024;  // 16 waves: 128 KiB of contributions + 16 x 2 KiB of staging fit the LDS
 constexpr uint32_t kRunStage = 64 * kRunGroup;  // staged run sums of a wave
 constexpr int kRunsAhead = 4;     // steps per wave whose loads are in flight
-// Stores every step issues (384 ranks; a step has about 300 runs at R-MAT
-// s24).  Measured: 6 always and the rest on demand beats 8 always by 4 %,
-// 3-5 by 2 %.
-constexpr uint32_t kRunStores = 6;
+// Stores every step issues, the rest on demand (a step has about 200 runs
+// at R-MAT s24, 0.39 V values per cold edge).  Measured at s24: 2 always
+// beats 6 by 4 % (FP32) and 5 % (BF16), 3-5 by 1-3 %.
+constexpr uint32_t kRunStores = 2;
 // Idle store lanes write a dummy area of their own wave (64 values, 256
 // bytes apart): one common dummy address would funnel a large share of
 // all stores into one cache line of one L2 channel.
@@ -765,35 +780,20 @@ __device__ __forceinline__ void runs_out(const float *stage, const RunStep &st, 
     }
 }
 
-// Run layout (float kinds): one block per source chunk (the chunks come in
-// plan order, the heaviest first); each wave takes steps of 64 consecutive
-// groups of 8 src_local entries.  The loads of the next kRunsAhead steps
-// are in flight while a step is summed; a wave always runs whole rounds of
-// kRunsAhead steps (steps past the chunk's last group load a valid group
-// and stage nothing).
-template <class T, bool W = false>
-__global__ void __launch_bounds__(kRunsBlock) bin_scatter_runs_kernel(RunsView S, const Slot *slots, uint32_t t,
-                                                                      const T *__restrict__ c_in, T *__restrict__ V)
+// Run layout (float kinds): the groups of `src_local` of one chunk, from g0
+// to g1, by the whole block: the chunk's contributions (sources c0..c1) are
+// copied to LDS, then each wave takes steps of 64 consecutive groups.  The
+// loads of the next kRunsAhead steps are in flight while a step is summed;
+// a wave always runs whole rounds of kRunsAhead steps (steps past g1 load a
+// valid group and stage nothing).
+template <class T, bool W>
+__device__ __forceinline__ void scatter_runs_piece(const RunsView &S, const T *__restrict__ c_in, T *__restrict__ V,
+                                                   T *cl, float *stage, uint32_t dummy_off, uint32_t c0, uint32_t c1,
+                                                   uint32_t g0, uint32_t g1)
 {
-    // the stop flag and the chunk's bounds are loaded together (one round
-    // trip before the contributions' loads instead of two); the empty asm
-    // keeps the bound loads ahead of the flag's branch
-    const uint32_t k = blockIdx.x;
-    const auto done = slots[t & 1].done;
-    const uint32_t c0 = S.chunk_lo[k], c1 = S.chunk_lo[k + 1];
-    const uint32_t g0 = S.chunk_grp[k], g1 = S.chunk_grp[k + 1];
-    asm volatile("" ::"s"(c0), "s"(c1), "s"(g0), "s"(g1));
-    if (done) return;
-    T *cl = reinterpret_cast<T *>(gbx_dyn_lds);
-    const uint32_t n = c1 - c0;
-    copy_to_lds<T, kRunsBlock>(cl, c_in + c0, n);
-    float *stage = reinterpret_cast<float *>(gbx_dyn_lds + S.stage_off) + (threadIdx.x >> 6) * kRunStage;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t dummy_off = S.dummy + ((blockIdx.x * (kRunsBlock / 64) + (threadIdx.x >> 6)) % kRunDummyAreas) *
-                                             kRunDummyStride + lane * (uint32_t)sizeof(T);
-    if (threadIdx.x == 0) cl[S.pad] = T(0);
+    copy_to_lds<T, kRunsBlock>(cl, c_in + c0, c1 - c0);
     __syncthreads();
-    if (g0 >= g1) return;
+    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t padw = S.pad | (S.pad << 16);
     const uint4 padv = make_uint4(padw, padw, padw, padw);
     const uint4 *src = reinterpret_cast<const uint4 *>(S.src_local);
@@ -803,7 +803,7 @@ __global__ void __launch_bounds__(kRunsBlock) bin_scatter_runs_kernel(RunsView S
     W8 wt[kRunsAhead];
     uint32_t sv[2];  // the V offset of the segment each lane would start, two steps ahead
     // a step's group data and block record; loads are always issued
-    // (clamped into the chunk), lanes outside it take pads when summed
+    // (clamped into [g0, g1)), lanes outside it take pads when summed
     auto load = [&](int d, uint32_t first) {
         const uint32_t g = min(max(first + lane, g0), g1 - 1u);
         w[d] = src[g];
@@ -832,6 +832,9 @@ __global__ void __launch_bounds__(kRunsBlock) bin_scatter_runs_kernel(RunsView S
         if constexpr (W) cwt = wt[d];
         const uint64_t bits = ((uint64_t)rec[d].w << 32) | rec[d].z;
         const bool start = act && ((bits >> lane) & 1u);
+        // the first active lane of a step leads the segment it is in (a
+        // step starts at a multiple of 64 groups, or at g0 = a chunk's
+        // first group, a segment start)
         const bool leader = start || (act && lane == 0);
         const uint32_t vbase = start ? sv[par] : rec[d].x;  // the slots are free for the next loads
         __builtin_amdgcn_sched_barrier(0);
@@ -868,6 +871,27 @@ __global__ void __launch_bounds__(kRunsBlock) bin_scatter_runs_kernel(RunsView S
     // the same loads and stores, and the waits inside it stay as short
     round(r);
     for (r += kRunsAhead * kRunsBlock; r < g1; r += kRunsAhead * kRunsBlock) round(r);
+}
+
+// Run layout (float kinds): one block per piece (a chunk or a part of one);
+// the pieces come the largest first.
+template <class T, bool W = false>
+__global__ void __launch_bounds__(kRunsBlock) bin_scatter_runs_kernel(RunsView S, const Slot *slots, uint32_t t,
+                                                                      const T *__restrict__ c_in, T *__restrict__ V)
+{
+    // the stop flag and the piece are loaded together (one round trip
+    // before the contributions' loads instead of two); the empty asm keeps
+    // the piece's loads ahead of the flag's branch
+    const auto done = slots[t & 1].done;
+    const uint4 pc = S.pieces[blockIdx.x];
+    asm volatile("" ::"s"(pc.x), "s"(pc.y), "s"(pc.z), "s"(pc.w));
+    if (done) return;
+    T *cl = reinterpret_cast<T *>(gbx_dyn_lds);
+    float *stage = reinterpret_cast<float *>(gbx_dyn_lds + S.stage_off) + (threadIdx.x >> 6) * kRunStage;
+    const uint32_t dummy_off = S.dummy + ((blockIdx.x * (kRunsBlock / 64) + (threadIdx.x >> 6)) % kRunDummyAreas) *
+                                             kRunDummyStride + (threadIdx.x & 63u) * (uint32_t)sizeof(T);
+    if (threadIdx.x == 0) cl[S.pad] = T(0);  // the pads' zero, past the chunk's contributions
+    scatter_runs_piece<T, W>(S, c_in, V, cl, stage, dummy_off, pc.x, pc.y, pc.z, pc.w);
 }
 
 struct AccumView {
@@ -1315,6 +1339,44 @@ static int build_runs(gbx_ctx *ctx, gbx_bplan *bp, uint32_t Ecold, const uint32_
     chunk_groups<<<grid_for(NC + 1, 256, 1024), 256, 0, ctx->stream>>>(
         cstart, NC, R.idx.as<uint32_t>(), Ecold, R.n, R.pstart.as<uint32_t>(), 3u, bp->chunk_grp.as<uint32_t>());
     GBX_HIP(ctx, hipGetLastError());
+    // the scatter's pieces: every chunk, the large ones cut into equal parts
+    // (at step starts), the largest first
+    {
+        std::vector<uint32_t> cg(NC + 1), lo(NC + 1);
+        GBX_HIP(ctx, hipMemcpyAsync(cg.data(), bp->chunk_grp.p, (size_t)(NC + 1) * 4, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        GBX_HIP(ctx, hipMemcpyAsync(lo.data(), bp->chunk_lo.p, (size_t)(NC + 1) * 4, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const uint64_t total = cg[NC] - cg[0];
+        const uint64_t target = std::max<uint64_t>(total / (2u * (uint64_t)std::max(1, ctx->num_cus)), kPieceGroups);
+        const uint32_t S = (uint32_t)((target + kRunBlockGroups - 1) &
+                                      ~(uint64_t)(kRunBlockGroups - 1));
+        std::vector<uint4> pc;
+        for (uint32_t k = 0; k < NC; k++) {
+            const uint32_t g0 = cg[k], g1 = cg[k + 1];
+            if (g0 >= g1) continue;
+            const uint32_t parts = (uint32_t)(((uint64_t)(g1 - g0) + S - 1) / S);
+            const uint64_t step = ((uint64_t)(g1 - g0) + parts - 1) / parts;
+            uint32_t a = g0;
+            for (uint32_t q = 1; q <= parts; q++) {
+                const uint32_t b = q == parts ? g1
+                                              : std::max(a, (uint32_t)((g0 + step * q) & ~(uint64_t)(kRunBlockGroups - 1)));
+                if (b > a) pc.push_back(make_uint4(lo[k], lo[k + 1], a, b));
+                a = b;
+            }
+        }
+        std::stable_sort(pc.begin(), pc.end(), [](const uint4 &x, const uint4 &y) { return x.w - x.z > y.w - y.z; });
+        if (getenv("GBX_VERBOSE"))
+            fprintf(stderr, "[gbx] scatter pieces: %zu of at most %u groups (%llu groups)\n", pc.size(), S,
+                    (unsigned long long)total);
+        GBX_TRY(dev_alloc(ctx, bp->pieces, pc.size() * sizeof(uint4) + 16));
+        if (!pc.empty())
+            GBX_HIP(ctx, hipMemcpyAsync(bp->pieces.p, pc.data(), pc.size() * sizeof(uint4), hipMemcpyHostToDevice,
+                                        ctx->stream));
+        GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        bp->NP = (uint32_t)pc.size();
+    }
     // run starts and their running count
     DevBuf head, hx, htot;
     GBX_TRY(dev_alloc(ctx, head, (size_t)Ecold * 4 + 4));
@@ -1392,14 +1454,14 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
 {
     const gbx_tuning &tu = ctx->tune;
     const uint32_t N = p->N, E = p->E;
-    const uint32_t EC = cut_edges_of(tu.binned_chunk_edges, ctx, E, runs), SC = chunk_sources_of(tu, hot_elem, runs);
+    const uint32_t EC = cut_edges_of(tu.binned_chunk_edges, ctx, E, runs, true), SC = chunk_sources_of(tu, hot_elem, runs);
     // rows per bin: what the LDS holds next to H hot contributions of hot_elem
     // bytes (a multiple of 8)
     const bool hot16 = runs && H > 0;
     const uint32_t hot_slots = hot16 ? kHot16Slots : H;
     const uint32_t SB =
         std::min(std::min(tu.binned_bin_rows, kMaxBinRows), (kAccumLds - hot_slots * hot_elem) / 8u) & ~7u;
-    const uint32_t EB = (std::max(cut_edges_of(tu.binned_bin_edges, ctx, E, runs), 8u) + 7u) & ~7u;
+    const uint32_t EB = (std::max(cut_edges_of(tu.binned_bin_edges, ctx, E, runs, false), 8u) + 7u) & ~7u;
     if (SB < 8) return set_error(ctx, GBX_ERR_ARG, "binned layout: %u hot sources leave no room for bins", H);
     bp->H = H;
     bp->SB = SB;
@@ -1706,9 +1768,9 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     if (getenv("GBX_VERBOSE"))
         fprintf(stderr,
                 "[gbx] binned layout: N=%u E=%u hot=%u (%.1f%% of edges) pad=%u chunks=%u bins=%u (<=%u rows) "
-                "units=%u split_bins=%u segments=%u V=%u (+%.1f%% of cold edges) hot_pairs=%u\n",
+                "units=%u split_bins=%u segments=%u V=%u (%.3f per cold edge) hot_pairs=%u scatter_blocks=%u\n",
                 N, E, H, E ? 100.0 * (double)Ehot / E : 0.0, pad, NC, NB, SB, bp->NU, n_multi, bp->nseg, bp->Vlen,
-                Ecold ? 100.0 * ((double)bp->Vlen / Ecold - 1.0) : 0.0, bp->Hlen);
+                Ecold ? (double)bp->Vlen / Ecold : 0.0, bp->Hlen, runs ? bp->NP : NC);
     return GBX_OK;
 }
 
@@ -1844,8 +1906,8 @@ int binned_create(gbx_pr *pr)
     const uint32_t hot_elem = std::max<uint32_t>((uint32_t)vel, 4u);
     const uint32_t H = std::min(std::min(tu.binned_hot_sources * 4u / hot_elem, runs ? kHot16Max : 65528u), p->N) &
                        ~7u;
-    const uint32_t want[8] = {cut_edges_of(tu.binned_chunk_edges, ctx, p->E, runs),
-                              (std::max(cut_edges_of(tu.binned_bin_edges, ctx, p->E, runs), 8u) + 7u) & ~7u,
+    const uint32_t want[8] = {cut_edges_of(tu.binned_chunk_edges, ctx, p->E, runs, true),
+                              (std::max(cut_edges_of(tu.binned_bin_edges, ctx, p->E, runs, false), 8u) + 7u) & ~7u,
                               std::min(tu.binned_bin_rows, kMaxBinRows),
                               chunk_sources_of(tu, hot_elem, runs),
                               pad,
@@ -1932,6 +1994,7 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
             RunsView RV;
             RV.chunk_lo = S.chunk_lo;
             RV.chunk_grp = S.chunk_grp;
+            RV.pieces = bp->pieces.as<uint4>();
             RV.src_local = S.src_local;
             RV.blk_rec = bp->blk_rec.as<uint4>();
             RV.seg_v = bp->seg_v.as<uint32_t>();
@@ -1940,8 +2003,8 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
             RV.stage_off = st->stage_off;
             RV.dummy = st->dummy_off;
             RV.wcold = st->wcold;
-            if (bp->NC)
-                bin_scatter_runs_kernel<T, is_weighted_kind(KIND)><<<bp->NC, kRunsBlock, st->shm_scatter, stream>>>(
+            if (bp->NP)
+                bin_scatter_runs_kernel<T, is_weighted_kind(KIND)><<<bp->NP, kRunsBlock, st->shm_scatter, stream>>>(
                     RV, A.slots, A.t, reinterpret_cast<const T *>(A.c_in), st->V.as<T>());
             S.NC = 0;
         }

@@ -243,7 +243,7 @@ void tuning_defaults(gbx_tuning *t)
     t->ppr_blocks_per_cu = 2;
     t->ppr_vectors_per_lane = 1;
     t->binned_min_vertices = 1u << 21;
-    t->binned_chunk_edges = 0;  // automatic: E / CUs in [64K, 512K]
+    t->binned_chunk_edges = 0;  // automatic
     t->binned_bin_edges = 0;
     t->binned_bin_rows = 20000;
     t->binned_chunk_rows = 32768;

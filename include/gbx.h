@@ -350,11 +350,13 @@ typedef struct gbx_tuning {
     uint32_t ppr_blocks_per_cu;    /* batched form and graph batches: blocks per CU */
     uint32_t ppr_vectors_per_lane; /* batched form: contributions per lane (1, 2 or 4) */
     uint32_t binned_min_vertices;  /* GBX_METHOD_AUTO picks the binned method from this many vertices */
-    uint32_t binned_chunk_edges;   /* binned: out-edges per source chunk (one scatter block each);
+    uint32_t binned_chunk_edges;   /* binned: out-edges per source chunk at most; 0 = E / CUs
+                                      clamped to [65536, 524288] (one scatter block per chunk);
+                                      with the run layout no limit (chunks cut by sources; the
+                                      scatter's blocks take equal shares of all chunks' edges) */
+    uint32_t binned_bin_edges;     /* binned: in-edges per destination bin (one accumulate unit each);
                                       0 = E / CUs clamped to [65536, 524288]; with the run
                                       layout 7/8 of E / CUs clamped to [65536, 1048576] */
-    uint32_t binned_bin_edges;     /* binned: in-edges per destination bin (one accumulate block each);
-                                      0 = as binned_chunk_edges */
     uint32_t binned_bin_rows;      /* binned: destination rows per bin (<= 20000: LDS int64 sums) */
     uint32_t binned_chunk_rows;    /* binned: sources per chunk (<= 32768: LDS contributions; 32736 with the run layout) */
     uint32_t binned_hot_sources;   /* binned: the first this many sources (plan order, the most

@@ -111,15 +111,27 @@ def test_s24_reference_semantics_vs_port_and_fp64(s24, method):
 
 def test_s24_precision_form_vs_fp64(s24):
     """BF16 contributions (binned: exact fixed-point row sums of the BF16
-    values; FP64 reductions) and FP32, stochastic, to tolerance 1e-7."""
+    values; FP64 reductions) and FP32, stochastic, to tolerance 1e-7.
+
+    With BF16 contributions (and BF16 run sums) the iteration is a map on a
+    finite set of values: once the change of the exact iteration falls below
+    the rounding, it ends in a cycle (measured at R-MAT s24: period 11 with
+    L1 changes of 1.4e-7..2.2e-7 for the binned method, period 8 with
+    2.5e-8..1e-7 for the pull kernel), so the 1e-7 stop is reached only if a
+    cycle dips below it.  The BF16 bar is its error against FP64 and a last
+    change of at most 1e-6."""
     g, (fwd, inv, rp) = s24
     r64, it64, _ = O.pr_f64(g.N, fwd.out_degree, rp, inv.dst, mode=1, iterations=300, epsilon=1e-10)
     for prec, bar in ((gbx.F32, 1e-5), (gbx.BF16, 2e-3)):
         r, rep = g.pagerank(iterations=300, epsilon=1e-7, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_L1, precision=prec)
         e = l1(r, r64)
-        print(f"[gpu] s24 stochastic precision {prec}: {rep['executed']} iterations, L1 vs FP64 {e:.3e}, "
+        print(f"[gpu] s24 stochastic precision {prec}: {rep['executed']} iterations (converged "
+              f"{rep['converged']}, last change {rep['error_total']:.3e}), L1 vs FP64 {e:.3e}, "
               f"max rel {_max_rel(r, r64):.3e} (FP64 to 1e-10: {it64} iterations)", flush=True)
-        assert rep["converged"] == 1
+        if prec == gbx.F32:
+            assert rep["converged"] == 1
+        else:
+            assert rep["converged"] == 1 or rep["error_total"] <= 1e-6
         assert e <= bar
         assert abs(rep["rank_sum"] - 1.0) < 1e-5
 
