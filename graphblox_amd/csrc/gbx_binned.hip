@@ -183,16 +183,39 @@ namespace gbx {
 
 // ------------------------------------------------------------------ build kernels
 
-__global__ void edge_rows(const uint32_t *__restrict__ rp, uint32_t N, uint32_t *__restrict__ rowid)
+// rowid[j] = the row of plan edge j: rows [0, nlong) by a wave each (the
+// plan order puts the long rows first), the others by a thread each.
+__global__ void edge_rows_long(const uint32_t *__restrict__ rp, uint32_t nlong, uint32_t *__restrict__ rowid)
 {
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+    const uint32_t lane = threadIdx.x & 63u, waves = gridDim.x * (blockDim.x / 64u);
+    for (uint32_t v = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6); v < nlong; v += waves)
+        for (uint32_t j = rp[v] + lane; j < rp[v + 1]; j += 64u) rowid[j] = v;
+}
+
+__global__ void edge_rows(const uint32_t *__restrict__ rp, uint32_t v0, uint32_t N, uint32_t *__restrict__ rowid)
+{
+    for (uint32_t v = v0 + blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
         for (uint32_t j = rp[v]; j < rp[v + 1]; j++) rowid[j] = v;
 }
 
-__global__ void column_counts(const uint32_t *__restrict__ col, uint32_t E, uint32_t *__restrict__ cnt)
+// cnt[u] += occurrences of u in col: ids below kColLds counted in an LDS
+// histogram per block first (the plan order puts the sources of most edges
+// there), the rest with global atomics.
+constexpr uint32_t kColLds = 16384;
+__global__ void __launch_bounds__(1024) column_counts(const uint32_t *__restrict__ col, uint32_t E,
+                                                      uint32_t *__restrict__ cnt)
 {
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x)
-        atomicAdd(cnt + col[e], 1u);
+    __shared__ uint32_t h[kColLds];
+    for (uint32_t i = threadIdx.x; i < kColLds; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        const uint32_t u = col[e];
+        if (u < kColLds) atomicAdd(h + u, 1u);
+        else atomicAdd(cnt + u, 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kColLds; i += blockDim.x)
+        if (h[i]) atomicAdd(cnt + i, h[i]);
 }
 
 // key = bin * NC + chunk (bin-major) or chunk * NB + bin; payload = plan edge index
@@ -1475,7 +1498,7 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     GBX_TRY(dev_alloc(ctx, dcnt, (size_t)N * 4 + 4));
     GBX_TRY(fill_u32(ctx, dcnt.as<uint32_t>(), 0, N));
     if (E) {
-        column_counts<<<grid_for(E, 256, 16384), 256, 0, ctx->stream>>>(p->col.as<uint32_t>(), E,
+        column_counts<<<(unsigned)std::max(1, ctx->num_cus) * 2u, 1024, 0, ctx->stream>>>(p->col.as<uint32_t>(), E,
                                                                           dcnt.as<uint32_t>());
         GBX_HIP(ctx, hipGetLastError());
     }
@@ -1556,7 +1579,14 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     DevBuf rowid;
     if (E) {
         GBX_TRY(dev_alloc(ctx, rowid, (size_t)E * 4));
-        edge_rows<<<grid_for(N, 256, 16384), 256, 0, ctx->stream>>>(p->rp.as<uint32_t>(), N, rowid.as<uint32_t>());
+        // rows of at least 64 edges at the front of the plan order
+        uint32_t nlong = 0;
+        while (nlong < N && rp[nlong + 1] - rp[nlong] >= 64u) nlong++;
+        if (nlong)
+            edge_rows_long<<<grid_for(nlong, 4, 16384), 256, 0, ctx->stream>>>(p->rp.as<uint32_t>(), nlong,
+                                                                            rowid.as<uint32_t>());
+        edge_rows<<<grid_for(N - nlong, 256, 16384), 256, 0, ctx->stream>>>(p->rp.as<uint32_t>(), nlong, N,
+                                                                            rowid.as<uint32_t>());
         GBX_HIP(ctx, hipGetLastError());
     }
     // hot edges: per bin, in plan edge order (rows ascending), padded to 8
