@@ -16,17 +16,20 @@ of its ranks against the FP64 oracle, the oracle's OpenMP port of
 pageRankPullGraphCSR timed on this host's cores on the same graph, wall time
 to convergence, and `forms`: the pull kernel, BF16, the 32-vector batched
 form, weighted PageRank and weighted SpMV on R-MAT s22, stochastic PageRank
-to L1 < 1e-6 on s22, s16 beside the CPU port, the batch of 256 graphs and the
-adjoint on s20 -- each with its error against FP64 (or the port).
+to L1 < 1e-6 on s22, s16 beside the CPU port, the batch of 256 graphs, the
+adjoint on s20 -- each with its error against FP64 (or the port) -- and the
+edge-file ingest (read + build) at s22 beside the CPU path and at s24.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--scale 24]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
+import tempfile
 import time
 
 # the CPU baseline's OpenMP threads spread over the cores (read when the
@@ -392,6 +395,80 @@ def forms_batch(gbx, O, ctx, args):
     return rec
 
 
+class _EdgeList(ctypes.Structure):
+    """struct EdgeList (edgeList.h:12-25) as the host adapter returns it."""
+    _fields_ = [("num_edges", ctypes.c_uint32), ("num_vertices", ctypes.c_uint32), ("avg_degree", ctypes.c_uint32),
+                ("max_weight", ctypes.c_float), ("edges_array_weight", ctypes.c_void_p),
+                ("edges_array_src", ctypes.c_void_p), ("edges_array_dest", ctypes.c_void_p),
+                ("mask_array", ctypes.c_void_p), ("label_array", ctypes.c_void_p),
+                ("inverse_label_array", ctypes.c_void_p)]
+
+
+def cpu_ingest_baseline(O, path, N_gpu, E_gpu):
+    """The CPU side of the edge-file ingest: the host adapter's
+    readEdgeListsbin (the reference's serial record loop, edgeList.c:551-677)
+    on the same file, then the oracle's restatement of the DIRECTED=1 build
+    (count sort by source and destination, countsort.c:262-270, the vertex
+    map, the transpose and its sort; serial C) on the arrays it read."""
+    L = ctypes.CDLL(os.path.join(ROOT, "graphblox_amd", "libgraphblox_hip.so"))
+    L.readEdgeListsbin.restype = ctypes.POINTER(_EdgeList)
+    L.readEdgeListsbin.argtypes = [ctypes.c_char_p, ctypes.c_uint8, ctypes.c_uint32, ctypes.c_uint32]
+    L.freeEdgeList.argtypes = [ctypes.POINTER(_EdgeList)]
+    t0 = time.perf_counter()
+    ep = L.readEdgeListsbin(os.fsencode(path), 0, 0, 0)
+    t_read = time.perf_counter() - t0
+    e = ep.contents
+    E, N = e.num_edges, e.num_vertices
+    s = np.ctypeslib.as_array(ctypes.cast(e.edges_array_src, ctypes.POINTER(ctypes.c_uint32)), (E,)).copy()
+    d = np.ctypeslib.as_array(ctypes.cast(e.edges_array_dest, ctypes.POINTER(ctypes.c_uint32)), (E,)).copy()
+    L.freeEdgeList(ep)
+    t0 = time.perf_counter()
+    O.build_graph(s, d, None, N, mode=1)
+    t_build = time.perf_counter() - t0
+    assert (N, E) == (N_gpu, E_gpu)
+    return {"read_s": round(t_read, 3), "build_s": round(t_build, 3), "total_s": round(t_read + t_build, 3),
+            "threads": 1, "kind": "port",
+            "sample": "the same file: the host adapter's readEdgeListsbin, then the oracle's serial restatement "
+                      "of the DIRECTED=1 CSR + transpose build, sort mode 1"}
+
+
+def forms_ingest(gbx, O, ctx, args, tmpdir):
+    """An R-MAT edge file (.bin records, written to local storage and read
+    back from the page cache) through the GPU ingest: gbx_edges_read_file
+    (the records to the device through pinned 64 MiB pieces, deinterleave,
+    max id) and gbx_graph_create_from_edges (sort by source and
+    destination, the vertex map, the transpose); wall time of each, at s22
+    beside the CPU path on the same file and at s24."""
+    out = {}
+    for scale in sorted({min(22, args.scale), args.scale}):
+        s, d, _ = ctx.rmat_edges(scale, args.edge_factor, SEED)
+        path = os.path.join(tmpdir, f"rmat_s{scale}.bin")
+        np.ascontiguousarray(np.stack([s, d], axis=1)).tofile(path)
+        del s, d
+        nbytes = os.path.getsize(path)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ed = ctx.edges_read_file(path)
+        t_read = time.perf_counter() - t0
+        N, E = ed.N, ed.E
+        t0 = time.perf_counter()
+        g = ed.graph(sort_mode=gbx.SORT_RADIX_SRC_DST)
+        ctx.sync()
+        t_build = time.perf_counter() - t0
+        ed.free()
+        g.free()
+        rec = {"file_bytes": nbytes, "num_vertices": N, "num_edges": E, "read_s": round(t_read, 4),
+               "read_gbs": round(nbytes / t_read / 1e9, 2), "build_s": round(t_build, 4),
+               "total_s": round(t_read + t_build, 4)}
+        if scale == min(22, args.scale) and not args.no_cpu:
+            rec["cpu"] = cpu_ingest_baseline(O, path, N, E)
+            rec["gpu_vs_cpu_speedup"] = round(rec["cpu"]["total_s"] / rec["total_s"], 1)
+        os.remove(path)
+        out[f"rmat_s{scale}"] = rec
+        log(f"ingest s{scale}: {rec}")
+    return out
+
+
 def forms_adjoint(gbx, O, ctx, args):
     """Weighted PageRank on R-MAT s20 with lognormal(0,1) weights and its
     adjoint (loss weights N(0,1)): forward and adjoint fixed points to
@@ -472,6 +549,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: OMP_NUM_THREADS, else every allowed CPU")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="the headline only")
+    ap.add_argument("--tmpdir", default=None, help="where the ingest measurement writes its edge files")
     ap.add_argument("--no-autotune", action="store_true", help="the library's default layout cut")
     ap.add_argument("--method", choices=("binned", "pull"), default="binned",
                     help="iteration of the headline (binned: the library default at this size)")
@@ -543,6 +621,8 @@ def main():
         forms["rmat_s16_gpu_and_cpu"] = forms_s16(gbx, O, ctx, args)
         forms["graph_batch_256"] = forms_batch(gbx, O, ctx, args)
         forms["adjoint_s20"] = forms_adjoint(gbx, O, ctx, args)
+        with tempfile.TemporaryDirectory(dir=args.tmpdir) as td:
+            forms["ingest"] = forms_ingest(gbx, O, ctx, args, td)
     out = {
         "metric": "GTEPS & % HBM roofline, pull PageRank R-MAT s24, 1 MI355X, vs plain FP32",
         "value": round(value, 3),

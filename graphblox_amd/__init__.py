@@ -129,6 +129,12 @@ def lib():
         "gbx_autotune": (i32, [_vp, _vp, C.POINTER(PrParams), u32, u32, C.POINTER(Tuning), C.POINTER(f64)]),
         "gbx_graph_batch_create": (i32, [_vp, u32, _vp, C.POINTER(_vp)]),
         "gbx_graph_batch_info": (i32, [_vp, C.POINTER(u32), _vp]),
+        "gbx_edges_from_records": (i32, [_vp, _vp, u64, i32, i32, i32, C.POINTER(_vp)]),
+        "gbx_edges_read_file": (i32, [_vp, C.c_char_p, i32, i32, i32, C.POINTER(_vp)]),
+        "gbx_edges_info": (i32, [_vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_float)]),
+        "gbx_edges_export": (i32, [_vp, _vp, _vp, _vp]),
+        "gbx_graph_create_from_edges": (i32, [_vp, _vp, i32, C.c_uint, _vp, _vp, C.POINTER(_vp)]),
+        "gbx_edges_free": (i32, [_vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -150,6 +156,8 @@ def exported_symbols():
         "gbx_ppr_create", "gbx_ppr_info", "gbx_pr_launch_graph", "gbx_graph_batch_create",
         "gbx_graph_batch_info", "gbx_tuning_get", "gbx_tuning_set", "gbx_autotune",
         "gbx_pr_adjoint", "gbx_spmv_pull_noalloc", "gbx_gather_accumulate_u32",
+        "gbx_edges_from_records", "gbx_edges_read_file", "gbx_edges_info", "gbx_edges_export",
+        "gbx_graph_create_from_edges", "gbx_edges_free",
     ]
 
 
@@ -283,6 +291,22 @@ class Context:
         g._batch = True
         return g
 
+    def edges_read_file(self, path, weighted_records=False, inverse=False, symmetric=False) -> "Edges":
+        """readEdgeListsbin on the GPU: the file's raw records to a device edge list."""
+        h = _vp()
+        self.check(self.L.gbx_edges_read_file(self.h, os.fsencode(path), int(weighted_records), int(inverse),
+                                              int(symmetric), C.byref(h)))
+        return Edges(self, h)
+
+    def edges_from_records(self, records, weighted_records=False, inverse=False, symmetric=False) -> "Edges":
+        """The same from records in host memory (bytes, or a numpy array of them)."""
+        buf = np.frombuffer(records, np.uint8) if isinstance(records, (bytes, bytearray, memoryview)) else \
+            np.ascontiguousarray(records).view(np.uint8).reshape(-1)
+        h = _vp()
+        self.check(self.L.gbx_edges_from_records(self.h, _ptr(buf) if buf.size else None, buf.size,
+                                                 int(weighted_records), int(inverse), int(symmetric), C.byref(h)))
+        return Edges(self, h)
+
     def rmat_graph(self, scale, edge_factor, seed, a=0.57, b=0.19, c=0.19, weight_kind=0, weight_seed=0,
                    sort_mode=SORT_RADIX_SRC_DST, flags=0) -> "Graph":
         h = _vp()
@@ -301,6 +325,58 @@ class Context:
                                          weight_seed, _ptr(s), _ptr(d), _ptr(w), C.byref(n)))
         k = n.value
         return s[:k].copy(), d[:k].copy(), w[:k].copy()
+
+
+class Edges:
+    """A device-resident edge list (gbx_edges): what readEdgeListsbin reads."""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx = ctx
+        self.h = h
+        ctx._adopt(self)
+        n, e, m = C.c_uint32(), C.c_uint32(), C.c_float()
+        ctx.check(ctx.L.gbx_edges_info(h, C.byref(n), C.byref(e), C.byref(m)))
+        self.N, self.E, self.max_weight = n.value, e.value, m.value
+
+    def export(self):
+        E = self.E
+        s, d, w = np.empty(E, np.uint32), np.empty(E, np.uint32), np.empty(E, np.float32)
+        self.ctx.check(self.ctx.L.gbx_edges_export(self.h, _ptr(s), _ptr(d), _ptr(w)))
+        return s, d, w
+
+    def graph(self, sort_mode=SORT_RADIX_SRC_DST, flags=0, lmode=(0, 0, 0), file_labels=None) -> "Graph":
+        """gbx_graph_create_from_edges: the edge list is consumed."""
+        rl = None
+        labels = None
+        if any(lmode) or file_labels is not None:
+            rl = Relabel()
+            rl.lmode[:] = list(lmode) + [0] * (3 - len(lmode))
+            fl = _u32(file_labels) if file_labels is not None else None
+            rl.file_labels = _ptr(fl)
+            rl.num_file_labels = fl.size if fl is not None else 0
+            labels = np.zeros(max(self.N, 1), np.uint32)
+        h = _vp()
+        try:
+            self.ctx.check(self.ctx.L.gbx_graph_create_from_edges(self.ctx.h, self.h, sort_mode, flags,
+                                                                  C.byref(rl) if rl is not None else None,
+                                                                  _ptr(labels), C.byref(h)))
+        finally:
+            self.E = 0
+        g = Graph(self.ctx, h)
+        if labels is not None:
+            g.labels = labels[:g.N]
+        return g
+
+    def free(self):
+        if self.h:
+            self.ctx.L.gbx_edges_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Graph:

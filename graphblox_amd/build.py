@@ -22,7 +22,8 @@ HOST = os.path.join(PKG, "host")
 INC = os.path.join(ROOT, "include")
 
 HIP_SOURCES = ["gbx_core.hip", "gbx_sort.hip", "gbx_rmat.hip", "gbx_pr.hip", "gbx_binned.hip", "gbx_ppr.hip",
-               "gbx_batch.hip", "gbx_tune.hip", "gbx_ddpush.hip", "gbx_relabel.hip", "gbx_noalloc.hip"]
+               "gbx_batch.hip", "gbx_tune.hip", "gbx_ddpush.hip", "gbx_relabel.hip", "gbx_noalloc.hip",
+               "gbx_ingest.hip"]
 HOST_SOURCES = ["graphblox_hip.c", "graphstats_hip.c"]
 
 LIB_HIP = os.path.join(PKG, "libgbx_hip.so")

@@ -381,6 +381,7 @@ const char *gbx_status_string(int status)
     case GBX_ERR_HIP: return "HIP runtime error";
     case GBX_ERR_RANGE: return "vertex id out of range";
     case GBX_ERR_STATE: return "invalid state";
+    case GBX_ERR_IO: return "file error";
     default: return "unknown status";
     }
 }

@@ -121,6 +121,9 @@ int build_csr_device(gbx_ctx *ctx, uint32_t *src, uint32_t *dst, float *w, uint3
 int graph_from_device_edges(gbx_ctx *ctx, DevBuf &src, DevBuf &dst, DevBuf &w, uint32_t E,
                             uint32_t N, int sort_mode, unsigned flags, gbx_graph **out,
                             const gbx_relabel *relabel = nullptr, uint32_t *labels_out = nullptr);
+// Ids below 2^30 (the column mask) and below N; N = 0 becomes max id + 1
+// (edgeList.c:679).
+int check_ids(gbx_ctx *ctx, const DevBuf &ds, const DevBuf &dd, uint32_t E, uint32_t *N);
 // ---- relabel levels (gbx_relabel.hip)
 bool relabel_requested(const gbx_relabel *rl);
 int relabel_validate(gbx_ctx *ctx, const gbx_relabel *rl);

@@ -414,7 +414,7 @@ static int upload_edges(gbx_ctx *ctx, const uint32_t *src, const uint32_t *dst, 
     return GBX_OK;
 }
 
-static int check_ids(gbx_ctx *ctx, const DevBuf &ds, const DevBuf &dd, uint32_t E, uint32_t *N)
+int check_ids(gbx_ctx *ctx, const DevBuf &ds, const DevBuf &dd, uint32_t E, uint32_t *N)
 {
     uint32_t ms = 0, md = 0;
     GBX_TRY(max_u32(ctx, ds.as<uint32_t>(), E, &ms));

@@ -396,9 +396,10 @@ static uint32_t *read_label_file(const char *fname, uint32_t num_vertices)
     return labels;
 }
 
-/* graphCSRPreProcessingStep, graphCSR.c:243-327: the reader runs on the
- * host; both count sorts, the -k dedupe, the relabel levels (lmode,
- * lmode_l2, lmode_l3) and the vertex mapping run on the GPU.  With
+/* graphCSRPreProcessingStep, graphCSR.c:243-327: the reader
+ * (readEdgeListsbin's records, inverse swap and symmetric append, max id),
+ * both count sorts, the -k dedupe, the relabel levels (lmode, lmode_l2,
+ * lmode_l3) and the vertex mapping run on the GPU.  With
  * DIRECTED=0 the GraphCSR holds the forward CSR only, whose in_degree the
  * reference leaves zero (vertex.c:39-45, written only under DIRECTED at
  * vertex.c:291-311); the device keeps its transpose as well. */
@@ -408,7 +409,15 @@ struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
     if (!ctx) exit(EXIT_FAILURE);
     if (arguments->mmode)
         fprintf(stderr, "graphblox_hip: mask modes (mmode) are not provided by this backend; ignored\n");
-    struct EdgeList *in = readEdgeListsbin(arguments->fnameb, 0, arguments->symmetric, arguments->weighted);
+    gbx_edges *in = NULL;
+    if (gbx_edges_read_file(ctx, arguments->fnameb, arguments->weighted ? 1 : 0, 0, arguments->symmetric ? 1 : 0,
+                            &in) != GBX_OK) {
+        fprintf(stderr, "graphblox_hip: %s\n", gbx_last_error(ctx));
+        exit(EXIT_FAILURE);
+    }
+    uint32_t in_vertices = 0;
+    float in_max_weight = 1.0f;
+    gbx_edges_info(in, &in_vertices, NULL, &in_max_weight);
     /* the relabel levels (graphCSR.c:268-290) run on the GPU inside the build */
     gbx_relabel rl;
     memset(&rl, 0, sizeof(rl));
@@ -418,10 +427,10 @@ struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
     const int relabel = rl.lmode[0] || rl.lmode[1] || rl.lmode[2];
     uint32_t *file_labels = NULL;
     if (rl.lmode[0] == 11 || rl.lmode[1] == 11 || rl.lmode[2] == 11)
-        file_labels = read_label_file(arguments->fnamel, in->num_vertices);
+        file_labels = read_label_file(arguments->fnamel, in_vertices);
     rl.file_labels = file_labels;
-    rl.num_file_labels = file_labels ? in->num_vertices : 0;
-    uint32_t *labels = relabel ? host_malloc(((size_t)in->num_vertices + 1) * 4) : NULL;
+    rl.num_file_labels = file_labels ? in_vertices : 0;
+    uint32_t *labels = relabel ? host_malloc(((size_t)in_vertices + 1) * 4) : NULL;
     unsigned flags = 0;
     uint32_t sort = arguments->sort;
     if (arguments->dflag) {
@@ -431,9 +440,8 @@ struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
     if (GBX_WEIGHTED) flags |= GBX_GRAPH_WEIGHTED;
     if (sort > 3) sort = 0; /* sortRun.c:59-64: unknown modes fall back to radix by source */
     gbx_graph *g = NULL;
-    int rc = gbx_graph_create_relabel(ctx, in->edges_array_src, in->edges_array_dest, in->edges_array_weight,
-                                      in->num_edges, in->num_vertices, (int)sort, flags, relabel ? &rl : NULL,
-                                      labels, &g);
+    int rc = gbx_graph_create_from_edges(ctx, in, (int)sort, flags, relabel ? &rl : NULL, labels, &g);
+    gbx_edges_free(in);
     free(file_labels);
     if (rc != GBX_OK) {
         fprintf(stderr, "graphblox_hip: CSR build failed: %s\n", gbx_last_error(ctx));
@@ -447,14 +455,14 @@ struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
     csr->num_edges = E;
     csr->avg_degree = E / N;
 #if GBX_WEIGHTED
-    csr->max_weight = in->max_weight;
+    csr->max_weight = in_max_weight;
 #endif
     for (int inv = 0; inv < 1 + GBX_DIRECTED; inv++) {
         struct Vertex *v = new_vertex_array(N);
         struct EdgeList *el = new_edge_list(E);
         el->num_vertices = N;
         el->avg_degree = E / N;
-        el->max_weight = in->max_weight;
+        el->max_weight = GBX_WEIGHTED ? in_max_weight : 1.0f;
         label_arrays(el, N);
         if (labels) /* reorder.c:376-380: old id -> new id and back */
             for (uint32_t x = 0; x < N; x++) {
@@ -478,7 +486,6 @@ struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
         csr->vertices = v;
         csr->sorted_edges_array = el;
     }
-    freeEdgeList(in);
     free(labels);
     register_graph(csr, g);
     printf(" -----------------------------------------------------\n");

@@ -10,6 +10,7 @@
  *
  * Reference interfaces each group replaces (paths relative to the GraphBlox
  * repository root):
+ *   gbx_edges_*                       readEdgeListsbin (01_host/src/structures/edgeList.c:450-706)
  *   gbx_csr_build, gbx_graph_create*  sortRunAlgorithms (01_host/src/preprocess/sortRun.c:26-69),
  *                                     countSortEdgesBySource/-AndDestination (countsort.c:28-270),
  *                                     mapVerticesWithInOutDegree_base (01_host/src/structures/vertex.c:204-328),
@@ -44,6 +45,7 @@ extern "C" {
 #define GBX_ERR_HIP (-3)     /* HIP runtime error */
 #define GBX_ERR_RANGE (-4)   /* vertex id >= 2^30 or beyond num_vertices */
 #define GBX_ERR_STATE (-5)   /* object used in a state it does not support */
+#define GBX_ERR_IO (-6)      /* a file could not be opened or read */
 
 typedef struct gbx_ctx gbx_ctx;
 typedef struct gbx_graph gbx_graph;
@@ -164,6 +166,40 @@ int gbx_graph_from_csr(gbx_ctx *ctx, uint32_t num_vertices, uint32_t num_edges,
                        gbx_graph **out);
 
 int gbx_graph_info(const gbx_graph *g, uint32_t *num_vertices, uint32_t *num_edges);
+
+/* ---------------------------------------------------------------- edge-list ingest */
+/* A device-resident edge list (struct EdgeList, edgeList.h:12-25): what
+ * readEdgeListsbin (edgeList.c:450-706) makes of a .bin / .wbin file, built
+ * on the GPU from the file's raw records.  Records are 2 u32 (src, dst), or
+ * with weighted_records 2 u32 + f32 (a .wbin; the weight's bits as stored);
+ * E = bytes / record size (a partial last record is ignored, as the
+ * reference's division does).  inverse swaps src and dst of every record;
+ * symmetric appends every (swapped) edge reversed at [E, 2E) (edgeList.c:
+ * 551-578).  Weights are 1.0f without weighted_records.  N = max id + 1
+ * (edgeList.c:671-679; ids must stay below 2^30, the column mask); the
+ * largest weight (max_weight, starting at 0.0f, NaN never larger) and the
+ * count come back from gbx_edges_info. */
+typedef struct gbx_edges gbx_edges;
+
+/* From records in host memory (for example a mapped file): copied to the
+ * device through pinned staging buffers, 64 MiB at a time. */
+int gbx_edges_from_records(gbx_ctx *ctx, const void *records, uint64_t num_bytes, int weighted_records,
+                           int inverse, int symmetric, gbx_edges **out);
+/* From a file: read in 64 MiB pieces into pinned buffers, each copied to
+ * the device while the next is read. */
+int gbx_edges_read_file(gbx_ctx *ctx, const char *path, int weighted_records, int inverse, int symmetric,
+                        gbx_edges **out);
+int gbx_edges_info(const gbx_edges *e, uint32_t *num_vertices, uint32_t *num_edges, float *max_weight);
+/* Host copies of the edge arrays (any may be NULL), in file order. */
+int gbx_edges_export(const gbx_edges *e, uint32_t *src, uint32_t *dst, float *w);
+/* gbx_graph_create_relabel on the device edge list (flags: GBX_GRAPH_DEDUPE,
+ * GBX_GRAPH_WEIGHTED; the symmetric copy is made at ingest).  The edge list
+ * is consumed: its arrays become the graph's, and it holds no edges after
+ * the call, whether or not the call succeeds (free it with gbx_edges_free).
+ * labels_out [N from gbx_edges_info], nullable. */
+int gbx_graph_create_from_edges(gbx_ctx *ctx, gbx_edges *e, int sort_mode, unsigned flags,
+                                const gbx_relabel *relabel, uint32_t *labels_out, gbx_graph **out);
+int gbx_edges_free(gbx_edges *e);
 
 /* Copy the forward (inverse = 0) or transpose (inverse = 1) CSR arrays to
  * host, in the reference's layout (struct Vertex / struct EdgeList of
