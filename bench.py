@@ -6,7 +6,9 @@ whole device-resident graph.  The headline runs the library's default method
 for this size, the binned (propagation-blocking) iteration -- a scatter kernel
 writes every edge's source contribution into destination-range bins and an
 accumulate kernel sums the bins in LDS and applies the update -- with its
-layout cut chosen by the library's autotuner on this graph.
+layout cut chosen by the library's autotuner on this graph; the timed
+iterations are replayed as one captured HIP graph (`--eager`: launched one
+by one; the other is reported among the forms).
 
 Prints one JSON line: GTEPS (E edges per iteration / wall time per
 iteration), the HBM roofline of the iteration (algorithmic bytes of the pull
@@ -550,6 +552,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="the headline only")
     ap.add_argument("--tmpdir", default=None, help="where the ingest measurement writes its edge files")
+    ap.add_argument("--eager", action="store_true", help="headline iterations launched one by one, not replayed")
     ap.add_argument("--no-autotune", action="store_true", help="the library's default layout cut")
     ap.add_argument("--method", choices=("binned", "pull"), default="binned",
                     help="iteration of the headline (binned: the library default at this size)")
@@ -589,7 +592,8 @@ def main():
     pr = g.session(iterations=args.warmup + args.steps, epsilon=0.0, mode=gbx.PR_REF, stop=gbx.STOP_NONE,
                    method=method)
     log(f"{args.method} iteration set up in {time.perf_counter() - t0:.2f}s")
-    kernel_ms, ms_per_step, ranks, rep = time_session(pr, args.warmup, args.steps)
+    # the timed iterations replayed as one captured HIP graph (captured and run once before)
+    kernel_ms, ms_per_step, ranks, rep = time_session(pr, args.warmup, args.steps, graph_replay=not args.eager)
     assert rep["executed"] == args.steps
     assert np.isfinite(ranks).all()
     value = E / (ms_per_step * 1e-3) / 1e9
@@ -613,6 +617,10 @@ def main():
     forms = None
     if not args.no_forms:
         forms = forms_s24(gbx, O, g, r64, args, value, args.method)
+        km, wm, re_, _ = time_session(pr, args.warmup, args.steps, graph_replay=args.eager)
+        forms[f"{args.method}_f32_" + ("replayed" if args.eager else "eager")] = form_record(
+            f"the headline's iteration {'replayed as a HIP graph' if args.eager else 'launched one by one'}", E, N,
+            1, km, wm, bytes_per_launch, **fp64_errors(re_, r64))
         del fo, io
         pr.free()
         g.free()
@@ -642,6 +650,7 @@ def main():
                         f"self loops and duplicates removed; {args.method} iteration",
             "scale": args.scale, "edge_factor": args.edge_factor, "seed": SEED,
             "num_vertices": N, "num_edges": E, "parallelism": "single GPU", "method": args.method,
+            "launch": "launched one by one" if args.eager else "the timed iterations replayed as one captured HIP graph",
             "autotuned": tuned,
         },
         "roofline": {

@@ -232,10 +232,11 @@ int max_u32(gbx_ctx *ctx, const uint32_t *a, size_t n, uint32_t *h_max)
 
 void tuning_defaults(gbx_tuning *t)
 {
-    static const uint32_t thr[7] = {8192, 256, 128, 64, 32, 16, 8};
+    // hub threshold and chunk 0: automatic, by the graph's size (auto_hub_threshold)
+    static const uint32_t thr[7] = {0, 256, 128, 64, 32, 16, 8};
     memset(t, 0, sizeof(*t));
     memcpy(t->bin_thresholds, thr, sizeof(thr));
-    t->hub_chunk = 32768;
+    t->hub_chunk = 0;
     t->lds_bytes = 159744;  // 156 KiB of the 160 KiB LDS of a CU (1024-thread block, one per CU)
     t->blocks_per_cu = 1;
     t->ppr_hub = 1024;
@@ -252,10 +253,12 @@ void tuning_defaults(gbx_tuning *t)
 
 bool tuning_valid(const gbx_tuning *t)
 {
-    if (t->bin_thresholds[0] == 0) return false;
-    for (int i = 1; i < 7; i++)
+    // the hub threshold may be 0 (automatic: at least 2048, above the lane thresholds' 256 limit)
+    if (t->bin_thresholds[0] != 0 && t->bin_thresholds[1] > t->bin_thresholds[0]) return false;
+    if (t->bin_thresholds[1] == 0 || (t->bin_thresholds[0] == 0 && t->bin_thresholds[1] > 2048)) return false;
+    for (int i = 2; i < 7; i++)
         if (t->bin_thresholds[i] == 0 || t->bin_thresholds[i] > t->bin_thresholds[i - 1]) return false;
-    if (t->hub_chunk < 256 || t->hub_chunk > (1u << 24)) return false;
+    if (t->hub_chunk != 0 && (t->hub_chunk < 256 || t->hub_chunk > (1u << 24))) return false;
     if (t->lds_bytes > 160 * 1024 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return false;
     if (t->ppr_hub < 64 || t->ppr_chunk < 256 || t->ppr_blocks_per_cu < 1 || t->ppr_blocks_per_cu > 64)
         return false;

@@ -137,6 +137,18 @@ uint32_t bits_for(uint32_t max_value);
 // ---- tuning (gbx_core.hip)
 void tuning_defaults(gbx_tuning *t);
 bool tuning_valid(const gbx_tuning *t);
+// The pull plan's hub threshold (rows of at least this degree are cut into
+// chunks of 4x as many edges) when the tuning leaves it 0, for a graph whose
+// largest member has E edges.  Measured with the pull kernel on R-MAT (us per
+// iteration, threshold 2048 / 4096 / 8192): s16 22 / 26 / 31, s18 34 / 41 / 58,
+// s20 79 / 80 / 88, s22 305 / 280 / 305, s24 1694 / 1320 / 1302.
+inline uint32_t auto_hub_threshold(uint64_t E)
+{
+    return E < (1ull << 25) ? 2048u : E < (1ull << 27) ? 4096u : 8192u;
+}
+// The graph's edge count that sets the automatic cuts: its largest member's
+// for a batch of graphs.
+uint64_t cut_edges(const gbx_graph *g);
 
 
 inline unsigned grid_for(size_t n, unsigned block, unsigned cap = 1u << 20) {

@@ -344,7 +344,8 @@ __global__ void fill_f32(float *p, float v, uint32_t n)
 
 // Degree thresholds of the bins (hub, then 64, 32, 16, 8, 4, 2 lanes per
 // row; rows below the last threshold use 1 lane) and the hub chunk size, from
-// the context's tuning.
+// the context's tuning (a hub threshold or chunk of 0 chosen by the graph's
+// size).
 struct BinConfig {
     uint32_t thr[7];
     uint32_t hub_chunk;
@@ -354,13 +355,29 @@ struct BinConfig {
     }
 };
 
-static BinConfig bin_config(const gbx_ctx *ctx)
+static BinConfig bin_config(const gbx_ctx *ctx, const gbx_graph *g)
 {
     BinConfig b;
     memcpy(b.thr, ctx->tune.bin_thresholds, sizeof(b.thr));
-    b.hub_chunk = ctx->tune.hub_chunk;
+    if (b.thr[0] == 0) b.thr[0] = auto_hub_threshold(cut_edges(g));
+    b.hub_chunk = ctx->tune.hub_chunk ? ctx->tune.hub_chunk : 4u * b.thr[0];
     return b;
 }
+
+}  // namespace gbx
+
+uint64_t gbx::cut_edges(const gbx_graph *g)
+{
+    uint64_t m = g->E;
+    if (g->seg_edge_off.size() >= 2) {
+        m = 0;
+        for (size_t k = 0; k + 1 < g->seg_edge_off.size(); k++)
+            m = std::max<uint64_t>(m, g->seg_edge_off[k + 1] - g->seg_edge_off[k]);
+    }
+    return m;
+}
+
+namespace gbx {
 
 // Batch of graphs: graph-pure tiles.  Inside graph k (plan rows
 // [seg_off[k], seg_off[k+1]), degrees descending) hub rows become chunk tiles
@@ -509,7 +526,7 @@ static int build_plan(gbx_ctx *ctx, gbx_graph *g, int direction, gbx_plan **out)
         if (hipGetLastError() != hipSuccess) return fail(set_error(ctx, GBX_ERR_HIP, "plan_edges"));
     }
     // bins
-    const BinConfig bc = bin_config(ctx);
+    const BinConfig bc = bin_config(ctx, g);
     p->bins_thr_chunk[7] = bc.hub_chunk;
     memcpy(p->bins_thr_chunk, bc.thr, sizeof(bc.thr));
     DevBuf dthr, dbound;
@@ -602,7 +619,7 @@ int ensure_plan(gbx_ctx *ctx, gbx_graph *g, int direction, gbx_plan **out)
     std::shared_ptr<gbx_plan> &slot = g->plan[direction];
     if (slot) {
         // rebuild when the tuning's bins differ from the ones the plan was cut with
-        const BinConfig bc = bin_config(ctx);
+        const BinConfig bc = bin_config(ctx, g);
         BinConfig built;
         memcpy(built.thr, slot->bins_thr_chunk, sizeof(built.thr));
         built.hub_chunk = slot->bins_thr_chunk[7];

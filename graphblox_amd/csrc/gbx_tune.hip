@@ -96,7 +96,8 @@ extern "C" int gbx_autotune(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, 
             // the lane thresholds move together
             for (const auto &ls : lane_sets) {
                 gbx_tuning t = cur;
-                for (int i = 0; i < 6; i++) t.bin_thresholds[i + 1] = std::min(ls[i], t.bin_thresholds[0]);
+                const uint32_t hub = t.bin_thresholds[0] ? t.bin_thresholds[0] : auto_hub_threshold(cut_edges(g));
+                for (int i = 0; i < 6; i++) t.bin_thresholds[i + 1] = std::min(ls[i], hub);
                 if (!tuning_valid(&t) || memcmp(&t, &cur, sizeof(t)) == 0) continue;
                 double ms = 0.0;
                 if ((rc = trial(ctx, g, p, num_vectors, iters_per_trial, t, &ms))) break;

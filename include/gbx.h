@@ -377,8 +377,10 @@ int gbx_graph_batch_info(const gbx_graph *g, uint32_t *num_graphs, uint32_t *ver
  * GBX_BINNED_HOT_SOURCES, GBX_BINNED_EDGE_VALUES).  A change applies to plans and objects created afterwards;
  * results stay within the same tolerances for every setting. */
 typedef struct gbx_tuning {
-    uint32_t bin_thresholds[7];    /* degree thresholds: hub, then 64, 32, 16, 8, 4, 2 lanes per row */
-    uint32_t hub_chunk;            /* edges per hub chunk (one block each) */
+    uint32_t bin_thresholds[7];    /* degree thresholds: hub, then 64, 32, 16, 8, 4, 2 lanes per row;
+                                      hub 0 = by the graph's (largest member's) edges E: 2048 below
+                                      2^25, 4096 below 2^27, else 8192 */
+    uint32_t hub_chunk;            /* edges per hub chunk (one block each); 0 = 4 x the hub threshold */
     uint32_t lds_bytes;            /* LDS bytes per block for the hottest contributions (0 = off) */
     uint32_t blocks_per_cu;        /* persistent blocks per CU, single-vector kernels */
     uint32_t ppr_hub;              /* batched form: rows of at least this degree are chunked */

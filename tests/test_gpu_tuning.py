@@ -20,7 +20,7 @@ def fresh_ctx():
 def test_tuning_roundtrip_and_validation(fresh_ctx):
     t = fresh_ctx.tuning()
     d = t.as_dict()
-    assert d["bin_thresholds"] == [8192, 256, 128, 64, 32, 16, 8] and d["lds_bytes"] == 159744
+    assert d["bin_thresholds"] == [0, 256, 128, 64, 32, 16, 8] and d["hub_chunk"] == 0 and d["lds_bytes"] == 159744
     t.hub_chunk = 16384
     t.ppr_vectors_per_lane = 2
     fresh_ctx.set_tuning(t)
