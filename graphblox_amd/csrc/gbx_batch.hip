@@ -19,18 +19,6 @@
 
 namespace gbx {
 
-struct GCoef {
-    float ti;  // 1/N_k
-    float fs;  // d / S_k
-    float fd;  // D_k / N_k
-    uint32_t done;
-};
-
-struct GStat {
-    double err, dang, sum;
-    uint32_t iters, conv_iter, n, pad;
-};
-
 struct SegView {
     const TileDesc *__restrict__ tiles;
     const SegRange *__restrict__ ranges;
@@ -52,6 +40,9 @@ struct SegView {
 struct gbx_seg_state {
     gbx::DevBuf r, c[2], coef, stat, slots, flags, out;
     uint32_t grid = 0;
+    // on the binned iteration (K_SEG): the units' wave partials and every
+    // graph's first plan row
+    gbx::DevBuf uslots, goff;
 };
 
 namespace gbx {
@@ -236,6 +227,114 @@ __global__ void __launch_bounds__(256) seg_reduce_kernel(SegView S, int stop, do
     *S.counter = 0;
 }
 
+// ---------------------------------------------------------------- batch on the binned iteration
+
+// r0 = 1/N_k, c0 = r0/outdeg for every row (graph from the plan's row_gid).
+__global__ void segbin_reset_rows(const uint32_t *__restrict__ gid, const uint32_t *__restrict__ outdeg,
+                                  const GCoef *__restrict__ coef, uint32_t N, float *__restrict__ r,
+                                  float *__restrict__ c)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const float t0 = coef[gid[i]].ti;
+        r[i] = t0;
+        const uint32_t od = outdeg[i];
+        c[i] = od ? t0 / (float)od : 0.0f;
+    }
+}
+
+// One block per graph: at the reset, the rank sum and dangling mass of the
+// graph's rows [goff[k], goff[k+1]); after an iteration, the sums of its
+// units' wave slots in ascending unit order.  Then the next coefficients and
+// the stop rule as seg_reduce_kernel; the last block writes the batch's slot
+// for the next iteration (done when every graph is, the rank sum over all
+// graphs for the accumulate's fixed-point scale).
+__global__ void __launch_bounds__(256) segbin_reduce_kernel(
+    GCoef *__restrict__ coef, GStat *__restrict__ stat, const double *__restrict__ uslots,
+    const uint32_t *__restrict__ uptr, const uint32_t *__restrict__ uidx, const uint32_t *__restrict__ goff,
+    const float *__restrict__ r, const uint32_t *__restrict__ outdeg, uint32_t nseg, float damp, int stop, double eps,
+    int is_reset, uint32_t *counter, uint32_t *all_done, Slot *nxt)
+{
+    const uint32_t k = blockIdx.x;
+    const bool was_done = coef[k].done != 0;
+    __shared__ double red[3][256];
+    __shared__ bool last;
+    double e = 0.0, d = 0.0, s = 0.0;
+    if (is_reset) {
+        for (uint32_t i = goff[k] + threadIdx.x; i < goff[k + 1]; i += 256) {
+            const double x = (double)r[i];
+            s += x;
+            if (!outdeg[i]) d += x;
+        }
+    } else if (!was_done) {
+        const uint32_t n = (uptr[k + 1] - uptr[k]) * kSegUnitSlots;
+        for (uint32_t x = threadIdx.x; x < n; x += 256) {
+            const uint32_t u = uidx[uptr[k] + x / kSegUnitSlots];
+            const double *q = uslots + ((size_t)u * kSegUnitSlots + x % kSegUnitSlots) * 3u;
+            e += q[0];
+            d += q[1];
+            s += q[2];
+        }
+    }
+    red[0][threadIdx.x] = e;
+    red[1][threadIdx.x] = d;
+    red[2][threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+            red[2][threadIdx.x] += red[2][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (is_reset || !was_done) {
+            GStat st = stat[k];
+            const double n = (double)st.n;
+            const double sum = red[2][0] > 0.0 ? red[2][0] : 1.0;
+            GCoef cf;
+            cf.ti = (float)(1.0 / n);
+            cf.fs = (float)((double)damp / sum);
+            cf.fd = (float)(red[1][0] / n);
+            cf.done = 0;
+            st.err = is_reset ? 0.0 : red[0][0];
+            st.dang = red[1][0];
+            st.sum = red[2][0];
+            if (is_reset) {
+                st.iters = 0;
+                st.conv_iter = 0;
+            } else {
+                st.iters += 1;
+                if (stop == GBX_STOP_L1 && st.err < eps) {
+                    cf.done = 1;
+                    st.conv_iter = st.iters;
+                }
+            }
+            coef[k] = cf;
+            stat[k] = st;
+        }
+        __threadfence();
+        last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    __threadfence();
+    const volatile GCoef *vc = coef;
+    const volatile GStat *vs = stat;
+    uint32_t all = 1;
+    double total = 0.0;
+    for (uint32_t j = 0; j < nseg; j++) {
+        all &= vc[j].done ? 1u : 0u;
+        total += vs[j].sum;
+    }
+    *all_done = all;
+    Slot sl{};
+    sl.sum = total;
+    sl.done = all;
+    *nxt = sl;
+    *counter = 0;
+}
+
 __global__ void seg_finalize_kernel(const float *__restrict__ r, const uint32_t *__restrict__ perm,
                                     const uint32_t *__restrict__ gid, const GStat *__restrict__ stat, uint32_t N,
                                     float *__restrict__ out)
@@ -331,6 +430,19 @@ int seg_create(gbx_pr *pr)
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const unsigned per_cu = ctx->tune.ppr_blocks_per_cu;
     st->grid = std::max(1u, std::min((unsigned)std::max(1, ctx->num_cus) * per_cu, p->n_seg_tiles));
+    if (pr->kind == K_SEG) {
+        // the binned iteration: its layout (bins cut at every graph's first
+        // row), the batch's slot pair and unit counter, the units' partials
+        GBX_TRY(dev_alloc(ctx, pr->slots, 2 * sizeof(Slot)));
+        GBX_TRY(dev_alloc(ctx, pr->counter, 64));
+        GBX_TRY(fill_u32(ctx, pr->counter.as<uint32_t>(), 0, 16));
+        GBX_TRY(binned_create(pr));
+        GBX_TRY(dev_alloc(ctx, st->uslots, ((size_t)binned_units(pr) * kSegUnitSlots + 1) * 3 * sizeof(double)));
+        GBX_TRY(dev_alloc(ctx, st->goff, g->seg_off.size() * 4));
+        GBX_HIP(ctx, hipMemcpyAsync(st->goff.p, g->seg_off.data(), g->seg_off.size() * 4, hipMemcpyHostToDevice,
+                                    ctx->stream));
+        GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     return GBX_OK;
 }
 
@@ -339,6 +451,22 @@ int seg_reset(gbx_pr *pr)
     gbx_ctx *ctx = pr->ctx;
     gbx_seg_state *st = pr->seg;
     pr->t = 0;
+    if (pr->kind == K_SEG) {
+        gbx_plan *p = pr->plan;
+        // the coefficient array still holds ti = 1/N_k in every state
+        segbin_reset_rows<<<grid_for(p->N, 256, 16384), 256, 0, ctx->stream>>>(
+            p->row_gid.as<uint32_t>(), p->outdeg.as<uint32_t>(), st->coef.as<GCoef>(), p->N, st->r.as<float>(),
+            st->c[0].as<float>());
+        GBX_HIP(ctx, hipGetLastError());
+        const uint32_t *uptr = nullptr, *uidx = nullptr;
+        binned_graph_units(pr, &uptr, &uidx);
+        segbin_reduce_kernel<<<p->nseg, 256, 0, ctx->stream>>>(
+            st->coef.as<GCoef>(), st->stat.as<GStat>(), st->uslots.as<double>(), uptr, uidx, st->goff.as<uint32_t>(),
+            st->r.as<float>(), p->outdeg.as<uint32_t>(), p->nseg, pr->p.damp, pr->p.stop, pr->p.epsilon, 1,
+            st->flags.as<uint32_t>() + 1, st->flags.as<uint32_t>(), pr->slots.as<Slot>());
+        GBX_HIP(ctx, hipGetLastError());
+        return GBX_OK;
+    }
     PlanView P = seg_plan_view(pr->plan);
     SegView S = seg_view(pr, 1);
     S.c_out = st->c[0].as<float>();
@@ -354,6 +482,21 @@ int seg_launch_iter(gbx_pr *pr)
 {
     gbx_ctx *ctx = pr->ctx;
     gbx_seg_state *st = pr->seg;
+    if (pr->kind == K_SEG) {
+        gbx_plan *p = pr->plan;
+        const uint32_t t = pr->t;
+        GBX_TRY(binned_seg_iter(pr, st->r.as<float>(), st->c[t & 1].as<float>(), st->c[(t + 1) & 1].as<float>(),
+                                st->coef.as<GCoef>(), st->uslots.as<double>()));
+        const uint32_t *uptr = nullptr, *uidx = nullptr;
+        binned_graph_units(pr, &uptr, &uidx);
+        segbin_reduce_kernel<<<p->nseg, 256, 0, ctx->stream>>>(
+            st->coef.as<GCoef>(), st->stat.as<GStat>(), st->uslots.as<double>(), uptr, uidx, st->goff.as<uint32_t>(),
+            st->r.as<float>(), p->outdeg.as<uint32_t>(), p->nseg, pr->p.damp, pr->p.stop, pr->p.epsilon, 0,
+            st->flags.as<uint32_t>() + 1, st->flags.as<uint32_t>(), pr->slots.as<Slot>() + ((t + 1) & 1));
+        GBX_HIP(ctx, hipGetLastError());
+        pr->t++;
+        return GBX_OK;
+    }
     PlanView P = seg_plan_view(pr->plan);
     SegView S = seg_view(pr, pr->t);
     seg_iter_kernel<<<st->grid, kBlock, 0, ctx->stream>>>(P, S);

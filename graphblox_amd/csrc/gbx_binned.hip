@@ -120,6 +120,7 @@ struct BinUnit {
     uint32_t multi;               // accumulator slot of a bin split over units, else kNoMulti
     uint32_t nunits;              // units of the bin
     uint32_t hbeg, hend;          // hot (row, source) pair range of this unit (multiples of 8)
+    uint32_t graph, pad;          // a batch of graphs: the graph of the bin's rows (bins never cross graphs)
 };
 
 // LDS of an accumulate block: int64 row sums, then the hot contributions.
@@ -153,6 +154,7 @@ struct gbx_bplan {
                                       // packed (hot16): u16[Hlen] source | row step << 14
     gbx::DevBuf hot_base;             // packed: u16[Hlen/8] row - bin first row of each group's first pair
     gbx::DevBuf units;                // BinUnit[NU]
+    gbx::DevBuf graph_uptr, graph_uidx;  // a batch of graphs: the units of each graph, ascending
     // run layout: the plan edge behind every src_local entry and every packed
     // hot pair (kNoEdge for pads and fillers), from which a per-edge weight
     // array of the plan is laid out beside them (weighted kinds)
@@ -929,6 +931,8 @@ struct AccumView {
     uint32_t *work;  // next unit (zero between iterations)
     uint32_t NU;     // units
     const float *__restrict__ whot;  // weighted kinds: the weight of every hot pair (0 for fillers)
+    const GCoef *__restrict__ coef;  // a batch of graphs: every graph's coefficients
+    double *__restrict__ uslots;     // a batch of graphs: kSegUnitSlots (err, dang, sum) per unit
 };
 
 // Values as loaded (raw) and as 64-bit fixed point.  Float kinds keep FP32
@@ -1015,11 +1019,88 @@ __device__ __forceinline__ void load_hot8_any(const AccumView &B, uint32_t e, co
     }
 }
 
+// A batch of graphs: the stochastic update of the batch's pull iteration
+// (r' = (1-d)/N_k + d/S_k (sum + D_k/N_k), the same float operations) with
+// the coefficients of the unit's graph, loaded when the unit starts; every
+// wave's (err, dang, sum) of the unit goes to the unit's slots.
+struct SegBinEpi {
+    float *__restrict__ r;
+    float *__restrict__ c_out;
+    const uint32_t *__restrict__ outdeg;
+    const GCoef *__restrict__ coef;
+    double *__restrict__ uslots;
+    float fa, ti, fs, fd;
+    double err, dang, sum;
+    Partial part;  // unused: the partials of a batch are per unit
+    struct In {
+        float r_old;
+        uint32_t od;
+    };
+    __device__ __forceinline__ float init(uint32_t) const { return 0.0f; }
+    // false when the unit's graph has converged (its rows stay as they are)
+    __device__ __forceinline__ bool begin(uint32_t graph)
+    {
+        const GCoef cf = coef[graph];
+        ti = cf.ti;
+        fs = cf.fs;
+        fd = cf.fd;
+        err = dang = sum = 0.0;
+        return cf.done == 0;
+    }
+    __device__ __forceinline__ In load(uint32_t i) const { return In{r[i], outdeg[i]}; }
+    __device__ __forceinline__ void apply(uint32_t i, float acc, const In &in)
+    {
+        const float r_new = fa * ti + fs * (acc + fd);
+        r[i] = r_new;
+        c_out[i] = in.od ? r_new / (float)in.od : 0.0f;
+        err += (double)fabsf(r_new - in.r_old);
+        if (!in.od) dang += (double)r_new;
+        sum += (double)r_new;
+    }
+    __device__ __forceinline__ void operator()(uint32_t i, float acc) { apply(i, acc, load(i)); }
+    __device__ __forceinline__ void put(uint32_t unit, uint32_t slot, double e, double d, double s2) const
+    {
+        double *q = uslots + ((size_t)unit * kSegUnitSlots + slot) * 3u;
+        q[0] = e;
+        q[1] = d;
+        q[2] = s2;
+    }
+    // the wave's sums into its slot of the unit
+    __device__ __forceinline__ void wave_slot(uint32_t unit, uint32_t wave)
+    {
+        double e = err, d = dang, s2 = sum;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            e += __shfl_xor(e, off, 64);
+            d += __shfl_xor(d, off, 64);
+            s2 += __shfl_xor(s2, off, 64);
+        }
+        if ((threadIdx.x & 63u) == 0) put(unit, wave, e, d, s2);
+    }
+};
+
+__device__ __forceinline__ SegBinEpi make_segbin_epi(const AccumView &B, const uint32_t *outdeg, const IterView &A)
+{
+    SegBinEpi e;
+    e.r = A.r;
+    e.c_out = reinterpret_cast<float *>(A.c_out);
+    e.outdeg = outdeg;
+    e.coef = B.coef;
+    e.uslots = B.uslots;
+    e.fa = 1.0f - A.damp;
+    e.ti = e.fs = e.fd = 0.0f;
+    e.err = e.dang = e.sum = 0.0;
+    e.part = Partial{0.0, 0.0, 0.0, 0u, 0u, 0u, 0u};
+    return e;
+}
+
 template <int KIND, class T>
 using EpiOf = std::conditional_t<
+    KIND == K_SEG, SegBinEpi,
+    std::conditional_t<
     KIND == K_SPMV_W, SpmvBinEpi,
     std::conditional_t<is_fixed_kind(KIND), FixEpi<KIND, T>,
-                       std::conditional_t<is_quant_kind(KIND), QuantEpi<KIND>, PrEpi<KIND, T>>>>;
+                       std::conditional_t<is_quant_kind(KIND), QuantEpi<KIND>, PrEpi<KIND, T>>>>>;
 
 constexpr int kAccumUnroll = 4;  // groups of 8 per lane in flight
 constexpr uint32_t kAccumSlab = 64u * 8u * kAccumUnroll;  // V entries a wave takes at a time
@@ -1046,15 +1127,18 @@ __device__ __forceinline__ void claim_unit(const AccumView &B, uint32_t *s_w, Bi
 // s_slab holds the two slab counters (zero on entry, zero again when the
 // update starts).
 template <int KIND, class T>
-__device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B, const T *__restrict__ V,
+__device__ __forceinline__ void accum_unit(const BinUnit &u, uint32_t uw, const AccumView &B, const T *__restrict__ V,
                                            EpiOf<KIND, T> &epi, double scale, double inv, uint32_t *s_slab,
                                            uint32_t *s_w, BinUnit *s_u)
 {
-    const uint32_t nrows = u.row_end - u.row_begin;
+    // a batch of graphs: the units of a converged graph sum and update nothing
+    bool live = true;
+    if constexpr (KIND == K_SEG) live = epi.begin(u.graph);
+    const uint32_t nrows = live ? u.row_end - u.row_begin : 0u;
     const uint32_t stride = kBlock * 8u;
     const T *ch = reinterpret_cast<const T *>(gbx_dyn_lds + B.hot_off);
-    const bool has_hot = u.hend > u.hbeg;
-    if (nrows == 1) {
+    const bool has_hot = live && u.hend > u.hbeg;
+    if (u.row_end - u.row_begin == 1) {
         // one row: lane sums, block sum, then the split-bin combine if any
         __shared__ long long wsum[kBlock / 64];
         long long s = 0;
@@ -1067,14 +1151,15 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
                 for (int i = 0; i < 8; i++) s += fixed_of<KIND>(v[i], scale);
             }
         }
-        for (uint32_t e0 = u.vbeg + threadIdx.x * 8u; e0 < u.vend; e0 += stride * kAccumUnroll) {
+        const uint32_t vend = live ? u.vend : u.vbeg;
+        for (uint32_t e0 = u.vbeg + threadIdx.x * 8u; e0 < vend; e0 += stride * kAccumUnroll) {
             RawVal<KIND> v[kAccumUnroll][8];
 #pragma unroll
             for (int q = 0; q < kAccumUnroll; q++)
-                if (e0 + q * stride < u.vend) load8<KIND, T>(V, e0 + q * stride, v[q]);
+                if (e0 + q * stride < vend) load8<KIND, T>(V, e0 + q * stride, v[q]);
 #pragma unroll
             for (int q = 0; q < kAccumUnroll; q++)
-                if (e0 + q * stride < u.vend) {
+                if (e0 + q * stride < vend) {
 #pragma unroll
                     for (int i = 0; i < 8; i++) s += fixed_of<KIND>(v[q][i], scale);
                 }
@@ -1086,8 +1171,8 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
         if (threadIdx.x == 0) {
             long long tot = 0;
             for (int w = 0; w < kBlock / 64; w++) tot += wsum[w];
-            bool apply = true;
-            if (u.multi != kNoMulti) {
+            bool apply = live;
+            if (live && u.multi != kNoMulti) {
                 atomicAdd(reinterpret_cast<unsigned long long *>(B.acc + u.multi), (unsigned long long)tot);
                 __threadfence();
                 const uint32_t ticket = atomicAdd(B.cnt + u.multi, 1u);
@@ -1103,6 +1188,12 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
                 if constexpr (is_int_kind(KIND)) epi(u.row_begin, (uint64_t)tot);
                 else epi(u.row_begin, (float)((double)tot * inv));
             }
+            if constexpr (KIND == K_SEG) {
+                // the row's partials in slot 0 (zeros for the units of a split bin that do not apply)
+                if (live)
+                    for (uint32_t w = 0; w < kSegUnitSlots; w++)
+                        epi.put(uw, w, w ? 0.0 : epi.err, w ? 0.0 : epi.dang, w ? 0.0 : epi.sum);
+            }
             claim_unit(B, s_w, s_u);
         }
     } else {
@@ -1113,8 +1204,8 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
         const uint32_t lane = threadIdx.x & 63u;
         constexpr int kHotUnroll = hot_unroll<KIND>();
         constexpr uint32_t kHotSlab = hot_slab<KIND>();
-        const uint32_t nh = (u.hend - u.hbeg + kHotSlab - 1) / kHotSlab;
-        const uint32_t nv = (u.vend - u.vbeg + kAccumSlab - 1) / kAccumSlab;
+        const uint32_t nh = live ? (u.hend - u.hbeg + kHotSlab - 1) / kHotSlab : 0u;
+        const uint32_t nv = live ? (u.vend - u.vbeg + kAccumSlab - 1) / kAccumSlab : 0u;
         constexpr uint32_t kq = 64u * 8u;  // entries of one group of 8 per lane
         for (;;) {
             uint32_t k = 0;
@@ -1198,6 +1289,9 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
             // wave 0 resets the slab counters (every wave has taken its last
             // slab) and claims the next unit while the other waves update
             if (threadIdx.x < 2) s_slab[threadIdx.x] = 0;
+            if constexpr (KIND == K_SEG) {
+                if (live && threadIdx.x == 0) epi.put(uw, 0, 0.0, 0.0, 0.0);  // wave 0 updates no row
+            }
             if (threadIdx.x == 0) claim_unit(B, s_w, s_u);
             return;
         }
@@ -1220,6 +1314,9 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, const AccumView &B,
                 }
             }
         }
+        if constexpr (KIND == K_SEG) {
+            if (live) epi.wave_slot(uw, threadIdx.x >> 6);
+        }
     }
 }
 
@@ -1240,6 +1337,7 @@ __global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const ui
     if constexpr (is_fixed_kind(KIND)) epi = make_fix_epi<KIND, T>(outdeg, N, A);
     else if constexpr (is_quant_kind(KIND)) epi = make_quant_epi<KIND>(outdeg, N, A, cur);
     else if constexpr (KIND == K_SPMV_W) epi = make_spmv_epi(A);
+    else if constexpr (KIND == K_SEG) epi = make_segbin_epi(B, outdeg, A);
     else epi = make_epi<KIND, T>(outdeg, N, A, cur);
     // float kinds: every row sum is at most the sum of the magnitudes of its
     // terms, and those of all rows together are at most: the previous rank
@@ -1277,7 +1375,21 @@ __global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const ui
         // thread 0 claims again only after the unit's sums barrier, when
         // every thread has its copy
         const BinUnit u = s_u;
-        accum_unit<KIND, T>(u, B, V, epi, scale, inv, s_slab, &s_w, &s_u);
+        const uint32_t uw = s_w;
+        accum_unit<KIND, T>(u, uw, B, V, epi, scale, inv, s_slab, &s_w, &s_u);
+    }
+    if constexpr (KIND == K_SEG) {
+        // the partials are per unit (reduced per graph by the next kernel); the
+        // last block to leave its unit loop clears the unit counter
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __threadfence();
+            if (atomicAdd(A.counter, 1u) == gridDim.x - 1) {
+                *B.work = 0;
+                *A.counter = 0;
+            }
+        }
+        return;
     }
     const uint32_t iters = cur->iters;
     const double eps = A.eps;
@@ -1473,7 +1585,7 @@ static int build_runs(gbx_ctx *ctx, gbx_bplan *bp, uint32_t Ecold, const uint32_
 }
 
 static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H, uint32_t hot_elem, bool runs,
-                            gbx_bplan *bp)
+                            const std::vector<uint32_t> *goff, gbx_bplan *bp)
 {
     const gbx_tuning &tu = ctx->tune;
     const uint32_t N = p->N, E = p->E;
@@ -1529,14 +1641,19 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
         }
     }
     const uint32_t NC = (uint32_t)clo.size() - 1;
-    // destination bins: <= SB rows and <= EB in-edges each (a longer row is a bin of its own)
+    // destination bins: <= SB rows and <= EB in-edges each (a longer row is a
+    // bin of its own); a batch of graphs also cuts at every graph's first row
     std::vector<uint32_t> vb{0};
     {
         uint64_t e = 0;
         uint32_t start = 0;
+        size_t gk = 1;
         for (uint32_t v = 0; v < N; v++) {
             const uint32_t d = rp[v + 1] - rp[v];
-            if (v > start && (v - start >= SB || e + d > EB)) {
+            bool gstart = false;
+            if (goff)
+                while (gk < goff->size() && (*goff)[gk] <= v) gstart = (*goff)[gk++] == v;
+            if (v > start && (v - start >= SB || e + d > EB || gstart)) {
                 vb.push_back(v);
                 start = v;
                 e = 0;
@@ -1757,7 +1874,10 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     std::vector<BinUnit> units;
     uint64_t vpos = 0;
     uint32_t n_multi = 0;
+    size_t bg = 0;  // graph of the current bin (a batch of graphs)
     for (uint32_t b = 0; b < NB; b++) {
+        if (goff)
+            while (bg + 2 < goff->size() && (*goff)[bg + 1] <= vb[b]) bg++;
         const uint32_t len = bin_len[b], rows = vb[b + 1] - vb[b];
         const uint32_t h0 = hot_start[b], hlen = hot_start[b + 1] - hot_start[b];
         uint32_t nu = 1;
@@ -1775,6 +1895,7 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
             un.hend = h0 + (uint32_t)std::min<uint64_t>((uint64_t)(q + 1) * hstep, hlen);
             un.multi = multi;
             un.nunits = nu;
+            un.graph = (uint32_t)bg;
             units.push_back(un);
         }
         vpos += len;
@@ -1791,6 +1912,22 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
                      [&](const BinUnit &a, const BinUnit &b) { return unit_bytes(a) > unit_bytes(b); });
     bp->NU = (uint32_t)units.size();
     bp->n_multi = n_multi;
+    if (goff) {
+        // the units of each graph, in ascending unit index (the order its partials are summed in)
+        const uint32_t ng = (uint32_t)goff->size() - 1;
+        std::vector<uint32_t> uptr(ng + 1, 0), uidx(units.size());
+        for (const BinUnit &un : units) uptr[un.graph + 1]++;
+        for (uint32_t k = 0; k < ng; k++) uptr[k + 1] += uptr[k];
+        std::vector<uint32_t> fill(uptr.begin(), uptr.end() - 1);
+        for (uint32_t w = 0; w < (uint32_t)units.size(); w++) uidx[fill[units[w].graph]++] = w;
+        GBX_TRY(dev_alloc(ctx, bp->graph_uptr, (size_t)(ng + 1) * 4));
+        GBX_TRY(dev_alloc(ctx, bp->graph_uidx, units.size() * 4 + 4));
+        GBX_HIP(ctx, hipMemcpyAsync(bp->graph_uptr.p, uptr.data(), (size_t)(ng + 1) * 4, hipMemcpyHostToDevice,
+                                    ctx->stream));
+        if (!units.empty())
+            GBX_HIP(ctx, hipMemcpyAsync(bp->graph_uidx.p, uidx.data(), units.size() * 4, hipMemcpyHostToDevice,
+                                        ctx->stream));
+    }
     GBX_TRY(dev_alloc(ctx, bp->units, units.size() * sizeof(BinUnit)));
     GBX_HIP(ctx, hipMemcpyAsync(bp->units.p, units.data(), units.size() * sizeof(BinUnit), hipMemcpyHostToDevice,
                                 ctx->stream));
@@ -1804,12 +1941,13 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     return GBX_OK;
 }
 
-static int build_bplan(gbx_ctx *ctx, gbx_plan *p, const uint32_t (&cut)[8], gbx_bplan **out)
+static int build_bplan(gbx_ctx *ctx, gbx_plan *p, const std::vector<uint32_t> *goff, const uint32_t (&cut)[8],
+                       gbx_bplan **out)
 {
     gbx_bplan *bp = new (std::nothrow) gbx_bplan();
     if (!bp) return set_error(ctx, GBX_ERR_NOMEM, "binned layout: host memory");
     memcpy(bp->cut, cut, sizeof(bp->cut));
-    const int rc = build_bplan_into(ctx, p, cut[4], cut[5], cut[6], cut[7] != 0, bp);
+    const int rc = build_bplan_into(ctx, p, cut[4], cut[5], cut[6], cut[7] != 0, goff, bp);
     if (rc) {
         delete bp;
         return rc;
@@ -1828,6 +1966,7 @@ static void grant_lds(K *kernel, size_t shm)
 
 bool binned_supported(const gbx_pr *pr)
 {
+    if (pr->kind == K_SEG) return pr->seg && pr->plan->segmented && pr->g->N && pr->elem == 4;
     if (pr->batch || pr->seg || pr->plan->segmented || pr->g->N == 0) return false;
     if (is_weighted_kind(pr->kind)) return pr->elem == 4;  // FP32 contributions and weights
     return pr->kind == K_PR_REF || pr->kind == K_PR_STOCH || is_int_kind(pr->kind);
@@ -1896,7 +2035,8 @@ constexpr int scatter_qk() { return is_quant_kind(KIND) ? KIND : 0; }
 template <int KIND, class T>
 static void grant_kind(const gbx_binned_state *st)
 {
-    if constexpr (KIND == K_PR_REF || KIND == K_PR_STOCH) grant_lds(&bin_scatter_runs_kernel<T>, st->shm_scatter);
+    if constexpr (KIND == K_PR_REF || KIND == K_PR_STOCH || KIND == K_SEG)
+        grant_lds(&bin_scatter_runs_kernel<T>, st->shm_scatter);
     if constexpr (is_weighted_kind(KIND)) grant_lds(&bin_scatter_runs_kernel<T, true>, st->shm_scatter);
     else grant_lds(&bin_scatter_kernel<UintOf<sizeof(T)>, scatter_qk<KIND>()>, st->shm_scatter);
     grant_lds(&bin_accum_kernel<KIND, T>, st->shm_accum);
@@ -1925,7 +2065,7 @@ int binned_create(gbx_pr *pr)
     // values (a multiple of 32 is required by the per-block V positions),
     // whole 128-byte lines of FP32 V, 64-byte halves of 16-bit values
     // (measured as fast as whole lines).
-    const bool runs = is_weighted_kind(pr->kind) ||
+    const bool runs = is_weighted_kind(pr->kind) || pr->kind == K_SEG ||
                       ((pr->kind == K_PR_REF || pr->kind == K_PR_STOCH) && tu.binned_edge_values == 0);
     const uint32_t pad = runs ? 8 : 32;
     // hot sources: the tuning value for contributions of up to 4 bytes (one
@@ -1949,7 +2089,7 @@ int binned_create(gbx_pr *pr)
         if (memcmp(c->cut, want, sizeof(want)) == 0) bp = c;
     if (!bp) {
         gbx_bplan *raw = nullptr;
-        GBX_TRY(build_bplan(ctx, p, want, &raw));
+        GBX_TRY(build_bplan(ctx, p, p->segmented ? &pr->g->seg_off : nullptr, want, &raw));
         bp = std::shared_ptr<gbx_bplan>(raw);
         p->bplans.push_back(bp);
     }
@@ -1993,6 +2133,7 @@ int binned_create(gbx_pr *pr)
     case K_PR_REF: pr->elem == 4 ? grant_kind<K_PR_REF, float>(st) : grant_kind<K_PR_REF, uint16_t>(st); break;
     case K_PR_STOCH: pr->elem == 4 ? grant_kind<K_PR_STOCH, float>(st) : grant_kind<K_PR_STOCH, uint16_t>(st); break;
     case K_PR_STOCH_W: grant_kind<K_PR_STOCH_W, float>(st); break;
+    case K_SEG: grant_kind<K_SEG, float>(st); break;
     case K_ADJ: grant_kind<K_ADJ, float>(st); break;
     case K_SPMV_W: grant_kind<K_SPMV_W, float>(st); break;
     case K_FIX64: grant_kind<K_FIX64, uint64_t>(st); break;
@@ -2007,7 +2148,7 @@ int binned_create(gbx_pr *pr)
 }
 
 template <int KIND, class T>
-static void launch_binned(gbx_pr *pr, const IterView &A)
+static void launch_binned(gbx_pr *pr, const IterView &A, const GCoef *coef = nullptr, double *uslots = nullptr)
 {
     using U = UintOf<sizeof(T)>;
     const gbx_binned_state *st = pr->bin;
@@ -2019,7 +2160,7 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
     S.src_local = bp->src_local.as<uint16_t>();
     S.bdst = bp->bdst.as<uint32_t>();
     S.NC = bp->NC;
-    if constexpr (KIND == K_PR_REF || KIND == K_PR_STOCH || is_weighted_kind(KIND)) {
+    if constexpr (KIND == K_PR_REF || KIND == K_PR_STOCH || KIND == K_SEG || is_weighted_kind(KIND)) {
         if (bp->runs) {
             RunsView RV;
             RV.chunk_lo = S.chunk_lo;
@@ -2056,6 +2197,8 @@ static void launch_binned(gbx_pr *pr, const IterView &A)
     B.work = st->work.as<uint32_t>();
     B.NU = bp->NU;
     B.whot = st->whot;
+    B.coef = coef;
+    B.uslots = uslots;
     bin_accum_kernel<KIND, T><<<st->accum_grid, kBlock, st->shm_accum, stream>>>(
         B, pr->plan->outdeg.as<uint32_t>(), pr->plan->N, st->V.as<T>(), A);
 }
@@ -2103,6 +2246,34 @@ int binned_launch_iter(gbx_pr *pr)
     GBX_HIP(pr->ctx, hipGetLastError());
     pr->t++;
     return GBX_OK;
+}
+
+int binned_seg_iter(gbx_pr *pr, float *r, const float *c_in, float *c_out, const GCoef *coef, double *uslots)
+{
+    IterView A;
+    A.r = r;
+    A.c_in = c_in;
+    A.c_out = c_out;
+    A.tele = nullptr;
+    A.slots = pr->slots.as<Slot>();
+    A.partials = nullptr;
+    A.counter = pr->counter.as<uint32_t>();
+    A.t = pr->t;
+    A.hot = 0;
+    A.damp = pr->p.damp;
+    A.eps = pr->p.epsilon;
+    A.stop = pr->p.stop;
+    launch_binned<K_SEG, float>(pr, A, coef, uslots);
+    GBX_HIP(pr->ctx, hipGetLastError());
+    return GBX_OK;
+}
+
+uint32_t binned_units(const gbx_pr *pr) { return pr->bin ? pr->bin->bp->NU : 0u; }
+
+void binned_graph_units(const gbx_pr *pr, const uint32_t **uptr, const uint32_t **uidx)
+{
+    *uptr = pr->bin->bp->graph_uptr.as<uint32_t>();
+    *uidx = pr->bin->bp->graph_uidx.as<uint32_t>();
 }
 
 void binned_state_free(gbx_binned_state *s) { delete s; }

@@ -201,8 +201,12 @@ int exclusive_scan_u32(gbx_ctx *ctx, const uint32_t *in, uint32_t *out, size_t n
         if (d_total) return fill_u32(ctx, d_total, 0, 1);
         return GBX_OK;
     }
-    // scratch: sum of level sizes < n/kScanTile * (1 + 1/2048 + ...) + levels
-    size_t need = n / kScanTile + 64;
+    // scratch: the tile sums of every level above the first
+    size_t need = 16;
+    for (size_t m = n; m > kScanTile;) {
+        m = (m + kScanTile - 1) / kScanTile;
+        need += m;
+    }
     GBX_TRY(dev_reserve(ctx, ctx->scan_tmp, need * sizeof(uint32_t)));
     return scan_level(ctx, in, out, n, d_total, ctx->scan_tmp.as<uint32_t>(),
                       ctx->scan_tmp.bytes / sizeof(uint32_t));

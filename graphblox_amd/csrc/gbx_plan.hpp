@@ -45,6 +45,9 @@ enum Kind : int {
     // (pageRank.c:1872-2202): residual pushes over a frontier (gbx_ddpush.hip)
     K_DDP = 14,
     K_DDPP = 15,
+    // stochastic PageRank of a batch of graphs on the binned iteration: each
+    // graph its own teleport 1/N_k, dangling mass and rank sum (gbx_batch.hip)
+    K_SEG = 16,
 };
 
 constexpr bool is_fixed_kind(int k) { return k >= K_FIX64 && k <= K_FIX8; }
@@ -98,6 +101,24 @@ struct Partial {
     uint32_t active, pad;
     uint32_t cmax, cnmin;
 };
+
+// Batch of graphs: the update coefficients and statistics of graph k.
+struct GCoef {
+    float ti;  // 1/N_k
+    float fs;  // d / S_k
+    float fd;  // D_k / N_k
+    uint32_t done;
+};
+
+struct GStat {
+    double err, dang, sum;
+    uint32_t iters, conv_iter, n, pad;
+};
+
+// Accumulate units of a batch of graphs on the binned iteration: every unit
+// belongs to one graph; each of its waves leaves (err, dang, sum) of the
+// rows it updated in its slot, kSegUnitSlots per unit.
+constexpr uint32_t kSegUnitSlots = 16;
 
 }  // namespace gbx
 
@@ -492,6 +513,13 @@ void seg_state_free(gbx_seg_state *s);
 bool binned_supported(const gbx_pr *pr);
 int binned_create(gbx_pr *pr);       // builds the plan's binned layout on first use
 int binned_launch_iter(gbx_pr *pr);  // scatter + accumulate: one iteration
+// A batch of graphs (K_SEG): one iteration from c_in to c_out, the update
+// with each unit's graph coefficients, every wave's partials into uslots.
+int binned_seg_iter(gbx_pr *pr, float *r, const float *c_in, float *c_out, const GCoef *coef, double *uslots);
+// The layout's accumulate units, and per graph the units it owns
+// (ascending unit index): uptr [nseg+1], uidx [units].
+uint32_t binned_units(const gbx_pr *pr);
+void binned_graph_units(const gbx_pr *pr, const uint32_t **uptr, const uint32_t **uidx);
 void binned_drop_unused(gbx_plan *p);  // drops the plan's layouts no object holds
 void binned_state_free(gbx_binned_state *s);
 

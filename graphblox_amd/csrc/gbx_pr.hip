@@ -994,7 +994,15 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
         pr->plan = plan;
         pr->plan_ref = g->plan[p->direction];
         pr->p = *p;
-        pr->kind = K_PR_STOCH;
+        // the binned iteration from binned_min_vertices (all graphs together), else graph-pure pull tiles
+        if (p->method != GBX_METHOD_AUTO && p->method != GBX_METHOD_PULL && p->method != GBX_METHOD_BINNED) {
+            delete pr;
+            return set_error(ctx, GBX_ERR_ARG, "gbx_pr_create: method %d", p->method);
+        }
+        const bool binned = p->method == GBX_METHOD_BINNED ||
+                            (p->method == GBX_METHOD_AUTO && g->N >= ctx->tune.binned_min_vertices);
+        pr->kind = binned ? K_SEG : K_PR_STOCH;
+        pr->elem = 4;
         int rc = seg_create(pr);
         if (!rc && (hipEventCreate(&pr->ev_begin) != hipSuccess || hipEventCreate(&pr->ev_end) != hipSuccess))
             rc = set_error(ctx, GBX_ERR_HIP, "gbx_pr_create: event creation");

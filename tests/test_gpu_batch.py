@@ -1,5 +1,7 @@
-"""Batches of independent graphs in one launch per iteration, and iterations
-replayed as HIP graphs."""
+"""Batches of independent graphs in one launch per iteration (graph-pure pull
+tiles, or the binned iteration with bins cut at every graph's first row and
+each unit updated with its graph's coefficients), and iterations replayed as
+HIP graphs."""
 import numpy as np
 import pytest
 
@@ -25,12 +27,21 @@ def _members(ctx, sizes, seed=0):
     return out
 
 
-def test_batch_matches_each_graph_alone(gpu_ctx):
-    members = _members(gpu_ctx, [6, 10, 8, 12, 7, 11, 9, 13], seed=1)
+def _tiny(ctx):
+    """A 2-cycle, a star whose leaves have no out-edges (dangling mass) and a
+    single self loop: graphs of 2, 9 and 1 vertices."""
+    return [ctx.graph(np.array([0, 1], np.uint32), np.array([1, 0], np.uint32)),
+            ctx.graph(np.zeros(8, np.uint32), np.arange(1, 9, dtype=np.uint32)),
+            ctx.graph(np.array([0], np.uint32), np.array([0], np.uint32))]
+
+
+@pytest.mark.parametrize("method", [gbx.METHOD_PULL, gbx.METHOD_BINNED])
+def test_batch_matches_each_graph_alone(gpu_ctx, method):
+    members = _members(gpu_ctx, [6, 10, 8, 12, 7, 11, 9, 13], seed=1) + _tiny(gpu_ctx)
     b = gpu_ctx.graph_batch(members)
     off = b.batch_offsets()
     assert len(off) == len(members) + 1 and off[-1] == b.N == sum(m.N for m in members)
-    s = b.session(iterations=200, epsilon=1e-7, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_L1)
+    s = b.session(iterations=200, epsilon=1e-7, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_L1, method=method)
     s.reset()
     s.launch(200)
     ranks, reps = s.read()
@@ -47,13 +58,14 @@ def test_batch_matches_each_graph_alone(gpu_ctx):
         assert l1(rk, alone) <= 2e-6, k
 
 
-def test_batch_of_256_graphs_log_uniform(gpu_ctx):
+@pytest.mark.parametrize("method", [gbx.METHOD_PULL, gbx.METHOD_BINNED])
+def test_batch_of_256_graphs_log_uniform(gpu_ctx, method):
     """Config shape: 256 graphs, N log-uniform, edge factor 8..32 (sizes capped for the test)."""
     rng = np.random.default_rng(256)
     scales = np.clip(np.round(rng.uniform(10, 15, 256)), 10, 15).astype(int)
     members = _members(gpu_ctx, scales, seed=2)
     b = gpu_ctx.graph_batch(members)
-    s = b.session(iterations=100, epsilon=1e-7, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_L1)
+    s = b.session(iterations=100, epsilon=1e-7, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_L1, method=method)
     s.reset()
     s.launch_graph(100)
     ranks, reps = s.read()
@@ -68,7 +80,8 @@ def test_batch_of_256_graphs_log_uniform(gpu_ctx):
         assert l1(ranks[off[k]:off[k + 1]], r64) <= 1e-5
 
 
-@pytest.mark.parametrize("form", ["single_ref", "single_bf16", "batched_vectors", "graph_batch"])
+@pytest.mark.parametrize("form", ["single_ref", "single_bf16", "batched_vectors", "graph_batch",
+                                  "graph_batch_binned"])
 def test_graph_replay_is_bit_identical_to_eager(gpu_ctx, golden, form):
     z = golden("rmat_s12")
     g = gpu_ctx.graph(z["edges_src"], z["edges_dst"], sort_mode=1)
@@ -82,7 +95,8 @@ def test_graph_replay_is_bit_identical_to_eager(gpu_ctx, golden, form):
                           stop=gbx.STOP_NONE)
     else:
         g = gpu_ctx.graph_batch(_members(gpu_ctx, [8, 11, 9], seed=3) + [g])
-        s = g.session(iterations=30, epsilon=0.0, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_NONE)
+        method = gbx.METHOD_BINNED if form == "graph_batch_binned" else gbx.METHOD_AUTO
+        s = g.session(iterations=30, epsilon=0.0, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_NONE, method=method)
     s.reset()
     s.launch(10)
     eager, _ = s.read()
@@ -104,3 +118,29 @@ def test_graph_replay_is_bit_identical_to_eager(gpu_ctx, golden, form):
     s.launch_graph(5)
     r, _ = s.read()
     np.testing.assert_array_equal(r, eager10)
+
+
+def test_binned_batch_freezes_converged_graphs(gpu_ctx):
+    """Each graph stops at its own L1 rule: after its stop its ranks stay as
+    they were, its iteration count is its own, and the others run on; the
+    stopping iterations agree with the pull batch's within one."""
+    members = _members(gpu_ctx, [12, 7, 13, 9], seed=9) + _tiny(gpu_ctx)
+    b = gpu_ctx.graph_batch(members)
+    off = b.batch_offsets()
+    out = {}
+    for method in (gbx.METHOD_PULL, gbx.METHOD_BINNED):
+        s = b.session(iterations=300, epsilon=1e-7, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_L1, method=method)
+        s.reset()
+        s.launch(300)
+        r300, reps = s.read()
+        s.reset()
+        s.launch(max(rp["executed"] for rp in reps) + 5)
+        rmore, reps2 = s.read()
+        np.testing.assert_array_equal(r300, rmore)  # converged graphs do not move
+        assert [rp["executed"] for rp in reps] == [rp["executed"] for rp in reps2]
+        out[method] = (r300, reps)
+        s.free()
+    (rp_, reps_p), (rb, reps_b) = out[gbx.METHOD_PULL], out[gbx.METHOD_BINNED]
+    for k in range(len(members)):
+        assert reps_b[k]["converged"] == 1 and abs(reps_b[k]["executed"] - reps_p[k]["executed"]) <= 1, k
+        assert l1(rb[off[k]:off[k + 1]], rp_[off[k]:off[k + 1]]) <= 1e-6, k
