@@ -347,9 +347,11 @@ def forms_s16(gbx, O, ctx, args):
 
 def forms_batch(gbx, O, ctx, args):
     """256 independent R-MAT graphs, 2^10..2^18 vertices (log-uniform), edge
-    factor 8..32: every graph's stochastic PageRank in one launch per
-    iteration, the iterations replayed as one captured HIP graph, launch
-    shape autotuned on the batch."""
+    factor 8..32: every graph's stochastic PageRank in the same kernels each
+    iteration (the binned iteration at this size: scatter, accumulate with
+    each unit's graph coefficients, per-graph reduction), the iterations
+    replayed as one captured HIP graph, the layout cut autotuned on the
+    batch; the pull tiles beside it."""
     rng = np.random.default_rng(256)
     scales = rng.integers(10, 19, 256)
     efs = rng.integers(8, 33, 256)
@@ -380,17 +382,26 @@ def forms_batch(gbx, O, ctx, args):
     _, creps = c.read()
     conv_ms = (time.perf_counter() - t0) * 1e3
     c.free()
-    rec = form_record("256 R-MAT graphs (2^10..2^18 vertices, edge factor 8..32), one launch per iteration, "
-                      "replayed as a HIP graph", E, N, 1, km, wm, 8 * E + 16 * N + 4 * 256,
+    rec = form_record("256 R-MAT graphs (2^10..2^18 vertices, edge factor 8..32), all graphs in each iteration's "
+                      "kernels (the library's default method for this size), replayed as a HIP graph", E, N, 1, km,
+                      wm, 8 * E + 16 * N + 4 * 256,
                       l1_vs_fp64=max(e["l1_vs_fp64"] for e in errs),
                       max_rel_vs_fp64=max(e["max_rel_vs_fp64"] for e in errs),
                       fp64_checked_graphs="4 of 256 (0, 85, 170, 255)",
-                      autotuned={"bin_thresholds": list(best.bin_thresholds), "hub_chunk": best.hub_chunk,
-                                 "ppr_blocks_per_cu": best.ppr_blocks_per_cu, "trial_ms": round(tuned_ms, 5)},
+                      autotuned={**{k: v for k, v in best.as_dict().items() if k.startswith("binned")},
+                                 "trial_ms": round(tuned_ms, 5)},
                       convergence={"epsilon": 1e-7, "wall_ms": round(conv_ms, 3),
                                    "iterations_max": max(r["executed"] for r in creps),
                                    "all_converged": all(r["converged"] for r in creps)})
     ctx.set_tuning(tune0)
+    sp = b.session(iterations=args.steps, epsilon=0.0, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_NONE,
+                   method=gbx.METHOD_PULL)
+    kp, wp, rp_, _ = time_session(sp, args.warmup, args.steps, graph_replay=True)
+    sp.free()
+    rec["pull_tiles"] = {"ms_per_iter": round(wp, 5), "kernel_ms": round(kp, 5),
+                         "gteps": round(E / (wp * 1e-3) / 1e9, 2),
+                         "hbm_frac": round((8 * E + 16 * N + 4 * 256) / (kp * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "l1_vs_default_method": float(np.abs(rp_.astype(np.float64) - ranks).sum() / 256)}
     b.free()
     for m in members:
         m.free()
