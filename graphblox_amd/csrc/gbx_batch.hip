@@ -41,8 +41,8 @@ struct gbx_seg_state {
     gbx::DevBuf r, c[2], coef, stat, slots, flags, out;
     uint32_t grid = 0;
     // on the binned iteration (K_SEG): the units' wave partials and every
-    // graph's first plan row
-    gbx::DevBuf uslots, goff;
+    // graph's count of rows without out-edges
+    gbx::DevBuf uslots, gdang;
 };
 
 namespace gbx {
@@ -242,17 +242,34 @@ __global__ void segbin_reset_rows(const uint32_t *__restrict__ gid, const uint32
     }
 }
 
-// One block per graph: at the reset, the rank sum and dangling mass of the
-// graph's rows [goff[k], goff[k+1]); after an iteration, the sums of its
-// units' wave slots in ascending unit order.  Then the next coefficients and
-// the stop rule as seg_reduce_kernel; the last block writes the batch's slot
-// for the next iteration (done when every graph is, the rank sum over all
-// graphs for the accumulate's fixed-point scale).
+// Rows without out-edges per graph (wave-aggregated where a wave's rows are
+// of one graph).
+__global__ void segbin_dangling_counts(const uint32_t *__restrict__ gid, const uint32_t *__restrict__ outdeg,
+                                       uint32_t N, uint32_t *__restrict__ cnt)
+{
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < N; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        const bool in = i < N;
+        const uint32_t g = in ? gid[i] : 0xFFFFFFFFu;
+        const bool dang = in && outdeg[i] == 0;
+        const uint32_t g0 = __builtin_amdgcn_readfirstlane(g);
+        const uint64_t same = __ballot(dang && g == g0);
+        if ((threadIdx.x & 63u) == 0 && same) atomicAdd(cnt + g0, (uint32_t)__popcll(same));
+        if (dang && g != g0) atomicAdd(cnt + g, 1u);
+    }
+}
+
+// One block per graph: at the reset, the rank sum and dangling mass of
+// r0 = 1/N_k (N_k and the dangling count times the float 1/N_k, exact in
+// FP64, as a sum of the rows' values is); after an iteration, the sums of
+// its units' wave slots in ascending unit order.  Then the next coefficients
+// and the stop rule as seg_reduce_kernel; the last block writes the batch's
+// slot for the next iteration (done when every graph is, the rank sum over
+// all graphs for the accumulate's fixed-point scale).
 __global__ void __launch_bounds__(256) segbin_reduce_kernel(
     GCoef *__restrict__ coef, GStat *__restrict__ stat, const double *__restrict__ uslots,
-    const uint32_t *__restrict__ uptr, const uint32_t *__restrict__ uidx, const uint32_t *__restrict__ goff,
-    const float *__restrict__ r, const uint32_t *__restrict__ outdeg, uint32_t nseg, float damp, int stop, double eps,
-    int is_reset, uint32_t *counter, uint32_t *all_done, Slot *nxt)
+    const uint32_t *__restrict__ uptr, const uint32_t *__restrict__ uidx, const uint32_t *__restrict__ gdang,
+    uint32_t nseg, float damp, int stop, double eps, int is_reset, uint32_t *counter, uint32_t *all_done, Slot *nxt)
 {
     const uint32_t k = blockIdx.x;
     const bool was_done = coef[k].done != 0;
@@ -260,10 +277,10 @@ __global__ void __launch_bounds__(256) segbin_reduce_kernel(
     __shared__ bool last;
     double e = 0.0, d = 0.0, s = 0.0;
     if (is_reset) {
-        for (uint32_t i = goff[k] + threadIdx.x; i < goff[k + 1]; i += 256) {
-            const double x = (double)r[i];
-            s += x;
-            if (!outdeg[i]) d += x;
+        if (threadIdx.x == 0) {
+            const double t0 = (double)coef[k].ti;
+            s = (double)stat[k].n * t0;
+            d = (double)gdang[k] * t0;
         }
     } else if (!was_done) {
         const uint32_t n = (uptr[k + 1] - uptr[k]) * kSegUnitSlots;
@@ -438,9 +455,11 @@ int seg_create(gbx_pr *pr)
         GBX_TRY(fill_u32(ctx, pr->counter.as<uint32_t>(), 0, 16));
         GBX_TRY(binned_create(pr));
         GBX_TRY(dev_alloc(ctx, st->uslots, ((size_t)binned_units(pr) * kSegUnitSlots + 1) * 3 * sizeof(double)));
-        GBX_TRY(dev_alloc(ctx, st->goff, g->seg_off.size() * 4));
-        GBX_HIP(ctx, hipMemcpyAsync(st->goff.p, g->seg_off.data(), g->seg_off.size() * 4, hipMemcpyHostToDevice,
-                                    ctx->stream));
+        GBX_TRY(dev_alloc(ctx, st->gdang, (size_t)p->nseg * 4 + 4));
+        GBX_TRY(fill_u32(ctx, st->gdang.as<uint32_t>(), 0, p->nseg));
+        segbin_dangling_counts<<<grid_for(p->N, 256, 16384), 256, 0, ctx->stream>>>(
+            p->row_gid.as<uint32_t>(), p->outdeg.as<uint32_t>(), p->N, st->gdang.as<uint32_t>());
+        GBX_HIP(ctx, hipGetLastError());
         GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     return GBX_OK;
@@ -461,8 +480,8 @@ int seg_reset(gbx_pr *pr)
         const uint32_t *uptr = nullptr, *uidx = nullptr;
         binned_graph_units(pr, &uptr, &uidx);
         segbin_reduce_kernel<<<p->nseg, 256, 0, ctx->stream>>>(
-            st->coef.as<GCoef>(), st->stat.as<GStat>(), st->uslots.as<double>(), uptr, uidx, st->goff.as<uint32_t>(),
-            st->r.as<float>(), p->outdeg.as<uint32_t>(), p->nseg, pr->p.damp, pr->p.stop, pr->p.epsilon, 1,
+            st->coef.as<GCoef>(), st->stat.as<GStat>(), st->uslots.as<double>(), uptr, uidx, st->gdang.as<uint32_t>(),
+            p->nseg, pr->p.damp, pr->p.stop, pr->p.epsilon, 1,
             st->flags.as<uint32_t>() + 1, st->flags.as<uint32_t>(), pr->slots.as<Slot>());
         GBX_HIP(ctx, hipGetLastError());
         return GBX_OK;
@@ -490,8 +509,8 @@ int seg_launch_iter(gbx_pr *pr)
         const uint32_t *uptr = nullptr, *uidx = nullptr;
         binned_graph_units(pr, &uptr, &uidx);
         segbin_reduce_kernel<<<p->nseg, 256, 0, ctx->stream>>>(
-            st->coef.as<GCoef>(), st->stat.as<GStat>(), st->uslots.as<double>(), uptr, uidx, st->goff.as<uint32_t>(),
-            st->r.as<float>(), p->outdeg.as<uint32_t>(), p->nseg, pr->p.damp, pr->p.stop, pr->p.epsilon, 0,
+            st->coef.as<GCoef>(), st->stat.as<GStat>(), st->uslots.as<double>(), uptr, uidx, st->gdang.as<uint32_t>(),
+            p->nseg, pr->p.damp, pr->p.stop, pr->p.epsilon, 0,
             st->flags.as<uint32_t>() + 1, st->flags.as<uint32_t>(), pr->slots.as<Slot>() + ((t + 1) & 1));
         GBX_HIP(ctx, hipGetLastError());
         pr->t++;
