@@ -734,7 +734,7 @@ struct W8 {
 };
 
 template <class T, bool W>
-__device__ __forceinline__ RunStep runs_sum(const T *cl, uint4 w, const W8 &wt, uint32_t vbase, bool leader)
+__device__ __forceinline__ RunStep runs_sum(const float *cl, uint4 w, const W8 &wt, uint32_t vbase, bool leader)
 {
     RunStep st;
     const uint32_t ew[4] = {w.x, w.y, w.z, w.w};
@@ -742,7 +742,7 @@ __device__ __forceinline__ RunStep runs_sum(const T *cl, uint4 w, const W8 &wt, 
 #pragma unroll
     for (int i = 0; i < 8; i++) {
         const uint32_t h = (i & 1) ? __builtin_amdgcn_ubfe(ew[i >> 1], 16, 15) : (ew[i >> 1] & (kRunEnd - 1u));
-        x[i] = f32_of(cl[h]);
+        x[i] = cl[h];
     }
     if constexpr (W) {
         const uint32_t wb[8] = {wt.a.x, wt.a.y, wt.a.z, wt.a.w, wt.b.x, wt.b.y, wt.b.z, wt.b.w};
@@ -805,6 +805,35 @@ __device__ __forceinline__ void runs_out(const float *stage, const RunStep &st, 
     }
 }
 
+// n contributions into the scatter's FP32 LDS copy: FP32 as they are, BF16
+// widened (exact) 8 at a time, so the gathers of the run layout read FP32.
+template <class T>
+__device__ __forceinline__ void copy_to_lds_f32(float *dst, const T *src, uint32_t n)
+{
+    if constexpr (sizeof(T) == 4) {
+        copy_to_lds<float, kRunsBlock>(dst, reinterpret_cast<const float *>(src), n);
+    } else {
+        const uint32_t nv = n / 8;
+        const uint4 *s8 = reinterpret_cast<const uint4 *>(src);
+        for (uint32_t i0 = threadIdx.x; i0 < nv; i0 += kRunsBlock * kCopyAhead) {
+            // all loads first (indexes past the last group repeat it: same value, same address)
+            uint4 w[kCopyAhead];
+#pragma unroll
+            for (int k = 0; k < kCopyAhead; k++) w[k] = s8[min(i0 + k * kRunsBlock, nv - 1u)];
+#pragma unroll
+            for (int k = 0; k < kCopyAhead; k++) {
+                const uint32_t u[4] = {w[k].x, w[k].y, w[k].z, w[k].w};
+                float4 *d4 = reinterpret_cast<float4 *>(dst + 8u * min(i0 + k * kRunsBlock, nv - 1u));
+                d4[0] = make_float4(__uint_as_float(u[0] << 16), __uint_as_float(u[0] & 0xFFFF0000u),
+                                    __uint_as_float(u[1] << 16), __uint_as_float(u[1] & 0xFFFF0000u));
+                d4[1] = make_float4(__uint_as_float(u[2] << 16), __uint_as_float(u[2] & 0xFFFF0000u),
+                                    __uint_as_float(u[3] << 16), __uint_as_float(u[3] & 0xFFFF0000u));
+            }
+        }
+        for (uint32_t i = nv * 8 + threadIdx.x; i < n; i += kRunsBlock) dst[i] = f32_of(src[i]);
+    }
+}
+
 // Run layout (float kinds): the groups of `src_local` of one chunk, from g0
 // to g1, by the whole block: the chunk's contributions (sources c0..c1) are
 // copied to LDS, then each wave takes steps of 64 consecutive groups.  The
@@ -813,10 +842,10 @@ __device__ __forceinline__ void runs_out(const float *stage, const RunStep &st, 
 // valid group and stage nothing).
 template <class T, bool W>
 __device__ __forceinline__ void scatter_runs_piece(const RunsView &S, const T *__restrict__ c_in, T *__restrict__ V,
-                                                   T *cl, float *stage, uint32_t dummy_off, uint32_t c0, uint32_t c1,
+                                                   float *cl, float *stage, uint32_t dummy_off, uint32_t c0, uint32_t c1,
                                                    uint32_t g0, uint32_t g1)
 {
-    copy_to_lds<T, kRunsBlock>(cl, c_in + c0, c1 - c0);
+    copy_to_lds_f32<T>(cl, c_in + c0, c1 - c0);
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t padw = S.pad | (S.pad << 16);
@@ -911,11 +940,11 @@ __global__ void __launch_bounds__(kRunsBlock) bin_scatter_runs_kernel(RunsView S
     const uint4 pc = S.pieces[blockIdx.x];
     asm volatile("" ::"s"(pc.x), "s"(pc.y), "s"(pc.z), "s"(pc.w));
     if (done) return;
-    T *cl = reinterpret_cast<T *>(gbx_dyn_lds);
+    float *cl = reinterpret_cast<float *>(gbx_dyn_lds);  // the chunk's contributions in FP32
     float *stage = reinterpret_cast<float *>(gbx_dyn_lds + S.stage_off) + (threadIdx.x >> 6) * kRunStage;
     const uint32_t dummy_off = S.dummy + ((blockIdx.x * (kRunsBlock / 64) + (threadIdx.x >> 6)) % kRunDummyAreas) *
                                              kRunDummyStride + (threadIdx.x & 63u) * (uint32_t)sizeof(T);
-    if (threadIdx.x == 0) cl[S.pad] = T(0);  // the pads' zero, past the chunk's contributions
+    if (threadIdx.x == 0) cl[S.pad] = 0.0f;  // the pads' zero, past the chunk's contributions
     scatter_runs_piece<T, W>(S, c_in, V, cl, stage, dummy_off, pc.x, pc.y, pc.z, pc.w);
 }
 
@@ -2123,9 +2152,9 @@ int binned_create(gbx_pr *pr)
     st->accum_grid = std::max(1u, std::min(bp->NU, (uint32_t)std::max(1, ctx->num_cus)));
     if (bp->NU > 16384) GBX_TRY(dev_alloc(ctx, pr->partials, (size_t)bp->NU * sizeof(Partial)));
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    // run layout: the chunk's contributions, a zero at index SC, then the
-    // per-wave staging of run sums and V positions
-    st->stage_off = bp->runs ? (uint32_t)((((size_t)bp->SC + 1) * vel + 15) & ~(size_t)15) : 0;
+    // run layout: the chunk's contributions in FP32 (BF16 widened), a zero at
+    // index SC, then the per-wave staging of run sums and V positions
+    st->stage_off = bp->runs ? (uint32_t)((((size_t)bp->SC + 1) * 4u + 15) & ~(size_t)15) : 0;
     st->shm_scatter = bp->runs ? st->stage_off + (size_t)(kRunsBlock / 64) * kRunStage * sizeof(float)
                                : (size_t)bp->cut[3] * vel;
     st->shm_accum = (size_t)bp->SB * 8 + (size_t)(bp->hot16 ? kHot16Slots : bp->H) * vel;

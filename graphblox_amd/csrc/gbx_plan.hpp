@@ -298,10 +298,8 @@ __device__ __forceinline__ float load_val(const uint16_t *p, uint32_t i)
 }
 __device__ __forceinline__ uint16_t f32_to_bf16(float v)
 {
-    // round to nearest even; values here are finite
-    uint32_t u = __float_as_uint(v);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    // round to nearest even: one v_cvt_pk_bf16_f32 on gfx950
+    return __builtin_bit_cast(uint16_t, (__bf16)v);
 }
 __device__ __forceinline__ void store_val(float *p, uint32_t i, float v) { st_stream(p + i, v); }
 __device__ __forceinline__ void store_val(uint16_t *p, uint32_t i, float v) { p[i] = f32_to_bf16(v); }
