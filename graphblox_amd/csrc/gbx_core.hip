@@ -247,7 +247,7 @@ void tuning_defaults(gbx_tuning *t)
     t->ppr_chunk = 2048;
     t->ppr_blocks_per_cu = 2;
     t->ppr_vectors_per_lane = 1;
-    t->binned_min_vertices = 1u << 21;
+    t->binned_min_vertices = 786432;  // measured crossover: pull faster at R-MAT s19, binned from s20
     t->binned_chunk_edges = 0;  // automatic
     t->binned_bin_edges = 0;
     t->binned_bin_rows = 20000;
