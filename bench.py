@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes
+import functools
 import json
 import os
 import sys
@@ -61,8 +62,11 @@ def traffic_from_profiles(workload_key: str):
     return t.get("hbm_bytes_per_launch")
 
 
+@functools.lru_cache(maxsize=None)
 def host_cores():
-    """Logical CPUs of the machine, CPUs this process may run on, physical cores."""
+    """Logical CPUs of the machine, CPUs this process may run on, physical cores.
+    Counted once, at start-up: once an OpenMP region has run with
+    OMP_PROC_BIND set, the runtime has bound this thread to one place."""
     logical = os.cpu_count() or 1
     try:
         allowed = len(os.sched_getaffinity(0))
@@ -122,12 +126,12 @@ def cpu_baseline(fo, io, iterations: int, threads: int, gpu_ranks=None):
     from oracle import oracle as O
 
     N, E = fo.N, fo.E
+    logical, allowed, physical = host_cores()
     t0 = time.perf_counter()
     out = O.pr_pull_f32(fo, io, iterations, 0.0, nthreads=threads, timings=True)
     wall = time.perf_counter() - t0
     ts = np.asarray(out["iter_seconds"], np.float64)
     med = float(np.median(ts[1:])) if ts.size > 1 else float(ts[0])
-    logical, allowed, physical = host_cores()
     rec = dict(value=round(E / med / 1e9, 4), unit="GTEPS", cores=threads, kind="port",
                sample=f"{iterations} iterations of pull PageRank (FP32, reference semantics, the oracle's "
                       f"OpenMP restatement of pageRankPullGraphCSR) on the same R-MAT graph (N={N}, E={E}), "
@@ -260,6 +264,12 @@ def forms_s22(gbx, O, ctx, args):
                                                            8 * Eb + 16 * Nb + 4, **fp64_errors(rr, r64s))
     rng = np.random.default_rng(SEED)
     sets = [rng.choice(Nb, int(rng.integers(1, 1001)), replace=False) for _ in range(32)]
+    # the batched form's lanes per row, hub cut and blocks per CU, searched
+    # by the library's autotuner on this graph with 32 vectors
+    t0 = time.perf_counter()
+    best, _ = ctx.autotune(gb, 32, 5, iterations=10, epsilon=0.0, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_NONE)
+    ppr_tuned = {k: v for k, v in best.as_dict().items() if k.startswith("ppr")}
+    log(f"batched form autotuned in {time.perf_counter() - t0:.2f} s: {ppr_tuned}")
     kb, wb, rb, sb = time_form(gb, args.warmup, args.steps, num_vectors=32, seed_sets=sets)
     sb.free()
     errs = []
@@ -272,7 +282,7 @@ def forms_s22(gbx, O, ctx, args):
         "32 personalised vectors, one multi-vector launch", Eb, Nb, 32, kb, wb,
         4 * (Nb + 1) + 4 * Nb + 4 * Eb + 32 * (4 * Eb + 8 * Nb), base32,
         l1_vs_fp64=max(e["l1_vs_fp64"] for e in errs), max_rel_vs_fp64=max(e["max_rel_vs_fp64"] for e in errs),
-        fp64_checked_vectors="4 of 32 (0, 8, 16, 24)")
+        fp64_checked_vectors="4 of 32 (0, 8, 16, 24)", autotuned=ppr_tuned)
     out["batched_ppr32"]["speedup_vs_binned_fp32_same_graph"] = round(
         out["batched_ppr32"]["gteps"] / out["fp32_binned_same_graph_as_batched"]["gteps"], 3)
     # stochastic PageRank to L1 < 1e-6 (at most 100 iterations), wall time from reset to read-back
@@ -552,6 +562,7 @@ def convergence(gbx, ctx, args, g, method, m):
 
 
 def main():
+    host_cores()  # before any OpenMP region binds this thread
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)

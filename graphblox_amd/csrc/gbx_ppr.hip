@@ -9,9 +9,11 @@
 // sum of vector q.  Ranks and contributions are stored vertex-major,
 // [vertex][V] with V = k rounded up to a power of two, so one gather of a
 // source vertex reads all k contributions from one or two cache lines.
-// A team of V lanes (lane = vector) walks one row; hub rows are split into
-// edge chunks whose per-team partial rows are reduced in LDS and combined in
-// chunk order by the last chunk to finish.  Per-vector FP64 block partials
+// A team of V/VPL lanes (VPL vectors a lane) walks one row, its gathers
+// issued a batch at a time with the next batch's columns, and the next row's
+// pointers, first columns and update inputs loaded during the current row;
+// hub rows are split into edge chunks whose per-team partial rows are
+// reduced in LDS and combined in chunk order by the last chunk to finish.  Per-vector FP64 block partials
 // are reduced by a second small kernel that also evaluates the stop rule.
 #include <algorithm>
 #include <cmath>
@@ -89,34 +91,60 @@ __device__ __forceinline__ void load_vec(const float *__restrict__ p, float (&x)
     }
 }
 
+template <int VPL>
+__device__ __forceinline__ void store_vec(float *__restrict__ p, const float (&x)[VPL])
+{
+    if constexpr (VPL == 1) {
+        p[0] = x[0];
+    } else if constexpr (VPL == 2) {
+        *reinterpret_cast<float2 *>(p) = make_float2(x[0], x[1]);
+    } else {
+        *reinterpret_cast<float4 *>(p) = make_float4(x[0], x[1], x[2], x[3]);
+    }
+}
+
+// Gathers in flight per lane: GBX_UNROLL, half as many with four vectors a
+// lane (the same bytes in flight, half the registers).
+template <int VPL>
+constexpr uint32_t team_batch() { return VPL >= 4 ? GBX_UNROLL / 2 : GBX_UNROLL; }
+
 // Sum of one row's contributions for this lane's VPL vectors, edges
-// j, j+step, ... in order; GBX_UNROLL gathers in flight.
+// j, j+step, ... in order, U = team_batch() at a time: each batch's U
+// gathers (predicated for a partial batch) overlap the next batch's column
+// loads, so a batch costs about one round trip.
 template <int V, int VPL>
 __device__ __forceinline__ void team_row_sum(uint32_t j, uint32_t end, uint32_t step,
                                              const uint32_t *__restrict__ col,
                                              const float *__restrict__ C, uint32_t q, float (&acc)[VPL])
 {
-    constexpr uint32_t U = GBX_UNROLL;
+    constexpr uint32_t U = team_batch<VPL>();
 #pragma unroll
     for (int v = 0; v < VPL; v++) acc[v] = 0.0f;
     const float *base = C + (size_t)q * VPL;
-    for (; j + (U - 1) * step < end; j += U * step) {
-        uint32_t c[U];
+    uint32_t c[U];
+#pragma unroll
+    for (uint32_t k = 0; k < U; k++) c[k] = j + k * step < end ? col[j + k * step] : 0u;
+    for (; j < end; j += U * step) {
         float x[U][VPL];
 #pragma unroll
-        for (uint32_t k = 0; k < U; k++) c[k] = col[j + k * step];
+        for (uint32_t k = 0; k < U; k++) {
+            if (j + k * step < end) {
+                load_vec<VPL>(base + (size_t)c[k] * V, x[k]);
+            } else {
 #pragma unroll
-        for (uint32_t k = 0; k < U; k++) load_vec<VPL>(base + (size_t)c[k] * V, x[k]);
+                for (int v = 0; v < VPL; v++) x[k][v] = 0.0f;
+            }
+        }
+        const uint32_t jn = j + U * step;
+        if (jn < end) {
+#pragma unroll
+            for (uint32_t k = 0; k < U; k++) c[k] = jn + k * step < end ? col[jn + k * step] : 0u;
+        }
 #pragma unroll
         for (uint32_t k = 0; k < U; k++)
+            if (j + k * step < end)
 #pragma unroll
-            for (int v = 0; v < VPL; v++) acc[v] += x[k][v];
-    }
-    for (; j < end; j += step) {
-        float x[VPL];
-        load_vec<VPL>(base + (size_t)col[j] * V, x);
-#pragma unroll
-        for (int v = 0; v < VPL; v++) acc[v] += x[v];
+                for (int v = 0; v < VPL; v++) acc[v] += x[k][v];
     }
 }
 
@@ -141,69 +169,142 @@ __global__ void __launch_bounds__(kBlock) ppr_iter_kernel(BPlan P, BatchView A)
         isz[v] = A.mask ? A.inv_size[q * VPL + v] : uni;
         perr[v] = pdang[v] = psum[v] = 0.0;
     }
-    auto epi = [&](uint32_t i, const float (&acc)[VPL]) {
-        const uint32_t od = P.outdeg[i];
-        const uint64_t m = A.mask ? A.mask[i] : 0ull;
+    // update inputs of row i: out-degree, seed bits, this lane's old ranks
+    struct RowIn {
+        uint32_t od;
+        uint64_t m;
+        float r_old[VPL];
+    };
+    auto row_in = [&](uint32_t i) {
+        RowIn in;
+        in.od = P.outdeg[i];
+        in.m = A.mask ? A.mask[i] : 0ull;
+        load_vec<VPL>(A.R + (size_t)i * V + q * VPL, in.r_old);
+        return in;
+    };
+    auto epi = [&](uint32_t i, const RowIn &in, const float (&acc)[VPL]) {
+        float r_new[VPL], c_new[VPL];
 #pragma unroll
         for (int v = 0; v < VPL; v++) {
             const uint32_t qv = q * VPL + v;
-            const size_t o = (size_t)i * V + qv;
-            const float r_old = A.R[o];
-            const float t = A.mask ? (((m >> qv) & 1ull) ? isz[v] : 0.0f) : uni;
-            const float r_new = fa * t + fs[v] * (acc[v] + fd[v]);
-            A.R[o] = r_new;
-            A.C_out[o] = od ? r_new / (float)od : 0.0f;
-            perr[v] += (double)fabsf(r_new - r_old);
-            if (!od) pdang[v] += (double)r_new;
-            psum[v] += (double)r_new;
+            const float t = A.mask ? (((in.m >> qv) & 1ull) ? isz[v] : 0.0f) : uni;
+            r_new[v] = fa * t + fs[v] * (acc[v] + fd[v]);
+            c_new[v] = in.od ? r_new[v] / (float)in.od : 0.0f;
+            perr[v] += (double)fabsf(r_new[v] - in.r_old[v]);
+            if (!in.od) pdang[v] += (double)r_new[v];
+            psum[v] += (double)r_new[v];
         }
+        const size_t o = (size_t)i * V + q * VPL;
+        store_vec<VPL>(A.R + o, r_new);
+        store_vec<VPL>(A.C_out + o, c_new);
     };
     __shared__ float red[kBlock * VPL];
     __shared__ bool last_chunk;
-    for (uint32_t tile = blockIdx.x; tile < P.n_tiles; tile += gridDim.x) {
-        if (tile < P.n_chunks) {
-            const HubChunk hc = P.chunks[tile];
-            float acc[VPL];
-            team_row_sum<V, VPL>(hc.ebeg + team, hc.eend, TPB, P.col, A.C_in, q, acc);
+    // hub chunks (the first n_chunks tiles), then rows
+    uint32_t tile = blockIdx.x;
+    for (; tile < P.n_tiles && tile < P.n_chunks; tile += gridDim.x) {
+        const HubChunk hc = P.chunks[tile];
+        float acc[VPL];
+        team_row_sum<V, VPL>(hc.ebeg + team, hc.eend, TPB, P.col, A.C_in, q, acc);
 #pragma unroll
-            for (int v = 0; v < VPL; v++) red[threadIdx.x * VPL + v] = acc[v];
-            __syncthreads();
-            if (team == 0) {
+        for (int v = 0; v < VPL; v++) red[threadIdx.x * VPL + v] = acc[v];
+        __syncthreads();
+        if (team == 0) {
 #pragma unroll
-                for (int v = 0; v < VPL; v++) {
-                    float s = red[q * VPL + v];
-                    for (uint32_t t2 = 1; t2 < TPB; t2++) s += red[(t2 * L + q) * VPL + v];
-                    P.hub_part[(size_t)tile * V + q * VPL + v] = s;
+            for (int v = 0; v < VPL; v++) {
+                float s = red[q * VPL + v];
+                for (uint32_t t2 = 1; t2 < TPB; t2++) s += red[(t2 * L + q) * VPL + v];
+                P.hub_part[(size_t)tile * V + q * VPL + v] = s;
+            }
+            __threadfence();
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t prev = atomicAdd(&P.hub_cnt[hc.hub], 1u);
+            last_chunk = prev == hc.nch - 1;
+        }
+        __syncthreads();
+        if (last_chunk && team == 0) {
+            __threadfence();
+            const volatile float *hp = P.hub_part;
+            float tot[VPL];
+#pragma unroll
+            for (int v = 0; v < VPL; v++) {
+                tot[v] = 0.0f;
+                for (uint32_t k = 0; k < hc.nch; k++) tot[v] += hp[(size_t)(hc.first + k) * V + q * VPL + v];
+            }
+            epi(hc.row, row_in(hc.row), tot);
+            if (q == 0) P.hub_cnt[hc.hub] = 0;
+        }
+        __syncthreads();
+    }
+    // Rows, software-pipelined across a team's successive rows: the next
+    // row's pointers are loaded when the current row starts, its first
+    // U columns and its update inputs when the current row's gathers
+    // are done (in flight during the current row's update), and each batch's
+    // gathers overlap the next batch's column loads.  Edges are summed in
+    // order, U at a time (the sums are team_row_sum's).
+    constexpr uint32_t U = team_batch<VPL>();
+    const float *cbase = A.C_in + (size_t)q * VPL;
+    auto row_of = [&](uint32_t t) { return P.row_begin + (t - P.n_chunks) * TPB + team; };
+    auto load_cols = [&](uint32_t j, uint32_t end, uint32_t (&c)[U]) {
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) c[k] = j + k < end ? P.col[j + k] : 0u;
+    };
+    uint32_t row = tile < P.n_tiles ? row_of(tile) : P.N;
+    bool valid = row < P.N;
+    uint32_t beg = 0, end = 0, c[U];
+    RowIn in{};
+    if (valid) {
+        beg = P.rp[row];
+        end = P.rp[row + 1];
+        load_cols(beg, end, c);
+        in = row_in(row);
+    }
+    while (tile < P.n_tiles) {
+        const uint32_t ntile = tile + gridDim.x;
+        const uint32_t nrow = ntile < P.n_tiles ? row_of(ntile) : P.N;
+        const bool nvalid = nrow < P.N;
+        uint32_t nbeg = 0, nend = 0;
+        if (nvalid) {
+            nbeg = P.rp[nrow];
+            nend = P.rp[nrow + 1];
+        }
+        float acc[VPL];
+#pragma unroll
+        for (int v = 0; v < VPL; v++) acc[v] = 0.0f;
+        if (valid) {
+            for (uint32_t j = beg; j < end; j += U) {
+                float x[U][VPL];
+#pragma unroll
+                for (uint32_t k = 0; k < U; k++) {
+                    if (j + k < end) {
+                        load_vec<VPL>(cbase + (size_t)c[k] * V, x[k]);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < VPL; v++) x[k][v] = 0.0f;
+                    }
                 }
-                __threadfence();
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                const uint32_t prev = atomicAdd(&P.hub_cnt[hc.hub], 1u);
-                last_chunk = prev == hc.nch - 1;
-            }
-            __syncthreads();
-            if (last_chunk && team == 0) {
-                __threadfence();
-                const volatile float *hp = P.hub_part;
-                float tot[VPL];
+                if (j + U < end) load_cols(j + U, end, c);
 #pragma unroll
-                for (int v = 0; v < VPL; v++) {
-                    tot[v] = 0.0f;
-                    for (uint32_t k = 0; k < hc.nch; k++) tot[v] += hp[(size_t)(hc.first + k) * V + q * VPL + v];
-                }
-                epi(hc.row, tot);
-                if (q == 0) P.hub_cnt[hc.hub] = 0;
-            }
-            __syncthreads();
-        } else {
-            const uint32_t row = P.row_begin + (tile - P.n_chunks) * TPB + team;
-            if (row < P.N) {
-                float acc[VPL];
-                team_row_sum<V, VPL>(P.rp[row], P.rp[row + 1], 1, P.col, A.C_in, q, acc);
-                epi(row, acc);
+                for (uint32_t k = 0; k < U; k++)
+                    if (j + k < end)
+#pragma unroll
+                        for (int v = 0; v < VPL; v++) acc[v] += x[k][v];
             }
         }
+        RowIn nin{};
+        if (nvalid) {
+            load_cols(nbeg, nend, c);
+            nin = row_in(nrow);
+        }
+        if (valid) epi(row, in, acc);
+        tile = ntile;
+        row = nrow;
+        valid = nvalid;
+        beg = nbeg;
+        end = nend;
+        in = nin;
     }
     // per-vector block partials: teams of a wave first (lanes L apart share
     // their vectors), then waves through LDS, in a fixed order
@@ -493,7 +594,7 @@ int ppr_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep)
 void ppr_state_free(gbx_ppr_state *s) { delete s; }
 
 // Vector width, hub threshold and chunk size: GBX_PPR_HUB (edges, default
-// 1024) and GBX_PPR_CHUNK (edges per hub chunk, default 2048).
+// 4096) and GBX_PPR_CHUNK (edges per hub chunk, default 8192).
 static int build_batch(gbx_pr *pr, uint32_t k, const uint32_t *seed_ptr, const uint32_t *seeds)
 {
     gbx_ctx *ctx = pr->ctx;

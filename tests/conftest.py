@@ -45,5 +45,22 @@ def gpu_ctx():
     ctx.close()
 
 
+@pytest.fixture
+def tuning(gpu_ctx):
+    """Sets tuning fields on the shared context for one test (the context
+    reads GBX_* environment overrides only when it is created), restoring
+    the context's tuning afterwards."""
+    saved = gpu_ctx.tuning()
+
+    def set_fields(**fields):
+        t = gpu_ctx.tuning()
+        for k, v in fields.items():
+            setattr(t, k, v)
+        gpu_ctx.set_tuning(t)
+
+    yield set_fields
+    gpu_ctx.set_tuning(saved)
+
+
 def l1(a, b):
     return float(np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)).sum())

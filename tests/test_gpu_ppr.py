@@ -54,10 +54,11 @@ def test_batched_uniform_equals_pagerank(gpu_ctx, golden):
         np.testing.assert_array_equal(ranks[:, q], ranks[:, 0])
 
 
-def test_batched_hub_chunks_and_full_width(gpu_ctx, monkeypatch):
-    """Scale-16 graph with small hub threshold/chunks so every path is used; 64 vectors."""
-    monkeypatch.setenv("GBX_PPR_HUB", "128")
-    monkeypatch.setenv("GBX_PPR_CHUNK", "256")
+@pytest.mark.parametrize("vpl", [1, 2, 4])
+def test_batched_hub_chunks_and_full_width(gpu_ctx, tuning, vpl):
+    """Scale-16 graph with small hub threshold/chunks so every path is used;
+    64 vectors, 1, 2 or 4 of them a lane (teams of 64, 32, 16 lanes)."""
+    tuning(ppr_hub=128, ppr_chunk=256, ppr_vectors_per_lane=vpl)
     g = gpu_ctx.rmat_graph(16, 16, 77)
     e = g.export(False), g.export(True)
     fwd = O.OracleCSR(g.N, g.E, e[0]["out_degree"], e[0]["in_degree"], e[0]["edges_idx"], e[0]["src"], e[0]["dst"], None)
@@ -67,7 +68,7 @@ def test_batched_hub_chunks_and_full_width(gpu_ctx, monkeypatch):
     sets = _seed_sets(rng, g.N, 64)
     s = g.ppr_session(64, sets, iterations=60, epsilon=0.0, stop=gbx.STOP_NONE)
     info = s.ppr_info()
-    assert info["vector_width"] == 64 and info["hub_chunks"] > info["hub_rows"] > 0
+    assert info["vector_width"] == 64 // vpl and info["hub_chunks"] > info["hub_rows"] > 0
     s.reset()
     s.launch(60)
     ranks, reps = s.read()

@@ -165,19 +165,19 @@ def test_pr_scale_20_against_oracle_f32(gpu_ctx):
 
 
 @pytest.mark.parametrize("lds_bytes", ["0", "4096", "65536"])
-def test_lds_hot_prefix_is_bit_identical(gpu_ctx, golden, monkeypatch, lds_bytes):
+def test_lds_hot_prefix_is_bit_identical(gpu_ctx, golden, tuning, lds_bytes):
     """The LDS copy of the hot contributions holds the same values, so the
     ranks must not change in any bit."""
     z = golden("rmat_s12")
     g = _graph(gpu_ctx, z)
-    monkeypatch.setenv("GBX_LDS_BYTES", "0")
-    r0, _ = g.pagerank(iterations=20, epsilon=0.0)
-    monkeypatch.setenv("GBX_LDS_BYTES", lds_bytes)
-    r1, _ = g.pagerank(iterations=20, epsilon=0.0)
+    tuning(lds_bytes=0)
+    r0, _ = g.pagerank(iterations=20, epsilon=0.0, method=gbx.METHOD_PULL)
+    tuning(lds_bytes=int(lds_bytes))
+    r1, _ = g.pagerank(iterations=20, epsilon=0.0, method=gbx.METHOD_PULL)
     np.testing.assert_array_equal(r0, r1)
-    r2, _ = g.pagerank(iterations=20, epsilon=0.0, precision=gbx.BF16)
-    monkeypatch.setenv("GBX_LDS_BYTES", "0")
-    r3, _ = g.pagerank(iterations=20, epsilon=0.0, precision=gbx.BF16)
+    r2, _ = g.pagerank(iterations=20, epsilon=0.0, precision=gbx.BF16, method=gbx.METHOD_PULL)
+    tuning(lds_bytes=0)
+    r3, _ = g.pagerank(iterations=20, epsilon=0.0, precision=gbx.BF16, method=gbx.METHOD_PULL)
     np.testing.assert_array_equal(r2, r3)
 
 
