@@ -1995,6 +1995,8 @@ static void grant_lds(K *kernel, size_t shm)
 
 bool binned_supported(const gbx_pr *pr)
 {
+    // a graph with no edges has nothing to bin (the pull iteration runs it)
+    if (pr->g->E == 0) return false;
     if (pr->kind == K_SEG) return pr->seg && pr->plan->segmented && pr->g->N && pr->elem == 4;
     if (pr->batch || pr->seg || pr->plan->segmented || pr->g->N == 0) return false;
     if (is_weighted_kind(pr->kind)) return pr->elem == 4;  // FP32 contributions and weights
