@@ -220,6 +220,31 @@ __global__ void __launch_bounds__(1024) column_counts(const uint32_t *__restrict
         if (h[i]) atomicAdd(cnt + i, h[i]);
 }
 
+// grp[2g] = pre[8g], grp[2g + 1] = pre[8g + 1] for the G groups of 8 of n
+// items (pre: n + 1 entries, 8g < n).
+__global__ void group_prefix(const uint32_t *__restrict__ pre, uint32_t n, uint32_t G, uint32_t *__restrict__ grp)
+{
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
+        grp[2 * g] = pre[8 * g];
+        grp[2 * g + 1] = pre[8 * g + 1];
+    }
+}
+
+// out[i] = the k < nr with bounds[k] <= i < bounds[k + 1] (0 below bounds[0]
+// and when nr = 0), i < n; bounds[0..nr] ascending.
+__global__ void range_index(const uint32_t *__restrict__ bounds, uint32_t nr, uint32_t n, uint32_t *__restrict__ out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = nr;  // the largest k in [lo, hi) with bounds[k] <= i, or 0
+        while (lo + 1 < hi) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (bounds[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        out[i] = lo;
+    }
+}
+
 // key = bin * NC + chunk (bin-major) or chunk * NB + bin; payload = plan edge index
 __global__ void bin_keys(const uint32_t *__restrict__ rowid, const uint32_t *__restrict__ col, uint32_t E,
                          const uint32_t *__restrict__ bin_of, const uint32_t *__restrict__ chunk_of,
@@ -1633,76 +1658,86 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     bp->runs = runs;
     bp->hot16 = hot16;
     if (hot16 && H > kHot16Max) return set_error(ctx, GBX_ERR_ARG, "binned layout: %u hot sources, packed pairs take %u", H, kHot16Max);
-    std::vector<uint32_t> rp(N + 1), colcnt(N);
+    std::vector<uint32_t> rp(N + 1);
     GBX_HIP(ctx, hipMemcpyAsync(rp.data(), p->rp.p, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
-    DevBuf dcnt;
+    // in-edge counts of the cold sources (u >= H; the hot ones count 0), their
+    // exclusive prefix pre[0..N] and, per group of 8 sources, pre at its first
+    // source and one past it: what the chunk cut reads
+    const uint32_t G = (N + 7) / 8;
+    DevBuf dcnt, dpre, dgrp;
     GBX_TRY(dev_alloc(ctx, dcnt, (size_t)N * 4 + 4));
+    GBX_TRY(dev_alloc(ctx, dpre, (size_t)(N + 1) * 4));
+    GBX_TRY(dev_alloc(ctx, dgrp, (size_t)G * 8 + 8));
     GBX_TRY(fill_u32(ctx, dcnt.as<uint32_t>(), 0, N));
     if (E) {
         column_counts<<<(unsigned)std::max(1, ctx->num_cus) * 2u, 1024, 0, ctx->stream>>>(p->col.as<uint32_t>(), E,
                                                                           dcnt.as<uint32_t>());
         GBX_HIP(ctx, hipGetLastError());
     }
-    GBX_HIP(ctx, hipMemcpyAsync(colcnt.data(), dcnt.p, (size_t)N * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GBX_TRY(fill_u32(ctx, dcnt.as<uint32_t>(), 0, std::min(H, N)));
+    GBX_TRY(exclusive_scan_u32(ctx, dcnt.as<uint32_t>(), dpre.as<uint32_t>(), N, dpre.as<uint32_t>() + N));
+    if (G) {
+        group_prefix<<<grid_for(G, 256, 16384), 256, 0, ctx->stream>>>(dpre.as<uint32_t>(), N, G, dgrp.as<uint32_t>());
+        GBX_HIP(ctx, hipGetLastError());
+    }
+    std::vector<uint32_t> grp((size_t)G * 2);
+    uint32_t Ecold = 0;
+    if (G) GBX_HIP(ctx, hipMemcpyAsync(grp.data(), dgrp.p, (size_t)G * 8, hipMemcpyDeviceToHost, ctx->stream));
+    GBX_HIP(ctx, hipMemcpyAsync(&Ecold, dpre.as<uint32_t>() + N, 4, hipMemcpyDeviceToHost, ctx->stream));
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    uint64_t Ehot = 0;
-    for (uint32_t u = 0; u < H; u++) Ehot += colcnt[u];
-    const uint32_t Ecold = E - (uint32_t)Ehot;
-    // cold source chunks: <= SC sources and about EC out-edges each, from H on
+    const uint64_t Ehot = (uint64_t)E - Ecold;
+    // cold source chunks: <= SC sources and about EC out-edges each, from H
+    // on, cut only at multiples of 8 sources (16-byte aligned contributions):
+    // a new chunk starts at u when the current one would pass SC sources
+    // with u's group, or EC edges with u itself (pre[u + 1] - pre[start])
     std::vector<uint32_t> clo{H}, cstart{0};
-    {
-        uint64_t e = 0, acc = 0;
-        uint32_t start = H;
-        for (uint32_t u = H; u < N; u++) {
-            // cuts only at multiples of 8 sources (16-byte aligned contributions)
-            if ((u & 7u) == 0 && u > start && (u + 8 - start > SC || e + colcnt[u] > EC)) {
+    if (N > H) {
+        uint32_t start = H, pstart = 0;  // pre[start] (0 at H: no cold source below it)
+        for (uint32_t g = (H + 7) / 8; g < G; g++) {
+            const uint32_t u = 8 * g;
+            if (u > start && (u + 8 - start > SC || (uint64_t)(grp[2 * g + 1] - pstart) > EC)) {
                 clo.push_back(u);
-                cstart.push_back((uint32_t)acc);
+                cstart.push_back(grp[2 * g]);
                 start = u;
-                e = 0;
+                pstart = grp[2 * g];
             }
-            e += colcnt[u];
-            acc += colcnt[u];
         }
-        if (N > H) {
-            clo.push_back(N);
-            cstart.push_back(Ecold);
-        }
+        clo.push_back(N);
+        cstart.push_back(Ecold);
     }
     const uint32_t NC = (uint32_t)clo.size() - 1;
     // destination bins: <= SB rows and <= EB in-edges each (a longer row is a
-    // bin of its own); a batch of graphs also cuts at every graph's first row
+    // bin of its own); a batch of graphs also cuts at every graph's first row.
+    // From a bin's first row `start` the next bin starts at the first of
+    // start + SB, the first row v > start with rp[v + 1] - rp[start] > EB
+    // (a binary search: rp ascends), and the next graph's first row.
     std::vector<uint32_t> vb{0};
     {
-        uint64_t e = 0;
         uint32_t start = 0;
         size_t gk = 1;
-        for (uint32_t v = 0; v < N; v++) {
-            const uint32_t d = rp[v + 1] - rp[v];
-            bool gstart = false;
-            if (goff)
-                while (gk < goff->size() && (*goff)[gk] <= v) gstart = (*goff)[gk++] == v;
-            if (v > start && (v - start >= SB || e + d > EB || gstart)) {
-                vb.push_back(v);
-                start = v;
-                e = 0;
+        while (start < N) {
+            uint32_t cut = (uint32_t)std::min<uint64_t>((uint64_t)start + SB, N);
+            const uint64_t T = (uint64_t)rp[start] + EB;
+            const uint32_t j = (uint32_t)(std::upper_bound(rp.begin() + start + 2, rp.end(), T,
+                                                           [](uint64_t t, uint32_t x) { return t < x; }) -
+                                          rp.begin());
+            cut = std::min(cut, j - 1u);
+            if (goff) {
+                while (gk < goff->size() && (*goff)[gk] <= start) gk++;
+                if (gk < goff->size()) cut = std::min(cut, (*goff)[gk]);
             }
-            e += d;
+            vb.push_back(cut);
+            start = cut;
         }
-        vb.push_back(N);
     }
     const uint32_t NB = (uint32_t)vb.size() - 1;
     if ((uint64_t)NB * std::max(NC, 1u) > (1ull << 28))
         return set_error(ctx, GBX_ERR_RANGE, "binned layout: %u bins x %u chunks too many", NB, NC);
     bp->NC = NC;
     bp->NB = NB;
-    std::vector<uint32_t> bin_of(N), chunk_of(N, 0), bin_row0(vb.begin(), vb.end() - 1);
-    for (uint32_t b = 0; b < NB; b++)
-        for (uint32_t v = vb[b]; v < vb[b + 1]; v++) bin_of[v] = b;
-    for (uint32_t k = 0; k < NC; k++)
-        for (uint32_t u = clo[k]; u < clo[k + 1]; u++) chunk_of[u] = k;
+    std::vector<uint32_t> bin_row0(vb.begin(), vb.end() - 1);
     int rc = GBX_OK;
-    DevBuf d_bin_of, d_chunk_of, d_bin_row0, d_cstart, d_bin_len, d_rp_bins, d_hot_start;
+    DevBuf d_bin_of, d_chunk_of, d_bin_row0, d_cstart, d_bin_len, d_rp_bins, d_hot_start, d_vb;
     auto up = [&](DevBuf &d, const void *h, size_t bytes) {
         if (rc) return;
         rc = dev_alloc(ctx, d, bytes + 16);
@@ -1711,12 +1746,23 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     };
     std::vector<uint32_t> rp_bins(NB + 1);
     for (uint32_t b = 0; b <= NB; b++) rp_bins[b] = rp[vb[b]];
-    up(d_bin_of, bin_of.data(), (size_t)N * 4);
-    up(d_chunk_of, chunk_of.data(), (size_t)N * 4);
+    // the bin of every row and the chunk of every source (0 for the hot
+    // ones), on the device from the cuts
+    up(d_vb, vb.data(), vb.size() * 4);
+    up(bp->chunk_lo, clo.data(), (size_t)(NC + 1) * 4);
+    GBX_TRY(rc);
+    GBX_TRY(dev_alloc(ctx, d_bin_of, (size_t)N * 4 + 16));
+    GBX_TRY(dev_alloc(ctx, d_chunk_of, (size_t)N * 4 + 16));
+    if (N) {
+        range_index<<<grid_for(N, 256, 16384), 256, 0, ctx->stream>>>(d_vb.as<uint32_t>(), NB, N,
+                                                                       d_bin_of.as<uint32_t>());
+        range_index<<<grid_for(N, 256, 16384), 256, 0, ctx->stream>>>(bp->chunk_lo.as<uint32_t>(), NC, N,
+                                                                       d_chunk_of.as<uint32_t>());
+        GBX_HIP(ctx, hipGetLastError());
+    }
     up(d_bin_row0, bin_row0.data(), (size_t)NB * 4);
     up(d_cstart, cstart.data(), (size_t)(NC + 1) * 4);
     up(d_rp_bins, rp_bins.data(), (size_t)(NB + 1) * 4);
-    up(bp->chunk_lo, clo.data(), (size_t)(NC + 1) * 4);
     GBX_TRY(rc);
     GBX_TRY(dev_alloc(ctx, d_bin_len, (size_t)NB * 4 + 4));
     GBX_TRY(fill_u32(ctx, d_bin_len.as<uint32_t>(), 0, NB));
