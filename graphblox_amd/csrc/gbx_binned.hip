@@ -230,6 +230,69 @@ __global__ void group_prefix(const uint32_t *__restrict__ pre, uint32_t n, uint3
     }
 }
 
+// Destination bins: from a bin's first row s the next bin starts at
+// nxt[s] = the first of s + SB, the first row v > s with
+// rp[v + 1] - rp[s] > EB (binary search: rp ascends), the first graph start
+// above s (goff, ngoff ascending offsets; none for one graph) and N.
+// Greedy from row 0, this cuts every bin at <= SB rows and <= EB in-edges
+// (a longer row is a bin of its own), never across a graph.
+__global__ void bin_next(const uint32_t *__restrict__ rp, uint32_t N, uint32_t SB, uint64_t EB,
+                         const uint32_t *__restrict__ goff, uint32_t ngoff, uint32_t *__restrict__ nxt)
+{
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < N; s += gridDim.x * blockDim.x) {
+        uint32_t cut = (uint32_t)min((uint64_t)s + SB, (uint64_t)N);
+        const uint64_t T = (uint64_t)rp[s] + EB;
+        // the first j in [s + 2, cut] with rp[j] > T makes v = j - 1 < cut
+        uint32_t lo = s + 2, hi = cut + 1;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if ((uint64_t)rp[mid] > T) hi = mid;
+            else lo = mid + 1;
+        }
+        if (lo <= cut) cut = lo - 1;
+        lo = 0;
+        hi = ngoff;
+        while (lo < hi) {  // the first offset above s
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (goff[mid] > s) hi = mid;
+            else lo = mid + 1;
+        }
+        if (lo < ngoff) cut = min(cut, goff[lo]);
+        nxt[s] = cut;
+    }
+}
+
+// vb[0..nb] = the bins' first rows from row 0 through nxt, then N; *nb_out =
+// nb.  One lane walks the chain; a bin always ends past its first row (a
+// step of at least one row keeps the walk finite whatever nxt holds).
+__global__ void bin_chase(const uint32_t *__restrict__ nxt, uint32_t N, uint32_t *__restrict__ vb,
+                          uint32_t *__restrict__ nb_out)
+{
+    if (threadIdx.x != 0) return;
+    uint32_t s = 0, k = 0;
+    while (s < N) {
+        vb[k++] = s;
+        const uint32_t t = nxt[s];
+        s = t > s ? t : s + 1u;
+    }
+    vb[k] = N;
+    *nb_out = k;
+}
+
+// *first = min(*first, the first row with fewer than `len` edges).
+__global__ void first_short_row(const uint32_t *__restrict__ rp, uint32_t N, uint32_t len, uint32_t *first)
+{
+    uint32_t m = N;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+        if (rp[v + 1] - rp[v] < len) {
+            m = v;
+            break;
+        }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, off, 64));
+    if ((threadIdx.x & 63u) == 0 && m < N) atomicMin(first, m);
+}
+
 // out[i] = the k < nr with bounds[k] <= i < bounds[k + 1] (0 below bounds[0]
 // and when nr = 0), i < n; bounds[0..nr] ascending.
 __global__ void range_index(const uint32_t *__restrict__ bounds, uint32_t nr, uint32_t n, uint32_t *__restrict__ out)
@@ -1658,33 +1721,65 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     bp->runs = runs;
     bp->hot16 = hot16;
     if (hot16 && H > kHot16Max) return set_error(ctx, GBX_ERR_ARG, "binned layout: %u hot sources, packed pairs take %u", H, kHot16Max);
-    std::vector<uint32_t> rp(N + 1);
-    GBX_HIP(ctx, hipMemcpyAsync(rp.data(), p->rp.p, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
     // in-edge counts of the cold sources (u >= H; the hot ones count 0), their
     // exclusive prefix pre[0..N] and, per group of 8 sources, pre at its first
     // source and one past it: what the chunk cut reads
     const uint32_t G = (N + 7) / 8;
     DevBuf dcnt, dpre, dgrp;
-    GBX_TRY(dev_alloc(ctx, dcnt, (size_t)N * 4 + 4));
     GBX_TRY(dev_alloc(ctx, dpre, (size_t)(N + 1) * 4));
     GBX_TRY(dev_alloc(ctx, dgrp, (size_t)G * 8 + 8));
-    GBX_TRY(fill_u32(ctx, dcnt.as<uint32_t>(), 0, N));
-    if (E) {
-        column_counts<<<(unsigned)std::max(1, ctx->num_cus) * 2u, 1024, 0, ctx->stream>>>(p->col.as<uint32_t>(), E,
-                                                                          dcnt.as<uint32_t>());
-        GBX_HIP(ctx, hipGetLastError());
+    GBX_TRY(fill_u32(ctx, dpre.as<uint32_t>(), 0, (size_t)std::min(H, N) + 1));
+    if (N > H) {
+        const uint32_t *cnt = nullptr;
+        if (p->direction == GBX_PULL_IN) {
+            // the columns are the rows' sources: a column's count is its
+            // source's out-degree, which the plan holds
+            cnt = p->outdeg.as<uint32_t>();
+        } else {
+            GBX_TRY(dev_alloc(ctx, dcnt, (size_t)N * 4 + 4));
+            GBX_TRY(fill_u32(ctx, dcnt.as<uint32_t>(), 0, N));
+            if (E) {
+                column_counts<<<(unsigned)std::max(1, ctx->num_cus) * 2u, 1024, 0, ctx->stream>>>(
+                    p->col.as<uint32_t>(), E, dcnt.as<uint32_t>());
+                GBX_HIP(ctx, hipGetLastError());
+            }
+            cnt = dcnt.as<uint32_t>();
+        }
+        GBX_TRY(exclusive_scan_u32(ctx, cnt + H, dpre.as<uint32_t>() + H, N - H, dpre.as<uint32_t>() + N));
     }
-    GBX_TRY(fill_u32(ctx, dcnt.as<uint32_t>(), 0, std::min(H, N)));
-    GBX_TRY(exclusive_scan_u32(ctx, dcnt.as<uint32_t>(), dpre.as<uint32_t>(), N, dpre.as<uint32_t>() + N));
     if (G) {
         group_prefix<<<grid_for(G, 256, 16384), 256, 0, ctx->stream>>>(dpre.as<uint32_t>(), N, G, dgrp.as<uint32_t>());
         GBX_HIP(ctx, hipGetLastError());
     }
+    // destination bins, on the device (bin_next, bin_chase): <= SB rows and
+    // <= EB in-edges each (a longer row is a bin of its own); a batch of
+    // graphs also cuts at every graph's first row
+    DevBuf d_nxt, d_vb, d_goff, d_nb;
+    GBX_TRY(dev_alloc(ctx, d_nxt, (size_t)N * 4 + 4));
+    GBX_TRY(dev_alloc(ctx, d_vb, ((size_t)N + 2) * 4));
+    GBX_TRY(dev_alloc(ctx, d_nb, 16));
+    const uint32_t ngoff = goff ? (uint32_t)goff->size() : 0u;
+    if (ngoff) {
+        GBX_TRY(dev_alloc(ctx, d_goff, (size_t)ngoff * 4));
+        GBX_HIP(ctx, hipMemcpyAsync(d_goff.p, goff->data(), (size_t)ngoff * 4, hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (N) {
+        bin_next<<<grid_for(N, 256, 16384), 256, 0, ctx->stream>>>(p->rp.as<uint32_t>(), N, SB, (uint64_t)EB,
+                                                                    ngoff ? d_goff.as<uint32_t>() : nullptr, ngoff,
+                                                                    d_nxt.as<uint32_t>());
+        GBX_HIP(ctx, hipGetLastError());
+    }
+    bin_chase<<<1, 64, 0, ctx->stream>>>(d_nxt.as<uint32_t>(), N, d_vb.as<uint32_t>(), d_nb.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
     std::vector<uint32_t> grp((size_t)G * 2);
-    uint32_t Ecold = 0;
+    uint32_t Ecold = 0, NB = 0;
     if (G) GBX_HIP(ctx, hipMemcpyAsync(grp.data(), dgrp.p, (size_t)G * 8, hipMemcpyDeviceToHost, ctx->stream));
     GBX_HIP(ctx, hipMemcpyAsync(&Ecold, dpre.as<uint32_t>() + N, 4, hipMemcpyDeviceToHost, ctx->stream));
+    GBX_HIP(ctx, hipMemcpyAsync(&NB, d_nb.p, 4, hipMemcpyDeviceToHost, ctx->stream));
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (NB > N + 1u) return set_error(ctx, GBX_ERR_STATE, "binned layout: %u bins for %u rows", NB, N);
+    std::vector<uint32_t> vb((size_t)NB + 1);
+    GBX_HIP(ctx, hipMemcpyAsync(vb.data(), d_vb.p, ((size_t)NB + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
     const uint64_t Ehot = (uint64_t)E - Ecold;
     // cold source chunks: <= SC sources and about EC out-edges each, from H
     // on, cut only at multiples of 8 sources (16-byte aligned contributions):
@@ -1706,49 +1801,20 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
         cstart.push_back(Ecold);
     }
     const uint32_t NC = (uint32_t)clo.size() - 1;
-    // destination bins: <= SB rows and <= EB in-edges each (a longer row is a
-    // bin of its own); a batch of graphs also cuts at every graph's first row.
-    // From a bin's first row `start` the next bin starts at the first of
-    // start + SB, the first row v > start with rp[v + 1] - rp[start] > EB
-    // (a binary search: rp ascends), and the next graph's first row.
-    std::vector<uint32_t> vb{0};
-    {
-        uint32_t start = 0;
-        size_t gk = 1;
-        while (start < N) {
-            uint32_t cut = (uint32_t)std::min<uint64_t>((uint64_t)start + SB, N);
-            const uint64_t T = (uint64_t)rp[start] + EB;
-            const uint32_t j = (uint32_t)(std::upper_bound(rp.begin() + start + 2, rp.end(), T,
-                                                           [](uint64_t t, uint32_t x) { return t < x; }) -
-                                          rp.begin());
-            cut = std::min(cut, j - 1u);
-            if (goff) {
-                while (gk < goff->size() && (*goff)[gk] <= start) gk++;
-                if (gk < goff->size()) cut = std::min(cut, (*goff)[gk]);
-            }
-            vb.push_back(cut);
-            start = cut;
-        }
-    }
-    const uint32_t NB = (uint32_t)vb.size() - 1;
     if ((uint64_t)NB * std::max(NC, 1u) > (1ull << 28))
         return set_error(ctx, GBX_ERR_RANGE, "binned layout: %u bins x %u chunks too many", NB, NC);
     bp->NC = NC;
     bp->NB = NB;
-    std::vector<uint32_t> bin_row0(vb.begin(), vb.end() - 1);
     int rc = GBX_OK;
-    DevBuf d_bin_of, d_chunk_of, d_bin_row0, d_cstart, d_bin_len, d_rp_bins, d_hot_start, d_vb;
+    DevBuf d_bin_of, d_chunk_of, d_cstart, d_bin_len, d_rp_bins, d_hot_start;
     auto up = [&](DevBuf &d, const void *h, size_t bytes) {
         if (rc) return;
         rc = dev_alloc(ctx, d, bytes + 16);
         if (!rc && bytes && hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
             rc = set_error(ctx, GBX_ERR_HIP, "binned layout: upload");
     };
-    std::vector<uint32_t> rp_bins(NB + 1);
-    for (uint32_t b = 0; b <= NB; b++) rp_bins[b] = rp[vb[b]];
     // the bin of every row and the chunk of every source (0 for the hot
-    // ones), on the device from the cuts
-    up(d_vb, vb.data(), vb.size() * 4);
+    // ones), from the cuts; each bin's first edge
     up(bp->chunk_lo, clo.data(), (size_t)(NC + 1) * 4);
     GBX_TRY(rc);
     GBX_TRY(dev_alloc(ctx, d_bin_of, (size_t)N * 4 + 16));
@@ -1760,10 +1826,14 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
                                                                        d_chunk_of.as<uint32_t>());
         GBX_HIP(ctx, hipGetLastError());
     }
-    up(d_bin_row0, bin_row0.data(), (size_t)NB * 4);
     up(d_cstart, cstart.data(), (size_t)(NC + 1) * 4);
-    up(d_rp_bins, rp_bins.data(), (size_t)(NB + 1) * 4);
     GBX_TRY(rc);
+    GBX_TRY(dev_alloc(ctx, d_rp_bins, ((size_t)NB + 1) * 4 + 16));
+    gather_u32<<<grid_for(NB + 1, 256, 1024), 256, 0, ctx->stream>>>(p->rp.as<uint32_t>(), d_vb.as<uint32_t>(), NB + 1,
+                                                                      d_rp_bins.as<uint32_t>());
+    GBX_HIP(ctx, hipGetLastError());
+    // the bins' first rows: the first NB entries of vb
+    const DevBuf &d_bin_row0 = d_vb;
     GBX_TRY(dev_alloc(ctx, d_bin_len, (size_t)NB * 4 + 4));
     GBX_TRY(fill_u32(ctx, d_bin_len.as<uint32_t>(), 0, NB));
     GBX_TRY(dev_alloc(ctx, bp->chunk_grp, (size_t)(NC + 1) * 4));
@@ -1772,8 +1842,17 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     if (E) {
         GBX_TRY(dev_alloc(ctx, rowid, (size_t)E * 4));
         // rows of at least 64 edges at the front of the plan order
-        uint32_t nlong = 0;
-        while (nlong < N && rp[nlong + 1] - rp[nlong] >= 64u) nlong++;
+        uint32_t nlong = N;
+        {
+            DevBuf d_first;
+            GBX_TRY(dev_alloc(ctx, d_first, 16));
+            GBX_TRY(fill_u32(ctx, d_first.as<uint32_t>(), N, 1));
+            first_short_row<<<grid_for(N, 256, 4096), 256, 0, ctx->stream>>>(p->rp.as<uint32_t>(), N, 64u,
+                                                                             d_first.as<uint32_t>());
+            GBX_HIP(ctx, hipGetLastError());
+            GBX_HIP(ctx, hipMemcpyAsync(&nlong, d_first.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+            GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        }
         if (nlong)
             edge_rows_long<<<grid_for(nlong, 4, 16384), 256, 0, ctx->stream>>>(p->rp.as<uint32_t>(), nlong,
                                                                             rowid.as<uint32_t>());
