@@ -230,6 +230,57 @@ __global__ void group_prefix(const uint32_t *__restrict__ pre, uint32_t n, uint3
     }
 }
 
+// Cold source chunks: from a chunk starting at source 8g (g < G) the next
+// one starts at group nxt[g] (>= G: none), the first group g' > g whose
+// first source u = 8g' would take the chunk past SC sources with its group
+// (u + 8 - 8g > SC) or past EC edges with u itself (pre[u + 1] - pre[8g]:
+// grp[2g' + 1] - grp[2g], a binary search: pre ascends); nxt[G] is the same
+// from the first cold source H, which need not start a group.
+__global__ void chunk_next(const uint32_t *__restrict__ grp, uint32_t G, uint32_t H, uint32_t SC, uint64_t EC,
+                           uint32_t *__restrict__ nxt)
+{
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += gridDim.x * blockDim.x) {
+        const uint32_t start = g < G ? 8u * g : H;
+        const uint32_t pstart = g < G ? grp[2 * g] : 0u;  // pre[H] = 0: no cold source below H
+        const uint32_t first = start / 8u + 1u;            // the first group past `start`
+        const uint32_t gsc = (uint32_t)min(((uint64_t)start + SC - 8u) / 8u + 1u, (uint64_t)G);
+        uint32_t lo = first, hi = gsc;  // the first group in [first, gsc) past EC edges, else gsc
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if ((uint64_t)(grp[2 * mid + 1] - pstart) > EC) hi = mid;
+            else lo = mid + 1;
+        }
+        nxt[g] = lo;
+    }
+}
+
+// clo[0..nc] = the chunks' first sources (H first, N last) and cstart[0..nc]
+// their first cold edges (pre at each, Ecold last); *nc_out = nc.  One lane
+// walks the chain (a step of at least one group keeps it finite).
+__global__ void chunk_chase(const uint32_t *__restrict__ nxt, const uint32_t *__restrict__ grp,
+                            const uint32_t *__restrict__ pre, uint32_t G, uint32_t H, uint32_t N,
+                            uint32_t *__restrict__ clo, uint32_t *__restrict__ cstart, uint32_t *__restrict__ nc_out)
+{
+    if (threadIdx.x != 0) return;
+    clo[0] = H;
+    cstart[0] = 0;
+    uint32_t k = 1;
+    if (N > H) {
+        uint32_t g = nxt[G];
+        while (g < G) {
+            clo[k] = 8u * g;
+            cstart[k] = grp[2 * g];
+            k++;
+            const uint32_t t = nxt[g];
+            g = t > g ? t : g + 1u;
+        }
+        clo[k] = N;
+        cstart[k] = pre[N];
+        k++;
+    }
+    *nc_out = k - 1;
+}
+
 // Destination bins: from a bin's first row s the next bin starts at
 // nxt[s] = the first of s + SB, the first row v > s with
 // rp[v + 1] - rp[s] > EB (binary search: rp ascends), the first graph start
@@ -1771,42 +1822,35 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     }
     bin_chase<<<1, 64, 0, ctx->stream>>>(d_nxt.as<uint32_t>(), N, d_vb.as<uint32_t>(), d_nb.as<uint32_t>());
     GBX_HIP(ctx, hipGetLastError());
-    std::vector<uint32_t> grp((size_t)G * 2);
-    uint32_t Ecold = 0, NB = 0;
-    if (G) GBX_HIP(ctx, hipMemcpyAsync(grp.data(), dgrp.p, (size_t)G * 8, hipMemcpyDeviceToHost, ctx->stream));
+    // cold source chunks, on the device (chunk_next, chunk_chase): <= SC
+    // sources and about EC out-edges each, from H on, cut only at multiples
+    // of 8 sources (16-byte aligned contributions)
+    DevBuf d_cnxt, d_cstart;
+    GBX_TRY(dev_alloc(ctx, d_cnxt, ((size_t)G + 1) * 4));
+    GBX_TRY(dev_alloc(ctx, bp->chunk_lo, ((size_t)G + 3) * 4));
+    GBX_TRY(dev_alloc(ctx, d_cstart, ((size_t)G + 3) * 4));
+    chunk_next<<<grid_for((size_t)G + 1, 256, 16384), 256, 0, ctx->stream>>>(dgrp.as<uint32_t>(), G, H, SC, (uint64_t)EC,
+                                                                              d_cnxt.as<uint32_t>());
+    chunk_chase<<<1, 64, 0, ctx->stream>>>(d_cnxt.as<uint32_t>(), dgrp.as<uint32_t>(), dpre.as<uint32_t>(), G, H, N,
+                                           bp->chunk_lo.as<uint32_t>(), d_cstart.as<uint32_t>(),
+                                           d_nb.as<uint32_t>() + 1);
+    GBX_HIP(ctx, hipGetLastError());
+    uint32_t Ecold = 0, counts[2] = {0, 0};  // bins, chunks
     GBX_HIP(ctx, hipMemcpyAsync(&Ecold, dpre.as<uint32_t>() + N, 4, hipMemcpyDeviceToHost, ctx->stream));
-    GBX_HIP(ctx, hipMemcpyAsync(&NB, d_nb.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    GBX_HIP(ctx, hipMemcpyAsync(counts, d_nb.p, 8, hipMemcpyDeviceToHost, ctx->stream));
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (NB > N + 1u) return set_error(ctx, GBX_ERR_STATE, "binned layout: %u bins for %u rows", NB, N);
+    const uint32_t NB = counts[0], NC = counts[1];
+    if (NB > N + 1u || NC > G + 1u)
+        return set_error(ctx, GBX_ERR_STATE, "binned layout: %u bins for %u rows, %u chunks", NB, N, NC);
     std::vector<uint32_t> vb((size_t)NB + 1);
     GBX_HIP(ctx, hipMemcpyAsync(vb.data(), d_vb.p, ((size_t)NB + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
     const uint64_t Ehot = (uint64_t)E - Ecold;
-    // cold source chunks: <= SC sources and about EC out-edges each, from H
-    // on, cut only at multiples of 8 sources (16-byte aligned contributions):
-    // a new chunk starts at u when the current one would pass SC sources
-    // with u's group, or EC edges with u itself (pre[u + 1] - pre[start])
-    std::vector<uint32_t> clo{H}, cstart{0};
-    if (N > H) {
-        uint32_t start = H, pstart = 0;  // pre[start] (0 at H: no cold source below it)
-        for (uint32_t g = (H + 7) / 8; g < G; g++) {
-            const uint32_t u = 8 * g;
-            if (u > start && (u + 8 - start > SC || (uint64_t)(grp[2 * g + 1] - pstart) > EC)) {
-                clo.push_back(u);
-                cstart.push_back(grp[2 * g]);
-                start = u;
-                pstart = grp[2 * g];
-            }
-        }
-        clo.push_back(N);
-        cstart.push_back(Ecold);
-    }
-    const uint32_t NC = (uint32_t)clo.size() - 1;
     if ((uint64_t)NB * std::max(NC, 1u) > (1ull << 28))
         return set_error(ctx, GBX_ERR_RANGE, "binned layout: %u bins x %u chunks too many", NB, NC);
     bp->NC = NC;
     bp->NB = NB;
     int rc = GBX_OK;
-    DevBuf d_bin_of, d_chunk_of, d_cstart, d_bin_len, d_rp_bins, d_hot_start;
+    DevBuf d_bin_of, d_chunk_of, d_bin_len, d_rp_bins, d_hot_start;
     auto up = [&](DevBuf &d, const void *h, size_t bytes) {
         if (rc) return;
         rc = dev_alloc(ctx, d, bytes + 16);
@@ -1815,8 +1859,6 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
     };
     // the bin of every row and the chunk of every source (0 for the hot
     // ones), from the cuts; each bin's first edge
-    up(bp->chunk_lo, clo.data(), (size_t)(NC + 1) * 4);
-    GBX_TRY(rc);
     GBX_TRY(dev_alloc(ctx, d_bin_of, (size_t)N * 4 + 16));
     GBX_TRY(dev_alloc(ctx, d_chunk_of, (size_t)N * 4 + 16));
     if (N) {
@@ -1826,8 +1868,6 @@ static int build_bplan_into(gbx_ctx *ctx, gbx_plan *p, uint32_t pad, uint32_t H,
                                                                        d_chunk_of.as<uint32_t>());
         GBX_HIP(ctx, hipGetLastError());
     }
-    up(d_cstart, cstart.data(), (size_t)(NC + 1) * 4);
-    GBX_TRY(rc);
     GBX_TRY(dev_alloc(ctx, d_rp_bins, ((size_t)NB + 1) * 4 + 16));
     gather_u32<<<grid_for(NB + 1, 256, 1024), 256, 0, ctx->stream>>>(p->rp.as<uint32_t>(), d_vb.as<uint32_t>(), NB + 1,
                                                                       d_rp_bins.as<uint32_t>());
