@@ -166,7 +166,20 @@ def timed_steps(pr, steps, graph_replay=False):
     return rep0["time_total_s"] * 1e3 / steps, wall_ms, ranks, rep
 
 
+# --no-graphs: every launch one by one (for profiler counter runs, which
+# have crashed inside a HIP graph launch)
+NO_GRAPHS = False
+
+
+def launch_iters(pr, n):
+    if NO_GRAPHS:
+        pr.launch(n)
+    else:
+        pr.launch_graph(n)
+
+
 def time_session(pr, warmup, steps, graph_replay=False):
+    graph_replay = graph_replay and not NO_GRAPHS
     pr.reset()
     if graph_replay:
         pr.launch_graph(steps)  # captured here, replayed in the timed region
@@ -265,11 +278,16 @@ def forms_s22(gbx, O, ctx, args):
     rng = np.random.default_rng(SEED)
     sets = [rng.choice(Nb, int(rng.integers(1, 1001)), replace=False) for _ in range(32)]
     # the batched form's lanes per row, hub cut and blocks per CU, searched
-    # by the library's autotuner on this graph with 32 vectors
-    t0 = time.perf_counter()
-    best, _ = ctx.autotune(gb, 32, 5, iterations=10, epsilon=0.0, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_NONE)
+    # by the library's autotuner on this graph with 32 vectors (--no-autotune:
+    # the library's defaults)
+    if args.no_autotune:
+        best = ctx.tuning()
+    else:
+        t0 = time.perf_counter()
+        best, _ = ctx.autotune(gb, 32, 5, iterations=10, epsilon=0.0, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_NONE)
+        log(f"batched form autotuned in {time.perf_counter() - t0:.2f} s")
     ppr_tuned = {k: v for k, v in best.as_dict().items() if k.startswith("ppr")}
-    log(f"batched form autotuned in {time.perf_counter() - t0:.2f} s: {ppr_tuned}")
+    log(f"batched form tuning: {ppr_tuned}")
     kb, wb, rb, sb = time_form(gb, args.warmup, args.steps, num_vectors=32, seed_sets=sets)
     sb.free()
     errs = []
@@ -383,12 +401,12 @@ def forms_batch(gbx, O, ctx, args):
     # to convergence (each graph its own L1 < 1e-7 stop), one replayed graph of 100 iterations
     c = b.session(iterations=100, epsilon=1e-7, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_L1)
     c.reset()
-    c.launch_graph(100)
+    launch_iters(c, 100)
     c.read()
     ctx.sync()
     t0 = time.perf_counter()
     c.reset()
-    c.launch_graph(100)
+    launch_iters(c, 100)
     _, creps = c.read()
     conv_ms = (time.perf_counter() - t0) * 1e3
     c.free()
@@ -575,12 +593,18 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="the headline only")
     ap.add_argument("--tmpdir", default=None, help="where the ingest measurement writes its edge files")
     ap.add_argument("--eager", action="store_true", help="headline iterations launched one by one, not replayed")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="no HIP graph launch anywhere (profiler counter runs); implies --eager")
     ap.add_argument("--no-autotune", action="store_true", help="the library's default layout cut")
     ap.add_argument("--method", choices=("binned", "pull"), default="binned",
                     help="iteration of the headline (binned: the library default at this size)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="run only a plain device copy of this many GiB (reference for PMC byte counters)")
     args = ap.parse_args()
+    if args.no_graphs:
+        global NO_GRAPHS
+        NO_GRAPHS = True
+        args.eager = True
     if args.calibrate_copy > 0:
         calibrate_copy(args.calibrate_copy)
         return
@@ -639,10 +663,11 @@ def main():
     forms = None
     if not args.no_forms:
         forms = forms_s24(gbx, O, g, r64, args, value, args.method)
-        km, wm, re_, _ = time_session(pr, args.warmup, args.steps, graph_replay=args.eager)
-        forms[f"{args.method}_f32_" + ("replayed" if args.eager else "eager")] = form_record(
-            f"the headline's iteration {'replayed as a HIP graph' if args.eager else 'launched one by one'}", E, N,
-            1, km, wm, bytes_per_launch, **fp64_errors(re_, r64))
+        if not NO_GRAPHS:
+            km, wm, re_, _ = time_session(pr, args.warmup, args.steps, graph_replay=args.eager)
+            forms[f"{args.method}_f32_" + ("replayed" if args.eager else "eager")] = form_record(
+                f"the headline's iteration {'replayed as a HIP graph' if args.eager else 'launched one by one'}",
+                E, N, 1, km, wm, bytes_per_launch, **fp64_errors(re_, r64))
         del fo, io
         pr.free()
         g.free()

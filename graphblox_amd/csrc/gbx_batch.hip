@@ -218,13 +218,17 @@ __global__ void __launch_bounds__(256) seg_reduce_kernel(SegView S, int stop, do
         last = atomicAdd(S.counter, 1u) == gridDim.x - 1;
     }
     __syncthreads();
-    if (!last || threadIdx.x != 0) return;
+    if (!last) return;
+    // the last block: every graph's flag, read by all of its threads
     __threadfence();
     const volatile GCoef *vc = S.coef;
-    uint32_t all = 1;
-    for (uint32_t j = 0; j < S.nseg; j++) all &= vc[j].done ? 1u : 0u;
-    *S.all_done = all;
-    *S.counter = 0;
+    int all = 1;
+    for (uint32_t j = threadIdx.x; j < S.nseg; j += blockDim.x) all &= vc[j].done ? 1 : 0;
+    all = __syncthreads_and(all);
+    if (threadIdx.x == 0) {
+        *S.all_done = all ? 1u : 0u;
+        *S.counter = 0;
+    }
 }
 
 // ---------------------------------------------------------------- batch on the binned iteration
@@ -334,22 +338,33 @@ __global__ void __launch_bounds__(256) segbin_reduce_kernel(
         last = atomicAdd(counter, 1u) == gridDim.x - 1;
     }
     __syncthreads();
-    if (!last || threadIdx.x != 0) return;
+    if (!last) return;
+    // the last block: every graph's flag and rank sum, read by all of its
+    // threads, the sums added in a fixed tree order (deterministic)
     __threadfence();
     const volatile GCoef *vc = coef;
     const volatile GStat *vs = stat;
-    uint32_t all = 1;
+    int all = 1;
     double total = 0.0;
-    for (uint32_t j = 0; j < nseg; j++) {
-        all &= vc[j].done ? 1u : 0u;
+    for (uint32_t j = threadIdx.x; j < nseg; j += 256) {
+        all &= vc[j].done ? 1 : 0;
         total += vs[j].sum;
     }
-    *all_done = all;
-    Slot sl{};
-    sl.sum = total;
-    sl.done = all;
-    *nxt = sl;
-    *counter = 0;
+    all = __syncthreads_and(all);
+    red[0][threadIdx.x] = total;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[0][threadIdx.x] += red[0][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *all_done = all ? 1u : 0u;
+        Slot sl{};
+        sl.sum = red[0][0];
+        sl.done = all ? 1u : 0u;
+        *nxt = sl;
+        *counter = 0;
+    }
 }
 
 __global__ void seg_finalize_kernel(const float *__restrict__ r, const uint32_t *__restrict__ perm,
