@@ -24,17 +24,61 @@ struct IterView {
 
 // ------------------------------------------------------------------ reductions
 
+// Cross-lane moves of the wave (DPP, no LDS round trip): row_shr:n within
+// rows of 16 lanes, the last lane of a row broadcast to the following
+// row(s).  Lanes without a source read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp32(uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROW_MASK, 0xF, true);
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp64(double x)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const uint64_t lo = dpp32<CTRL, ROW_MASK>((uint32_t)u), hi = dpp32<CTRL, ROW_MASK>((uint32_t)(u >> 32));
+    return __longlong_as_double((long long)((hi << 32) | lo));
+}
+
+__device__ __forceinline__ double readlane64(double x, int lane)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __longlong_as_double((long long)((hi << 32) | lo));
+}
+
+// One step of an inclusive scan of every field (sums; maxima of the
+// non-negative 32-bit fields, for which a missing source's 0 is neutral).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void scan_step(Partial &p)
+{
+    p.err += dpp64<CTRL, ROW_MASK>(p.err);
+    p.dang += dpp64<CTRL, ROW_MASK>(p.dang);
+    p.sum += dpp64<CTRL, ROW_MASK>(p.sum);
+    p.active += dpp32<CTRL, ROW_MASK>(p.active);
+    p.cmax = max(p.cmax, dpp32<CTRL, ROW_MASK>(p.cmax));
+    p.cnmin = max(p.cnmin, dpp32<CTRL, ROW_MASK>(p.cnmin));
+}
+
+// The wave's totals in every lane (all 64 lanes active): an inclusive scan
+// leaves them in lane 63, which every lane then reads.  The additions follow
+// a fixed order, so the result is deterministic.
 __device__ __forceinline__ void wave_reduce(Partial &p)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        p.err += __shfl_xor(p.err, off, 64);
-        p.dang += __shfl_xor(p.dang, off, 64);
-        p.sum += __shfl_xor(p.sum, off, 64);
-        p.active += __shfl_xor(p.active, off, 64);
-        p.cmax = max(p.cmax, (uint32_t)__shfl_xor(p.cmax, off, 64));
-        p.cnmin = max(p.cnmin, (uint32_t)__shfl_xor(p.cnmin, off, 64));
-    }
+    scan_step<0x111, 0xF>(p);  // row_shr:1
+    scan_step<0x112, 0xF>(p);  // row_shr:2
+    scan_step<0x114, 0xF>(p);  // row_shr:4
+    scan_step<0x118, 0xF>(p);  // row_shr:8
+    scan_step<0x142, 0xA>(p);  // row_bcast:15 into rows 1 and 3
+    scan_step<0x143, 0xC>(p);  // row_bcast:31 into rows 2 and 3
+    p.err = readlane64(p.err, 63);
+    p.dang = readlane64(p.dang, 63);
+    p.sum = readlane64(p.sum, 63);
+    p.active = (uint32_t)__builtin_amdgcn_readlane((int)p.active, 63);
+    p.cmax = (uint32_t)__builtin_amdgcn_readlane((int)p.cmax, 63);
+    p.cnmin = (uint32_t)__builtin_amdgcn_readlane((int)p.cnmin, 63);
 }
 
 __device__ __forceinline__ void add_partial(Partial &q, const Partial &b)
@@ -48,46 +92,48 @@ __device__ __forceinline__ void add_partial(Partial &q, const Partial &b)
 }
 
 // Block partial -> partials[blockIdx]; the last block reduces all partials
-// in block order and calls fin(total) on thread 0.
+// and calls fin(total) on thread 0.  Within a block the waves' partials are
+// combined by wave 0 (lane w holds wave w's); the last block reads every
+// block's partial with ordinary loads after its acquire fence (one round trip
+// each, where element-wise volatile loads would each wait for memory), then
+// reduces them in a fixed tree order, so the result is deterministic.
 template <class Fin>
 __device__ __forceinline__ void grid_reduce(Partial p, Partial *partials, uint32_t *counter, Fin fin)
 {
-    __shared__ Partial sp[kBlock / 64];
+    constexpr int kWaves = kBlock / 64;
+    __shared__ Partial sp[kWaves];
     __shared__ bool last;
+    const Partial zero = {0.0, 0.0, 0.0, 0u, 0u, 0u, 0u};
     wave_reduce(p);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) sp[wid] = p;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        Partial q = sp[0];
-        for (int w = 1; w < kBlock / 64; w++) add_partial(q, sp[w]);
-        partials[blockIdx.x] = q;
-        __threadfence();
-        uint32_t ticket = atomicAdd(counter, 1u);
-        last = ticket == gridDim.x - 1;
+    if (wid == 0) {
+        Partial q = lane < kWaves ? sp[lane] : zero;
+        wave_reduce(q);
+        if (lane == 0) {
+            partials[blockIdx.x] = q;
+            __threadfence();
+            const uint32_t ticket = atomicAdd(counter, 1u);
+            last = ticket == gridDim.x - 1;
+        }
     }
     __syncthreads();
     if (!last) return;
     __threadfence();
-    Partial q = {0.0, 0.0, 0.0, 0u, 0u};
-    const volatile Partial *vp = partials;
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
-        q.err += vp[b].err;
-        q.dang += vp[b].dang;
-        q.sum += vp[b].sum;
-        q.active += vp[b].active;
-        q.cmax = max(q.cmax, (uint32_t)vp[b].cmax);
-        q.cnmin = max(q.cnmin, (uint32_t)vp[b].cnmin);
-    }
+    Partial q = zero;
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) add_partial(q, partials[b]);
     wave_reduce(q);
     __syncthreads();
     if (lane == 0) sp[wid] = q;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        Partial tot = sp[0];
-        for (int w = 1; w < kBlock / 64; w++) add_partial(tot, sp[w]);
-        *counter = 0;
-        fin(tot);
+    if (wid == 0) {
+        Partial tot = lane < kWaves ? sp[lane] : zero;
+        wave_reduce(tot);
+        if (lane == 0) {
+            *counter = 0;
+            fin(tot);
+        }
     }
 }
 

@@ -482,15 +482,25 @@ __device__ __forceinline__ void tile_loop(const PlanView &P, const Gather &g, Ep
     }
 }
 
-// Copy the hot prefix of the contribution vector into LDS.
+// Copy the hot prefix of the contribution vector into LDS: each thread's
+// loads (up to 8 of 16 bytes) issued before its first LDS write, so a full
+// 156 KiB copy takes two round trips, not ten.
 template <class T>
 __device__ __forceinline__ void load_hot(const T *__restrict__ c_in, uint32_t hot)
 {
+    constexpr uint32_t K = 8;
     T *h = reinterpret_cast<T *>(gbx_dyn_lds);
     const uint32_t words = (uint32_t)((hot * sizeof(T) + 15) / 16);
     const uint4 *src = reinterpret_cast<const uint4 *>(c_in);
     uint4 *dst = reinterpret_cast<uint4 *>(h);
-    for (uint32_t i = threadIdx.x; i < words; i += kBlock) dst[i] = src[i];
+    for (uint32_t i0 = threadIdx.x; i0 < words; i0 += kBlock * K) {
+        // indexes past the last word repeat it (same value, same address)
+        uint4 w[K];
+#pragma unroll
+        for (uint32_t k = 0; k < K; k++) w[k] = src[min(i0 + k * kBlock, words - 1u)];
+#pragma unroll
+        for (uint32_t k = 0; k < K; k++) dst[min(i0 + k * kBlock, words - 1u)] = w[k];
+    }
     __syncthreads();
 }
 
