@@ -27,22 +27,33 @@ def _oracle_csr(z):
     return O.build_graph(z["edges_src"], z["edges_dst"], None, int(z["m1_N"]), mode=1)
 
 
-@pytest.fixture()
-def small_cut_ctx():
-    """A context whose binned layout uses tiny chunks and bins."""
+@pytest.fixture(params=[16, 64])
+def small_cut_ctx(request):
+    """A context whose binned layout uses tiny chunks and bins and only 16 or
+    64 hot sources, so that most edges of these small graphs take the cold
+    path (with the default 16,384 hot sources every source of a graph this
+    small is hot)."""
     ctx = gbx.Context(0)
     t = ctx.tuning()
     t.binned_chunk_edges = 96
     t.binned_bin_edges = 64
     t.binned_bin_rows = 37
     t.binned_chunk_rows = 50
+    t.binned_hot_sources = request.param
     ctx.set_tuning(t)
     yield ctx
     ctx.close()
 
 
+# hot sources: the default (every source of these small graphs is hot, so
+# only the hot pairs carry edges) and 16 (most edges through the scatter)
+HOT = [16384, 16]
+
+
+@pytest.mark.parametrize("hot", HOT)
 @pytest.mark.parametrize("name", GRAPHS)
-def test_binned_ref_matches_reference(gpu_ctx, golden, name):
+def test_binned_ref_matches_reference(gpu_ctx, golden, tuning, name, hot):
+    tuning(binned_hot_sources=hot)
     z = golden(name)
     print(f"[gpu] binned REF {name}", flush=True)
     g = _graph(gpu_ctx, z)
@@ -68,16 +79,20 @@ def test_binned_ref_epsilon_stop(gpu_ctx, golden, name):
         assert l1(ranks, z["pr_eps_ranks"]) <= TOL_F32
 
 
+@pytest.mark.parametrize("hot", HOT)
 @pytest.mark.parametrize("name", ["rmat_s12", "fixture_v8192_e32768"])
-def test_binned_forward_direction(gpu_ctx, golden, name):
+def test_binned_forward_direction(gpu_ctx, golden, tuning, name, hot):
+    tuning(binned_hot_sources=hot)
     z = golden(name)
     g = _graph(gpu_ctx, z)
     ranks, _ = g.pagerank(iterations=20, epsilon=0.0, direction=gbx.PULL_FWD, method=BINNED)
     assert l1(ranks, z["pr_fwd_r20"]) <= TOL_F32
 
 
+@pytest.mark.parametrize("hot", HOT)
 @pytest.mark.parametrize("name", ["rmat_s10", "rmat_s12", "fixture_v8192_e32768"])
-def test_binned_stochastic_matches_fp64(gpu_ctx, golden, name):
+def test_binned_stochastic_matches_fp64(gpu_ctx, golden, tuning, name, hot):
+    tuning(binned_hot_sources=hot)
     z = golden(name)
     g = _graph(gpu_ctx, z)
     fwd, inv = _oracle_csr(z)
@@ -106,10 +121,11 @@ def _edge_values(ctx, on):
 
 @pytest.mark.parametrize("name", ["rmat_s12", "fixture_v8192_e32768", "fixture_v500_e500"])
 def test_binned_small_cut_layout(small_cut_ctx, golden, name):
-    """Thousands of segments, bins of <= 37 rows, rows split over several
-    accumulate blocks.  Run layout: the cut changes which edges one FP32 run
-    sum takes, so ranks agree with the default cut's to rounding.  One V
-    value per edge: exact integer sums make them bit-identical."""
+    """Thousands of segments, bins of <= 32 rows, chunks of <= 48 sources,
+    rows split over several accumulate blocks.  Run layout: the cut changes
+    which edges one FP32 run sum takes, so ranks agree with the default cut's
+    (every source hot here) to rounding.  One V value per edge: exact integer
+    sums make them bit-identical."""
     z = golden(name)
     print(f"[gpu] binned small cut {name}", flush=True)
     g = _graph(small_cut_ctx, z)
