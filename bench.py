@@ -188,9 +188,10 @@ def time_session(pr, warmup, steps, graph_replay=False):
     return timed_steps(pr, steps, graph_replay)
 
 
-def time_form(g, warmup, steps, **kw):
-    """Warm up, then time `steps` iterations (timed_steps).  Returns
-    (kernel_ms, wall_ms, ranks, session)."""
+def time_form(g, warmup, steps, graph_replay=False, **kw):
+    """Warm up, then time `steps` iterations (timed_steps), launched one by
+    one or replayed as a captured HIP graph.  Returns (kernel_ms, wall_ms,
+    ranks, session)."""
     import graphblox_amd as gbx
 
     k = kw.pop("num_vectors", 0)
@@ -199,7 +200,7 @@ def time_form(g, warmup, steps, **kw):
         pr = g.ppr_session(k, seeds, iterations=warmup + steps, epsilon=0.0, stop=gbx.STOP_NONE)
     else:
         pr = g.session(iterations=warmup + steps, epsilon=0.0, stop=gbx.STOP_NONE, **kw)
-    kernel_ms, wall_ms, ranks, _ = time_session(pr, warmup, steps)
+    kernel_ms, wall_ms, ranks, _ = time_session(pr, warmup, steps, graph_replay)
     return kernel_ms, wall_ms, ranks, pr
 
 
@@ -288,7 +289,7 @@ def forms_s22(gbx, O, ctx, args):
         log(f"batched form autotuned in {time.perf_counter() - t0:.2f} s")
     ppr_tuned = {k: v for k, v in best.as_dict().items() if k.startswith("ppr")}
     log(f"batched form tuning: {ppr_tuned}")
-    kb, wb, rb, sb = time_form(gb, args.warmup, args.steps, num_vectors=32, seed_sets=sets)
+    kb, wb, rb, sb = time_form(gb, args.warmup, args.steps, graph_replay=True, num_vectors=32, seed_sets=sets)
     sb.free()
     errs = []
     for q in range(0, 32, 8):
@@ -297,7 +298,8 @@ def forms_s22(gbx, O, ctx, args):
         r64q, _, _ = O.pr_f64(Nb, fo.out_degree, rp, io.dst, mode=1, iterations=args.steps, teleport=t)
         errs.append(fp64_errors(rb[:, q], r64q))
     out["batched_ppr32"] = form_record(
-        "32 personalised vectors, one multi-vector launch", Eb, Nb, 32, kb, wb,
+        "32 personalised vectors, one multi-vector launch per iteration, the iterations replayed as a HIP graph",
+        Eb, Nb, 32, kb, wb,
         4 * (Nb + 1) + 4 * Nb + 4 * Eb + 32 * (4 * Eb + 8 * Nb), base32,
         l1_vs_fp64=max(e["l1_vs_fp64"] for e in errs), max_rel_vs_fp64=max(e["max_rel_vs_fp64"] for e in errs),
         fp64_checked_vectors="4 of 32 (0, 8, 16, 24)", autotuned=ppr_tuned)

@@ -65,6 +65,25 @@
 
 namespace gbx {
 
+// Streaming (read-once) arrays of the binned iteration -- src_local and the
+// block records' groups in the scatter, V, dst_local and the hot pairs in the
+// accumulate, and their weights -- are loaded non-temporally, so that the
+// caches keep what is read again: the rank, degree and contribution arrays of
+// the accumulate's update and V itself between the scatter that writes it and
+// the accumulate that reads it.  Measured at R-MAT s24: iteration -4.5 %;
+// non-temporal stores of V made the scatter 16 % slower and are not used.
+typedef unsigned int nt_u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint4 ld_nt16(const void *p)
+{
+    const nt_u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_u32x4 *>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+}  // namespace gbx
+
+namespace gbx {
+
 constexpr uint32_t kMaxBinRows = 20000;     // LDS int64 row sums of one bin
 constexpr uint32_t kNoEdge = 0xFFFFFFFFu;   // no plan edge behind a layout entry
 constexpr uint32_t kMaxChunkRows = 32768;   // LDS contributions of one chunk; u16 local ids
@@ -999,10 +1018,10 @@ __device__ __forceinline__ void scatter_runs_piece(const RunsView &S, const T *_
     // (clamped into [g0, g1)), lanes outside it take pads when summed
     auto load = [&](int d, uint32_t first) {
         const uint32_t g = min(max(first + lane, g0), g1 - 1u);
-        w[d] = src[g];
+        w[d] = ld_nt16(src + g);
         if constexpr (W) {
-            wt[d].a = wsrc[2u * g];
-            wt[d].b = wsrc[2u * g + 1u];
+            wt[d].a = ld_nt16(wsrc + 2u * g);
+            wt[d].b = ld_nt16(wsrc + 2u * g + 1u);
         }
         rec[d] = S.blk_rec[min(first / kRunBlockGroups, blk_last)];
     };
@@ -1132,7 +1151,20 @@ __device__ __forceinline__ long long fixed_of(RawVal<KIND> v, double scale)
 template <int KIND, class T>
 __device__ __forceinline__ void load8(const T *V, uint32_t e, RawVal<KIND> (&v)[8])
 {
-    const Vec8<T> w = *reinterpret_cast<const Vec8<T> *>(V + e);
+    Vec8<T> w;
+    if constexpr (sizeof(T) == 4) {
+        const uint4 a = ld_nt16(V + e), b = ld_nt16(V + e + 4);
+        const uint32_t u[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int i = 0; i < 8; i++) w.v[i] = __builtin_bit_cast(T, u[i]);
+    } else if constexpr (sizeof(T) == 2) {
+        const uint4 a = ld_nt16(V + e);
+        const uint32_t u[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int i = 0; i < 8; i++) w.v[i] = (T)((u[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu);
+    } else {
+        w = *reinterpret_cast<const Vec8<T> *>(V + e);
+    }
 #pragma unroll
     for (int i = 0; i < 8; i++) v[i] = raw_of<KIND, T>(w.v[i]);
 }
@@ -1142,8 +1174,8 @@ template <int KIND, class T>
 __device__ __forceinline__ void load_hot8(const uint32_t *pairs, uint32_t e, const T *ch, uint32_t (&d)[8],
                                           RawVal<KIND> (&v)[8])
 {
-    const uint4 a = *reinterpret_cast<const uint4 *>(pairs + e);
-    const uint4 b = *reinterpret_cast<const uint4 *>(pairs + e + 4);
+    const uint4 a = ld_nt16(pairs + e);
+    const uint4 b = ld_nt16(pairs + e + 4);
     const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
     for (int i = 0; i < 8; i++) {
@@ -1159,7 +1191,7 @@ template <int KIND, class T>
 __device__ __forceinline__ void load_hot8_packed(const uint16_t *pairs, const uint16_t *base, uint32_t e,
                                                  const T *ch, uint32_t (&d)[8], RawVal<KIND> (&v)[8])
 {
-    const uint4 a = *reinterpret_cast<const uint4 *>(pairs + e);
+    const uint4 a = ld_nt16(pairs + e);
     const uint32_t w[4] = {a.x, a.y, a.z, a.w};
     uint32_t r = base[e >> 3];
 #pragma unroll
@@ -1181,9 +1213,10 @@ __device__ __forceinline__ void load_hot8_any(const AccumView &B, uint32_t e, co
         load_hot8<KIND, T>(B.hot_pairs, e, ch, d, v);
     if constexpr (is_weighted_kind(KIND)) {
         // the pair's edge weight times its source's value (weights of fillers are 0)
-        const Vec8<float> w = *reinterpret_cast<const Vec8<float> *>(B.whot + e);
+        const uint4 a = ld_nt16(B.whot + e), b = ld_nt16(B.whot + e + 4);
+        const uint32_t u[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-        for (int i = 0; i < 8; i++) v[i] = w.v[i] * v[i];
+        for (int i = 0; i < 8; i++) v[i] = __uint_as_float(u[i]) * v[i];
     }
 }
 
@@ -1426,7 +1459,7 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, uint32_t uw, const 
 #pragma unroll
             for (int q = 0; q < kAccumUnroll; q++)
                 if (e0 + q * kq < u.vend) {
-                    dw[q] = *reinterpret_cast<const uint4 *>(B.dst_local + e0 + q * kq);
+                    dw[q] = ld_nt16(B.dst_local + e0 + q * kq);
                     load8<KIND, T>(V, e0 + q * kq, v[q]);
                 }
 #pragma unroll
