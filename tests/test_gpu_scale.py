@@ -12,6 +12,9 @@ build on every graph the goldens hold, tests/test_oracle.py):
   iterations, both methods, against the port and FP64; the precision form
   (BF16 contributions) and FP32 to tolerance 1e-7 against FP64 (L1 and the
   largest relative error reported);
+* R-MAT s26 (67,108,864 vertices, 1.06 billion edges): reference semantics
+  after 20 iterations against the port and FP64, stochastic ranks summing
+  to 1;
 * 256 independent R-MAT graphs with 2^10..2^18 vertices (log-uniform) and
   edge factors 8..32 in one batch, every graph against FP64;
 * 32 personalised vectors (seed sets of 1..1000 vertices) at s22, eight of
@@ -134,6 +137,29 @@ def test_s24_precision_form_vs_fp64(s24):
             assert rep["converged"] == 1 or rep["error_total"] <= 1e-6
         assert e <= bar
         assert abs(rep["rank_sum"] - 1.0) < 1e-5
+
+
+def test_s26_binned_vs_port_and_fp64(gpu_ctx):
+    """The largest graph of the suite: R-MAT s26 (67,108,864 vertices, 1.06
+    billion edges), reference semantics with the binned iteration (the
+    library's default at this size) after 20 iterations against the port and
+    FP64, and stochastic ranks summing to 1."""
+    g = gpu_ctx.rmat_graph(26, 16, 26)
+    print(f"[gpu] R-MAT s26: N={g.N} E={g.E}", flush=True)
+    try:
+        r, rep = g.pagerank(iterations=20, epsilon=0.0)
+        rs, reps = g.pagerank(iterations=20, epsilon=0.0, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_NONE)
+        fwd, inv, rp = _oracle_of(g)
+    finally:
+        g.free()
+    port = O.pr_pull_f32(fwd, inv, 20, 0.0, nthreads=0)["ranks"]
+    r64, _, _ = O.pr_f64(fwd.N, fwd.out_degree, rp, inv.dst, mode=0, iterations=20)
+    e_port, e64 = l1(r, port), l1(r, r64)
+    print(f"[gpu] s26 REF 20 iterations: L1 vs port {e_port:.3e}, vs FP64 {e64:.3e}, "
+          f"max rel vs FP64 {_max_rel(r, r64):.3e}", flush=True)
+    assert rep["executed"] == 20
+    assert e_port <= 1e-5 and e64 <= 1e-5
+    assert abs(reps["rank_sum"] - 1.0) < 1e-5 and abs(float(np.sum(rs, dtype=np.float64)) - 1.0) < 1e-5
 
 
 def test_batch_of_256_graphs_at_full_sizes(gpu_ctx):
