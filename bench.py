@@ -381,7 +381,8 @@ def forms_batch(gbx, O, ctx, args):
     iteration (the binned iteration at this size: scatter, accumulate with
     each unit's graph coefficients, per-graph reduction), the iterations
     replayed as one captured HIP graph, the layout cut autotuned on the
-    batch; the pull tiles beside it."""
+    batch (--no-autotune: the library's default cut); the pull tiles beside
+    it."""
     rng = np.random.default_rng(256)
     scales = rng.integers(10, 19, 256)
     efs = rng.integers(8, 33, 256)
@@ -390,7 +391,11 @@ def forms_batch(gbx, O, ctx, args):
     off = b.batch_offsets()
     N, E = b.N, b.E
     tune0 = ctx.tuning()
-    best, tuned_ms = ctx.autotune(b, 0, 5, iterations=10, epsilon=0.0, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_NONE)
+    if args.no_autotune:  # the library's default cut (counter runs want one layout)
+        best, tuned_ms = tune0, None
+    else:
+        best, tuned_ms = ctx.autotune(b, 0, 5, iterations=10, epsilon=0.0, mode=gbx.PR_STOCHASTIC,
+                                      stop=gbx.STOP_NONE)
     s = b.session(iterations=args.steps, epsilon=0.0, mode=gbx.PR_STOCHASTIC, stop=gbx.STOP_NONE)
     km, wm, ranks, reps = time_session(s, args.warmup, args.steps, graph_replay=True)
     s.free()
@@ -419,7 +424,7 @@ def forms_batch(gbx, O, ctx, args):
                       max_rel_vs_fp64=max(e["max_rel_vs_fp64"] for e in errs),
                       fp64_checked_graphs="4 of 256 (0, 85, 170, 255)",
                       autotuned={**{k: v for k, v in best.as_dict().items() if k.startswith("binned")},
-                                 "trial_ms": round(tuned_ms, 5)},
+                                 "trial_ms": None if tuned_ms is None else round(tuned_ms, 5)},
                       convergence={"epsilon": 1e-7, "wall_ms": round(conv_ms, 3),
                                    "iterations_max": max(r["executed"] for r in creps),
                                    "all_converged": all(r["converged"] for r in creps)})
