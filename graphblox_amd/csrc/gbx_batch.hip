@@ -211,19 +211,17 @@ __global__ void __launch_bounds__(256) seg_reduce_kernel(SegView S, int stop, do
                     st.conv_iter = st.iters;
                 }
             }
-            S.coef[k] = cf;
-            S.stat[k] = st;
+            handoff_store_words<uint32_t>(S.coef + k, cf);
+            handoff_store_words<uint64_t>(S.stat + k, st);
         }
-        __threadfence();
+        stores_performed();
         last = atomicAdd(S.counter, 1u) == gridDim.x - 1;
     }
     __syncthreads();
     if (!last) return;
     // the last block: every graph's flag, read by all of its threads
-    __threadfence();
-    const volatile GCoef *vc = S.coef;
     int all = 1;
-    for (uint32_t j = threadIdx.x; j < S.nseg; j += blockDim.x) all &= vc[j].done ? 1 : 0;
+    for (uint32_t j = threadIdx.x; j < S.nseg; j += blockDim.x) all &= handoff_load(&S.coef[j].done) ? 1 : 0;
     all = __syncthreads_and(all);
     if (threadIdx.x == 0) {
         *S.all_done = all ? 1u : 0u;
@@ -331,24 +329,21 @@ __global__ void __launch_bounds__(256) segbin_reduce_kernel(
                     st.conv_iter = st.iters;
                 }
             }
-            coef[k] = cf;
-            stat[k] = st;
+            handoff_store_words<uint32_t>(coef + k, cf);
+            handoff_store_words<uint64_t>(stat + k, st);
         }
-        __threadfence();
+        stores_performed();
         last = atomicAdd(counter, 1u) == gridDim.x - 1;
     }
     __syncthreads();
     if (!last) return;
     // the last block: every graph's flag and rank sum, read by all of its
     // threads, the sums added in a fixed tree order (deterministic)
-    __threadfence();
-    const volatile GCoef *vc = coef;
-    const volatile GStat *vs = stat;
     int all = 1;
     double total = 0.0;
     for (uint32_t j = threadIdx.x; j < nseg; j += 256) {
-        all &= vc[j].done ? 1 : 0;
-        total += vs[j].sum;
+        all &= handoff_load(&coef[j].done) ? 1 : 0;
+        total += handoff_load(&stat[j].sum);
     }
     all = __syncthreads_and(all);
     red[0][threadIdx.x] = total;

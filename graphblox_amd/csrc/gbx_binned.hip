@@ -1374,12 +1374,14 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, uint32_t uw, const 
             for (int w = 0; w < kBlock / 64; w++) tot += wsum[w];
             bool apply = live;
             if (live && u.multi != kNoMulti) {
+                // both the sum and the ticket are device-scope atomics: the
+                // sum is performed before the ticket is taken, and the last
+                // unit reads it back with an atomic (no fences)
                 atomicAdd(reinterpret_cast<unsigned long long *>(B.acc + u.multi), (unsigned long long)tot);
-                __threadfence();
+                stores_performed();
                 const uint32_t ticket = atomicAdd(B.cnt + u.multi, 1u);
                 apply = ticket == u.nunits - 1;
                 if (apply) {
-                    __threadfence();
                     tot = (long long)atomicAdd(reinterpret_cast<unsigned long long *>(B.acc + u.multi), 0ull);
                     B.acc[u.multi] = 0;
                     B.cnt[u.multi] = 0;
@@ -1584,7 +1586,6 @@ __global__ void __launch_bounds__(kBlock) bin_accum_kernel(AccumView B, const ui
         // last block to leave its unit loop clears the unit counter
         __syncthreads();
         if (threadIdx.x == 0) {
-            __threadfence();
             if (atomicAdd(A.counter, 1u) == gridDim.x - 1) {
                 *B.work = 0;
                 *A.counter = 0;

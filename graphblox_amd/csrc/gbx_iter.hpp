@@ -93,10 +93,10 @@ __device__ __forceinline__ void add_partial(Partial &q, const Partial &b)
 
 // Block partial -> partials[blockIdx]; the last block reduces all partials
 // and calls fin(total) on thread 0.  Within a block the waves' partials are
-// combined by wave 0 (lane w holds wave w's); the last block reads every
-// block's partial with ordinary loads after its acquire fence (one round trip
-// each, where element-wise volatile loads would each wait for memory), then
-// reduces them in a fixed tree order, so the result is deterministic.
+// combined by wave 0 (lane w holds wave w's); the partials are handed over
+// with the hand-off stores and loads of gbx_plan.hpp (no device-scope fence),
+// each thread of the last block loading its blocks' words in one round trip,
+// then reduced in a fixed tree order, so the result is deterministic.
 template <class Fin>
 __device__ __forceinline__ void grid_reduce(Partial p, Partial *partials, uint32_t *counter, Fin fin)
 {
@@ -112,17 +112,16 @@ __device__ __forceinline__ void grid_reduce(Partial p, Partial *partials, uint32
         Partial q = lane < kWaves ? sp[lane] : zero;
         wave_reduce(q);
         if (lane == 0) {
-            partials[blockIdx.x] = q;
-            __threadfence();
+            handoff_store_words<uint64_t>(partials + blockIdx.x, q);
+            stores_performed();
             const uint32_t ticket = atomicAdd(counter, 1u);
             last = ticket == gridDim.x - 1;
         }
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();
     Partial q = zero;
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) add_partial(q, partials[b]);
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) add_partial(q, handoff_load_words<uint64_t>(partials + b));
     wave_reduce(q);
     __syncthreads();
     if (lane == 0) sp[wid] = q;

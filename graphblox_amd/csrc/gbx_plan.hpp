@@ -204,6 +204,67 @@ __device__ __forceinline__ void st_stream(T *p, T v)
 #endif
 }
 
+// ------------------------------------------------------------------ cross-block hand-off
+// Values that one block of a launch hands to another (a split row's chunk
+// sums, the blocks' partials of a grid reduction, per-graph flags) are
+// written and read with device-scope atomic stores and loads.  gfx950 keeps
+// those coherent across the L2s of its XCDs (they are performed at the
+// device's coherence point), so neither side needs a device-scope fence:
+// such a release writes back the whole L2 of the writer's XCD (every block
+// of an iteration fences, so the write-backs queue behind each other) and
+// the acquire invalidates the reader's L2, together about 7 us of a 21 us
+// iteration at R-MAT s16.  The protocol:
+//   writer: handoff_store(...) of every value, stores_performed(), then the
+//           ticket (a device-scope atomicAdd);
+//   reader: the ticket's result says every writer has taken its ticket,
+//           then handoff_load(...) of the values.
+// A wave's stores are performed when its vmcnt has drained (gfx9 counts
+// stores there too), so every value is at the coherence point before its
+// writer's ticket is; the reader's loads issue after its own ticket
+// returned and read the coherence point.  Values read back by later
+// launches need nothing: a launch ends with its writes visible.
+template <class W>
+__device__ __forceinline__ void handoff_store(W *p, W v)
+{
+    static_assert(sizeof(W) == 4 || sizeof(W) == 8, "one atomic store per value");
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class W>
+__device__ __forceinline__ W handoff_load(const W *p)
+{
+    static_assert(sizeof(W) == 4 || sizeof(W) == 8, "one atomic load per value");
+    return __hip_atomic_load(const_cast<W *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Every store this wave issued has been performed (waits for the vector
+// memory counter; the memory clobber keeps the compiler from moving memory
+// accesses across it).
+__device__ __forceinline__ void stores_performed() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// A struct of 8-byte words (or 4-byte ones), word by word.
+template <class W, class S>
+__device__ __forceinline__ void handoff_store_words(S *p, const S &v)
+{
+    static_assert(sizeof(S) % sizeof(W) == 0, "whole words");
+    W w[sizeof(S) / sizeof(W)];
+    __builtin_memcpy(w, &v, sizeof(S));
+#pragma unroll
+    for (size_t k = 0; k < sizeof(S) / sizeof(W); k++) handoff_store(reinterpret_cast<W *>(p) + k, w[k]);
+}
+
+template <class W, class S>
+__device__ __forceinline__ S handoff_load_words(const S *p)
+{
+    static_assert(sizeof(S) % sizeof(W) == 0, "whole words");
+    W w[sizeof(S) / sizeof(W)];
+#pragma unroll
+    for (size_t k = 0; k < sizeof(S) / sizeof(W); k++) w[k] = handoff_load(reinterpret_cast<const W *>(p) + k);
+    S v;
+    __builtin_memcpy(&v, w, sizeof(S));
+    return v;
+}
+
 // ------------------------------------------------------------------ accumulators
 
 // Plain FP32 running sum: `acc += x` in visit order, as pageRank.c:313-314.
@@ -441,15 +502,14 @@ __device__ __forceinline__ void do_hub(uint32_t tile, const PlanView &P, const G
             // the whole row in one chunk: no cross-block combine, no fences
             epi(hc.row, epi.init(hc.row) + s.value());
         } else {
-            reinterpret_cast<Part *>(P.hub_part)[tile] = s.value();
-            __threadfence();
+            Part *hp = reinterpret_cast<Part *>(P.hub_part);
+            handoff_store(hp + tile, (Part)s.value());
+            stores_performed();
             uint32_t prev = atomicAdd(&P.hub_cnt[hc.hub], 1u);
             if (prev == hc.nch - 1) {
-                __threadfence();
                 Acc tot;
                 tot.init(Val(0));
-                const volatile Part *hp = reinterpret_cast<const volatile Part *>(P.hub_part);
-                for (uint32_t k = 0; k < hc.nch; k++) tot.add(hp[hc.first + k]);
+                for (uint32_t k = 0; k < hc.nch; k++) tot.add(handoff_load(hp + hc.first + k));
                 P.hub_cnt[hc.hub] = 0;
                 epi(hc.row, epi.init(hc.row) + tot.value());
             }

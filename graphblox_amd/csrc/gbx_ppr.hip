@@ -214,9 +214,9 @@ __global__ void __launch_bounds__(kBlock) ppr_iter_kernel(BPlan P, BatchView A)
             for (int v = 0; v < VPL; v++) {
                 float s = red[q * VPL + v];
                 for (uint32_t t2 = 1; t2 < TPB; t2++) s += red[(t2 * L + q) * VPL + v];
-                P.hub_part[(size_t)tile * V + q * VPL + v] = s;
+                handoff_store(P.hub_part + (size_t)tile * V + q * VPL + v, s);
             }
-            __threadfence();
+            stores_performed();  // before the block's ticket (after the barrier)
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -225,13 +225,13 @@ __global__ void __launch_bounds__(kBlock) ppr_iter_kernel(BPlan P, BatchView A)
         }
         __syncthreads();
         if (last_chunk && team == 0) {
-            __threadfence();
-            const volatile float *hp = P.hub_part;
+            const float *hp = P.hub_part;
             float tot[VPL];
 #pragma unroll
             for (int v = 0; v < VPL; v++) {
                 tot[v] = 0.0f;
-                for (uint32_t k = 0; k < hc.nch; k++) tot[v] += hp[(size_t)(hc.first + k) * V + q * VPL + v];
+                for (uint32_t k = 0; k < hc.nch; k++)
+                    tot[v] += handoff_load(hp + (size_t)(hc.first + k) * V + q * VPL + v);
             }
             epi(hc.row, row_in(hc.row), tot);
             if (q == 0) P.hub_cnt[hc.hub] = 0;
@@ -410,13 +410,12 @@ ppr_reduce_kernel(const double *__restrict__ partials, uint32_t nblocks, uint32_
     }
     __shared__ bool last;
     if (threadIdx.x == 0) {
-        scal_out[q] = BScal{red[0][0], red[1][0], red[2][0]};
-        __threadfence();
+        handoff_store_words<uint64_t>(scal_out + q, BScal{red[0][0], red[1][0], red[2][0]});
+        stores_performed();
         last = atomicAdd(counter, 1u) == gridDim.x - 1;
     }
     __syncthreads();
     if (!last || threadIdx.x != 0) return;
-    __threadfence();
     *counter = 0;
     if (is_reset) {
         state->done = 0;
@@ -427,9 +426,8 @@ ppr_reduce_kernel(const double *__restrict__ partials, uint32_t nblocks, uint32_
     }
     const uint32_t it = state->iters + 1;
     bool all = true;
-    const volatile BScal *vs = scal_out;
     for (uint32_t v = 0; v < k; v++) {
-        const bool ok = stop == GBX_STOP_L1 && vs[v].err < eps;
+        const bool ok = stop == GBX_STOP_L1 && handoff_load(&scal_out[v].err) < eps;
         if (ok && state->conv_iter[v] == 0) state->conv_iter[v] = it;
         all = all && ok;
     }
