@@ -836,6 +836,7 @@ constexpr int kRunsAhead = 4;     // steps per wave whose loads are in flight
 // at R-MAT s24, 0.39 V values per cold edge).  Measured at s24: 2 always
 // beats 6 by 4 % (FP32) and 5 % (BF16), 3-5 by 1-3 %.
 constexpr uint32_t kRunStores = 2;
+constexpr uint32_t kRunEarly = 4;  // stores whose staged sums are read ahead
 // Idle store lanes write a dummy area of their own wave (64 values, 256
 // bytes apart): one common dummy address would funnel a large share of
 // all stores into one cache line of one L2 channel.
@@ -877,13 +878,19 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v)
 // rank's delta from the last leader at or below it.  The write-out always issues at least
 // kRunStores stores (ranks past the wave's runs store to the wave's dummy
 // area), so the loads of later steps stay in flight across them.
+//
+// Staging writes every entry, without branches: an entry inside a run goes
+// to the slot of the run's end, which overwrites it later (a lane's writes
+// are in entry order, and its last entry ends a run), so a lane only ever
+// writes its own ranks; a pad lane (no runs) writes the slot past the wave's
+// runs, which the write-out never takes (it has at most 8 x 63 runs then).
 struct RunStep {
-    float v[8];      // running sum after each entry (a run's sum at its end)
-    uint32_t endm;   // entries that end a run
-    uint32_t pre;    // rank of the lane's first run
-    uint32_t delta;  // V index - rank, constant over a segment
-    uint32_t total;  // runs of the wave
-    uint64_t lead;   // lanes that lead a segment
+    float v[8];         // running sum after each entry (a run's sum at its end)
+    uint32_t slot[8];   // stage slot of each entry
+    uint32_t pre;       // rank of the lane's first run
+    uint32_t delta;     // V index - rank, constant over a segment
+    uint32_t total;     // runs of the wave
+    uint64_t lead;      // lanes that lead a segment
 };
 
 // Weights of a group of 8 entries (weighted kinds), two 16-byte loads.
@@ -907,31 +914,36 @@ __device__ __forceinline__ RunStep runs_sum(const float *cl, uint4 w, const W8 &
 #pragma unroll
         for (int i = 0; i < 8; i++) x[i] = __uint_as_float(wb[i]) * x[i];  // the edge's weight times its source's value
     }
-    st.endm = 0;
+    // an entry ends its run when the top bit of its 16-bit half is set
+    bool end[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) st.endm |= ((ew[i >> 1] >> ((i & 1) ? 31 : 15)) & 1u) << i;
-    const uint32_t cnt = (uint32_t)__popc(st.endm);
+    for (int i = 0; i < 8; i++)
+        end[i] = (i & 1) ? (int32_t)ew[i >> 1] < 0 : (int16_t)(uint16_t)ew[i >> 1] < 0;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) cnt += (uint32_t)__popc(ew[k] & (kRunEnd | (kRunEnd << 16)));
     const uint32_t inc = wave_inclusive_sum(cnt);
     st.pre = inc - cnt;
     st.total = __builtin_amdgcn_readlane(inc, 63);
     st.delta = vbase - st.pre;  // meaningful on leaders
     st.lead = __ballot(leader);
+    uint32_t o = cnt ? st.pre : st.total;
     float s = 0.f;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
         s += x[i];
         st.v[i] = s;
-        s = ((st.endm >> i) & 1u) ? 0.f : s;
+        st.slot[i] = o;
+        o += end[i] ? 1u : 0u;
+        s = end[i] ? 0.f : s;
     }
     return st;
 }
 
 __device__ __forceinline__ void runs_stage(float *stage, const RunStep &st)
 {
-    uint32_t o = st.pre;
 #pragma unroll
-    for (int i = 0; i < 8; i++)
-        if ((st.endm >> i) & 1u) stage[o++] = st.v[i];
+    for (int i = 0; i < 8; i++) stage[st.slot[i]] = st.v[i];
 }
 
 template <class T>
@@ -939,15 +951,21 @@ __device__ __forceinline__ void runs_out(const float *stage, const RunStep &st, 
                                          T *__restrict__ V)
 {
     char *Vb = reinterpret_cast<char *>(V);
+    // the first kRunStores stores always (their count is what the loads'
+    // waits assume), more only while ranks are left; the staged sums of the
+    // first kRunEarly stores (all of a step's at R-MAT s24, about 200 runs)
+    // are read before the first store, the others just before theirs
+    const uint32_t nst = max(kRunStores, (st.total + 63u) / 64u);
+    float vals[kRunEarly];
+#pragma unroll
+    for (uint32_t r = 0; r < kRunEarly; r++) vals[r] = stage[lane + 64u * r];
     uint64_t lead = st.lead;  // the lowest leader has rank 0 when the step has runs
     uint32_t cur = 0;
     uint32_t nl = lead ? (uint32_t)__builtin_ctzll(lead) : 0u;
     uint32_t np = lead ? __builtin_amdgcn_readlane(st.pre, nl) : 0xFFFFFFFFu;
 #pragma unroll
     for (uint32_t r = 0; r < kRunGroup; r++) {
-        // the first kRunStores stores always (their count is what the loads'
-        // waits assume), more only while ranks are left
-        if (r >= kRunStores && 64u * r >= st.total) break;
+        if (r >= nst) break;
         const uint32_t q = lane + 64u * r;
         uint32_t d = cur;
         while (np < 64u * (r + 1)) {  // segments whose first rank falls in this store
@@ -958,7 +976,7 @@ __device__ __forceinline__ void runs_out(const float *stage, const RunStep &st, 
             nl = lead ? (uint32_t)__builtin_ctzll(lead) : 0u;
             np = lead ? __builtin_amdgcn_readlane(st.pre, nl) : 0xFFFFFFFFu;
         }
-        const float v = stage[q];
+        const float v = r < kRunEarly ? vals[r] : stage[q];
         put_f32(reinterpret_cast<T *>(Vb + (q < st.total ? (q + d) * (uint32_t)sizeof(T) : dummy_off)), 0, v);
     }
 }
