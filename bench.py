@@ -188,10 +188,11 @@ def time_session(pr, warmup, steps, graph_replay=False):
     return timed_steps(pr, steps, graph_replay)
 
 
-def time_form(g, warmup, steps, graph_replay=False, **kw):
-    """Warm up, then time `steps` iterations (timed_steps), launched one by
-    one or replayed as a captured HIP graph.  Returns (kernel_ms, wall_ms,
-    ranks, session)."""
+def time_form(g, warmup, steps, graph_replay=True, **kw):
+    """Warm up, then time `steps` iterations (timed_steps), replayed as a
+    captured HIP graph like the headline (launched one by one with
+    --no-graphs or graph_replay=False).  Returns (kernel_ms, wall_ms, ranks,
+    session)."""
     import graphblox_amd as gbx
 
     k = kw.pop("num_vectors", 0)
@@ -289,7 +290,7 @@ def forms_s22(gbx, O, ctx, args):
         log(f"batched form autotuned in {time.perf_counter() - t0:.2f} s")
     ppr_tuned = {k: v for k, v in best.as_dict().items() if k.startswith("ppr")}
     log(f"batched form tuning: {ppr_tuned}")
-    kb, wb, rb, sb = time_form(gb, args.warmup, args.steps, graph_replay=True, num_vectors=32, seed_sets=sets)
+    kb, wb, rb, sb = time_form(gb, args.warmup, args.steps, num_vectors=32, seed_sets=sets)
     sb.free()
     errs = []
     for q in range(0, 32, 8):
@@ -561,12 +562,18 @@ def convergence(gbx, ctx, args, g, method, m):
     """Reference semantics, stop when no vertex changes by 1e-4 or more
     (pageRank.c:327-339), at most 100 iterations; wall time from reset to
     read-back of the ranks, on the graph whose plan and binned layout exist
-    (warm), and one-shot on a graph just built, which pays both (cold)."""
+    (warm: the 100 iterations replayed as a HIP graph, like the headline),
+    and one-shot on a graph just built, which pays both and launches its
+    iterations eagerly, 8 between stop checks (cold: gbx_pr_pull, the
+    drop-in call)."""
     conv_s = g.session(iterations=100, epsilon=1e-4, mode=gbx.PR_REF, stop=gbx.STOP_ACTIVE, method=m)
+    conv_s.reset()
+    launch_iters(conv_s, 100)  # the 100 iterations captured as a HIP graph once, outside the timing
+    conv_s.read()
     ctx.sync()
     t0 = time.perf_counter()
     conv_s.reset()
-    conv_s.launch(100)
+    launch_iters(conv_s, 100)  # iterations after the stop rule fired exit at once
     _, conv_rep = conv_s.read()
     warm_ms = (time.perf_counter() - t0) * 1e3
     conv_s.free()

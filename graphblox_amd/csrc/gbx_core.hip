@@ -244,13 +244,14 @@ void tuning_defaults(gbx_tuning *t)
     t->lds_bytes = 159744;  // 156 KiB of the 160 KiB LDS of a CU (1024-thread block, one per CU)
     t->blocks_per_cu = 1;
     // batched form, measured at R-MAT s22 with 32 vectors (ms per
-    // iteration): rows of 4,096+ edges chunked in 8,192 (1,024 in 2,048:
-    // 2.51, 4,096: 1.60); one, two, four contributions a lane 1.58, 1.78,
-    // 1.60; four with four blocks per CU 1.48 (the autotuner's pick; the
-    // blocks per CU also size a batch of graphs' tile grid)
+    // iteration, HIP graph replays): rows of 4,096+ edges chunked in 8,192;
+    // four contributions a lane with one, two, four blocks per CU 1.64,
+    // 1.40, 1.27 (one a lane: 1.39 with four, 1.38 with two); rows from
+    // 1,024 edges chunked: 1.39 at one to four blocks per CU (the blocks
+    // per CU also size a batch of graphs' tile grid)
     t->ppr_hub = 4096;
     t->ppr_chunk = 8192;
-    t->ppr_blocks_per_cu = 2;
+    t->ppr_blocks_per_cu = 4;
     t->ppr_vectors_per_lane = 4;
     t->binned_min_vertices = 786432;  // measured crossover: pull faster at R-MAT s19, binned from s20
     t->binned_chunk_edges = 0;  // automatic

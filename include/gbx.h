@@ -416,8 +416,10 @@ int gbx_tuning_set(gbx_ctx *ctx, const gbx_tuning *t);
  * hub chunk, LDS prefix and blocks per CU for the pull kernel; edges per
  * chunk and per bin, rows per bin and hot sources for the binned one; bin
  * thresholds, hub chunk and ppr_blocks_per_cu for a batch of graphs; lanes,
- * hub cut and blocks per CU for the batched form), installs the fastest in
- * ctx and returns it in *best with its mean iteration time. */
+ * hub cut and blocks per CU for the batched form), confirms the pick against
+ * the starting setting on longer runs (the start stays unless the pick is
+ * faster), installs it in ctx and returns it in *best with its iteration
+ * time. */
 int gbx_autotune(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, uint32_t num_vectors,
                  uint32_t iters_per_trial, gbx_tuning *best, double *best_ms);
 
