@@ -3,6 +3,7 @@
 #include <cstdarg>
 #include <cstdlib>
 #include <mutex>
+#include <thread>
 
 #include "gbx_plan.hpp"
 
@@ -234,6 +235,67 @@ int max_u32(gbx_ctx *ctx, const uint32_t *a, size_t n, uint32_t *h_max)
     return GBX_OK;
 }
 
+// Large copies go through the context's pinned staging buffer, two chunks
+// in flight: the DMA of chunk i + 1 runs while host threads copy chunk i
+// out (a copy straight into pageable memory runs at about 6 GB/s: 11 ms for
+// the 16.7M ranks of R-MAT s24).
+int copy_to_host(gbx_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    constexpr size_t kDirect = 1u << 20, kChunk = 32u << 20;
+    if (bytes <= kDirect) {
+        GBX_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return GBX_OK;
+    }
+    const size_t chunk = std::min(bytes, kChunk);
+    if (ctx->h_stage_bytes < 2 * chunk) {
+        if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+        ctx->h_stage = nullptr;
+        ctx->h_stage_bytes = 0;
+        if (hipHostMalloc(&ctx->h_stage, 2 * chunk, 0) != hipSuccess) {
+            ctx->h_stage = nullptr;
+            return set_error(ctx, GBX_ERR_NOMEM, "copy_to_host: pinned staging of %zu bytes", 2 * chunk);
+        }
+        ctx->h_stage_bytes = 2 * chunk;
+    }
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int rc = GBX_OK;
+    for (auto &e : ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            rc = set_error(ctx, GBX_ERR_HIP, "copy_to_host: event");
+    char *stage = static_cast<char *>(ctx->h_stage);
+    const char *s = static_cast<const char *>(src);
+    char *d = static_cast<char *>(dst);
+    const size_t nch = (bytes + chunk - 1) / chunk;
+    auto issue = [&](size_t i) {
+        const size_t off = i * chunk, n = std::min(chunk, bytes - off);
+        if (hipMemcpyAsync(stage + (i & 1) * chunk, s + off, n, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipEventRecord(ev[i & 1], ctx->stream) != hipSuccess)
+            rc = set_error(ctx, GBX_ERR_HIP, "copy_to_host: %s", hipGetErrorString(hipGetLastError()));
+    };
+    const unsigned nthr = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    if (!rc) issue(0);
+    for (size_t i = 0; i < nch && !rc; i++) {
+        if (i + 1 < nch) issue(i + 1);  // its slot was copied out in the previous round
+        if (rc) break;
+        if (hipEventSynchronize(ev[i & 1]) != hipSuccess) {
+            rc = set_error(ctx, GBX_ERR_HIP, "copy_to_host: %s", hipGetErrorString(hipGetLastError()));
+            break;
+        }
+        const size_t off = i * chunk, n = std::min(chunk, bytes - off), part = (n + nthr - 1) / nthr;
+        const char *from = stage + (i & 1) * chunk;
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < nthr && t * part < n; t++)
+            th.emplace_back([=] { std::memcpy(d + off + t * part, from + t * part, std::min(part, n - t * part)); });
+        std::memcpy(d + off, from, std::min(part, n));
+        for (auto &x : th) x.join();
+    }
+    if (rc) (void)hipStreamSynchronize(ctx->stream);  // no DMA left in flight into the staging
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
+}
+
 void tuning_defaults(gbx_tuning *t)
 {
     // hub threshold and chunk 0: automatic, by the graph's size (auto_hub_threshold)
@@ -376,6 +438,7 @@ int gbx_free(gbx_ctx *ctx)
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
     }
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     delete ctx;
     return GBX_OK;
 }

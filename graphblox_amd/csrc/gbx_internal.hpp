@@ -74,6 +74,9 @@ struct gbx_ctx {
     gbx_tuning tune;
     // scratch reused across calls (grow-only)
     gbx::DevBuf scan_tmp, sort_hist, sort_tmp[3], misc;
+    // pinned staging for copies into the caller's host memory (grow-only)
+    void *h_stage = nullptr;
+    size_t h_stage_bytes = 0;
 };
 
 // One CSR in the reference's layout, device resident.
@@ -108,6 +111,9 @@ int exclusive_scan_u32(gbx_ctx *ctx, const uint32_t *in, uint32_t *out, size_t n
                        uint32_t *d_total);
 int fill_u32(gbx_ctx *ctx, uint32_t *p, uint32_t v, size_t n);
 int max_u32(gbx_ctx *ctx, const uint32_t *a, size_t n, uint32_t *h_max);
+// Device -> host copy on ctx->stream into the caller's memory, which is
+// often pageable; returns when the data is there.
+int copy_to_host(gbx_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 // ---- sort / CSR (gbx_sort.hip)
 // Stable LSD radix sort of (key, pay[, w]) by the low `bits` bits of key.

@@ -108,6 +108,25 @@ __global__ void __launch_bounds__(kBlock) pr_reset_kernel(PlanView P, IterView A
     });
 }
 
+// FP64 sum of the returned ranks into dst->sum (fixed grid, fixed order:
+// deterministic); the read-back reports it, so the host does not sum N
+// floats after the copy.
+__global__ void __launch_bounds__(kBlock) pr_out_sum_kernel(const float *__restrict__ out, uint32_t N,
+                                                            Partial *partials, uint32_t *counter, Slot *dst)
+{
+    Partial part = {0.0, 0.0, 0.0, 0u, 0u, 0u, 0u};
+    double a = 0.0, b = 0.0;  // two chains a thread
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t step = gridDim.x * kBlock;
+    for (; i + step < N; i += 2 * step) {
+        a += (double)out[i];
+        b += (double)out[i + step];
+    }
+    if (i < N) a += (double)out[i];
+    part.sum = a + b;
+    grid_reduce(part, partials, counter, [&](const Partial &tot) { dst->sum = tot.sum; });
+}
+
 // ranks in original order: REF divides by N (pageRank.c:344-350); the
 // stochastic forms divide by the last FP64 rank sum.
 __global__ void pr_finalize_kernel(const float *__restrict__ r, const uint32_t *__restrict__ perm,
@@ -1046,13 +1065,13 @@ static int pr_create_kind(gbx_ctx *ctx, gbx_graph *g, const gbx_pr_params *p, in
     if (!rc) rc = dev_alloc(ctx, pr->c[0], cb);
     if (!rc) rc = dev_alloc(ctx, pr->c[1], cb);
     if (!rc) rc = dev_alloc(ctx, pr->out, nb);
-    if (!rc) rc = dev_alloc(ctx, pr->slots, 2 * sizeof(Slot));
+    if (!rc) rc = dev_alloc(ctx, pr->slots, 3 * sizeof(Slot));  // ping-pong, and the read-back's rank sum
     if (!rc) rc = dev_alloc(ctx, pr->partials, (size_t)std::max(pr->grid, 16384u) * sizeof(Partial));
     if (!rc) rc = dev_alloc(ctx, pr->counter, 64);
     if (!rc) rc = fill_u32(ctx, pr->counter.as<uint32_t>(), 0, 16);
     if (!rc && (hipEventCreate(&pr->ev_begin) != hipSuccess || hipEventCreate(&pr->ev_end) != hipSuccess))
         rc = set_error(ctx, GBX_ERR_HIP, "gbx_pr_create: event creation");
-    if (!rc && hipHostMalloc((void **)&pr->h_slot, sizeof(Slot), 0) != hipSuccess)
+    if (!rc && hipHostMalloc((void **)&pr->h_slot, 2 * sizeof(Slot), 0) != hipSuccess)
         rc = set_error(ctx, GBX_ERR_NOMEM, "gbx_pr_create: pinned slot");
     if (!rc) {
         bool binned = false;
@@ -1259,9 +1278,14 @@ int gbx_pr_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep)
     pr_finalize_kernel<<<grid_for(N, 256, 16384), 256, 0, ctx->stream>>>(
         pr->r.as<float>(), perm, N, slot, pr->kind == K_ADJ ? 2 : (ref_update ? 1 : 0), pr->out.as<float>());
     GBX_HIP(ctx, hipGetLastError());
+    Slot *sum_slot = pr->slots.as<Slot>() + 2;
+    pr_out_sum_kernel<<<std::max(1u, std::min((N + kBlock - 1) / kBlock, 4u * (unsigned)std::max(1, ctx->num_cus))),
+                        kBlock, 0, ctx->stream>>>(pr->out.as<float>(), N, pr->partials.as<Partial>(),
+                                                  pr->counter.as<uint32_t>(), sum_slot);
+    GBX_HIP(ctx, hipGetLastError());
     GBX_HIP(ctx, hipMemcpyAsync(pr->h_slot, slot, sizeof(Slot), hipMemcpyDeviceToHost, ctx->stream));
-    if (ranks_out)
-        GBX_HIP(ctx, hipMemcpyAsync(ranks_out, pr->out.p, (size_t)N * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GBX_HIP(ctx, hipMemcpyAsync(pr->h_slot + 1, sum_slot, sizeof(Slot), hipMemcpyDeviceToHost, ctx->stream));
+    if (ranks_out) GBX_TRY(copy_to_host(ctx, ranks_out, pr->out.p, (size_t)N * 4));
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (pr->timing_started) {
         float ms = 0.0f;
@@ -1274,16 +1298,7 @@ int gbx_pr_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep)
         rep->iterations = s.conv ? (s.iters ? s.iters - 1 : 0) : s.iters;
         rep->active = s.active;
         rep->error_total = s.err;
-        // eight independent FP64 partial sums (one chain of N dependent adds
-        // took ~15 ms at N = 16.7M), combined pairwise in a fixed order
-        double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        if (ranks_out) {
-            uint32_t i = 0;
-            for (; i + 8u <= N; i += 8u)
-                for (int j = 0; j < 8; j++) acc[j] += ranks_out[i + j];
-            for (; i < N; i++) acc[i & 7u] += ranks_out[i];
-        }
-        rep->rank_sum = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        rep->rank_sum = pr->h_slot[1].sum;  // FP64 sum of the returned ranks, summed on the GPU
         rep->time_total_s = pr->elapsed_ms * 1e-3;
     }
     return GBX_OK;
