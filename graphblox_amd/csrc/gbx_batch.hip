@@ -547,8 +547,7 @@ int seg_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep)
     GBX_HIP(ctx, hipGetLastError());
     std::vector<GStat> hs(p->nseg);
     GBX_HIP(ctx, hipMemcpyAsync(hs.data(), st->stat.p, hs.size() * sizeof(GStat), hipMemcpyDeviceToHost, ctx->stream));
-    if (ranks_out)
-        GBX_HIP(ctx, hipMemcpyAsync(ranks_out, st->out.p, (size_t)p->N * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (ranks_out) GBX_TRY(copy_to_host(ctx, ranks_out, st->out.p, (size_t)p->N * 4));
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (rep) {
         const std::vector<uint32_t> &off = pr->g->seg_off;

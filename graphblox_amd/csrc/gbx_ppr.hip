@@ -566,9 +566,7 @@ int ppr_read(gbx_pr *pr, float *ranks_out, gbx_pr_report *rep)
     GBX_HIP(ctx, hipGetLastError());
     std::vector<BScal> hsc(b->k);
     GBX_HIP(ctx, hipMemcpyAsync(hsc.data(), sc, b->k * sizeof(BScal), hipMemcpyDeviceToHost, ctx->stream));
-    if (ranks_out)
-        GBX_HIP(ctx, hipMemcpyAsync(ranks_out, b->out.p, (size_t)N * b->k * 4, hipMemcpyDeviceToHost,
-                                    ctx->stream));
+    if (ranks_out) GBX_TRY(copy_to_host(ctx, ranks_out, b->out.p, (size_t)N * b->k * 4));
     GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (rep) {
         // one report per vector
