@@ -391,6 +391,25 @@ __global__ void bin_keys(const uint32_t *__restrict__ rowid, const uint32_t *__r
     }
 }
 
+// The run layout's sort keys (chunk-major), with the payloads that the
+// layout needs once the edges are sorted, so nothing is gathered after the
+// sort: each edge's local ids (row in its bin << 16 | source in its chunk)
+// and its plan edge.
+__global__ void run_keys(const uint32_t *__restrict__ rowid, const uint32_t *__restrict__ col,
+                         const uint32_t *__restrict__ eidx, uint32_t E, const uint32_t *__restrict__ bin_of,
+                         const uint32_t *__restrict__ chunk_of, const uint32_t *__restrict__ bin_row0,
+                         const uint32_t *__restrict__ chunk_lo, uint32_t NB, uint32_t *__restrict__ key,
+                         uint32_t *__restrict__ local, uint32_t *__restrict__ pedge)
+{
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        const uint32_t row = rowid[e], c = col[e];
+        const uint32_t b = bin_of[row], k = chunk_of[c];
+        key[e] = k * NB + b;
+        local[e] = ((row - bin_row0[b]) << 16) | (c - chunk_lo[k]);
+        pedge[e] = eidx[e];
+    }
+}
+
 // Runs of equal keys in a sorted key array: flag[i] = 1 at a run start.
 __global__ void run_flags(const uint32_t *__restrict__ key, uint32_t n, uint32_t *__restrict__ flag)
 {
@@ -529,15 +548,16 @@ constexpr uint32_t kRunEnd = 0x8000u;  // src_local flag: the entry ends its run
 // (segment starts are group boundaries).  j indexes the chunk-major sorted
 // cold edges; segment r = runidx + flag - 1 sits at src_local position
 // pstart[r].
-__global__ void run_heads(const uint32_t *__restrict__ pay, uint32_t n, const uint32_t *__restrict__ flag,
+__global__ void run_heads(const uint32_t *__restrict__ local, uint32_t n, const uint32_t *__restrict__ flag,
                           const uint32_t *__restrict__ runidx, const uint32_t *__restrict__ runstart,
-                          const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ rowid_c,
-                          uint32_t *__restrict__ head)
+                          const uint32_t *__restrict__ pstart, uint32_t *__restrict__ head)
 {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         const uint32_t r = runidx[j] + flag[j] - 1u;
         const uint32_t p = pstart[r] + (j - runstart[r]);
-        head[j] = (p % kRunGroup == 0 || rowid_c[pay[j]] != rowid_c[pay[j - 1]]) ? 1u : 0u;
+        // a segment starts at a multiple of 8 (p % 8 == 0), so the rows
+        // compared below are in one bin: equal local rows, equal rows
+        head[j] = (p % kRunGroup == 0 || (local[j] >> 16) != (local[j - 1] >> 16)) ? 1u : 0u;
     }
 }
 
@@ -570,33 +590,29 @@ __global__ void scatter_u32(const uint32_t *__restrict__ a, const uint32_t *__re
 // block of 64 groups the V position of its run (the block's first run).  A
 // segment's last run takes the pads of its group (zeros): its end flag sits
 // on the group's last entry, so every group ends with a run end.
-__global__ void run_fill(const uint32_t *__restrict__ key, const uint32_t *__restrict__ pay, uint32_t n,
+__global__ void run_fill(const uint32_t *__restrict__ local, const uint32_t *__restrict__ pedge, uint32_t n,
                          const uint32_t *__restrict__ flag, const uint32_t *__restrict__ runidx,
                          const uint32_t *__restrict__ runstart, const uint32_t *__restrict__ pstart,
                          const uint32_t *__restrict__ head, const uint32_t *__restrict__ hx,
-                         const uint32_t *__restrict__ voff, const uint32_t *__restrict__ rowid_c,
-                         const uint32_t *__restrict__ col_c, const uint32_t *__restrict__ chunk_lo,
-                         const uint32_t *__restrict__ bin_row0, uint32_t NB, uint32_t pad,
-                         const uint32_t *__restrict__ eidx_c, uint16_t *__restrict__ src_local,
+                         const uint32_t *__restrict__ voff, uint32_t pad, uint16_t *__restrict__ src_local,
                          uint16_t *__restrict__ dst_local, uint32_t *__restrict__ blk_v,
                          uint32_t *__restrict__ cold_eidx)
 {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         const uint32_t r = runidx[j] + flag[j] - 1u, s0 = runstart[r];
         const uint32_t p = pstart[r] + (j - s0);
-        const uint32_t kk = key[j], k = kk / NB, b = kk % NB;
         const uint32_t vpos = voff[r] + (hx[j] + head[j] - 1u - hx[s0]);
         const bool end = j + 1 == n || head[j + 1];
         const bool seg_end = j + 1 == n || flag[j + 1];
-        const uint32_t e = pay[j];
+        const uint32_t loc = local[j];
         bool flag_here = end;
         if (seg_end && p % kRunGroup != kRunGroup - 1) {
             flag_here = false;
             src_local[p | (kRunGroup - 1)] = (uint16_t)(pad | kRunEnd);
         }
-        src_local[p] = (uint16_t)((col_c[e] - chunk_lo[k]) | (flag_here ? kRunEnd : 0u));
-        cold_eidx[p] = eidx_c[e];
-        if (end) dst_local[vpos] = (uint16_t)(rowid_c[e] - bin_row0[b]);
+        src_local[p] = (uint16_t)((loc & 0xFFFFu) | (flag_here ? kRunEnd : 0u));
+        cold_eidx[p] = pedge[j];
+        if (end) dst_local[vpos] = (uint16_t)(loc >> 16);
         if (p % (kRunGroup * kRunBlockGroups) == 0) blk_v[p / (kRunGroup * kRunBlockGroups)] = vpos;
     }
 }
@@ -1680,13 +1696,17 @@ static int build_runs(gbx_ctx *ctx, gbx_bplan *bp, uint32_t Ecold, const uint32_
     const uint32_t NB = bp->NB, NC = bp->NC;
     const uint32_t kbits = bits_for(NB * NC - 1);
     const unsigned gE = grid_for(Ecold, 256, 16384);
-    DevBuf key, pay;
+    DevBuf key, local, pedge;
     GBX_TRY(dev_alloc(ctx, key, (size_t)Ecold * 4));
-    GBX_TRY(dev_alloc(ctx, pay, (size_t)Ecold * 4));
-    bin_keys<<<gE, 256, 0, ctx->stream>>>(rowid_c, col_c, Ecold, bin_of, chunk_of, NB, NC, 1, key.as<uint32_t>(),
-                                          pay.as<uint32_t>());
+    GBX_TRY(dev_alloc(ctx, local, (size_t)Ecold * 4));
+    GBX_TRY(dev_alloc(ctx, pedge, (size_t)Ecold * 4));
+    run_keys<<<gE, 256, 0, ctx->stream>>>(rowid_c, col_c, eidx_c, Ecold, bin_of, chunk_of, bin_row0,
+                                          bp->chunk_lo.as<uint32_t>(), NB, key.as<uint32_t>(), local.as<uint32_t>(),
+                                          pedge.as<uint32_t>());
     GBX_HIP(ctx, hipGetLastError());
-    GBX_TRY(radix_sort_pairs(ctx, key.as<uint32_t>(), pay.as<uint32_t>(), nullptr, Ecold, kbits));
+    // the plan edges ride along as the sort's third payload (moved as bits)
+    GBX_TRY(radix_sort_pairs(ctx, key.as<uint32_t>(), local.as<uint32_t>(), reinterpret_cast<float *>(pedge.p),
+                             Ecold, kbits));
     Runs R;
     GBX_TRY(find_runs(ctx, key.as<uint32_t>(), Ecold, kRunGroup, R));
     bp->nseg = R.n;
@@ -1737,9 +1757,8 @@ static int build_runs(gbx_ctx *ctx, gbx_bplan *bp, uint32_t Ecold, const uint32_
     GBX_TRY(dev_alloc(ctx, head, (size_t)Ecold * 4 + 4));
     GBX_TRY(dev_alloc(ctx, hx, (size_t)(Ecold + 1) * 4));
     GBX_TRY(dev_alloc(ctx, htot, 16));
-    run_heads<<<gE, 256, 0, ctx->stream>>>(pay.as<uint32_t>(), Ecold, R.flag.as<uint32_t>(), R.idx.as<uint32_t>(),
-                                           R.start.as<uint32_t>(), R.pstart.as<uint32_t>(), rowid_c,
-                                           head.as<uint32_t>());
+    run_heads<<<gE, 256, 0, ctx->stream>>>(local.as<uint32_t>(), Ecold, R.flag.as<uint32_t>(), R.idx.as<uint32_t>(),
+                                           R.start.as<uint32_t>(), R.pstart.as<uint32_t>(), head.as<uint32_t>());
     GBX_HIP(ctx, hipGetLastError());
     GBX_TRY(exclusive_scan_u32(ctx, head.as<uint32_t>(), hx.as<uint32_t>(), Ecold, htot.as<uint32_t>()));
     GBX_HIP(ctx, hipMemcpyAsync(hx.as<uint32_t>() + Ecold, htot.p, 4, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1783,10 +1802,9 @@ static int build_runs(gbx_ctx *ctx, gbx_bplan *bp, uint32_t Ecold, const uint32_
     GBX_TRY(dev_alloc(ctx, bp->cold_eidx, (size_t)bp->Plen * 4 + 16));
     GBX_TRY(fill_u32(ctx, bp->cold_eidx.as<uint32_t>(), kNoEdge, bp->Plen + 4));
 
-    run_fill<<<gE, 256, 0, ctx->stream>>>(key.as<uint32_t>(), pay.as<uint32_t>(), Ecold, R.flag.as<uint32_t>(),
+    run_fill<<<gE, 256, 0, ctx->stream>>>(local.as<uint32_t>(), pedge.as<uint32_t>(), Ecold, R.flag.as<uint32_t>(),
                                           R.idx.as<uint32_t>(), R.start.as<uint32_t>(), R.pstart.as<uint32_t>(),
-                                          head.as<uint32_t>(), hx.as<uint32_t>(), voff.as<uint32_t>(), rowid_c, col_c,
-                                          bp->chunk_lo.as<uint32_t>(), bin_row0, NB, bp->SC, eidx_c,
+                                          head.as<uint32_t>(), hx.as<uint32_t>(), voff.as<uint32_t>(), bp->SC,
                                           bp->src_local.as<uint16_t>(), bp->dst_local.as<uint16_t>(),
                                           blk_v.as<uint32_t>(), bp->cold_eidx.as<uint32_t>());
     GBX_HIP(ctx, hipGetLastError());
