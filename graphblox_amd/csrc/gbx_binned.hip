@@ -1435,6 +1435,18 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, uint32_t uw, const 
         }
     } else {
         long long *y = reinterpret_cast<long long *>(gbx_dyn_lds);
+        // The update inputs of this thread's first batch of rows load now and
+        // arrive while the unit's sums stream in (the update runs on waves
+        // 1.. after the barrier; its first batch then waits for nothing).
+        constexpr uint32_t kEpiThreads = kBlock - 64;
+        typename EpiOf<KIND, T>::In first[kEpiBatch];
+        if (threadIdx.x >= 64) {
+#pragma unroll
+            for (int q = 0; q < kEpiBatch; q++) {
+                const uint32_t i = threadIdx.x - 64 + q * kEpiThreads;
+                if (i < nrows) first[q] = epi.load(u.row_begin + i);
+            }
+        }
         // The waves take slabs (kHotSlab hot pairs first, then kAccumSlab V
         // entries) from an LDS counter: waves that run ahead take more, so the
         // block reaches the update barrier together.
@@ -1533,13 +1545,9 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, uint32_t uw, const 
             return;
         }
         // the update on waves 1..: inputs of kEpiBatch rows per thread loaded
-        // together; each row sum is cleared once read
-        constexpr uint32_t kEpiThreads = kBlock - 64;
-        for (uint32_t i0 = threadIdx.x - 64; i0 < nrows; i0 += kEpiThreads * kEpiBatch) {
-            typename EpiOf<KIND, T>::In in[kEpiBatch];
-#pragma unroll
-            for (int q = 0; q < kEpiBatch; q++)
-                if (i0 + q * kEpiThreads < nrows) in[q] = epi.load(u.row_begin + i0 + q * kEpiThreads);
+        // together (the first batch's before the sums); each row sum is
+        // cleared once read
+        auto update = [&](uint32_t i0, const typename EpiOf<KIND, T>::In (&in)[kEpiBatch]) {
 #pragma unroll
             for (int q = 0; q < kEpiBatch; q++) {
                 const uint32_t i = i0 + q * kEpiThreads;
@@ -1550,6 +1558,14 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, uint32_t uw, const 
                     else epi.apply(u.row_begin + i, (float)((double)yi * inv), in[q]);
                 }
             }
+        };
+        if (threadIdx.x - 64 < nrows) update(threadIdx.x - 64, first);
+        for (uint32_t i0 = threadIdx.x - 64 + kEpiThreads * kEpiBatch; i0 < nrows; i0 += kEpiThreads * kEpiBatch) {
+            typename EpiOf<KIND, T>::In in[kEpiBatch];
+#pragma unroll
+            for (int q = 0; q < kEpiBatch; q++)
+                if (i0 + q * kEpiThreads < nrows) in[q] = epi.load(u.row_begin + i0 + q * kEpiThreads);
+            update(i0, in);
         }
         if constexpr (KIND == K_SEG) {
             if (live) epi.wave_slot(uw, threadIdx.x >> 6);
