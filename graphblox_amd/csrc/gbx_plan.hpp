@@ -543,12 +543,12 @@ __device__ __forceinline__ void tile_loop(const PlanView &P, const Gather &g, Ep
 }
 
 // Copy the hot prefix of the contribution vector into LDS: each thread's
-// loads (up to 8 of 16 bytes) issued before its first LDS write, so a full
-// 156 KiB copy takes two round trips, not ten.
+// loads (up to 10 of 16 bytes) issued before its first LDS write, so a full
+// 156 KiB copy (9,984 words over 1,024 threads) takes one round trip.
 template <class T>
 __device__ __forceinline__ void load_hot(const T *__restrict__ c_in, uint32_t hot)
 {
-    constexpr uint32_t K = 8;
+    constexpr uint32_t K = 10;
     T *h = reinterpret_cast<T *>(gbx_dyn_lds);
     const uint32_t words = (uint32_t)((hot * sizeof(T) + 15) / 16);
     const uint4 *src = reinterpret_cast<const uint4 *>(c_in);
