@@ -1345,7 +1345,12 @@ template <int KIND>
 constexpr int hot_unroll() { return is_weighted_kind(KIND) ? 1 : 2; }
 template <int KIND>
 constexpr uint32_t hot_slab() { return 64u * 8u * hot_unroll<KIND>(); }  // hot pairs a wave takes at a time
-constexpr int kEpiBatch = 8;     // rows per thread whose update inputs load together
+// Rows per thread whose update inputs load together: 13 for reference
+// semantics, so that a unit of up to 12,480 rows is one batch, loaded before
+// its sums stream in (the other kinds hold more per row in their update and
+// stay at 8: at 13 the stochastic one spills).
+template <int KIND>
+constexpr int epi_batch() { return KIND == K_PR_REF ? 13 : 8; }
 
 // Thread 0 takes the next unit from the counter and loads its record.
 __device__ __forceinline__ void claim_unit(const AccumView &B, uint32_t *s_w, BinUnit *s_u)
@@ -1439,6 +1444,7 @@ __device__ __forceinline__ void accum_unit(const BinUnit &u, uint32_t uw, const 
         // arrive while the unit's sums stream in (the update runs on waves
         // 1.. after the barrier; its first batch then waits for nothing).
         constexpr uint32_t kEpiThreads = kBlock - 64;
+        constexpr int kEpiBatch = epi_batch<KIND>();
         typename EpiOf<KIND, T>::In first[kEpiBatch];
         if (threadIdx.x >= 64) {
 #pragma unroll
