@@ -242,21 +242,22 @@ int max_u32(gbx_ctx *ctx, const uint32_t *a, size_t n, uint32_t *h_max)
 int copy_to_host(gbx_ctx *ctx, void *dst, const void *src, size_t bytes)
 {
     constexpr size_t kDirect = 1u << 20, kChunk = 32u << 20;
-    if (bytes <= kDirect) {
-        GBX_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-        GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        return GBX_OK;
-    }
     const size_t chunk = std::min(bytes, kChunk);
-    if (ctx->h_stage_bytes < 2 * chunk) {
+    if (bytes > kDirect && ctx->h_stage_bytes < 2 * chunk) {
         if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
         ctx->h_stage = nullptr;
         ctx->h_stage_bytes = 0;
-        if (hipHostMalloc(&ctx->h_stage, 2 * chunk, 0) != hipSuccess) {
-            ctx->h_stage = nullptr;
-            return set_error(ctx, GBX_ERR_NOMEM, "copy_to_host: pinned staging of %zu bytes", 2 * chunk);
+        if (hipHostMalloc(&ctx->h_stage, 2 * chunk, 0) == hipSuccess) {
+            ctx->h_stage_bytes = 2 * chunk;
+        } else {
+            ctx->h_stage = nullptr;  // no pinned memory to spare: the runtime's own copy below
+            (void)hipGetLastError();
         }
-        ctx->h_stage_bytes = 2 * chunk;
+    }
+    if (bytes <= kDirect || !ctx->h_stage) {
+        GBX_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        GBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return GBX_OK;
     }
     hipEvent_t ev[2] = {nullptr, nullptr};
     int rc = GBX_OK;
